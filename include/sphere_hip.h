@@ -1,0 +1,165 @@
+/*
+ * sphere_hip: MI355X-native DEM particle timestep for sphere, C ABI.
+ *
+ * The reference has no in-process API: its boundary is a process
+ * (python/sphere.py:4482-4540 writes input/<sid>.bin and runs
+ * `./sphere [-q] [--dry] [-d N] input/<sid>.bin`, src/main.cpp:28-223) plus the
+ * versioned .bin format (src/file_io.cpp:30-731).  Each entry point below
+ * replaces one piece of that boundary; the citation names the reference
+ * code whose behaviour it carries.
+ *
+ * Conventions: plain pointers and sizes only; 0 = success, negative = error
+ * (the library never calls exit()); sph_last_error() returns the message.
+ * A context owns all device memory and one HIP stream; one host thread per
+ * context.
+ */
+#ifndef SPHERE_HIP_H_
+#define SPHERE_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Maximum contacts per particle (src/constants.h:21) and walls (:12). */
+#define SPH_NC 32
+#define SPH_MAXWALLS 6
+/* The .bin version this library reads and writes (src/version.h:5). */
+#define SPH_VERSION 2.15
+
+/* Error codes. */
+#define SPH_OK 0
+#define SPH_E_IO (-1)        /* file cannot be opened / short read        */
+#define SPH_E_VERSION (-2)   /* .bin version != 2.15 (src/file_io.cpp:49-54) */
+#define SPH_E_INPUT (-3)     /* checkValues failure (src/sphere.cpp:145-345) */
+#define SPH_E_DEVICE (-4)    /* HIP runtime error                           */
+#define SPH_E_ARG (-5)       /* bad argument (field id, size, index)        */
+#define SPH_E_UNSUPPORTED (-6)
+
+/* Field ids for sph_get_field / sph_set_field.  Particle fields are indexed
+ * by original particle index (the .bin order), as the reference's dev_* arrays
+ * are.  Layouts (little-endian, row-major):
+ *   X         np x 4 f64 (x, y, z, radius)         k.x        src/sphere.h:63
+ *   VEL       np x 4 f64 (vx, vy, vz, fixvel)      k.vel
+ *   ANGVEL .. ANGACC  np x 3 f64
+ *   ES_DOT .. P       np f64
+ *   CONTACTS  np x NC u32 (original index of the partner, np = empty slot)
+ *   DELTA_T   np x NC x 3 f64 (accumulated tangential displacement)
+ *   DISTMOD   np x NC x 4 f64 (periodic shift x,y,z; radius of the partner)
+ *   CELLID    np u32 sorted cell keys; PERM np u32 sorted index -> original
+ *   CELLSTART/CELLEND  ncells u32 (cellEnd is meaningful only where
+ *             cellStart != 0xffffffff, src/device.cu:1089-1090)
+ *   WALLS_NX, WALLS_MVFD  nw x 4 f64; WALLS_ACC nw f64; WALLS_FORCE_PP nw x np
+ *   BONDS nb0 x 2 u32; BONDS_DELTA, BONDS_OMEGA nb0 x 4 f64 (t.x,t.y,t.z,n)
+ *   COLOR np i32
+ */
+enum sph_field {
+    SPH_F_X = 0,
+    SPH_F_VEL = 1,
+    SPH_F_ANGVEL = 2,
+    SPH_F_ANGPOS = 3,
+    SPH_F_FORCE = 4,
+    SPH_F_TORQUE = 5,
+    SPH_F_XYZSUM = 6,
+    SPH_F_ACC = 7,
+    SPH_F_ANGACC = 8,
+    SPH_F_ES_DOT = 9,
+    SPH_F_ES = 10,
+    SPH_F_EV_DOT = 11,
+    SPH_F_EV = 12,
+    SPH_F_P = 13,
+    SPH_F_CONTACTS = 14,
+    SPH_F_DELTA_T = 15,
+    SPH_F_DISTMOD = 16,
+    SPH_F_CELLID = 17,
+    SPH_F_PERM = 18,
+    SPH_F_CELLSTART = 19,
+    SPH_F_CELLEND = 20,
+    SPH_F_WALLS_NX = 21,
+    SPH_F_WALLS_MVFD = 22,
+    SPH_F_WALLS_ACC = 23,
+    SPH_F_WALLS_FORCE_PP = 24,
+    SPH_F_BONDS = 25,
+    SPH_F_BONDS_DELTA = 26,
+    SPH_F_BONDS_OMEGA = 27,
+    SPH_F_COLOR = 28,
+    SPH_F_COUNT = 29
+};
+
+/* Scalar state ids for sph_get_scalar / sph_set_scalar. */
+enum sph_scalar {
+    SPH_S_TIME = 0,        /* time.current  */
+    SPH_S_TIME_TOTAL = 1,  /* time.total    */
+    SPH_S_DT = 2,          /* time.dt       */
+    SPH_S_FILE_DT = 3,     /* time.file_dt  */
+    SPH_S_ITER = 4,        /* loop counter (restarts at 0 per run, src/device.cu:919) */
+    SPH_S_STEP_COUNT = 5,  /* time.step_count */
+    SPH_S_NP = 6,
+    SPH_S_NW = 7,
+    SPH_S_NB0 = 8,
+    SPH_S_NCELLS = 9
+};
+
+/* Open flags. */
+#define SPH_OPEN_QUIET 0x1u      /* no stdout progress lines                     */
+#define SPH_OPEN_NOCHECK 0x2u    /* skip checkValues (src/sphere.cpp:145-345)     */
+
+typedef struct sph_ctx sph_ctx;
+
+/* readbin + checkValues + H2D: DEM::DEM(inputbin, ...) with initCuda=1,
+ * src/sphere.cpp:16-95 and src/file_io.cpp:30-418.  device < 0 = current. */
+int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out);
+/* Same, from a .bin image already in host memory. */
+int sph_open_mem(const void* bytes, size_t nbytes, int device, unsigned flags,
+                 sph_ctx** out);
+
+/* Run nsteps iterations of the fluid-free loop body of DEM::startTime,
+ * src/device.cu:1031-1200,2423-2543, without file output and without the
+ * t <= t_total test.  Asynchronous w.r.t. the host until a get/sync call. */
+int sph_step(sph_ctx* ctx, uint64_t nsteps);
+/* Block until all queued steps are complete. */
+int sph_sync(sph_ctx* ctx);
+
+/* DEM::startTime (src/device.cu:826-2700): writes out_dir/<sid>.output00000.bin
+ * when step_count == 0, steps while t <= t_total, writes
+ * out_dir/<sid>.output%05d.bin every file_dt and out_dir/<sid>.status.dat. */
+int sph_run(sph_ctx* ctx, const char* out_dir, const char* sid);
+
+/* DEM::writebin (src/file_io.cpp:420-731) of the current state. */
+int sph_write_bin(sph_ctx* ctx, const char* path);
+/* Same, to memory.  Returns the image size in *nbytes; writes only when
+ * cap >= that size (call with dst = NULL to query). */
+int sph_write_mem(sph_ctx* ctx, void* dst, size_t cap, size_t* nbytes);
+
+/* D2H / H2D of one field (layouts above); bytes must equal the field size. */
+int sph_get_field(sph_ctx* ctx, int field, void* host_dst, size_t bytes);
+int sph_set_field(sph_ctx* ctx, int field, const void* host_src, size_t bytes);
+int64_t sph_field_bytes(sph_ctx* ctx, int field);
+int sph_get_scalar(sph_ctx* ctx, int which, double* value);
+int sph_set_scalar(sph_ctx* ctx, int which, double value);
+
+/* Bit-exact check hooks (original-index keyed, as the reference's
+ * dev_contacts / dev_delta_t / dev_distmod, src/contactsearch.cuh:273-291). */
+int sph_get_contacts(sph_ctx* ctx, uint32_t* idx, double* delta_t, double* distmod);
+int sph_get_cells(sph_ctx* ctx, uint32_t* cell_id, uint32_t* perm,
+                  uint32_t* cell_start, uint32_t* cell_end);
+
+/* Per-kernel timing: mean device microseconds of the last sph_step call's
+ * kernels, measured with HIP events on the context stream
+ * (the reference's PROFILING timers, src/device.cu:947-987,2711-2808). */
+int sph_profile(sph_ctx* ctx, int enable);
+int sph_profile_report(sph_ctx* ctx, char* buf, size_t cap);
+
+void sph_close(sph_ctx* ctx);
+const char* sph_last_error(const sph_ctx* ctx);
+/* Error message of the last failed sph_open / sph_open_mem on this thread. */
+const char* sph_open_error(void);
+const char* sph_version_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,171 @@
+"""ctypes binding of the CPU oracle (liboracle.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
+import this package.  It is the checker for the HIP product path and is
+never the thing measured or shipped.
+"""
+
+import ctypes
+import os
+
+import numpy
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+NC = 32
+
+# Field ids and layouts, shared with include/sphere_hip.h.
+FIELDS = {
+    "x": (0, numpy.float64, (-1, 4)),
+    "vel": (1, numpy.float64, (-1, 4)),
+    "angvel": (2, numpy.float64, (-1, 3)),
+    "angpos": (3, numpy.float64, (-1, 3)),
+    "force": (4, numpy.float64, (-1, 3)),
+    "torque": (5, numpy.float64, (-1, 3)),
+    "xyzsum": (6, numpy.float64, (-1, 3)),
+    "acc": (7, numpy.float64, (-1, 3)),
+    "angacc": (8, numpy.float64, (-1, 3)),
+    "es_dot": (9, numpy.float64, (-1,)),
+    "es": (10, numpy.float64, (-1,)),
+    "ev_dot": (11, numpy.float64, (-1,)),
+    "ev": (12, numpy.float64, (-1,)),
+    "p": (13, numpy.float64, (-1,)),
+    "contacts": (14, numpy.uint32, (-1, NC)),
+    "delta_t": (15, numpy.float64, (-1, NC, 3)),
+    "distmod": (16, numpy.float64, (-1, NC, 4)),
+    "cellid": (17, numpy.uint32, (-1,)),
+    "perm": (18, numpy.uint32, (-1,)),
+    "cellstart": (19, numpy.uint32, (-1,)),
+    "cellend": (20, numpy.uint32, (-1,)),
+    "walls_nx": (21, numpy.float64, (-1, 4)),
+    "walls_mvfd": (22, numpy.float64, (-1, 4)),
+    "walls_acc": (23, numpy.float64, (-1,)),
+    "walls_force_pp": (24, numpy.float64, (-1,)),
+    "bonds": (25, numpy.uint32, (-1, 2)),
+    "bonds_delta": (26, numpy.float64, (-1, 4)),
+    "bonds_omega": (27, numpy.float64, (-1, 4)),
+    "color": (28, numpy.int32, (-1,)),
+}
+
+STAGES = {"hash": 0, "sort": 1, "reorder": 2, "topology": 3, "interact": 4,
+          "bonds": 5, "integrate": 6, "walls": 7, "advance": 8}
+
+
+def lib_path():
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    if force or not os.path.exists(lib_path()):
+        subprocess.check_call(["make", "-s", "-B" if force else "all"], cwd=_HERE)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(lib_path()):
+            build()
+        L = ctypes.CDLL(lib_path())
+        L.orc_open_mem.restype = ctypes.c_void_p
+        L.orc_open_mem.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p,
+                                   ctypes.c_size_t]
+        L.orc_close.argtypes = [ctypes.c_void_p]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.orc_stage.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_get_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_size_t]
+        L.orc_set_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                    ctypes.c_size_t]
+        L.orc_field_bytes.restype = ctypes.c_int64
+        L.orc_field_bytes.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_write_mem.restype = ctypes.c_int64
+        L.orc_write_mem.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+        L.orc_np.restype = ctypes.c_uint32
+        L.orc_np.argtypes = [ctypes.c_void_p]
+        L.orc_time.restype = ctypes.c_double
+        L.orc_time.argtypes = [ctypes.c_void_p]
+        L.orc_iter.restype = ctypes.c_int64
+        L.orc_iter.argtypes = [ctypes.c_void_p]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def set_threads(n):
+    _lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return _lib().orc_get_threads()
+
+
+class Oracle:
+    """One oracle simulation, created from a v2.15 .bin image."""
+
+    def __init__(self, image):
+        L = _lib()
+        buf = ctypes.create_string_buffer(bytes(image), len(image))
+        err = ctypes.create_string_buffer(256)
+        h = L.orc_open_mem(buf, len(image), err, 256)
+        if not h:
+            raise ValueError("oracle: " + err.value.decode())
+        self._h = ctypes.c_void_p(h)
+
+    def close(self):
+        if self._h:
+            _lib().orc_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def np(self):
+        return int(_lib().orc_np(self._h))
+
+    @property
+    def time(self):
+        return float(_lib().orc_time(self._h))
+
+    @property
+    def iter(self):
+        return int(_lib().orc_iter(self._h))
+
+    def step(self, n=1):
+        rc = _lib().orc_step(self._h, int(n))
+        if rc != 0:
+            raise RuntimeError("orc_step failed: %d" % rc)
+
+    def stage(self, name):
+        rc = _lib().orc_stage(self._h, STAGES[name])
+        if rc != 0:
+            raise RuntimeError("orc_stage failed: %d" % rc)
+
+    def get(self, name):
+        fid, dt, shape = FIELDS[name]
+        nbytes = _lib().orc_field_bytes(self._h, fid)
+        out = numpy.empty(nbytes // numpy.dtype(dt).itemsize, dtype=dt)
+        rc = _lib().orc_get_field(self._h, fid, out.ctypes.data, nbytes)
+        if rc != 0:
+            raise RuntimeError("orc_get_field(%s) failed: %d" % (name, rc))
+        return out.reshape(shape)
+
+    def set(self, name, value):
+        fid, dt, shape = FIELDS[name]
+        a = numpy.ascontiguousarray(value, dtype=dt)
+        rc = _lib().orc_set_field(self._h, fid, a.ctypes.data, a.nbytes)
+        if rc != 0:
+            raise RuntimeError("orc_set_field(%s) failed: %d" % (name, rc))
+
+    def tobytes(self):
+        n = _lib().orc_write_mem(self._h, None, 0)
+        buf = ctypes.create_string_buffer(n)
+        _lib().orc_write_mem(self._h, buf, n)
+        return buf.raw

@@ -1,0 +1,69 @@
+// Device array set of one simulation context.  All particle arrays are
+// indexed by original particle index (the .bin order) unless named *_s
+// (sorted order), mirroring the reference's dev_* / dev_*_sorted split
+// (src/datatypes.h:16-61).
+#ifndef SPHERE_AMD_DEM_ARRAYS_HPP_
+#define SPHERE_AMD_DEM_ARRAYS_HPP_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_types.hpp"
+
+namespace sphx {
+
+struct DevArrays {
+    // kinematics, 32 B per particle each (x,y,z,w)
+    double4* x;        // position, radius in w
+    double4* vel;      // velocity, fixvel in w
+    double4* angvel;
+    double4* angpos;
+    double4* xyzsum;
+    double4* acc;      // previous-step acceleration (TY3 acc0)
+    double4* angacc;
+    double4* force;
+    double4* torque;
+    double* es_dot;
+    double* es;
+    double* ev_dot;
+    double* ev;
+    double* p;
+    // sorting
+    uint32_t* key;       // cell key per original index (hash output)
+    uint32_t* key_prev;  // key of the last sorted step, per original index
+    uint32_t* skey;      // sorted keys
+    uint32_t* perm;      // sorted index -> original index
+    uint32_t* key_tmp;   // radix ping-pong
+    uint32_t* val_tmp;
+    uint32_t* rhist;     // radix digit histograms, 256 x tiles
+    uint32_t* cell_start;
+    uint32_t* cell_end;
+    double4* xs;         // sorted copy of x
+    // contact history (NC words per particle, original-index rows)
+    uint32_t* crow;      // packed partner index + periodic image code
+    double4* delta_t;    // tangential displacement per slot (w unused)
+    // walls
+    double4* w_nx;       // normal (xyz) and position (w)
+    double4* w_mvfd;     // mass, velocity, force, sigma0
+    double* w_acc;
+    double* force_pp;    // nw x np wall force per particle
+    double* tau_pp;      // np shear stress per particle (wmode 3)
+    double* partial;     // nblocks block sums
+    double* tau_partial;
+    double* tau_eff;     // serial sum of tau partials (1 value)
+    // bonds
+    uint32_t* bonds;     // nb0 x 2
+    double4* bonds_delta;
+    double4* bonds_omega;
+    double4* bond_f;     // per-bond force on particle i (this step)
+    double4* bond_t;     // per-bond torque t_j (this step)
+    uint32_t* bond_ok;   // 1 if the bond applied forces this step
+    uint32_t* badj_off;  // CSR offsets (np + 1) of bonds per particle
+    uint32_t* badj;      // CSR bond ids, ascending per particle
+    // loop state
+    DevLoop* loop;
+};
+
+}  // namespace sphx
+
+#endif

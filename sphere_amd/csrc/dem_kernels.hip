@@ -1,0 +1,1088 @@
+// HIP kernels of the DEM particle timestep for CDNA4 (gfx950).
+//
+// One workgroup = 256 work-items = four 64-wide wavefronts.  All arithmetic
+// is FP64 with the reference's operation order; this file must be compiled
+// with -ffp-contract=off (no fused multiply-add), as the reference is with
+// nvcc --fmad=false (src/CMakeLists.txt:25).
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_arrays.hpp"
+#include "dem_kernels.hpp"
+#include "dem_math.hpp"
+#include "dem_types.hpp"
+
+namespace sphx {
+
+namespace {
+
+constexpr double PI = 3.14159265358979;  // src/constants.h:14
+
+__device__ __forceinline__ double grid_Lz(const DevParams& p, const DevArrays& a) {
+    // DEM::updateGridSize (src/device.cu:363-393): with an adaptive grid the
+    // step runs with L[2] equal to the top wall position at its start.
+    return (p.adaptive == 1 && p.nw > 0) ? a.w_nx[0].w : p.L[2];
+}
+
+// calcCellID, src/sorting.cuh:7-20: the integer coordinate is the double
+// floor of (x - origo) / (L / num), converted to unsigned.
+__device__ __forceinline__ uint32_t to_u32(double v) {
+    // Values are >= 0 for particles inside the grid; a negative value (a
+    // particle below origo) converts to 0 here, where the reference's
+    // conversion is undefined.
+    return (uint32_t)v;
+}
+
+__device__ __forceinline__ uint32_t calc_cell_id(const DevParams& p, double Lz, double x,
+                                                 double y, double z) {
+    const uint32_t ix = to_u32(floor((x - p.origo[0]) / (p.L[0] / p.num[0])));
+    const uint32_t iy = to_u32(floor((y - p.origo[1]) / (p.L[1] / p.num[1])));
+    const uint32_t iz = to_u32(floor((z - p.origo[2]) / (Lz / p.num[2])));
+    return (iz * p.num[1]) * p.num[0] + iy * p.num[0] + ix;
+}
+
+// ---------------------------------------------------------------------------
+// Hash
+
+__global__ __launch_bounds__(BLOCK) void k_hash(const DevParams p, const DevArrays a) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    uint32_t key = 0, changed = 0, nonfinite = 0;
+    if (i < p.np) {
+        const double Lz = grid_Lz(p, a);
+        const double4 x = a.x[i];
+        key = calc_cell_id(p, Lz, x.x, x.y, x.z);
+        if (x.x != x.x || x.y != x.y || x.z != x.z) nonfinite = 1;
+        changed = key != a.key_prev[i];
+        a.key[i] = key;
+    }
+    // One atomic per wavefront for the OR of keys and the change flag.
+    uint32_t kor = key;
+    for (int off = 32; off > 0; off >>= 1) kor |= __shfl_xor(kor, off, 64);
+    const unsigned long long anych = __ballot(changed);
+    const unsigned long long anynf = __ballot(nonfinite);
+    if ((threadIdx.x & 63) == 0) {
+        if (kor) atomicOr(&a.loop->key_or, kor);
+        if (anych) atomicOr(&a.loop->cells_changed, 1u);
+        if (anynf) atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_NONFINITE);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stable LSD radix sort, 8-bit digits.  Tile t covers keys
+// [t*TILE, (t+1)*TILE) in rounds of 256: element order inside a tile is
+// (round, thread), which is the global index order, so ranking each round
+// after all earlier rounds keeps the sort stable.
+
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict__ kin,
+                                                      uint32_t n, int shift,
+                                                      uint32_t* __restrict__ hist,
+                                                      uint32_t ntiles) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t base = blockIdx.x * RADIX_TILE;
+#pragma unroll
+    for (int j = 0; j < RADIX_ITEMS; ++j) {
+        const uint32_t e = base + j * BLOCK + threadIdx.x;
+        if (e < n) atomicAdd(&h[(kin[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of the digit-major histogram (256 x ntiles) in place.
+__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist,
+                                                     uint32_t total) {
+    __shared__ uint32_t s[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t chunk = (total + 1023u) / 1024u;
+    const uint32_t b0 = t * chunk;
+    uint32_t sum = 0;
+    for (uint32_t k = 0; k < chunk; ++k) {
+        const uint32_t e = b0 + k;
+        if (e < total) sum += hist[e];
+    }
+    s[t] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = t >= off ? s[t - off] : 0u;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = s[t] - sum;  // exclusive prefix of this chunk
+    for (uint32_t k = 0; k < chunk; ++k) {
+        const uint32_t e = b0 + k;
+        if (e < total) {
+            const uint32_t v = hist[e];
+            hist[e] = run;
+            run += v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_radix_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
+    const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    __shared__ uint32_t running[256];
+    __shared__ uint32_t whist[BLOCK / 64][256];
+    const uint32_t t = threadIdx.x;
+    const uint32_t w = t >> 6;
+    const uint32_t lane = t & 63u;
+    running[t] = hist[t * ntiles + blockIdx.x];
+    const uint32_t base = blockIdx.x * RADIX_TILE;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int j = 0; j < RADIX_ITEMS; ++j) {
+#pragma unroll
+        for (int q = 0; q < BLOCK / 64; ++q) whist[q][t] = 0u;
+        __syncthreads();
+        const uint32_t e = base + j * BLOCK + t;
+        const bool valid = e < n;
+        const uint32_t k = valid ? kin[e] : 0u;
+        const uint32_t v = valid ? (vin ? vin[e] : e) : 0u;
+        const uint32_t d = (k >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (d >> bit) & 1u;
+            const unsigned long long b = __ballot(on);
+            peers &= on ? b : ~b;
+        }
+        const uint32_t rank = __popcll(peers & lt);
+        if (valid && rank == 0) whist[w][d] = __popcll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t off = running[d] + rank;
+            for (uint32_t q = 0; q < w; ++q) off += whist[q][d];
+            kout[off] = k;
+            vout[off] = v;
+        }
+        __syncthreads();
+        uint32_t add = 0;
+#pragma unroll
+        for (int q = 0; q < BLOCK / 64; ++q) add += whist[q][t];
+        running[t] += add;
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cells and reorder
+
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* __restrict__ dst, uint32_t n,
+                                                    uint32_t v) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArrays a,
+                                                 int sorted_bits) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= p.np) return;
+    const uint32_t key = a.skey[i];
+    if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
+        atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_KEY_RANGE);
+    if (i == 0 || key != a.skey[i - 1]) {
+        if (key < p.ncells) a.cell_start[key] = i;
+        if (i > 0) {
+            const uint32_t pk = a.skey[i - 1];
+            if (pk < p.ncells) a.cell_end[pk] = i;
+        }
+    }
+    if (i == p.np - 1 && key < p.ncells) a.cell_end[key] = i + 1;
+    const uint32_t o = a.perm[i];
+    a.xs[i] = a.x[o];
+    a.key_prev[o] = key;
+}
+
+// ---------------------------------------------------------------------------
+// Contact models (src/contactmodels.cuh)
+
+struct Acc {
+    d3 F, T;
+    double es_dot, ev_dot, p;
+};
+
+// contactLinear_wall, src/contactmodels.cuh:10-117.
+__device__ __noinline__ double contact_linear_wall(const DevParams& p, Acc& A, double ra,
+                                                   const double4 vt, const double4 at, d3 n,
+                                                   double delta, double wvel) {
+    const d3 vel_linear = xyz(vt);
+    const d3 angvel = xyz(at);
+    const d3 vel = vel_linear + (ra + delta / 2.0) * cross(n, angvel) + wvel;
+    const double vel_n = dot(vel_linear, n);
+    const d3 vel_t = vel - n * (dot(n, vel));
+    const double vel_t_length = length(vel_t);
+    double k_n = p.k_n;
+    if (p.E > .001) k_n = 3.141592654 / 2.0 * p.E * ra;
+    const d3 f_n = fmax(0.0, -k_n * delta - p.gamma_wn * vel_n) * n;
+    const double f_n_length = length(f_n);
+    A.ev_dot += p.gamma_wn * vel_n * vel_n;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    if (vel_t_length > 0.0 && p.gamma_wt > 0.0) {
+        const d3 f_t_visc = p.gamma_wt * vel_t;
+        const double f_t_visc_length = length(f_t_visc);
+        const double f_t_limit = p.mu_wd * f_n_length;
+        if (f_t_visc_length < f_t_limit) {
+            f_t = -1.0 * f_t_visc;
+        } else {
+            f_t = -f_t_limit * vel_t / vel_t_length;
+            A.es_dot += length(length(f_t) * vel_t * p.dt) / p.dt;
+        }
+    }
+    A.F += f_n + f_t;
+    A.T += cross(-(ra + delta * 0.5) * n, f_t);
+    A.p += f_n_length / (4.0 * PI * ra * ra);
+    return dot(f_n, n);
+}
+
+// contactLinear (model 2), src/contactmodels.cuh:244-411.  Returns the new
+// tangential displacement (stored by the caller).
+__device__ __forceinline__ d3 contact_linear(const DevParams& p, Acc& A, const double4 vel_a,
+                                             d3 angvel_a, const double4 vel_b, d3 angvel_b,
+                                             double ra, double rb, d3 x, double x_length,
+                                             double delta, const double4 dt0) {
+    const d3 delta_t0_uncor = mk3(dt0.x, dt0.y, dt0.z);
+    const d3 n = x / x_length;
+    const d3 vel_linear = mk3(vel_a.x - vel_b.x, vel_a.y - vel_b.y, vel_a.z - vel_b.z);
+    const d3 vel = vel_linear + (ra + delta / 2.0) * cross(n, angvel_a) +
+                   (rb + delta / 2.0) * cross(n, angvel_b);
+    const double vel_n = -dot(vel_linear, n);
+    const d3 vel_t = vel - n * dot(n, vel);
+    const double vel_t_length = length(vel_t);
+    const d3 delta_t0 = delta_t0_uncor - (n * dot(n, delta_t0_uncor));
+    const double delta_t0_length = length(delta_t0);
+    d3 delta_t = delta_t0;  // the value stored when neither branch assigns it
+    double k_n = p.k_n, k_t = p.k_t;
+    if (p.E > .001) {
+        k_n = 3.141592654 / 2.0 * p.E * (ra + rb) / 2.;
+        k_t = k_n;
+    }
+    const d3 f_n = fmax(0.0, -k_n * delta + p.gamma_n * vel_n) * n;
+    const double f_n_length = length(f_n);
+    A.ev_dot += 0.5 * p.gamma_n * vel_n * vel_n;
+    const d3 f_c = -p.kappa * sqrt32(ra * rb) * n;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
+        delta_t = delta_t0 + vel_t * p.dt;
+        const d3 f_t_elast = -k_t * delta_t;
+        const d3 f_t_visc = -p.gamma_t * vel_t;
+        f_t = f_t_elast + f_t_visc;
+        const double f_t_length = length(f_t);
+        const double f_t_limit = p.mu_s * length(f_n - f_c);
+        A.ev_dot += 0.5 * p.gamma_t * vel_t_length * vel_t_length;
+        if (f_t_length > f_t_limit) {
+            const d3 t = f_t / length(f_t);
+            f_t = f_t_limit * t;
+            delta_t = -1.0 / k_t * (p.mu_d * length(f_n - f_c) * t + p.gamma_t * vel_t);
+            A.es_dot += 0.5 * length(length(f_t) * vel_t * p.dt) / p.dt;
+        }
+    }
+    A.F += f_n + f_t + f_c;
+    A.T += cross(-(ra + delta * 0.5) * n, f_t);
+    A.p += f_n_length / (4.0 * PI * ra * ra);
+    return delta_t;
+}
+
+// contactHertz (model 3), src/contactmodels.cuh:417-600, with powers of
+// |delta| through correctly rounded FP64 square roots and a repulsive
+// elastic normal term (the reference's powf of a negative overlap is NaN).
+__device__ __forceinline__ d3 contact_hertz(const DevParams& p, Acc& A, const double4 vel_a,
+                                            d3 angvel_a, const double4 vel_b, d3 angvel_b,
+                                            double ra, double rb, d3 x_ab, double x_ab_length,
+                                            double delta, const double4 dt0) {
+    const d3 delta_t0_uncor = mk3(dt0.x, dt0.y, dt0.z);
+    const d3 n_ab = x_ab / x_ab_length;
+    const d3 vel_ab_linear = mk3(vel_a.x - vel_b.x, vel_a.y - vel_b.y, vel_a.z - vel_b.z);
+    const d3 vel_ab = vel_ab_linear + (ra + delta / 2.0) * cross(n_ab, angvel_a) +
+                      (rb + delta / 2.0) * cross(n_ab, angvel_b);
+    const d3 angvel_ab = angvel_a - angvel_b;
+    const double angvel_ab_length = length(angvel_ab);
+    const double vel_n_ab = dot(vel_ab_linear, n_ab);
+    const d3 vel_t_ab = vel_ab - (n_ab * dot(vel_ab, n_ab));
+    const double vel_t_ab_length = length(vel_t_ab);
+    const d3 delta_t0 = delta_t0_uncor - (n_ab * dot(delta_t0_uncor, n_ab));
+    const double delta_t0_length = length(delta_t0);
+    d3 delta_t = delta_t0;
+    double k_n = p.k_n, k_t = p.k_t;
+    if (p.E > .001) {
+        k_n = 3.141592654 / 2.0 * p.E * (ra + rb) / 2.;
+        k_t = k_n;
+    }
+    const double ad = fabs(delta);
+    const double p12 = sqrt_rn_f64(ad);
+    const double p14 = sqrt_rn_f64(p12);
+    const double p32 = ad * p12;
+    d3 f_n = (k_n * p32 - p.gamma_n * p14 * vel_n_ab) * n_ab;
+    A.ev_dot += p.gamma_n * vel_n_ab * vel_n_ab;
+    if (dot(f_n, n_ab) < 0.0) f_n = mk3(0.0, 0.0, 0.0);
+    const double f_n_length = length(f_n);
+    const d3 f_c = -p.kappa * sqrt32(ra * rb) * n_ab;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    d3 T_res = mk3(0.0, 0.0, 0.0);
+    if (delta_t0_length > 0.0 || vel_t_ab_length > 0.0) {
+        const d3 f_t_elast = -k_t * p12 * delta_t0;
+        const d3 f_t_visc = -p.gamma_t * p14 * vel_t_ab;
+        double f_t_limit;
+        if (vel_t_ab_length > (double)0.001f)
+            f_t_limit = p.mu_d * length(f_n - f_c);
+        else
+            f_t_limit = p.mu_s * length(f_n - f_c);
+        f_t = f_t_elast + f_t_visc;
+        const double f_t_length = length(f_t);
+        if (f_t_length > f_t_limit) {
+            f_t *= f_t_limit / f_t_length;
+            delta_t = (f_t + p.gamma_t * p14 * vel_t_ab) / (-k_t * p12);
+            A.es_dot += length(delta_t0 - delta_t) * k_t / p.dt;
+        } else {
+            delta_t = delta_t0 + vel_t_ab * p.dt;
+        }
+    }
+    if (angvel_ab_length > 0.0) {
+        T_res = -1.0 * fmin(p.gamma_r * ra * angvel_ab_length, p.mu_r * ra * f_n_length) *
+                angvel_ab / angvel_ab_length;
+    }
+    A.F += f_n + f_t + f_c;
+    A.T += -(ra + delta / 2.0) * cross(n_ab, f_t) + T_res;
+    A.p += f_n_length / (4.0 * PI * ra * ra);
+    return delta_t;
+}
+
+// contactLinearViscous (model 1), src/contactmodels.cuh:122-240.
+__device__ __forceinline__ void contact_linear_viscous(const DevParams& p, Acc& A,
+                                                       const double4 vel_a, d3 angvel_a,
+                                                       const double4 vel_b, d3 angvel_b,
+                                                       double ra, double rb, d3 x_ab,
+                                                       double x_ab_length, double delta,
+                                                       double kappa) {
+    const d3 n_ab = x_ab / x_ab_length;
+    const d3 vel_ab_linear = mk3(vel_a.x - vel_b.x, vel_a.y - vel_b.y, vel_a.z - vel_b.z);
+    const d3 vel_ab = vel_ab_linear + (ra + delta / 2.0) * cross(n_ab, angvel_a) +
+                      (rb + delta / 2.0) * cross(n_ab, angvel_b);
+    const double vel_n_ab = dot(vel_ab_linear, n_ab);
+    const d3 vel_t_ab = vel_ab - (n_ab * dot(vel_ab, n_ab));
+    const double vel_t_ab_length = length(vel_t_ab);
+    double k_n = p.k_n;
+    if (p.E > .001) k_n = 3.141592654 / 2.0 * p.E * (ra + rb) / 2.;
+    d3 f_n = fmax(0.0, -k_n * delta - p.gamma_n * vel_n_ab) * n_ab;
+    if (dot(f_n, n_ab) < 0.0) f_n = mk3(0.0, 0.0, 0.0);
+    const double f_n_length = length(f_n);
+    const d3 f_c = -kappa * sqrt32(ra * rb) * n_ab;
+    d3 f_t = mk3(0.0, 0.0, 0.0);
+    if (vel_t_ab_length > 0.0) {
+        const double f_t_visc = p.gamma_t * vel_t_ab_length;
+        double f_t_limit;
+        if (vel_t_ab_length > (double)0.001f)
+            f_t_limit = p.mu_d * length(f_n - f_c);
+        else
+            f_t_limit = p.mu_s * length(f_n - f_c);
+        if (f_t_visc < f_t_limit)
+            f_t = -f_t_visc * vel_t_ab / vel_t_ab_length;
+        else
+            f_t = -f_t_limit * vel_t_ab / vel_t_ab_length;
+    }
+    A.F += f_n + f_t + f_c;
+    A.T += -(ra + delta / 2.0) * cross(n_ab, f_t);
+    A.p += f_n_length / (4.0 * PI * ra * ra);
+}
+
+// capillaryCohesion_exp, src/cohesion.cuh:237-277 (FP32 sqrtf and expf as
+// there; expf's last bit is not pinned across libraries).
+__device__ __forceinline__ void capillary_cohesion_exp(const DevParams& p, Acc& A, double ra,
+                                                       double rb, double delta, d3 x_ab,
+                                                       double x_ab_length, double kappa) {
+    const d3 n_ab = x_ab / x_ab_length;
+    double r;
+    if ((ra / rb) > (rb / ra)) r = ra / rb; else r = rb / ra;
+    const double h = -sqrt32(r);
+    const double R_har = (2.0 * ra * rb) / (ra + rb);
+    const double R_geo = sqrt32(ra * rb);
+    const double lambda = 0.9 * h * sqrt32(p.V_b / R_har);
+    const d3 f_c = -kappa * R_geo * (double)expf((float)(-delta / lambda)) * n_ab;
+    A.F += f_c;
+}
+
+// findDistMod, src/contactsearch.cuh:12-83.  Returns the image code (see
+// CW_SHIFT_*) or -1 when the target cell lies outside the grid.
+__device__ __forceinline__ int find_dist_mod(const DevParams& p, int& tx, int& ty, int tz) {
+    const int nx = (int)p.num[0], ny = (int)p.num[1], nz = (int)p.num[2];
+    int code = 0;
+    if (p.periodic == 1) {
+        if (tx < 0) { tx += nx; code |= 1; }
+        if (tx >= nx) { tx -= nx; code |= 2; }
+        if (ty < 0) { ty += ny; code |= 4; }
+        if (ty >= ny) { ty -= ny; code |= 8; }
+    }
+    if (p.periodic == 2) {
+        if (tx < 0) { tx += nx; code |= 1; }
+        if (tx >= nx) { tx -= nx; code |= 2; }
+        if (ty < 0 || ty >= ny) return -1;
+    }
+    if (p.periodic > 2 || p.periodic < 1) {
+        if (tx < 0 || tx >= nx) return -1;
+        if (ty < 0 || ty >= ny) return -1;
+    }
+    if (tz < 0 || tz >= nz) return -1;
+    // A particle far outside the grid can leave tx/ty out of range after one
+    // periodic shift; the reference then reads outside cellStart.  Such a
+    // cell is skipped here.
+    if (tx < 0 || tx >= nx || ty < 0 || ty >= ny) return -1;
+    return code;
+}
+
+// The image shift of a code, exactly as findDistMod builds it from zero.
+__device__ __forceinline__ d3 shift_of(const DevParams& p, uint32_t code) {
+    d3 s = mk3(0.0, 0.0, 0.0);
+    if (code & 1u) s.x = 0.0 + p.L[0];
+    if (code & 2u) s.x = 0.0 - p.L[0];
+    if (code & 4u) s.y = 0.0 + p.L[1];
+    if (code & 8u) s.y = 0.0 - p.L[1];
+    return s;
+}
+
+__device__ __forceinline__ uint32_t pack_word(uint32_t idx, uint32_t code) {
+    // code bits: 1 (+Lx), 2 (-Lx), 4 (+Ly), 8 (-Ly)
+    uint32_t w = idx;
+    if (code & 1u) w |= 1u << CW_SHIFT_X;
+    if (code & 2u) w |= 2u << CW_SHIFT_X;
+    if (code & 4u) w |= 1u << CW_SHIFT_Y;
+    if (code & 8u) w |= 2u << CW_SHIFT_Y;
+    return w;
+}
+__device__ __forceinline__ uint32_t word_code(uint32_t w) {
+    const uint32_t cx = (w >> CW_SHIFT_X) & 3u, cy = (w >> CW_SHIFT_Y) & 3u;
+    return (cx == 1u ? 1u : 0u) | (cx == 2u ? 2u : 0u) | (cy == 1u ? 4u : 0u) |
+           (cy == 2u ? 8u : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// topology + interact, one work-item per particle in sorted order
+// (src/contactsearch.cuh:329-371 then :385-726).  The particle's contact
+// row lives in LDS for the duration (33-word stride: no bank conflicts).
+
+constexpr int ROW_STRIDE = NC + 1;
+
+__global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevArrays a) {
+    __shared__ uint32_t srow[BLOCK * ROW_STRIDE];
+    const uint32_t ia = blockIdx.x * BLOCK + threadIdx.x;
+    if (ia >= p.np) return;
+    uint32_t* row = srow + threadIdx.x * ROW_STRIDE;
+    const uint32_t np = p.np;
+    const uint32_t ia_orig = a.perm[ia];
+    const double4 xa = a.xs[ia];
+    const double ra = xa.w;
+    const double Lz = grid_Lz(p, a);
+    const double4 vel_a = a.vel[ia_orig];
+    const double4 av_a = a.angvel[ia_orig];
+    const d3 angvel_a = xyz(av_a);
+    Acc A;
+    A.F = mk3(0.0, 0.0, 0.0);
+    A.T = mk3(0.0, 0.0, 0.0);
+    A.es_dot = 0.0;
+    A.ev_dot = 0.0;
+    A.p = 0.0;
+    const uint32_t cm = p.contactmodel;
+    // src/contactsearch.cuh:352-354
+    const int gx = (int)floor((xa.x - p.origo[0]) / (p.L[0] / p.num[0]));
+    const int gy = (int)floor((xa.y - p.origo[1]) / (p.L[1] / p.num[1]));
+    const int gz = (int)floor((xa.z - p.origo[2]) / (Lz / p.num[2]));
+    if (cm == 2 || cm == 3) {
+        const uint4* grow = reinterpret_cast<const uint4*>(a.crow + (size_t)ia_orig * NC);
+#pragma unroll
+        for (int q = 0; q < NC / 4; ++q) {
+            const uint4 v = grow[q];
+            row[4 * q] = v.x; row[4 * q + 1] = v.y; row[4 * q + 2] = v.z; row[4 * q + 3] = v.w;
+        }
+        // topology: findContactsInCell over the 27 cells, z-major.
+        int nc = 0;
+        for (int dz = -1; dz < 2; ++dz)
+            for (int dy = -1; dy < 2; ++dy)
+                for (int dx = -1; dx < 2; ++dx) {
+                    int tx = gx + dx, ty = gy + dy;
+                    const int tz = gz + dz;
+                    const int code = find_dist_mod(p, tx, ty, tz);
+                    if (code < 0) continue;
+                    const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                                          (p.num[0] * p.num[1]) * (uint32_t)tz;
+                    const uint32_t start = a.cell_start[cell];
+                    if (start == 0xffffffffu) continue;
+                    const uint32_t end = a.cell_end[cell];
+                    const d3 dm = shift_of(p, (uint32_t)code);
+                    for (uint32_t ib = start; ib < end; ++ib) {
+                        if (ib == ia) continue;
+                        const double4 xb = a.xs[ib];
+                        const double rb = xb.w;
+                        d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                        xab += dm;
+                        const double dd = dot(xab, xab);
+                        const double S = ra + rb;
+                        // Conservative reject: len32 > S whenever
+                        // dd >= S^2 (1 + 2^-16) (the two FP32 roundings move
+                        // the root by < 2^-23 relative).
+                        if (dd >= S * S * 1.0000152587890625) continue;
+                        const double len = sqrt32(dd);
+                        const double delta = len - S;
+                        if (delta < 0.0) {
+                            const uint32_t ib_orig = a.perm[ib];
+                            int cpos = nc;
+                            for (int i = 0; i < NC; ++i) {
+                                const uint32_t cidx = row[i] & CW_IDX_MASK;
+                                if (cidx == np) {
+                                    cpos = i;
+                                } else if (cidx == ib_orig) {
+                                    cpos = i;
+                                    break;
+                                }
+                            }
+                            // cpos == NC would write into the next particle's
+                            // row (src/contactsearch.cuh:286); suppressed.
+                            if (cpos < NC) row[cpos] = pack_word(ib_orig, (uint32_t)code);
+                            ++nc;
+                            if (nc > NC) {
+                                atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_CONTACT_OVERFLOW);
+                                break;
+                            }
+                        }
+                    }
+                }
+        // interact: the contact loop, src/contactsearch.cuh:484-559.
+        const bool zero_empty = a.loop->zero_empty != 0u;
+        for (int i = 0; i < NC; ++i) {
+            const size_t mp = (size_t)ia_orig * NC + i;
+            const uint32_t wrd = row[i];
+            const uint32_t ib_orig = wrd & CW_IDX_MASK;
+            if (ib_orig != np) {
+                const d3 dm = shift_of(p, word_code(wrd));
+                const double4 xb = a.x[ib_orig];
+                const double rb = xb.w;
+                const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
+                const double len = length(xab);
+                const double dn = len - (ra + rb);
+                if (dn < 0.0) {
+                    const double4 vel_b = a.vel[ib_orig];
+                    const d3 angvel_b = xyz(a.angvel[ib_orig]);
+                    const double4 dt0 = a.delta_t[mp];
+                    d3 nd;
+                    if (cm == 2)
+                        nd = contact_linear(p, A, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
+                                            len, dn, dt0);
+                    else
+                        nd = contact_hertz(p, A, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
+                                           len, dn, dt0);
+                    a.delta_t[mp] = make_double4(nd.x, nd.y, nd.z, 0.0);
+                } else {
+                    row[i] = np;
+                    a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
+                }
+            } else if (zero_empty) {
+                // Empty slots hold a zero displacement once every slot has been
+                // through one interact pass (src/contactsearch.cuh:554-558);
+                // only the first pass after the history was loaded writes them.
+                a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+        }
+        uint4* wrow = reinterpret_cast<uint4*>(a.crow + (size_t)ia_orig * NC);
+#pragma unroll
+        for (int q = 0; q < NC / 4; ++q)
+            wrow[q] = make_uint4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
+    } else if (cm == 1) {
+        // findAndProcessContactsInCell, src/contactsearch.cuh:91-186.
+        for (int dz = -1; dz < 2; ++dz)
+            for (int dy = -1; dy < 2; ++dy)
+                for (int dx = -1; dx < 2; ++dx) {
+                    int tx = gx + dx, ty = gy + dy;
+                    const int tz = gz + dz;
+                    const int code = find_dist_mod(p, tx, ty, tz);
+                    if (code < 0) continue;
+                    const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                                          (p.num[0] * p.num[1]) * (uint32_t)tz;
+                    const uint32_t start = a.cell_start[cell];
+                    if (start == 0xffffffffu) continue;
+                    const uint32_t end = a.cell_end[cell];
+                    const d3 dm = shift_of(p, (uint32_t)code);
+                    for (uint32_t ib = start; ib < end; ++ib) {
+                        if (ib == ia) continue;
+                        const double4 xb = a.xs[ib];
+                        const double rb = xb.w;
+                        const double kappa = p.kappa;
+                        d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                        xab += dm;
+                        const double len = length(xab);
+                        const double delta = len - (ra + rb);
+                        if (delta < 0.0) {
+                            const uint32_t ib_orig = a.perm[ib];
+                            contact_linear_viscous(p, A, vel_a, angvel_a, a.vel[ib_orig],
+                                                   xyz(a.angvel[ib_orig]), ra, rb, xab, len,
+                                                   delta, kappa);
+                        } else if (delta < p.db) {
+                            capillary_cohesion_exp(p, A, ra, rb, delta, xab, len, kappa);
+                        }
+                    }
+                }
+    }
+    // Walls, src/contactsearch.cuh:602-701: top, bottom, then the side walls.
+    double4 wn[5];
+    double wv[5];
+    wn[0] = make_double4(0.0, 0.0, -1.0, Lz);
+    wn[1] = make_double4(-1.0, 0.0, 0.0, p.L[0]);
+    wn[2] = make_double4(1.0, 0.0, 0.0, 0.0);
+    wn[3] = make_double4(0.0, -1.0, 0.0, p.L[1]);
+    wn[4] = make_double4(0.0, 1.0, 0.0, 0.0);
+#pragma unroll
+    for (int w = 0; w < 5; ++w) wv[w] = 0.0;
+    for (uint32_t w = 0; w < p.nw && w < 5; ++w) {
+        wn[w] = a.w_nx[w];
+        wv[w] = a.w_mvfd[w].y;
+    }
+    double wf[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    // (wall index, delta) in the reference's order; index 5 = bottom wall.
+    for (int k = 0; k < 6; ++k) {
+        int w;
+        double dw;
+        d3 n;
+        double vel_w;
+        if (k == 0) {
+            w = 0; dw = wn[0].w - (xa.z + ra); n = xyz(wn[0]); vel_w = wv[0];
+        } else if (k == 1) {
+            w = 5; dw = xa.z - ra - p.origo[2]; n = mk3(0.0, 0.0, 1.0); vel_w = 0.0;
+        } else {
+            if (p.periodic == 0) {
+                w = k - 1;
+            } else if (p.periodic == 2) {
+                if (k < 4) continue;
+                w = k - 1;
+            } else {
+                break;
+            }
+            if (w == 1) dw = wn[1].w - (xa.x + ra);
+            else if (w == 2) dw = xa.x - ra - wn[2].w;
+            else if (w == 3) dw = wn[3].w - (xa.y + ra);
+            else dw = xa.y - ra - wn[4].w;
+            n = xyz(wn[w]);
+            vel_w = wv[w];
+        }
+        if (dw < 0.0) {
+            const double f = contact_linear_wall(p, A, ra, vel_a, av_a, n, dw, vel_w);
+            if (w < 5) wf[w] = f;
+        }
+    }
+    // Outputs at the original index, src/contactsearch.cuh:706-724.
+    const uint32_t o = ia_orig;
+    a.force[o] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
+    a.torque[o] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
+    a.es_dot[o] = A.es_dot;
+    a.ev_dot[o] = A.ev_dot;
+    a.es[o] += A.es_dot * p.dt;
+    a.ev[o] += A.ev_dot * p.dt;
+    a.p[o] = A.p;
+    for (uint32_t w = 0; w < p.nw && w < 5; ++w) a.force_pp[(size_t)w * np + o] = wf[w];
+}
+
+// ---------------------------------------------------------------------------
+// Block tree sum of the reference's `summation` kernel (one element per
+// work-item, then the halving tree, src/integration.cuh:309-341).
+
+__device__ __forceinline__ double block_tree_sum(double v, double* cache) {
+    double temp = 0.0;
+    temp += v;
+    cache[threadIdx.x] = temp;
+    __syncthreads();
+    for (unsigned i = BLOCK / 2; i != 0; i /= 2) {
+        if (threadIdx.x < i) cache[threadIdx.x] += cache[threadIdx.x + i];
+        __syncthreads();
+    }
+    return cache[0];
+}
+
+// Arrival counter: returns true in exactly one (the last) workgroup, after
+// every workgroup's partial is visible.
+__device__ __forceinline__ bool last_block(uint32_t* counter, uint32_t nblocks) {
+    __shared__ uint32_t is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t prev = atomicAdd(counter, 1u);
+        is_last = (prev == nblocks - 1) ? 1u : 0u;
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last != 0u;
+}
+
+// findShearStressOnFixedMovingParticles + summation + the serial sum of
+// integrate (src/integration.cuh:457-484, 309-341, 64-68).
+__global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const DevArrays a) {
+    __shared__ double cache[BLOCK];
+    const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+    const double Lz = grid_Lz(p, a);
+    double v = 0.0;
+    bool in = idx < p.np;
+    if (in) {
+        const double z = a.x[idx].z;
+        const double fixvel = a.vel[idx].w;
+        const double force_x = a.force[idx].x;
+        double f_x = 0.0;
+        if (fixvel > 0.0 && z > Lz * 0.5) f_x = force_x;
+        v = f_x / (p.L[0] * p.L[1]);
+        a.tau_pp[idx] = v;
+    }
+    const double s = block_tree_sum(in ? v : 0.0, cache);
+    if (threadIdx.x == 0) a.tau_partial[blockIdx.x] = s;
+    if (last_block(&a.loop->tau_blocks_done, gridDim.x) && threadIdx.x == 0) {
+        double tau = 0.0;
+        for (uint32_t i = 0; i < p.nblocks; ++i) tau += a.tau_partial[i];
+        a.tau_eff[0] = tau;
+        a.loop->tau_blocks_done = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// integrate (TY3), src/integration.cuh:14-305, one work-item per particle in
+// original order (the reference's sorted-order pass reads copies of the
+// same values; every output is per particle).
+
+__global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const DevArrays a) {
+    const uint32_t orig = blockIdx.x * BLOCK + threadIdx.x;
+    if (orig >= p.np) return;
+    const DevLoop* lp = a.loop;
+    const double t_now = lp->t;
+    const int64_t iter = lp->iter;
+    // Rate-state switch decided by the host loop before integrate
+    // (src/device.cu:2454-2462).
+    int cvs = 0;
+    if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) cvs = 1;
+    else if (t_now >= 15.0 && lp->velocity_state == 2) cvs = -1;
+    const double Lz = grid_Lz(p, a);
+    const double4 force = a.force[orig];
+    const double4 torque = a.torque[orig];
+    const double4 angpos = a.angpos[orig];
+    const double4 x = a.x[orig];
+    const double4 vel = a.vel[orig];
+    const double4 angvel = a.angvel[orig];
+    double4 xyzsum = a.xyzsum[orig];
+    const int wall0mode = p.nw > 0 ? p.wmode[0] : 0;
+    const double wall0mass = p.nw > 0 ? a.w_mvfd[0].x : 0.0;
+    double tau_eff_x = 0.0;
+    if (p.nw > 0 && wall0mode == 3) tau_eff_x = a.tau_eff[0];
+    double4 acc0, angacc0;
+    if (iter == 0) {
+        acc0 = make_double4(0.0, 0.0, 0.0, 0.0);
+        angacc0 = make_double4(0.0, 0.0, 0.0, 0.0);
+    } else {
+        acc0 = a.acc[orig];
+        angacc0 = a.angacc[orig];
+    }
+    const double radius = x.w;
+    const double dt = p.dt;
+    const double m = 4.0 / 3.0 * PI * radius * radius * radius * p.rho;
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    acc.x = force.x / m + p.g[0];
+    acc.y = force.y / m + p.g[1];
+    acc.z = force.z / m + p.g[2];
+    double4 angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    angacc.x = torque.x * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    angacc.y = torque.y * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    angacc.z = torque.z * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    if (wall0mode == 3 && vel.w > 0.0001 && x.z > Lz * 0.5) {
+        const double f_tau_x = (p.tau_x + tau_eff_x) * p.L[0] * p.L[1];
+        acc.x = f_tau_x / wall0mass;
+        acc.y = 0.0;
+        acc.z -= p.g[2];
+        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    } else if (vel.w > 0.0001 && vel.w < 10.0) {
+        acc.x = 0.0;
+        acc.y = 0.0;
+        acc.z -= p.g[2];
+        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    if (vel.w >= 10.0) {
+        acc.y = 0.0;
+        angacc.x = 0.0;
+        angacc.z = 0.0;
+    }
+    if (vel.w < -0.0001) {
+        acc.x = 0.0;
+        acc.y = 0.0;
+        acc.z = 0.0;
+        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    const d3 dacc_dt = mk3((acc.x - acc0.x) / dt, (acc.y - acc0.y) / dt, (acc.z - acc0.z) / dt);
+    const d3 dangacc_dt = mk3((angacc.x - angacc0.x) / dt, (angacc.y - angacc0.y) / dt,
+                              (angacc.z - angacc0.z) / dt);
+    double4 x_new, vel_new, angpos_new, angvel_new;
+    x_new.x = x.x + vel.x * dt + 0.5 * acc.x * dt * dt + 1.0 / 6.0 * dacc_dt.x * dt * dt * dt;
+    x_new.y = x.y + vel.y * dt + 0.5 * acc.y * dt * dt + 1.0 / 6.0 * dacc_dt.y * dt * dt * dt;
+    x_new.z = x.z + vel.z * dt + 0.5 * acc.z * dt * dt + 1.0 / 6.0 * dacc_dt.z * dt * dt * dt;
+    x_new.w = x.w;
+    vel_new.x = vel.x + acc.x * dt + 0.5 * dacc_dt.x * dt * dt;
+    vel_new.y = vel.y + acc.y * dt + 0.5 * dacc_dt.y * dt * dt;
+    vel_new.z = vel.z + acc.z * dt + 0.5 * dacc_dt.z * dt * dt;
+    vel_new.w = vel.w;
+    angpos_new.x = angpos.x + angvel.x * dt + 0.5 * angacc.x * dt * dt +
+                   1.0 / 6.0 * dangacc_dt.x * dt * dt * dt;
+    angpos_new.y = angpos.y + angvel.y * dt + 0.5 * angacc.y * dt * dt +
+                   1.0 / 6.0 * dangacc_dt.y * dt * dt * dt;
+    angpos_new.z = angpos.z + angvel.z * dt + 0.5 * angacc.z * dt * dt +
+                   1.0 / 6.0 * dangacc_dt.z * dt * dt * dt;
+    angpos_new.w = angpos.w;
+    angvel_new.x = angvel.x + angacc.x * dt + 0.5 * dangacc_dt.x * dt * dt;
+    angvel_new.y = angvel.y + angacc.y * dt + 0.5 * dangacc_dt.y * dt * dt;
+    angvel_new.z = angvel.z + angacc.z * dt + 0.5 * dangacc_dt.z * dt * dt;
+    angvel_new.w = angvel.w;
+    xyzsum.x += vel.x * dt + 0.5 * acc.x * dt * dt + 1.0 / 6.0 * dacc_dt.x * dt * dt * dt;
+    xyzsum.y += vel.y * dt + 0.5 * acc.y * dt * dt + 1.0 / 6.0 * dacc_dt.y * dt * dt * dt;
+    xyzsum.z += vel.z * dt + 0.5 * acc.z * dt * dt + 1.0 / 6.0 * dacc_dt.z * dt * dt * dt;
+    // Periodic wrap, src/integration.cuh:267-283 (note `> L`, P17).
+    if (p.periodic == 1) {
+        if (x_new.x < p.origo[0]) x_new.x += p.L[0];
+        if (x_new.x > p.L[0]) x_new.x -= p.L[0];
+        if (x_new.y < p.origo[1]) x_new.y += p.L[1];
+        if (x_new.y > p.L[1]) x_new.y -= p.L[1];
+    } else if (p.periodic == 2) {
+        if (x_new.x < p.origo[0]) x_new.x += p.L[0];
+        if (x_new.x > p.L[0]) x_new.x -= p.L[0];
+    }
+    if (vel.w > 5.0 && vel.w < 10.0) {
+        if (cvs == 1) vel_new.x *= 10.0;
+        else if (cvs == -1) vel_new.x /= 10.0;
+    }
+    a.xyzsum[orig] = xyzsum;
+    a.acc[orig] = acc;
+    a.angacc[orig] = angacc;
+    a.angvel[orig] = angvel_new;
+    a.vel[orig] = vel_new;
+    a.angpos[orig] = angpos_new;
+    a.x[orig] = x_new;
+}
+
+// ---------------------------------------------------------------------------
+// summation + integrateWalls + loop advance.  Every dynamic wall integrates
+// wall 0's force sum (src/device.cu:2504-2505, P8).
+
+__global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArrays a) {
+    __shared__ double cache[BLOCK];
+    const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+    const bool walls = p.nw > 0 && p.np > 0;
+    if (walls) {
+        const double v = idx < p.np ? a.force_pp[idx] : 0.0;
+        const double s = block_tree_sum(v, cache);
+        if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+    }
+    if (!last_block(&a.loop->wall_blocks_done, gridDim.x)) return;
+    if (threadIdx.x != 0) return;
+    DevLoop* lp = a.loop;
+    const double t_now = lp->t;
+    const double dt = p.dt;
+    if (p.adaptive == 1 && p.nw > 0 && p.np > 0) lp->grid_Lz = a.w_nx[0].w;
+    for (uint32_t w = 0; walls && w < p.nw; ++w) {
+        const int wmode = p.wmode[w];
+        if (wmode == 0) continue;
+        double4 w_nx = a.w_nx[w];
+        double4 w_mvfd = a.w_mvfd[w];
+        double acc0;
+        if (lp->iter == 0) acc0 = 0.0; else acc0 = a.w_acc[w];
+        w_mvfd.z = 0.0;
+        for (uint32_t i = 0; i < p.nblocks; ++i) w_mvfd.z += a.partial[i];
+        const double sigma_0 = w_mvfd.w + p.sigma0_A * sin(2.0 * PI * p.sigma0_f * t_now);
+        const double N = -sigma_0 * p.L[0] * p.L[1];
+        double acc = (w_mvfd.z + N) / w_mvfd.x;
+        if (wmode == 2) acc = 0.0;
+        const double dacc_dt = (acc - acc0) / dt;
+        w_nx.w += w_mvfd.y * dt + 0.5 * acc * dt * dt + 1.0 / 6.0 * dacc_dt * dt * dt * dt;
+        w_mvfd.y += acc * dt + 0.5 * dacc_dt * dt * dt;
+        a.w_nx[w] = w_nx;
+        a.w_mvfd[w] = w_mvfd;
+        a.w_acc[w] = acc;
+    }
+    // Host loop bookkeeping (src/device.cu:2454-2462,2536-2538).
+    if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
+    else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
+    lp->t = t_now + dt;
+    lp->iter = lp->iter + 1;
+    lp->zero_empty = 0u;
+    lp->cells_changed = 0u;
+    lp->key_or = 0u;
+    lp->wall_blocks_done = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Bonds.  k_bond_forces evaluates each bond once (src/cohesion.cuh:8-233);
+// k_bond_gather adds the contributions to each particle in ascending bond
+// order, which is the order the reference's serial reading applies them.
+
+__global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevParams p, const DevArrays a) {
+    const uint32_t b = blockIdx.x * BLOCK + threadIdx.x;
+    if (b >= p.nb0) return;
+    const uint32_t bx = a.bonds[2 * b], by = a.bonds[2 * b + 1];
+    a.bond_ok[b] = 0u;
+    if (!(bx < p.np)) return;
+    const double dt = p.dt;
+    const double4 d0 = a.bonds_delta[b], o0 = a.bonds_omega[b];
+    d3 delta0_t = mk3(d0.x, d0.y, d0.z);
+    const double delta0_n = d0.w;
+    d3 omega0_t = mk3(o0.x, o0.y, o0.z);
+    const double omega0_n = o0.w;
+    const double4 x_i = a.x[bx], x_j = a.x[by];
+    const double4 vel_i = a.vel[bx], vel_j = a.vel[by];
+    const d3 angvel_i = xyz(a.angvel[bx]), angvel_j = xyz(a.angvel[by]);
+    const double R_bar = p.lambda_bar * fmin(x_i.w, x_j.w);
+    const double A = PI * R_bar * R_bar;
+    const double I = 0.25 * PI * R_bar * R_bar * R_bar * R_bar;
+    const double J = I * 2.0;
+    const d3 x = mk3(x_i.x - x_j.x, x_i.y - x_j.y, x_i.z - x_j.z);
+    const double x_length = length(x);
+    const double overlap = fmin(0.0, x_length - (x_i.w + x_j.w));
+    const d3 n = x / x_length;
+    delta0_t = delta0_t - (n * dot(n, delta0_t));
+    const d3 ddelta = (mk3(vel_i.x - vel_j.x, vel_i.y - vel_j.y, vel_i.z - vel_j.z) +
+                       (x_i.w + overlap / 2.0) * cross(n, angvel_i) +
+                       (x_j.w + overlap / 2.0) * cross(n, angvel_j)) * dt;
+    const double ddelta_n = -dot(ddelta, n);
+    const double delta_n = delta0_n + ddelta_n;
+    const d3 ddelta_t = ddelta - n * dot(n, ddelta);
+    const d3 delta_t = delta0_t + ddelta_t;
+    const d3 f_n = (p.k_n * A * delta_n + p.gamma_n * ddelta_n / dt) * n;
+    const d3 f_t = -p.k_t * A * delta_t - p.gamma_t * ddelta_t / dt;
+    const d3 f = f_n + f_t;
+    const d3 mn = -n;
+    omega0_t = omega0_t - mn * dot(omega0_t, mn);
+    const d3 domega = mk3(angvel_j.x - angvel_i.x, angvel_j.y - angvel_i.y,
+                          angvel_j.z - angvel_i.z) * dt;
+    const double domega_n = dot(domega, mn);
+    const double omega_n = omega0_n + domega_n;
+    const d3 domega_t = domega - domega_n * mn;
+    const d3 omega_t = omega0_t + domega_t;
+    const d3 t_n = -p.k_t * J * omega_n * mn;
+    const d3 t_t = -p.k_n * I * omega_t;
+    const d3 t_j = t_n + t_t;
+    const double sigma_max = length(f_n) / A + length(t_t) * R_bar / I;
+    const double tau_max = length(f_t) / A + length(t_n) * R_bar / J;
+    if (sigma_max >= p.sigma_b || tau_max >= p.tau_b) {
+        a.bonds[2 * b] = p.nb0;  // src/cohesion.cuh:205
+        if (p.nb0 < p.np) atomicOr(&a.loop->bond_retargeted, 1u);
+        return;
+    }
+    a.bonds_delta[b] = make_double4(delta_t.x, delta_t.y, delta_t.z, delta_n);
+    a.bonds_omega[b] = make_double4(omega_t.x, omega_t.y, omega_t.z, omega_n);
+    a.bond_f[b] = make_double4(f.x, f.y, f.z, 0.0);
+    a.bond_t[b] = make_double4(t_j.x, t_j.y, t_j.z, 0.0);
+    a.bond_ok[b] = 1u;
+}
+
+// Per-particle gather over the bonds that touch the particle, in ascending
+// bond id (CSR of the initial bond list, built at open).  A broken bond has
+// its first index reset to nb0 (src/cohesion.cuh:205); once that has
+// happened, particle nb0 scans the whole bond list instead.
+__device__ __forceinline__ void apply_bond(const DevArrays& a, uint32_t i, uint32_t b,
+                                           double4& F, double4& T, bool& touched) {
+    if (!a.bond_ok[b]) return;
+    const uint32_t bx = a.bonds[2 * b], by = a.bonds[2 * b + 1];
+    if (bx != i && by != i) return;
+    const double4 f = a.bond_f[b], t = a.bond_t[b];
+    if (bx == i) {
+        F.x += f.x; F.y += f.y; F.z += f.z; F.w += 0.0;
+        T.x -= t.x; T.y -= t.y; T.z -= t.z; T.w -= 0.0;
+    }
+    if (by == i) {
+        F.x -= f.x; F.y -= f.y; F.z -= f.z; F.w -= 0.0;
+        T.x += t.x; T.y += t.y; T.z += t.z; T.w += 0.0;
+    }
+    touched = true;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevParams p, const DevArrays a) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= p.np) return;
+    const bool full = (i == p.nb0) && a.loop->bond_retargeted != 0u;
+    const uint32_t b0 = full ? 0u : a.badj_off[i];
+    const uint32_t b1 = full ? p.nb0 : a.badj_off[i + 1];
+    if (b0 == b1) return;
+    double4 F = a.force[i], T = a.torque[i];
+    bool touched = false;
+    for (uint32_t k = b0; k < b1; ++k) apply_bond(a, i, full ? k : a.badj[k], F, T, touched);
+    if (touched) {
+        a.force[i] = F;
+        a.torque[i] = T;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launch wrappers
+
+hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(n)), dim3(BLOCK), 0, s, dst, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_hash(const DevParams& p, const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort(const DevParams& p, const DevArrays& a, int passes, hipStream_t s) {
+    const uint32_t n = p.np;
+    const uint32_t nt = n_tiles(n);
+    for (int pass = 0; pass < passes; ++pass) {
+        const int shift = 8 * pass;
+        const bool to_final = ((passes - 1 - pass) % 2) == 0;
+        const uint32_t* kin;
+        const uint32_t* vin;
+        if (pass == 0) {
+            kin = a.key;
+            vin = nullptr;
+        } else {
+            const bool prev_final = ((passes - pass) % 2) == 0;
+            kin = prev_final ? a.skey : a.key_tmp;
+            vin = prev_final ? a.perm : a.val_tmp;
+        }
+        uint32_t* kout = to_final ? a.skey : a.key_tmp;
+        uint32_t* vout = to_final ? a.perm : a.val_tmp;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, kin, n, shift, a.rhist, nt);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.rhist, 256u * nt);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, kin, vin, kout, vout, n,
+                           shift, a.rhist, nt);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits, hipStream_t s) {
+    hipError_t e = launch_fill_u32(a.cell_start, p.ncells, 0xffffffffu, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cells, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a, sorted_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_contact(const DevParams& p, const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_contact, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bonds(const DevParams& p, const DevArrays& a, hipStream_t s) {
+    if (p.nb0 == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bond_forces, dim3(n_blocks(p.nb0)), dim3(BLOCK), 0, s, p, a);
+    hipLaunchKernelGGL(k_bond_gather, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_shear_stress(const DevParams& p, const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_shear_stress, dim3(p.nblocks), dim3(BLOCK), 0, s, p, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate(const DevParams& p, const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_integrate, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_walls(const DevParams& p, const DevArrays& a, hipStream_t s) {
+    const uint32_t nb = (p.nw > 0 && p.np > 0) ? p.nblocks : 1u;
+    hipLaunchKernelGGL(k_walls, dim3(nb), dim3(BLOCK), 0, s, p, a);
+    return hipGetLastError();
+}
+
+}  // namespace sphx

@@ -1,0 +1,45 @@
+// Launch wrappers of the DEM step kernels (definitions in dem_kernels.hip).
+#ifndef SPHERE_AMD_DEM_KERNELS_HPP_
+#define SPHERE_AMD_DEM_KERNELS_HPP_
+
+#include <hip/hip_runtime.h>
+
+#include "dem_arrays.hpp"
+#include "dem_types.hpp"
+
+namespace sphx {
+
+constexpr int RADIX_ITEMS = 8;                        // keys per thread per tile
+constexpr int RADIX_TILE = BLOCK * RADIX_ITEMS;       // keys per tile
+
+inline unsigned n_tiles(unsigned n) { return (n + RADIX_TILE - 1) / RADIX_TILE; }
+inline unsigned n_blocks(unsigned n) { return (n + BLOCK - 1) / BLOCK; }
+
+// Cell key of every particle (calcParticleCellID, src/sorting.cuh:25-51).
+hipError_t launch_hash(const DevParams& p, const DevArrays& a, hipStream_t s);
+// Stable LSD radix sort of (key, identity) into (skey, perm), `passes` digits
+// of 8 bits (replaces thrust::sort_by_key, src/device.cu:1075-1078).
+hipError_t launch_sort(const DevParams& p, const DevArrays& a, int passes, hipStream_t s);
+// cellStart reset, cellStart/cellEnd, sorted position copy
+// (src/device.cu:1089-1090, src/sorting.cuh:56-132).
+hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits,
+                        hipStream_t s);
+// topology + interact fused per particle (src/contactsearch.cuh:194-726).
+hipError_t launch_contact(const DevParams& p, const DevArrays& a, hipStream_t s);
+// bondsLinear per bond, then a per-particle gather in bond order
+// (src/cohesion.cuh:8-233).
+hipError_t launch_bonds(const DevParams& p, const DevArrays& a, hipStream_t s);
+// findShearStressOnFixedMovingParticles + summation + serial sum
+// (src/integration.cuh:457-484,309-341,64-68), wmode 3 only.
+hipError_t launch_shear_stress(const DevParams& p, const DevArrays& a, hipStream_t s);
+// integrate (TY3), src/integration.cuh:14-305.
+hipError_t launch_integrate(const DevParams& p, const DevArrays& a, hipStream_t s);
+// summation + integrateWalls + loop advance (src/integration.cuh:309-450,
+// src/device.cu:2500-2538).
+hipError_t launch_walls(const DevParams& p, const DevArrays& a, hipStream_t s);
+// Fill n words with v.
+hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s);
+
+}  // namespace sphx
+
+#endif

@@ -1,0 +1,82 @@
+// Types shared by the host engine and the HIP kernels.
+//
+// The parameter block replaces the reference's __constant__ singletons
+// (devC_params, devC_grid, devC_np, devC_nw, devC_dt; src/constants.cuh:10-17,
+// src/datatypes.h:45-101) with one struct passed by value to every kernel.
+#ifndef SPHERE_AMD_DEM_TYPES_HPP_
+#define SPHERE_AMD_DEM_TYPES_HPP_
+
+#include <stdint.h>
+
+namespace sphx {
+
+constexpr int NC = 32;          // contacts per particle, src/constants.h:21
+constexpr int MAXWALLS = 6;     // src/constants.h:12
+constexpr int BLOCK = 256;      // particles per workgroup in the per-particle kernels
+
+// A contact-row word packs the partner's original index with the periodic
+// image it was found through.  The image shift components are exactly
+// 0.0, +L or -L (src/contactsearch.cuh:17-58 adds/subtracts L to a zero
+// vector), so two bits per periodic axis reproduce them bit for bit.
+constexpr uint32_t CW_IDX_BITS = 28;
+constexpr uint32_t CW_IDX_MASK = (1u << CW_IDX_BITS) - 1u;
+constexpr uint32_t CW_SHIFT_X = 28;   // 0: none, 1: +L[0], 2: -L[0]
+constexpr uint32_t CW_SHIFT_Y = 30;   // 0: none, 1: +L[1], 2: -L[1]
+constexpr uint32_t MAX_NP = 1u << 27; // the reference's own u32 row index limit
+
+struct DevParams {
+    // grid (src/datatypes.h:45-51)
+    double origo[3];
+    double L[3];
+    uint32_t num[3];
+    int32_t periodic;
+    int32_t adaptive;
+    uint32_t ncells;
+    // sizes
+    uint32_t np;
+    uint32_t nw;
+    uint32_t nb0;
+    uint32_t nblocks;     // ceil(np / 256): the reference's blocksPerGrid
+    double dt;
+    // physical parameters (src/datatypes.h:74-101)
+    double g[3];
+    double k_n, k_t, k_r, E;
+    double gamma_n, gamma_t, gamma_r;
+    double mu_s, mu_d, mu_r;
+    double gamma_wn, gamma_wt, mu_ws, mu_wd;
+    double rho;
+    uint32_t contactmodel;
+    double kappa, db, V_b;
+    double lambda_bar, sigma_b, tau_b;
+    double sigma0_A, sigma0_f;
+    double tau_x;
+    int32_t wmode[MAXWALLS];
+};
+
+// Loop state kept on the device so that captured step sequences can be
+// replayed: time.current, iter, and the rate-state velocity switch
+// (src/device.cu:919,993-999,2454-2462,2536-2538).
+struct DevLoop {
+    double t;
+    int64_t iter;
+    double grid_Lz;             // L[2] used by the current step (adaptive grid)
+    int32_t velocity_state;
+    uint32_t wall_blocks_done;  // arrival counter of the wall reduction
+    uint32_t tau_blocks_done;   // arrival counter of the shear-stress reduction
+    uint32_t cells_changed;     // some particle's cell key changed this step
+    uint32_t key_or;            // OR of all cell keys (pass count check)
+    uint32_t error_bits;        // sticky device-side flags (DevFlag)
+    uint32_t zero_empty;        // zero delta_t of empty slots this step
+    uint32_t bond_retargeted;   // some broken bond now names particle nb0
+};
+
+enum DevFlag : uint32_t {
+    DFLAG_KEY_RANGE = 1u,          // a cell key exceeds the sorted bit range
+    DFLAG_CONTACT_OVERFLOW = 2u,   // more than NC overlaps for a particle (P5);
+                                   // informational, the reference's behaviour
+    DFLAG_NONFINITE = 4u,          // NaN position at hashing (src/sorting.cuh:39-43)
+};
+
+}  // namespace sphx
+
+#endif

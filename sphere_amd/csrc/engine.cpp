@@ -1,0 +1,1142 @@
+// Host side of the sphere_hip C ABI: context lifetime, .bin I/O, input
+// validation, the step loop and the output loop of DEM::startTime.
+//
+// Reference behaviour followed here: DEM::DEM and checkValues
+// (src/sphere.cpp:16-95,145-345), readbin/writebin (src/file_io.cpp:30-731),
+// DEM::startTime (src/device.cu:826-2700).  The library reports errors
+// through return codes and sph_last_error() instead of exit().
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/sphere_hip.h"
+#include "dem_arrays.hpp"
+#include "dem_kernels.hpp"
+#include "dem_types.hpp"
+
+using namespace sphx;
+
+namespace {
+
+thread_local std::string g_open_error;
+
+struct Host {
+    // header (src/file_io.cpp:47-114)
+    double version = SPH_VERSION;
+    int32_t nd = 3;
+    uint32_t np = 0;
+    double dt = 0, t = 0, t_total = 0, file_dt = 0;
+    uint32_t step_count = 0;
+    double origo[3] = {0, 0, 0}, L[3] = {0, 0, 0};
+    uint32_t num[3] = {0, 0, 0};
+    int32_t periodic = 0, adaptive = 0;
+    // particles, AoS 4 doubles as on the device
+    std::vector<double> x, xyzsum, vel, force, angpos, angvel, torque;
+    std::vector<double> es_dot, es, ev_dot, ev, p;
+    std::vector<int32_t> color;
+    // params (src/datatypes.h:74-101)
+    double g[3] = {0, 0, 0};
+    double k_n = 0, k_t = 0, k_r = 0, E = 0, gamma_n = 0, gamma_t = 0, gamma_r = 0;
+    double mu_s = 0, mu_d = 0, mu_r = 0, gamma_wn = 0, gamma_wt = 0, mu_ws = 0, mu_wd = 0;
+    double rho = 0;
+    uint32_t contactmodel = 0;
+    double kappa = 0, db = 0, V_b = 0;
+    uint32_t nw = 0;
+    int32_t wmode[MAXWALLS] = {0, 0, 0, 0, 0, 0};
+    std::vector<double> w_nx, w_mvfd;  // nw x 4
+    double sigma0_A = 0, sigma0_f = 0, tau_x = 0;
+    double lambda_bar = 1, sigma_b = 0, tau_b = 0;
+    uint32_t nb0 = 0;
+    std::vector<uint32_t> bonds;       // nb0 x 2
+    std::vector<double> bonds_delta, bonds_omega;  // nb0 x 4 (t.x, t.y, t.z, n)
+};
+
+struct Reader {
+    const unsigned char* p;
+    size_t n, off;
+    bool ok;
+    template <class T> T get() {
+        T v;
+        if (off + sizeof(T) > n) {
+            ok = false;
+            memset(&v, 0, sizeof(T));
+            return v;
+        }
+        memcpy(&v, p + off, sizeof(T));
+        off += sizeof(T);
+        return v;
+    }
+    void getv(double* dst, size_t k) { for (size_t i = 0; i < k; ++i) dst[i] = get<double>(); }
+};
+
+struct Writer {
+    std::vector<unsigned char> b;
+    template <class T> void put(const T& v) {
+        const unsigned char* q = reinterpret_cast<const unsigned char*>(&v);
+        b.insert(b.end(), q, q + sizeof(T));
+    }
+};
+
+}  // namespace
+
+struct sph_ctx {
+    Host h;
+    DevParams P;
+    DevArrays A;
+    std::vector<void*> allocs;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    unsigned flags = 0;
+    std::string err;
+    int sorted_bits = 32;
+    int passes = 4;
+    bool shear_stress = false;
+    bool profile = false;
+    // host mirror of the loop state
+    int64_t iter = 0;
+    // per-kernel profile (stage name, total ms, launches)
+    std::vector<std::string> prof_names;
+    std::vector<double> prof_ms;
+    std::vector<long> prof_n;
+};
+
+namespace {
+
+int fail(sph_ctx* c, int code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_open_error = buf;
+    return code;
+}
+
+int hip_fail(sph_ctx* c, hipError_t e, const char* where) {
+    return fail(c, SPH_E_DEVICE, "%s: %s", where, hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr, where)                        \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) return hip_fail(c, e_, where); \
+    } while (0)
+
+// .bin v2.15 reader, fluid == 0 (src/file_io.cpp:30-418).
+int parse_bin(Host& h, const void* bytes, size_t nbytes, std::string& err) {
+    Reader r = {static_cast<const unsigned char*>(bytes), nbytes, 0, true};
+    h.version = r.get<double>();
+    if (!r.ok) { err = "Could not read input binary file (empty)"; return SPH_E_IO; }
+    if (h.version != SPH_VERSION) {
+        char m[256];
+        snprintf(m, sizeof m,
+                 "Error: The input file is written by sphere version %g, which is "
+                 "incompatible with this version (%g).", h.version, SPH_VERSION);
+        err = m;
+        return SPH_E_VERSION;
+    }
+    h.nd = r.get<int32_t>();
+    h.np = r.get<uint32_t>();
+    if (h.nd != 3) {
+        err = "Dimensionality mismatch between dataset and this SPHERE program.";
+        return SPH_E_INPUT;
+    }
+    h.dt = r.get<double>(); h.t = r.get<double>(); h.t_total = r.get<double>();
+    h.file_dt = r.get<double>(); h.step_count = r.get<uint32_t>();
+    r.getv(h.origo, 3); r.getv(h.L, 3);
+    for (int d = 0; d < 3; ++d) h.num[d] = r.get<uint32_t>();
+    h.periodic = r.get<int32_t>(); h.adaptive = r.get<int32_t>();
+    const size_t np = h.np;
+    if (!r.ok || np * 228 > nbytes - std::min(nbytes, r.off)) {
+        err = "short read: file too small for np particles";
+        return SPH_E_IO;
+    }
+    h.x.assign(np * 4, 0.0); h.xyzsum.assign(np * 4, 0.0); h.vel.assign(np * 4, 0.0);
+    h.force.assign(np * 4, 0.0); h.angpos.assign(np * 4, 0.0); h.angvel.assign(np * 4, 0.0);
+    h.torque.assign(np * 4, 0.0);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.x[4 * i], 4);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.xyzsum[4 * i], 3);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.vel[4 * i], 4);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.force[4 * i], 3);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.angpos[4 * i], 3);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.angvel[4 * i], 3);
+    for (size_t i = 0; i < np; ++i) r.getv(&h.torque[4 * i], 3);
+    std::vector<double>* e[5] = {&h.es_dot, &h.es, &h.ev_dot, &h.ev, &h.p};
+    for (int k = 0; k < 5; ++k) { e[k]->assign(np, 0.0); r.getv(e[k]->data(), np); }
+    r.getv(h.g, 3);
+    h.k_n = r.get<double>(); h.k_t = r.get<double>(); h.k_r = r.get<double>();
+    h.E = r.get<double>(); h.gamma_n = r.get<double>(); h.gamma_t = r.get<double>();
+    h.gamma_r = r.get<double>(); h.mu_s = r.get<double>(); h.mu_d = r.get<double>();
+    h.mu_r = r.get<double>(); h.gamma_wn = r.get<double>(); h.gamma_wt = r.get<double>();
+    h.mu_ws = r.get<double>(); h.mu_wd = r.get<double>(); h.rho = r.get<double>();
+    h.contactmodel = r.get<uint32_t>();
+    h.kappa = r.get<double>(); h.db = r.get<double>(); h.V_b = r.get<double>();
+    h.nw = r.get<uint32_t>();
+    if (!r.ok) { err = "short read"; return SPH_E_IO; }
+    if (h.nw > (uint32_t)MAXWALLS) {
+        char m[160];
+        snprintf(m, sizeof m, "Error; MAXWALLS (%d) in datatypes.h is smaller than the number "
+                 "of walls specified in the input file (%u).", MAXWALLS, h.nw);
+        err = m;
+        return SPH_E_INPUT;
+    }
+    for (uint32_t i = 0; i < h.nw; ++i) h.wmode[i] = r.get<int32_t>();
+    h.w_nx.assign(h.nw * 4, 0.0); h.w_mvfd.assign(h.nw * 4, 0.0);
+    for (uint32_t i = 0; i < h.nw; ++i) r.getv(&h.w_nx[4 * i], 4);
+    for (uint32_t i = 0; i < h.nw; ++i) r.getv(&h.w_mvfd[4 * i], 4);
+    h.sigma0_A = r.get<double>(); h.sigma0_f = r.get<double>(); h.tau_x = r.get<double>();
+    h.lambda_bar = r.get<double>(); h.nb0 = r.get<uint32_t>();
+    h.sigma_b = r.get<double>(); h.tau_b = r.get<double>();
+    if (!r.ok || (size_t)h.nb0 * 72 > nbytes - std::min(nbytes, r.off)) {
+        err = "short read";
+        return SPH_E_IO;
+    }
+    const size_t nb = h.nb0;
+    h.bonds.assign(nb * 2, 0u); h.bonds_delta.assign(nb * 4, 0.0); h.bonds_omega.assign(nb * 4, 0.0);
+    for (size_t i = 0; i < nb; ++i) { h.bonds[2 * i] = r.get<uint32_t>(); h.bonds[2 * i + 1] = r.get<uint32_t>(); }
+    for (size_t i = 0; i < nb; ++i) h.bonds_delta[4 * i + 3] = r.get<double>();
+    for (size_t i = 0; i < nb; ++i) r.getv(&h.bonds_delta[4 * i], 3);
+    for (size_t i = 0; i < nb; ++i) h.bonds_omega[4 * i + 3] = r.get<double>();
+    for (size_t i = 0; i < nb; ++i) r.getv(&h.bonds_omega[4 * i], 3);
+    h.color.assign(np, 0);
+    for (size_t i = 0; i < np; ++i) h.color[i] = r.get<int32_t>();
+    if (!r.ok) { err = "short read"; return SPH_E_IO; }
+    return SPH_OK;
+}
+
+void write_bin(const Host& h, Writer& w) {
+    w.put(h.version); w.put(h.nd); w.put(h.np);
+    w.put(h.dt); w.put(h.t); w.put(h.t_total); w.put(h.file_dt); w.put(h.step_count);
+    for (int d = 0; d < 3; ++d) w.put(h.origo[d]);
+    for (int d = 0; d < 3; ++d) w.put(h.L[d]);
+    for (int d = 0; d < 3; ++d) w.put(h.num[d]);
+    w.put(h.periodic); w.put(h.adaptive);
+    const size_t np = h.np;
+    w.b.reserve(w.b.size() + np * 228 + 4096);
+    for (size_t i = 0; i < np; ++i) for (int k = 0; k < 4; ++k) w.put(h.x[4 * i + k]);
+    for (size_t i = 0; i < np; ++i) for (int k = 0; k < 3; ++k) w.put(h.xyzsum[4 * i + k]);
+    for (size_t i = 0; i < np; ++i) for (int k = 0; k < 4; ++k) w.put(h.vel[4 * i + k]);
+    const std::vector<double>* v3[4] = {&h.force, &h.angpos, &h.angvel, &h.torque};
+    for (int a = 0; a < 4; ++a)
+        for (size_t i = 0; i < np; ++i) for (int k = 0; k < 3; ++k) w.put((*v3[a])[4 * i + k]);
+    const std::vector<double>* e[5] = {&h.es_dot, &h.es, &h.ev_dot, &h.ev, &h.p};
+    for (int a = 0; a < 5; ++a) for (size_t i = 0; i < np; ++i) w.put((*e[a])[i]);
+    for (int d = 0; d < 3; ++d) w.put(h.g[d]);
+    w.put(h.k_n); w.put(h.k_t); w.put(h.k_r); w.put(h.E); w.put(h.gamma_n); w.put(h.gamma_t);
+    w.put(h.gamma_r); w.put(h.mu_s); w.put(h.mu_d); w.put(h.mu_r); w.put(h.gamma_wn);
+    w.put(h.gamma_wt); w.put(h.mu_ws); w.put(h.mu_wd); w.put(h.rho); w.put(h.contactmodel);
+    w.put(h.kappa); w.put(h.db); w.put(h.V_b);
+    w.put(h.nw);
+    for (uint32_t i = 0; i < h.nw; ++i) w.put(h.wmode[i]);
+    for (uint32_t i = 0; i < h.nw; ++i) for (int k = 0; k < 4; ++k) w.put(h.w_nx[4 * i + k]);
+    for (uint32_t i = 0; i < h.nw; ++i) for (int k = 0; k < 4; ++k) w.put(h.w_mvfd[4 * i + k]);
+    w.put(h.sigma0_A); w.put(h.sigma0_f); w.put(h.tau_x);
+    w.put(h.lambda_bar); w.put(h.nb0); w.put(h.sigma_b); w.put(h.tau_b);
+    const size_t nb = h.nb0;
+    for (size_t i = 0; i < nb; ++i) { w.put(h.bonds[2 * i]); w.put(h.bonds[2 * i + 1]); }
+    for (size_t i = 0; i < nb; ++i) w.put(h.bonds_delta[4 * i + 3]);
+    for (size_t i = 0; i < nb; ++i) for (int k = 0; k < 3; ++k) w.put(h.bonds_delta[4 * i + k]);
+    for (size_t i = 0; i < nb; ++i) w.put(h.bonds_omega[4 * i + 3]);
+    for (size_t i = 0; i < nb; ++i) for (int k = 0; k < 3; ++k) w.put(h.bonds_omega[4 * i + k]);
+    for (size_t i = 0; i < np; ++i) w.put(h.color[i]);
+}
+
+// checkValues, src/sphere.cpp:145-345 (fluid == 0).
+int check_values(sph_ctx* c) {
+    const Host& h = c->h;
+    if (h.t > h.t_total || h.t < 0.0)
+        return fail(c, SPH_E_INPUT, "Error: time.current = %g s, time.total = %g s", h.t, h.t_total);
+    if (h.origo[0] < 0.0 || h.origo[1] < 0.0 || h.origo[2] < 0.0)
+        return fail(c, SPH_E_INPUT,
+                    "Error: Negative values in grid.origo are known to cause problems. "
+                    "grid.origo[0] = %g m, grid.origo[1] = %g m, grid.origo[2] = %g m.",
+                    h.origo[0], h.origo[1], h.origo[2]);
+    if (h.L[0] <= 0.0 || h.L[1] <= 0.0 || h.L[2] <= 0.0)
+        return fail(c, SPH_E_INPUT, "Error: grid.L[0] = %g m, grid.L[1] = %g m, grid.L[2] = %g m.",
+                    h.L[0], h.L[1], h.L[2]);
+    if (h.num[0] == 0 || h.num[1] == 0 || h.num[2] == 0)
+        return fail(c, SPH_E_INPUT, "Error: grid.num[0] = %u, grid.num[1] = %u, grid.num[2] = %u.",
+                    h.num[0], h.num[1], h.num[2]);
+    if (h.periodic == 2 && h.num[0] < 3)
+        return fail(c, SPH_E_INPUT, "Error: When 1st dimension boundaries are periodic, there "
+                    "must be at least 3 cells in that dimension.");
+    if (h.periodic == 1 && (h.num[0] < 3 || h.num[1] < 3))
+        return fail(c, SPH_E_INPUT, "Error: When 1st and 2nd dimension boundaries are periodic, "
+                    "there must be at least 3 cells in each of those dimensions.");
+    for (uint32_t i = 0; i < h.np; ++i) {
+        const double* x = &h.x[4 * i];
+        const double* v = &h.vel[4 * i];
+        if (x[3] <= 0.0)
+            return fail(c, SPH_E_INPUT, "Error: Particle %u has a radius of %g m.", i, x[3]);
+        for (int k = 0; k < 4; ++k)
+            if (x[k] != x[k])
+                return fail(c, SPH_E_INPUT, "Error: Particle %u has a non-real position", i);
+        for (int k = 0; k < 4; ++k)
+            if (v[k] != v[k])
+                return fail(c, SPH_E_INPUT, "Error: Particle %u has a non-real velocity", i);
+        if (x[0] < h.origo[0] || x[1] < h.origo[1] || x[2] < h.origo[2] || x[0] > h.L[0] ||
+            x[1] > h.L[1] || x[2] > h.L[2])
+            return fail(c, SPH_E_INPUT,
+                        "Error: Particle %u is outside of the computational grid\n"
+                        "k.x[i] = [%g, %g, %g]\ngrid.origo = [%g, %g, %g], grid.L = [%g, %g, %g].",
+                        i, x[0], x[1], x[2], h.origo[0], h.origo[1], h.origo[2], h.L[0], h.L[1], h.L[2]);
+    }
+    if (h.nw > 0) {
+        double z_max = 0.0;
+        for (uint32_t i = 0; i < h.np; ++i)
+            if (h.x[4 * i + 2] > z_max) z_max = h.x[4 * i + 2];
+        if (h.w_nx[3] < z_max)
+            return fail(c, SPH_E_INPUT,
+                        "Error: One or more particles have centres above the upper, dynamic wall");
+    }
+    if (h.k_n <= 0.0) return fail(c, SPH_E_INPUT, "Error: k_n = %g N/m", h.k_n);
+    if (h.E < 0.0) return fail(c, SPH_E_INPUT, "Error: E = %g N/m^3", h.E);
+    if (h.rho <= 0.0) return fail(c, SPH_E_INPUT, "Error: rho = %g kg/m3", h.rho);
+    return SPH_OK;
+}
+
+int bit_length(uint64_t v) {
+    int b = 0;
+    while (v) { ++b; v >>= 1; }
+    return b;
+}
+
+void fill_params(sph_ctx* c) {
+    const Host& h = c->h;
+    DevParams& P = c->P;
+    memset(&P, 0, sizeof P);
+    for (int d = 0; d < 3; ++d) {
+        P.origo[d] = h.origo[d];
+        P.L[d] = h.L[d];
+        P.num[d] = h.num[d];
+        P.g[d] = h.g[d];
+    }
+    P.periodic = h.periodic;
+    P.adaptive = h.adaptive;
+    P.ncells = h.num[0] * h.num[1] * h.num[2];
+    P.np = h.np;
+    P.nw = h.nw;
+    P.nb0 = h.nb0;
+    P.nblocks = (h.np + BLOCK - 1) / BLOCK;
+    P.dt = h.dt;
+    P.k_n = h.k_n; P.k_t = h.k_t; P.k_r = h.k_r; P.E = h.E;
+    P.gamma_n = h.gamma_n; P.gamma_t = h.gamma_t; P.gamma_r = h.gamma_r;
+    P.mu_s = h.mu_s; P.mu_d = h.mu_d; P.mu_r = h.mu_r;
+    P.gamma_wn = h.gamma_wn; P.gamma_wt = h.gamma_wt; P.mu_ws = h.mu_ws; P.mu_wd = h.mu_wd;
+    P.rho = h.rho;
+    P.contactmodel = h.contactmodel;
+    P.kappa = h.kappa; P.db = h.db; P.V_b = h.V_b;
+    P.lambda_bar = h.lambda_bar; P.sigma_b = h.sigma_b; P.tau_b = h.tau_b;
+    P.sigma0_A = h.sigma0_A; P.sigma0_f = h.sigma0_f;
+    P.tau_x = h.tau_x;
+    for (int w = 0; w < MAXWALLS; ++w) P.wmode[w] = h.wmode[w];
+    // Keys reach at most ncells + num[0] + num[0]*num[1] (a particle at
+    // exactly x == L or y == L survives the periodic wrap, P17).
+    const uint64_t kmax = (uint64_t)P.ncells + h.num[0] + (uint64_t)h.num[0] * h.num[1];
+    c->sorted_bits = std::max(1, bit_length(kmax));
+    if (c->sorted_bits > 32) c->sorted_bits = 32;
+    c->passes = (c->sorted_bits + 7) / 8;
+    c->shear_stress = h.nw > 0 && h.wmode[0] == 3;
+}
+
+template <class T> int dev_alloc(sph_ctx* c, T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
+    c->allocs.push_back((void*)*p);
+    return SPH_OK;
+}
+
+#define ALLOC(ptr, n)                               \
+    do {                                            \
+        int rc_ = dev_alloc(c, &(ptr), (size_t)(n)); \
+        if (rc_) return rc_;                         \
+    } while (0)
+
+int allocate(sph_ctx* c) {
+    const size_t np = c->h.np, nw = c->h.nw, nb = c->h.nb0;
+    const size_t nblk = c->P.nblocks, ncells = c->P.ncells;
+    DevArrays& A = c->A;
+    memset(&A, 0, sizeof A);
+    ALLOC(A.x, np); ALLOC(A.vel, np); ALLOC(A.angvel, np); ALLOC(A.angpos, np);
+    ALLOC(A.xyzsum, np); ALLOC(A.acc, np); ALLOC(A.angacc, np); ALLOC(A.force, np);
+    ALLOC(A.torque, np);
+    ALLOC(A.es_dot, np); ALLOC(A.es, np); ALLOC(A.ev_dot, np); ALLOC(A.ev, np); ALLOC(A.p, np);
+    ALLOC(A.key, np); ALLOC(A.key_prev, np); ALLOC(A.skey, np); ALLOC(A.perm, np);
+    ALLOC(A.key_tmp, np); ALLOC(A.val_tmp, np);
+    ALLOC(A.rhist, 256 * (size_t)n_tiles((unsigned)np));
+    ALLOC(A.cell_start, ncells); ALLOC(A.cell_end, ncells);
+    ALLOC(A.xs, np);
+    ALLOC(A.crow, np * NC);
+    ALLOC(A.delta_t, np * NC);
+    ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
+    ALLOC(A.force_pp, nw * np);
+    ALLOC(A.tau_pp, np);
+    ALLOC(A.partial, nblk); ALLOC(A.tau_partial, nblk); ALLOC(A.tau_eff, 1);
+    ALLOC(A.bonds, 2 * nb); ALLOC(A.bonds_delta, nb); ALLOC(A.bonds_omega, nb);
+    ALLOC(A.bond_f, nb); ALLOC(A.bond_t, nb); ALLOC(A.bond_ok, nb);
+    ALLOC(A.badj_off, np + 1);
+    ALLOC(A.badj, 2 * nb);
+    ALLOC(A.loop, 1);
+    return SPH_OK;
+}
+
+int upload_state(sph_ctx* c) {
+    const Host& h = c->h;
+    const size_t np = h.np, nw = h.nw, nb = h.nb0;
+    hipStream_t s = c->stream;
+    DevArrays& A = c->A;
+#define H2D(dst, src, bytes) \
+    HIPCHK(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, s), "H2D")
+    if (np) {
+        H2D(A.x, h.x.data(), np * 32);
+        H2D(A.vel, h.vel.data(), np * 32);
+        H2D(A.angvel, h.angvel.data(), np * 32);
+        H2D(A.angpos, h.angpos.data(), np * 32);
+        H2D(A.xyzsum, h.xyzsum.data(), np * 32);
+        H2D(A.force, h.force.data(), np * 32);
+        H2D(A.torque, h.torque.data(), np * 32);
+        H2D(A.es_dot, h.es_dot.data(), np * 8);
+        H2D(A.es, h.es.data(), np * 8);
+        H2D(A.ev_dot, h.ev_dot.data(), np * 8);
+        H2D(A.ev, h.ev.data(), np * 8);
+        H2D(A.p, h.p.data(), np * 8);
+    }
+    std::vector<double> zeros4(np * 4, 0.0);
+    if (np) {
+        H2D(A.acc, zeros4.data(), np * 32);
+        H2D(A.angacc, zeros4.data(), np * 32);
+    }
+    // Contact rows start empty (partner index np, src/device.cu:458-462)
+    // with zero displacement.
+    std::vector<uint32_t> rows(np * NC, (uint32_t)np);
+    if (np) {
+        H2D(A.crow, rows.data(), np * NC * 4);
+        HIPCHK(c, hipMemsetAsync(A.delta_t, 0, np * NC * sizeof(double4), s), "memset");
+        std::vector<uint32_t> ff(np, 0xffffffffu);
+        H2D(A.key_prev, ff.data(), np * 4);
+        std::vector<uint32_t> ident(np);
+        for (size_t i = 0; i < np; ++i) ident[i] = (uint32_t)i;
+        H2D(A.perm, ident.data(), np * 4);
+        HIPCHK(c, hipMemsetAsync(A.skey, 0, np * 4, s), "memset");
+        HIPCHK(c, hipMemsetAsync(A.xs, 0, np * 32, s), "memset");
+    }
+    if (c->P.ncells) {
+        std::vector<uint32_t> cs(c->P.ncells, 0xffffffffu);
+        H2D(A.cell_start, cs.data(), (size_t)c->P.ncells * 4);
+        HIPCHK(c, hipMemsetAsync(A.cell_end, 0, (size_t)c->P.ncells * 4, s), "memset");
+    }
+    if (nw) {
+        H2D(A.w_nx, h.w_nx.data(), nw * 32);
+        H2D(A.w_mvfd, h.w_mvfd.data(), nw * 32);
+        HIPCHK(c, hipMemsetAsync(A.w_acc, 0, nw * 8, s), "memset");
+        if (np) HIPCHK(c, hipMemsetAsync(A.force_pp, 0, nw * np * 8, s), "memset");
+    }
+    HIPCHK(c, hipMemsetAsync(A.tau_eff, 0, 8, s), "memset");
+    if (nb) {
+        H2D(A.bonds, h.bonds.data(), nb * 8);
+        H2D(A.bonds_delta, h.bonds_delta.data(), nb * 32);
+        H2D(A.bonds_omega, h.bonds_omega.data(), nb * 32);
+    }
+    // Bond adjacency (CSR, ascending bond id per particle).
+    std::vector<uint32_t> off(np + 1, 0u), adj;
+    if (nb) {
+        for (size_t b = 0; b < nb; ++b) {
+            const uint32_t i = h.bonds[2 * b], j = h.bonds[2 * b + 1];
+            if (i < np) off[i + 1]++;
+            if (j < np && j != i) off[j + 1]++;
+        }
+        for (size_t i = 0; i < np; ++i) off[i + 1] += off[i];
+        adj.assign(off[np], 0u);
+        std::vector<uint32_t> fillp(off.begin(), off.end() - 1);
+        for (size_t b = 0; b < nb; ++b) {
+            const uint32_t i = h.bonds[2 * b], j = h.bonds[2 * b + 1];
+            if (i < np) adj[fillp[i]++] = (uint32_t)b;
+            if (j < np && j != i) adj[fillp[j]++] = (uint32_t)b;
+        }
+        H2D(A.badj, adj.data(), adj.size() * 4);
+    }
+    H2D(A.badj_off, off.data(), (np + 1) * 4);
+    DevLoop L;
+    memset(&L, 0, sizeof L);
+    L.t = h.t;
+    L.iter = 0;
+    L.grid_Lz = h.L[2];
+    L.velocity_state = 1;
+    L.zero_empty = 1u;
+    H2D(A.loop, &L, sizeof L);
+#undef H2D
+    HIPCHK(c, hipStreamSynchronize(s), "upload");
+    return SPH_OK;
+}
+
+int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sph_ctx** out) {
+    *out = nullptr;
+    sph_ctx* c = new sph_ctx();
+    c->flags = flags;
+    std::string err;
+    int rc = parse_bin(c->h, bytes, nbytes, err);
+    if (rc != SPH_OK) {
+        g_open_error = err;
+        delete c;
+        return rc;
+    }
+    for (uint32_t b = 0; b < c->h.nb0; ++b) {
+        if (c->h.bonds[2 * b] >= c->h.np || c->h.bonds[2 * b + 1] >= c->h.np) {
+            g_open_error = "Error: bond " + std::to_string(b) + " names a particle index >= np";
+            delete c;
+            return SPH_E_INPUT;
+        }
+    }
+    if (c->h.np >= MAX_NP) {
+        g_open_error = "np exceeds the supported maximum (2^27)";
+        delete c;
+        return SPH_E_INPUT;
+    }
+    if (!(flags & SPH_OPEN_NOCHECK)) {
+        rc = check_values(c);
+        if (rc != SPH_OK) {
+            g_open_error = c->err;
+            delete c;
+            return rc;
+        }
+    }
+    hipError_t e;
+    if (device >= 0) {
+        e = hipSetDevice(device);
+        if (e != hipSuccess) {
+            g_open_error = std::string("hipSetDevice: ") + hipGetErrorString(e);
+            delete c;
+            return SPH_E_DEVICE;
+        }
+    }
+    (void)hipGetDevice(&c->device);
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        g_open_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
+        delete c;
+        return SPH_E_DEVICE;
+    }
+    fill_params(c);
+    rc = allocate(c);
+    if (rc == SPH_OK) rc = upload_state(c);
+    if (rc != SPH_OK) {
+        g_open_error = c->err;
+        sph_close(c);
+        return rc;
+    }
+    c->iter = 0;
+    *out = c;
+    return SPH_OK;
+}
+
+// -- step loop --------------------------------------------------------------
+
+struct Prof {
+    sph_ctx* c;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int idx = -1;
+    void begin(const char* name) {
+        if (!c->profile) return;
+        idx = -1;
+        for (size_t k = 0; k < c->prof_names.size(); ++k)
+            if (c->prof_names[k] == name) idx = (int)k;
+        if (idx < 0) {
+            c->prof_names.push_back(name);
+            c->prof_ms.push_back(0.0);
+            c->prof_n.push_back(0);
+            idx = (int)c->prof_names.size() - 1;
+        }
+        (void)hipEventCreate(&e0);
+        (void)hipEventCreate(&e1);
+        (void)hipEventRecord(e0, c->stream);
+    }
+    void end() {
+        if (!c->profile || idx < 0) return;
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        c->prof_ms[idx] += ms;
+        c->prof_n[idx] += 1;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        idx = -1;
+    }
+};
+
+int enqueue_step(sph_ctx* c) {
+    const DevParams& P = c->P;
+    const DevArrays& A = c->A;
+    hipStream_t s = c->stream;
+    Prof pr{c};
+    if (P.np > 0) {
+        pr.begin("hash");
+        HIPCHK(c, launch_hash(P, A, s), "hash");
+        pr.end();
+        pr.begin("sort");
+        HIPCHK(c, launch_sort(P, A, c->passes, s), "sort");
+        pr.end();
+        pr.begin("cells");
+        HIPCHK(c, launch_cells(P, A, c->sorted_bits, s), "cells");
+        pr.end();
+        pr.begin("contact");
+        HIPCHK(c, launch_contact(P, A, s), "contact");
+        pr.end();
+        if (P.nb0 > 0) {
+            pr.begin("bonds");
+            HIPCHK(c, launch_bonds(P, A, s), "bonds");
+            pr.end();
+        }
+        if (c->shear_stress) {
+            pr.begin("shear_stress");
+            HIPCHK(c, launch_shear_stress(P, A, s), "shear_stress");
+            pr.end();
+        }
+        pr.begin("integrate");
+        HIPCHK(c, launch_integrate(P, A, s), "integrate");
+        pr.end();
+    }
+    pr.begin("walls");
+    HIPCHK(c, launch_walls(P, A, s), "walls");
+    pr.end();
+    // Host mirror of the loop counters (src/device.cu:2536-2538).
+    c->h.t += c->h.dt;
+    c->iter += 1;
+    return SPH_OK;
+}
+
+int read_loop(sph_ctx* c, DevLoop* L) {
+    HIPCHK(c, hipMemcpyAsync(L, c->A.loop, sizeof *L, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+    return SPH_OK;
+}
+
+int check_device_flags(sph_ctx* c) {
+    DevLoop L;
+    int rc = read_loop(c, &L);
+    if (rc) return rc;
+    if (L.error_bits & DFLAG_KEY_RANGE)
+        return fail(c, SPH_E_INPUT,
+                    "One or more particles moved outside the grid.\nThis could possibly be "
+                    "caused by a numerical instability.\nIs the computational time step too large?");
+    return SPH_OK;
+}
+
+// D2H of all state the .bin carries.
+int download_state(sph_ctx* c) {
+    Host& h = c->h;
+    const size_t np = h.np, nw = h.nw, nb = h.nb0;
+    hipStream_t s = c->stream;
+    const DevArrays& A = c->A;
+#define D2H(dst, src, bytes) \
+    HIPCHK(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToHost, s), "D2H")
+    if (np) {
+        D2H(h.x.data(), A.x, np * 32);
+        D2H(h.vel.data(), A.vel, np * 32);
+        D2H(h.angvel.data(), A.angvel, np * 32);
+        D2H(h.angpos.data(), A.angpos, np * 32);
+        D2H(h.xyzsum.data(), A.xyzsum, np * 32);
+        D2H(h.force.data(), A.force, np * 32);
+        D2H(h.torque.data(), A.torque, np * 32);
+        D2H(h.es_dot.data(), A.es_dot, np * 8);
+        D2H(h.es.data(), A.es, np * 8);
+        D2H(h.ev_dot.data(), A.ev_dot, np * 8);
+        D2H(h.ev.data(), A.ev, np * 8);
+        D2H(h.p.data(), A.p, np * 8);
+    }
+    if (nw) {
+        D2H(h.w_nx.data(), A.w_nx, nw * 32);
+        D2H(h.w_mvfd.data(), A.w_mvfd, nw * 32);
+    }
+    if (nb) {
+        D2H(h.bonds.data(), A.bonds, nb * 8);
+        D2H(h.bonds_delta.data(), A.bonds_delta, nb * 32);
+        D2H(h.bonds_omega.data(), A.bonds_omega, nb * 32);
+    }
+#undef D2H
+    DevLoop L;
+    int rc = read_loop(c, &L);
+    if (rc) return rc;
+    h.t = L.t;
+    if (h.adaptive == 1 && h.nw > 0 && L.iter > 0) h.L[2] = L.grid_Lz;
+    return SPH_OK;
+}
+
+int64_t field_bytes(const sph_ctx* c, int f) {
+    const int64_t np = c->h.np, nw = c->h.nw, nb = c->h.nb0;
+    switch (f) {
+        case SPH_F_X: case SPH_F_VEL: return np * 32;
+        case SPH_F_ANGVEL: case SPH_F_ANGPOS: case SPH_F_FORCE: case SPH_F_TORQUE:
+        case SPH_F_XYZSUM: case SPH_F_ACC: case SPH_F_ANGACC: return np * 24;
+        case SPH_F_ES_DOT: case SPH_F_ES: case SPH_F_EV_DOT: case SPH_F_EV: case SPH_F_P:
+            return np * 8;
+        case SPH_F_CONTACTS: return np * NC * 4;
+        case SPH_F_DELTA_T: return np * NC * 24;
+        case SPH_F_DISTMOD: return np * NC * 32;
+        case SPH_F_CELLID: case SPH_F_PERM: return np * 4;
+        case SPH_F_CELLSTART: case SPH_F_CELLEND: return (int64_t)c->P.ncells * 4;
+        case SPH_F_WALLS_NX: case SPH_F_WALLS_MVFD: return nw * 32;
+        case SPH_F_WALLS_ACC: return nw * 8;
+        case SPH_F_WALLS_FORCE_PP: return nw * np * 8;
+        case SPH_F_BONDS: return nb * 8;
+        case SPH_F_BONDS_DELTA: case SPH_F_BONDS_OMEGA: return nb * 32;
+        case SPH_F_COLOR: return np * 4;
+        default: return -1;
+    }
+}
+
+int d2h(sph_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return SPH_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+    return SPH_OK;
+}
+int h2d(sph_ctx* c, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return SPH_OK;
+    HIPCHK(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream), "H2D");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+    return SPH_OK;
+}
+
+// double4 rows -> n x 3
+void pack3(const std::vector<double>& v4, void* dst, size_t n) {
+    double* d = static_cast<double*>(dst);
+    for (size_t i = 0; i < n; ++i) {
+        d[3 * i] = v4[4 * i]; d[3 * i + 1] = v4[4 * i + 1]; d[3 * i + 2] = v4[4 * i + 2];
+    }
+}
+
+int get_vec3(sph_ctx* c, const double4* src, size_t n, void* dst) {
+    std::vector<double> tmp(n * 4);
+    int rc = d2h(c, tmp.data(), src, n * 32);
+    if (rc) return rc;
+    pack3(tmp, dst, n);
+    return SPH_OK;
+}
+
+int set_vec3(sph_ctx* c, double4* dst, size_t n, const void* src) {
+    std::vector<double> tmp(n * 4);
+    int rc = d2h(c, tmp.data(), dst, n * 32);
+    if (rc) return rc;
+    const double* s = static_cast<const double*>(src);
+    for (size_t i = 0; i < n; ++i) {
+        tmp[4 * i] = s[3 * i]; tmp[4 * i + 1] = s[3 * i + 1]; tmp[4 * i + 2] = s[3 * i + 2];
+    }
+    return h2d(c, dst, tmp.data(), n * 32);
+}
+
+int mkdir_p(const std::string& path) {
+    if (path.empty()) return 0;
+    std::string cur;
+    for (size_t i = 0; i < path.size(); ++i) {
+        cur += path[i];
+        if (path[i] == '/' || i + 1 == path.size()) mkdir(cur.c_str(), 0755);
+    }
+    return 0;
+}
+
+int write_status(sph_ctx* c, const std::string& fn) {
+    FILE* fp = fopen(fn.c_str(), "w");
+    if (!fp) return fail(c, SPH_E_IO, "could not write %s", fn.c_str());
+    fprintf(fp, "%2.4e %2.4e %d\n", c->h.t, 100.0 * c->h.t / c->h.t_total, c->h.step_count);
+    fclose(fp);
+    return SPH_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+int sph_open_mem(const void* bytes, size_t nbytes, int device, unsigned flags, sph_ctx** out) {
+    if (!out || (!bytes && nbytes)) return SPH_E_ARG;
+    return open_common(bytes, nbytes, device, flags, out);
+}
+
+int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out) {
+    if (!out || !bin_path) return SPH_E_ARG;
+    *out = nullptr;
+    FILE* fp = fopen(bin_path, "rb");
+    if (!fp) {
+        g_open_error = std::string("Could not read input binary file '") + bin_path + "'";
+        return SPH_E_IO;
+    }
+    std::vector<unsigned char> buf;
+    unsigned char tmp[1 << 16];
+    size_t k;
+    while ((k = fread(tmp, 1, sizeof tmp, fp)) > 0) buf.insert(buf.end(), tmp, tmp + k);
+    fclose(fp);
+    return open_common(buf.data(), buf.size(), device, flags, out);
+}
+
+int sph_step(sph_ctx* c, uint64_t nsteps) {
+    if (!c) return SPH_E_ARG;
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        int rc = enqueue_step(c);
+        if (rc) return rc;
+    }
+    return SPH_OK;
+}
+
+int sph_sync(sph_ctx* c) {
+    if (!c) return SPH_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+    return check_device_flags(c);
+}
+
+int sph_write_mem(sph_ctx* c, void* dst, size_t cap, size_t* nbytes) {
+    if (!c) return SPH_E_ARG;
+    int rc = download_state(c);
+    if (rc) return rc;
+    Writer w;
+    write_bin(c->h, w);
+    if (nbytes) *nbytes = w.b.size();
+    if (dst && cap >= w.b.size()) memcpy(dst, w.b.data(), w.b.size());
+    return SPH_OK;
+}
+
+int sph_write_bin(sph_ctx* c, const char* path) {
+    if (!c || !path) return SPH_E_ARG;
+    int rc = download_state(c);
+    if (rc) return rc;
+    Writer w;
+    write_bin(c->h, w);
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return fail(c, SPH_E_IO, "could create output binary file '%s'", path);
+    const size_t k = fwrite(w.b.data(), 1, w.b.size(), fp);
+    fclose(fp);
+    if (k != w.b.size()) return fail(c, SPH_E_IO, "short write to '%s'", path);
+    return SPH_OK;
+}
+
+int sph_run(sph_ctx* c, const char* out_dir, const char* sid) {
+    if (!c || !out_dir || !sid) return SPH_E_ARG;
+    Host& h = c->h;
+    const std::string dir(out_dir);
+    mkdir_p(dir);
+    const bool verbose = !(c->flags & SPH_OPEN_QUIET);
+    char file[4096];
+    int rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+    if (h.step_count == 0) {
+        snprintf(file, sizeof file, "%s/%s.output00000.bin", dir.c_str(), sid);
+        if ((rc = sph_write_bin(c, file))) return rc;
+    }
+    double filetimeclock = 0.0;
+    const std::string status = dir + "/" + sid + ".status.dat";
+    if ((rc = write_status(c, status))) return rc;
+    const double t_start = h.t;
+    const clock_t tic = clock();
+    long iter_run = 0;
+    while (h.t <= h.t_total) {
+        // Steps until the next output or the end, computed with the same
+        // double additions the reference's host loop performs.
+        uint64_t k = 0;
+        double t = h.t, ftc = filetimeclock;
+        bool out = false;
+        while (t <= h.t_total) {
+            t += h.dt;
+            ftc += h.dt;
+            ++k;
+            if (ftc >= h.file_dt) { out = true; break; }
+            if (k >= 1000) break;
+        }
+        const double t_before = h.t;
+        if ((rc = sph_step(c, k))) return rc;
+        // sph_step advanced h.t by the same additions.
+        (void)t_before;
+        filetimeclock = ftc;
+        iter_run += (long)k;
+        if (verbose && (iter_run % 10 == 0 || out)) {
+            const double spent = (double)(clock() - tic) / CLOCKS_PER_SEC;
+            const double ratio = spent / (h.t - t_start);
+            printf("\r  Current time: %g/%g s. (%g s_real/s_sim)       ", h.t, h.t_total, ratio);
+            fflush(stdout);
+        }
+        if (out) {
+            if ((rc = sph_sync(c))) return rc;
+            h.step_count += 1;
+            snprintf(file, sizeof file, "%s/%s.output%05d.bin", dir.c_str(), sid, h.step_count);
+            if ((rc = sph_write_bin(c, file))) return rc;
+            if ((rc = write_status(c, status))) return rc;
+            filetimeclock = 0.0;
+        }
+    }
+    if ((rc = sph_sync(c))) return rc;
+    if (verbose) printf("\nSimulation ended. Last output file number: %u\n", h.step_count);
+    return SPH_OK;
+}
+
+int64_t sph_field_bytes(sph_ctx* c, int field) {
+    if (!c) return SPH_E_ARG;
+    return field_bytes(c, field);
+}
+
+int sph_get_field(sph_ctx* c, int f, void* dst, size_t bytes) {
+    if (!c) return SPH_E_ARG;
+    const int64_t need = field_bytes(c, f);
+    if (need < 0 || (size_t)need != bytes || (!dst && bytes))
+        return fail(c, SPH_E_ARG, "field %d: expected %lld bytes", f, (long long)need);
+    const size_t np = c->h.np;
+    const DevArrays& A = c->A;
+    switch (f) {
+        case SPH_F_X: return d2h(c, dst, A.x, bytes);
+        case SPH_F_VEL: return d2h(c, dst, A.vel, bytes);
+        case SPH_F_ANGVEL: return get_vec3(c, A.angvel, np, dst);
+        case SPH_F_ANGPOS: return get_vec3(c, A.angpos, np, dst);
+        case SPH_F_FORCE: return get_vec3(c, A.force, np, dst);
+        case SPH_F_TORQUE: return get_vec3(c, A.torque, np, dst);
+        case SPH_F_XYZSUM: return get_vec3(c, A.xyzsum, np, dst);
+        case SPH_F_ACC: return get_vec3(c, A.acc, np, dst);
+        case SPH_F_ANGACC: return get_vec3(c, A.angacc, np, dst);
+        case SPH_F_ES_DOT: return d2h(c, dst, A.es_dot, bytes);
+        case SPH_F_ES: return d2h(c, dst, A.es, bytes);
+        case SPH_F_EV_DOT: return d2h(c, dst, A.ev_dot, bytes);
+        case SPH_F_EV: return d2h(c, dst, A.ev, bytes);
+        case SPH_F_P: return d2h(c, dst, A.p, bytes);
+        case SPH_F_CONTACTS: {
+            int rc = d2h(c, dst, A.crow, bytes);
+            if (rc) return rc;
+            uint32_t* w = static_cast<uint32_t*>(dst);
+            for (size_t i = 0; i < np * NC; ++i) w[i] &= CW_IDX_MASK;
+            return SPH_OK;
+        }
+        case SPH_F_DELTA_T: return get_vec3(c, A.delta_t, np * NC, dst);
+        case SPH_F_DISTMOD: {
+            // Reconstructed from the packed image code and the partner's
+            // radius; meaningful where the slot is occupied.
+            std::vector<uint32_t> rows(np * NC);
+            std::vector<double> x(np * 4);
+            int rc = d2h(c, rows.data(), A.crow, np * NC * 4);
+            if (!rc) rc = d2h(c, x.data(), A.x, np * 32);
+            if (rc) return rc;
+            double* d = static_cast<double*>(dst);
+            for (size_t k = 0; k < np * NC; ++k) {
+                const uint32_t wv = rows[k], idx = wv & CW_IDX_MASK;
+                const uint32_t cx = (wv >> CW_SHIFT_X) & 3u, cy = (wv >> CW_SHIFT_Y) & 3u;
+                d[4 * k] = cx == 1 ? c->h.L[0] : cx == 2 ? -c->h.L[0] : 0.0;
+                d[4 * k + 1] = cy == 1 ? c->h.L[1] : cy == 2 ? -c->h.L[1] : 0.0;
+                d[4 * k + 2] = 0.0;
+                d[4 * k + 3] = idx < np ? x[4 * idx + 3] : 0.0;
+            }
+            return SPH_OK;
+        }
+        case SPH_F_CELLID: return d2h(c, dst, A.skey, bytes);
+        case SPH_F_PERM: return d2h(c, dst, A.perm, bytes);
+        case SPH_F_CELLSTART: return d2h(c, dst, A.cell_start, bytes);
+        case SPH_F_CELLEND: return d2h(c, dst, A.cell_end, bytes);
+        case SPH_F_WALLS_NX: return d2h(c, dst, A.w_nx, bytes);
+        case SPH_F_WALLS_MVFD: return d2h(c, dst, A.w_mvfd, bytes);
+        case SPH_F_WALLS_ACC: return d2h(c, dst, A.w_acc, bytes);
+        case SPH_F_WALLS_FORCE_PP: return d2h(c, dst, A.force_pp, bytes);
+        case SPH_F_BONDS: return d2h(c, dst, A.bonds, bytes);
+        case SPH_F_BONDS_DELTA: return d2h(c, dst, A.bonds_delta, bytes);
+        case SPH_F_BONDS_OMEGA: return d2h(c, dst, A.bonds_omega, bytes);
+        case SPH_F_COLOR:
+            if (bytes) memcpy(dst, c->h.color.data(), bytes);
+            return SPH_OK;
+        default: return fail(c, SPH_E_ARG, "unknown field %d", f);
+    }
+}
+
+int sph_set_field(sph_ctx* c, int f, const void* src, size_t bytes) {
+    if (!c) return SPH_E_ARG;
+    const int64_t need = field_bytes(c, f);
+    if (need < 0 || (size_t)need != bytes || (!src && bytes))
+        return fail(c, SPH_E_ARG, "field %d: expected %lld bytes", f, (long long)need);
+    const size_t np = c->h.np;
+    DevArrays& A = c->A;
+    switch (f) {
+        case SPH_F_X: return h2d(c, A.x, src, bytes);
+        case SPH_F_VEL: return h2d(c, A.vel, src, bytes);
+        case SPH_F_ANGVEL: return set_vec3(c, A.angvel, np, src);
+        case SPH_F_ANGPOS: return set_vec3(c, A.angpos, np, src);
+        case SPH_F_FORCE: return set_vec3(c, A.force, np, src);
+        case SPH_F_TORQUE: return set_vec3(c, A.torque, np, src);
+        case SPH_F_XYZSUM: return set_vec3(c, A.xyzsum, np, src);
+        case SPH_F_ACC: return set_vec3(c, A.acc, np, src);
+        case SPH_F_ANGACC: return set_vec3(c, A.angacc, np, src);
+        case SPH_F_ES_DOT: return h2d(c, A.es_dot, src, bytes);
+        case SPH_F_ES: return h2d(c, A.es, src, bytes);
+        case SPH_F_EV_DOT: return h2d(c, A.ev_dot, src, bytes);
+        case SPH_F_EV: return h2d(c, A.ev, src, bytes);
+        case SPH_F_P: return h2d(c, A.p, src, bytes);
+        case SPH_F_CONTACTS: {
+            // Keep each slot's image code; replace the partner index.
+            std::vector<uint32_t> rows(np * NC);
+            int rc = d2h(c, rows.data(), A.crow, bytes);
+            if (rc) return rc;
+            const uint32_t* s = static_cast<const uint32_t*>(src);
+            for (size_t k = 0; k < np * NC; ++k) {
+                if (s[k] > np) return fail(c, SPH_E_ARG, "contact index out of range");
+                rows[k] = (rows[k] & ~CW_IDX_MASK) | s[k];
+            }
+            rc = h2d(c, A.crow, rows.data(), bytes);
+            if (rc) return rc;
+            const uint32_t one = 1u;
+            return h2d(c, &A.loop->zero_empty, &one, 4);
+        }
+        case SPH_F_DELTA_T: {
+            int rc = set_vec3(c, A.delta_t, np * NC, src);
+            if (rc) return rc;
+            const uint32_t one = 1u;
+            return h2d(c, &A.loop->zero_empty, &one, 4);
+        }
+        case SPH_F_DISTMOD: {
+            std::vector<uint32_t> rows(np * NC);
+            int rc = d2h(c, rows.data(), A.crow, np * NC * 4);
+            if (rc) return rc;
+            const double* d = static_cast<const double*>(src);
+            for (size_t k = 0; k < np * NC; ++k) {
+                uint32_t code = 0;
+                const double sx = d[4 * k], sy = d[4 * k + 1];
+                if (sx == c->h.L[0]) code |= 1u << CW_SHIFT_X;
+                else if (sx == -c->h.L[0]) code |= 2u << CW_SHIFT_X;
+                else if (sx != 0.0) return fail(c, SPH_E_ARG, "distmod x must be 0 or +-L[0]");
+                if (sy == c->h.L[1]) code |= 1u << CW_SHIFT_Y;
+                else if (sy == -c->h.L[1]) code |= 2u << CW_SHIFT_Y;
+                else if (sy != 0.0) return fail(c, SPH_E_ARG, "distmod y must be 0 or +-L[1]");
+                if (d[4 * k + 2] != 0.0) return fail(c, SPH_E_ARG, "distmod z must be 0");
+                rows[k] = (rows[k] & CW_IDX_MASK) | code;
+            }
+            return h2d(c, A.crow, rows.data(), np * NC * 4);
+        }
+        case SPH_F_WALLS_NX: {
+            int rc = h2d(c, A.w_nx, src, bytes);
+            if (!rc && bytes) memcpy(c->h.w_nx.data(), src, bytes);
+            return rc;
+        }
+        case SPH_F_WALLS_MVFD: {
+            int rc = h2d(c, A.w_mvfd, src, bytes);
+            if (!rc && bytes) memcpy(c->h.w_mvfd.data(), src, bytes);
+            return rc;
+        }
+        case SPH_F_WALLS_ACC: return h2d(c, A.w_acc, src, bytes);
+        case SPH_F_BONDS_DELTA: return h2d(c, A.bonds_delta, src, bytes);
+        case SPH_F_BONDS_OMEGA: return h2d(c, A.bonds_omega, src, bytes);
+        case SPH_F_COLOR:
+            if (bytes) memcpy(c->h.color.data(), src, bytes);
+            return SPH_OK;
+        default: return fail(c, SPH_E_ARG, "field %d is read-only", f);
+    }
+}
+
+int sph_get_scalar(sph_ctx* c, int which, double* v) {
+    if (!c || !v) return SPH_E_ARG;
+    switch (which) {
+        case SPH_S_TIME: {
+            DevLoop L;
+            int rc = read_loop(c, &L);
+            if (rc) return rc;
+            *v = L.t;
+            return SPH_OK;
+        }
+        case SPH_S_TIME_TOTAL: *v = c->h.t_total; return SPH_OK;
+        case SPH_S_DT: *v = c->h.dt; return SPH_OK;
+        case SPH_S_FILE_DT: *v = c->h.file_dt; return SPH_OK;
+        case SPH_S_ITER: {
+            DevLoop L;
+            int rc = read_loop(c, &L);
+            if (rc) return rc;
+            *v = (double)L.iter;
+            return SPH_OK;
+        }
+        case SPH_S_STEP_COUNT: *v = c->h.step_count; return SPH_OK;
+        case SPH_S_NP: *v = c->h.np; return SPH_OK;
+        case SPH_S_NW: *v = c->h.nw; return SPH_OK;
+        case SPH_S_NB0: *v = c->h.nb0; return SPH_OK;
+        case SPH_S_NCELLS: *v = c->P.ncells; return SPH_OK;
+        default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
+    }
+}
+
+int sph_set_scalar(sph_ctx* c, int which, double v) {
+    if (!c) return SPH_E_ARG;
+    switch (which) {
+        case SPH_S_TIME: {
+            c->h.t = v;
+            return h2d(c, &c->A.loop->t, &v, 8);
+        }
+        case SPH_S_TIME_TOTAL: c->h.t_total = v; return SPH_OK;
+        case SPH_S_DT: c->h.dt = v; c->P.dt = v; return SPH_OK;
+        case SPH_S_FILE_DT: c->h.file_dt = v; return SPH_OK;
+        case SPH_S_ITER: {
+            const int64_t it = (int64_t)v;
+            c->iter = it;
+            return h2d(c, &c->A.loop->iter, &it, 8);
+        }
+        case SPH_S_STEP_COUNT: c->h.step_count = (uint32_t)v; return SPH_OK;
+        default: return fail(c, SPH_E_ARG, "scalar %d is read-only", which);
+    }
+}
+
+int sph_get_contacts(sph_ctx* c, uint32_t* idx, double* delta_t, double* distmod) {
+    if (!c) return SPH_E_ARG;
+    const size_t np = c->h.np;
+    int rc = SPH_OK;
+    if (idx) rc = sph_get_field(c, SPH_F_CONTACTS, idx, np * NC * 4);
+    if (!rc && delta_t) rc = sph_get_field(c, SPH_F_DELTA_T, delta_t, np * NC * 24);
+    if (!rc && distmod) rc = sph_get_field(c, SPH_F_DISTMOD, distmod, np * NC * 32);
+    return rc;
+}
+
+int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_start,
+                  uint32_t* cell_end) {
+    if (!c) return SPH_E_ARG;
+    const size_t np = c->h.np, nc = c->P.ncells;
+    int rc = SPH_OK;
+    if (cell_id) rc = sph_get_field(c, SPH_F_CELLID, cell_id, np * 4);
+    if (!rc && perm) rc = sph_get_field(c, SPH_F_PERM, perm, np * 4);
+    if (!rc && cell_start) rc = sph_get_field(c, SPH_F_CELLSTART, cell_start, nc * 4);
+    if (!rc && cell_end) rc = sph_get_field(c, SPH_F_CELLEND, cell_end, nc * 4);
+    return rc;
+}
+
+int sph_profile(sph_ctx* c, int enable) {
+    if (!c) return SPH_E_ARG;
+    c->profile = enable != 0;
+    c->prof_names.clear();
+    c->prof_ms.clear();
+    c->prof_n.clear();
+    return SPH_OK;
+}
+
+int sph_profile_report(sph_ctx* c, char* buf, size_t cap) {
+    if (!c) return SPH_E_ARG;
+    std::string out;
+    char line[256];
+    for (size_t k = 0; k < c->prof_names.size(); ++k) {
+        snprintf(line, sizeof line, "%s %.3f %ld\n", c->prof_names[k].c_str(),
+                 c->prof_n[k] ? 1000.0 * c->prof_ms[k] / c->prof_n[k] : 0.0, c->prof_n[k]);
+        out += line;
+    }
+    if (buf && cap) snprintf(buf, cap, "%s", out.c_str());
+    return (int)out.size();
+}
+
+void sph_close(sph_ctx* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (void* p : c->allocs) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : g_open_error.c_str(); }
+const char* sph_open_error(void) { return g_open_error.c_str(); }
+const char* sph_version_string(void) { return "sphere_hip 0.1 (.bin 2.15, gfx950)"; }
+
+}  // extern "C"

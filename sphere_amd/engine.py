@@ -1,0 +1,227 @@
+"""ctypes binding of libsphere_hip.so, the HIP engine's C ABI.
+
+The product path: every call here runs the HIP kernels of
+sphere_amd/csrc.  There is no CPU fallback; a missing or unloadable
+library raises EngineUnavailable.
+"""
+
+import ctypes
+import os
+
+import numpy
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsphere_hip.so")
+NC = 32
+
+_LIB = None
+
+
+class EngineUnavailable(RuntimeError):
+    pass
+
+
+class EngineError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sphere_hip error %d: %s" % (code, msg))
+        self.code = code
+
+
+# Field ids and shapes (include/sphere_hip.h, enum sph_field).
+FIELDS = {
+    "x": (0, numpy.float64, (-1, 4)),
+    "vel": (1, numpy.float64, (-1, 4)),
+    "angvel": (2, numpy.float64, (-1, 3)),
+    "angpos": (3, numpy.float64, (-1, 3)),
+    "force": (4, numpy.float64, (-1, 3)),
+    "torque": (5, numpy.float64, (-1, 3)),
+    "xyzsum": (6, numpy.float64, (-1, 3)),
+    "acc": (7, numpy.float64, (-1, 3)),
+    "angacc": (8, numpy.float64, (-1, 3)),
+    "es_dot": (9, numpy.float64, (-1,)),
+    "es": (10, numpy.float64, (-1,)),
+    "ev_dot": (11, numpy.float64, (-1,)),
+    "ev": (12, numpy.float64, (-1,)),
+    "p": (13, numpy.float64, (-1,)),
+    "contacts": (14, numpy.uint32, (-1, NC)),
+    "delta_t": (15, numpy.float64, (-1, NC, 3)),
+    "distmod": (16, numpy.float64, (-1, NC, 4)),
+    "cellid": (17, numpy.uint32, (-1,)),
+    "perm": (18, numpy.uint32, (-1,)),
+    "cellstart": (19, numpy.uint32, (-1,)),
+    "cellend": (20, numpy.uint32, (-1,)),
+    "walls_nx": (21, numpy.float64, (-1, 4)),
+    "walls_mvfd": (22, numpy.float64, (-1, 4)),
+    "walls_acc": (23, numpy.float64, (-1,)),
+    "walls_force_pp": (24, numpy.float64, (-1,)),
+    "bonds": (25, numpy.uint32, (-1, 2)),
+    "bonds_delta": (26, numpy.float64, (-1, 4)),
+    "bonds_omega": (27, numpy.float64, (-1, 4)),
+    "color": (28, numpy.int32, (-1,)),
+}
+
+SCALARS = {"time": 0, "time_total": 1, "dt": 2, "file_dt": 3, "iter": 4, "step_count": 5,
+           "np": 6, "nw": 7, "nb0": 8, "ncells": 9}
+
+# Exported symbols of the C ABI (include/sphere_hip.h).
+SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_write_bin",
+           "sph_write_mem", "sph_get_field", "sph_set_field", "sph_field_bytes",
+           "sph_get_scalar", "sph_set_scalar", "sph_get_contacts", "sph_get_cells",
+           "sph_profile", "sph_profile_report", "sph_close", "sph_last_error",
+           "sph_open_error", "sph_version_string")
+
+QUIET = 0x1
+NOCHECK = 0x2
+
+
+def load():
+    """Load libsphere_hip.so (raises EngineUnavailable if absent)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise EngineUnavailable(
+            "libsphere_hip.so is not built (%s); run `python -m sphere_amd.build` "
+            "or __graft_entry__.build()" % LIB_PATH)
+    try:
+        L = ctypes.CDLL(LIB_PATH)
+    except OSError as e:
+        raise EngineUnavailable("cannot load %s: %s" % (LIB_PATH, e))
+    vp = ctypes.c_void_p
+    L.sph_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_uint, ctypes.POINTER(vp)]
+    L.sph_open_mem.argtypes = [vp, ctypes.c_size_t, ctypes.c_int, ctypes.c_uint,
+                               ctypes.POINTER(vp)]
+    L.sph_step.argtypes = [vp, ctypes.c_uint64]
+    L.sph_sync.argtypes = [vp]
+    L.sph_run.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p]
+    L.sph_write_bin.argtypes = [vp, ctypes.c_char_p]
+    L.sph_write_mem.argtypes = [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+    L.sph_get_field.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+    L.sph_set_field.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+    L.sph_field_bytes.argtypes = [vp, ctypes.c_int]
+    L.sph_field_bytes.restype = ctypes.c_int64
+    L.sph_get_scalar.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.sph_set_scalar.argtypes = [vp, ctypes.c_int, ctypes.c_double]
+    L.sph_get_contacts.argtypes = [vp, vp, vp, vp]
+    L.sph_get_cells.argtypes = [vp, vp, vp, vp, vp]
+    L.sph_profile.argtypes = [vp, ctypes.c_int]
+    L.sph_profile_report.argtypes = [vp, ctypes.c_char_p, ctypes.c_size_t]
+    L.sph_close.argtypes = [vp]
+    L.sph_close.restype = None
+    L.sph_last_error.argtypes = [vp]
+    L.sph_last_error.restype = ctypes.c_char_p
+    L.sph_open_error.restype = ctypes.c_char_p
+    L.sph_version_string.restype = ctypes.c_char_p
+    _LIB = L
+    return L
+
+
+class Context:
+    """One simulation resident on the GPU (a sph_ctx)."""
+
+    def __init__(self, image=None, path=None, device=-1, quiet=True, check=True):
+        L = load()
+        h = ctypes.c_void_p()
+        flags = (QUIET if quiet else 0) | (0 if check else NOCHECK)
+        if path is not None:
+            rc = L.sph_open(path.encode(), device, flags, ctypes.byref(h))
+        else:
+            buf = ctypes.create_string_buffer(bytes(image), len(image))
+            rc = L.sph_open_mem(buf, len(image), device, flags, ctypes.byref(h))
+        if rc != 0:
+            raise EngineError(rc, L.sph_open_error().decode())
+        self._h = h
+        self._L = L
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise EngineError(rc, self._L.sph_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sph_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n=1, sync=True):
+        self._chk(self._L.sph_step(self._h, int(n)))
+        if sync:
+            self.sync()
+
+    def sync(self):
+        self._chk(self._L.sph_sync(self._h))
+
+    def get(self, name):
+        fid, dt, shape = FIELDS[name]
+        nbytes = self._L.sph_field_bytes(self._h, fid)
+        out = numpy.empty(nbytes // numpy.dtype(dt).itemsize, dtype=dt)
+        self._chk(self._L.sph_get_field(self._h, fid, out.ctypes.data, nbytes))
+        return out.reshape(shape)
+
+    def set(self, name, value):
+        fid, dt, shape = FIELDS[name]
+        a = numpy.ascontiguousarray(value, dtype=dt)
+        self._chk(self._L.sph_set_field(self._h, fid, a.ctypes.data, a.nbytes))
+
+    def scalar(self, name):
+        v = ctypes.c_double()
+        self._chk(self._L.sph_get_scalar(self._h, SCALARS[name], ctypes.byref(v)))
+        return v.value
+
+    def set_scalar(self, name, value):
+        self._chk(self._L.sph_set_scalar(self._h, SCALARS[name], float(value)))
+
+    def tobytes(self):
+        n = ctypes.c_size_t()
+        self._chk(self._L.sph_write_mem(self._h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        self._chk(self._L.sph_write_mem(self._h, buf, n.value, ctypes.byref(n)))
+        return buf.raw
+
+    def write_bin(self, path):
+        self._chk(self._L.sph_write_bin(self._h, path.encode()))
+
+    def run(self, out_dir, sid):
+        self._chk(self._L.sph_run(self._h, out_dir.encode(), sid.encode()))
+
+    def profile(self, enable=True):
+        self._chk(self._L.sph_profile(self._h, 1 if enable else 0))
+
+    def profile_report(self):
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._L.sph_profile_report(self._h, buf, len(buf))
+        rep = {}
+        for line in buf.value.decode().splitlines():
+            name, us, n = line.split()
+            rep[name] = (float(us), int(n))
+        return rep
+
+
+def run_file(path, out_dir, sid, verbose=True, dry=False, device=-1):
+    """Open a .bin and run DEM::startTime's loop; returns 0 or a negative code."""
+    try:
+        ctx = Context(path=path, device=device, quiet=not verbose)
+    except EngineError as e:
+        print(str(e))
+        return e.code
+    try:
+        if dry:
+            return 0
+        ctx.run(out_dir, sid)
+        return 0
+    except EngineError as e:
+        print(str(e))
+        return e.code
+    finally:
+        ctx.close()

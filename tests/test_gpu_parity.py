@@ -1,0 +1,104 @@
+"""HIP engine vs CPU oracle, through the C ABI, on identical .bin images.
+
+Integer and index outputs (cell keys, permutation, cell ranges, contact
+lists) must be bit-exact; the FP64 state is also bit-exact because both
+sides evaluate the reference's expressions in the same order without FMA
+contraction and with correctly rounded square roots.  Where a test allows
+a tolerance it says so.
+"""
+
+import numpy
+import pytest
+
+import oracle
+from sphere_amd import engine, scenes
+
+pytestmark = pytest.mark.gpu
+
+EXACT_FIELDS = ("perm", "cellid", "contacts", "x", "vel", "angvel", "angpos", "force",
+                "torque", "xyzsum", "acc", "angacc", "es_dot", "es", "ev_dot", "ev", "p",
+                "delta_t")
+
+
+def _compare(ctx, ref, fields=EXACT_FIELDS):
+    bad = []
+    for name in fields:
+        a, b = ctx.get(name), ref.get(name)
+        if a.shape != b.shape or not numpy.array_equal(a, b):
+            if a.dtype.kind == "f" and a.shape == b.shape:
+                d = numpy.abs(a - b)
+                bad.append("%s max|d|=%g at %s" % (name, d.max(), numpy.unravel_index(
+                    numpy.argmax(d), d.shape)))
+            else:
+                bad.append(name)
+    return bad
+
+
+def _occupied_cells_equal(ctx, ref):
+    cs_a, cs_b = ctx.get("cellstart"), ref.get("cellstart")
+    assert numpy.array_equal(cs_a, cs_b)
+    occ = cs_b != 0xFFFFFFFF
+    assert numpy.array_equal(ctx.get("cellend")[occ], ref.get("cellend")[occ])
+
+
+def _run_pair(s, steps, chunks=1):
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    per = steps // chunks
+    for _ in range(chunks):
+        ref.step(per)
+        ctx.step(per)
+    return ctx, ref
+
+
+@pytest.mark.parametrize("n,seed", [(300, 11), (1500, 12)])
+def test_settling_box_bit_exact(n, seed):
+    s = scenes.settling_box(n=n, seed=seed, box=(0.12, 0.12, 0.3))
+    # Start with downward velocities so particles reach the floor and walls
+    # within the test.
+    rng = numpy.random.default_rng(seed)
+    s.vel[:, 2] = -rng.uniform(0.5, 2.0, n)
+    s.vel[:, 0] = rng.uniform(-0.5, 0.5, n)
+    ctx, ref = _run_pair(s, 200, chunks=4)
+    assert _compare(ctx, ref) == []
+    _occupied_cells_equal(ctx, ref)
+    ctx.close()
+
+
+@pytest.mark.parametrize("n,seed,shrink", [(500, 21, 0.97), (2000, 22, 0.95)])
+def test_shear_box_bit_exact(n, seed, shrink):
+    s = scenes.shear_box(n=n, seed=seed, phi=0.3, shrink=shrink)
+    ctx, ref = _run_pair(s, 60, chunks=3)
+    bad = _compare(ctx, ref)
+    bad += _compare(ctx, ref, ("walls_nx", "walls_mvfd", "walls_acc", "walls_force_pp"))
+    assert bad == []
+    _occupied_cells_equal(ctx, ref)
+    c = ctx.get("contacts")
+    assert (c != n).sum() > n  # the scene is contact-rich
+    ctx.close()
+
+
+def test_contact_model_1_bit_exact():
+    s = scenes.shear_box(n=800, seed=31, phi=0.3, shrink=0.96)
+    s.contactmodel[0] = 1
+    ctx, ref = _run_pair(s, 40, chunks=2)
+    assert _compare(ctx, ref, [f for f in EXACT_FIELDS if f not in ("contacts", "delta_t")]) == []
+    ctx.close()
+
+
+def test_contact_model_3_bit_exact():
+    s = scenes.shear_box(n=800, seed=32, phi=0.3, shrink=0.96)
+    s.contactmodel[0] = 3
+    s.gamma_r[0] = 1.0
+    s.mu_r[0] = 0.1
+    ctx, ref = _run_pair(s, 40, chunks=2)
+    assert _compare(ctx, ref) == []
+    ctx.close()
+
+
+def test_write_bin_matches_oracle():
+    s = scenes.shear_box(n=400, seed=41, phi=0.3, shrink=0.97)
+    ctx, ref = _run_pair(s, 25)
+    assert ctx.tobytes() == ref.tobytes()
+    ctx.close()
