@@ -171,3 +171,101 @@ def shear_box(n=2000, seed=2001, median=0.01, sigma=0.2, phi=0.25, lxy=None, k_n
     s.time_total[0] = 1.0
     s.time_file_dt[0] = 1.0
     return s
+
+
+def grow_pack(s, open_stepper, phi_rsa=0.28, stages=40, steps_per_stage=200, settle_steps=3000,
+              seed=0, friction_during_growth=0.0, log=None):
+    """Densify a periodic-x/y packing by growing the radii in place.
+
+    `s` carries the true radii, the box (origo, L with L[2] the fixed top
+    during growth) and the final contact parameters.  Particles are first
+    inserted without overlap with radii scaled by s0 = (phi_rsa / phi)^(1/3)
+    (phi the final packing fraction of the box), then the radii grow back to
+    their true values in `stages` increments, each followed by
+    `steps_per_stage` DEM steps without gravity and without friction.  Then
+    gravity and friction are restored and the bed settles for `settle_steps`.
+
+    `open_stepper(image)` returns an object with step(n), get(name),
+    set(name, value) and tobytes() -- a GPU engine context or the CPU
+    oracle.  Returns the settled sim.
+    """
+    rng = numpy.random.default_rng(seed)
+    r = numpy.asarray(s.radius, dtype=numpy.float64).copy()
+    n = r.size
+    vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
+    box = float((s.L[0] - s.origo[0]) * (s.L[1] - s.origo[1]) * (s.L[2] - s.origo[2]))
+    phi = vol / box
+    s0 = (phi_rsa / phi) ** (1.0 / 3.0)
+    if s0 >= 1.0:
+        s0 = 1.0
+    s.x = rsa_positions(r * s0, list(s.origo), list(s.L), rng,
+                        periodic=(s.periodic[0] in (1, 2), s.periodic[0] == 1, False))
+    g = s.g.copy()
+    mu = (float(s.mu_s[0]), float(s.mu_d[0]), float(s.mu_ws[0]), float(s.mu_wd[0]))
+    s.g[:] = 0.0
+    s.mu_s[0] = s.mu_d[0] = friction_during_growth
+    s.mu_ws[0] = s.mu_wd[0] = friction_during_growth
+    s.radius = r * s0
+    st = open_stepper(s.tobytes())
+    for k in range(1, stages + 1):
+        sk = s0 + (1.0 - s0) * k / stages
+        x = st.get("x").copy()
+        x[:, 3] = r * sk
+        st.set("x", x)
+        st.step(steps_per_stage)
+        if log:
+            log("grow stage %d/%d scale %.4f" % (k, stages, sk))
+    s.frombytes(st.tobytes())
+    s.g[:] = g
+    s.mu_s[0], s.mu_d[0], s.mu_ws[0], s.mu_wd[0] = mu
+    s.vel[:] = 0.0
+    s.angvel[:] = 0.0
+    s.time_current[0] = 0.0
+    if settle_steps:
+        st = open_stepper(s.tobytes())
+        st.step(settle_steps)
+        s.frombytes(st.tobytes())
+        s.time_current[0] = 0.0
+    return s
+
+
+def shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi=0.58, k_n=1.16e7,
+               damping_frac=0.5, mu=0.5, normal_stress=10.0e3, shear_vel=1.0e-3, rho=2600.0,
+               open_stepper=None, stages=40, steps_per_stage=200, settle_steps=3000,
+               consolidate_steps=2000, log=None):
+    """The direct-shear workload: lognormal radii, a packing grown to
+    fraction `phi` in a box periodic in x and y, settled under gravity, then
+    consolidate(normal_stress) (wmode 1) and shear() with the top layer at
+    `shear_vel` (python/sphere.py:3019-3048,3097-3165).  With `open_stepper`
+    (see grow_pack) the packing is relaxed by DEM steps; the final
+    consolidation also runs `consolidate_steps` before the shear setup."""
+    rng = numpy.random.default_rng(seed)
+    r = lognormal_radii(n, median, sigma, rng)
+    vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
+    if lxy is None:
+        lxy = max(6 * 2.1 * r.max(), (vol / phi / 1.5) ** (1.0 / 3.0))
+    h = vol / phi / (lxy * lxy)
+    s = sim(np=n, sid="shear-%d" % n)
+    s.radius = r
+    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
+    _set_contact_params(s, k_n, damping_frac, mu, r.min())
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.periodicBoundariesXY()
+    s.defineWorldBoundaries(L=[lxy, lxy, h])
+    dt = dt_rule(r.min(), rho, k_n)
+    s.time_dt[0] = dt
+    s.time_current[0] = 0.0
+    s.time_total[0] = 1.0e3
+    s.time_file_dt[0] = 1.0e3
+    grow_pack(s, open_stepper, stages=stages, steps_per_stage=steps_per_stage,
+              settle_steps=settle_steps, seed=seed + 1, log=log)
+    s.consolidate(normal_stress)
+    if consolidate_steps:
+        st = open_stepper(s.tobytes())
+        st.step(consolidate_steps)
+        s.frombytes(st.tobytes())
+        s.time_current[0] = 0.0
+    z_min = float(numpy.min(s.x[:, 2] - s.radius))
+    z_max = float(numpy.max(s.x[:, 2] + s.radius))
+    s.shear(shear_strain_rate=shear_vel / (z_max - z_min))
+    return s

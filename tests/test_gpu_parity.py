@@ -13,6 +13,8 @@ import pytest
 import oracle
 from sphere_amd import engine, scenes
 
+from . import cases
+
 pytestmark = pytest.mark.gpu
 
 EXACT_FIELDS = ("perm", "cellid", "contacts", "x", "vel", "angvel", "angpos", "force",
@@ -52,35 +54,43 @@ def _run_pair(s, steps, chunks=1):
     return ctx, ref
 
 
-@pytest.mark.parametrize("n,seed", [(300, 11), (1500, 12)])
+@pytest.mark.parametrize("n,seed", [(300, 11), (1200, 12)])
 def test_settling_box_bit_exact(n, seed):
-    s = scenes.settling_box(n=n, seed=seed, box=(0.12, 0.12, 0.3))
-    # Start with downward velocities so particles reach the floor and walls
-    # within the test.
-    rng = numpy.random.default_rng(seed)
-    s.vel[:, 2] = -rng.uniform(0.5, 2.0, n)
-    s.vel[:, 0] = rng.uniform(-0.5, 0.5, n)
+    s = cases.settling_case(n, seed)
     ctx, ref = _run_pair(s, 200, chunks=4)
     assert _compare(ctx, ref) == []
     _occupied_cells_equal(ctx, ref)
     ctx.close()
 
 
-@pytest.mark.parametrize("n,seed,shrink", [(500, 21, 0.97), (2000, 22, 0.95)])
-def test_shear_box_bit_exact(n, seed, shrink):
-    s = scenes.shear_box(n=n, seed=seed, phi=0.3, shrink=shrink)
-    ctx, ref = _run_pair(s, 60, chunks=3)
+@pytest.mark.parametrize("name,steps", [("shear1k.bin", 300), ("cube-init.output00029.bin", 200),
+                                        ("init-1e4-slab.bin", 200)])
+def test_fixture_bit_exact(name, steps):
+    """Contact-rich beds: the reference's own settled outputs (upgraded to
+    v2.15) and a grown shear cell with a stress-controlled top wall."""
+    s = cases.fixture(name)
+    n = int(s.np[0])
+    ctx, ref = _run_pair(s, steps, chunks=4)
     bad = _compare(ctx, ref)
     bad += _compare(ctx, ref, ("walls_nx", "walls_mvfd", "walls_acc", "walls_force_pp"))
     assert bad == []
     _occupied_cells_equal(ctx, ref)
     c = ctx.get("contacts")
-    assert (c != n).sum() > n  # the scene is contact-rich
+    assert (c != n).sum() > n  # contact-rich: more than one partner per particle
+    ctx.close()
+
+
+def test_settling_full_trajectory_bit_exact():
+    """The 4,096-sphere settling workload, all 2,000 steps."""
+    s = scenes.settling_box()
+    ctx, ref = _run_pair(s, 2000, chunks=8)
+    assert _compare(ctx, ref) == []
+    _occupied_cells_equal(ctx, ref)
     ctx.close()
 
 
 def test_contact_model_1_bit_exact():
-    s = scenes.shear_box(n=800, seed=31, phi=0.3, shrink=0.96)
+    s = cases.fixture("shear1k.bin")
     s.contactmodel[0] = 1
     ctx, ref = _run_pair(s, 40, chunks=2)
     assert _compare(ctx, ref, [f for f in EXACT_FIELDS if f not in ("contacts", "delta_t")]) == []
@@ -88,7 +98,7 @@ def test_contact_model_1_bit_exact():
 
 
 def test_contact_model_3_bit_exact():
-    s = scenes.shear_box(n=800, seed=32, phi=0.3, shrink=0.96)
+    s = cases.fixture("init-1e4-slab.bin")
     s.contactmodel[0] = 3
     s.gamma_r[0] = 1.0
     s.mu_r[0] = 0.1
