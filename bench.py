@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: DEM direct shear, 1e5 particles, FP64, on one MI355X.
+
+Prints one JSON line.  A step is one iteration of the fluid-free loop body
+of DEM::startTime (hash, sort, cells, contact search + interaction,
+integration, wall reduction and wall integration) over all particles; the
+metric is particle-steps/s over the timed steps, with the state resident in
+HBM before timing starts.
+
+Workload: 100,000 spheres, lognormal radii (median 0.01 m, sigma 0.2),
+density 2600 kg/m^3, in a 1.0 x 1.0 m box periodic in x and y, prepacked by
+seeded gravity settling (sphere_amd.scenes.settled_shear_cell), consolidated
+under a 10 kPa stress-controlled top wall, then sheared with the top layer at
+1e-3 m/s; linear visco-elastic contacts (contactmodel 2) with Coulomb
+mu = 0.5, k_n = k_t = 1.16e7 N/m, gamma = 0.5 critical, dt = 0.075
+sqrt(m_min/k_n).
+"""
+
+import argparse
+import hashlib
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def kernel_source_hash():
+    h = hashlib.sha256()
+    for f in ("dem_kernels.hip", "dem_math.hpp", "dem_types.hpp", "dem_arrays.hpp"):
+        with open(os.path.join(ROOT, "sphere_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(z, nw):
+    """HBM bytes per launch of the contact kernel from the committed PMC
+    summary, when it was measured on the current kernel sources."""
+    fn = os.path.join(ROOT, "profiles", "traffic_contact.json")
+    if not os.path.exists(fn):
+        return None
+    try:
+        with open(fn) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel_source_sha256") != kernel_source_hash():
+        return None
+    return t.get("bytes_per_launch")
+
+
+def build_workload(args):
+    from sphere_amd import engine, scenes
+    if args.load_image:
+        with open(args.load_image, "rb") as fh:
+            return fh.read()
+    t0 = time.time()
+
+    def opener(img):
+        return engine.Context(img)
+
+    s = scenes.settled_shear_cell(args.np, args.seed, open_stepper=opener,
+                                  settle_steps=args.settle_steps,
+                                  consolidate_steps=args.consolidate_steps, log=log)
+    img = s.tobytes()
+    log("workload prepacked in %.1f s" % (time.time() - t0))
+    if args.save_image:
+        with open(args.save_image, "wb") as fh:
+            fh.write(img)
+    return img
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--np", type=int, default=100000)
+    ap.add_argument("--seed", type=int, default=2001)
+    ap.add_argument("--settle-steps", type=int, default=70000)
+    ap.add_argument("--consolidate-steps", type=int, default=10000)
+    ap.add_argument("--profile-steps", type=int, default=200)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-image")
+    ap.add_argument("--load-image")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this engine runs the DEM step on one GPU; --gpus %d is not supported "
+              "(see DESIGN.md for how the path would shard)." % max(args.gpus, world),
+              file=sys.stderr)
+        return 2
+
+    from sphere_amd import engine, scenes
+
+    img = build_workload(args)
+    ctx = engine.Context(img)
+    n = int(ctx.scalar("np"))
+    nw = int(ctx.scalar("nw"))
+    dt = ctx.scalar("dt")
+    log("np %d, nw %d, ncells %d, dt %.4g s" % (n, nw, int(ctx.scalar("ncells")), dt))
+    ctx.step(args.warmup, sync=True)
+    z = scenes.contact_number(ctx, n)
+
+    # Timed region: K steps, synchronised on both sides.
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(args.steps, sync=False)
+    ctx.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = n * args.steps / elapsed
+    z_end = scenes.contact_number(ctx, n)
+    log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f)"
+        % (args.steps, elapsed, value, z, z_end))
+
+    # Kernel timing pass with HIP events on the context stream.
+    ctx.profile(True)
+    ctx.step(args.profile_steps, sync=True)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    log("kernel means (us): " + ", ".join("%s %.2f" % (k, v[0]) for k, v in rep.items()))
+    zz = 0.5 * (z + z_end)
+    # Algorithmic bytes of the fused contact kernel per particle: the search
+    # (36 + 128 + 36 Z) plus the interaction (84 + 128 + 72 Z + 104 + 8 nw).
+    b_contact = 480.0 + 108.0 * zz + 8.0 * nw
+    t_contact = rep.get("contact", (float("nan"), 0))[0] * 1e-6
+    achieved = n * b_contact / t_contact / 1e9
+    traffic = measured_traffic(zz, nw)
+    roofline = {"bound": "hbm", "kernel": "k_contact", "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "algorithmic_bytes_per_launch": round(n * b_contact),
+                "mean_launch_us": round(t_contact * 1e6, 3),
+                "contacts_per_particle": round(zz, 3)}
+    # Whole-step algorithmic floor, B_min = 528 + 72 Z bytes per particle-step.
+    step_bytes = 528.0 + 72.0 * zz
+    step_gbs = step_bytes * value / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        oracle.set_threads(threads)
+        state = ctx.tobytes()
+        o = oracle.Oracle(state)
+        o.step(1)
+        c0 = time.perf_counter()
+        o.step(args.cpu_steps)
+        c1 = time.perf_counter()
+        cpu_rate = n * args.cpu_steps / (c1 - c0)
+        cpu = {"value": round(cpu_rate, 1), "unit": "particle-steps/s", "cores": threads,
+               "kind": "port",
+               "sample": "CPU oracle (oracle/sphere_oracle.cpp, g++ -O3 -fopenmp "
+                         "-ffp-contract=off), %d steps of the same %d-particle shear state "
+                         "after the GPU timed region, %.1f s, CPU: %s"
+                         % (args.cpu_steps, n, c1 - c0, cpu_model())}
+        log("cpu baseline: %.4e particle-steps/s on %d threads" % (cpu_rate, threads))
+    ctx.close()
+
+    line = {
+        "metric": "particle-steps/s at 1e5 particles FP64",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "direct shear, contactmodel 2, settled lognormal bed",
+                   "np": n, "nw": nw, "periodic": "xy", "normal_stress_Pa": 1.0e4,
+                   "shear_velocity_m_s": 1.0e-3, "k_n": 1.16e7, "mu": 0.5, "dt_s": dt,
+                   "contacts_per_particle": round(zz, 3), "seed": args.seed},
+        "roofline": roofline,
+        "step_roofline": {"bytes_per_particle_step": round(step_bytes, 1),
+                          "achieved_GBs": round(step_gbs, 2),
+                          "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
+        "cpu_baseline": cpu,
+        "kernel_us": {k: v[0] for k, v in rep.items()},
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

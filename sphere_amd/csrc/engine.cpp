@@ -89,6 +89,8 @@ struct Writer {
 
 }  // namespace
 
+struct sph_prof_state;
+
 struct sph_ctx {
     Host h;
     DevParams P;
@@ -108,6 +110,7 @@ struct sph_ctx {
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
     std::vector<long> prof_n;
+    sph_prof_state* prof = nullptr;
 };
 
 namespace {
@@ -544,38 +547,86 @@ int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sp
 
 // -- step loop --------------------------------------------------------------
 
+// Per-kernel timing with HIP events on the context stream.  Events are
+// recorded around each launch without synchronising; durations are
+// collected when the report is asked for.
+struct ProfEvent {
+    int stage;
+    hipEvent_t e0, e1;
+};
+
 struct Prof {
     sph_ctx* c;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
     int idx = -1;
-    void begin(const char* name) {
-        if (!c->profile) return;
-        idx = -1;
-        for (size_t k = 0; k < c->prof_names.size(); ++k)
-            if (c->prof_names[k] == name) idx = (int)k;
-        if (idx < 0) {
-            c->prof_names.push_back(name);
-            c->prof_ms.push_back(0.0);
-            c->prof_n.push_back(0);
-            idx = (int)c->prof_names.size() - 1;
-        }
-        (void)hipEventCreate(&e0);
-        (void)hipEventCreate(&e1);
-        (void)hipEventRecord(e0, c->stream);
-    }
-    void end() {
-        if (!c->profile || idx < 0) return;
-        (void)hipEventRecord(e1, c->stream);
-        (void)hipEventSynchronize(e1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        c->prof_ms[idx] += ms;
-        c->prof_n[idx] += 1;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        idx = -1;
-    }
+    hipEvent_t e0 = nullptr;
+    void begin(const char* name);
+    void end();
 };
+
+}  // namespace
+
+struct sph_prof_state {
+    std::vector<ProfEvent> pending;
+    std::vector<hipEvent_t> pool;
+};
+
+namespace {
+
+sph_prof_state& prof_state(sph_ctx* c) {
+    if (!c->prof) c->prof = new sph_prof_state();
+    return *c->prof;
+}
+
+hipEvent_t take_event(sph_ctx* c) {
+    sph_prof_state& ps = prof_state(c);
+    if (!ps.pool.empty()) {
+        hipEvent_t e = ps.pool.back();
+        ps.pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void Prof::begin(const char* name) {
+    if (!c->profile) return;
+    idx = -1;
+    for (size_t k = 0; k < c->prof_names.size(); ++k)
+        if (c->prof_names[k] == name) idx = (int)k;
+    if (idx < 0) {
+        c->prof_names.push_back(name);
+        c->prof_ms.push_back(0.0);
+        c->prof_n.push_back(0);
+        idx = (int)c->prof_names.size() - 1;
+    }
+    e0 = take_event(c);
+    (void)hipEventRecord(e0, c->stream);
+}
+
+void Prof::end() {
+    if (!c->profile || idx < 0) return;
+    hipEvent_t e1 = take_event(c);
+    (void)hipEventRecord(e1, c->stream);
+    prof_state(c).pending.push_back(ProfEvent{idx, e0, e1});
+    idx = -1;
+}
+
+// Fold the recorded events into the per-stage totals.
+void prof_collect(sph_ctx* c) {
+    sph_prof_state& ps = prof_state(c);
+    if (ps.pending.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (const ProfEvent& pe : ps.pending) {
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, pe.e0, pe.e1);
+        c->prof_ms[pe.stage] += ms;
+        c->prof_n[pe.stage] += 1;
+        ps.pool.push_back(pe.e0);
+        ps.pool.push_back(pe.e1);
+    }
+    ps.pending.clear();
+}
 
 int enqueue_step(sph_ctx* c) {
     const DevParams& P = c->P;
@@ -1107,6 +1158,7 @@ int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_
 
 int sph_profile(sph_ctx* c, int enable) {
     if (!c) return SPH_E_ARG;
+    prof_collect(c);
     c->profile = enable != 0;
     c->prof_names.clear();
     c->prof_ms.clear();
@@ -1116,6 +1168,7 @@ int sph_profile(sph_ctx* c, int enable) {
 
 int sph_profile_report(sph_ctx* c, char* buf, size_t cap) {
     if (!c) return SPH_E_ARG;
+    prof_collect(c);
     std::string out;
     char line[256];
     for (size_t k = 0; k < c->prof_names.size(); ++k) {
@@ -1130,6 +1183,11 @@ int sph_profile_report(sph_ctx* c, char* buf, size_t cap) {
 void sph_close(sph_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->prof) {
+        prof_collect(c);
+        for (hipEvent_t e : c->prof->pool) (void)hipEventDestroy(e);
+        delete c->prof;
+    }
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

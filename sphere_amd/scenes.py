@@ -173,78 +173,127 @@ def shear_box(n=2000, seed=2001, median=0.01, sigma=0.2, phi=0.25, lxy=None, k_n
     return s
 
 
-def grow_pack(s, open_stepper, phi_rsa=0.28, stages=40, steps_per_stage=200, settle_steps=3000,
-              seed=0, friction_during_growth=0.0, log=None):
-    """Densify a periodic-x/y packing by growing the radii in place.
+def _quench(st):
+    """Remove all kinetic energy, of the particles (keeping fixvel in vel.w)
+    and of the dynamic walls."""
+    v = st.get("vel").copy()
+    v[:, :3] = 0.0
+    st.set("vel", v)
+    st.set("angvel", numpy.zeros((v.shape[0], 3)))
+    w = st.get("walls_mvfd")
+    if w.shape[0]:
+        w = w.copy()
+        w[:, 1] = 0.0
+        st.set("walls_mvfd", w)
 
-    `s` carries the true radii, the box (origo, L with L[2] the fixed top
-    during growth) and the final contact parameters.  Particles are first
-    inserted without overlap with radii scaled by s0 = (phi_rsa / phi)^(1/3)
-    (phi the final packing fraction of the box), then the radii grow back to
-    their true values in `stages` increments, each followed by
-    `steps_per_stage` DEM steps without gravity and without friction.  Then
-    gravity and friction are restored and the bed settles for `settle_steps`.
 
-    `open_stepper(image)` returns an object with step(n), get(name),
-    set(name, value) and tobytes() -- a GPU engine context or the CPU
-    oracle.  Returns the settled sim.
+def _relax(st, steps, quench_every):
+    done = 0
+    while done < steps:
+        k = min(quench_every, steps - done) if quench_every else steps - done
+        if quench_every:
+            _quench(st)
+        st.step(k)
+        done += k
+
+
+def _median_pressure(st):
+    """Median of the per-particle pressure over all particles: it becomes
+    non-zero only when most particles carry load (a jammed packing)."""
+    p = st.get("p")
+    return float(numpy.median(p)) if p.size else 0.0
+
+
+def jam_pack(s, open_stepper, p_target, phi_rsa=0.30, d_scale=0.02, max_stages=400,
+             steps_per_stage=100, quench_every=50, seed=0, log=None):
+    """Jam a packing at a target particle pressure, then rescale to the
+    true radii.
+
+    Particles are inserted without overlap into the box of `s` with radii
+    r * s0 (s0 from phi_rsa).  The radii then grow (without gravity and
+    friction, kinetic energy quenched every `quench_every` steps) by
+    factors of (1 + d_scale); whenever the median particle pressure p
+    (the reference's per-particle p field) exceeds 2 p_target the last
+    increment is undone and the increment halved, until p lies within
+    [p_target / 2, 2 p_target].  Finally every length (positions, box, grid
+    origin) is divided by the final radius scale, which restores the true
+    radii without changing the contact geometry.  `open_stepper(image)`
+    returns an engine context or oracle (step/get/set/tobytes).
     """
     rng = numpy.random.default_rng(seed)
     r = numpy.asarray(s.radius, dtype=numpy.float64).copy()
-    n = r.size
     vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
     box = float((s.L[0] - s.origo[0]) * (s.L[1] - s.origo[1]) * (s.L[2] - s.origo[2]))
-    phi = vol / box
-    s0 = (phi_rsa / phi) ** (1.0 / 3.0)
-    if s0 >= 1.0:
-        s0 = 1.0
-    s.x = rsa_positions(r * s0, list(s.origo), list(s.L), rng,
+    sc = min(1.0, (phi_rsa * box / vol) ** (1.0 / 3.0))
+    s.x = rsa_positions(r * sc, list(s.origo), list(s.L), rng,
                         periodic=(s.periodic[0] in (1, 2), s.periodic[0] == 1, False))
     g = s.g.copy()
     mu = (float(s.mu_s[0]), float(s.mu_d[0]), float(s.mu_ws[0]), float(s.mu_wd[0]))
     s.g[:] = 0.0
-    s.mu_s[0] = s.mu_d[0] = friction_during_growth
-    s.mu_ws[0] = s.mu_wd[0] = friction_during_growth
-    s.radius = r * s0
+    s.mu_s[0] = s.mu_d[0] = s.mu_ws[0] = s.mu_wd[0] = 0.0
+    s.radius = r * sc
     st = open_stepper(s.tobytes())
-    for k in range(1, stages + 1):
-        sk = s0 + (1.0 - s0) * k / stages
-        x = st.get("x").copy()
-        x[:, 3] = r * sk
+    d = d_scale
+    for stage in range(max_stages):
+        x_prev = st.get("x").copy()
+        trial = sc * (1.0 + d)
+        x = x_prev.copy()
+        x[:, 3] = r * trial
         st.set("x", x)
-        st.step(steps_per_stage)
+        _relax(st, steps_per_stage, quench_every)
+        pm = _median_pressure(st)
         if log:
-            log("grow stage %d/%d scale %.4f" % (k, stages, sk))
+            log("jam stage %d scale %.6f step %.2e median p %.3e" % (stage, trial, d, pm))
+        if pm > 2.0 * p_target and d > 1e-7:
+            st.set("x", x_prev)
+            _relax(st, steps_per_stage, quench_every)
+            d *= 0.5
+            continue
+        sc = trial
+        if pm >= 0.5 * p_target:
+            break
+        if pm > 0.0:
+            d = min(d, 0.5 * d_scale)
     s.frombytes(st.tobytes())
+    inv = 1.0 / sc
+    s.x = s.x * inv
+    s.radius = r.copy()
+    s.L = s.L * inv
+    s.origo = s.origo * inv
     s.g[:] = g
     s.mu_s[0], s.mu_d[0], s.mu_ws[0], s.mu_wd[0] = mu
     s.vel[:] = 0.0
     s.angvel[:] = 0.0
     s.time_current[0] = 0.0
-    if settle_steps:
-        st = open_stepper(s.tobytes())
-        st.step(settle_steps)
-        s.frombytes(st.tobytes())
-        s.time_current[0] = 0.0
-    return s
+    # Cells of at least 2.1 r_max after the rescale.
+    for dd in range(3):
+        s.num[dd] = max(3, int(numpy.floor((s.L[dd] - s.origo[dd]) / (2.1 * r.max()))))
+    return sc
 
 
-def shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi=0.58, k_n=1.16e7,
-               damping_frac=0.5, mu=0.5, normal_stress=10.0e3, shear_vel=1.0e-3, rho=2600.0,
-               open_stepper=None, stages=40, steps_per_stage=200, settle_steps=3000,
-               consolidate_steps=2000, log=None):
-    """The direct-shear workload: lognormal radii, a packing grown to
-    fraction `phi` in a box periodic in x and y, settled under gravity, then
-    consolidate(normal_stress) (wmode 1) and shear() with the top layer at
-    `shear_vel` (python/sphere.py:3019-3048,3097-3165).  With `open_stepper`
-    (see grow_pack) the packing is relaxed by DEM steps; the final
-    consolidation also runs `consolidate_steps` before the shear setup."""
+def contact_number(st, n):
+    """Mean number of occupied contact slots per particle."""
+    return float((st.get("contacts") != n).sum()) / max(n, 1)
+
+
+def shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, k_n=1.16e7, damping_frac=0.5,
+               mu=0.5, normal_stress=10.0e3, shear_vel=1.0e-3, rho=2600.0, open_stepper=None,
+               steps_per_stage=100, consolidate_steps=2000, quench_every=50, relax_steps=1000,
+               log=None):
+    """The direct-shear workload: lognormal radii in a box periodic in x and
+    y, jammed at the target normal stress (jam_pack), then consolidate
+    (normal_stress on the top wall, wmode 1) under gravity, then shear() with
+    the top layer moving at `shear_vel` (python/sphere.py:3019-3048,
+    3097-3165).  `open_stepper` runs the DEM relaxations: consolidation runs
+    `consolidate_steps` with kinetic-energy quenching, then `relax_steps`
+    without."""
     rng = numpy.random.default_rng(seed)
     r = lognormal_radii(n, median, sigma, rng)
     vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
+    phi0 = 0.58
     if lxy is None:
-        lxy = max(6 * 2.1 * r.max(), (vol / phi / 1.5) ** (1.0 / 3.0))
-    h = vol / phi / (lxy * lxy)
+        lxy = max(6 * 2.1 * r.max(), (vol / phi0 / 1.5) ** (1.0 / 3.0))
+    h = vol / phi0 / (lxy * lxy)
     s = sim(np=n, sid="shear-%d" % n)
     s.radius = r
     s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
@@ -257,14 +306,101 @@ def shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi=0.58, k_n=1.16e7,
     s.time_current[0] = 0.0
     s.time_total[0] = 1.0e3
     s.time_file_dt[0] = 1.0e3
-    grow_pack(s, open_stepper, stages=stages, steps_per_stage=steps_per_stage,
-              settle_steps=settle_steps, seed=seed + 1, log=log)
+    jam_pack(s, open_stepper, p_target=normal_stress, steps_per_stage=steps_per_stage,
+             quench_every=quench_every, seed=seed + 1, log=log)
     s.consolidate(normal_stress)
-    if consolidate_steps:
-        st = open_stepper(s.tobytes())
-        st.step(consolidate_steps)
-        s.frombytes(st.tobytes())
-        s.time_current[0] = 0.0
+    st = open_stepper(s.tobytes())
+    _relax(st, consolidate_steps, quench_every)
+    _relax(st, relax_steps, 0)
+    if log:
+        log("consolidated: mean contacts per particle %.2f" % contact_number(st, n))
+    s.frombytes(st.tobytes())
+    s.time_current[0] = 0.0
+    z_min = float(numpy.min(s.x[:, 2] - s.radius))
+    z_max = float(numpy.max(s.x[:, 2] + s.radius))
+    s.shear(shear_strain_rate=shear_vel / (z_max - z_min))
+    return s
+
+
+def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi_rsa=0.33, k_n=1.16e7,
+                       damping_frac=0.5, mu=0.5, normal_stress=10.0e3, shear_vel=1.0e-3,
+                       rho=2600.0, open_stepper=None, gravity_factor=100.0, settle_steps=60000,
+                       ramp_stages=8, ramp_steps=2000, quench_every=500,
+                       consolidate_steps=10000, free_steps=2000, chunk=2000, log=None):
+    """The direct-shear workload, prepacked by seeded gravity settling.
+
+    Lognormal radii (median, sigma) are inserted without overlap at fraction
+    `phi_rsa` into a box periodic in x and y (side `lxy`, default 1.0 m for
+    1e5 particles, scaled with sqrt(n)); the bed settles under
+    `gravity_factor` x 9.81 m/s^2 for `settle_steps` (an accelerated
+    pluviation: the fall time shrinks by sqrt(gravity_factor)), then gravity
+    returns to 9.81 m/s^2, consolidate(normal_stress) puts the stress-
+    controlled top wall on the bed (python/sphere.py:3019-3048) and it
+    relaxes for `consolidate_steps` (gravity is lowered in `ramp_stages`
+    geometric stages of `ramp_steps` each, with the kinetic energy removed
+    every `quench_every` steps, before the wall is placed); finally shear()
+    fixes the bottom layer
+    and drives the top layer at `shear_vel` (python/sphere.py:3097-3165).
+    `open_stepper(image)` runs the DEM steps (GPU context or oracle)."""
+    rng = numpy.random.default_rng(seed)
+    r = lognormal_radii(n, median, sigma, rng)
+    vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
+    if lxy is None:
+        lxy = max(6 * 2.1 * r.max(), 1.0 * math.sqrt(n / 1.0e5))
+    h = vol / phi_rsa / (lxy * lxy)
+    s = sim(np=n, sid="shear-%d" % n)
+    s.radius = r
+    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
+    _set_contact_params(s, k_n, damping_frac, mu, r.min())
+    s.periodicBoundariesXY()
+    s.defineWorldBoundaries(L=[lxy, lxy, h])
+    s.x = rsa_positions(r, [0.0, 0.0, 0.0], [lxy, lxy, h], rng, periodic=(True, True, False))
+    s.g[:] = [0.0, 0.0, -9.81 * gravity_factor]
+    dt = dt_rule(r.min(), rho, k_n)
+    s.time_dt[0] = dt
+    s.time_current[0] = 0.0
+    s.time_total[0] = 1.0e3
+    s.time_file_dt[0] = 1.0e3
+    st = open_stepper(s.tobytes())
+    done = 0
+    while done < settle_steps:
+        k = min(chunk, settle_steps - done)
+        st.step(k)
+        done += k
+        if log:
+            log("settle %d/%d steps, contacts per particle %.2f" % (done, settle_steps,
+                                                                   contact_number(st, n)))
+    # Lower gravity back to 9.81 m/s^2 in geometric stages, removing the
+    # kinetic energy the bed gains as its elastic compression relaxes.
+    if gravity_factor > 1.0:
+        for gk in numpy.geomspace(gravity_factor, 1.0, ramp_stages + 1)[1:]:
+            s.frombytes(st.tobytes())
+            s.g[:] = [0.0, 0.0, -9.81 * gk]
+            s.vel[:] = 0.0
+            s.angvel[:] = 0.0
+            s.time_current[0] = 0.0
+            st = open_stepper(s.tobytes())
+            _relax(st, ramp_steps, quench_every)
+            if log:
+                log("gravity x%.2f: contacts per particle %.2f" % (gk, contact_number(st, n)))
+    s.frombytes(st.tobytes())
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.time_current[0] = 0.0
+    s.consolidate(normal_stress)
+    st = open_stepper(s.tobytes())
+    done = 0
+    while done < consolidate_steps:
+        k = min(chunk, consolidate_steps - done)
+        # The load is applied at once; quenching damps the wall-bed
+        # oscillation, except over the last `free_steps`.
+        q = quench_every if done + k <= consolidate_steps - free_steps else 0
+        _relax(st, k, q)
+        done += k
+        if log:
+            log("consolidate %d/%d steps, contacts per particle %.2f" % (
+                done, consolidate_steps, contact_number(st, n)))
+    s.frombytes(st.tobytes())
+    s.time_current[0] = 0.0
     z_min = float(numpy.min(s.x[:, 2] - s.radius))
     z_max = float(numpy.max(s.x[:, 2] + s.radius))
     s.shear(shear_strain_rate=shear_vel / (z_max - z_min))
