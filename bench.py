@@ -8,8 +8,9 @@ metric is particle-steps/s over the timed steps, with the state resident in
 HBM before timing starts.
 
 Workload: 100,000 spheres, lognormal radii (median 0.01 m, sigma 0.2),
-density 2600 kg/m^3, in a 1.0 x 1.0 m box periodic in x and y, prepacked by
-seeded gravity settling (sphere_amd.scenes.settled_shear_cell), consolidated
+density 2600 kg/m^3, in a 1.6 x 1.6 m cell periodic in x and y (about
+0.35 m deep), prepacked by seeded gravity settling
+(sphere_amd.scenes.settled_shear_cell), consolidated
 under a 10 kPa stress-controlled top wall, then sheared with the top layer at
 1e-3 m/s; linear visco-elastic contacts (contactmodel 2) with Coulomb
 mu = 0.5, k_n = k_t = 1.16e7 N/m, gamma = 0.5 critical, dt = 0.075
@@ -99,7 +100,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--np", type=int, default=100000)
     ap.add_argument("--seed", type=int, default=2001)
-    ap.add_argument("--settle-steps", type=int, default=70000)
+    ap.add_argument("--settle-steps", type=int, default=40000)
     ap.add_argument("--consolidate-steps", type=int, default=10000)
     ap.add_argument("--profile-steps", type=int, default=200)
     ap.add_argument("--cpu-steps", type=int, default=20)

@@ -89,6 +89,10 @@ def _lib():
         L.orc_time.argtypes = [ctypes.c_void_p]
         L.orc_iter.restype = ctypes.c_int64
         L.orc_iter.argtypes = [ctypes.c_void_p]
+        L.orc_sqrt32.restype = ctypes.c_double
+        L.orc_sqrt32.argtypes = [ctypes.c_double]
+        L.orc_tree_sum.restype = ctypes.c_double
+        L.orc_tree_sum.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _LIB = L
@@ -169,3 +173,15 @@ class Oracle:
         buf = ctypes.create_string_buffer(n)
         _lib().orc_write_mem(self._h, buf, n)
         return buf.raw
+
+
+def sqrt32(v):
+    """The reference's length() rule on one value: FP32 root of v, widened."""
+    return float(_lib().orc_sqrt32(float(v)))
+
+
+def tree_sum(values):
+    """The reference's wall-force reduction of `values` (block trees of 256,
+    then a serial sum of the block results)."""
+    a = numpy.ascontiguousarray(values, dtype=numpy.float64)
+    return float(_lib().orc_tree_sum(a.ctypes.data, a.size))

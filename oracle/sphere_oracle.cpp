@@ -1244,6 +1244,16 @@ int64_t orc_write_mem(const orc_sim* s, void* dst, size_t cap) {
     return (int64_t)w.b.size();
 }
 
+double orc_sqrt32(double v) { return sqrt32(v); }
+
+double orc_tree_sum(const double* v, uint32_t n) {
+    std::vector<Real> partial;
+    block_tree_sum(v, n, partial);
+    Real sum = 0.0;
+    for (size_t i = 0; i < partial.size(); ++i) sum += partial[i];
+    return sum;
+}
+
 uint32_t orc_np(const orc_sim* s) { return s->np; }
 uint32_t orc_nc(void) { return NC; }
 double orc_time(const orc_sim* s) { return s->t; }

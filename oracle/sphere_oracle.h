@@ -63,6 +63,12 @@ uint32_t orc_np(const orc_sim* s);
 uint32_t orc_nc(void);
 double orc_time(const orc_sim* s);
 int64_t orc_iter(const orc_sim* s);
+/* Test hooks: the FP32 length rule on one value (sqrtf((float)v) widened),
+ * and the reference's wall-force reduction (256-wide block trees, then a
+ * serial sum over blocks) of n values. */
+double orc_sqrt32(double v);
+double orc_tree_sum(const double* v, uint32_t n);
+
 /* Number of OpenMP threads the per-particle stages use (0 = runtime default). */
 void orc_set_threads(int n);
 int orc_get_threads(void);

@@ -91,12 +91,19 @@ def rsa_positions(radii, lo, hi, rng, periodic=(False, False, False), max_tries=
 
 
 def _set_contact_params(s, k_n, damping_frac, mu, r_min):
+    """k_t = k_n; gamma_n = gamma_t = damping_frac * critical (true m_min);
+    the wall viscosities take the same value.  (The reference's default wall
+    viscosity of 1e4 N s/m, python/sphere.py:3773-3774, makes the explicit
+    update unstable for the smallest particles at dt = 0.075 sqrt(m_min/k_n):
+    gamma dt / m_min exceeds 2.)"""
     rho = float(s.rho[0])
     s.k_n[0] = k_n
     s.k_t[0] = k_n
     gam = damping_frac * critical_damping(r_min, rho, k_n)
     s.gamma_n[0] = gam
     s.gamma_t[0] = gam
+    s.gamma_wn[0] = gam
+    s.gamma_wt[0] = gam
     s.mu_s[0] = mu
     s.mu_d[0] = mu
     s.mu_ws[0] = mu
@@ -117,7 +124,7 @@ def settling_box(n=4096, seed=1001, box=(0.2, 0.2, 0.6), r_range=(0.005, 0.01), 
     s = sim(np=n, sid="settling-%d" % n)
     s.radius = rng.uniform(r_range[0], r_range[1], n)
     s.x = rsa_positions(s.radius, [0.0, 0.0, 0.0], list(box), rng)
-    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n, gamma_wn=1.0e4, gamma_wt=1.0e4)
+    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
     _set_contact_params(s, k_n, damping_frac, mu, s.radius.min())
     s.g[:] = [0.0, 0.0, -9.81]
     s.normalBoundariesXY()
@@ -195,6 +202,47 @@ def _relax(st, steps, quench_every):
             _quench(st)
         st.step(k)
         done += k
+
+
+def grow_pack(s, open_stepper, phi_rsa=0.30, stages=30, steps_per_stage=100, seed=0,
+              quench_every=100, log=None):
+    """Grow radii in place inside the box of `s` (top at L[2], nw == 0).
+
+    Particles are inserted without overlap with radii r * s0,
+    s0 = (phi_rsa / phi)^(1/3) for the fraction phi the true radii fill in
+    the box, then grow back to r in `stages` equal increments, each followed
+    by `steps_per_stage` DEM steps without gravity or friction; kinetic
+    energy is removed every `quench_every` steps.  Meant for fractions well
+    below jamming, where the increments leave no lasting overlap.
+    `open_stepper(image)` returns an engine context or oracle."""
+    rng = numpy.random.default_rng(seed)
+    r = numpy.asarray(s.radius, dtype=numpy.float64).copy()
+    vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
+    box = float((s.L[0] - s.origo[0]) * (s.L[1] - s.origo[1]) * (s.L[2] - s.origo[2]))
+    s0 = min(1.0, (phi_rsa * box / vol) ** (1.0 / 3.0))
+    s.x = rsa_positions(r * s0, list(s.origo), list(s.L), rng,
+                        periodic=(s.periodic[0] in (1, 2), s.periodic[0] == 1, False))
+    g = s.g.copy()
+    mu = (float(s.mu_s[0]), float(s.mu_d[0]), float(s.mu_ws[0]), float(s.mu_wd[0]))
+    s.g[:] = 0.0
+    s.mu_s[0] = s.mu_d[0] = s.mu_ws[0] = s.mu_wd[0] = 0.0
+    s.radius = r * s0
+    st = open_stepper(s.tobytes())
+    for k in range(1, stages + 1):
+        x = st.get("x").copy()
+        x[:, 3] = r * (s0 + (1.0 - s0) * k / stages)
+        st.set("x", x)
+        _relax(st, steps_per_stage, quench_every)
+        if log:
+            log("grow stage %d/%d" % (k, stages))
+    s.frombytes(st.tobytes())
+    s.radius = r.copy()
+    s.g[:] = g
+    s.mu_s[0], s.mu_d[0], s.mu_ws[0], s.mu_wd[0] = mu
+    s.vel[:] = 0.0
+    s.angvel[:] = 0.0
+    s.time_current[0] = 0.0
+    return s
 
 
 def _median_pressure(st):
@@ -322,16 +370,18 @@ def shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, k_n=1.16e7, damping_fr
     return s
 
 
-def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi_rsa=0.33, k_n=1.16e7,
-                       damping_frac=0.5, mu=0.5, normal_stress=10.0e3, shear_vel=1.0e-3,
-                       rho=2600.0, open_stepper=None, gravity_factor=100.0, settle_steps=60000,
-                       ramp_stages=8, ramp_steps=2000, quench_every=500,
-                       consolidate_steps=10000, free_steps=2000, chunk=2000, log=None):
+def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, depth=0.35, phi_rsa=0.33,
+                       k_n=1.16e7, damping_frac=0.5, mu=0.5, normal_stress=10.0e3,
+                       shear_vel=1.0e-3, rho=2600.0, open_stepper=None, gravity_factor=30.0,
+                       settle_steps=40000, ramp_stages=8, ramp_steps=2000, quench_every=500,
+                       consolidate_steps=10000, free_steps=2000, chunk=2000,
+                       grow_to_phi=0.5, grow_stages=30, grow_steps=100, log=None):
     """The direct-shear workload, prepacked by seeded gravity settling.
 
     Lognormal radii (median, sigma) are inserted without overlap at fraction
-    `phi_rsa` into a box periodic in x and y (side `lxy`, default 1.0 m for
-    1e5 particles, scaled with sqrt(n)); the bed settles under
+    `phi_rsa` into a box periodic in x and y (side `lxy`; by default the side
+    that gives a settled bed about `depth` deep, 1.6 m for 1e5 particles);
+    the bed settles under
     `gravity_factor` x 9.81 m/s^2 for `settle_steps` (an accelerated
     pluviation: the fall time shrinks by sqrt(gravity_factor)), then gravity
     returns to 9.81 m/s^2, consolidate(normal_stress) puts the stress-
@@ -341,12 +391,16 @@ def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi_rsa=0.33, 
     every `quench_every` steps, before the wall is placed); finally shear()
     fixes the bottom layer
     and drives the top layer at `shear_vel` (python/sphere.py:3097-3165).
+    With `grow_to_phi`, the particles are first inserted at `phi_rsa` into a
+    box of that fraction and grown to their true radii (grow_pack) before
+    the box top is lifted and the bed settles.
     `open_stepper(image)` runs the DEM steps (GPU context or oracle)."""
     rng = numpy.random.default_rng(seed)
     r = lognormal_radii(n, median, sigma, rng)
     vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
     if lxy is None:
-        lxy = max(6 * 2.1 * r.max(), 1.0 * math.sqrt(n / 1.0e5))
+        # A wide, shallow cell: `depth` of settled bed at a fraction near 0.56.
+        lxy = max(6 * 2.1 * r.max(), math.sqrt(vol / (0.56 * depth)))
     h = vol / phi_rsa / (lxy * lxy)
     s = sim(np=n, sid="shear-%d" % n)
     s.radius = r
@@ -354,13 +408,22 @@ def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, phi_rsa=0.33, 
     _set_contact_params(s, k_n, damping_frac, mu, r.min())
     s.periodicBoundariesXY()
     s.defineWorldBoundaries(L=[lxy, lxy, h])
-    s.x = rsa_positions(r, [0.0, 0.0, 0.0], [lxy, lxy, h], rng, periodic=(True, True, False))
-    s.g[:] = [0.0, 0.0, -9.81 * gravity_factor]
     dt = dt_rule(r.min(), rho, k_n)
     s.time_dt[0] = dt
     s.time_current[0] = 0.0
     s.time_total[0] = 1.0e3
     s.time_file_dt[0] = 1.0e3
+    if grow_to_phi:
+        # Shrink the box to `grow_to_phi` and grow the radii into it: the bed
+        # starts dense but unjammed, so the fall that follows is short.
+        s.L[2] = vol / grow_to_phi / (lxy * lxy)
+        grow_pack(s, open_stepper, phi_rsa=phi_rsa, stages=grow_stages,
+                  steps_per_stage=grow_steps, seed=seed + 1, quench_every=grow_steps)
+        s.L[2] = h
+    else:
+        s.x = rsa_positions(r, [0.0, 0.0, 0.0], [lxy, lxy, h], rng,
+                            periodic=(True, True, False))
+    s.g[:] = [0.0, 0.0, -9.81 * gravity_factor]
     st = open_stepper(s.tobytes())
     done = 0
     while done < settle_steps:
