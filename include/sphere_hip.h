@@ -99,7 +99,10 @@ enum sph_scalar {
     SPH_S_NP = 6,
     SPH_S_NW = 7,
     SPH_S_NB0 = 8,
-    SPH_S_NCELLS = 9
+    SPH_S_NCELLS = 9,
+    SPH_S_VL_SKIN = 10,    /* candidate-list skin (m); 0 = scan 27 cells every step */
+    SPH_S_SORTS = 11,      /* re-sorts performed so far (diagnostic, read-only)      */
+    SPH_S_VL_BUILDS = 12   /* candidate-list rebuilds so far (diagnostic, read-only) */
 };
 
 /* Open flags. */

@@ -39,6 +39,11 @@ struct DevArrays {
     uint32_t* cell_start;
     uint32_t* cell_end;
     double4* xs;         // sorted copy of x
+    uint32_t* rank;      // original index -> sorted index (inverse of perm)
+    // candidate lists (sorted-index rows of NV entries)
+    uint2* vlist;
+    uint32_t* vcount;    // entries per row, or VL_OVERFLOW
+    double4* x_build;    // position at the last list build, original index
     // contact history (NC words per particle, original-index rows)
     uint32_t* crow;      // packed partner index + periodic image code
     double4* delta_t;    // tangential displacement per slot (w unused)

@@ -47,23 +47,29 @@ __device__ __forceinline__ uint32_t calc_cell_id(const DevParams& p, double Lz, 
 
 __global__ __launch_bounds__(BLOCK) void k_hash(const DevParams p, const DevArrays a) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t key = 0, changed = 0, nonfinite = 0;
+    uint32_t changed = 0, nonfinite = 0, moved = 0;
     if (i < p.np) {
         const double Lz = grid_Lz(p, a);
         const double4 x = a.x[i];
-        key = calc_cell_id(p, Lz, x.x, x.y, x.z);
+        const uint32_t key = calc_cell_id(p, Lz, x.x, x.y, x.z);
         if (x.x != x.x || x.y != x.y || x.z != x.z) nonfinite = 1;
         changed = key != a.key_prev[i];
         a.key[i] = key;
+        if (p.vl_enabled) {
+            // Candidate lists stay valid while every particle is within
+            // skin/2 of where the lists were built.
+            const double4 xb = a.x_build[i];
+            const double dx = x.x - xb.x, dy = x.y - xb.y, dz = x.z - xb.z;
+            moved = !(dx * dx + dy * dy + dz * dz <= p.vl_trigger2);
+        }
     }
-    // One atomic per wavefront for the OR of keys and the change flag.
-    uint32_t kor = key;
-    for (int off = 32; off > 0; off >>= 1) kor |= __shfl_xor(kor, off, 64);
+    // At most one atomic per wavefront, and only when something happened.
     const unsigned long long anych = __ballot(changed);
+    const unsigned long long anymv = __ballot(moved);
     const unsigned long long anynf = __ballot(nonfinite);
     if ((threadIdx.x & 63) == 0) {
-        if (kor) atomicOr(&a.loop->key_or, kor);
         if (anych) atomicOr(&a.loop->cells_changed, 1u);
+        if (anych | anymv) atomicOr(&a.loop->vl_rebuild, 1u);
         if (anynf) atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_NONFINITE);
     }
 }
@@ -74,10 +80,14 @@ __global__ __launch_bounds__(BLOCK) void k_hash(const DevParams p, const DevArra
 // (round, thread), which is the global index order, so ranking each round
 // after all earlier rounds keeps the sort stable.
 
-__global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict__ kin,
+// Every sort-stage kernel returns at once when *gate == 0: no particle
+// changed cell, so the stable order of (cell, index) is unchanged.
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict__ gate,
+                                                      const uint32_t* __restrict__ kin,
                                                       uint32_t n, int shift,
                                                       uint32_t* __restrict__ hist,
                                                       uint32_t ntiles) {
+    if (*gate == 0u) return;
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
@@ -92,8 +102,10 @@ __global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict
 }
 
 // Exclusive scan of the digit-major histogram (256 x ntiles) in place.
-__global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist,
+__global__ __launch_bounds__(1024) void k_radix_scan(const uint32_t* __restrict__ gate,
+                                                     uint32_t* __restrict__ hist,
                                                      uint32_t total) {
+    if (*gate == 0u) return;
     __shared__ uint32_t s[1024];
     const uint32_t t = threadIdx.x;
     const uint32_t chunk = (total + 1023u) / 1024u;
@@ -123,9 +135,10 @@ __global__ __launch_bounds__(1024) void k_radix_scan(uint32_t* __restrict__ hist
 }
 
 __global__ __launch_bounds__(BLOCK) void k_radix_scatter(
-    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
-    const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    const uint32_t* __restrict__ gate, const uint32_t* __restrict__ kin,
+    const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+    uint32_t n, int shift, const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    if (*gate == 0u) return;
     __shared__ uint32_t running[256];
     __shared__ uint32_t whist[BLOCK / 64][256];
     const uint32_t t = threadIdx.x;
@@ -171,15 +184,19 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(
 // ---------------------------------------------------------------------------
 // Cells and reorder
 
-__global__ __launch_bounds__(BLOCK) void k_fill_u32(uint32_t* __restrict__ dst, uint32_t n,
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(const uint32_t* __restrict__ gate,
+                                                    uint32_t* __restrict__ dst, uint32_t n,
                                                     uint32_t v) {
+    if (gate && *gate == 0u) return;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) dst[i] = v;
 }
 
 __global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArrays a,
                                                  int sorted_bits) {
+    if (a.loop->cells_changed == 0u) return;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i == 0) atomicAdd(&a.loop->sorts, 1u);
     if (i >= p.np) return;
     const uint32_t key = a.skey[i];
     if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
@@ -194,6 +211,7 @@ __global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArr
     if (i == p.np - 1 && key < p.ncells) a.cell_end[key] = i + 1;
     const uint32_t o = a.perm[i];
     a.xs[i] = a.x[o];
+    a.rank[o] = i;
     a.key_prev[o] = key;
 }
 
@@ -464,6 +482,33 @@ __device__ __forceinline__ uint32_t word_code(uint32_t w) {
 
 constexpr int ROW_STRIDE = NC + 1;
 
+// The contact-slot policy of findContactsInCell for one overlapping
+// partner (src/contactsearch.cuh:265-299): the last empty slot, else the
+// partner's own slot, else slot nc.  Returns true when nc exceeds NC, which
+// ends the scan of the current cell.
+__device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t np, uint32_t ib_orig,
+                                                 uint32_t code, int& nc, DevLoop* loop) {
+    int cpos = nc;
+    for (int i = 0; i < NC; ++i) {
+        const uint32_t cidx = row[i] & CW_IDX_MASK;
+        if (cidx == np) {
+            cpos = i;
+        } else if (cidx == ib_orig) {
+            cpos = i;
+            break;
+        }
+    }
+    // cpos == NC would write into the next particle's row
+    // (src/contactsearch.cuh:286); that write is suppressed.
+    if (cpos < NC) row[cpos] = pack_word(ib_orig, code);
+    ++nc;
+    if (nc > NC) {
+        atomicOr(&loop->error_bits, (uint32_t)DFLAG_CONTACT_OVERFLOW);
+        return true;
+    }
+    return false;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevArrays a) {
     __shared__ uint32_t srow[BLOCK * ROW_STRIDE];
     const uint32_t ia = blockIdx.x * BLOCK + threadIdx.x;
@@ -495,58 +540,89 @@ __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevA
             const uint4 v = grow[q];
             row[4 * q] = v.x; row[4 * q + 1] = v.y; row[4 * q + 2] = v.z; row[4 * q + 3] = v.w;
         }
-        // topology: findContactsInCell over the 27 cells, z-major.
+        // topology (findContactsInCell over the 27 cells, z-major, each cell
+        // in ascending sorted index).  Three ways to visit the same candidate
+        // sequence: the full scan that also rebuilds this particle's
+        // candidate list, the full scan alone (list overflowed), or the list.
         int nc = 0;
-        for (int dz = -1; dz < 2; ++dz)
-            for (int dy = -1; dy < 2; ++dy)
-                for (int dx = -1; dx < 2; ++dx) {
-                    int tx = gx + dx, ty = gy + dy;
-                    const int tz = gz + dz;
-                    const int code = find_dist_mod(p, tx, ty, tz);
-                    if (code < 0) continue;
-                    const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
-                                          (p.num[0] * p.num[1]) * (uint32_t)tz;
-                    const uint32_t start = a.cell_start[cell];
-                    if (start == 0xffffffffu) continue;
-                    const uint32_t end = a.cell_end[cell];
-                    const d3 dm = shift_of(p, (uint32_t)code);
-                    for (uint32_t ib = start; ib < end; ++ib) {
-                        if (ib == ia) continue;
-                        const double4 xb = a.xs[ib];
-                        const double rb = xb.w;
-                        d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
-                        xab += dm;
-                        const double dd = dot(xab, xab);
-                        const double S = ra + rb;
-                        // Conservative reject: len32 > S whenever
-                        // dd >= S^2 (1 + 2^-16) (the two FP32 roundings move
-                        // the root by < 2^-23 relative).
-                        if (dd >= S * S * 1.0000152587890625) continue;
-                        const double len = sqrt32(dd);
-                        const double delta = len - S;
-                        if (delta < 0.0) {
-                            const uint32_t ib_orig = a.perm[ib];
-                            int cpos = nc;
-                            for (int i = 0; i < NC; ++i) {
-                                const uint32_t cidx = row[i] & CW_IDX_MASK;
-                                if (cidx == np) {
-                                    cpos = i;
-                                } else if (cidx == ib_orig) {
-                                    cpos = i;
-                                    break;
+        const bool rebuild = !p.vl_enabled || a.loop->vl_rebuild != 0u;
+        const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
+        if (rebuild || vcnt == VL_OVERFLOW) {
+            const bool build = p.vl_enabled && rebuild;
+            uint32_t nl = 0;
+            uint2* vrow = a.vlist + (size_t)ia * NV;
+            for (int dz = -1; dz < 2; ++dz)
+                for (int dy = -1; dy < 2; ++dy)
+                    for (int dx = -1; dx < 2; ++dx) {
+                        int tx = gx + dx, ty = gy + dy;
+                        const int tz = gz + dz;
+                        const int code = find_dist_mod(p, tx, ty, tz);
+                        if (code < 0) continue;
+                        const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                                              (p.num[0] * p.num[1]) * (uint32_t)tz;
+                        const uint32_t start = a.cell_start[cell];
+                        if (start == 0xffffffffu) continue;
+                        const uint32_t end = a.cell_end[cell];
+                        const d3 dm = shift_of(p, (uint32_t)code);
+                        const uint32_t celloff = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+                        bool cell_done = false;
+                        for (uint32_t ib = start; ib < end; ++ib) {
+                            if (ib == ia) continue;
+                            const double4 xb = a.xs[ib];
+                            const double rb = xb.w;
+                            d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                            xab += dm;
+                            const double dd = dot(xab, xab);
+                            const double S = ra + rb;
+                            if (build) {
+                                const double Sv = S + p.vl_skin;
+                                if (dd < Sv * Sv) {
+                                    if (nl < (uint32_t)NV)
+                                        vrow[nl] = make_uint2(ib | (celloff << VL_CELL_SHIFT),
+                                                              pack_word(a.perm[ib], (uint32_t)code));
+                                    ++nl;
                                 }
                             }
-                            // cpos == NC would write into the next particle's
-                            // row (src/contactsearch.cuh:286); suppressed.
-                            if (cpos < NC) row[cpos] = pack_word(ib_orig, (uint32_t)code);
-                            ++nc;
-                            if (nc > NC) {
-                                atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_CONTACT_OVERFLOW);
-                                break;
-                            }
+                            if (cell_done) continue;
+                            // Conservative reject: len32 > S whenever
+                            // dd >= S^2 (1 + 2^-16) (the two FP32 roundings
+                            // move the root by < 2^-23 relative).
+                            if (dd >= S * S * 1.0000152587890625) continue;
+                            const double len = sqrt32(dd);
+                            if (len - S < 0.0)
+                                cell_done = register_contact(row, np, a.perm[ib], (uint32_t)code,
+                                                             nc, a.loop);
                         }
                     }
+            if (build) {
+                a.vcount[ia] = nl <= (uint32_t)NV ? nl : VL_OVERFLOW;
+                a.x_build[ia_orig] = xa;
+            }
+        } else {
+            const uint2* vrow = a.vlist + (size_t)ia * NV;
+            uint32_t last_cell = 0xffffffffu;
+            bool cell_done = false;
+            for (uint32_t k = 0; k < vcnt; ++k) {
+                const uint2 e = vrow[k];
+                const uint32_t celloff = e.x >> VL_CELL_SHIFT;
+                if (celloff != last_cell) {
+                    last_cell = celloff;
+                    cell_done = false;
                 }
+                if (cell_done) continue;
+                const double4 xb = a.xs[e.x & VL_IB_MASK];
+                const double rb = xb.w;
+                const uint32_t code = word_code(e.y);
+                d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                xab += shift_of(p, code);
+                const double dd = dot(xab, xab);
+                const double S = ra + rb;
+                if (dd >= S * S * 1.0000152587890625) continue;
+                const double len = sqrt32(dd);
+                if (len - S < 0.0)
+                    cell_done = register_contact(row, np, e.y & CW_IDX_MASK, code, nc, a.loop);
+            }
+        }
         // interact: the contact loop, src/contactsearch.cuh:484-559.
         const bool zero_empty = a.loop->zero_empty != 0u;
         for (int i = 0; i < NC; ++i) {
@@ -711,6 +787,24 @@ __device__ __forceinline__ bool last_block(uint32_t* counter, uint32_t nblocks) 
     return is_last != 0u;
 }
 
+// The serial left-to-right sum of n partials (integrateWalls'
+// `for i < blocksPerGrid: sum += partial[i]`), with the partials staged
+// through LDS by the whole workgroup.  The result is valid in thread 0.
+__device__ __forceinline__ double serial_sum(const double* __restrict__ part, uint32_t n,
+                                             double* stage) {
+    constexpr uint32_t CH = 2048;
+    double sum = 0.0;
+    for (uint32_t base = 0; base < n; base += CH) {
+        const uint32_t cnt = (n - base) < CH ? (n - base) : CH;
+        for (uint32_t j = threadIdx.x; j < cnt; j += BLOCK) stage[j] = part[base + j];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (uint32_t j = 0; j < cnt; ++j) sum += stage[j];
+        __syncthreads();
+    }
+    return sum;
+}
+
 // findShearStressOnFixedMovingParticles + summation + the serial sum of
 // integrate (src/integration.cuh:457-484, 309-341, 64-68).
 __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const DevArrays a) {
@@ -730,9 +824,10 @@ __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const
     }
     const double s = block_tree_sum(in ? v : 0.0, cache);
     if (threadIdx.x == 0) a.tau_partial[blockIdx.x] = s;
-    if (last_block(&a.loop->tau_blocks_done, gridDim.x) && threadIdx.x == 0) {
-        double tau = 0.0;
-        for (uint32_t i = 0; i < p.nblocks; ++i) tau += a.tau_partial[i];
+    if (!last_block(&a.loop->tau_blocks_done, gridDim.x)) return;
+    __shared__ double stage[2048];
+    const double tau = serial_sum(a.tau_partial, p.nblocks, stage);
+    if (threadIdx.x == 0) {
         a.tau_eff[0] = tau;
         a.loop->tau_blocks_done = 0u;
     }
@@ -855,6 +950,9 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const De
     a.vel[orig] = vel_new;
     a.angpos[orig] = angpos_new;
     a.x[orig] = x_new;
+    // Keep the sorted copy current: valid for the next step whenever the
+    // order does not change (otherwise the next sort rebuilds it).
+    a.xs[a.rank[orig]] = x_new;
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +969,8 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
         if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
     }
     if (!last_block(&a.loop->wall_blocks_done, gridDim.x)) return;
+    __shared__ double stage[2048];
+    const double fsum = walls ? serial_sum(a.partial, p.nblocks, stage) : 0.0;
     if (threadIdx.x != 0) return;
     DevLoop* lp = a.loop;
     const double t_now = lp->t;
@@ -883,8 +983,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
         double4 w_mvfd = a.w_mvfd[w];
         double acc0;
         if (lp->iter == 0) acc0 = 0.0; else acc0 = a.w_acc[w];
-        w_mvfd.z = 0.0;
-        for (uint32_t i = 0; i < p.nblocks; ++i) w_mvfd.z += a.partial[i];
+        w_mvfd.z = fsum;  // 0.0 + partial[0] + partial[1] + ... in order
         const double sigma_0 = w_mvfd.w + p.sigma0_A * sin(2.0 * PI * p.sigma0_f * t_now);
         const double N = -sigma_0 * p.L[0] * p.L[1];
         double acc = (w_mvfd.z + N) / w_mvfd.x;
@@ -903,6 +1002,8 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
     lp->iter = lp->iter + 1;
     lp->zero_empty = 0u;
     lp->cells_changed = 0u;
+    if (lp->vl_rebuild) lp->vl_builds += 1u;
+    lp->vl_rebuild = 0u;
     lp->key_or = 0u;
     lp->wall_blocks_done = 0u;
 }
@@ -1015,7 +1116,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevParams p, const 
 
 hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(n)), dim3(BLOCK), 0, s, dst, n, v);
+    hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(n)), dim3(BLOCK), 0, s, nullptr, dst, n, v);
     return hipGetLastError();
 }
 
@@ -1042,17 +1143,20 @@ hipError_t launch_sort(const DevParams& p, const DevArrays& a, int passes, hipSt
         }
         uint32_t* kout = to_final ? a.skey : a.key_tmp;
         uint32_t* vout = to_final ? a.perm : a.val_tmp;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, kin, n, shift, a.rhist, nt);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.rhist, 256u * nt);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, kin, vin, kout, vout, n,
-                           shift, a.rhist, nt);
+        const uint32_t* gate = &a.loop->cells_changed;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, gate, kin, n, shift, a.rhist,
+                           nt);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, gate, a.rhist, 256u * nt);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, gate, kin, vin, kout, vout,
+                           n, shift, a.rhist, nt);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits, hipStream_t s) {
-    hipError_t e = launch_fill_u32(a.cell_start, p.ncells, 0xffffffffu, s);
-    if (e != hipSuccess) return e;
+    if (p.ncells)
+        hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(p.ncells)), dim3(BLOCK), 0, s,
+                           &a.loop->cells_changed, a.cell_start, p.ncells, 0xffffffffu);
     hipLaunchKernelGGL(k_cells, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a, sorted_bits);
     return hipGetLastError();
 }

@@ -24,6 +24,14 @@ constexpr uint32_t CW_SHIFT_X = 28;   // 0: none, 1: +L[0], 2: -L[0]
 constexpr uint32_t CW_SHIFT_Y = 30;   // 0: none, 1: +L[1], 2: -L[1]
 constexpr uint32_t MAX_NP = 1u << 27; // the reference's own u32 row index limit
 
+// Candidate (Verlet) lists: per particle, in sorted order, the neighbours
+// within r_a + r_b + skin found by the 27-cell scan, kept in scan order.
+// An entry is {sorted index | cell offset << 27, packed contact word}.
+constexpr int NV = 32;                   // entries per particle
+constexpr uint32_t VL_OVERFLOW = 0xffffffffu;
+constexpr uint32_t VL_IB_MASK = (1u << 27) - 1u;
+constexpr uint32_t VL_CELL_SHIFT = 27;
+
 struct DevParams {
     // grid (src/datatypes.h:45-51)
     double origo[3];
@@ -51,6 +59,11 @@ struct DevParams {
     double sigma0_A, sigma0_f;
     double tau_x;
     int32_t wmode[MAXWALLS];
+    // candidate lists
+    double vl_skin;            // list radius margin (m); 0 disables the lists
+    double vl_trigger2;        // squared displacement that forces a rebuild
+    int32_t vl_enabled;
+    int32_t pad1;
 };
 
 // Loop state kept on the device so that captured step sequences can be
@@ -68,6 +81,10 @@ struct DevLoop {
     uint32_t error_bits;        // sticky device-side flags (DevFlag)
     uint32_t zero_empty;        // zero delta_t of empty slots this step
     uint32_t bond_retargeted;   // some broken bond now names particle nb0
+    uint32_t vl_rebuild;        // candidate lists must be rebuilt this step
+    uint32_t vl_builds;         // number of list rebuilds so far (diagnostic)
+    uint32_t sorts;             // number of re-sorts so far (diagnostic)
+    uint32_t pad0;
 };
 
 enum DevFlag : uint32_t {

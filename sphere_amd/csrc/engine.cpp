@@ -309,6 +309,15 @@ int check_values(sph_ctx* c) {
     return SPH_OK;
 }
 
+void set_skin(sph_ctx* c, double skin) {
+    c->P.vl_skin = skin > 0.0 ? skin : 0.0;
+    c->P.vl_enabled = skin > 0.0 ? 1 : 0;
+    // Rebuild once any particle has moved 0.45 skin (< skin / 2) since the
+    // lists were built.
+    const double trig = 0.45 * c->P.vl_skin;
+    c->P.vl_trigger2 = trig * trig;
+}
+
 int bit_length(uint64_t v) {
     int b = 0;
     while (v) { ++b; v >>= 1; }
@@ -351,6 +360,13 @@ void fill_params(sph_ctx* c) {
     if (c->sorted_bits > 32) c->sorted_bits = 32;
     c->passes = (c->sorted_bits + 7) / 8;
     c->shear_stress = h.nw > 0 && h.wmode[0] == 3;
+    // Candidate lists: skin of 0.1 r_min by default.  Model 1 with a
+    // capillary debonding distance keeps the full scan (its interaction
+    // range extends beyond overlap).
+    double rmin = 0.0;
+    for (uint32_t i = 0; i < h.np; ++i)
+        if (i == 0 || h.x[4 * i + 3] < rmin) rmin = h.x[4 * i + 3];
+    set_skin(c, (h.contactmodel == 1 && h.db > 0.0) ? 0.0 : 0.1 * rmin);
 }
 
 template <class T> int dev_alloc(sph_ctx* c, T** p, size_t count) {
@@ -382,6 +398,10 @@ int allocate(sph_ctx* c) {
     ALLOC(A.rhist, 256 * (size_t)n_tiles((unsigned)np));
     ALLOC(A.cell_start, ncells); ALLOC(A.cell_end, ncells);
     ALLOC(A.xs, np);
+    ALLOC(A.rank, np);
+    ALLOC(A.vlist, np * NV);
+    ALLOC(A.vcount, np);
+    ALLOC(A.x_build, np);
     ALLOC(A.crow, np * NC);
     ALLOC(A.delta_t, np * NC);
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
@@ -433,6 +453,9 @@ int upload_state(sph_ctx* c) {
         std::vector<uint32_t> ident(np);
         for (size_t i = 0; i < np; ++i) ident[i] = (uint32_t)i;
         H2D(A.perm, ident.data(), np * 4);
+        H2D(A.rank, ident.data(), np * 4);
+        HIPCHK(c, hipMemsetAsync(A.vcount, 0, np * 4, s), "memset");
+        H2D(A.x_build, h.x.data(), np * 32);
         HIPCHK(c, hipMemsetAsync(A.skey, 0, np * 4, s), "memset");
         HIPCHK(c, hipMemsetAsync(A.xs, 0, np * 32, s), "memset");
     }
@@ -479,6 +502,7 @@ int upload_state(sph_ctx* c) {
     L.grid_Lz = h.L[2];
     L.velocity_state = 1;
     L.zero_empty = 1u;
+    L.vl_rebuild = 1u;
     H2D(A.loop, &L, sizeof L);
 #undef H2D
     HIPCHK(c, hipStreamSynchronize(s), "upload");
@@ -1011,7 +1035,14 @@ int sph_set_field(sph_ctx* c, int f, const void* src, size_t bytes) {
     const size_t np = c->h.np;
     DevArrays& A = c->A;
     switch (f) {
-        case SPH_F_X: return h2d(c, A.x, src, bytes);
+        case SPH_F_X: {
+            // New positions: the next step re-sorts and rebuilds the lists.
+            int rc = h2d(c, A.x, src, bytes);
+            if (rc) return rc;
+            HIPCHK(c, hipMemsetAsync(A.key_prev, 0xff, np * 4, c->stream), "memset");
+            const uint32_t one = 1u;
+            return h2d(c, &A.loop->vl_rebuild, &one, 4);
+        }
         case SPH_F_VEL: return h2d(c, A.vel, src, bytes);
         case SPH_F_ANGVEL: return set_vec3(c, A.angvel, np, src);
         case SPH_F_ANGPOS: return set_vec3(c, A.angpos, np, src);
@@ -1110,6 +1141,15 @@ int sph_get_scalar(sph_ctx* c, int which, double* v) {
         case SPH_S_NW: *v = c->h.nw; return SPH_OK;
         case SPH_S_NB0: *v = c->h.nb0; return SPH_OK;
         case SPH_S_NCELLS: *v = c->P.ncells; return SPH_OK;
+        case SPH_S_VL_SKIN: *v = c->P.vl_skin; return SPH_OK;
+        case SPH_S_SORTS:
+        case SPH_S_VL_BUILDS: {
+            DevLoop L;
+            int rc = read_loop(c, &L);
+            if (rc) return rc;
+            *v = which == SPH_S_SORTS ? (double)L.sorts : (double)L.vl_builds;
+            return SPH_OK;
+        }
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1130,6 +1170,11 @@ int sph_set_scalar(sph_ctx* c, int which, double v) {
             return h2d(c, &c->A.loop->iter, &it, 8);
         }
         case SPH_S_STEP_COUNT: c->h.step_count = (uint32_t)v; return SPH_OK;
+        case SPH_S_VL_SKIN: {
+            set_skin(c, v);
+            const uint32_t one = 1u;
+            return h2d(c, &c->A.loop->vl_rebuild, &one, 4);
+        }
         default: return fail(c, SPH_E_ARG, "scalar %d is read-only", which);
     }
 }
