@@ -7,8 +7,9 @@ container; the GPU tests only read the committed files).
   (output/init-1e4.output00199.bin, v2.11) with centres below z = 0.1 m,
   upgraded to v2.15 by sphere_amd's reader and written by its writer.
 * shear1k.bin: a 1000-particle direct-shear cell (lognormal radii, median
-  0.01 m, sigma 0.2) grown, settled and consolidated at 10 kPa by the CPU
-  oracle (sphere_amd.scenes.shear_cell), seed 1105.
+  0.01 m, sigma 0.2) settled under gravity and consolidated at 10 kPa by the
+  CPU oracle (sphere_amd.scenes.settled_shear_cell), seed 1105, then set up
+  for shear (fixed bottom layer, top layer at 1e-3 m/s).
 
 fixtures.json records the SHA-256 of each file.
 """
@@ -54,8 +55,7 @@ def main():
     out["init-1e4-slab.bin"] = sha(dst)
 
     oracle.set_threads(os.cpu_count() or 1)
-    s = scenes.shear_cell(1000, 1105, open_stepper=oracle.Oracle, stages=40,
-                          steps_per_stage=150, settle_steps=2000, consolidate_steps=1500)
+    s = scenes.settled_shear_cell(1000, 1105, open_stepper=oracle.Oracle)
     s.sid = "shear1k"
     dst = os.path.join(HERE, "shear1k.bin")
     with open(dst, "wb") as fh:
