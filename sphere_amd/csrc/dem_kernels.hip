@@ -224,7 +224,7 @@ struct Acc {
 };
 
 // contactLinear_wall, src/contactmodels.cuh:10-117.
-__device__ __noinline__ double contact_linear_wall(const DevParams& p, Acc& A, double ra,
+__device__ __forceinline__ double contact_linear_wall(const DevParams& p, Acc& A, double ra,
                                                    const double4 vt, const double4 at, d3 n,
                                                    double delta, double wvel) {
     const d3 vel_linear = xyz(vt);
@@ -481,6 +481,9 @@ __device__ __forceinline__ uint32_t word_code(uint32_t w) {
 // row lives in LDS for the duration (33-word stride: no bank conflicts).
 
 constexpr int ROW_STRIDE = NC + 1;
+#ifndef CONTACT_MIN_BLOCKS
+#define CONTACT_MIN_BLOCKS 4
+#endif
 
 // The contact-slot policy of findContactsInCell for one overlapping
 // partner (src/contactsearch.cuh:265-299): the last empty slot, else the
@@ -509,7 +512,11 @@ __device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t np, uin
     return false;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevArrays a) {
+// CM is the contact model (1, 2 or 3); one instantiation per model keeps
+// only that model's registers live.
+template <int CM>
+__global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const DevParams p,
+                                                                      const DevArrays a) {
     __shared__ uint32_t srow[BLOCK * ROW_STRIDE];
     const uint32_t ia = blockIdx.x * BLOCK + threadIdx.x;
     if (ia >= p.np) return;
@@ -528,12 +535,12 @@ __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevA
     A.es_dot = 0.0;
     A.ev_dot = 0.0;
     A.p = 0.0;
-    const uint32_t cm = p.contactmodel;
+    constexpr uint32_t cm = (uint32_t)CM;
     // src/contactsearch.cuh:352-354
     const int gx = (int)floor((xa.x - p.origo[0]) / (p.L[0] / p.num[0]));
     const int gy = (int)floor((xa.y - p.origo[1]) / (p.L[1] / p.num[1]));
     const int gz = (int)floor((xa.z - p.origo[2]) / (Lz / p.num[2]));
-    if (cm == 2 || cm == 3) {
+    if constexpr (CM == 2 || CM == 3) {
         const uint4* grow = reinterpret_cast<const uint4*>(a.crow + (size_t)ia_orig * NC);
 #pragma unroll
         for (int q = 0; q < NC / 4; ++q) {
@@ -641,7 +648,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevA
                     const d3 angvel_b = xyz(a.angvel[ib_orig]);
                     const double4 dt0 = a.delta_t[mp];
                     d3 nd;
-                    if (cm == 2)
+                    if constexpr (CM == 2)
                         nd = contact_linear(p, A, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
                                             len, dn, dt0);
                     else
@@ -663,7 +670,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevA
 #pragma unroll
         for (int q = 0; q < NC / 4; ++q)
             wrow[q] = make_uint4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
-    } else if (cm == 1) {
+    } else if constexpr (CM == 1) {
         // findAndProcessContactsInCell, src/contactsearch.cuh:91-186.
         for (int dz = -1; dz < 2; ++dz)
             for (int dy = -1; dy < 2; ++dy)
@@ -698,50 +705,52 @@ __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevA
                     }
                 }
     }
-    // Walls, src/contactsearch.cuh:602-701: top, bottom, then the side walls.
-    double4 wn[5];
-    double wv[5];
-    wn[0] = make_double4(0.0, 0.0, -1.0, Lz);
-    wn[1] = make_double4(-1.0, 0.0, 0.0, p.L[0]);
-    wn[2] = make_double4(1.0, 0.0, 0.0, 0.0);
-    wn[3] = make_double4(0.0, -1.0, 0.0, p.L[1]);
-    wn[4] = make_double4(0.0, 1.0, 0.0, 0.0);
-#pragma unroll
-    for (int w = 0; w < 5; ++w) wv[w] = 0.0;
-    for (uint32_t w = 0; w < p.nw && w < 5; ++w) {
-        wn[w] = a.w_nx[w];
-        wv[w] = a.w_mvfd[w].y;
+    // Walls, src/contactsearch.cuh:602-701: top, bottom, then the side
+    // walls (all four when periodic == 0, the y walls when periodic == 2).
+    // Each wall is spelled out so that nothing is indexed dynamically.
+    double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
+    {
+        double4 wn = make_double4(0.0, 0.0, -1.0, Lz);
+        double wv = 0.0;
+        if (p.nw >= 1) { wn = a.w_nx[0]; wv = a.w_mvfd[0].y; }
+        const double dw = wn.w - (xa.z + ra);
+        if (dw < 0.0) wf0 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
     }
-    double wf[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    // (wall index, delta) in the reference's order; index 5 = bottom wall.
-    for (int k = 0; k < 6; ++k) {
-        int w;
-        double dw;
-        d3 n;
-        double vel_w;
-        if (k == 0) {
-            w = 0; dw = wn[0].w - (xa.z + ra); n = xyz(wn[0]); vel_w = wv[0];
-        } else if (k == 1) {
-            w = 5; dw = xa.z - ra - p.origo[2]; n = mk3(0.0, 0.0, 1.0); vel_w = 0.0;
-        } else {
-            if (p.periodic == 0) {
-                w = k - 1;
-            } else if (p.periodic == 2) {
-                if (k < 4) continue;
-                w = k - 1;
-            } else {
-                break;
-            }
-            if (w == 1) dw = wn[1].w - (xa.x + ra);
-            else if (w == 2) dw = xa.x - ra - wn[2].w;
-            else if (w == 3) dw = wn[3].w - (xa.y + ra);
-            else dw = xa.y - ra - wn[4].w;
-            n = xyz(wn[w]);
-            vel_w = wv[w];
+    {
+        const double dw = xa.z - ra - p.origo[2];
+        if (dw < 0.0)
+            (void)contact_linear_wall(p, A, ra, vel_a, av_a, mk3(0.0, 0.0, 1.0), dw, 0.0);
+    }
+    if (p.periodic == 0) {
+        {
+            double4 wn = make_double4(-1.0, 0.0, 0.0, p.L[0]);
+            double wv = 0.0;
+            if (p.nw >= 2) { wn = a.w_nx[1]; wv = a.w_mvfd[1].y; }
+            const double dw = wn.w - (xa.x + ra);
+            if (dw < 0.0) wf1 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
-        if (dw < 0.0) {
-            const double f = contact_linear_wall(p, A, ra, vel_a, av_a, n, dw, vel_w);
-            if (w < 5) wf[w] = f;
+        {
+            double4 wn = make_double4(1.0, 0.0, 0.0, 0.0);
+            double wv = 0.0;
+            if (p.nw >= 3) { wn = a.w_nx[2]; wv = a.w_mvfd[2].y; }
+            const double dw = xa.x - ra - wn.w;
+            if (dw < 0.0) wf2 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+        }
+    }
+    if (p.periodic == 0 || p.periodic == 2) {
+        {
+            double4 wn = make_double4(0.0, -1.0, 0.0, p.L[1]);
+            double wv = 0.0;
+            if (p.nw >= 4) { wn = a.w_nx[3]; wv = a.w_mvfd[3].y; }
+            const double dw = wn.w - (xa.y + ra);
+            if (dw < 0.0) wf3 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+        }
+        {
+            double4 wn = make_double4(0.0, 1.0, 0.0, 0.0);
+            double wv = 0.0;
+            if (p.nw >= 5) { wn = a.w_nx[4]; wv = a.w_mvfd[4].y; }
+            const double dw = xa.y - ra - wn.w;
+            if (dw < 0.0) wf4 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
     }
     // Outputs at the original index, src/contactsearch.cuh:706-724.
@@ -753,7 +762,11 @@ __global__ __launch_bounds__(BLOCK) void k_contact(const DevParams p, const DevA
     a.es[o] += A.es_dot * p.dt;
     a.ev[o] += A.ev_dot * p.dt;
     a.p[o] = A.p;
-    for (uint32_t w = 0; w < p.nw && w < 5; ++w) a.force_pp[(size_t)w * np + o] = wf[w];
+    if (p.nw > 0) a.force_pp[o] = wf0;
+    if (p.nw > 1) a.force_pp[(size_t)np + o] = wf1;
+    if (p.nw > 2) a.force_pp[(size_t)np * 2 + o] = wf2;
+    if (p.nw > 3) a.force_pp[(size_t)np * 3 + o] = wf3;
+    if (p.nw > 4) a.force_pp[(size_t)np * 4 + o] = wf4;
 }
 
 // ---------------------------------------------------------------------------
@@ -1162,7 +1175,15 @@ hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits,
 }
 
 hipError_t launch_contact(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_contact, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+    const dim3 g(n_blocks(p.np)), b(BLOCK);
+    if (p.contactmodel == 1)
+        hipLaunchKernelGGL(k_contact<1>, g, b, 0, s, p, a);
+    else if (p.contactmodel == 3)
+        hipLaunchKernelGGL(k_contact<3>, g, b, 0, s, p, a);
+    else if (p.contactmodel == 2)
+        hipLaunchKernelGGL(k_contact<2>, g, b, 0, s, p, a);
+    else
+        hipLaunchKernelGGL(k_contact<0>, g, b, 0, s, p, a);
     return hipGetLastError();
 }
 

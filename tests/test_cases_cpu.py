@@ -31,3 +31,34 @@ def test_fixture_loads_and_is_contact_rich(name):
     o = oracle.Oracle(s.tobytes())
     o.step(1)
     assert (o.get("contacts") != n).sum() > n
+
+
+def test_bonded_cases_build():
+    from .test_gpu_bonds import _bonded
+    import numpy
+    for frac, seed, sb in ((0.2, 3, numpy.inf), (0.3, 4, 300.0)):
+        s = _bonded(frac=frac, seed=seed, sigma_b=sb)
+        assert int(s.nb0[0]) > 10
+        o = oracle.Oracle(s.tobytes())
+        o.step(2)
+
+
+def test_bond_breakage_case_breaks_on_oracle():
+    from .test_gpu_bonds import _bonded
+    s = _bonded(frac=0.3, seed=4, sigma_b=300.0)
+    o = oracle.Oracle(s.tobytes())
+    o.step(120)
+    assert (o.get("bonds")[:, 0] == int(s.nb0[0])).any()
+
+
+def test_wmode3_adaptive_rate_state_cases_build():
+    s = cases.fixture("shear1k.bin")
+    s.shear(shear_stress=2.0e3)
+    assert int(s.wmode[0]) == 3
+    oracle.Oracle(s.tobytes()).step(2)
+    s = cases.fixture("shear1k.bin")
+    s.adaptiveGrid()
+    oracle.Oracle(s.tobytes()).step(2)
+    s = cases.fixture("shear1k.bin")
+    top = s.fixvel > 0
+    assert top.any()
