@@ -106,9 +106,28 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--contact-lanes", type=int, default=0,
+                    help="work-items per particle in the contact kernel (1, 4, 8, 16)")
+    ap.add_argument("--per-stage-sort", action="store_true",
+                    help="re-sort with one launch per stage instead of one cooperative launch")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (5 times): the "
+                         "reference kernel for scaling HBM byte counters")
     ap.add_argument("--save-image")
     ap.add_argument("--load-image")
     args = ap.parse_args()
+
+    if args.calibrate_copy:
+        import torch
+        nbytes = args.calibrate_copy << 20
+        src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
+                          "launches": 5}), flush=True)
+        return 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -121,6 +140,10 @@ def main():
 
     img = build_workload(args)
     ctx = engine.Context(img)
+    if args.per_stage_sort:
+        ctx.set_scalar("coop_sort", 0)
+    if args.contact_lanes:
+        ctx.set_scalar("contact_lanes", args.contact_lanes)
     n = int(ctx.scalar("np"))
     nw = int(ctx.scalar("nw"))
     dt = ctx.scalar("dt")
@@ -128,6 +151,7 @@ def main():
     ctx.step(args.warmup, sync=True)
     z = scenes.contact_number(ctx, n)
 
+    sorts0, builds0 = ctx.scalar("sorts"), ctx.scalar("vl_builds")
     # Timed region: K steps, synchronised on both sides.
     ctx.sync()
     t0 = time.perf_counter()
@@ -137,8 +161,12 @@ def main():
     elapsed = t1 - t0
     value = n * args.steps / elapsed
     z_end = scenes.contact_number(ctx, n)
-    log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f)"
-        % (args.steps, elapsed, value, z, z_end))
+    sorts = int(ctx.scalar("sorts") - sorts0)
+    builds = int(ctx.scalar("vl_builds") - builds0)
+    log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f, "
+        "%d re-sorts, %d list rebuilds, cooperative sort %d, contact lanes %d)"
+        % (args.steps, elapsed, value, z, z_end, sorts, builds, int(ctx.scalar("coop_sort")),
+           int(ctx.scalar("contact_lanes"))))
 
     # Kernel timing pass with HIP events on the context stream.
     ctx.profile(True)
@@ -206,6 +234,7 @@ def main():
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
         "kernel_us": {k: v[0] for k, v in rep.items()},
+        "timed_events": {"re_sorts": sorts, "list_rebuilds": builds},
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -102,7 +102,13 @@ enum sph_scalar {
     SPH_S_NCELLS = 9,
     SPH_S_VL_SKIN = 10,    /* candidate-list skin (m); 0 = scan 27 cells every step */
     SPH_S_SORTS = 11,      /* re-sorts performed so far (diagnostic, read-only)      */
-    SPH_S_VL_BUILDS = 12   /* candidate-list rebuilds so far (diagnostic, read-only) */
+    SPH_S_VL_BUILDS = 12,  /* candidate-list rebuilds so far (diagnostic, read-only) */
+    SPH_S_COOP_SORT = 13,  /* 1: re-sort in one cooperative launch (default where the
+                              device supports it); 0: one launch per sort stage.
+                              Results are identical either way. */
+    SPH_S_CONTACT_LANES = 14 /* work-items per particle in the contact kernel for
+                                contact models 2 and 3: 1, 4, 8 (default) or 16.
+                                Results are identical for every value. */
 };
 
 /* Open flags. */

@@ -5,6 +5,7 @@
 // with -ffp-contract=off (no fused multiply-add), as the reference is with
 // nvcc --fmad=false (src/CMakeLists.txt:25).
 
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -14,6 +15,8 @@
 #include "dem_types.hpp"
 
 namespace sphx {
+
+namespace cg = cooperative_groups;
 
 namespace {
 
@@ -42,88 +45,104 @@ __device__ __forceinline__ uint32_t calc_cell_id(const DevParams& p, double Lz, 
     return (iz * p.num[1]) * p.num[0] + iy * p.num[0] + ix;
 }
 
+// Per-step flags live in slot iter & 1 of the loop state.
+__device__ __forceinline__ uint32_t step_slot(const DevLoop* lp) {
+    return (uint32_t)(lp->iter & 1);
+}
+__device__ __forceinline__ bool sort_needed(const DevLoop* lp) {
+    return lp->changed[step_slot(lp)] != 0u;
+}
+
 // ---------------------------------------------------------------------------
-// Hash
+// Hash (calcParticleCellID, src/sorting.cuh:25-51), plus the two tests that
+// decide whether the step may reuse the previous order and candidate lists.
+
+struct HashFlags {
+    uint32_t changed = 0, moved = 0, nonfinite = 0;
+};
+
+__device__ __forceinline__ void hash_particle(const DevParams& p, const DevArrays& a,
+                                              uint32_t i, double Lz, const double4 x,
+                                              HashFlags& f) {
+    const uint32_t key = calc_cell_id(p, Lz, x.x, x.y, x.z);
+    if (x.x != x.x || x.y != x.y || x.z != x.z) f.nonfinite = 1;
+    f.changed = key != a.key_prev[i];
+    a.key[i] = key;
+    if (p.vl_enabled) {
+        // Candidate lists stay valid while every particle is within
+        // skin/2 of where the lists were built.
+        const double4 xb = a.x_build[i];
+        const double dx = x.x - xb.x, dy = x.y - xb.y, dz = x.z - xb.z;
+        f.moved = !(dx * dx + dy * dy + dz * dz <= p.vl_trigger2);
+    }
+}
+
+// Called by every work-item of the wavefront: at most one atomic per
+// wavefront, and only when something happened.
+__device__ __forceinline__ void publish_hash(DevLoop* lp, uint32_t slot, const HashFlags& f) {
+    const unsigned long long anych = __ballot(f.changed);
+    const unsigned long long anymv = __ballot(f.moved);
+    const unsigned long long anynf = __ballot(f.nonfinite);
+    if ((threadIdx.x & 63) == 0) {
+        if (anych) atomicOr(&lp->changed[slot], 1u);
+        if (anych | anymv) atomicOr(&lp->rebuild[slot], 1u);
+        if (anynf) atomicOr(&lp->error_bits, (uint32_t)DFLAG_NONFINITE);
+    }
+}
 
 __global__ __launch_bounds__(BLOCK) void k_hash(const DevParams p, const DevArrays a) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    uint32_t changed = 0, nonfinite = 0, moved = 0;
-    if (i < p.np) {
-        const double Lz = grid_Lz(p, a);
-        const double4 x = a.x[i];
-        const uint32_t key = calc_cell_id(p, Lz, x.x, x.y, x.z);
-        if (x.x != x.x || x.y != x.y || x.z != x.z) nonfinite = 1;
-        changed = key != a.key_prev[i];
-        a.key[i] = key;
-        if (p.vl_enabled) {
-            // Candidate lists stay valid while every particle is within
-            // skin/2 of where the lists were built.
-            const double4 xb = a.x_build[i];
-            const double dx = x.x - xb.x, dy = x.y - xb.y, dz = x.z - xb.z;
-            moved = !(dx * dx + dy * dy + dz * dz <= p.vl_trigger2);
-        }
-    }
-    // At most one atomic per wavefront, and only when something happened.
-    const unsigned long long anych = __ballot(changed);
-    const unsigned long long anymv = __ballot(moved);
-    const unsigned long long anynf = __ballot(nonfinite);
-    if ((threadIdx.x & 63) == 0) {
-        if (anych) atomicOr(&a.loop->cells_changed, 1u);
-        if (anych | anymv) atomicOr(&a.loop->vl_rebuild, 1u);
-        if (anynf) atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_NONFINITE);
-    }
+    HashFlags f;
+    if (i < p.np) hash_particle(p, a, i, grid_Lz(p, a), a.x[i], f);
+    publish_hash(a.loop, step_slot(a.loop), f);
 }
 
 // ---------------------------------------------------------------------------
 // Stable LSD radix sort, 8-bit digits.  Tile t covers keys
 // [t*TILE, (t+1)*TILE) in rounds of 256: element order inside a tile is
 // (round, thread), which is the global index order, so ranking each round
-// after all earlier rounds keeps the sort stable.
+// after all earlier rounds keeps the sort stable.  The stages are written
+// per tile so that the same code runs as one launch per stage or inside the
+// single cooperative launch below.
 
-// Every sort-stage kernel returns at once when *gate == 0: no particle
-// changed cell, so the stable order of (cell, index) is unchanged.
-__global__ __launch_bounds__(BLOCK) void k_radix_hist(const uint32_t* __restrict__ gate,
-                                                      const uint32_t* __restrict__ kin,
-                                                      uint32_t n, int shift,
-                                                      uint32_t* __restrict__ hist,
-                                                      uint32_t ntiles) {
-    if (*gate == 0u) return;
+__device__ __forceinline__ void radix_hist_tile(const uint32_t* __restrict__ kin, uint32_t n,
+                                                int shift, uint32_t* __restrict__ hist,
+                                                uint32_t ntiles, uint32_t tile) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = blockIdx.x * RADIX_TILE;
+    const uint32_t base = tile * RADIX_TILE;
 #pragma unroll
     for (int j = 0; j < RADIX_ITEMS; ++j) {
         const uint32_t e = base + j * BLOCK + threadIdx.x;
         if (e < n) atomicAdd(&h[(kin[e] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+    hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (256 x ntiles) in place.
-__global__ __launch_bounds__(1024) void k_radix_scan(const uint32_t* __restrict__ gate,
-                                                     uint32_t* __restrict__ hist,
-                                                     uint32_t total) {
-    if (*gate == 0u) return;
-    __shared__ uint32_t s[1024];
+// Exclusive scan in place of the digit-major histogram (256 x ntiles) by
+// one workgroup of T work-items.
+template <int T>
+__device__ __forceinline__ void radix_scan_block(uint32_t* __restrict__ hist, uint32_t total) {
+    __shared__ uint32_t sh[T];
     const uint32_t t = threadIdx.x;
-    const uint32_t chunk = (total + 1023u) / 1024u;
+    const uint32_t chunk = (total + T - 1) / T;
     const uint32_t b0 = t * chunk;
     uint32_t sum = 0;
     for (uint32_t k = 0; k < chunk; ++k) {
         const uint32_t e = b0 + k;
         if (e < total) sum += hist[e];
     }
-    s[t] = sum;
+    sh[t] = sum;
     __syncthreads();
-    for (uint32_t off = 1; off < 1024; off <<= 1) {
-        const uint32_t v = t >= off ? s[t - off] : 0u;
+    for (uint32_t off = 1; off < (uint32_t)T; off <<= 1) {
+        const uint32_t v = t >= off ? sh[t - off] : 0u;
         __syncthreads();
-        s[t] += v;
+        sh[t] += v;
         __syncthreads();
     }
-    uint32_t run = s[t] - sum;  // exclusive prefix of this chunk
+    uint32_t run = sh[t] - sum;  // exclusive prefix of this chunk
     for (uint32_t k = 0; k < chunk; ++k) {
         const uint32_t e = b0 + k;
         if (e < total) {
@@ -132,20 +151,22 @@ __global__ __launch_bounds__(1024) void k_radix_scan(const uint32_t* __restrict_
             run += v;
         }
     }
+    __syncthreads();
 }
 
-__global__ __launch_bounds__(BLOCK) void k_radix_scatter(
-    const uint32_t* __restrict__ gate, const uint32_t* __restrict__ kin,
-    const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-    uint32_t n, int shift, const uint32_t* __restrict__ hist, uint32_t ntiles) {
-    if (*gate == 0u) return;
+__device__ __forceinline__ void radix_scatter_tile(const uint32_t* __restrict__ kin,
+                                                   const uint32_t* __restrict__ vin,
+                                                   uint32_t* __restrict__ kout,
+                                                   uint32_t* __restrict__ vout, uint32_t n,
+                                                   int shift, const uint32_t* __restrict__ hist,
+                                                   uint32_t ntiles, uint32_t tile) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t whist[BLOCK / 64][256];
     const uint32_t t = threadIdx.x;
     const uint32_t w = t >> 6;
     const uint32_t lane = t & 63u;
-    running[t] = hist[t * ntiles + blockIdx.x];
-    const uint32_t base = blockIdx.x * RADIX_TILE;
+    running[t] = hist[t * ntiles + tile];
+    const uint32_t base = tile * RADIX_TILE;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int j = 0; j < RADIX_ITEMS; ++j) {
 #pragma unroll
@@ -181,23 +202,71 @@ __global__ __launch_bounds__(BLOCK) void k_radix_scatter(
     }
 }
 
+// Buffers of pass `pass` out of `passes`: the last pass lands in (skey, perm);
+// pass 0 reads the hash keys with the identity as values.
+struct RadixPass {
+    const uint32_t* kin;
+    const uint32_t* vin;
+    uint32_t* kout;
+    uint32_t* vout;
+};
+__host__ __device__ inline RadixPass radix_pass(const DevArrays& a, int pass, int passes) {
+    RadixPass r;
+    const bool to_final = ((passes - 1 - pass) % 2) == 0;
+    if (pass == 0) {
+        r.kin = a.key;
+        r.vin = nullptr;
+    } else {
+        const bool prev_final = ((passes - pass) % 2) == 0;
+        r.kin = prev_final ? a.skey : a.key_tmp;
+        r.vin = prev_final ? a.perm : a.val_tmp;
+    }
+    r.kout = to_final ? a.skey : a.key_tmp;
+    r.vout = to_final ? a.perm : a.val_tmp;
+    return r;
+}
+
+// Every per-stage kernel returns at once when no particle changed cell this
+// step: the stable order of (cell, index) is then unchanged.
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(const DevLoop* __restrict__ lp,
+                                                      const uint32_t* __restrict__ kin,
+                                                      uint32_t n, int shift,
+                                                      uint32_t* __restrict__ hist,
+                                                      uint32_t ntiles) {
+    if (!sort_needed(lp)) return;
+    radix_hist_tile(kin, n, shift, hist, ntiles, blockIdx.x);
+}
+
+__global__ __launch_bounds__(1024) void k_radix_scan(const DevLoop* __restrict__ lp,
+                                                     uint32_t* __restrict__ hist,
+                                                     uint32_t total) {
+    if (!sort_needed(lp)) return;
+    radix_scan_block<1024>(hist, total);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_radix_scatter(
+    const DevLoop* __restrict__ lp, const uint32_t* __restrict__ kin,
+    const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+    uint32_t n, int shift, const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    if (!sort_needed(lp)) return;
+    radix_scatter_tile(kin, vin, kout, vout, n, shift, hist, ntiles, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Cells and reorder
 
-__global__ __launch_bounds__(BLOCK) void k_fill_u32(const uint32_t* __restrict__ gate,
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(const DevLoop* __restrict__ lp,
                                                     uint32_t* __restrict__ dst, uint32_t n,
                                                     uint32_t v) {
-    if (gate && *gate == 0u) return;
+    if (lp && !sort_needed(lp)) return;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) dst[i] = v;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArrays a,
-                                                 int sorted_bits) {
-    if (a.loop->cells_changed == 0u) return;
-    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i == 0) atomicAdd(&a.loop->sorts, 1u);
-    if (i >= p.np) return;
+// reorderArrays (src/sorting.cuh:56-132) for sorted index i: cell ranges,
+// the sorted position copy, and the inverse permutation.
+__device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a, uint32_t i,
+                                          int sorted_bits) {
     const uint32_t key = a.skey[i];
     if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
         atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_KEY_RANGE);
@@ -215,6 +284,44 @@ __global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArr
     a.key_prev[o] = key;
 }
 
+__global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArrays a,
+                                                 int sorted_bits) {
+    if (!sort_needed(a.loop)) return;
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i == 0) atomicAdd(&a.loop->sorts, 1u);
+    if (i < p.np) cells_one(p, a, i, sorted_bits);
+}
+
+// The whole re-sort as one cooperative launch: cellStart reset, the radix
+// passes and the cell ranges, with grid barriers between the stages.  A
+// step in which no particle changed cell costs one launch that returns at
+// once (the flag is read before any barrier, so every workgroup takes the
+// same branch).
+__global__ __launch_bounds__(BLOCK) void k_sort_coop(const DevParams p, const DevArrays a,
+                                                     int passes, int sorted_bits) {
+    if (!sort_needed(a.loop)) return;
+    cg::grid_group grid = cg::this_grid();
+    const uint32_t n = p.np;
+    const uint32_t nt = n_tiles(n);
+    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
+    const uint32_t gstride = gridDim.x * BLOCK;
+    for (uint32_t i = gtid; i < p.ncells; i += gstride) a.cell_start[i] = 0xffffffffu;
+    for (int pass = 0; pass < passes; ++pass) {
+        const RadixPass r = radix_pass(a, pass, passes);
+        const int shift = 8 * pass;
+        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x)
+            radix_hist_tile(r.kin, n, shift, a.rhist, nt, t);
+        grid.sync();
+        if (blockIdx.x == 0) radix_scan_block<BLOCK>(a.rhist, 256u * nt);
+        grid.sync();
+        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x)
+            radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, n, shift, a.rhist, nt, t);
+        grid.sync();
+    }
+    if (gtid == 0) atomicAdd(&a.loop->sorts, 1u);
+    for (uint32_t i = gtid; i < n; i += gstride) cells_one(p, a, i, sorted_bits);
+}
+
 // ---------------------------------------------------------------------------
 // Contact models (src/contactmodels.cuh)
 
@@ -222,6 +329,37 @@ struct Acc {
     d3 F, T;
     double es_dot, ev_dot, p;
 };
+
+// What one particle-particle contact adds to the accumulators, kept apart so
+// that contacts evaluated on different lanes are added in slot order.  The
+// energy rates get up to two separate additions per contact (the normal and
+// the tangential dissipation), which are replayed as two additions.
+struct Contrib {
+    d3 F, T;
+    double p, ev0, ev1, es;
+    int nev, nes;
+};
+
+__device__ __forceinline__ void contrib_init(Contrib& c) {
+    c.nev = 0;
+    c.nes = 0;
+}
+__device__ __forceinline__ void add_ev(Contrib& c, double v) {
+    if (c.nev == 0) c.ev0 = v; else c.ev1 = v;
+    ++c.nev;
+}
+__device__ __forceinline__ void add_es(Contrib& c, double v) {
+    c.es = v;
+    c.nes = 1;
+}
+__device__ __forceinline__ void apply(Acc& A, const Contrib& c) {
+    A.F += c.F;
+    A.T += c.T;
+    A.p += c.p;
+    if (c.nev > 0) A.ev_dot += c.ev0;
+    if (c.nev > 1) A.ev_dot += c.ev1;
+    if (c.nes > 0) A.es_dot += c.es;
+}
 
 // contactLinear_wall, src/contactmodels.cuh:10-117.
 __device__ __forceinline__ double contact_linear_wall(const DevParams& p, Acc& A, double ra,
@@ -258,7 +396,7 @@ __device__ __forceinline__ double contact_linear_wall(const DevParams& p, Acc& A
 
 // contactLinear (model 2), src/contactmodels.cuh:244-411.  Returns the new
 // tangential displacement (stored by the caller).
-__device__ __forceinline__ d3 contact_linear(const DevParams& p, Acc& A, const double4 vel_a,
+__device__ __forceinline__ d3 contact_linear(const DevParams& p, Contrib& C, const double4 vel_a,
                                              d3 angvel_a, const double4 vel_b, d3 angvel_b,
                                              double ra, double rb, d3 x, double x_length,
                                              double delta, const double4 dt0) {
@@ -280,7 +418,7 @@ __device__ __forceinline__ d3 contact_linear(const DevParams& p, Acc& A, const d
     }
     const d3 f_n = fmax(0.0, -k_n * delta + p.gamma_n * vel_n) * n;
     const double f_n_length = length(f_n);
-    A.ev_dot += 0.5 * p.gamma_n * vel_n * vel_n;
+    add_ev(C, 0.5 * p.gamma_n * vel_n * vel_n);
     const d3 f_c = -p.kappa * sqrt32(ra * rb) * n;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
@@ -290,24 +428,24 @@ __device__ __forceinline__ d3 contact_linear(const DevParams& p, Acc& A, const d
         f_t = f_t_elast + f_t_visc;
         const double f_t_length = length(f_t);
         const double f_t_limit = p.mu_s * length(f_n - f_c);
-        A.ev_dot += 0.5 * p.gamma_t * vel_t_length * vel_t_length;
+        add_ev(C, 0.5 * p.gamma_t * vel_t_length * vel_t_length);
         if (f_t_length > f_t_limit) {
             const d3 t = f_t / length(f_t);
             f_t = f_t_limit * t;
             delta_t = -1.0 / k_t * (p.mu_d * length(f_n - f_c) * t + p.gamma_t * vel_t);
-            A.es_dot += 0.5 * length(length(f_t) * vel_t * p.dt) / p.dt;
+            add_es(C, 0.5 * length(length(f_t) * vel_t * p.dt) / p.dt);
         }
     }
-    A.F += f_n + f_t + f_c;
-    A.T += cross(-(ra + delta * 0.5) * n, f_t);
-    A.p += f_n_length / (4.0 * PI * ra * ra);
+    C.F = f_n + f_t + f_c;
+    C.T = cross(-(ra + delta * 0.5) * n, f_t);
+    C.p = f_n_length / (4.0 * PI * ra * ra);
     return delta_t;
 }
 
 // contactHertz (model 3), src/contactmodels.cuh:417-600, with powers of
 // |delta| through correctly rounded FP64 square roots and a repulsive
 // elastic normal term (the reference's powf of a negative overlap is NaN).
-__device__ __forceinline__ d3 contact_hertz(const DevParams& p, Acc& A, const double4 vel_a,
+__device__ __forceinline__ d3 contact_hertz(const DevParams& p, Contrib& C, const double4 vel_a,
                                             d3 angvel_a, const double4 vel_b, d3 angvel_b,
                                             double ra, double rb, d3 x_ab, double x_ab_length,
                                             double delta, const double4 dt0) {
@@ -334,7 +472,7 @@ __device__ __forceinline__ d3 contact_hertz(const DevParams& p, Acc& A, const do
     const double p14 = sqrt_rn_f64(p12);
     const double p32 = ad * p12;
     d3 f_n = (k_n * p32 - p.gamma_n * p14 * vel_n_ab) * n_ab;
-    A.ev_dot += p.gamma_n * vel_n_ab * vel_n_ab;
+    add_ev(C, p.gamma_n * vel_n_ab * vel_n_ab);
     if (dot(f_n, n_ab) < 0.0) f_n = mk3(0.0, 0.0, 0.0);
     const double f_n_length = length(f_n);
     const d3 f_c = -p.kappa * sqrt32(ra * rb) * n_ab;
@@ -353,7 +491,7 @@ __device__ __forceinline__ d3 contact_hertz(const DevParams& p, Acc& A, const do
         if (f_t_length > f_t_limit) {
             f_t *= f_t_limit / f_t_length;
             delta_t = (f_t + p.gamma_t * p14 * vel_t_ab) / (-k_t * p12);
-            A.es_dot += length(delta_t0 - delta_t) * k_t / p.dt;
+            add_es(C, length(delta_t0 - delta_t) * k_t / p.dt);
         } else {
             delta_t = delta_t0 + vel_t_ab * p.dt;
         }
@@ -362,9 +500,9 @@ __device__ __forceinline__ d3 contact_hertz(const DevParams& p, Acc& A, const do
         T_res = -1.0 * fmin(p.gamma_r * ra * angvel_ab_length, p.mu_r * ra * f_n_length) *
                 angvel_ab / angvel_ab_length;
     }
-    A.F += f_n + f_t + f_c;
-    A.T += -(ra + delta / 2.0) * cross(n_ab, f_t) + T_res;
-    A.p += f_n_length / (4.0 * PI * ra * ra);
+    C.F = f_n + f_t + f_c;
+    C.T = -(ra + delta / 2.0) * cross(n_ab, f_t) + T_res;
+    C.p = f_n_length / (4.0 * PI * ra * ra);
     return delta_t;
 }
 
@@ -512,6 +650,77 @@ __device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t np, uin
     return false;
 }
 
+// Walls (src/contactsearch.cuh:602-701): top, bottom, then the side walls
+// (all four when periodic == 0, the y walls when periodic == 2), each
+// spelled out so that nothing is indexed dynamically; then the particle's
+// outputs at its original index o when `write` is set.
+__device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevArrays& a, Acc& A,
+                                                  const double4 xa, const double4 vel_a,
+                                                  const double4 av_a, double Lz, uint32_t o,
+                                                  bool write) {
+    const double ra = xa.w;
+    double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
+    {
+        double4 wn = make_double4(0.0, 0.0, -1.0, Lz);
+        double wv = 0.0;
+        if (p.nw >= 1) { wn = a.w_nx[0]; wv = a.w_mvfd[0].y; }
+        const double dw = wn.w - (xa.z + ra);
+        if (dw < 0.0) wf0 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+    }
+    {
+        const double dw = xa.z - ra - p.origo[2];
+        if (dw < 0.0)
+            (void)contact_linear_wall(p, A, ra, vel_a, av_a, mk3(0.0, 0.0, 1.0), dw, 0.0);
+    }
+    if (p.periodic == 0) {
+        {
+            double4 wn = make_double4(-1.0, 0.0, 0.0, p.L[0]);
+            double wv = 0.0;
+            if (p.nw >= 2) { wn = a.w_nx[1]; wv = a.w_mvfd[1].y; }
+            const double dw = wn.w - (xa.x + ra);
+            if (dw < 0.0) wf1 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+        }
+        {
+            double4 wn = make_double4(1.0, 0.0, 0.0, 0.0);
+            double wv = 0.0;
+            if (p.nw >= 3) { wn = a.w_nx[2]; wv = a.w_mvfd[2].y; }
+            const double dw = xa.x - ra - wn.w;
+            if (dw < 0.0) wf2 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+        }
+    }
+    if (p.periodic == 0 || p.periodic == 2) {
+        {
+            double4 wn = make_double4(0.0, -1.0, 0.0, p.L[1]);
+            double wv = 0.0;
+            if (p.nw >= 4) { wn = a.w_nx[3]; wv = a.w_mvfd[3].y; }
+            const double dw = wn.w - (xa.y + ra);
+            if (dw < 0.0) wf3 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+        }
+        {
+            double4 wn = make_double4(0.0, 1.0, 0.0, 0.0);
+            double wv = 0.0;
+            if (p.nw >= 5) { wn = a.w_nx[4]; wv = a.w_mvfd[4].y; }
+            const double dw = xa.y - ra - wn.w;
+            if (dw < 0.0) wf4 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+        }
+    }
+    // Outputs at the original index, src/contactsearch.cuh:706-724.
+    if (!write) return;
+    const uint32_t np = p.np;
+    a.force[o] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
+    a.torque[o] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
+    a.es_dot[o] = A.es_dot;
+    a.ev_dot[o] = A.ev_dot;
+    a.es[o] += A.es_dot * p.dt;
+    a.ev[o] += A.ev_dot * p.dt;
+    a.p[o] = A.p;
+    if (p.nw > 0) a.force_pp[o] = wf0;
+    if (p.nw > 1) a.force_pp[(size_t)np + o] = wf1;
+    if (p.nw > 2) a.force_pp[(size_t)np * 2 + o] = wf2;
+    if (p.nw > 3) a.force_pp[(size_t)np * 3 + o] = wf3;
+    if (p.nw > 4) a.force_pp[(size_t)np * 4 + o] = wf4;
+}
+
 // CM is the contact model (1, 2 or 3); one instantiation per model keeps
 // only that model's registers live.
 template <int CM>
@@ -552,7 +761,9 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
         // sequence: the full scan that also rebuilds this particle's
         // candidate list, the full scan alone (list overflowed), or the list.
         int nc = 0;
-        const bool rebuild = !p.vl_enabled || a.loop->vl_rebuild != 0u;
+        const DevLoop* lp = a.loop;
+        const bool rebuild =
+            !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
         const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
         if (rebuild || vcnt == VL_OVERFLOW) {
             const bool build = p.vl_enabled && rebuild;
@@ -648,12 +859,15 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     const d3 angvel_b = xyz(a.angvel[ib_orig]);
                     const double4 dt0 = a.delta_t[mp];
                     d3 nd;
+                    Contrib C;
+                    contrib_init(C);
                     if constexpr (CM == 2)
-                        nd = contact_linear(p, A, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
+                        nd = contact_linear(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
                                             len, dn, dt0);
                     else
-                        nd = contact_hertz(p, A, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
+                        nd = contact_hertz(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab,
                                            len, dn, dt0);
+                    apply(A, C);
                     a.delta_t[mp] = make_double4(nd.x, nd.y, nd.z, 0.0);
                 } else {
                     row[i] = np;
@@ -705,68 +919,307 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     }
                 }
     }
-    // Walls, src/contactsearch.cuh:602-701: top, bottom, then the side
-    // walls (all four when periodic == 0, the y walls when periodic == 2).
-    // Each wall is spelled out so that nothing is indexed dynamically.
-    double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
-    {
-        double4 wn = make_double4(0.0, 0.0, -1.0, Lz);
-        double wv = 0.0;
-        if (p.nw >= 1) { wn = a.w_nx[0]; wv = a.w_mvfd[0].y; }
-        const double dw = wn.w - (xa.z + ra);
-        if (dw < 0.0) wf0 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, true);
+}
+
+// ---------------------------------------------------------------------------
+// topology + interact with LPP lanes per particle (contact models 2 and 3).
+// The per-particle work is a chain of dependent gathers (candidate entry ->
+// candidate position -> partner state); one lane per particle leaves the
+// chip with ~1.5 wavefronts per SIMD at 1e5 particles and nothing to hide
+// that latency.  Here a group of LPP consecutive lanes shares a particle:
+// the candidates are tested LPP at a time, the contact slots are held LPP-
+// strided in registers (slot s on lane s % LPP), the occupied slots are
+// evaluated in parallel, and everything whose order the reference fixes is
+// replayed in that order: slot registration in scan order, and the
+// accumulation of contact contributions in slot order.  Every branch a group
+// takes is taken by all of its lanes, so group ballots and shuffles see the
+// whole group.
+
+template <int LPP>
+struct LaneGroup {
+    static_assert(LPP == 4 || LPP == 8 || LPP == 16, "group size");
+    static constexpr uint32_t MASK = (1u << LPP) - 1u;
+    uint32_t q;     // lane within the group
+    uint32_t base;  // wavefront lane of the group's lane 0
+    __device__ __forceinline__ LaneGroup() {
+        const uint32_t l = __lane_id();
+        q = l % LPP;
+        base = l - q;
     }
-    {
-        const double dw = xa.z - ra - p.origo[2];
-        if (dw < 0.0)
-            (void)contact_linear_wall(p, A, ra, vel_a, av_a, mk3(0.0, 0.0, 1.0), dw, 0.0);
+    // Bit i = predicate of group lane i.
+    __device__ __forceinline__ uint32_t bits(bool v) const {
+        return (uint32_t)(__ballot(v) >> base) & MASK;
     }
-    if (p.periodic == 0) {
-        {
-            double4 wn = make_double4(-1.0, 0.0, 0.0, p.L[0]);
-            double wv = 0.0;
-            if (p.nw >= 2) { wn = a.w_nx[1]; wv = a.w_mvfd[1].y; }
-            const double dw = wn.w - (xa.x + ra);
-            if (dw < 0.0) wf1 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+    __device__ __forceinline__ uint32_t from(uint32_t v, uint32_t src) const {
+        return (uint32_t)__shfl((int)v, (int)(base + src), 64);
+    }
+    __device__ __forceinline__ double from(double v, uint32_t src) const {
+        return __shfl(v, (int)(base + src), 64);
+    }
+    __device__ __forceinline__ d3 from(d3 v, uint32_t src) const {
+        return mk3(from(v.x, src), from(v.y, src), from(v.z, src));
+    }
+};
+
+// The contact-slot policy (src/contactsearch.cuh:265-299) over the slot
+// registers of a group: the first slot holding the partner, else the last
+// empty slot, else slot nc (whose write is suppressed, as in the one-lane
+// kernel).  Returns true when nc exceeds NC, which ends the current cell.
+template <int LPP>
+__device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t (&slot)[NC / LPP],
+                                               uint32_t np, uint32_t word, int& nc,
+                                               DevLoop* loop) {
+    constexpr int SPL = NC / LPP;
+    const uint32_t ib_orig = word & CW_IDX_MASK;
+    uint32_t match = 0, empty = 0;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        const uint32_t cidx = slot[j] & CW_IDX_MASK;
+        match |= g.bits(cidx == ib_orig) << (LPP * j);
+        empty |= g.bits(cidx == np) << (LPP * j);
+    }
+    int cpos = nc;
+    if (match) cpos = __ffs(match) - 1;
+    else if (empty) cpos = 31 - __clz((int)empty);
+    if (cpos < NC && (uint32_t)cpos % LPP == g.q) {
+#pragma unroll
+        for (int j = 0; j < SPL; ++j)
+            if (j == cpos / LPP) slot[j] = word;
+    }
+    ++nc;
+    if (nc > NC) {
+        if (g.q == 0) atomicOr(&loop->error_bits, (uint32_t)DFLAG_CONTACT_OVERFLOW);
+        return true;
+    }
+    return false;
+}
+
+template <int CM, int LPP>
+__global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const DevArrays a) {
+    static_assert(CM == 2 || CM == 3, "history models only");
+    constexpr int SPL = NC / LPP;       // slots per lane
+    constexpr int PPB = BLOCK / LPP;    // particles per workgroup
+    const LaneGroup<LPP> g;
+    const uint32_t ia = blockIdx.x * PPB + threadIdx.x / LPP;
+    if (ia >= p.np) return;  // whole groups leave together
+    const uint32_t np = p.np;
+    const uint32_t ia_orig = a.perm[ia];
+    const double4 xa = a.xs[ia];
+    const double ra = xa.w;
+    const double Lz = grid_Lz(p, a);
+    const double4 vel_a = a.vel[ia_orig];
+    const double4 av_a = a.angvel[ia_orig];
+    const d3 angvel_a = xyz(av_a);
+    const DevLoop* lp = a.loop;
+    uint32_t slot[SPL];
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) slot[j] = a.crow[(size_t)ia_orig * NC + g.q + LPP * j];
+
+    // topology: the candidate sequence of the 27-cell scan, LPP at a time.
+    int nc = 0;
+    const bool rebuild =
+        !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
+    const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
+    const uint32_t below = (1u << g.q) - 1u;
+    if (rebuild || vcnt == VL_OVERFLOW) {
+        const bool build = p.vl_enabled && rebuild;
+        uint32_t nl = 0;
+        uint2* vrow = a.vlist + (size_t)ia * NV;
+        const int gx = (int)floor((xa.x - p.origo[0]) / (p.L[0] / p.num[0]));
+        const int gy = (int)floor((xa.y - p.origo[1]) / (p.L[1] / p.num[1]));
+        const int gz = (int)floor((xa.z - p.origo[2]) / (Lz / p.num[2]));
+        for (int dz = -1; dz < 2; ++dz)
+            for (int dy = -1; dy < 2; ++dy)
+                for (int dx = -1; dx < 2; ++dx) {
+                    int tx = gx + dx, ty = gy + dy;
+                    const int tz = gz + dz;
+                    const int code = find_dist_mod(p, tx, ty, tz);
+                    if (code < 0) continue;
+                    const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                                          (p.num[0] * p.num[1]) * (uint32_t)tz;
+                    const uint32_t start = a.cell_start[cell];
+                    if (start == 0xffffffffu) continue;
+                    const uint32_t end = a.cell_end[cell];
+                    const d3 dm = shift_of(p, (uint32_t)code);
+                    const uint32_t celloff = (uint32_t)((dz + 1) * 9 + (dy + 1) * 3 + (dx + 1));
+                    bool cell_done = false;
+                    for (uint32_t r = start; r < end; r += LPP) {
+                        const uint32_t ib = r + g.q;
+                        bool hit = false, near = false;
+                        uint32_t word = 0;
+                        if (ib < end && ib != ia) {
+                            const double4 xb = a.xs[ib];
+                            const double rb = xb.w;
+                            d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                            xab += dm;
+                            const double dd = dot(xab, xab);
+                            const double S = ra + rb;
+                            if (build) {
+                                const double Sv = S + p.vl_skin;
+                                near = dd < Sv * Sv;
+                            }
+                            // Conservative reject: len32 > S whenever
+                            // dd >= S^2 (1 + 2^-16).
+                            if (!(dd >= S * S * 1.0000152587890625)) {
+                                const double len = sqrt32(dd);
+                                hit = len - S < 0.0;
+                            }
+                            if (hit || near) word = pack_word(a.perm[ib], (uint32_t)code);
+                        }
+                        if (build) {
+                            const uint32_t nm = g.bits(near);
+                            const uint32_t pos = nl + __popc(nm & below);
+                            if (near && pos < (uint32_t)NV)
+                                vrow[pos] = make_uint2(ib | (celloff << VL_CELL_SHIFT), word);
+                            nl += __popc(nm);
+                        }
+                        uint32_t hm = g.bits(hit);
+                        while (hm && !cell_done) {
+                            const uint32_t b = __ffs(hm) - 1;
+                            hm &= hm - 1u;
+                            cell_done = group_register(g, slot, np, g.from(word, b), nc, a.loop);
+                        }
+                    }
+                }
+        if (build && g.q == 0) {
+            a.vcount[ia] = nl <= (uint32_t)NV ? nl : VL_OVERFLOW;
+            a.x_build[ia_orig] = xa;
         }
-        {
-            double4 wn = make_double4(1.0, 0.0, 0.0, 0.0);
-            double wv = 0.0;
-            if (p.nw >= 3) { wn = a.w_nx[2]; wv = a.w_mvfd[2].y; }
-            const double dw = xa.x - ra - wn.w;
-            if (dw < 0.0) wf2 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
+    } else {
+        const uint2* vrow = a.vlist + (size_t)ia * NV;
+        uint32_t last_cell = 0xffffffffu;
+        bool cell_done = false;
+        for (uint32_t r = 0; r < vcnt; r += LPP) {
+            const uint32_t k = r + g.q;
+            bool hit = false;
+            uint2 e = make_uint2(0u, 0u);
+            if (k < vcnt) {
+                e = vrow[k];
+                const double4 xb = a.xs[e.x & VL_IB_MASK];
+                const double rb = xb.w;
+                d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                xab += shift_of(p, word_code(e.y));
+                const double dd = dot(xab, xab);
+                const double S = ra + rb;
+                if (!(dd >= S * S * 1.0000152587890625)) {
+                    const double len = sqrt32(dd);
+                    hit = len - S < 0.0;
+                }
+            }
+            // Only overlapping candidates register; the cells of the others
+            // cannot matter, as each cell's entries are contiguous.
+            uint32_t hm = g.bits(hit);
+            while (hm) {
+                const uint32_t b = __ffs(hm) - 1;
+                hm &= hm - 1u;
+                const uint32_t celloff = g.from(e.x, b) >> VL_CELL_SHIFT;
+                if (celloff != last_cell) {
+                    last_cell = celloff;
+                    cell_done = false;
+                }
+                if (cell_done) continue;
+                cell_done = group_register(g, slot, np, g.from(e.y, b), nc, a.loop);
+            }
         }
     }
-    if (p.periodic == 0 || p.periodic == 2) {
-        {
-            double4 wn = make_double4(0.0, -1.0, 0.0, p.L[1]);
-            double wv = 0.0;
-            if (p.nw >= 4) { wn = a.w_nx[3]; wv = a.w_mvfd[3].y; }
-            const double dw = wn.w - (xa.y + ra);
-            if (dw < 0.0) wf3 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
-        }
-        {
-            double4 wn = make_double4(0.0, 1.0, 0.0, 0.0);
-            double wv = 0.0;
-            if (p.nw >= 5) { wn = a.w_nx[4]; wv = a.w_mvfd[4].y; }
-            const double dw = xa.y - ra - wn.w;
-            if (dw < 0.0) wf4 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
-        }
+
+    // interact (src/contactsearch.cuh:484-559): the occupied slots, LPP at a
+    // time in ascending slot order, each on its own lane; contributions are
+    // then added in slot order on every lane of the group.
+    const bool zero_empty = lp->zero_empty != 0u;
+    uint32_t occ = 0;
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        const bool o = (slot[j] & CW_IDX_MASK) != np;
+        occ |= g.bits(o) << (LPP * j);
+        if (!o && zero_empty)
+            a.delta_t[(size_t)ia_orig * NC + g.q + LPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
-    // Outputs at the original index, src/contactsearch.cuh:706-724.
-    const uint32_t o = ia_orig;
-    a.force[o] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
-    a.torque[o] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
-    a.es_dot[o] = A.es_dot;
-    a.ev_dot[o] = A.ev_dot;
-    a.es[o] += A.es_dot * p.dt;
-    a.ev[o] += A.ev_dot * p.dt;
-    a.p[o] = A.p;
-    if (p.nw > 0) a.force_pp[o] = wf0;
-    if (p.nw > 1) a.force_pp[(size_t)np + o] = wf1;
-    if (p.nw > 2) a.force_pp[(size_t)np * 2 + o] = wf2;
-    if (p.nw > 3) a.force_pp[(size_t)np * 3 + o] = wf3;
-    if (p.nw > 4) a.force_pp[(size_t)np * 4 + o] = wf4;
+    Acc A;
+    A.F = mk3(0.0, 0.0, 0.0);
+    A.T = mk3(0.0, 0.0, 0.0);
+    A.es_dot = 0.0;
+    A.ev_dot = 0.0;
+    A.p = 0.0;
+    uint32_t rem = occ;
+    while (rem) {
+        // This lane takes the q-th occupied slot of the round.
+        uint32_t m = rem;
+        for (uint32_t t = 0; t < g.q && m; ++t) m &= m - 1u;
+        const int take = __popc(rem) < LPP ? __popc(rem) : LPP;
+        const bool mine = m != 0u && (int)g.q < take;
+        const uint32_t sidx = mine ? (uint32_t)(__ffs(m) - 1) : 0u;
+        uint32_t wrd = 0;
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) {
+            const uint32_t v = g.from(slot[j], sidx % LPP);
+            if ((uint32_t)j == sidx / LPP) wrd = v;
+        }
+        Contrib C;
+        contrib_init(C);
+        C.F = mk3(0.0, 0.0, 0.0);
+        C.T = mk3(0.0, 0.0, 0.0);
+        C.p = 0.0;
+        C.ev0 = 0.0;
+        C.ev1 = 0.0;
+        C.es = 0.0;
+        bool touching = false, cleared = false;
+        if (mine) {
+            const size_t mp = (size_t)ia_orig * NC + sidx;
+            const uint32_t ib_orig = wrd & CW_IDX_MASK;
+            const d3 dm = shift_of(p, word_code(wrd));
+            const double4 xb = a.x[ib_orig];
+            const double rb = xb.w;
+            const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
+            const double len = length(xab);
+            const double dn = len - (ra + rb);
+            if (dn < 0.0) {
+                touching = true;
+                const double4 vel_b = a.vel[ib_orig];
+                const d3 angvel_b = xyz(a.angvel[ib_orig]);
+                const double4 dt0 = a.delta_t[mp];
+                d3 nd;
+                if constexpr (CM == 2)
+                    nd = contact_linear(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab, len,
+                                        dn, dt0);
+                else
+                    nd = contact_hertz(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab, len,
+                                       dn, dt0);
+                a.delta_t[mp] = make_double4(nd.x, nd.y, nd.z, 0.0);
+            } else {
+                cleared = true;
+                a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+        }
+        const uint32_t tm = g.bits(touching);
+        const uint32_t cm = g.bits(cleared);
+        for (int t = 0; t < take; ++t) {
+            const uint32_t st = g.from(sidx, (uint32_t)t);
+            if ((tm >> t) & 1u) {
+                Contrib c;
+                c.F = g.from(C.F, (uint32_t)t);
+                c.T = g.from(C.T, (uint32_t)t);
+                c.p = g.from(C.p, (uint32_t)t);
+                c.nev = (int)g.from((uint32_t)C.nev, (uint32_t)t);
+                c.nes = (int)g.from((uint32_t)C.nes, (uint32_t)t);
+                c.ev0 = g.from(C.ev0, (uint32_t)t);
+                c.ev1 = g.from(C.ev1, (uint32_t)t);
+                c.es = g.from(C.es, (uint32_t)t);
+                apply(A, c);
+            }
+            if (((cm >> t) & 1u) && st % LPP == g.q) {
+#pragma unroll
+                for (int j = 0; j < SPL; ++j)
+                    if ((uint32_t)j == st / LPP) slot[j] = np;
+            }
+        }
+        // Drop the round's slots from the remaining set.
+        for (int t = 0; t < take; ++t) rem &= rem - 1u;
+    }
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) a.crow[(size_t)ia_orig * NC + g.q + LPP * j] = slot[j];
+    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q == 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -785,21 +1238,6 @@ __device__ __forceinline__ double block_tree_sum(double v, double* cache) {
     return cache[0];
 }
 
-// Arrival counter: returns true in exactly one (the last) workgroup, after
-// every workgroup's partial is visible.
-__device__ __forceinline__ bool last_block(uint32_t* counter, uint32_t nblocks) {
-    __shared__ uint32_t is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t prev = atomicAdd(counter, 1u);
-        is_last = (prev == nblocks - 1) ? 1u : 0u;
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last != 0u;
-}
-
 // The serial left-to-right sum of n partials (integrateWalls'
 // `for i < blocksPerGrid: sum += partial[i]`), with the partials staged
 // through LDS by the whole workgroup.  The result is valid in thread 0.
@@ -811,15 +1249,29 @@ __device__ __forceinline__ double serial_sum(const double* __restrict__ part, ui
         const uint32_t cnt = (n - base) < CH ? (n - base) : CH;
         for (uint32_t j = threadIdx.x; j < cnt; j += BLOCK) stage[j] = part[base + j];
         __syncthreads();
-        if (threadIdx.x == 0)
-            for (uint32_t j = 0; j < cnt; ++j) sum += stage[j];
+        if (threadIdx.x == 0) {
+            // Sixteen LDS reads in flight, then the dependent adds in order:
+            // the chain costs one LDS round trip per sixteen partials.
+            uint32_t j = 0;
+            for (; j + 16 <= cnt; j += 16) {
+                double v[16];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) v[k] = stage[j + k];
+#pragma unroll
+                for (int k = 0; k < 16; ++k) sum += v[k];
+            }
+            for (; j < cnt; ++j) sum += stage[j];
+        }
         __syncthreads();
     }
     return sum;
 }
 
-// findShearStressOnFixedMovingParticles + summation + the serial sum of
-// integrate (src/integration.cuh:457-484, 309-341, 64-68).
+// findShearStressOnFixedMovingParticles + summation (src/integration.cuh:
+// 457-484, 309-341): per-particle stress and the 256-item block partials.
+// Partials cross to the serial sum through a kernel boundary, not through
+// an in-kernel arrival counter: a device-scope release in every workgroup
+// writes back each XCD's L2 and costs more than the launch it saves.
 __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const DevArrays a) {
     __shared__ double cache[BLOCK];
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
@@ -837,13 +1289,14 @@ __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const
     }
     const double s = block_tree_sum(in ? v : 0.0, cache);
     if (threadIdx.x == 0) a.tau_partial[blockIdx.x] = s;
-    if (!last_block(&a.loop->tau_blocks_done, gridDim.x)) return;
+}
+
+// The serial sum of the stress partials that integrate reads as tau_eff_x
+// (src/integration.cuh:64-68); one workgroup.
+__global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevParams p, const DevArrays a) {
     __shared__ double stage[2048];
     const double tau = serial_sum(a.tau_partial, p.nblocks, stage);
-    if (threadIdx.x == 0) {
-        a.tau_eff[0] = tau;
-        a.loop->tau_blocks_done = 0u;
-    }
+    if (threadIdx.x == 0) a.tau_eff[0] = tau;
 }
 
 // ---------------------------------------------------------------------------
@@ -851,9 +1304,8 @@ __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const
 // original order (the reference's sorted-order pass reads copies of the
 // same values; every output is per particle).
 
-__global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const DevArrays a) {
-    const uint32_t orig = blockIdx.x * BLOCK + threadIdx.x;
-    if (orig >= p.np) return;
+__device__ __forceinline__ double4 integrate_particle(const DevParams& p, const DevArrays& a,
+                                                      uint32_t orig) {
     const DevLoop* lp = a.loop;
     const double t_now = lp->t;
     const int64_t iter = lp->iter;
@@ -966,23 +1418,41 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const De
     // Keep the sorted copy current: valid for the next step whenever the
     // order does not change (otherwise the next sort rebuilds it).
     a.xs[a.rank[orig]] = x_new;
+    return x_new;
 }
 
 // ---------------------------------------------------------------------------
-// summation + integrateWalls + loop advance.  Every dynamic wall integrates
-// wall 0's force sum (src/device.cu:2504-2505, P8).
+// integrate + the next step's hash + the wall force block partials.  The
+// grid is the summation kernel's (256 work-items per block over np, so the
+// partials are the reference's, src/integration.cuh:309-341).  With a fixed
+// grid height the cell keys of step k + 1 depend only on the positions this
+// pass writes, so they are computed here into flag slot (k + 1) & 1 and the
+// next step needs no hash launch; with an adaptive grid (L[2] follows the
+// top wall, which moves after this pass) hash_next is 0.
 
-__global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArrays a) {
+__global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const DevArrays a,
+                                                     int hash_next) {
     __shared__ double cache[BLOCK];
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
-    const bool walls = p.nw > 0 && p.np > 0;
-    if (walls) {
+    HashFlags f;
+    if (idx < p.np) {
+        const double4 x_new = integrate_particle(p, a, idx);
+        if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
+    }
+    if (hash_next) publish_hash(a.loop, step_slot(a.loop) ^ 1u, f);
+    if (p.nw > 0) {
         const double v = idx < p.np ? a.force_pp[idx] : 0.0;
         const double s = block_tree_sum(v, cache);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
     }
-    if (!last_block(&a.loop->wall_blocks_done, gridDim.x)) return;
+}
+
+// integrateWalls + the loop advance (src/integration.cuh:344-450,
+// src/device.cu:2454-2462,2500-2538), one workgroup.  Every dynamic wall
+// integrates wall 0's force sum (src/device.cu:2504-2505).
+__global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArrays a) {
     __shared__ double stage[2048];
+    const bool walls = p.nw > 0 && p.np > 0;
     const double fsum = walls ? serial_sum(a.partial, p.nblocks, stage) : 0.0;
     if (threadIdx.x != 0) return;
     DevLoop* lp = a.loop;
@@ -1008,17 +1478,18 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
         a.w_mvfd[w] = w_mvfd;
         a.w_acc[w] = acc;
     }
-    // Host loop bookkeeping (src/device.cu:2454-2462,2536-2538).
     if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
     else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
+    // Retire this step's flag slot; the next step's slot was filled by the
+    // integration pass (or is filled by its own hash launch).
+    const uint32_t slot = step_slot(lp);
+    if (p.vl_enabled && (lp->rebuild[slot] || lp->force_rebuild)) lp->vl_builds += 1u;
+    lp->changed[slot] = 0u;
+    lp->rebuild[slot] = 0u;
+    lp->force_rebuild = 0u;
+    lp->zero_empty = 0u;
     lp->t = t_now + dt;
     lp->iter = lp->iter + 1;
-    lp->zero_empty = 0u;
-    lp->cells_changed = 0u;
-    if (lp->vl_rebuild) lp->vl_builds += 1u;
-    lp->vl_rebuild = 0u;
-    lp->key_or = 0u;
-    lp->wall_blocks_done = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1142,40 +1613,69 @@ hipError_t launch_sort(const DevParams& p, const DevArrays& a, int passes, hipSt
     const uint32_t n = p.np;
     const uint32_t nt = n_tiles(n);
     for (int pass = 0; pass < passes; ++pass) {
+        const RadixPass r = radix_pass(a, pass, passes);
         const int shift = 8 * pass;
-        const bool to_final = ((passes - 1 - pass) % 2) == 0;
-        const uint32_t* kin;
-        const uint32_t* vin;
-        if (pass == 0) {
-            kin = a.key;
-            vin = nullptr;
-        } else {
-            const bool prev_final = ((passes - pass) % 2) == 0;
-            kin = prev_final ? a.skey : a.key_tmp;
-            vin = prev_final ? a.perm : a.val_tmp;
-        }
-        uint32_t* kout = to_final ? a.skey : a.key_tmp;
-        uint32_t* vout = to_final ? a.perm : a.val_tmp;
-        const uint32_t* gate = &a.loop->cells_changed;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, gate, kin, n, shift, a.rhist,
-                           nt);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, gate, a.rhist, 256u * nt);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, gate, kin, vin, kout, vout,
-                           n, shift, a.rhist, nt);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, a.loop, r.kin, n, shift,
+                           a.rhist, nt);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.loop, a.rhist, 256u * nt);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, a.loop, r.kin, r.vin,
+                           r.kout, r.vout, n, shift, a.rhist, nt);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits, hipStream_t s) {
     if (p.ncells)
-        hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(p.ncells)), dim3(BLOCK), 0, s,
-                           &a.loop->cells_changed, a.cell_start, p.ncells, 0xffffffffu);
+        hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(p.ncells)), dim3(BLOCK), 0, s, a.loop,
+                           a.cell_start, p.ncells, 0xffffffffu);
     hipLaunchKernelGGL(k_cells, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a, sorted_bits);
     return hipGetLastError();
 }
 
-hipError_t launch_contact(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    const dim3 g(n_blocks(p.np)), b(BLOCK);
+unsigned sort_coop_grid(unsigned np) {
+    int dev = 0, coop = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
+        !coop)
+        return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sort_coop, BLOCK, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return 0;
+    // Enough workgroups for the grid-stride stages, never more than can be
+    // resident at once.
+    const unsigned want = n_blocks(np) < 1u ? 1u : n_blocks(np);
+    const unsigned cap = (unsigned)cus * (unsigned)(per_cu < 2 ? per_cu : 2);
+    return want < cap ? want : cap;
+}
+
+hipError_t launch_sort_coop(const DevParams& p, const DevArrays& a, int passes,
+                            int sorted_bits, unsigned grid, hipStream_t s) {
+    DevParams pp = p;
+    DevArrays aa = a;
+    void* args[] = {&pp, &aa, &passes, &sorted_bits};
+    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_sort_coop), dim3(grid),
+                                      dim3(BLOCK), args, 0, s);
+}
+
+hipError_t launch_contact(const DevParams& p, const DevArrays& a, int lanes, hipStream_t s) {
+    const dim3 b(BLOCK);
+    if (lanes > 1 && (p.contactmodel == 2 || p.contactmodel == 3)) {
+        const dim3 g((p.np + BLOCK / lanes - 1) / (BLOCK / lanes));
+        if (p.contactmodel == 2) {
+            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<2, 4>), g, b, 0, s, p, a);
+            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<2, 8>), g, b, 0, s, p, a);
+            else hipLaunchKernelGGL((k_contact_g<2, 16>), g, b, 0, s, p, a);
+        } else {
+            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<3, 4>), g, b, 0, s, p, a);
+            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<3, 8>), g, b, 0, s, p, a);
+            else hipLaunchKernelGGL((k_contact_g<3, 16>), g, b, 0, s, p, a);
+        }
+        return hipGetLastError();
+    }
+    const dim3 g(n_blocks(p.np));
     if (p.contactmodel == 1)
         hipLaunchKernelGGL(k_contact<1>, g, b, 0, s, p, a);
     else if (p.contactmodel == 3)
@@ -1196,17 +1696,18 @@ hipError_t launch_bonds(const DevParams& p, const DevArrays& a, hipStream_t s) {
 
 hipError_t launch_shear_stress(const DevParams& p, const DevArrays& a, hipStream_t s) {
     hipLaunchKernelGGL(k_shear_stress, dim3(p.nblocks), dim3(BLOCK), 0, s, p, a);
+    hipLaunchKernelGGL(k_tau_sum, dim3(1), dim3(BLOCK), 0, s, p, a);
     return hipGetLastError();
 }
 
-hipError_t launch_integrate(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_integrate, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+hipError_t launch_integrate(const DevParams& p, const DevArrays& a, int hash_next,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_integrate, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a, hash_next);
     return hipGetLastError();
 }
 
 hipError_t launch_walls(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    const uint32_t nb = (p.nw > 0 && p.np > 0) ? p.nblocks : 1u;
-    hipLaunchKernelGGL(k_walls, dim3(nb), dim3(BLOCK), 0, s, p, a);
+    hipLaunchKernelGGL(k_walls, dim3(1), dim3(BLOCK), 0, s, p, a);
     return hipGetLastError();
 }
 

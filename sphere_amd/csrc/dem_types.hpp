@@ -68,20 +68,20 @@ struct DevParams {
 
 // Loop state kept on the device so that captured step sequences can be
 // replayed: time.current, iter, and the rate-state velocity switch
-// (src/device.cu:919,993-999,2454-2462,2536-2538).
+// (src/device.cu:919,993-999,2454-2462,2536-2538).  The per-step flags are
+// double-buffered by iteration parity: step k consumes slot k & 1 while its
+// integration pass already hashes the new positions into slot (k + 1) & 1.
 struct DevLoop {
     double t;
     int64_t iter;
     double grid_Lz;             // L[2] used by the current step (adaptive grid)
     int32_t velocity_state;
-    uint32_t wall_blocks_done;  // arrival counter of the wall reduction
-    uint32_t tau_blocks_done;   // arrival counter of the shear-stress reduction
-    uint32_t cells_changed;     // some particle's cell key changed this step
-    uint32_t key_or;            // OR of all cell keys (pass count check)
     uint32_t error_bits;        // sticky device-side flags (DevFlag)
+    uint32_t changed[2];        // some particle's cell key changed: re-sort
+    uint32_t rebuild[2];        // candidate lists must be rebuilt
+    uint32_t force_rebuild;     // host request: rebuild the lists this step
     uint32_t zero_empty;        // zero delta_t of empty slots this step
     uint32_t bond_retargeted;   // some broken bond now names particle nb0
-    uint32_t vl_rebuild;        // candidate lists must be rebuilt this step
     uint32_t vl_builds;         // number of list rebuilds so far (diagnostic)
     uint32_t sorts;             // number of re-sorts so far (diagnostic)
     uint32_t pad0;

@@ -104,6 +104,15 @@ struct sph_ctx {
     int passes = 4;
     bool shear_stress = false;
     bool profile = false;
+    // The next step starts with a hash launch (keys not yet computed from
+    // the current positions); otherwise the previous integration pass left
+    // them, except with an adaptive grid, which hashes every step.
+    bool hash_pending = true;
+    // Workgroups of the cooperative sort launch (0 = one launch per stage).
+    unsigned coop_grid = 0;
+    bool coop_sort = true;
+    // Work-items per particle in the contact kernel (models 2 and 3).
+    int contact_lanes = 8;
     // host mirror of the loop state
     int64_t iter = 0;
     // per-kernel profile (stage name, total ms, launches)
@@ -502,8 +511,9 @@ int upload_state(sph_ctx* c) {
     L.grid_Lz = h.L[2];
     L.velocity_state = 1;
     L.zero_empty = 1u;
-    L.vl_rebuild = 1u;
+    L.force_rebuild = 1u;
     H2D(A.loop, &L, sizeof L);
+    c->hash_pending = true;
 #undef H2D
     HIPCHK(c, hipStreamSynchronize(s), "upload");
     return SPH_OK;
@@ -557,6 +567,7 @@ int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sp
         return SPH_E_DEVICE;
     }
     fill_params(c);
+    c->coop_grid = sort_coop_grid(c->h.np);
     rc = allocate(c);
     if (rc == SPH_OK) rc = upload_state(c);
     if (rc != SPH_OK) {
@@ -658,17 +669,41 @@ int enqueue_step(sph_ctx* c) {
     hipStream_t s = c->stream;
     Prof pr{c};
     if (P.np > 0) {
-        pr.begin("hash");
-        HIPCHK(c, launch_hash(P, A, s), "hash");
-        pr.end();
-        pr.begin("sort");
-        HIPCHK(c, launch_sort(P, A, c->passes, s), "sort");
-        pr.end();
-        pr.begin("cells");
-        HIPCHK(c, launch_cells(P, A, c->sorted_bits, s), "cells");
-        pr.end();
+        // With an adaptive grid L[2] is the top wall's position at the step
+        // start, known only after the previous step's wall update.
+        const bool hash_in_integrate = !(P.adaptive == 1 && P.nw > 0);
+        if (c->hash_pending || !hash_in_integrate) {
+            pr.begin("hash");
+            HIPCHK(c, launch_hash(P, A, s), "hash");
+            pr.end();
+            c->hash_pending = false;
+        }
+        bool sorted = false;
+        if (c->coop_sort && c->coop_grid) {
+            pr.begin("sort");
+            if (launch_sort_coop(P, A, c->passes, c->sorted_bits, c->coop_grid, s) ==
+                hipSuccess) {
+                sorted = true;
+            } else {
+                // A cooperative launch can be refused at run time (for one,
+                // while another process holds the device's cooperative
+                // resources); nothing was launched, so this step and the
+                // following ones use the per-stage launches.
+                (void)hipGetLastError();
+                c->coop_grid = 0;
+            }
+            pr.end();
+        }
+        if (!sorted) {
+            pr.begin("sort");
+            HIPCHK(c, launch_sort(P, A, c->passes, s), "sort");
+            pr.end();
+            pr.begin("cells");
+            HIPCHK(c, launch_cells(P, A, c->sorted_bits, s), "cells");
+            pr.end();
+        }
         pr.begin("contact");
-        HIPCHK(c, launch_contact(P, A, s), "contact");
+        HIPCHK(c, launch_contact(P, A, c->contact_lanes, s), "contact");
         pr.end();
         if (P.nb0 > 0) {
             pr.begin("bonds");
@@ -681,7 +716,7 @@ int enqueue_step(sph_ctx* c) {
             pr.end();
         }
         pr.begin("integrate");
-        HIPCHK(c, launch_integrate(P, A, s), "integrate");
+        HIPCHK(c, launch_integrate(P, A, hash_in_integrate ? 1 : 0, s), "integrate");
         pr.end();
     }
     pr.begin("walls");
@@ -1040,8 +1075,9 @@ int sph_set_field(sph_ctx* c, int f, const void* src, size_t bytes) {
             int rc = h2d(c, A.x, src, bytes);
             if (rc) return rc;
             HIPCHK(c, hipMemsetAsync(A.key_prev, 0xff, np * 4, c->stream), "memset");
+            c->hash_pending = true;
             const uint32_t one = 1u;
-            return h2d(c, &A.loop->vl_rebuild, &one, 4);
+            return h2d(c, &A.loop->force_rebuild, &one, 4);
         }
         case SPH_F_VEL: return h2d(c, A.vel, src, bytes);
         case SPH_F_ANGVEL: return set_vec3(c, A.angvel, np, src);
@@ -1150,6 +1186,8 @@ int sph_get_scalar(sph_ctx* c, int which, double* v) {
             *v = which == SPH_S_SORTS ? (double)L.sorts : (double)L.vl_builds;
             return SPH_OK;
         }
+        case SPH_S_COOP_SORT: *v = (c->coop_sort && c->coop_grid) ? 1.0 : 0.0; return SPH_OK;
+        case SPH_S_CONTACT_LANES: *v = c->contact_lanes; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1165,16 +1203,24 @@ int sph_set_scalar(sph_ctx* c, int which, double v) {
         case SPH_S_DT: c->h.dt = v; c->P.dt = v; return SPH_OK;
         case SPH_S_FILE_DT: c->h.file_dt = v; return SPH_OK;
         case SPH_S_ITER: {
+            // The per-step flags are kept by iteration parity: hash again.
             const int64_t it = (int64_t)v;
             c->iter = it;
+            c->hash_pending = true;
             return h2d(c, &c->A.loop->iter, &it, 8);
         }
         case SPH_S_STEP_COUNT: c->h.step_count = (uint32_t)v; return SPH_OK;
         case SPH_S_VL_SKIN: {
             set_skin(c, v);
             const uint32_t one = 1u;
-            return h2d(c, &c->A.loop->vl_rebuild, &one, 4);
+            return h2d(c, &c->A.loop->force_rebuild, &one, 4);
         }
+        case SPH_S_COOP_SORT: c->coop_sort = v != 0.0; return SPH_OK;
+        case SPH_S_CONTACT_LANES:
+            if (v != 1.0 && v != 4.0 && v != 8.0 && v != 16.0)
+                return fail(c, SPH_E_ARG, "contact lanes must be 1, 4, 8 or 16 (got %g)", v);
+            c->contact_lanes = (int)v;
+            return SPH_OK;
         default: return fail(c, SPH_E_ARG, "scalar %d is read-only", which);
     }
 }

@@ -1,5 +1,6 @@
 """Every scene the GPU tests use is built (and stepped on the oracle) here."""
 
+import numpy
 import pytest
 
 import oracle
@@ -62,3 +63,26 @@ def test_wmode3_adaptive_rate_state_cases_build():
     s = cases.fixture("shear1k.bin")
     top = s.fixvel > 0
     assert top.any()
+
+
+def test_crowded_case_overflows_contact_row():
+    s = cases.crowded_case()
+    n = int(s.np[0])
+    o = oracle.Oracle(s.tobytes())
+    o.step(3)
+    row = o.get("contacts")[0]
+    assert (row != n).all()  # all NC slots of the large sphere are taken
+    d = numpy.linalg.norm(s.x[1:] - s.x[0], axis=1) - (s.radius[1:] + s.radius[0])
+    assert (d < 0).sum() > 32
+
+
+def test_capillary_case_builds_and_bridges():
+    s = cases.fixture("shear1k.bin")
+    s.contactmodel[0] = 1
+    s.kappa[0] = 0.5
+    s.db[0] = 2.0e-4
+    s.V_b[0] = 1.0e-9
+    o = oracle.Oracle(s.tobytes())
+    o.step(2)
+    f = o.get("force")
+    assert numpy.isfinite(f).all()

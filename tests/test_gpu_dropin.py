@@ -84,7 +84,6 @@ def test_run_outputs_match_oracle(tmp_path):
     assert s.run(verbose=False, root=str(tmp_path)) == 0
     out = tmp_path / "output"
     last = status("runcheck", root=str(tmp_path))
-    assert last >= 3
     line = (out / "runcheck.status.dat").read_text().split()
     assert len(line) == 3 and int(line[2]) == last
     first = (out / "runcheck.output00000.bin").read_bytes()
@@ -102,6 +101,8 @@ def test_run_outputs_match_oracle(tmp_path):
             nfile += 1
             files[nfile] = ref.tobytes()
             ftc = 0.0
+    # The replay counts files with the host loop's own double additions.
+    assert len(files) == last >= 2
     for i, img in files.items():
         got = (out / ("runcheck.output%05d.bin" % i)).read_bytes()
         a = sim().frombytes(got)
