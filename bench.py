@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--contact-lanes", type=int, default=0,
                     help="work-items per particle in the contact kernel (1, 4, 8, 16)")
+    ap.add_argument("--sort-mode", type=int, default=-1,
+                    help="0: sort launches every step, 1: end-of-step one-workgroup "
+                         "re-sort, 2: automatic (default)")
     ap.add_argument("--per-stage-sort", action="store_true",
                     help="re-sort with one launch per stage instead of one cooperative launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -123,7 +126,7 @@ def main():
         src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         for _ in range(5):
-            dst.copy_(src)
+            torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads and writes nbytes
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
                           "launches": 5}), flush=True)
@@ -144,6 +147,8 @@ def main():
         ctx.set_scalar("coop_sort", 0)
     if args.contact_lanes:
         ctx.set_scalar("contact_lanes", args.contact_lanes)
+    if args.sort_mode >= 0:
+        ctx.set_scalar("sort_mode", args.sort_mode)
     n = int(ctx.scalar("np"))
     nw = int(ctx.scalar("nw"))
     dt = ctx.scalar("dt")
@@ -164,9 +169,9 @@ def main():
     sorts = int(ctx.scalar("sorts") - sorts0)
     builds = int(ctx.scalar("vl_builds") - builds0)
     log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f, "
-        "%d re-sorts, %d list rebuilds, cooperative sort %d, contact lanes %d)"
+        "%d re-sorts, %d list rebuilds, cooperative sort %d, contact lanes %d, sort mode %d)"
         % (args.steps, elapsed, value, z, z_end, sorts, builds, int(ctx.scalar("coop_sort")),
-           int(ctx.scalar("contact_lanes"))))
+           int(ctx.scalar("contact_lanes")), int(ctx.scalar("sort_mode"))))
 
     # Kernel timing pass with HIP events on the context stream.
     ctx.profile(True)

@@ -103,12 +103,19 @@ enum sph_scalar {
     SPH_S_VL_SKIN = 10,    /* candidate-list skin (m); 0 = scan 27 cells every step */
     SPH_S_SORTS = 11,      /* re-sorts performed so far (diagnostic, read-only)      */
     SPH_S_VL_BUILDS = 12,  /* candidate-list rebuilds so far (diagnostic, read-only) */
-    SPH_S_COOP_SORT = 13,  /* 1: re-sort in one cooperative launch (default where the
-                              device supports it); 0: one launch per sort stage.
+    SPH_S_COOP_SORT = 13,  /* 1: re-sort in one cooperative launch where the device
+                              allows it; 0 (default): one launch per sort stage.
                               Results are identical either way. */
-    SPH_S_CONTACT_LANES = 14 /* work-items per particle in the contact kernel for
-                                contact models 2 and 3: 1, 4, 8 (default) or 16.
-                                Results are identical for every value. */
+    SPH_S_CONTACT_LANES = 14, /* work-items per particle in the contact kernel for
+                                 contact models 2 and 3: 1, 4 (default), 8 or 16.
+                                 Results are identical for every value. */
+    SPH_S_SORT_MODE = 15,   /* where the re-sort runs: 0 = sort launches at each step
+                               start, 1 = one workgroup at the end of the previous step
+                               when some particle changed cell, 2 (default) = chosen
+                               from the measured re-sort rate.  Identical results. */
+    SPH_S_RESORT_MERGE_MAX = 16 /* particles changing cell up to which the end-of-step
+                                   re-sort merges them into the previous order
+                                   (0..2048, default 2048); above it, it sorts anew. */
 };
 
 /* Open flags. */

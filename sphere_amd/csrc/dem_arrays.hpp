@@ -40,6 +40,7 @@ struct DevArrays {
     uint32_t* cell_end;
     double4* xs;         // sorted copy of x
     uint32_t* rank;      // original index -> sorted index (inverse of perm)
+    uint32_t* movers;    // MV_CAP original indices whose key changed (unordered)
     // candidate lists (sorted-index rows of NV entries)
     uint2* vlist;
     uint32_t* vcount;    // entries per row, or VL_OVERFLOW

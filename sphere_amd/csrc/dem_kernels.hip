@@ -1439,7 +1439,22 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const De
         const double4 x_new = integrate_particle(p, a, idx);
         if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
     }
-    if (hash_next) publish_hash(a.loop, step_slot(a.loop) ^ 1u, f);
+    if (hash_next) {
+        publish_hash(a.loop, step_slot(a.loop) ^ 1u, f);
+        // List the particles that changed cell for the end-of-step re-sort
+        // (one atomic per wavefront; the list is unordered, the count keeps
+        // growing past MV_CAP so that an overfull list is recognised).
+        const unsigned long long mv = __ballot(f.changed);
+        if (mv) {
+            const uint32_t lane = threadIdx.x & 63u;
+            uint32_t base = 0;
+            if (lane == 0) base = atomicAdd(&a.loop->nmovers, (uint32_t)__popcll(mv));
+            base = (uint32_t)__shfl((int)base, 0, 64);
+            const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+            const uint32_t pos = base + (uint32_t)__popcll(mv & lt);
+            if (f.changed && pos < MV_CAP) a.movers[pos] = idx;
+        }
+    }
     if (p.nw > 0) {
         const double v = idx < p.np ? a.force_pp[idx] : 0.0;
         const double s = block_tree_sum(v, cache);
@@ -1447,15 +1462,163 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const De
     }
 }
 
+// ---------------------------------------------------------------------------
+// The re-sort for the next step inside one workgroup (run at the end of a
+// step by k_walls).  When few particles changed cell, the new stable order
+// of (cell key, original index) is the old order with those particles taken
+// out and merged back at their new keys: the movers are sorted in LDS, each
+// one's place among the unchanged particles is a binary search in the old
+// order, and every other position copies from the old order at a known
+// shift.  Otherwise the workgroup runs the radix passes itself.  Either way
+// the result is the same permutation the per-stage sort produces, followed
+// by the same cell pass.
+
+template <typename T>
+__device__ __forceinline__ void bitonic_sort_lds(T* v, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < n; i += BLOCK) {
+                const uint32_t ixj = i ^ j;
+                if (ixj > i) {
+                    const T x = v[i], y = v[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((x > y) == up) {
+                        v[i] = y;
+                        v[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Number of entries of the ascending v[0, n) that are < x.
+template <typename T>
+__device__ __forceinline__ uint32_t count_below(const T* v, uint32_t n, T x) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (v[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ unsigned long long composite(uint32_t key, uint32_t kmask,
+                                                        uint32_t idx) {
+    return ((unsigned long long)(key & kmask) << 32) | idx;
+}
+
+__device__ void resort_one_block(const DevParams& p, const DevArrays& a, int passes,
+                                 int sorted_bits) {
+    __shared__ unsigned long long comp[MV_CAP];  // movers' (new key, index), ascending
+    __shared__ uint32_t rk[MV_CAP];              // movers' old sorted positions, ascending
+    __shared__ uint32_t npos[MV_CAP];            // movers' new sorted positions
+    const uint32_t tid = threadIdx.x;
+    const uint32_t np = p.np;
+    const uint32_t M = a.loop->nmovers;
+    // The radix sort orders by the low 8 * passes bits of the key.
+    const uint32_t kmask = passes >= 4 ? 0xffffffffu : ((1u << (8 * passes)) - 1u);
+    const bool merge = M <= p.mv_max && M <= MV_CAP;
+    if (merge) {
+        uint32_t n2 = 1;
+        while (n2 < M) n2 <<= 1;
+        for (uint32_t m = tid; m < n2; m += BLOCK) {
+            if (m < M) {
+                const uint32_t o = a.movers[m];
+                comp[m] = composite(a.key[o], kmask, o);
+                rk[m] = a.rank[o];
+            } else {
+                comp[m] = ~0ull;
+                rk[m] = 0xffffffffu;
+            }
+        }
+        __syncthreads();
+        bitonic_sort_lds(comp, n2);
+        bitonic_sort_lds(rk, n2);
+        // A mover's new position: the unchanged particles before it (old
+        // elements below its composite, minus the movers among them) plus
+        // the movers before it.
+        for (uint32_t m = tid; m < M; m += BLOCK) {
+            const unsigned long long c = comp[m];
+            uint32_t lo = 0, hi = np;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (composite(a.skey[mid], kmask, a.perm[mid]) < c) lo = mid + 1; else hi = mid;
+            }
+            npos[m] = lo - count_below(rk, M, lo) + m;
+        }
+        __syncthreads();
+        // New positions, a contiguous chunk per work-item.  The n-th
+        // unchanged particle sits at old position n + d, d = the number of
+        // movers with rk[d'] - d' <= n.
+        const uint32_t chunk = (np + BLOCK - 1) / BLOCK;
+        const uint32_t j0 = tid * chunk;
+        const uint32_t j1 = j0 + chunk < np ? j0 + chunk : np;
+        if (j0 < j1) {
+            uint32_t e = count_below(npos, M, j0);
+            uint32_t n = j0 - e;
+            uint32_t lo = 0, hi = M;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (rk[mid] - mid <= n) lo = mid + 1; else hi = mid;
+            }
+            uint32_t d = lo;
+            for (uint32_t j = j0; j < j1; ++j) {
+                if (e < M && npos[e] == j) {
+                    const uint32_t o = (uint32_t)(comp[e] & 0xffffffffull);
+                    a.key_tmp[j] = a.key[o];
+                    a.val_tmp[j] = o;
+                    ++e;
+                } else {
+                    while (d < M && rk[d] - d <= n) ++d;
+                    const uint32_t i = n + d;
+                    a.key_tmp[j] = a.skey[i];
+                    a.val_tmp[j] = a.perm[i];
+                    ++n;
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t j = tid; j < np; j += BLOCK) {
+            a.skey[j] = a.key_tmp[j];
+            a.perm[j] = a.val_tmp[j];
+        }
+    } else {
+        const uint32_t nt = n_tiles(np);
+        for (int pass = 0; pass < passes; ++pass) {
+            const RadixPass r = radix_pass(a, pass, passes);
+            const int shift = 8 * pass;
+            for (uint32_t t = 0; t < nt; ++t) radix_hist_tile(r.kin, np, shift, a.rhist, nt, t);
+            __syncthreads();
+            radix_scan_block<BLOCK>(a.rhist, 256u * nt);
+            for (uint32_t t = 0; t < nt; ++t)
+                radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, np, shift, a.rhist, nt, t);
+            __syncthreads();
+        }
+    }
+    for (uint32_t c = tid; c < p.ncells; c += BLOCK) a.cell_start[c] = 0xffffffffu;
+    __syncthreads();
+    for (uint32_t j = tid; j < np; j += BLOCK) cells_one(p, a, j, sorted_bits);
+    __syncthreads();
+}
+
 // integrateWalls + the loop advance (src/integration.cuh:344-450,
 // src/device.cu:2454-2462,2500-2538), one workgroup.  Every dynamic wall
-// integrates wall 0's force sum (src/device.cu:2504-2505).
-__global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArrays a) {
+// integrates wall 0's force sum (src/device.cu:2504-2505).  With `resort`
+// set it first re-sorts for the next step when the integration pass found
+// particles that changed cell (the next step then launches no sort).
+__global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArrays a,
+                                                 int resort, int passes, int sorted_bits) {
     __shared__ double stage[2048];
+    __shared__ uint32_t do_sort;
+    DevLoop* lp = a.loop;
+    if (threadIdx.x == 0) do_sort = (resort && lp->changed[step_slot(lp) ^ 1u]) ? 1u : 0u;
     const bool walls = p.nw > 0 && p.np > 0;
     const double fsum = walls ? serial_sum(a.partial, p.nblocks, stage) : 0.0;
+    __syncthreads();
+    if (do_sort) resort_one_block(p, a, passes, sorted_bits);
     if (threadIdx.x != 0) return;
-    DevLoop* lp = a.loop;
+    if (do_sort) lp->sorts += 1u;
     const double t_now = lp->t;
     const double dt = p.dt;
     if (p.adaptive == 1 && p.nw > 0 && p.np > 0) lp->grid_Lz = a.w_nx[0].w;
@@ -1488,6 +1651,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
     lp->rebuild[slot] = 0u;
     lp->force_rebuild = 0u;
     lp->zero_empty = 0u;
+    lp->nmovers = 0u;
     lp->t = t_now + dt;
     lp->iter = lp->iter + 1;
 }
@@ -1706,8 +1870,9 @@ hipError_t launch_integrate(const DevParams& p, const DevArrays& a, int hash_nex
     return hipGetLastError();
 }
 
-hipError_t launch_walls(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_walls, dim3(1), dim3(BLOCK), 0, s, p, a);
+hipError_t launch_walls(const DevParams& p, const DevArrays& a, int resort, int passes,
+                        int sorted_bits, hipStream_t s) {
+    hipLaunchKernelGGL(k_walls, dim3(1), dim3(BLOCK), 0, s, p, a, resort, passes, sorted_bits);
     return hipGetLastError();
 }
 

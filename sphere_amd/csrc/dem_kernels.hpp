@@ -49,7 +49,10 @@ hipError_t launch_integrate(const DevParams& p, const DevArrays& a, int hash_nex
                             hipStream_t s);
 // Serial sum of the partials + integrateWalls + loop advance
 // (src/integration.cuh:344-450, src/device.cu:2500-2538); also with np == 0.
-hipError_t launch_walls(const DevParams& p, const DevArrays& a, hipStream_t s);
+// With resort != 0 it also re-sorts for the next step, in one workgroup,
+// when the integration pass (launched with hash_next) found a cell change.
+hipError_t launch_walls(const DevParams& p, const DevArrays& a, int resort, int passes,
+                        int sorted_bits, hipStream_t s);
 // Fill n words with v.
 hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s);
 

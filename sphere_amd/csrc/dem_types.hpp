@@ -32,6 +32,10 @@ constexpr uint32_t VL_OVERFLOW = 0xffffffffu;
 constexpr uint32_t VL_IB_MASK = (1u << 27) - 1u;
 constexpr uint32_t VL_CELL_SHIFT = 27;
 
+// Particles that changed cell, listed by the integration pass for the
+// one-workgroup re-sort at the end of the step.
+constexpr uint32_t MV_CAP = 2048;
+
 struct DevParams {
     // grid (src/datatypes.h:45-51)
     double origo[3];
@@ -63,7 +67,10 @@ struct DevParams {
     double vl_skin;            // list radius margin (m); 0 disables the lists
     double vl_trigger2;        // squared displacement that forces a rebuild
     int32_t vl_enabled;
-    int32_t pad1;
+    // Largest number of particles changing cell that the one-workgroup
+    // re-sort merges into the previous order (<= MV_CAP); beyond it, it
+    // sorts from scratch.
+    uint32_t mv_max;
 };
 
 // Loop state kept on the device so that captured step sequences can be
@@ -84,7 +91,7 @@ struct DevLoop {
     uint32_t bond_retargeted;   // some broken bond now names particle nb0
     uint32_t vl_builds;         // number of list rebuilds so far (diagnostic)
     uint32_t sorts;             // number of re-sorts so far (diagnostic)
-    uint32_t pad0;
+    uint32_t nmovers;           // particles whose key changed for the next step
 };
 
 enum DevFlag : uint32_t {

@@ -108,11 +108,26 @@ struct sph_ctx {
     // the current positions); otherwise the previous integration pass left
     // them, except with an adaptive grid, which hashes every step.
     bool hash_pending = true;
-    // Workgroups of the cooperative sort launch (0 = one launch per stage).
+    // Workgroups of the cooperative sort launch (0 = unavailable).  Off by
+    // default: on MI355X a cooperative launch costs about as much as the
+    // seven ordinary launches it replaces.
     unsigned coop_grid = 0;
-    bool coop_sort = true;
+    bool coop_sort = false;
     // Work-items per particle in the contact kernel (models 2 and 3).
-    int contact_lanes = 8;
+    int contact_lanes = 4;
+    // Where a step's re-sort runs: 0 = the per-stage sort launches at the
+    // step start (every step; they return at once when nothing changed),
+    // 1 = one workgroup at the end of the previous step, only when needed,
+    // 2 = choose between the two from the measured re-sort rate.
+    int sort_mode = 2;
+    bool resort_at_end = true;     // the choice for the steps being enqueued
+    bool prev_walls_resorted = false;
+    // Asynchronous sample of the loop counters for the automatic choice.
+    DevLoop* loop_sample = nullptr;   // pinned host memory
+    hipEvent_t sample_ev = nullptr;
+    bool sample_pending = false;
+    int64_t sample_iter0 = 0;
+    uint32_t sample_sorts0 = 0;
     // host mirror of the loop state
     int64_t iter = 0;
     // per-kernel profile (stage name, total ms, launches)
@@ -368,6 +383,7 @@ void fill_params(sph_ctx* c) {
     c->sorted_bits = std::max(1, bit_length(kmax));
     if (c->sorted_bits > 32) c->sorted_bits = 32;
     c->passes = (c->sorted_bits + 7) / 8;
+    P.mv_max = MV_CAP;
     c->shear_stress = h.nw > 0 && h.wmode[0] == 3;
     // Candidate lists: skin of 0.1 r_min by default.  Model 1 with a
     // capillary debonding distance keeps the full scan (its interaction
@@ -411,6 +427,7 @@ int allocate(sph_ctx* c) {
     ALLOC(A.vlist, np * NV);
     ALLOC(A.vcount, np);
     ALLOC(A.x_build, np);
+    ALLOC(A.movers, MV_CAP);
     ALLOC(A.crow, np * NC);
     ALLOC(A.delta_t, np * NC);
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
@@ -514,6 +531,7 @@ int upload_state(sph_ctx* c) {
     L.force_rebuild = 1u;
     H2D(A.loop, &L, sizeof L);
     c->hash_pending = true;
+    c->prev_walls_resorted = false;
 #undef H2D
     HIPCHK(c, hipStreamSynchronize(s), "upload");
     return SPH_OK;
@@ -568,6 +586,15 @@ int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sp
     }
     fill_params(c);
     c->coop_grid = sort_coop_grid(c->h.np);
+    if (hipHostMalloc((void**)&c->loop_sample, sizeof(DevLoop), hipHostMallocDefault) !=
+            hipSuccess ||
+        hipEventCreateWithFlags(&c->sample_ev, hipEventDisableTiming) != hipSuccess) {
+        // Without the sample the automatic placement keeps its first choice.
+        (void)hipGetLastError();
+        if (c->loop_sample) (void)hipHostFree(c->loop_sample);
+        c->loop_sample = nullptr;
+        c->sample_ev = nullptr;
+    }
     rc = allocate(c);
     if (rc == SPH_OK) rc = upload_state(c);
     if (rc != SPH_OK) {
@@ -663,23 +690,57 @@ void prof_collect(sph_ctx* c) {
     ps.pending.clear();
 }
 
+// Re-sort placement for the steps about to be enqueued.  In automatic mode
+// the choice follows the re-sort rate seen over the previous sph_step call,
+// read from an asynchronous copy of the loop counters (no synchronisation):
+// the end-of-step workgroup wins while fewer than one step in ten re-sorts.
+void choose_sort_placement(sph_ctx* c) {
+    if (c->sort_mode != 2) {
+        c->resort_at_end = c->sort_mode == 1;
+        return;
+    }
+    if (!c->sample_pending || !c->loop_sample) return;
+    if (hipEventQuery(c->sample_ev) != hipSuccess) return;
+    c->sample_pending = false;
+    const DevLoop& L = *c->loop_sample;
+    const int64_t steps = L.iter - c->sample_iter0;
+    const uint32_t sorts = L.sorts - c->sample_sorts0;
+    if (steps >= 20) c->resort_at_end = (double)sorts < 0.1 * (double)steps;
+    c->sample_iter0 = L.iter;
+    c->sample_sorts0 = L.sorts;
+}
+
+void sample_loop(sph_ctx* c) {
+    if (c->sort_mode != 2 || !c->loop_sample || c->sample_pending) return;
+    if (hipMemcpyAsync(c->loop_sample, c->A.loop, sizeof(DevLoop), hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess ||
+        hipEventRecord(c->sample_ev, c->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return;
+    }
+    c->sample_pending = true;
+}
+
 int enqueue_step(sph_ctx* c) {
     const DevParams& P = c->P;
     const DevArrays& A = c->A;
     hipStream_t s = c->stream;
     Prof pr{c};
+    // With an adaptive grid L[2] is the top wall's position at the step
+    // start, known only after the previous step's wall update.
+    const bool hash_in_integrate = !(P.adaptive == 1 && P.nw > 0);
     if (P.np > 0) {
-        // With an adaptive grid L[2] is the top wall's position at the step
-        // start, known only after the previous step's wall update.
-        const bool hash_in_integrate = !(P.adaptive == 1 && P.nw > 0);
-        if (c->hash_pending || !hash_in_integrate) {
+        const bool hash_now = c->hash_pending || !hash_in_integrate;
+        if (hash_now) {
             pr.begin("hash");
             HIPCHK(c, launch_hash(P, A, s), "hash");
             pr.end();
             c->hash_pending = false;
         }
-        bool sorted = false;
-        if (c->coop_sort && c->coop_grid) {
+        // The previous step's walls kernel already re-sorted for this step
+        // unless this step hashes for itself.
+        bool sorted = c->prev_walls_resorted && !hash_now;
+        if (!sorted && c->coop_sort && c->coop_grid) {
             pr.begin("sort");
             if (launch_sort_coop(P, A, c->passes, c->sorted_bits, c->coop_grid, s) ==
                 hipSuccess) {
@@ -719,9 +780,11 @@ int enqueue_step(sph_ctx* c) {
         HIPCHK(c, launch_integrate(P, A, hash_in_integrate ? 1 : 0, s), "integrate");
         pr.end();
     }
+    const bool resort = P.np > 0 && hash_in_integrate && c->resort_at_end;
     pr.begin("walls");
-    HIPCHK(c, launch_walls(P, A, s), "walls");
+    HIPCHK(c, launch_walls(P, A, resort ? 1 : 0, c->passes, c->sorted_bits, s), "walls");
     pr.end();
+    c->prev_walls_resorted = resort;
     // Host mirror of the loop counters (src/device.cu:2536-2538).
     c->h.t += c->h.dt;
     c->iter += 1;
@@ -894,10 +957,12 @@ int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out) {
 
 int sph_step(sph_ctx* c, uint64_t nsteps) {
     if (!c) return SPH_E_ARG;
+    choose_sort_placement(c);
     for (uint64_t k = 0; k < nsteps; ++k) {
         int rc = enqueue_step(c);
         if (rc) return rc;
     }
+    sample_loop(c);
     return SPH_OK;
 }
 
@@ -1188,6 +1253,8 @@ int sph_get_scalar(sph_ctx* c, int which, double* v) {
         }
         case SPH_S_COOP_SORT: *v = (c->coop_sort && c->coop_grid) ? 1.0 : 0.0; return SPH_OK;
         case SPH_S_CONTACT_LANES: *v = c->contact_lanes; return SPH_OK;
+        case SPH_S_SORT_MODE: *v = c->sort_mode; return SPH_OK;
+        case SPH_S_RESORT_MERGE_MAX: *v = c->P.mv_max; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1220,6 +1287,17 @@ int sph_set_scalar(sph_ctx* c, int which, double v) {
             if (v != 1.0 && v != 4.0 && v != 8.0 && v != 16.0)
                 return fail(c, SPH_E_ARG, "contact lanes must be 1, 4, 8 or 16 (got %g)", v);
             c->contact_lanes = (int)v;
+            return SPH_OK;
+        case SPH_S_SORT_MODE:
+            if (v != 0.0 && v != 1.0 && v != 2.0)
+                return fail(c, SPH_E_ARG, "sort mode must be 0, 1 or 2 (got %g)", v);
+            c->sort_mode = (int)v;
+            if (c->sort_mode != 2) c->resort_at_end = c->sort_mode == 1;
+            return SPH_OK;
+        case SPH_S_RESORT_MERGE_MAX:
+            if (!(v >= 0.0 && v <= (double)MV_CAP))
+                return fail(c, SPH_E_ARG, "merge limit must be in [0, %u] (got %g)", MV_CAP, v);
+            c->P.mv_max = (uint32_t)v;
             return SPH_OK;
         default: return fail(c, SPH_E_ARG, "scalar %d is read-only", which);
     }
@@ -1274,6 +1352,8 @@ int sph_profile_report(sph_ctx* c, char* buf, size_t cap) {
 void sph_close(sph_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->sample_ev) (void)hipEventDestroy(c->sample_ev);
+    if (c->loop_sample) (void)hipHostFree(c->loop_sample);
     if (c->prof) {
         prof_collect(c);
         for (hipEvent_t e : c->prof->pool) (void)hipEventDestroy(e);

@@ -193,3 +193,30 @@ def test_contact_lanes_bit_exact(case, lanes):
         ctx.step(steps // 2)
     assert _compare(ctx, ref) == []
     ctx.close()
+
+
+@pytest.mark.parametrize("mode,merge_max", [(0, 2048), (1, 2048), (1, 3), (1, 0), (2, 2048)])
+@pytest.mark.parametrize("case", ["settling", "shear1k"])
+def test_resort_placement_bit_exact(case, mode, merge_max):
+    """The re-sort at each step start (mode 0), in one workgroup at the end of
+    the previous step (mode 1: merging the particles that changed cell into
+    the old order, or sorting anew when more than merge_max changed), and the
+    automatic choice (mode 2) all give the oracle's order and state."""
+    if case == "settling":
+        s, steps = cases.settling_case(1200, 12), 240
+    else:
+        s = cases.fixture("shear1k.bin")
+        s.vel[:, :2] += 5.0  # a uniform drift: particles keep crossing cell faces
+        steps = 240
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    ctx.set_scalar("sort_mode", mode)
+    ctx.set_scalar("resort_merge_max", merge_max)
+    for _ in range(4):
+        ref.step(steps // 4)
+        ctx.step(steps // 4)
+    assert ctx.scalar("sorts") > 2
+    assert _compare(ctx, ref) == []
+    _occupied_cells_equal(ctx, ref)
+    ctx.close()
