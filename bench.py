@@ -111,8 +111,6 @@ def main():
     ap.add_argument("--sort-mode", type=int, default=-1,
                     help="0: sort launches every step, 1: end-of-step one-workgroup "
                          "re-sort, 2: automatic (default)")
-    ap.add_argument("--per-stage-sort", action="store_true",
-                    help="re-sort with one launch per stage instead of one cooperative launch")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the "
                          "reference kernel for scaling HBM byte counters")
@@ -143,8 +141,6 @@ def main():
 
     img = build_workload(args)
     ctx = engine.Context(img)
-    if args.per_stage_sort:
-        ctx.set_scalar("coop_sort", 0)
     if args.contact_lanes:
         ctx.set_scalar("contact_lanes", args.contact_lanes)
     if args.sort_mode >= 0:
@@ -169,8 +165,8 @@ def main():
     sorts = int(ctx.scalar("sorts") - sorts0)
     builds = int(ctx.scalar("vl_builds") - builds0)
     log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f, "
-        "%d re-sorts, %d list rebuilds, cooperative sort %d, contact lanes %d, sort mode %d)"
-        % (args.steps, elapsed, value, z, z_end, sorts, builds, int(ctx.scalar("coop_sort")),
+        "%d re-sorts, %d list rebuilds, contact lanes %d, sort mode %d)"
+        % (args.steps, elapsed, value, z, z_end, sorts, builds,
            int(ctx.scalar("contact_lanes")), int(ctx.scalar("sort_mode"))))
 
     # Kernel timing pass with HIP events on the context stream.

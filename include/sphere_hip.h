@@ -103,9 +103,7 @@ enum sph_scalar {
     SPH_S_VL_SKIN = 10,    /* candidate-list skin (m); 0 = scan 27 cells every step */
     SPH_S_SORTS = 11,      /* re-sorts performed so far (diagnostic, read-only)      */
     SPH_S_VL_BUILDS = 12,  /* candidate-list rebuilds so far (diagnostic, read-only) */
-    SPH_S_COOP_SORT = 13,  /* 1: re-sort in one cooperative launch where the device
-                              allows it; 0 (default): one launch per sort stage.
-                              Results are identical either way. */
+    SPH_S_NSIMS = 13,      /* simulations in the context (read-only) */
     SPH_S_CONTACT_LANES = 14, /* work-items per particle in the contact kernel for
                                  contact models 2 and 3: 1, 4 (default), 8 or 16.
                                  Results are identical for every value. */
@@ -131,6 +129,19 @@ int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out);
 int sph_open_mem(const void* bytes, size_t nbytes, int device, unsigned flags,
                  sph_ctx** out);
 
+/* Batched contexts (no counterpart in the reference, whose binary runs one
+ * simulation per process): nsims independent simulations, each from its own
+ * .bin (own np, grid, walls, bonds, parameters and dt; one contact model for
+ * all), stepped together by the same kernel launches.  Simulation k's
+ * results are identical to opening its .bin alone.  The plain entry points
+ * act on simulation 0; the *_sim variants take the simulation index.  ITER
+ * and the performance scalars apply to the whole context. */
+int sph_open_batch(const char* const* bin_paths, int nsims, int device, unsigned flags,
+                   sph_ctx** out);
+int sph_open_batch_mem(const void* const* images, const size_t* sizes, int nsims, int device,
+                       unsigned flags, sph_ctx** out);
+int sph_num_sims(const sph_ctx* ctx);
+
 /* Run nsteps iterations of the fluid-free loop body of DEM::startTime,
  * src/device.cu:1031-1200,2423-2543, without file output and without the
  * t <= t_total test.  Asynchronous w.r.t. the host until a get/sync call. */
@@ -145,9 +156,11 @@ int sph_run(sph_ctx* ctx, const char* out_dir, const char* sid);
 
 /* DEM::writebin (src/file_io.cpp:420-731) of the current state. */
 int sph_write_bin(sph_ctx* ctx, const char* path);
+int sph_write_bin_sim(sph_ctx* ctx, int sim, const char* path);
 /* Same, to memory.  Returns the image size in *nbytes; writes only when
  * cap >= that size (call with dst = NULL to query). */
 int sph_write_mem(sph_ctx* ctx, void* dst, size_t cap, size_t* nbytes);
+int sph_write_mem_sim(sph_ctx* ctx, int sim, void* dst, size_t cap, size_t* nbytes);
 
 /* D2H / H2D of one field (layouts above); bytes must equal the field size. */
 int sph_get_field(sph_ctx* ctx, int field, void* host_dst, size_t bytes);
@@ -155,6 +168,11 @@ int sph_set_field(sph_ctx* ctx, int field, const void* host_src, size_t bytes);
 int64_t sph_field_bytes(sph_ctx* ctx, int field);
 int sph_get_scalar(sph_ctx* ctx, int which, double* value);
 int sph_set_scalar(sph_ctx* ctx, int which, double value);
+int sph_get_field_sim(sph_ctx* ctx, int sim, int field, void* host_dst, size_t bytes);
+int sph_set_field_sim(sph_ctx* ctx, int sim, int field, const void* host_src, size_t bytes);
+int64_t sph_field_bytes_sim(sph_ctx* ctx, int sim, int field);
+int sph_get_scalar_sim(sph_ctx* ctx, int sim, int which, double* value);
+int sph_set_scalar_sim(sph_ctx* ctx, int sim, int which, double value);
 
 /* Bit-exact check hooks (original-index keyed, as the reference's
  * dev_contacts / dev_delta_t / dev_distmod, src/contactsearch.cuh:273-291). */

@@ -12,7 +12,21 @@
 
 namespace sphx {
 
+// All simulations of a context share these arrays: simulation s owns the
+// particle range [ptab[s].base, ptab[s].base + ptab[s].npad), which is a
+// whole number of 256-particle chunks, its walls, bonds and cells.
 struct DevArrays {
+    // batch layout
+    const DevParams* ptab;      // per-simulation parameters
+    const uint32_t* chunk_sim;  // simulation of each 256-particle chunk
+    const uint32_t* bond_sim;   // simulation of each bond
+    DevGlobal* glob;
+    uint32_t nsim;
+    uint32_t ntot;              // particles including padding (= 256 * nchunk)
+    uint32_t nchunk;
+    uint32_t nbond;
+    uint32_t nkeys;             // global cell keys (length of cell_start/cell_end)
+    uint32_t pad0;
     // kinematics, 32 B per particle each (x,y,z,w)
     double4* x;        // position, radius in w
     double4* vel;      // velocity, fixvel in w
@@ -38,9 +52,11 @@ struct DevArrays {
     uint32_t* rhist;     // radix digit histograms, 256 x tiles
     uint32_t* cell_start;
     uint32_t* cell_end;
-    double4* xs;         // sorted copy of x
+    double4* xs;         // sorted copies of x, vel and angvel: the contact kernel
+    double4* vs;         // reads its own and its partners' state in sorted order,
+    double4* ws;         // which keeps those gathers local
     uint32_t* rank;      // original index -> sorted index (inverse of perm)
-    uint32_t* movers;    // MV_CAP original indices whose key changed (unordered)
+    uint32_t* movers;    // per simulation, MV_CAP indices whose key changed (unordered)
     // candidate lists (sorted-index rows of NV entries)
     uint2* vlist;
     uint32_t* vcount;    // entries per row, or VL_OVERFLOW
@@ -52,11 +68,11 @@ struct DevArrays {
     double4* w_nx;       // normal (xyz) and position (w)
     double4* w_mvfd;     // mass, velocity, force, sigma0
     double* w_acc;
-    double* force_pp;    // nw x np wall force per particle
+    double* force_pp;    // MAXWALLS x ntot wall force per particle
     double* tau_pp;      // np shear stress per particle (wmode 3)
-    double* partial;     // nblocks block sums
+    double* partial;     // block sums, one per 256-particle chunk
     double* tau_partial;
-    double* tau_eff;     // serial sum of tau partials (1 value)
+    double* tau_eff;     // serial sum of the tau partials, one per simulation
     // bonds
     uint32_t* bonds;     // nb0 x 2
     double4* bonds_delta;
@@ -66,7 +82,7 @@ struct DevArrays {
     uint32_t* bond_ok;   // 1 if the bond applied forces this step
     uint32_t* badj_off;  // CSR offsets (np + 1) of bonds per particle
     uint32_t* badj;      // CSR bond ids, ascending per particle
-    // loop state
+    // loop state, one per simulation
     DevLoop* loop;
 };
 

@@ -5,7 +5,6 @@
 // with -ffp-contract=off (no fused multiply-add), as the reference is with
 // nvcc --fmad=false (src/CMakeLists.txt:25).
 
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -16,8 +15,6 @@
 
 namespace sphx {
 
-namespace cg = cooperative_groups;
-
 namespace {
 
 constexpr double PI = 3.14159265358979;  // src/constants.h:14
@@ -25,7 +22,7 @@ constexpr double PI = 3.14159265358979;  // src/constants.h:14
 __device__ __forceinline__ double grid_Lz(const DevParams& p, const DevArrays& a) {
     // DEM::updateGridSize (src/device.cu:363-393): with an adaptive grid the
     // step runs with L[2] equal to the top wall position at its start.
-    return (p.adaptive == 1 && p.nw > 0) ? a.w_nx[0].w : p.L[2];
+    return (p.adaptive == 1 && p.nw > 0) ? a.w_nx[p.wall_base].w : p.L[2];
 }
 
 // calcCellID, src/sorting.cuh:7-20: the integer coordinate is the double
@@ -45,12 +42,30 @@ __device__ __forceinline__ uint32_t calc_cell_id(const DevParams& p, double Lz, 
     return (iz * p.num[1]) * p.num[0] + iy * p.num[0] + ix;
 }
 
-// Per-step flags live in slot iter & 1 of the loop state.
+// Per-step flags live in slot iter & 1 of the loop state.  All
+// simulations of a context step together, so their parities agree.
 __device__ __forceinline__ uint32_t step_slot(const DevLoop* lp) {
     return (uint32_t)(lp->iter & 1);
 }
-__device__ __forceinline__ bool sort_needed(const DevLoop* lp) {
-    return lp->changed[step_slot(lp)] != 0u;
+// Some simulation of the batch needs this step's re-sort.
+__device__ __forceinline__ bool sort_needed(const DevGlobal* g, const DevLoop* lp0) {
+    return g->changed[step_slot(lp0)] != 0u;
+}
+
+// The simulation that owns a 256-particle chunk.
+__device__ __forceinline__ const DevParams& chunk_params(const DevArrays& a, uint32_t chunk) {
+    return a.ptab[a.chunk_sim[chunk]];
+}
+
+// Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD
+// b % 8), and each XCD has its own L2.  Renumbering so that each XCD takes
+// one contiguous range of workgroups keeps the neighbours a workgroup
+// gathers (close in sorted order) in the L2 of the XCD that reads them.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+    constexpr uint32_t X = 8;
+    const uint32_t per = n / X, rem = n % X;
+    const uint32_t x = b % X, k = b / X;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
 }
 
 // ---------------------------------------------------------------------------
@@ -58,13 +73,22 @@ __device__ __forceinline__ bool sort_needed(const DevLoop* lp) {
 // decide whether the step may reuse the previous order and candidate lists.
 
 struct HashFlags {
-    uint32_t changed = 0, moved = 0, nonfinite = 0;
+    uint32_t changed = 0, moved = 0, nonfinite = 0, outside = 0;
 };
 
 __device__ __forceinline__ void hash_particle(const DevParams& p, const DevArrays& a,
                                               uint32_t i, double Lz, const double4 x,
                                               HashFlags& f) {
-    const uint32_t key = calc_cell_id(p, Lz, x.x, x.y, x.z);
+    uint32_t lk = calc_cell_id(p, Lz, x.x, x.y, x.z);
+    // A particle inside the grid (or exactly on its x == L or y == L face,
+    // P17) has a key below the padding key.  One beyond is an error (as in
+    // the reference, src/sorting.cuh:39-43); it is kept inside the
+    // simulation's key range so that it cannot reach a neighbour's cells.
+    if (lk >= p.pad_key - p.cell_base) {
+        f.outside = 1;
+        lk = p.pad_key - p.cell_base - 1;
+    }
+    const uint32_t key = p.cell_base + lk;
     if (x.x != x.x || x.y != x.y || x.z != x.z) f.nonfinite = 1;
     f.changed = key != a.key_prev[i];
     a.key[i] = key;
@@ -79,22 +103,32 @@ __device__ __forceinline__ void hash_particle(const DevParams& p, const DevArray
 
 // Called by every work-item of the wavefront: at most one atomic per
 // wavefront, and only when something happened.
-__device__ __forceinline__ void publish_hash(DevLoop* lp, uint32_t slot, const HashFlags& f) {
+__device__ __forceinline__ void publish_hash(DevLoop* lp, DevGlobal* g, uint32_t slot,
+                                             const HashFlags& f) {
     const unsigned long long anych = __ballot(f.changed);
     const unsigned long long anymv = __ballot(f.moved);
     const unsigned long long anynf = __ballot(f.nonfinite);
+    const unsigned long long anyout = __ballot(f.outside);
     if ((threadIdx.x & 63) == 0) {
-        if (anych) atomicOr(&lp->changed[slot], 1u);
+        if (anyout) atomicOr(&lp->error_bits, (uint32_t)DFLAG_KEY_RANGE);
+        if (anych) {
+            atomicOr(&lp->changed[slot], 1u);
+            atomicOr(&g->changed[slot], 1u);
+        }
         if (anych | anymv) atomicOr(&lp->rebuild[slot], 1u);
         if (anynf) atomicOr(&lp->error_bits, (uint32_t)DFLAG_NONFINITE);
     }
 }
 
-__global__ __launch_bounds__(BLOCK) void k_hash(const DevParams p, const DevArrays a) {
+// One workgroup per 256-particle chunk; every wavefront of a workgroup
+// belongs to one simulation.
+__global__ __launch_bounds__(BLOCK) void k_hash(const DevArrays a) {
+    const DevParams& p = chunk_params(a, blockIdx.x);
+    DevLoop* lp = a.loop + p.sim;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     HashFlags f;
-    if (i < p.np) hash_particle(p, a, i, grid_Lz(p, a), a.x[i], f);
-    publish_hash(a.loop, step_slot(a.loop), f);
+    if (i - p.base < p.np) hash_particle(p, a, i, grid_Lz(p, a), a.x[i], f);
+    publish_hash(lp, a.glob, step_slot(lp), f);
 }
 
 // ---------------------------------------------------------------------------
@@ -154,12 +188,14 @@ __device__ __forceinline__ void radix_scan_block(uint32_t* __restrict__ hist, ui
     __syncthreads();
 }
 
+// Without vin the values are vbase + the element index.
 __device__ __forceinline__ void radix_scatter_tile(const uint32_t* __restrict__ kin,
                                                    const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout,
                                                    uint32_t* __restrict__ vout, uint32_t n,
                                                    int shift, const uint32_t* __restrict__ hist,
-                                                   uint32_t ntiles, uint32_t tile) {
+                                                   uint32_t ntiles, uint32_t tile,
+                                                   uint32_t vbase) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t whist[BLOCK / 64][256];
     const uint32_t t = threadIdx.x;
@@ -175,7 +211,7 @@ __device__ __forceinline__ void radix_scatter_tile(const uint32_t* __restrict__ 
         const uint32_t e = base + j * BLOCK + t;
         const bool valid = e < n;
         const uint32_t k = valid ? kin[e] : 0u;
-        const uint32_t v = valid ? (vin ? vin[e] : e) : 0u;
+        const uint32_t v = valid ? (vin ? vin[e] : vbase + e) : 0u;
         const uint32_t d = (k >> shift) & 255u;
         unsigned long long peers = __ballot(valid);
 #pragma unroll
@@ -202,124 +238,109 @@ __device__ __forceinline__ void radix_scatter_tile(const uint32_t* __restrict__ 
     }
 }
 
-// Buffers of pass `pass` out of `passes`: the last pass lands in (skey, perm);
-// pass 0 reads the hash keys with the identity as values.
+// Buffers of pass `pass` out of `passes` over the elements [b, ...): the
+// last pass lands in (skey, perm); pass 0 reads the hash keys with the
+// identity as values.
 struct RadixPass {
     const uint32_t* kin;
     const uint32_t* vin;
     uint32_t* kout;
     uint32_t* vout;
 };
-__host__ __device__ inline RadixPass radix_pass(const DevArrays& a, int pass, int passes) {
+__host__ __device__ inline RadixPass radix_pass(const DevArrays& a, int pass, int passes,
+                                                uint32_t b) {
     RadixPass r;
     const bool to_final = ((passes - 1 - pass) % 2) == 0;
     if (pass == 0) {
-        r.kin = a.key;
+        r.kin = a.key + b;
         r.vin = nullptr;
     } else {
         const bool prev_final = ((passes - pass) % 2) == 0;
-        r.kin = prev_final ? a.skey : a.key_tmp;
-        r.vin = prev_final ? a.perm : a.val_tmp;
+        r.kin = (prev_final ? a.skey : a.key_tmp) + b;
+        r.vin = (prev_final ? a.perm : a.val_tmp) + b;
     }
-    r.kout = to_final ? a.skey : a.key_tmp;
-    r.vout = to_final ? a.perm : a.val_tmp;
+    r.kout = (to_final ? a.skey : a.key_tmp) + b;
+    r.vout = (to_final ? a.perm : a.val_tmp) + b;
     return r;
 }
 
 // Every per-stage kernel returns at once when no particle changed cell this
 // step: the stable order of (cell, index) is then unchanged.
-__global__ __launch_bounds__(BLOCK) void k_radix_hist(const DevLoop* __restrict__ lp,
+__global__ __launch_bounds__(BLOCK) void k_radix_hist(const DevGlobal* __restrict__ g,
+                                                      const DevLoop* __restrict__ lp,
                                                       const uint32_t* __restrict__ kin,
                                                       uint32_t n, int shift,
                                                       uint32_t* __restrict__ hist,
                                                       uint32_t ntiles) {
-    if (!sort_needed(lp)) return;
+    if (!sort_needed(g, lp)) return;
     radix_hist_tile(kin, n, shift, hist, ntiles, blockIdx.x);
 }
 
-__global__ __launch_bounds__(1024) void k_radix_scan(const DevLoop* __restrict__ lp,
+__global__ __launch_bounds__(1024) void k_radix_scan(const DevGlobal* __restrict__ g,
+                                                     const DevLoop* __restrict__ lp,
                                                      uint32_t* __restrict__ hist,
                                                      uint32_t total) {
-    if (!sort_needed(lp)) return;
+    if (!sort_needed(g, lp)) return;
     radix_scan_block<1024>(hist, total);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_radix_scatter(
-    const DevLoop* __restrict__ lp, const uint32_t* __restrict__ kin,
-    const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-    uint32_t n, int shift, const uint32_t* __restrict__ hist, uint32_t ntiles) {
-    if (!sort_needed(lp)) return;
-    radix_scatter_tile(kin, vin, kout, vout, n, shift, hist, ntiles, blockIdx.x);
+    const DevGlobal* __restrict__ g, const DevLoop* __restrict__ lp,
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
+    const uint32_t* __restrict__ hist, uint32_t ntiles) {
+    if (!sort_needed(g, lp)) return;
+    radix_scatter_tile(kin, vin, kout, vout, n, shift, hist, ntiles, blockIdx.x, 0u);
 }
 
 // ---------------------------------------------------------------------------
 // Cells and reorder
 
-__global__ __launch_bounds__(BLOCK) void k_fill_u32(const DevLoop* __restrict__ lp,
+__global__ __launch_bounds__(BLOCK) void k_fill_u32(const DevGlobal* __restrict__ g,
+                                                    const DevLoop* __restrict__ lp,
                                                     uint32_t* __restrict__ dst, uint32_t n,
                                                     uint32_t v) {
-    if (lp && !sort_needed(lp)) return;
+    if (g && !sort_needed(g, lp)) return;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i < n) dst[i] = v;
 }
 
-// reorderArrays (src/sorting.cuh:56-132) for sorted index i: cell ranges,
-// the sorted position copy, and the inverse permutation.
+// reorderArrays (src/sorting.cuh:56-132) for sorted index i of simulation
+// p: cell ranges, the sorted copies, and the inverse permutation.  Keys are
+// global (cell_base + the simulation's own key); only keys of real cells
+// get a range (a key beyond the grid, or of a padding particle, does not).
 __device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a, uint32_t i,
                                           int sorted_bits) {
     const uint32_t key = a.skey[i];
     if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
-        atomicOr(&a.loop->error_bits, (uint32_t)DFLAG_KEY_RANGE);
+        atomicOr(&a.loop[p.sim].error_bits, (uint32_t)DFLAG_KEY_RANGE);
+    const uint32_t lk = key - p.cell_base;
     if (i == 0 || key != a.skey[i - 1]) {
-        if (key < p.ncells) a.cell_start[key] = i;
+        if (lk < p.ncells) a.cell_start[key] = i;
         if (i > 0) {
             const uint32_t pk = a.skey[i - 1];
-            if (pk < p.ncells) a.cell_end[pk] = i;
+            if (pk - p.cell_base < p.ncells) a.cell_end[pk] = i;
         }
     }
-    if (i == p.np - 1 && key < p.ncells) a.cell_end[key] = i + 1;
+    if (i == p.base + p.np - 1 && lk < p.ncells) a.cell_end[key] = i + 1;
     const uint32_t o = a.perm[i];
     a.xs[i] = a.x[o];
+    a.vs[i] = a.vel[o];
+    a.ws[i] = a.angvel[o];
     a.rank[o] = i;
     a.key_prev[o] = key;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_cells(const DevParams p, const DevArrays a,
-                                                 int sorted_bits) {
-    if (!sort_needed(a.loop)) return;
+// Sorted ranges coincide with particle ranges (each simulation's keys,
+// padding included, sit between its neighbours' keys), so chunk i / 256 of
+// the sorted order belongs to the same simulation as chunk i / 256 of the
+// particles.
+__global__ __launch_bounds__(BLOCK) void k_cells(const DevArrays a, int sorted_bits) {
+    if (!sort_needed(a.glob, a.loop)) return;
+    const DevParams& p = chunk_params(a, blockIdx.x);
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i == 0) atomicAdd(&a.loop->sorts, 1u);
-    if (i < p.np) cells_one(p, a, i, sorted_bits);
-}
-
-// The whole re-sort as one cooperative launch: cellStart reset, the radix
-// passes and the cell ranges, with grid barriers between the stages.  A
-// step in which no particle changed cell costs one launch that returns at
-// once (the flag is read before any barrier, so every workgroup takes the
-// same branch).
-__global__ __launch_bounds__(BLOCK) void k_sort_coop(const DevParams p, const DevArrays a,
-                                                     int passes, int sorted_bits) {
-    if (!sort_needed(a.loop)) return;
-    cg::grid_group grid = cg::this_grid();
-    const uint32_t n = p.np;
-    const uint32_t nt = n_tiles(n);
-    const uint32_t gtid = blockIdx.x * BLOCK + threadIdx.x;
-    const uint32_t gstride = gridDim.x * BLOCK;
-    for (uint32_t i = gtid; i < p.ncells; i += gstride) a.cell_start[i] = 0xffffffffu;
-    for (int pass = 0; pass < passes; ++pass) {
-        const RadixPass r = radix_pass(a, pass, passes);
-        const int shift = 8 * pass;
-        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x)
-            radix_hist_tile(r.kin, n, shift, a.rhist, nt, t);
-        grid.sync();
-        if (blockIdx.x == 0) radix_scan_block<BLOCK>(a.rhist, 256u * nt);
-        grid.sync();
-        for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x)
-            radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, n, shift, a.rhist, nt, t);
-        grid.sync();
-    }
-    if (gtid == 0) atomicAdd(&a.loop->sorts, 1u);
-    for (uint32_t i = gtid; i < n; i += gstride) cells_one(p, a, i, sorted_bits);
+    if (i == p.base) atomicAdd(&a.loop[p.sim].sorts, 1u);
+    cells_one(p, a, i, sorted_bits);
 }
 
 // ---------------------------------------------------------------------------
@@ -627,12 +648,12 @@ constexpr int ROW_STRIDE = NC + 1;
 // partner (src/contactsearch.cuh:265-299): the last empty slot, else the
 // partner's own slot, else slot nc.  Returns true when nc exceeds NC, which
 // ends the scan of the current cell.
-__device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t np, uint32_t ib_orig,
+__device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t ib_orig,
                                                  uint32_t code, int& nc, DevLoop* loop) {
     int cpos = nc;
     for (int i = 0; i < NC; ++i) {
         const uint32_t cidx = row[i] & CW_IDX_MASK;
-        if (cidx == np) {
+        if (cidx == EMPTY) {
             cpos = i;
         } else if (cidx == ib_orig) {
             cpos = i;
@@ -663,7 +684,7 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
     {
         double4 wn = make_double4(0.0, 0.0, -1.0, Lz);
         double wv = 0.0;
-        if (p.nw >= 1) { wn = a.w_nx[0]; wv = a.w_mvfd[0].y; }
+        if (p.nw >= 1) { wn = a.w_nx[p.wall_base]; wv = a.w_mvfd[p.wall_base].y; }
         const double dw = wn.w - (xa.z + ra);
         if (dw < 0.0) wf0 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
     }
@@ -676,14 +697,14 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
         {
             double4 wn = make_double4(-1.0, 0.0, 0.0, p.L[0]);
             double wv = 0.0;
-            if (p.nw >= 2) { wn = a.w_nx[1]; wv = a.w_mvfd[1].y; }
+            if (p.nw >= 2) { wn = a.w_nx[p.wall_base + 1]; wv = a.w_mvfd[p.wall_base + 1].y; }
             const double dw = wn.w - (xa.x + ra);
             if (dw < 0.0) wf1 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
         {
             double4 wn = make_double4(1.0, 0.0, 0.0, 0.0);
             double wv = 0.0;
-            if (p.nw >= 3) { wn = a.w_nx[2]; wv = a.w_mvfd[2].y; }
+            if (p.nw >= 3) { wn = a.w_nx[p.wall_base + 2]; wv = a.w_mvfd[p.wall_base + 2].y; }
             const double dw = xa.x - ra - wn.w;
             if (dw < 0.0) wf2 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
@@ -692,21 +713,21 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
         {
             double4 wn = make_double4(0.0, -1.0, 0.0, p.L[1]);
             double wv = 0.0;
-            if (p.nw >= 4) { wn = a.w_nx[3]; wv = a.w_mvfd[3].y; }
+            if (p.nw >= 4) { wn = a.w_nx[p.wall_base + 3]; wv = a.w_mvfd[p.wall_base + 3].y; }
             const double dw = wn.w - (xa.y + ra);
             if (dw < 0.0) wf3 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
         {
             double4 wn = make_double4(0.0, 1.0, 0.0, 0.0);
             double wv = 0.0;
-            if (p.nw >= 5) { wn = a.w_nx[4]; wv = a.w_mvfd[4].y; }
+            if (p.nw >= 5) { wn = a.w_nx[p.wall_base + 4]; wv = a.w_mvfd[p.wall_base + 4].y; }
             const double dw = xa.y - ra - wn.w;
             if (dw < 0.0) wf4 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
     }
     // Outputs at the original index, src/contactsearch.cuh:706-724.
     if (!write) return;
-    const uint32_t np = p.np;
+    const size_t np = a.ntot;  // wall-major stride of force_pp
     a.force[o] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
     a.torque[o] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
     a.es_dot[o] = A.es_dot;
@@ -724,19 +745,20 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
 // CM is the contact model (1, 2 or 3); one instantiation per model keeps
 // only that model's registers live.
 template <int CM>
-__global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const DevParams p,
-                                                                      const DevArrays a) {
+__global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const DevArrays a) {
     __shared__ uint32_t srow[BLOCK * ROW_STRIDE];
-    const uint32_t ia = blockIdx.x * BLOCK + threadIdx.x;
-    if (ia >= p.np) return;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const DevParams& p = chunk_params(a, blk);
+    const uint32_t ia = blk * BLOCK + threadIdx.x;
+    if (ia - p.base >= p.np) return;
     uint32_t* row = srow + threadIdx.x * ROW_STRIDE;
-    const uint32_t np = p.np;
+    DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
     const double4 xa = a.xs[ia];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
-    const double4 vel_a = a.vel[ia_orig];
-    const double4 av_a = a.angvel[ia_orig];
+    const double4 vel_a = a.vs[ia];
+    const double4 av_a = a.ws[ia];
     const d3 angvel_a = xyz(av_a);
     Acc A;
     A.F = mk3(0.0, 0.0, 0.0);
@@ -744,7 +766,6 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
     A.es_dot = 0.0;
     A.ev_dot = 0.0;
     A.p = 0.0;
-    constexpr uint32_t cm = (uint32_t)CM;
     // src/contactsearch.cuh:352-354
     const int gx = (int)floor((xa.x - p.origo[0]) / (p.L[0] / p.num[0]));
     const int gy = (int)floor((xa.y - p.origo[1]) / (p.L[1] / p.num[1]));
@@ -761,7 +782,6 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
         // sequence: the full scan that also rebuilds this particle's
         // candidate list, the full scan alone (list overflowed), or the list.
         int nc = 0;
-        const DevLoop* lp = a.loop;
         const bool rebuild =
             !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
         const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
@@ -776,7 +796,8 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                         const int tz = gz + dz;
                         const int code = find_dist_mod(p, tx, ty, tz);
                         if (code < 0) continue;
-                        const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                        const uint32_t cell = p.cell_base + (uint32_t)tx +
+                                              (uint32_t)ty * p.num[0] +
                                               (p.num[0] * p.num[1]) * (uint32_t)tz;
                         const uint32_t start = a.cell_start[cell];
                         if (start == 0xffffffffu) continue;
@@ -808,8 +829,8 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                             if (dd >= S * S * 1.0000152587890625) continue;
                             const double len = sqrt32(dd);
                             if (len - S < 0.0)
-                                cell_done = register_contact(row, np, a.perm[ib], (uint32_t)code,
-                                                             nc, a.loop);
+                                cell_done = register_contact(row, a.perm[ib], (uint32_t)code,
+                                                             nc, lp);
                         }
                     }
             if (build) {
@@ -838,25 +859,26 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                 if (dd >= S * S * 1.0000152587890625) continue;
                 const double len = sqrt32(dd);
                 if (len - S < 0.0)
-                    cell_done = register_contact(row, np, e.y & CW_IDX_MASK, code, nc, a.loop);
+                    cell_done = register_contact(row, e.y & CW_IDX_MASK, code, nc, lp);
             }
         }
         // interact: the contact loop, src/contactsearch.cuh:484-559.
-        const bool zero_empty = a.loop->zero_empty != 0u;
+        const bool zero_empty = lp->zero_empty != 0u;
         for (int i = 0; i < NC; ++i) {
             const size_t mp = (size_t)ia_orig * NC + i;
             const uint32_t wrd = row[i];
             const uint32_t ib_orig = wrd & CW_IDX_MASK;
-            if (ib_orig != np) {
+            if (ib_orig != EMPTY) {
                 const d3 dm = shift_of(p, word_code(wrd));
-                const double4 xb = a.x[ib_orig];
+                const uint32_t rb_s = a.rank[ib_orig];
+                const double4 xb = a.xs[rb_s];
                 const double rb = xb.w;
                 const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
                 const double len = length(xab);
                 const double dn = len - (ra + rb);
                 if (dn < 0.0) {
-                    const double4 vel_b = a.vel[ib_orig];
-                    const d3 angvel_b = xyz(a.angvel[ib_orig]);
+                    const double4 vel_b = a.vs[rb_s];
+                    const d3 angvel_b = xyz(a.ws[rb_s]);
                     const double4 dt0 = a.delta_t[mp];
                     d3 nd;
                     Contrib C;
@@ -870,7 +892,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     apply(A, C);
                     a.delta_t[mp] = make_double4(nd.x, nd.y, nd.z, 0.0);
                 } else {
-                    row[i] = np;
+                    row[i] = EMPTY;
                     a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
                 }
             } else if (zero_empty) {
@@ -893,7 +915,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     const int tz = gz + dz;
                     const int code = find_dist_mod(p, tx, ty, tz);
                     if (code < 0) continue;
-                    const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                    const uint32_t cell = p.cell_base + (uint32_t)tx + (uint32_t)ty * p.num[0] +
                                           (p.num[0] * p.num[1]) * (uint32_t)tz;
                     const uint32_t start = a.cell_start[cell];
                     if (start == 0xffffffffu) continue;
@@ -909,9 +931,8 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                         const double len = length(xab);
                         const double delta = len - (ra + rb);
                         if (delta < 0.0) {
-                            const uint32_t ib_orig = a.perm[ib];
-                            contact_linear_viscous(p, A, vel_a, angvel_a, a.vel[ib_orig],
-                                                   xyz(a.angvel[ib_orig]), ra, rb, xab, len,
+                            contact_linear_viscous(p, A, vel_a, angvel_a, a.vs[ib],
+                                                   xyz(a.ws[ib]), ra, rb, xab, len,
                                                    delta, kappa);
                         } else if (delta < p.db) {
                             capillary_cohesion_exp(p, A, ra, rb, delta, xab, len, kappa);
@@ -968,8 +989,7 @@ struct LaneGroup {
 // kernel).  Returns true when nc exceeds NC, which ends the current cell.
 template <int LPP>
 __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t (&slot)[NC / LPP],
-                                               uint32_t np, uint32_t word, int& nc,
-                                               DevLoop* loop) {
+                                               uint32_t word, int& nc, DevLoop* loop) {
     constexpr int SPL = NC / LPP;
     const uint32_t ib_orig = word & CW_IDX_MASK;
     uint32_t match = 0, empty = 0;
@@ -977,7 +997,7 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
     for (int j = 0; j < SPL; ++j) {
         const uint32_t cidx = slot[j] & CW_IDX_MASK;
         match |= g.bits(cidx == ib_orig) << (LPP * j);
-        empty |= g.bits(cidx == np) << (LPP * j);
+        empty |= g.bits(cidx == EMPTY) << (LPP * j);
     }
     int cpos = nc;
     if (match) cpos = __ffs(match) - 1;
@@ -996,22 +1016,23 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
 }
 
 template <int CM, int LPP>
-__global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const DevArrays a) {
+__global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     static_assert(CM == 2 || CM == 3, "history models only");
     constexpr int SPL = NC / LPP;       // slots per lane
     constexpr int PPB = BLOCK / LPP;    // particles per workgroup
     const LaneGroup<LPP> g;
-    const uint32_t ia = blockIdx.x * PPB + threadIdx.x / LPP;
-    if (ia >= p.np) return;  // whole groups leave together
-    const uint32_t np = p.np;
+    const uint32_t ia = xcd_block(blockIdx.x, gridDim.x) * PPB + threadIdx.x / LPP;
+    // A workgroup's particles lie in one 256-particle chunk.
+    const DevParams& p = chunk_params(a, ia / BLOCK);
+    if (ia - p.base >= p.np) return;  // whole groups leave together
+    DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
     const double4 xa = a.xs[ia];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
-    const double4 vel_a = a.vel[ia_orig];
-    const double4 av_a = a.angvel[ia_orig];
+    const double4 vel_a = a.vs[ia];
+    const double4 av_a = a.ws[ia];
     const d3 angvel_a = xyz(av_a);
-    const DevLoop* lp = a.loop;
     uint32_t slot[SPL];
 #pragma unroll
     for (int j = 0; j < SPL; ++j) slot[j] = a.crow[(size_t)ia_orig * NC + g.q + LPP * j];
@@ -1036,7 +1057,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const De
                     const int tz = gz + dz;
                     const int code = find_dist_mod(p, tx, ty, tz);
                     if (code < 0) continue;
-                    const uint32_t cell = (uint32_t)tx + (uint32_t)ty * p.num[0] +
+                    const uint32_t cell = p.cell_base + (uint32_t)tx + (uint32_t)ty * p.num[0] +
                                           (p.num[0] * p.num[1]) * (uint32_t)tz;
                     const uint32_t start = a.cell_start[cell];
                     if (start == 0xffffffffu) continue;
@@ -1078,7 +1099,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const De
                         while (hm && !cell_done) {
                             const uint32_t b = __ffs(hm) - 1;
                             hm &= hm - 1u;
-                            cell_done = group_register(g, slot, np, g.from(word, b), nc, a.loop);
+                            cell_done = group_register(g, slot, g.from(word, b), nc, lp);
                         }
                     }
                 }
@@ -1119,7 +1140,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const De
                     cell_done = false;
                 }
                 if (cell_done) continue;
-                cell_done = group_register(g, slot, np, g.from(e.y, b), nc, a.loop);
+                cell_done = group_register(g, slot, g.from(e.y, b), nc, lp);
             }
         }
     }
@@ -1131,7 +1152,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const De
     uint32_t occ = 0;
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
-        const bool o = (slot[j] & CW_IDX_MASK) != np;
+        const bool o = (slot[j] & CW_IDX_MASK) != EMPTY;
         occ |= g.bits(o) << (LPP * j);
         if (!o && zero_empty)
             a.delta_t[(size_t)ia_orig * NC + g.q + LPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1169,15 +1190,16 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const De
             const size_t mp = (size_t)ia_orig * NC + sidx;
             const uint32_t ib_orig = wrd & CW_IDX_MASK;
             const d3 dm = shift_of(p, word_code(wrd));
-            const double4 xb = a.x[ib_orig];
+            const uint32_t rb_s = a.rank[ib_orig];
+            const double4 xb = a.xs[rb_s];
             const double rb = xb.w;
             const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
             const double len = length(xab);
             const double dn = len - (ra + rb);
             if (dn < 0.0) {
                 touching = true;
-                const double4 vel_b = a.vel[ib_orig];
-                const d3 angvel_b = xyz(a.angvel[ib_orig]);
+                const double4 vel_b = a.vs[rb_s];
+                const d3 angvel_b = xyz(a.ws[rb_s]);
                 const double4 dt0 = a.delta_t[mp];
                 d3 nd;
                 if constexpr (CM == 2)
@@ -1211,7 +1233,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevParams p, const De
             if (((cm >> t) & 1u) && st % LPP == g.q) {
 #pragma unroll
                 for (int j = 0; j < SPL; ++j)
-                    if ((uint32_t)j == st / LPP) slot[j] = np;
+                    if ((uint32_t)j == st / LPP) slot[j] = EMPTY;
             }
         }
         // Drop the round's slots from the remaining set.
@@ -1272,12 +1294,13 @@ __device__ __forceinline__ double serial_sum(const double* __restrict__ part, ui
 // Partials cross to the serial sum through a kernel boundary, not through
 // an in-kernel arrival counter: a device-scope release in every workgroup
 // writes back each XCD's L2 and costs more than the launch it saves.
-__global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const DevArrays a) {
+__global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevArrays a) {
     __shared__ double cache[BLOCK];
+    const DevParams& p = chunk_params(a, blockIdx.x);
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const double Lz = grid_Lz(p, a);
     double v = 0.0;
-    bool in = idx < p.np;
+    bool in = idx - p.base < p.np;
     if (in) {
         const double z = a.x[idx].z;
         const double fixvel = a.vel[idx].w;
@@ -1292,11 +1315,12 @@ __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevParams p, const
 }
 
 // The serial sum of the stress partials that integrate reads as tau_eff_x
-// (src/integration.cuh:64-68); one workgroup.
-__global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevParams p, const DevArrays a) {
+// (src/integration.cuh:64-68); one workgroup per simulation.
+__global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevArrays a) {
     __shared__ double stage[2048];
-    const double tau = serial_sum(a.tau_partial, p.nblocks, stage);
-    if (threadIdx.x == 0) a.tau_eff[0] = tau;
+    const DevParams& p = a.ptab[blockIdx.x];
+    const double tau = serial_sum(a.tau_partial + p.chunk_base, p.nblocks, stage);
+    if (threadIdx.x == 0) a.tau_eff[p.sim] = tau;
 }
 
 // ---------------------------------------------------------------------------
@@ -1306,7 +1330,7 @@ __global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevParams p, const DevA
 
 __device__ __forceinline__ double4 integrate_particle(const DevParams& p, const DevArrays& a,
                                                       uint32_t orig) {
-    const DevLoop* lp = a.loop;
+    const DevLoop* lp = a.loop + p.sim;
     const double t_now = lp->t;
     const int64_t iter = lp->iter;
     // Rate-state switch decided by the host loop before integrate
@@ -1323,9 +1347,9 @@ __device__ __forceinline__ double4 integrate_particle(const DevParams& p, const 
     const double4 angvel = a.angvel[orig];
     double4 xyzsum = a.xyzsum[orig];
     const int wall0mode = p.nw > 0 ? p.wmode[0] : 0;
-    const double wall0mass = p.nw > 0 ? a.w_mvfd[0].x : 0.0;
+    const double wall0mass = p.nw > 0 ? a.w_mvfd[p.wall_base].x : 0.0;
     double tau_eff_x = 0.0;
-    if (p.nw > 0 && wall0mode == 3) tau_eff_x = a.tau_eff[0];
+    if (p.nw > 0 && wall0mode == 3) tau_eff_x = a.tau_eff[p.sim];
     double4 acc0, angacc0;
     if (iter == 0) {
         acc0 = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1415,9 +1439,12 @@ __device__ __forceinline__ double4 integrate_particle(const DevParams& p, const 
     a.vel[orig] = vel_new;
     a.angpos[orig] = angpos_new;
     a.x[orig] = x_new;
-    // Keep the sorted copy current: valid for the next step whenever the
-    // order does not change (otherwise the next sort rebuilds it).
-    a.xs[a.rank[orig]] = x_new;
+    // Keep the sorted copies current: valid for the next step whenever the
+    // order does not change (otherwise the next sort rebuilds them).
+    const uint32_t r = a.rank[orig];
+    a.xs[r] = x_new;
+    a.vs[r] = vel_new;
+    a.ws[r] = angvel_new;
     return x_new;
 }
 
@@ -1430,17 +1457,19 @@ __device__ __forceinline__ double4 integrate_particle(const DevParams& p, const 
 // next step needs no hash launch; with an adaptive grid (L[2] follows the
 // top wall, which moves after this pass) hash_next is 0.
 
-__global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const DevArrays a,
-                                                     int hash_next) {
+__global__ __launch_bounds__(BLOCK) void k_integrate(const DevArrays a, int hash_next) {
     __shared__ double cache[BLOCK];
+    const DevParams& p = chunk_params(a, blockIdx.x);
+    DevLoop* lp = a.loop + p.sim;
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+    const bool in = idx - p.base < p.np;
     HashFlags f;
-    if (idx < p.np) {
+    if (in) {
         const double4 x_new = integrate_particle(p, a, idx);
         if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
     }
     if (hash_next) {
-        publish_hash(a.loop, step_slot(a.loop) ^ 1u, f);
+        publish_hash(lp, a.glob, step_slot(lp) ^ 1u, f);
         // List the particles that changed cell for the end-of-step re-sort
         // (one atomic per wavefront; the list is unordered, the count keeps
         // growing past MV_CAP so that an overfull list is recognised).
@@ -1448,15 +1477,15 @@ __global__ __launch_bounds__(BLOCK) void k_integrate(const DevParams p, const De
         if (mv) {
             const uint32_t lane = threadIdx.x & 63u;
             uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&a.loop->nmovers, (uint32_t)__popcll(mv));
+            if (lane == 0) base = atomicAdd(&lp->nmovers, (uint32_t)__popcll(mv));
             base = (uint32_t)__shfl((int)base, 0, 64);
             const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
             const uint32_t pos = base + (uint32_t)__popcll(mv & lt);
-            if (f.changed && pos < MV_CAP) a.movers[pos] = idx;
+            if (f.changed && pos < MV_CAP) a.movers[(size_t)p.sim * MV_CAP + pos] = idx;
         }
     }
     if (p.nw > 0) {
-        const double v = idx < p.np ? a.force_pp[idx] : 0.0;
+        const double v = in ? a.force_pp[idx] : 0.0;
         const double s = block_tree_sum(v, cache);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
     }
@@ -1508,14 +1537,18 @@ __device__ __forceinline__ unsigned long long composite(uint32_t key, uint32_t k
     return ((unsigned long long)(key & kmask) << 32) | idx;
 }
 
+// Re-sort simulation p's segment [B, B + N) of the sorted order (N = npad:
+// its particles, then its padding, whose keys never change).
 __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int passes,
                                  int sorted_bits) {
     __shared__ unsigned long long comp[MV_CAP];  // movers' (new key, index), ascending
-    __shared__ uint32_t rk[MV_CAP];              // movers' old sorted positions, ascending
-    __shared__ uint32_t npos[MV_CAP];            // movers' new sorted positions
+    __shared__ uint32_t rk[MV_CAP];              // movers' old positions, ascending
+    __shared__ uint32_t npos[MV_CAP];            // movers' new positions
     const uint32_t tid = threadIdx.x;
-    const uint32_t np = p.np;
-    const uint32_t M = a.loop->nmovers;
+    const uint32_t B = p.base, N = p.npad;
+    const DevLoop* lp = a.loop + p.sim;
+    const uint32_t* mv = a.movers + (size_t)p.sim * MV_CAP;
+    const uint32_t M = lp->nmovers;
     // The radix sort orders by the low 8 * passes bits of the key.
     const uint32_t kmask = passes >= 4 ? 0xffffffffu : ((1u << (8 * passes)) - 1u);
     const bool merge = M <= p.mv_max && M <= MV_CAP;
@@ -1524,9 +1557,9 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
         while (n2 < M) n2 <<= 1;
         for (uint32_t m = tid; m < n2; m += BLOCK) {
             if (m < M) {
-                const uint32_t o = a.movers[m];
+                const uint32_t o = mv[m];
                 comp[m] = composite(a.key[o], kmask, o);
-                rk[m] = a.rank[o];
+                rk[m] = a.rank[o] - B;
             } else {
                 comp[m] = ~0ull;
                 rk[m] = 0xffffffffu;
@@ -1540,10 +1573,11 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
         // the movers before it.
         for (uint32_t m = tid; m < M; m += BLOCK) {
             const unsigned long long c = comp[m];
-            uint32_t lo = 0, hi = np;
+            uint32_t lo = 0, hi = N;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (composite(a.skey[mid], kmask, a.perm[mid]) < c) lo = mid + 1; else hi = mid;
+                if (composite(a.skey[B + mid], kmask, a.perm[B + mid]) < c) lo = mid + 1;
+                else hi = mid;
             }
             npos[m] = lo - count_below(rk, M, lo) + m;
         }
@@ -1551,9 +1585,9 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
         // New positions, a contiguous chunk per work-item.  The n-th
         // unchanged particle sits at old position n + d, d = the number of
         // movers with rk[d'] - d' <= n.
-        const uint32_t chunk = (np + BLOCK - 1) / BLOCK;
+        const uint32_t chunk = (N + BLOCK - 1) / BLOCK;
         const uint32_t j0 = tid * chunk;
-        const uint32_t j1 = j0 + chunk < np ? j0 + chunk : np;
+        const uint32_t j1 = j0 + chunk < N ? j0 + chunk : N;
         if (j0 < j1) {
             uint32_t e = count_below(npos, M, j0);
             uint32_t n = j0 - e;
@@ -1566,39 +1600,40 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
             for (uint32_t j = j0; j < j1; ++j) {
                 if (e < M && npos[e] == j) {
                     const uint32_t o = (uint32_t)(comp[e] & 0xffffffffull);
-                    a.key_tmp[j] = a.key[o];
-                    a.val_tmp[j] = o;
+                    a.key_tmp[B + j] = a.key[o];
+                    a.val_tmp[B + j] = o;
                     ++e;
                 } else {
                     while (d < M && rk[d] - d <= n) ++d;
                     const uint32_t i = n + d;
-                    a.key_tmp[j] = a.skey[i];
-                    a.val_tmp[j] = a.perm[i];
+                    a.key_tmp[B + j] = a.skey[B + i];
+                    a.val_tmp[B + j] = a.perm[B + i];
                     ++n;
                 }
             }
         }
         __syncthreads();
-        for (uint32_t j = tid; j < np; j += BLOCK) {
-            a.skey[j] = a.key_tmp[j];
-            a.perm[j] = a.val_tmp[j];
+        for (uint32_t j = tid; j < N; j += BLOCK) {
+            a.skey[B + j] = a.key_tmp[B + j];
+            a.perm[B + j] = a.val_tmp[B + j];
         }
     } else {
-        const uint32_t nt = n_tiles(np);
+        const uint32_t nt = n_tiles(N);
+        uint32_t* hist = a.rhist + p.rhist_base;
         for (int pass = 0; pass < passes; ++pass) {
-            const RadixPass r = radix_pass(a, pass, passes);
+            const RadixPass r = radix_pass(a, pass, passes, B);
             const int shift = 8 * pass;
-            for (uint32_t t = 0; t < nt; ++t) radix_hist_tile(r.kin, np, shift, a.rhist, nt, t);
+            for (uint32_t t = 0; t < nt; ++t) radix_hist_tile(r.kin, N, shift, hist, nt, t);
             __syncthreads();
-            radix_scan_block<BLOCK>(a.rhist, 256u * nt);
+            radix_scan_block<BLOCK>(hist, 256u * nt);
             for (uint32_t t = 0; t < nt; ++t)
-                radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, np, shift, a.rhist, nt, t);
+                radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, N, shift, hist, nt, t, B);
             __syncthreads();
         }
     }
-    for (uint32_t c = tid; c < p.ncells; c += BLOCK) a.cell_start[c] = 0xffffffffu;
+    for (uint32_t c = tid; c < p.ncells; c += BLOCK) a.cell_start[p.cell_base + c] = 0xffffffffu;
     __syncthreads();
-    for (uint32_t j = tid; j < np; j += BLOCK) cells_one(p, a, j, sorted_bits);
+    for (uint32_t j = tid; j < N; j += BLOCK) cells_one(p, a, B + j, sorted_bits);
     __syncthreads();
 }
 
@@ -1607,28 +1642,30 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
 // integrates wall 0's force sum (src/device.cu:2504-2505).  With `resort`
 // set it first re-sorts for the next step when the integration pass found
 // particles that changed cell (the next step then launches no sort).
-__global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArrays a,
-                                                 int resort, int passes, int sorted_bits) {
+__global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, int passes,
+                                                 int sorted_bits) {
     __shared__ double stage[2048];
     __shared__ uint32_t do_sort;
-    DevLoop* lp = a.loop;
+    const DevParams& p = a.ptab[blockIdx.x];  // one workgroup per simulation
+    DevLoop* lp = a.loop + p.sim;
     if (threadIdx.x == 0) do_sort = (resort && lp->changed[step_slot(lp) ^ 1u]) ? 1u : 0u;
     const bool walls = p.nw > 0 && p.np > 0;
-    const double fsum = walls ? serial_sum(a.partial, p.nblocks, stage) : 0.0;
+    const double fsum = walls ? serial_sum(a.partial + p.chunk_base, p.nblocks, stage) : 0.0;
     __syncthreads();
     if (do_sort) resort_one_block(p, a, passes, sorted_bits);
     if (threadIdx.x != 0) return;
     if (do_sort) lp->sorts += 1u;
     const double t_now = lp->t;
     const double dt = p.dt;
-    if (p.adaptive == 1 && p.nw > 0 && p.np > 0) lp->grid_Lz = a.w_nx[0].w;
+    if (p.adaptive == 1 && p.nw > 0 && p.np > 0) lp->grid_Lz = a.w_nx[p.wall_base].w;
     for (uint32_t w = 0; walls && w < p.nw; ++w) {
         const int wmode = p.wmode[w];
         if (wmode == 0) continue;
-        double4 w_nx = a.w_nx[w];
-        double4 w_mvfd = a.w_mvfd[w];
+        const uint32_t wi = p.wall_base + w;
+        double4 w_nx = a.w_nx[wi];
+        double4 w_mvfd = a.w_mvfd[wi];
         double acc0;
-        if (lp->iter == 0) acc0 = 0.0; else acc0 = a.w_acc[w];
+        if (lp->iter == 0) acc0 = 0.0; else acc0 = a.w_acc[wi];
         w_mvfd.z = fsum;  // 0.0 + partial[0] + partial[1] + ... in order
         const double sigma_0 = w_mvfd.w + p.sigma0_A * sin(2.0 * PI * p.sigma0_f * t_now);
         const double N = -sigma_0 * p.L[0] * p.L[1];
@@ -1637,9 +1674,9 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
         const double dacc_dt = (acc - acc0) / dt;
         w_nx.w += w_mvfd.y * dt + 0.5 * acc * dt * dt + 1.0 / 6.0 * dacc_dt * dt * dt * dt;
         w_mvfd.y += acc * dt + 0.5 * dacc_dt * dt * dt;
-        a.w_nx[w] = w_nx;
-        a.w_mvfd[w] = w_mvfd;
-        a.w_acc[w] = acc;
+        a.w_nx[wi] = w_nx;
+        a.w_mvfd[wi] = w_mvfd;
+        a.w_acc[wi] = acc;
     }
     if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
     else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
@@ -1648,6 +1685,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
     const uint32_t slot = step_slot(lp);
     if (p.vl_enabled && (lp->rebuild[slot] || lp->force_rebuild)) lp->vl_builds += 1u;
     lp->changed[slot] = 0u;
+    a.glob->changed[slot] = 0u;  // every simulation's block writes the same
     lp->rebuild[slot] = 0u;
     lp->force_rebuild = 0u;
     lp->zero_empty = 0u;
@@ -1661,12 +1699,14 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevParams p, const DevArr
 // k_bond_gather adds the contributions to each particle in ascending bond
 // order, which is the order the reference's serial reading applies them.
 
-__global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevParams p, const DevArrays a) {
+// Bond indices are batch indices (simulation base + the .bin's index).
+__global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevArrays a) {
     const uint32_t b = blockIdx.x * BLOCK + threadIdx.x;
-    if (b >= p.nb0) return;
+    if (b >= a.nbond) return;
+    const DevParams& p = a.ptab[a.bond_sim[b]];
     const uint32_t bx = a.bonds[2 * b], by = a.bonds[2 * b + 1];
     a.bond_ok[b] = 0u;
-    if (!(bx < p.np)) return;
+    if (!(bx - p.base < p.np)) return;
     const double dt = p.dt;
     const double4 d0 = a.bonds_delta[b], o0 = a.bonds_omega[b];
     d3 delta0_t = mk3(d0.x, d0.y, d0.z);
@@ -1709,8 +1749,8 @@ __global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevParams p, const 
     const double sigma_max = length(f_n) / A + length(t_t) * R_bar / I;
     const double tau_max = length(f_t) / A + length(t_n) * R_bar / J;
     if (sigma_max >= p.sigma_b || tau_max >= p.tau_b) {
-        a.bonds[2 * b] = p.nb0;  // src/cohesion.cuh:205
-        if (p.nb0 < p.np) atomicOr(&a.loop->bond_retargeted, 1u);
+        a.bonds[2 * b] = p.base + p.nb0;  // src/cohesion.cuh:205
+        if (p.nb0 < p.np) atomicOr(&a.loop[p.sim].bond_retargeted, 1u);
         return;
     }
     a.bonds_delta[b] = make_double4(delta_t.x, delta_t.y, delta_t.z, delta_n);
@@ -1741,12 +1781,13 @@ __device__ __forceinline__ void apply_bond(const DevArrays& a, uint32_t i, uint3
     touched = true;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevParams p, const DevArrays a) {
+__global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevArrays a) {
+    const DevParams& p = chunk_params(a, blockIdx.x);
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i >= p.np) return;
-    const bool full = (i == p.nb0) && a.loop->bond_retargeted != 0u;
-    const uint32_t b0 = full ? 0u : a.badj_off[i];
-    const uint32_t b1 = full ? p.nb0 : a.badj_off[i + 1];
+    if (i - p.base >= p.np) return;
+    const bool full = (i - p.base == p.nb0) && a.loop[p.sim].bond_retargeted != 0u;
+    const uint32_t b0 = full ? p.bond_base : a.badj_off[i];
+    const uint32_t b1 = full ? p.bond_base + p.nb0 : a.badj_off[i + 1];
     if (b0 == b1) return;
     double4 F = a.force[i], T = a.torque[i];
     bool touched = false;
@@ -1764,115 +1805,88 @@ __global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevParams p, const 
 
 hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(n)), dim3(BLOCK), 0, s, nullptr, dst, n, v);
+    hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(n)), dim3(BLOCK), 0, s, nullptr, nullptr, dst,
+                       n, v);
     return hipGetLastError();
 }
 
-hipError_t launch_hash(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_hash, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+hipError_t launch_hash(const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_hash, dim3(a.nchunk), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_sort(const DevParams& p, const DevArrays& a, int passes, hipStream_t s) {
-    const uint32_t n = p.np;
+hipError_t launch_sort(const DevArrays& a, int passes, hipStream_t s) {
+    const uint32_t n = a.ntot;
     const uint32_t nt = n_tiles(n);
     for (int pass = 0; pass < passes; ++pass) {
-        const RadixPass r = radix_pass(a, pass, passes);
+        const RadixPass r = radix_pass(a, pass, passes, 0u);
         const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, a.loop, r.kin, n, shift,
-                           a.rhist, nt);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.loop, a.rhist, 256u * nt);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, a.loop, r.kin, r.vin,
-                           r.kout, r.vout, n, shift, a.rhist, nt);
+        hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, a.glob, a.loop, r.kin, n,
+                           shift, a.rhist, nt);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.glob, a.loop, a.rhist,
+                           256u * nt);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, a.glob, a.loop, r.kin,
+                           r.vin, r.kout, r.vout, n, shift, a.rhist, nt);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits, hipStream_t s) {
-    if (p.ncells)
-        hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(p.ncells)), dim3(BLOCK), 0, s, a.loop,
-                           a.cell_start, p.ncells, 0xffffffffu);
-    hipLaunchKernelGGL(k_cells, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a, sorted_bits);
+hipError_t launch_cells(const DevArrays& a, int sorted_bits, hipStream_t s) {
+    if (a.nkeys)
+        hipLaunchKernelGGL(k_fill_u32, dim3(n_blocks(a.nkeys)), dim3(BLOCK), 0, s, a.glob, a.loop,
+                           a.cell_start, a.nkeys, 0xffffffffu);
+    hipLaunchKernelGGL(k_cells, dim3(a.nchunk), dim3(BLOCK), 0, s, a, sorted_bits);
     return hipGetLastError();
 }
 
-unsigned sort_coop_grid(unsigned np) {
-    int dev = 0, coop = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 0;
-    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess ||
-        !coop)
-        return 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_sort_coop, BLOCK, 0) !=
-            hipSuccess ||
-        per_cu < 1)
-        return 0;
-    // Enough workgroups for the grid-stride stages, never more than can be
-    // resident at once.
-    const unsigned want = n_blocks(np) < 1u ? 1u : n_blocks(np);
-    const unsigned cap = (unsigned)cus * (unsigned)(per_cu < 2 ? per_cu : 2);
-    return want < cap ? want : cap;
-}
-
-hipError_t launch_sort_coop(const DevParams& p, const DevArrays& a, int passes,
-                            int sorted_bits, unsigned grid, hipStream_t s) {
-    DevParams pp = p;
-    DevArrays aa = a;
-    void* args[] = {&pp, &aa, &passes, &sorted_bits};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_sort_coop), dim3(grid),
-                                      dim3(BLOCK), args, 0, s);
-}
-
-hipError_t launch_contact(const DevParams& p, const DevArrays& a, int lanes, hipStream_t s) {
+hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipStream_t s) {
     const dim3 b(BLOCK);
-    if (lanes > 1 && (p.contactmodel == 2 || p.contactmodel == 3)) {
-        const dim3 g((p.np + BLOCK / lanes - 1) / (BLOCK / lanes));
-        if (p.contactmodel == 2) {
-            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<2, 4>), g, b, 0, s, p, a);
-            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<2, 8>), g, b, 0, s, p, a);
-            else hipLaunchKernelGGL((k_contact_g<2, 16>), g, b, 0, s, p, a);
+    if (lanes > 1 && (contactmodel == 2 || contactmodel == 3)) {
+        const dim3 g(a.nchunk * (unsigned)lanes);  // BLOCK / lanes particles per workgroup
+        if (contactmodel == 2) {
+            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<2, 4>), g, b, 0, s, a);
+            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<2, 8>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_contact_g<2, 16>), g, b, 0, s, a);
         } else {
-            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<3, 4>), g, b, 0, s, p, a);
-            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<3, 8>), g, b, 0, s, p, a);
-            else hipLaunchKernelGGL((k_contact_g<3, 16>), g, b, 0, s, p, a);
+            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<3, 4>), g, b, 0, s, a);
+            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<3, 8>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_contact_g<3, 16>), g, b, 0, s, a);
         }
         return hipGetLastError();
     }
-    const dim3 g(n_blocks(p.np));
-    if (p.contactmodel == 1)
-        hipLaunchKernelGGL(k_contact<1>, g, b, 0, s, p, a);
-    else if (p.contactmodel == 3)
-        hipLaunchKernelGGL(k_contact<3>, g, b, 0, s, p, a);
-    else if (p.contactmodel == 2)
-        hipLaunchKernelGGL(k_contact<2>, g, b, 0, s, p, a);
+    const dim3 g(a.nchunk);
+    if (contactmodel == 1)
+        hipLaunchKernelGGL(k_contact<1>, g, b, 0, s, a);
+    else if (contactmodel == 3)
+        hipLaunchKernelGGL(k_contact<3>, g, b, 0, s, a);
+    else if (contactmodel == 2)
+        hipLaunchKernelGGL(k_contact<2>, g, b, 0, s, a);
     else
-        hipLaunchKernelGGL(k_contact<0>, g, b, 0, s, p, a);
+        hipLaunchKernelGGL(k_contact<0>, g, b, 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_bonds(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    if (p.nb0 == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_bond_forces, dim3(n_blocks(p.nb0)), dim3(BLOCK), 0, s, p, a);
-    hipLaunchKernelGGL(k_bond_gather, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a);
+hipError_t launch_bonds(const DevArrays& a, hipStream_t s) {
+    if (a.nbond == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_bond_forces, dim3(n_blocks(a.nbond)), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_bond_gather, dim3(a.nchunk), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_shear_stress(const DevParams& p, const DevArrays& a, hipStream_t s) {
-    hipLaunchKernelGGL(k_shear_stress, dim3(p.nblocks), dim3(BLOCK), 0, s, p, a);
-    hipLaunchKernelGGL(k_tau_sum, dim3(1), dim3(BLOCK), 0, s, p, a);
+hipError_t launch_shear_stress(const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_shear_stress, dim3(a.nchunk), dim3(BLOCK), 0, s, a);
+    hipLaunchKernelGGL(k_tau_sum, dim3(a.nsim), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_integrate(const DevParams& p, const DevArrays& a, int hash_next,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_integrate, dim3(n_blocks(p.np)), dim3(BLOCK), 0, s, p, a, hash_next);
+hipError_t launch_integrate(const DevArrays& a, int hash_next, hipStream_t s) {
+    hipLaunchKernelGGL(k_integrate, dim3(a.nchunk), dim3(BLOCK), 0, s, a, hash_next);
     return hipGetLastError();
 }
 
-hipError_t launch_walls(const DevParams& p, const DevArrays& a, int resort, int passes,
-                        int sorted_bits, hipStream_t s) {
-    hipLaunchKernelGGL(k_walls, dim3(1), dim3(BLOCK), 0, s, p, a, resort, passes, sorted_bits);
+hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_walls, dim3(a.nsim), dim3(BLOCK), 0, s, a, resort, passes, sorted_bits);
     return hipGetLastError();
 }
 

@@ -17,42 +17,37 @@ __host__ __device__ inline unsigned n_tiles(unsigned n) {
 }
 __host__ __device__ inline unsigned n_blocks(unsigned n) { return (n + BLOCK - 1) / BLOCK; }
 
+// All launches cover every simulation of the context (DevArrays holds the
+// batch layout); grids are whole 256-particle chunks.
+
 // Cell key of every particle (calcParticleCellID, src/sorting.cuh:25-51)
 // into the current step's flag slot.
-hipError_t launch_hash(const DevParams& p, const DevArrays& a, hipStream_t s);
+hipError_t launch_hash(const DevArrays& a, hipStream_t s);
 // Stable LSD radix sort of (key, identity) into (skey, perm), `passes` digits
 // of 8 bits (replaces thrust::sort_by_key, src/device.cu:1075-1078), one
-// launch per stage.
-hipError_t launch_sort(const DevParams& p, const DevArrays& a, int passes, hipStream_t s);
-// cellStart reset, cellStart/cellEnd, sorted position copy
+// launch per stage, over the whole batch (keys are global).
+hipError_t launch_sort(const DevArrays& a, int passes, hipStream_t s);
+// cellStart reset, cellStart/cellEnd, sorted copies
 // (src/device.cu:1089-1090, src/sorting.cuh:56-132).
-hipError_t launch_cells(const DevParams& p, const DevArrays& a, int sorted_bits,
-                        hipStream_t s);
-// launch_sort + launch_cells as one cooperative launch of `grid` workgroups;
-// sort_coop_grid gives the grid for the current device (0: cooperative
-// launches unavailable).
-unsigned sort_coop_grid(unsigned np);
-hipError_t launch_sort_coop(const DevParams& p, const DevArrays& a, int passes,
-                            int sorted_bits, unsigned grid, hipStream_t s);
+hipError_t launch_cells(const DevArrays& a, int sorted_bits, hipStream_t s);
 // topology + interact fused per particle (src/contactsearch.cuh:194-726);
 // `lanes` (1, 4, 8 or 16) work-items per particle for contact models 2 and 3.
-hipError_t launch_contact(const DevParams& p, const DevArrays& a, int lanes, hipStream_t s);
+hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipStream_t s);
 // bondsLinear per bond, then a per-particle gather in bond order
 // (src/cohesion.cuh:8-233).
-hipError_t launch_bonds(const DevParams& p, const DevArrays& a, hipStream_t s);
+hipError_t launch_bonds(const DevArrays& a, hipStream_t s);
 // findShearStressOnFixedMovingParticles + summation + serial sum
 // (src/integration.cuh:457-484,309-341,64-68), wmode 3 only.
-hipError_t launch_shear_stress(const DevParams& p, const DevArrays& a, hipStream_t s);
+hipError_t launch_shear_stress(const DevArrays& a, hipStream_t s);
 // integrate (TY3), src/integration.cuh:14-305, the wall force block
-// partials, and (hash_next != 0) the next step's cell keys.  np > 0.
-hipError_t launch_integrate(const DevParams& p, const DevArrays& a, int hash_next,
-                            hipStream_t s);
-// Serial sum of the partials + integrateWalls + loop advance
-// (src/integration.cuh:344-450, src/device.cu:2500-2538); also with np == 0.
-// With resort != 0 it also re-sorts for the next step, in one workgroup,
-// when the integration pass (launched with hash_next) found a cell change.
-hipError_t launch_walls(const DevParams& p, const DevArrays& a, int resort, int passes,
-                        int sorted_bits, hipStream_t s);
+// partials, and (hash_next != 0) the next step's cell keys.
+hipError_t launch_integrate(const DevArrays& a, int hash_next, hipStream_t s);
+// Per simulation: serial sum of the partials + integrateWalls + loop advance
+// (src/integration.cuh:344-450, src/device.cu:2500-2538).  With resort != 0
+// it also re-sorts the simulation for the next step, in one workgroup, when
+// the integration pass (launched with hash_next) found a cell change.
+hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
+                        hipStream_t s);
 // Fill n words with v.
 hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s);
 

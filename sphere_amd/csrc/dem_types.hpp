@@ -36,6 +36,11 @@ constexpr uint32_t VL_CELL_SHIFT = 27;
 // one-workgroup re-sort at the end of the step.
 constexpr uint32_t MV_CAP = 2048;
 
+// An empty contact slot (the reference stores np there; np differs between
+// the simulations of a batch, so the engine stores this marker and the
+// field accessors translate).
+constexpr uint32_t EMPTY = CW_IDX_MASK;
+
 struct DevParams {
     // grid (src/datatypes.h:45-51)
     double origo[3];
@@ -71,6 +76,20 @@ struct DevParams {
     // re-sort merges into the previous order (<= MV_CAP); beyond it, it
     // sorts from scratch.
     uint32_t mv_max;
+    // Place of this simulation in the batch arrays.  Its particles occupy
+    // original and sorted indices [base, base + npad): the np real ones,
+    // then padding particles whose key pad_key sorts after every real key
+    // of the simulation and before the next simulation's keys.
+    uint32_t sim;
+    uint32_t base;
+    uint32_t npad;
+    uint32_t chunk_base;  // base / 256: the first block partial of the simulation
+    uint32_t cell_base;   // global key of the simulation's cell 0
+    uint32_t pad_key;
+    uint32_t wall_base;   // first wall of the simulation
+    uint32_t bond_base;   // first bond of the simulation
+    uint32_t rhist_base;  // radix histogram space of the one-workgroup re-sort
+    uint32_t pad2;
 };
 
 // Loop state kept on the device so that captured step sequences can be
@@ -92,6 +111,11 @@ struct DevLoop {
     uint32_t vl_builds;         // number of list rebuilds so far (diagnostic)
     uint32_t sorts;             // number of re-sorts so far (diagnostic)
     uint32_t nmovers;           // particles whose key changed for the next step
+};
+
+// Batch-wide state.
+struct DevGlobal {
+    uint32_t changed[2];  // some simulation needs a re-sort (per step parity)
 };
 
 enum DevFlag : uint32_t {

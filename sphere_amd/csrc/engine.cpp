@@ -91,45 +91,50 @@ struct Writer {
 
 struct sph_prof_state;
 
-struct sph_ctx {
+// One simulation of a context: its .bin state on the host, its parameter
+// block (also in the device table), and the host mirror of its loop.
+struct Sim {
     Host h;
     DevParams P;
+    int64_t iter = 0;
+};
+
+struct sph_ctx {
+    std::vector<Sim> sims;  // one for sph_open, several for sph_open_batch
+    Host& h0() { return sims[0].h; }
     DevArrays A;
+    DevParams* d_ptab = nullptr;  // device copy of the parameter blocks
     std::vector<void*> allocs;
     hipStream_t stream = nullptr;
     int device = 0;
     unsigned flags = 0;
     std::string err;
+    // Sort width over the batch's global keys.
     int sorted_bits = 32;
     int passes = 4;
-    bool shear_stress = false;
+    uint32_t contactmodel = 0;   // shared by all simulations of a batch
+    bool shear_stress = false;   // some simulation has a wmode-3 top wall
+    bool any_adaptive = false;   // some simulation has an adaptive grid
     bool profile = false;
     // The next step starts with a hash launch (keys not yet computed from
     // the current positions); otherwise the previous integration pass left
     // them, except with an adaptive grid, which hashes every step.
     bool hash_pending = true;
-    // Workgroups of the cooperative sort launch (0 = unavailable).  Off by
-    // default: on MI355X a cooperative launch costs about as much as the
-    // seven ordinary launches it replaces.
-    unsigned coop_grid = 0;
-    bool coop_sort = false;
     // Work-items per particle in the contact kernel (models 2 and 3).
     int contact_lanes = 4;
     // Where a step's re-sort runs: 0 = the per-stage sort launches at the
     // step start (every step; they return at once when nothing changed),
-    // 1 = one workgroup at the end of the previous step, only when needed,
-    // 2 = choose between the two from the measured re-sort rate.
+    // 1 = one workgroup per simulation at the end of the previous step, only
+    // when needed, 2 = choose between the two from the measured rate.
     int sort_mode = 2;
     bool resort_at_end = true;     // the choice for the steps being enqueued
     bool prev_walls_resorted = false;
     // Asynchronous sample of the loop counters for the automatic choice.
-    DevLoop* loop_sample = nullptr;   // pinned host memory
+    DevLoop* loop_sample = nullptr;   // pinned host memory (all simulations)
     hipEvent_t sample_ev = nullptr;
     bool sample_pending = false;
     int64_t sample_iter0 = 0;
-    uint32_t sample_sorts0 = 0;
-    // host mirror of the loop state
-    int64_t iter = 0;
+    uint64_t sample_sorts0 = 0;
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -280,8 +285,7 @@ void write_bin(const Host& h, Writer& w) {
 }
 
 // checkValues, src/sphere.cpp:145-345 (fluid == 0).
-int check_values(sph_ctx* c) {
-    const Host& h = c->h;
+int check_values(sph_ctx* c, const Host& h) {
     if (h.t > h.t_total || h.t < 0.0)
         return fail(c, SPH_E_INPUT, "Error: time.current = %g s, time.total = %g s", h.t, h.t_total);
     if (h.origo[0] < 0.0 || h.origo[1] < 0.0 || h.origo[2] < 0.0)
@@ -333,13 +337,13 @@ int check_values(sph_ctx* c) {
     return SPH_OK;
 }
 
-void set_skin(sph_ctx* c, double skin) {
-    c->P.vl_skin = skin > 0.0 ? skin : 0.0;
-    c->P.vl_enabled = skin > 0.0 ? 1 : 0;
+void set_skin(DevParams& P, double skin) {
+    P.vl_skin = skin > 0.0 ? skin : 0.0;
+    P.vl_enabled = skin > 0.0 ? 1 : 0;
     // Rebuild once any particle has moved 0.45 skin (< skin / 2) since the
     // lists were built.
-    const double trig = 0.45 * c->P.vl_skin;
-    c->P.vl_trigger2 = trig * trig;
+    const double trig = 0.45 * P.vl_skin;
+    P.vl_trigger2 = trig * trig;
 }
 
 int bit_length(uint64_t v) {
@@ -348,9 +352,11 @@ int bit_length(uint64_t v) {
     return b;
 }
 
-void fill_params(sph_ctx* c) {
-    const Host& h = c->h;
-    DevParams& P = c->P;
+// The physical and grid parameters of one simulation (its place in the
+// batch is set by layout()).
+void fill_params(Sim& sim) {
+    const Host& h = sim.h;
+    DevParams& P = sim.P;
     memset(&P, 0, sizeof P);
     for (int d = 0; d < 3; ++d) {
         P.origo[d] = h.origo[d];
@@ -377,21 +383,57 @@ void fill_params(sph_ctx* c) {
     P.sigma0_A = h.sigma0_A; P.sigma0_f = h.sigma0_f;
     P.tau_x = h.tau_x;
     for (int w = 0; w < MAXWALLS; ++w) P.wmode[w] = h.wmode[w];
-    // Keys reach at most ncells + num[0] + num[0]*num[1] (a particle at
-    // exactly x == L or y == L survives the periodic wrap, P17).
-    const uint64_t kmax = (uint64_t)P.ncells + h.num[0] + (uint64_t)h.num[0] * h.num[1];
-    c->sorted_bits = std::max(1, bit_length(kmax));
-    if (c->sorted_bits > 32) c->sorted_bits = 32;
-    c->passes = (c->sorted_bits + 7) / 8;
     P.mv_max = MV_CAP;
-    c->shear_stress = h.nw > 0 && h.wmode[0] == 3;
     // Candidate lists: skin of 0.1 r_min by default.  Model 1 with a
     // capillary debonding distance keeps the full scan (its interaction
     // range extends beyond overlap).
     double rmin = 0.0;
     for (uint32_t i = 0; i < h.np; ++i)
         if (i == 0 || h.x[4 * i + 3] < rmin) rmin = h.x[4 * i + 3];
-    set_skin(c, (h.contactmodel == 1 && h.db > 0.0) ? 0.0 : 0.1 * rmin);
+    set_skin(P, (h.contactmodel == 1 && h.db > 0.0) ? 0.0 : 0.1 * rmin);
+}
+
+// Place every simulation in the batch arrays: whole 256-particle chunks of
+// particles, then walls, bonds, and a key range that holds the simulation's
+// own keys (up to ncells + num[0] + num[0]*num[1]: a particle at exactly
+// x == L or y == L survives the periodic wrap, P17) and its padding key.
+void layout(sph_ctx* c) {
+    uint32_t base = 0, walls = 0, bonds = 0, keys = 0, rh = 0;
+    bool shear = false, adaptive = false;
+    for (size_t k = 0; k < c->sims.size(); ++k) {
+        Sim& sim = c->sims[k];
+        DevParams& P = sim.P;
+        const Host& h = sim.h;
+        P.sim = (uint32_t)k;
+        P.base = base;
+        P.npad = std::max<uint32_t>(BLOCK, (h.np + BLOCK - 1) / BLOCK * BLOCK);
+        P.chunk_base = base / BLOCK;
+        P.cell_base = keys;
+        const uint32_t kmax = P.ncells + h.num[0] + h.num[0] * h.num[1];
+        P.pad_key = keys + kmax + 1;
+        P.wall_base = walls;
+        P.bond_base = bonds;
+        P.rhist_base = rh;
+        base += P.npad;
+        keys += kmax + 2;
+        walls += h.nw;
+        bonds += h.nb0;
+        rh += 256u * n_tiles(P.npad);
+        shear = shear || (h.nw > 0 && h.wmode[0] == 3);
+        adaptive = adaptive || (h.adaptive == 1 && h.nw > 0);
+    }
+    DevArrays& A = c->A;
+    A.nsim = (uint32_t)c->sims.size();
+    A.ntot = base;
+    A.nchunk = base / BLOCK;
+    A.nbond = bonds;
+    A.nkeys = keys;
+    c->sorted_bits = std::max(1, bit_length(keys));
+    if (c->sorted_bits > 32) c->sorted_bits = 32;
+    c->passes = (c->sorted_bits + 7) / 8;
+    c->shear_stress = shear;
+    c->any_adaptive = adaptive;
+    c->contactmodel = c->sims[0].h.contactmodel;
 }
 
 template <class T> int dev_alloc(sph_ctx* c, T** p, size_t count) {
@@ -410,163 +452,241 @@ template <class T> int dev_alloc(sph_ctx* c, T** p, size_t count) {
     } while (0)
 
 int allocate(sph_ctx* c) {
-    const size_t np = c->h.np, nw = c->h.nw, nb = c->h.nb0;
-    const size_t nblk = c->P.nblocks, ncells = c->P.ncells;
     DevArrays& A = c->A;
-    memset(&A, 0, sizeof A);
+    const size_t np = A.ntot, nb = A.nbond, nkeys = A.nkeys, ns = A.nsim;
+    size_t nw = 0, rh = 0;
+    for (const Sim& sim : c->sims) {
+        nw += sim.h.nw;
+        rh += 256u * n_tiles(sim.P.npad);
+    }
     ALLOC(A.x, np); ALLOC(A.vel, np); ALLOC(A.angvel, np); ALLOC(A.angpos, np);
     ALLOC(A.xyzsum, np); ALLOC(A.acc, np); ALLOC(A.angacc, np); ALLOC(A.force, np);
     ALLOC(A.torque, np);
     ALLOC(A.es_dot, np); ALLOC(A.es, np); ALLOC(A.ev_dot, np); ALLOC(A.ev, np); ALLOC(A.p, np);
     ALLOC(A.key, np); ALLOC(A.key_prev, np); ALLOC(A.skey, np); ALLOC(A.perm, np);
     ALLOC(A.key_tmp, np); ALLOC(A.val_tmp, np);
-    ALLOC(A.rhist, 256 * (size_t)n_tiles((unsigned)np));
-    ALLOC(A.cell_start, ncells); ALLOC(A.cell_end, ncells);
+    // The batch-wide sort and the per-simulation one-workgroup sorts.
+    ALLOC(A.rhist, std::max<size_t>(256 * (size_t)n_tiles((unsigned)np), rh));
+    ALLOC(A.cell_start, nkeys); ALLOC(A.cell_end, nkeys);
     ALLOC(A.xs, np);
+    ALLOC(A.vs, np);
+    ALLOC(A.ws, np);
     ALLOC(A.rank, np);
     ALLOC(A.vlist, np * NV);
     ALLOC(A.vcount, np);
     ALLOC(A.x_build, np);
-    ALLOC(A.movers, MV_CAP);
+    ALLOC(A.movers, ns * MV_CAP);
     ALLOC(A.crow, np * NC);
     ALLOC(A.delta_t, np * NC);
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
-    ALLOC(A.force_pp, nw * np);
+    ALLOC(A.force_pp, (size_t)MAXWALLS * np);
     ALLOC(A.tau_pp, np);
-    ALLOC(A.partial, nblk); ALLOC(A.tau_partial, nblk); ALLOC(A.tau_eff, 1);
+    ALLOC(A.partial, A.nchunk); ALLOC(A.tau_partial, A.nchunk); ALLOC(A.tau_eff, ns);
     ALLOC(A.bonds, 2 * nb); ALLOC(A.bonds_delta, nb); ALLOC(A.bonds_omega, nb);
     ALLOC(A.bond_f, nb); ALLOC(A.bond_t, nb); ALLOC(A.bond_ok, nb);
     ALLOC(A.badj_off, np + 1);
     ALLOC(A.badj, 2 * nb);
-    ALLOC(A.loop, 1);
+    ALLOC(A.loop, ns);
+    ALLOC(A.glob, 1);
+    ALLOC(c->d_ptab, ns);
+    A.ptab = c->d_ptab;
+    uint32_t* chunk_sim = nullptr;
+    uint32_t* bond_sim = nullptr;
+    ALLOC(chunk_sim, A.nchunk);
+    ALLOC(bond_sim, nb);
+    A.chunk_sim = chunk_sim;
+    A.bond_sim = bond_sim;
+    return SPH_OK;
+}
+
+// The parameter table on the device (after a host-side parameter change).
+int upload_params(sph_ctx* c) {
+    std::vector<DevParams> tab(c->sims.size());
+    for (size_t k = 0; k < c->sims.size(); ++k) tab[k] = c->sims[k].P;
+    HIPCHK(c, hipMemcpyAsync(c->d_ptab, tab.data(), tab.size() * sizeof(DevParams),
+                             hipMemcpyHostToDevice, c->stream), "H2D");
+    // The copy reads `tab` asynchronously: wait before it goes out of scope.
+    HIPCHK(c, hipStreamSynchronize(c->stream), "H2D");
     return SPH_OK;
 }
 
 int upload_state(sph_ctx* c) {
-    const Host& h = c->h;
-    const size_t np = h.np, nw = h.nw, nb = h.nb0;
     hipStream_t s = c->stream;
     DevArrays& A = c->A;
+    const size_t N = A.ntot;
 #define H2D(dst, src, bytes) \
     HIPCHK(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyHostToDevice, s), "H2D")
-    if (np) {
-        H2D(A.x, h.x.data(), np * 32);
-        H2D(A.vel, h.vel.data(), np * 32);
-        H2D(A.angvel, h.angvel.data(), np * 32);
-        H2D(A.angpos, h.angpos.data(), np * 32);
-        H2D(A.xyzsum, h.xyzsum.data(), np * 32);
-        H2D(A.force, h.force.data(), np * 32);
-        H2D(A.torque, h.torque.data(), np * 32);
-        H2D(A.es_dot, h.es_dot.data(), np * 8);
-        H2D(A.es, h.es.data(), np * 8);
-        H2D(A.ev_dot, h.ev_dot.data(), np * 8);
-        H2D(A.ev, h.ev.data(), np * 8);
-        H2D(A.p, h.p.data(), np * 8);
-    }
-    std::vector<double> zeros4(np * 4, 0.0);
-    if (np) {
-        H2D(A.acc, zeros4.data(), np * 32);
-        H2D(A.angacc, zeros4.data(), np * 32);
-    }
-    // Contact rows start empty (partner index np, src/device.cu:458-462)
+    // Whole-batch initial values.  Padding particles keep zero state, their
+    // padding key (never re-hashed) and empty rows.
+    HIPCHK(c, hipMemsetAsync(A.x, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.vel, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.angvel, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.angpos, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.xyzsum, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.acc, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.angacc, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.force, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.torque, 0, N * 32, s), "memset");
+    double* e[5] = {A.es_dot, A.es, A.ev_dot, A.ev, A.p};
+    for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemsetAsync(e[k], 0, N * 8, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.delta_t, 0, N * NC * sizeof(double4), s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.vcount, 0, N * 4, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.xs, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.vs, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.ws, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.x_build, 0, N * 32, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.force_pp, 0, (size_t)MAXWALLS * N * 8, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.tau_pp, 0, N * 8, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.tau_eff, 0, A.nsim * 8, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.glob, 0, sizeof(DevGlobal), s), "memset");
+    // Contact rows start empty (the reference fills np, src/device.cu:458-462)
     // with zero displacement.
-    std::vector<uint32_t> rows(np * NC, (uint32_t)np);
-    if (np) {
-        H2D(A.crow, rows.data(), np * NC * 4);
-        HIPCHK(c, hipMemsetAsync(A.delta_t, 0, np * NC * sizeof(double4), s), "memset");
-        std::vector<uint32_t> ff(np, 0xffffffffu);
-        H2D(A.key_prev, ff.data(), np * 4);
-        std::vector<uint32_t> ident(np);
-        for (size_t i = 0; i < np; ++i) ident[i] = (uint32_t)i;
-        H2D(A.perm, ident.data(), np * 4);
-        H2D(A.rank, ident.data(), np * 4);
-        HIPCHK(c, hipMemsetAsync(A.vcount, 0, np * 4, s), "memset");
-        H2D(A.x_build, h.x.data(), np * 32);
-        HIPCHK(c, hipMemsetAsync(A.skey, 0, np * 4, s), "memset");
-        HIPCHK(c, hipMemsetAsync(A.xs, 0, np * 32, s), "memset");
-    }
-    if (c->P.ncells) {
-        std::vector<uint32_t> cs(c->P.ncells, 0xffffffffu);
-        H2D(A.cell_start, cs.data(), (size_t)c->P.ncells * 4);
-        HIPCHK(c, hipMemsetAsync(A.cell_end, 0, (size_t)c->P.ncells * 4, s), "memset");
-    }
-    if (nw) {
-        H2D(A.w_nx, h.w_nx.data(), nw * 32);
-        H2D(A.w_mvfd, h.w_mvfd.data(), nw * 32);
-        HIPCHK(c, hipMemsetAsync(A.w_acc, 0, nw * 8, s), "memset");
-        if (np) HIPCHK(c, hipMemsetAsync(A.force_pp, 0, nw * np * 8, s), "memset");
-    }
-    HIPCHK(c, hipMemsetAsync(A.tau_eff, 0, 8, s), "memset");
-    if (nb) {
-        H2D(A.bonds, h.bonds.data(), nb * 8);
-        H2D(A.bonds_delta, h.bonds_delta.data(), nb * 32);
-        H2D(A.bonds_omega, h.bonds_omega.data(), nb * 32);
-    }
-    // Bond adjacency (CSR, ascending bond id per particle).
-    std::vector<uint32_t> off(np + 1, 0u), adj;
-    if (nb) {
-        for (size_t b = 0; b < nb; ++b) {
-            const uint32_t i = h.bonds[2 * b], j = h.bonds[2 * b + 1];
-            if (i < np) off[i + 1]++;
-            if (j < np && j != i) off[j + 1]++;
+    std::vector<uint32_t> rows(N * NC, EMPTY);
+    H2D(A.crow, rows.data(), N * NC * 4);
+    std::vector<uint32_t> ident(N), keys(N), ff(N, 0xffffffffu), chunk_sim(A.nchunk);
+    for (size_t i = 0; i < N; ++i) ident[i] = (uint32_t)i;
+    H2D(A.perm, ident.data(), N * 4);
+    H2D(A.rank, ident.data(), N * 4);
+    H2D(A.key_prev, ff.data(), N * 4);
+    std::vector<uint32_t> cs(A.nkeys, 0xffffffffu);
+    H2D(A.cell_start, cs.data(), (size_t)A.nkeys * 4);
+    HIPCHK(c, hipMemsetAsync(A.cell_end, 0, (size_t)A.nkeys * 4, s), "memset");
+    std::vector<double> wn, wm;
+    std::vector<uint32_t> bonds, bond_sim;
+    std::vector<double> bd, bo;
+    std::vector<uint32_t> off(N + 1, 0u);
+    std::vector<DevLoop> loops(A.nsim);
+    for (const Sim& sim : c->sims) {
+        const Host& h = sim.h;
+        const DevParams& P = sim.P;
+        const size_t np = h.np, b0 = P.base;
+        for (uint32_t i = 0; i < P.npad / BLOCK; ++i) chunk_sim[P.chunk_base + i] = P.sim;
+        for (size_t i = 0; i < P.npad; ++i) keys[b0 + i] = P.pad_key;
+        if (np) {
+            H2D(A.x + b0, h.x.data(), np * 32);
+            H2D(A.vel + b0, h.vel.data(), np * 32);
+            H2D(A.angvel + b0, h.angvel.data(), np * 32);
+            H2D(A.angpos + b0, h.angpos.data(), np * 32);
+            H2D(A.xyzsum + b0, h.xyzsum.data(), np * 32);
+            H2D(A.force + b0, h.force.data(), np * 32);
+            H2D(A.torque + b0, h.torque.data(), np * 32);
+            H2D(A.es_dot + b0, h.es_dot.data(), np * 8);
+            H2D(A.es + b0, h.es.data(), np * 8);
+            H2D(A.ev_dot + b0, h.ev_dot.data(), np * 8);
+            H2D(A.ev + b0, h.ev.data(), np * 8);
+            H2D(A.p + b0, h.p.data(), np * 8);
+            H2D(A.x_build + b0, h.x.data(), np * 32);
         }
-        for (size_t i = 0; i < np; ++i) off[i + 1] += off[i];
-        adj.assign(off[np], 0u);
-        std::vector<uint32_t> fillp(off.begin(), off.end() - 1);
+        wn.insert(wn.end(), h.w_nx.begin(), h.w_nx.end());
+        wm.insert(wm.end(), h.w_mvfd.begin(), h.w_mvfd.end());
+        // Bonds in batch indices, with the adjacency (CSR, ascending bond id
+        // per particle).
+        const size_t nb = h.nb0;
         for (size_t b = 0; b < nb; ++b) {
             const uint32_t i = h.bonds[2 * b], j = h.bonds[2 * b + 1];
-            if (i < np) adj[fillp[i]++] = (uint32_t)b;
-            if (j < np && j != i) adj[fillp[j]++] = (uint32_t)b;
+            bonds.push_back((uint32_t)(b0 + i));
+            bonds.push_back((uint32_t)(b0 + j));
+            bond_sim.push_back(P.sim);
+            if (i < np) off[b0 + i + 1]++;
+            if (j < np && j != i) off[b0 + j + 1]++;
+        }
+        bd.insert(bd.end(), h.bonds_delta.begin(), h.bonds_delta.end());
+        bo.insert(bo.end(), h.bonds_omega.begin(), h.bonds_omega.end());
+        DevLoop& L = loops[P.sim];
+        memset(&L, 0, sizeof L);
+        L.t = h.t;
+        L.iter = 0;
+        L.grid_Lz = h.L[2];
+        L.velocity_state = 1;
+        L.zero_empty = 1u;
+        L.force_rebuild = 1u;
+    }
+    // Padding keys, and real keys before the first hash (the first step
+    // always hashes).
+    H2D(A.key, keys.data(), N * 4);
+    H2D(A.skey, keys.data(), N * 4);
+    H2D(const_cast<uint32_t*>(A.chunk_sim), chunk_sim.data(), chunk_sim.size() * 4);
+    if (!wn.empty()) {
+        H2D(A.w_nx, wn.data(), wn.size() * 8);
+        H2D(A.w_mvfd, wm.data(), wm.size() * 8);
+        HIPCHK(c, hipMemsetAsync(A.w_acc, 0, wn.size() / 4 * 8, s), "memset");
+    }
+    std::vector<uint32_t> adj;
+    if (!bonds.empty()) {
+        H2D(A.bonds, bonds.data(), bonds.size() * 4);
+        H2D(const_cast<uint32_t*>(A.bond_sim), bond_sim.data(), bond_sim.size() * 4);
+        H2D(A.bonds_delta, bd.data(), bd.size() * 8);
+        H2D(A.bonds_omega, bo.data(), bo.size() * 8);
+        for (size_t i = 0; i < N; ++i) off[i + 1] += off[i];
+        adj.assign(off[N], 0u);
+        std::vector<uint32_t> fillp(off.begin(), off.end() - 1);
+        for (size_t b = 0; b < bond_sim.size(); ++b) {
+            const Sim& sim = c->sims[bond_sim[b]];
+            const uint32_t i = bonds[2 * b], j = bonds[2 * b + 1];
+            const uint32_t b0 = sim.P.base, np = sim.h.np;
+            if (i - b0 < np) adj[fillp[i]++] = (uint32_t)b;
+            if (j - b0 < np && j != i) adj[fillp[j]++] = (uint32_t)b;
         }
         H2D(A.badj, adj.data(), adj.size() * 4);
     }
-    H2D(A.badj_off, off.data(), (np + 1) * 4);
-    DevLoop L;
-    memset(&L, 0, sizeof L);
-    L.t = h.t;
-    L.iter = 0;
-    L.grid_Lz = h.L[2];
-    L.velocity_state = 1;
-    L.zero_empty = 1u;
-    L.force_rebuild = 1u;
-    H2D(A.loop, &L, sizeof L);
+    H2D(A.badj_off, off.data(), (N + 1) * 4);
+    H2D(A.loop, loops.data(), loops.size() * sizeof(DevLoop));
     c->hash_pending = true;
     c->prev_walls_resorted = false;
+    for (Sim& sim : c->sims) sim.iter = 0;
 #undef H2D
     HIPCHK(c, hipStreamSynchronize(s), "upload");
-    return SPH_OK;
+    return upload_params(c);
 }
 
-int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sph_ctx** out) {
+// readbin + checkValues for every image, then one set of device arrays for
+// all of them.
+int open_images(const std::vector<std::pair<const void*, size_t>>& images, int device,
+                unsigned flags, sph_ctx** out) {
     *out = nullptr;
+    if (images.empty()) {
+        g_open_error = "no simulations given";
+        return SPH_E_ARG;
+    }
     sph_ctx* c = new sph_ctx();
     c->flags = flags;
-    std::string err;
-    int rc = parse_bin(c->h, bytes, nbytes, err);
-    if (rc != SPH_OK) {
-        g_open_error = err;
-        delete c;
-        return rc;
-    }
-    for (uint32_t b = 0; b < c->h.nb0; ++b) {
-        if (c->h.bonds[2 * b] >= c->h.np || c->h.bonds[2 * b + 1] >= c->h.np) {
-            g_open_error = "Error: bond " + std::to_string(b) + " names a particle index >= np";
-            delete c;
-            return SPH_E_INPUT;
-        }
-    }
-    if (c->h.np >= MAX_NP) {
-        g_open_error = "np exceeds the supported maximum (2^27)";
-        delete c;
-        return SPH_E_INPUT;
-    }
-    if (!(flags & SPH_OPEN_NOCHECK)) {
-        rc = check_values(c);
+    c->sims.resize(images.size());
+    uint64_t total = 0;
+    for (size_t k = 0; k < images.size(); ++k) {
+        Host& h = c->sims[k].h;
+        std::string err;
+        int rc = parse_bin(h, images[k].first, images[k].second, err);
         if (rc != SPH_OK) {
-            g_open_error = c->err;
+            g_open_error = images.size() > 1 ? "simulation " + std::to_string(k) + ": " + err : err;
             delete c;
             return rc;
         }
+        for (uint32_t b = 0; b < h.nb0; ++b) {
+            if (h.bonds[2 * b] >= h.np || h.bonds[2 * b + 1] >= h.np) {
+                g_open_error = "Error: bond " + std::to_string(b) + " names a particle index >= np";
+                delete c;
+                return SPH_E_INPUT;
+            }
+        }
+        total += std::max<uint32_t>(BLOCK, (h.np + BLOCK - 1) / BLOCK * BLOCK);
+        if (!(flags & SPH_OPEN_NOCHECK)) {
+            rc = check_values(c, h);
+            if (rc != SPH_OK) {
+                g_open_error = c->err;
+                delete c;
+                return rc;
+            }
+        }
+        if (h.contactmodel != c->sims[0].h.contactmodel) {
+            g_open_error = "all simulations of a batch must use the same contact model";
+            delete c;
+            return SPH_E_UNSUPPORTED;
+        }
+    }
+    if (total >= MAX_NP) {
+        g_open_error = "np exceeds the supported maximum (2^27)";
+        delete c;
+        return SPH_E_INPUT;
     }
     hipError_t e;
     if (device >= 0) {
@@ -584,10 +704,11 @@ int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sp
         delete c;
         return SPH_E_DEVICE;
     }
-    fill_params(c);
-    c->coop_grid = sort_coop_grid(c->h.np);
-    if (hipHostMalloc((void**)&c->loop_sample, sizeof(DevLoop), hipHostMallocDefault) !=
-            hipSuccess ||
+    memset(&c->A, 0, sizeof c->A);
+    for (Sim& sim : c->sims) fill_params(sim);
+    layout(c);
+    if (hipHostMalloc((void**)&c->loop_sample, c->sims.size() * sizeof(DevLoop),
+                      hipHostMallocDefault) != hipSuccess ||
         hipEventCreateWithFlags(&c->sample_ev, hipEventDisableTiming) != hipSuccess) {
         // Without the sample the automatic placement keeps its first choice.
         (void)hipGetLastError();
@@ -595,14 +716,13 @@ int open_common(const void* bytes, size_t nbytes, int device, unsigned flags, sp
         c->loop_sample = nullptr;
         c->sample_ev = nullptr;
     }
-    rc = allocate(c);
+    int rc = allocate(c);
     if (rc == SPH_OK) rc = upload_state(c);
     if (rc != SPH_OK) {
         g_open_error = c->err;
         sph_close(c);
         return rc;
     }
-    c->iter = 0;
     *out = c;
     return SPH_OK;
 }
@@ -699,19 +819,25 @@ void choose_sort_placement(sph_ctx* c) {
         c->resort_at_end = c->sort_mode == 1;
         return;
     }
+    // A batch re-sorts each simulation in its own workgroup, in parallel,
+    // which beats a batch-wide sort whenever any simulation changes.
+    if (c->sims.size() > 1) {
+        c->resort_at_end = true;
+        return;
+    }
     if (!c->sample_pending || !c->loop_sample) return;
     if (hipEventQuery(c->sample_ev) != hipSuccess) return;
     c->sample_pending = false;
-    const DevLoop& L = *c->loop_sample;
+    const DevLoop& L = c->loop_sample[0];
     const int64_t steps = L.iter - c->sample_iter0;
-    const uint32_t sorts = L.sorts - c->sample_sorts0;
+    const uint64_t sorts = L.sorts - c->sample_sorts0;
     if (steps >= 20) c->resort_at_end = (double)sorts < 0.1 * (double)steps;
     c->sample_iter0 = L.iter;
     c->sample_sorts0 = L.sorts;
 }
 
 void sample_loop(sph_ctx* c) {
-    if (c->sort_mode != 2 || !c->loop_sample || c->sample_pending) return;
+    if (c->sort_mode != 2 || c->sims.size() > 1 || !c->loop_sample || c->sample_pending) return;
     if (hipMemcpyAsync(c->loop_sample, c->A.loop, sizeof(DevLoop), hipMemcpyDeviceToHost,
                        c->stream) != hipSuccess ||
         hipEventRecord(c->sample_ev, c->stream) != hipSuccess) {
@@ -722,134 +848,126 @@ void sample_loop(sph_ctx* c) {
 }
 
 int enqueue_step(sph_ctx* c) {
-    const DevParams& P = c->P;
     const DevArrays& A = c->A;
     hipStream_t s = c->stream;
     Prof pr{c};
     // With an adaptive grid L[2] is the top wall's position at the step
     // start, known only after the previous step's wall update.
-    const bool hash_in_integrate = !(P.adaptive == 1 && P.nw > 0);
-    if (P.np > 0) {
-        const bool hash_now = c->hash_pending || !hash_in_integrate;
-        if (hash_now) {
-            pr.begin("hash");
-            HIPCHK(c, launch_hash(P, A, s), "hash");
-            pr.end();
-            c->hash_pending = false;
-        }
-        // The previous step's walls kernel already re-sorted for this step
-        // unless this step hashes for itself.
-        bool sorted = c->prev_walls_resorted && !hash_now;
-        if (!sorted && c->coop_sort && c->coop_grid) {
-            pr.begin("sort");
-            if (launch_sort_coop(P, A, c->passes, c->sorted_bits, c->coop_grid, s) ==
-                hipSuccess) {
-                sorted = true;
-            } else {
-                // A cooperative launch can be refused at run time (for one,
-                // while another process holds the device's cooperative
-                // resources); nothing was launched, so this step and the
-                // following ones use the per-stage launches.
-                (void)hipGetLastError();
-                c->coop_grid = 0;
-            }
-            pr.end();
-        }
-        if (!sorted) {
-            pr.begin("sort");
-            HIPCHK(c, launch_sort(P, A, c->passes, s), "sort");
-            pr.end();
-            pr.begin("cells");
-            HIPCHK(c, launch_cells(P, A, c->sorted_bits, s), "cells");
-            pr.end();
-        }
-        pr.begin("contact");
-        HIPCHK(c, launch_contact(P, A, c->contact_lanes, s), "contact");
+    const bool hash_in_integrate = !c->any_adaptive;
+    const bool hash_now = c->hash_pending || !hash_in_integrate;
+    if (hash_now) {
+        pr.begin("hash");
+        HIPCHK(c, launch_hash(A, s), "hash");
         pr.end();
-        if (P.nb0 > 0) {
-            pr.begin("bonds");
-            HIPCHK(c, launch_bonds(P, A, s), "bonds");
-            pr.end();
-        }
-        if (c->shear_stress) {
-            pr.begin("shear_stress");
-            HIPCHK(c, launch_shear_stress(P, A, s), "shear_stress");
-            pr.end();
-        }
-        pr.begin("integrate");
-        HIPCHK(c, launch_integrate(P, A, hash_in_integrate ? 1 : 0, s), "integrate");
+        c->hash_pending = false;
+    }
+    // The previous step's walls kernel already re-sorted for this step
+    // unless this step hashes for itself.
+    if (!(c->prev_walls_resorted && !hash_now)) {
+        pr.begin("sort");
+        HIPCHK(c, launch_sort(A, c->passes, s), "sort");
+        pr.end();
+        pr.begin("cells");
+        HIPCHK(c, launch_cells(A, c->sorted_bits, s), "cells");
         pr.end();
     }
-    const bool resort = P.np > 0 && hash_in_integrate && c->resort_at_end;
+    pr.begin("contact");
+    HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
+    pr.end();
+    if (A.nbond > 0) {
+        pr.begin("bonds");
+        HIPCHK(c, launch_bonds(A, s), "bonds");
+        pr.end();
+    }
+    if (c->shear_stress) {
+        pr.begin("shear_stress");
+        HIPCHK(c, launch_shear_stress(A, s), "shear_stress");
+        pr.end();
+    }
+    pr.begin("integrate");
+    HIPCHK(c, launch_integrate(A, hash_in_integrate ? 1 : 0, s), "integrate");
+    pr.end();
+    const bool resort = hash_in_integrate && c->resort_at_end;
     pr.begin("walls");
-    HIPCHK(c, launch_walls(P, A, resort ? 1 : 0, c->passes, c->sorted_bits, s), "walls");
+    HIPCHK(c, launch_walls(A, resort ? 1 : 0, c->passes, c->sorted_bits, s), "walls");
     pr.end();
     c->prev_walls_resorted = resort;
     // Host mirror of the loop counters (src/device.cu:2536-2538).
-    c->h.t += c->h.dt;
-    c->iter += 1;
+    for (Sim& sim : c->sims) {
+        sim.h.t += sim.h.dt;
+        sim.iter += 1;
+    }
     return SPH_OK;
 }
 
-int read_loop(sph_ctx* c, DevLoop* L) {
-    HIPCHK(c, hipMemcpyAsync(L, c->A.loop, sizeof *L, hipMemcpyDeviceToHost, c->stream), "D2H");
+int read_loop(sph_ctx* c, int sim, DevLoop* L) {
+    HIPCHK(c, hipMemcpyAsync(L, c->A.loop + sim, sizeof *L, hipMemcpyDeviceToHost, c->stream),
+           "D2H");
     HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
     return SPH_OK;
 }
 
 int check_device_flags(sph_ctx* c) {
-    DevLoop L;
-    int rc = read_loop(c, &L);
-    if (rc) return rc;
-    if (L.error_bits & DFLAG_KEY_RANGE)
-        return fail(c, SPH_E_INPUT,
-                    "One or more particles moved outside the grid.\nThis could possibly be "
-                    "caused by a numerical instability.\nIs the computational time step too large?");
+    std::vector<DevLoop> L(c->sims.size());
+    HIPCHK(c, hipMemcpyAsync(L.data(), c->A.loop, L.size() * sizeof(DevLoop),
+                             hipMemcpyDeviceToHost, c->stream), "D2H");
+    HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+    for (size_t k = 0; k < L.size(); ++k)
+        if (L[k].error_bits & DFLAG_KEY_RANGE)
+            return fail(c, SPH_E_INPUT,
+                        "%sOne or more particles moved outside the grid.\nThis could possibly "
+                        "be caused by a numerical instability.\nIs the computational time step "
+                        "too large?",
+                        L.size() > 1 ? ("simulation " + std::to_string(k) + ": ").c_str() : "");
     return SPH_OK;
 }
 
-// D2H of all state the .bin carries.
-int download_state(sph_ctx* c) {
-    Host& h = c->h;
+// D2H of all state the .bin carries, for simulation k.
+int download_state(sph_ctx* c, int k) {
+    Sim& sim = c->sims[k];
+    Host& h = sim.h;
     const size_t np = h.np, nw = h.nw, nb = h.nb0;
+    const size_t b0 = sim.P.base, w0 = sim.P.wall_base, bb = sim.P.bond_base;
     hipStream_t s = c->stream;
     const DevArrays& A = c->A;
 #define D2H(dst, src, bytes) \
     HIPCHK(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToHost, s), "D2H")
     if (np) {
-        D2H(h.x.data(), A.x, np * 32);
-        D2H(h.vel.data(), A.vel, np * 32);
-        D2H(h.angvel.data(), A.angvel, np * 32);
-        D2H(h.angpos.data(), A.angpos, np * 32);
-        D2H(h.xyzsum.data(), A.xyzsum, np * 32);
-        D2H(h.force.data(), A.force, np * 32);
-        D2H(h.torque.data(), A.torque, np * 32);
-        D2H(h.es_dot.data(), A.es_dot, np * 8);
-        D2H(h.es.data(), A.es, np * 8);
-        D2H(h.ev_dot.data(), A.ev_dot, np * 8);
-        D2H(h.ev.data(), A.ev, np * 8);
-        D2H(h.p.data(), A.p, np * 8);
+        D2H(h.x.data(), A.x + b0, np * 32);
+        D2H(h.vel.data(), A.vel + b0, np * 32);
+        D2H(h.angvel.data(), A.angvel + b0, np * 32);
+        D2H(h.angpos.data(), A.angpos + b0, np * 32);
+        D2H(h.xyzsum.data(), A.xyzsum + b0, np * 32);
+        D2H(h.force.data(), A.force + b0, np * 32);
+        D2H(h.torque.data(), A.torque + b0, np * 32);
+        D2H(h.es_dot.data(), A.es_dot + b0, np * 8);
+        D2H(h.es.data(), A.es + b0, np * 8);
+        D2H(h.ev_dot.data(), A.ev_dot + b0, np * 8);
+        D2H(h.ev.data(), A.ev + b0, np * 8);
+        D2H(h.p.data(), A.p + b0, np * 8);
     }
     if (nw) {
-        D2H(h.w_nx.data(), A.w_nx, nw * 32);
-        D2H(h.w_mvfd.data(), A.w_mvfd, nw * 32);
+        D2H(h.w_nx.data(), A.w_nx + w0, nw * 32);
+        D2H(h.w_mvfd.data(), A.w_mvfd + w0, nw * 32);
     }
     if (nb) {
-        D2H(h.bonds.data(), A.bonds, nb * 8);
-        D2H(h.bonds_delta.data(), A.bonds_delta, nb * 32);
-        D2H(h.bonds_omega.data(), A.bonds_omega, nb * 32);
+        D2H(h.bonds.data(), A.bonds + 2 * bb, nb * 8);
+        D2H(h.bonds_delta.data(), A.bonds_delta + bb, nb * 32);
+        D2H(h.bonds_omega.data(), A.bonds_omega + bb, nb * 32);
     }
 #undef D2H
     DevLoop L;
-    int rc = read_loop(c, &L);
+    int rc = read_loop(c, k, &L);
     if (rc) return rc;
+    for (size_t i = 0; i < 2 * nb; ++i) h.bonds[i] -= (uint32_t)b0;  // batch -> .bin index
     h.t = L.t;
     if (h.adaptive == 1 && h.nw > 0 && L.iter > 0) h.L[2] = L.grid_Lz;
     return SPH_OK;
 }
 
-int64_t field_bytes(const sph_ctx* c, int f) {
-    const int64_t np = c->h.np, nw = c->h.nw, nb = c->h.nb0;
+int64_t field_bytes(const sph_ctx* c, int k, int f) {
+    const Sim& sim = c->sims[k];
+    const int64_t np = sim.h.np, nw = sim.h.nw, nb = sim.h.nb0;
     switch (f) {
         case SPH_F_X: case SPH_F_VEL: return np * 32;
         case SPH_F_ANGVEL: case SPH_F_ANGPOS: case SPH_F_FORCE: case SPH_F_TORQUE:
@@ -860,7 +978,7 @@ int64_t field_bytes(const sph_ctx* c, int f) {
         case SPH_F_DELTA_T: return np * NC * 24;
         case SPH_F_DISTMOD: return np * NC * 32;
         case SPH_F_CELLID: case SPH_F_PERM: return np * 4;
-        case SPH_F_CELLSTART: case SPH_F_CELLEND: return (int64_t)c->P.ncells * 4;
+        case SPH_F_CELLSTART: case SPH_F_CELLEND: return (int64_t)sim.P.ncells * 4;
         case SPH_F_WALLS_NX: case SPH_F_WALLS_MVFD: return nw * 32;
         case SPH_F_WALLS_ACC: return nw * 8;
         case SPH_F_WALLS_FORCE_PP: return nw * np * 8;
@@ -924,7 +1042,8 @@ int mkdir_p(const std::string& path) {
 int write_status(sph_ctx* c, const std::string& fn) {
     FILE* fp = fopen(fn.c_str(), "w");
     if (!fp) return fail(c, SPH_E_IO, "could not write %s", fn.c_str());
-    fprintf(fp, "%2.4e %2.4e %d\n", c->h.t, 100.0 * c->h.t / c->h.t_total, c->h.step_count);
+    const Host& h = c->h0();
+    fprintf(fp, "%2.4e %2.4e %d\n", h.t, 100.0 * h.t / h.t_total, h.step_count);
     fclose(fp);
     return SPH_OK;
 }
@@ -936,24 +1055,60 @@ extern "C" {
 
 int sph_open_mem(const void* bytes, size_t nbytes, int device, unsigned flags, sph_ctx** out) {
     if (!out || (!bytes && nbytes)) return SPH_E_ARG;
-    return open_common(bytes, nbytes, device, flags, out);
+    return open_images({{bytes, nbytes}}, device, flags, out);
 }
 
-int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out) {
-    if (!out || !bin_path) return SPH_E_ARG;
-    *out = nullptr;
-    FILE* fp = fopen(bin_path, "rb");
+namespace {
+int read_file(const char* path, std::vector<unsigned char>& buf) {
+    FILE* fp = fopen(path, "rb");
     if (!fp) {
-        g_open_error = std::string("Could not read input binary file '") + bin_path + "'";
+        g_open_error = std::string("Could not read input binary file '") + path + "'";
         return SPH_E_IO;
     }
-    std::vector<unsigned char> buf;
     unsigned char tmp[1 << 16];
     size_t k;
     while ((k = fread(tmp, 1, sizeof tmp, fp)) > 0) buf.insert(buf.end(), tmp, tmp + k);
     fclose(fp);
-    return open_common(buf.data(), buf.size(), device, flags, out);
+    return SPH_OK;
 }
+}  // namespace
+
+int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out) {
+    if (!out || !bin_path) return SPH_E_ARG;
+    *out = nullptr;
+    std::vector<unsigned char> buf;
+    int rc = read_file(bin_path, buf);
+    if (rc) return rc;
+    return open_images({{buf.data(), buf.size()}}, device, flags, out);
+}
+
+int sph_open_batch_mem(const void* const* images, const size_t* sizes, int nsims, int device,
+                       unsigned flags, sph_ctx** out) {
+    if (!out || !images || !sizes || nsims <= 0) return SPH_E_ARG;
+    std::vector<std::pair<const void*, size_t>> v;
+    for (int k = 0; k < nsims; ++k) {
+        if (!images[k] && sizes[k]) return SPH_E_ARG;
+        v.push_back({images[k], sizes[k]});
+    }
+    return open_images(v, device, flags, out);
+}
+
+int sph_open_batch(const char* const* bin_paths, int nsims, int device, unsigned flags,
+                   sph_ctx** out) {
+    if (!out || !bin_paths || nsims <= 0) return SPH_E_ARG;
+    *out = nullptr;
+    std::vector<std::vector<unsigned char>> bufs(nsims);
+    std::vector<std::pair<const void*, size_t>> v;
+    for (int k = 0; k < nsims; ++k) {
+        if (!bin_paths[k]) return SPH_E_ARG;
+        int rc = read_file(bin_paths[k], bufs[k]);
+        if (rc) return rc;
+        v.push_back({bufs[k].data(), bufs[k].size()});
+    }
+    return open_images(v, device, flags, out);
+}
+
+int sph_num_sims(const sph_ctx* c) { return c ? (int)c->sims.size() : SPH_E_ARG; }
 
 int sph_step(sph_ctx* c, uint64_t nsteps) {
     if (!c) return SPH_E_ARG;
@@ -972,23 +1127,31 @@ int sph_sync(sph_ctx* c) {
     return check_device_flags(c);
 }
 
-int sph_write_mem(sph_ctx* c, void* dst, size_t cap, size_t* nbytes) {
-    if (!c) return SPH_E_ARG;
-    int rc = download_state(c);
+#define CHECK_SIM(c, sim) \
+    do { if (!(c) || (sim) < 0 || (size_t)(sim) >= (c)->sims.size()) return SPH_E_ARG; } while (0)
+
+int sph_write_mem_sim(sph_ctx* c, int sim, void* dst, size_t cap, size_t* nbytes) {
+    CHECK_SIM(c, sim);
+    int rc = download_state(c, sim);
     if (rc) return rc;
     Writer w;
-    write_bin(c->h, w);
+    write_bin(c->sims[sim].h, w);
     if (nbytes) *nbytes = w.b.size();
     if (dst && cap >= w.b.size()) memcpy(dst, w.b.data(), w.b.size());
     return SPH_OK;
 }
 
-int sph_write_bin(sph_ctx* c, const char* path) {
-    if (!c || !path) return SPH_E_ARG;
-    int rc = download_state(c);
+int sph_write_mem(sph_ctx* c, void* dst, size_t cap, size_t* nbytes) {
+    return sph_write_mem_sim(c, 0, dst, cap, nbytes);
+}
+
+int sph_write_bin_sim(sph_ctx* c, int sim, const char* path) {
+    CHECK_SIM(c, sim);
+    if (!path) return SPH_E_ARG;
+    int rc = download_state(c, sim);
     if (rc) return rc;
     Writer w;
-    write_bin(c->h, w);
+    write_bin(c->sims[sim].h, w);
     FILE* fp = fopen(path, "wb");
     if (!fp) return fail(c, SPH_E_IO, "could create output binary file '%s'", path);
     const size_t k = fwrite(w.b.data(), 1, w.b.size(), fp);
@@ -997,9 +1160,14 @@ int sph_write_bin(sph_ctx* c, const char* path) {
     return SPH_OK;
 }
 
+int sph_write_bin(sph_ctx* c, const char* path) { return sph_write_bin_sim(c, 0, path); }
+
 int sph_run(sph_ctx* c, const char* out_dir, const char* sid) {
     if (!c || !out_dir || !sid) return SPH_E_ARG;
-    Host& h = c->h;
+    if (c->sims.size() != 1)
+        return fail(c, SPH_E_UNSUPPORTED, "sph_run drives one simulation; step a batch with "
+                    "sph_step and write each simulation with sph_write_bin_sim");
+    Host& h = c->h0();
     const std::string dir(out_dir);
     mkdir_p(dir);
     const bool verbose = !(c->flags & SPH_OPEN_QUIET);
@@ -1055,234 +1223,294 @@ int sph_run(sph_ctx* c, const char* out_dir, const char* sid) {
     return SPH_OK;
 }
 
-int64_t sph_field_bytes(sph_ctx* c, int field) {
-    if (!c) return SPH_E_ARG;
-    return field_bytes(c, field);
+int64_t sph_field_bytes_sim(sph_ctx* c, int sim, int field) {
+    CHECK_SIM(c, sim);
+    return field_bytes(c, sim, field);
 }
 
-int sph_get_field(sph_ctx* c, int f, void* dst, size_t bytes) {
-    if (!c) return SPH_E_ARG;
-    const int64_t need = field_bytes(c, f);
+int64_t sph_field_bytes(sph_ctx* c, int field) { return sph_field_bytes_sim(c, 0, field); }
+
+int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
+    CHECK_SIM(c, sim);
+    const int64_t need = field_bytes(c, sim, f);
     if (need < 0 || (size_t)need != bytes || (!dst && bytes))
         return fail(c, SPH_E_ARG, "field %d: expected %lld bytes", f, (long long)need);
-    const size_t np = c->h.np;
+    const Sim& S = c->sims[sim];
+    const size_t np = S.h.np, b0 = S.P.base, w0 = S.P.wall_base, bb = S.P.bond_base;
     const DevArrays& A = c->A;
     switch (f) {
-        case SPH_F_X: return d2h(c, dst, A.x, bytes);
-        case SPH_F_VEL: return d2h(c, dst, A.vel, bytes);
-        case SPH_F_ANGVEL: return get_vec3(c, A.angvel, np, dst);
-        case SPH_F_ANGPOS: return get_vec3(c, A.angpos, np, dst);
-        case SPH_F_FORCE: return get_vec3(c, A.force, np, dst);
-        case SPH_F_TORQUE: return get_vec3(c, A.torque, np, dst);
-        case SPH_F_XYZSUM: return get_vec3(c, A.xyzsum, np, dst);
-        case SPH_F_ACC: return get_vec3(c, A.acc, np, dst);
-        case SPH_F_ANGACC: return get_vec3(c, A.angacc, np, dst);
-        case SPH_F_ES_DOT: return d2h(c, dst, A.es_dot, bytes);
-        case SPH_F_ES: return d2h(c, dst, A.es, bytes);
-        case SPH_F_EV_DOT: return d2h(c, dst, A.ev_dot, bytes);
-        case SPH_F_EV: return d2h(c, dst, A.ev, bytes);
-        case SPH_F_P: return d2h(c, dst, A.p, bytes);
+        case SPH_F_X: return d2h(c, dst, A.x + b0, bytes);
+        case SPH_F_VEL: return d2h(c, dst, A.vel + b0, bytes);
+        case SPH_F_ANGVEL: return get_vec3(c, A.angvel + b0, np, dst);
+        case SPH_F_ANGPOS: return get_vec3(c, A.angpos + b0, np, dst);
+        case SPH_F_FORCE: return get_vec3(c, A.force + b0, np, dst);
+        case SPH_F_TORQUE: return get_vec3(c, A.torque + b0, np, dst);
+        case SPH_F_XYZSUM: return get_vec3(c, A.xyzsum + b0, np, dst);
+        case SPH_F_ACC: return get_vec3(c, A.acc + b0, np, dst);
+        case SPH_F_ANGACC: return get_vec3(c, A.angacc + b0, np, dst);
+        case SPH_F_ES_DOT: return d2h(c, dst, A.es_dot + b0, bytes);
+        case SPH_F_ES: return d2h(c, dst, A.es + b0, bytes);
+        case SPH_F_EV_DOT: return d2h(c, dst, A.ev_dot + b0, bytes);
+        case SPH_F_EV: return d2h(c, dst, A.ev + b0, bytes);
+        case SPH_F_P: return d2h(c, dst, A.p + b0, bytes);
         case SPH_F_CONTACTS: {
-            int rc = d2h(c, dst, A.crow, bytes);
+            // The reference's row format: partner's .bin index, np when empty.
+            int rc = d2h(c, dst, A.crow + b0 * NC, bytes);
             if (rc) return rc;
             uint32_t* w = static_cast<uint32_t*>(dst);
-            for (size_t i = 0; i < np * NC; ++i) w[i] &= CW_IDX_MASK;
+            for (size_t i = 0; i < np * NC; ++i) {
+                const uint32_t idx = w[i] & CW_IDX_MASK;
+                w[i] = idx == EMPTY ? (uint32_t)np : idx - (uint32_t)b0;
+            }
             return SPH_OK;
         }
-        case SPH_F_DELTA_T: return get_vec3(c, A.delta_t, np * NC, dst);
+        case SPH_F_DELTA_T: return get_vec3(c, A.delta_t + b0 * NC, np * NC, dst);
         case SPH_F_DISTMOD: {
             // Reconstructed from the packed image code and the partner's
             // radius; meaningful where the slot is occupied.
             std::vector<uint32_t> rows(np * NC);
             std::vector<double> x(np * 4);
-            int rc = d2h(c, rows.data(), A.crow, np * NC * 4);
-            if (!rc) rc = d2h(c, x.data(), A.x, np * 32);
+            int rc = d2h(c, rows.data(), A.crow + b0 * NC, np * NC * 4);
+            if (!rc) rc = d2h(c, x.data(), A.x + b0, np * 32);
             if (rc) return rc;
             double* d = static_cast<double*>(dst);
             for (size_t k = 0; k < np * NC; ++k) {
-                const uint32_t wv = rows[k], idx = wv & CW_IDX_MASK;
+                const uint32_t wv = rows[k], idx = (wv & CW_IDX_MASK) - (uint32_t)b0;
                 const uint32_t cx = (wv >> CW_SHIFT_X) & 3u, cy = (wv >> CW_SHIFT_Y) & 3u;
-                d[4 * k] = cx == 1 ? c->h.L[0] : cx == 2 ? -c->h.L[0] : 0.0;
-                d[4 * k + 1] = cy == 1 ? c->h.L[1] : cy == 2 ? -c->h.L[1] : 0.0;
+                d[4 * k] = cx == 1 ? S.h.L[0] : cx == 2 ? -S.h.L[0] : 0.0;
+                d[4 * k + 1] = cy == 1 ? S.h.L[1] : cy == 2 ? -S.h.L[1] : 0.0;
                 d[4 * k + 2] = 0.0;
                 d[4 * k + 3] = idx < np ? x[4 * idx + 3] : 0.0;
             }
             return SPH_OK;
         }
-        case SPH_F_CELLID: return d2h(c, dst, A.skey, bytes);
-        case SPH_F_PERM: return d2h(c, dst, A.perm, bytes);
-        case SPH_F_CELLSTART: return d2h(c, dst, A.cell_start, bytes);
-        case SPH_F_CELLEND: return d2h(c, dst, A.cell_end, bytes);
-        case SPH_F_WALLS_NX: return d2h(c, dst, A.w_nx, bytes);
-        case SPH_F_WALLS_MVFD: return d2h(c, dst, A.w_mvfd, bytes);
-        case SPH_F_WALLS_ACC: return d2h(c, dst, A.w_acc, bytes);
-        case SPH_F_WALLS_FORCE_PP: return d2h(c, dst, A.force_pp, bytes);
-        case SPH_F_BONDS: return d2h(c, dst, A.bonds, bytes);
-        case SPH_F_BONDS_DELTA: return d2h(c, dst, A.bonds_delta, bytes);
-        case SPH_F_BONDS_OMEGA: return d2h(c, dst, A.bonds_omega, bytes);
+        case SPH_F_CELLID: {
+            int rc = d2h(c, dst, A.skey + b0, bytes);
+            uint32_t* w = static_cast<uint32_t*>(dst);
+            for (size_t i = 0; !rc && i < np; ++i) w[i] -= S.P.cell_base;
+            return rc;
+        }
+        case SPH_F_PERM: {
+            int rc = d2h(c, dst, A.perm + b0, bytes);
+            uint32_t* w = static_cast<uint32_t*>(dst);
+            for (size_t i = 0; !rc && i < np; ++i) w[i] -= (uint32_t)b0;
+            return rc;
+        }
+        case SPH_F_CELLSTART:
+        case SPH_F_CELLEND: {
+            const uint32_t* src = (f == SPH_F_CELLSTART ? A.cell_start : A.cell_end);
+            int rc = d2h(c, dst, src + S.P.cell_base, bytes);
+            uint32_t* w = static_cast<uint32_t*>(dst);
+            for (size_t i = 0; !rc && i < S.P.ncells; ++i)
+                if (w[i] != 0xffffffffu) w[i] -= (uint32_t)b0;
+            return rc;
+        }
+        case SPH_F_WALLS_NX: return d2h(c, dst, A.w_nx + w0, bytes);
+        case SPH_F_WALLS_MVFD: return d2h(c, dst, A.w_mvfd + w0, bytes);
+        case SPH_F_WALLS_ACC: return d2h(c, dst, A.w_acc + w0, bytes);
+        case SPH_F_WALLS_FORCE_PP: {
+            char* d = static_cast<char*>(dst);
+            for (uint32_t w = 0; w < S.h.nw; ++w) {
+                int rc = d2h(c, d + (size_t)w * np * 8, A.force_pp + (size_t)w * A.ntot + b0,
+                             np * 8);
+                if (rc) return rc;
+            }
+            return SPH_OK;
+        }
+        case SPH_F_BONDS: {
+            int rc = d2h(c, dst, A.bonds + 2 * bb, bytes);
+            uint32_t* w = static_cast<uint32_t*>(dst);
+            for (size_t i = 0; !rc && i < 2 * (size_t)S.h.nb0; ++i) w[i] -= (uint32_t)b0;
+            return rc;
+        }
+        case SPH_F_BONDS_DELTA: return d2h(c, dst, A.bonds_delta + bb, bytes);
+        case SPH_F_BONDS_OMEGA: return d2h(c, dst, A.bonds_omega + bb, bytes);
         case SPH_F_COLOR:
-            if (bytes) memcpy(dst, c->h.color.data(), bytes);
+            if (bytes) memcpy(dst, S.h.color.data(), bytes);
             return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown field %d", f);
     }
 }
 
-int sph_set_field(sph_ctx* c, int f, const void* src, size_t bytes) {
-    if (!c) return SPH_E_ARG;
-    const int64_t need = field_bytes(c, f);
+int sph_get_field(sph_ctx* c, int f, void* dst, size_t bytes) {
+    return sph_get_field_sim(c, 0, f, dst, bytes);
+}
+
+namespace {
+int set_loop_flag(sph_ctx* c, int sim, uint32_t DevLoop::*flag) {
+    const uint32_t one = 1u;
+    return h2d(c, &(c->A.loop[sim].*flag), &one, 4);
+}
+}  // namespace
+
+int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes) {
+    CHECK_SIM(c, sim);
+    const int64_t need = field_bytes(c, sim, f);
     if (need < 0 || (size_t)need != bytes || (!src && bytes))
         return fail(c, SPH_E_ARG, "field %d: expected %lld bytes", f, (long long)need);
-    const size_t np = c->h.np;
+    Sim& S = c->sims[sim];
+    const size_t np = S.h.np, b0 = S.P.base, w0 = S.P.wall_base, bb = S.P.bond_base;
     DevArrays& A = c->A;
     switch (f) {
-        case SPH_F_X: {
-            // New positions: the next step re-sorts and rebuilds the lists.
-            int rc = h2d(c, A.x, src, bytes);
+        case SPH_F_X:
+        case SPH_F_VEL:
+        case SPH_F_ANGVEL: {
+            // New positions or velocities: the next step hashes and re-sorts
+            // (refreshing the sorted copies the contact search reads), and
+            // new positions rebuild the candidate lists.
+            int rc = f == SPH_F_X     ? h2d(c, A.x + b0, src, bytes)
+                     : f == SPH_F_VEL ? h2d(c, A.vel + b0, src, bytes)
+                                      : set_vec3(c, A.angvel + b0, np, src);
             if (rc) return rc;
-            HIPCHK(c, hipMemsetAsync(A.key_prev, 0xff, np * 4, c->stream), "memset");
+            HIPCHK(c, hipMemsetAsync(A.key_prev + b0, 0xff, np * 4, c->stream), "memset");
             c->hash_pending = true;
-            const uint32_t one = 1u;
-            return h2d(c, &A.loop->force_rebuild, &one, 4);
+            return f == SPH_F_X ? set_loop_flag(c, sim, &DevLoop::force_rebuild) : SPH_OK;
         }
-        case SPH_F_VEL: return h2d(c, A.vel, src, bytes);
-        case SPH_F_ANGVEL: return set_vec3(c, A.angvel, np, src);
-        case SPH_F_ANGPOS: return set_vec3(c, A.angpos, np, src);
-        case SPH_F_FORCE: return set_vec3(c, A.force, np, src);
-        case SPH_F_TORQUE: return set_vec3(c, A.torque, np, src);
-        case SPH_F_XYZSUM: return set_vec3(c, A.xyzsum, np, src);
-        case SPH_F_ACC: return set_vec3(c, A.acc, np, src);
-        case SPH_F_ANGACC: return set_vec3(c, A.angacc, np, src);
-        case SPH_F_ES_DOT: return h2d(c, A.es_dot, src, bytes);
-        case SPH_F_ES: return h2d(c, A.es, src, bytes);
-        case SPH_F_EV_DOT: return h2d(c, A.ev_dot, src, bytes);
-        case SPH_F_EV: return h2d(c, A.ev, src, bytes);
-        case SPH_F_P: return h2d(c, A.p, src, bytes);
+        case SPH_F_ANGPOS: return set_vec3(c, A.angpos + b0, np, src);
+        case SPH_F_FORCE: return set_vec3(c, A.force + b0, np, src);
+        case SPH_F_TORQUE: return set_vec3(c, A.torque + b0, np, src);
+        case SPH_F_XYZSUM: return set_vec3(c, A.xyzsum + b0, np, src);
+        case SPH_F_ACC: return set_vec3(c, A.acc + b0, np, src);
+        case SPH_F_ANGACC: return set_vec3(c, A.angacc + b0, np, src);
+        case SPH_F_ES_DOT: return h2d(c, A.es_dot + b0, src, bytes);
+        case SPH_F_ES: return h2d(c, A.es + b0, src, bytes);
+        case SPH_F_EV_DOT: return h2d(c, A.ev_dot + b0, src, bytes);
+        case SPH_F_EV: return h2d(c, A.ev + b0, src, bytes);
+        case SPH_F_P: return h2d(c, A.p + b0, src, bytes);
         case SPH_F_CONTACTS: {
             // Keep each slot's image code; replace the partner index.
             std::vector<uint32_t> rows(np * NC);
-            int rc = d2h(c, rows.data(), A.crow, bytes);
+            int rc = d2h(c, rows.data(), A.crow + b0 * NC, bytes);
             if (rc) return rc;
             const uint32_t* s = static_cast<const uint32_t*>(src);
             for (size_t k = 0; k < np * NC; ++k) {
                 if (s[k] > np) return fail(c, SPH_E_ARG, "contact index out of range");
-                rows[k] = (rows[k] & ~CW_IDX_MASK) | s[k];
+                const uint32_t idx = s[k] == np ? EMPTY : s[k] + (uint32_t)b0;
+                rows[k] = (rows[k] & ~CW_IDX_MASK) | idx;
             }
-            rc = h2d(c, A.crow, rows.data(), bytes);
+            rc = h2d(c, A.crow + b0 * NC, rows.data(), bytes);
             if (rc) return rc;
-            const uint32_t one = 1u;
-            return h2d(c, &A.loop->zero_empty, &one, 4);
+            return set_loop_flag(c, sim, &DevLoop::zero_empty);
         }
         case SPH_F_DELTA_T: {
-            int rc = set_vec3(c, A.delta_t, np * NC, src);
+            int rc = set_vec3(c, A.delta_t + b0 * NC, np * NC, src);
             if (rc) return rc;
-            const uint32_t one = 1u;
-            return h2d(c, &A.loop->zero_empty, &one, 4);
+            return set_loop_flag(c, sim, &DevLoop::zero_empty);
         }
         case SPH_F_DISTMOD: {
             std::vector<uint32_t> rows(np * NC);
-            int rc = d2h(c, rows.data(), A.crow, np * NC * 4);
+            int rc = d2h(c, rows.data(), A.crow + b0 * NC, np * NC * 4);
             if (rc) return rc;
             const double* d = static_cast<const double*>(src);
             for (size_t k = 0; k < np * NC; ++k) {
                 uint32_t code = 0;
                 const double sx = d[4 * k], sy = d[4 * k + 1];
-                if (sx == c->h.L[0]) code |= 1u << CW_SHIFT_X;
-                else if (sx == -c->h.L[0]) code |= 2u << CW_SHIFT_X;
+                if (sx == S.h.L[0]) code |= 1u << CW_SHIFT_X;
+                else if (sx == -S.h.L[0]) code |= 2u << CW_SHIFT_X;
                 else if (sx != 0.0) return fail(c, SPH_E_ARG, "distmod x must be 0 or +-L[0]");
-                if (sy == c->h.L[1]) code |= 1u << CW_SHIFT_Y;
-                else if (sy == -c->h.L[1]) code |= 2u << CW_SHIFT_Y;
+                if (sy == S.h.L[1]) code |= 1u << CW_SHIFT_Y;
+                else if (sy == -S.h.L[1]) code |= 2u << CW_SHIFT_Y;
                 else if (sy != 0.0) return fail(c, SPH_E_ARG, "distmod y must be 0 or +-L[1]");
                 if (d[4 * k + 2] != 0.0) return fail(c, SPH_E_ARG, "distmod z must be 0");
                 rows[k] = (rows[k] & CW_IDX_MASK) | code;
             }
-            return h2d(c, A.crow, rows.data(), np * NC * 4);
+            return h2d(c, A.crow + b0 * NC, rows.data(), np * NC * 4);
         }
         case SPH_F_WALLS_NX: {
-            int rc = h2d(c, A.w_nx, src, bytes);
-            if (!rc && bytes) memcpy(c->h.w_nx.data(), src, bytes);
+            int rc = h2d(c, A.w_nx + w0, src, bytes);
+            if (!rc && bytes) memcpy(S.h.w_nx.data(), src, bytes);
             return rc;
         }
         case SPH_F_WALLS_MVFD: {
-            int rc = h2d(c, A.w_mvfd, src, bytes);
-            if (!rc && bytes) memcpy(c->h.w_mvfd.data(), src, bytes);
+            int rc = h2d(c, A.w_mvfd + w0, src, bytes);
+            if (!rc && bytes) memcpy(S.h.w_mvfd.data(), src, bytes);
             return rc;
         }
-        case SPH_F_WALLS_ACC: return h2d(c, A.w_acc, src, bytes);
-        case SPH_F_BONDS_DELTA: return h2d(c, A.bonds_delta, src, bytes);
-        case SPH_F_BONDS_OMEGA: return h2d(c, A.bonds_omega, src, bytes);
+        case SPH_F_WALLS_ACC: return h2d(c, A.w_acc + w0, src, bytes);
+        case SPH_F_BONDS_DELTA: return h2d(c, A.bonds_delta + bb, src, bytes);
+        case SPH_F_BONDS_OMEGA: return h2d(c, A.bonds_omega + bb, src, bytes);
         case SPH_F_COLOR:
-            if (bytes) memcpy(c->h.color.data(), src, bytes);
+            if (bytes) memcpy(S.h.color.data(), src, bytes);
             return SPH_OK;
         default: return fail(c, SPH_E_ARG, "field %d is read-only", f);
     }
 }
 
-int sph_get_scalar(sph_ctx* c, int which, double* v) {
-    if (!c || !v) return SPH_E_ARG;
+int sph_set_field(sph_ctx* c, int f, const void* src, size_t bytes) {
+    return sph_set_field_sim(c, 0, f, src, bytes);
+}
+
+int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
+    CHECK_SIM(c, sim);
+    if (!v) return SPH_E_ARG;
+    const Sim& S = c->sims[sim];
     switch (which) {
-        case SPH_S_TIME: {
-            DevLoop L;
-            int rc = read_loop(c, &L);
-            if (rc) return rc;
-            *v = L.t;
-            return SPH_OK;
-        }
-        case SPH_S_TIME_TOTAL: *v = c->h.t_total; return SPH_OK;
-        case SPH_S_DT: *v = c->h.dt; return SPH_OK;
-        case SPH_S_FILE_DT: *v = c->h.file_dt; return SPH_OK;
-        case SPH_S_ITER: {
-            DevLoop L;
-            int rc = read_loop(c, &L);
-            if (rc) return rc;
-            *v = (double)L.iter;
-            return SPH_OK;
-        }
-        case SPH_S_STEP_COUNT: *v = c->h.step_count; return SPH_OK;
-        case SPH_S_NP: *v = c->h.np; return SPH_OK;
-        case SPH_S_NW: *v = c->h.nw; return SPH_OK;
-        case SPH_S_NB0: *v = c->h.nb0; return SPH_OK;
-        case SPH_S_NCELLS: *v = c->P.ncells; return SPH_OK;
-        case SPH_S_VL_SKIN: *v = c->P.vl_skin; return SPH_OK;
+        case SPH_S_TIME:
+        case SPH_S_ITER:
         case SPH_S_SORTS:
         case SPH_S_VL_BUILDS: {
             DevLoop L;
-            int rc = read_loop(c, &L);
+            int rc = read_loop(c, sim, &L);
             if (rc) return rc;
-            *v = which == SPH_S_SORTS ? (double)L.sorts : (double)L.vl_builds;
+            *v = which == SPH_S_TIME    ? L.t
+                 : which == SPH_S_ITER  ? (double)L.iter
+                 : which == SPH_S_SORTS ? (double)L.sorts
+                                        : (double)L.vl_builds;
             return SPH_OK;
         }
-        case SPH_S_COOP_SORT: *v = (c->coop_sort && c->coop_grid) ? 1.0 : 0.0; return SPH_OK;
+        case SPH_S_TIME_TOTAL: *v = S.h.t_total; return SPH_OK;
+        case SPH_S_DT: *v = S.h.dt; return SPH_OK;
+        case SPH_S_FILE_DT: *v = S.h.file_dt; return SPH_OK;
+        case SPH_S_STEP_COUNT: *v = S.h.step_count; return SPH_OK;
+        case SPH_S_NP: *v = S.h.np; return SPH_OK;
+        case SPH_S_NW: *v = S.h.nw; return SPH_OK;
+        case SPH_S_NB0: *v = S.h.nb0; return SPH_OK;
+        case SPH_S_NCELLS: *v = S.P.ncells; return SPH_OK;
+        case SPH_S_VL_SKIN: *v = S.P.vl_skin; return SPH_OK;
         case SPH_S_CONTACT_LANES: *v = c->contact_lanes; return SPH_OK;
         case SPH_S_SORT_MODE: *v = c->sort_mode; return SPH_OK;
-        case SPH_S_RESORT_MERGE_MAX: *v = c->P.mv_max; return SPH_OK;
+        case SPH_S_RESORT_MERGE_MAX: *v = S.P.mv_max; return SPH_OK;
+        case SPH_S_NSIMS: *v = (double)c->sims.size(); return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
 
-int sph_set_scalar(sph_ctx* c, int which, double v) {
-    if (!c) return SPH_E_ARG;
+int sph_get_scalar(sph_ctx* c, int which, double* v) { return sph_get_scalar_sim(c, 0, which, v); }
+
+int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
+    CHECK_SIM(c, sim);
+    Sim& S = c->sims[sim];
     switch (which) {
         case SPH_S_TIME: {
-            c->h.t = v;
-            return h2d(c, &c->A.loop->t, &v, 8);
+            S.h.t = v;
+            return h2d(c, &c->A.loop[sim].t, &v, 8);
         }
-        case SPH_S_TIME_TOTAL: c->h.t_total = v; return SPH_OK;
-        case SPH_S_DT: c->h.dt = v; c->P.dt = v; return SPH_OK;
-        case SPH_S_FILE_DT: c->h.file_dt = v; return SPH_OK;
+        case SPH_S_TIME_TOTAL: S.h.t_total = v; return SPH_OK;
+        case SPH_S_DT:
+            S.h.dt = v;
+            S.P.dt = v;
+            return upload_params(c);
+        case SPH_S_FILE_DT: S.h.file_dt = v; return SPH_OK;
         case SPH_S_ITER: {
-            // The per-step flags are kept by iteration parity: hash again.
+            // The per-step flags are kept by iteration parity, which all
+            // simulations of a context share: the counter is set for all of
+            // them, and the next step hashes again.
             const int64_t it = (int64_t)v;
-            c->iter = it;
             c->hash_pending = true;
-            return h2d(c, &c->A.loop->iter, &it, 8);
+            for (size_t k = 0; k < c->sims.size(); ++k) {
+                c->sims[k].iter = it;
+                int rc = h2d(c, &c->A.loop[k].iter, &it, 8);
+                if (rc) return rc;
+            }
+            return SPH_OK;
         }
-        case SPH_S_STEP_COUNT: c->h.step_count = (uint32_t)v; return SPH_OK;
+        case SPH_S_STEP_COUNT: S.h.step_count = (uint32_t)v; return SPH_OK;
         case SPH_S_VL_SKIN: {
-            set_skin(c, v);
-            const uint32_t one = 1u;
-            return h2d(c, &c->A.loop->force_rebuild, &one, 4);
+            set_skin(S.P, v);
+            int rc = upload_params(c);
+            if (rc) return rc;
+            return set_loop_flag(c, sim, &DevLoop::force_rebuild);
         }
-        case SPH_S_COOP_SORT: c->coop_sort = v != 0.0; return SPH_OK;
         case SPH_S_CONTACT_LANES:
             if (v != 1.0 && v != 4.0 && v != 8.0 && v != 16.0)
                 return fail(c, SPH_E_ARG, "contact lanes must be 1, 4, 8 or 16 (got %g)", v);
@@ -1297,15 +1525,17 @@ int sph_set_scalar(sph_ctx* c, int which, double v) {
         case SPH_S_RESORT_MERGE_MAX:
             if (!(v >= 0.0 && v <= (double)MV_CAP))
                 return fail(c, SPH_E_ARG, "merge limit must be in [0, %u] (got %g)", MV_CAP, v);
-            c->P.mv_max = (uint32_t)v;
-            return SPH_OK;
+            S.P.mv_max = (uint32_t)v;
+            return upload_params(c);
         default: return fail(c, SPH_E_ARG, "scalar %d is read-only", which);
     }
 }
 
+int sph_set_scalar(sph_ctx* c, int which, double v) { return sph_set_scalar_sim(c, 0, which, v); }
+
 int sph_get_contacts(sph_ctx* c, uint32_t* idx, double* delta_t, double* distmod) {
     if (!c) return SPH_E_ARG;
-    const size_t np = c->h.np;
+    const size_t np = c->h0().np;
     int rc = SPH_OK;
     if (idx) rc = sph_get_field(c, SPH_F_CONTACTS, idx, np * NC * 4);
     if (!rc && delta_t) rc = sph_get_field(c, SPH_F_DELTA_T, delta_t, np * NC * 24);
@@ -1316,7 +1546,7 @@ int sph_get_contacts(sph_ctx* c, uint32_t* idx, double* delta_t, double* distmod
 int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_start,
                   uint32_t* cell_end) {
     if (!c) return SPH_E_ARG;
-    const size_t np = c->h.np, nc = c->P.ncells;
+    const size_t np = c->h0().np, nc = c->sims[0].P.ncells;
     int rc = SPH_OK;
     if (cell_id) rc = sph_get_field(c, SPH_F_CELLID, cell_id, np * 4);
     if (!rc && perm) rc = sph_get_field(c, SPH_F_PERM, perm, np * 4);

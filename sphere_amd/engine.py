@@ -62,14 +62,17 @@ FIELDS = {
 
 SCALARS = {"time": 0, "time_total": 1, "dt": 2, "file_dt": 3, "iter": 4, "step_count": 5,
            "np": 6, "nw": 7, "nb0": 8, "ncells": 9, "vl_skin": 10, "sorts": 11, "vl_builds": 12,
-           "coop_sort": 13, "contact_lanes": 14, "sort_mode": 15, "resort_merge_max": 16}
+           "nsims": 13, "contact_lanes": 14, "sort_mode": 15, "resort_merge_max": 16}
 
 # Exported symbols of the C ABI (include/sphere_hip.h).
 SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_write_bin",
            "sph_write_mem", "sph_get_field", "sph_set_field", "sph_field_bytes",
            "sph_get_scalar", "sph_set_scalar", "sph_get_contacts", "sph_get_cells",
            "sph_profile", "sph_profile_report", "sph_close", "sph_last_error",
-           "sph_open_error", "sph_version_string")
+           "sph_open_error", "sph_version_string", "sph_open_batch", "sph_open_batch_mem",
+           "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
+           "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
+           "sph_set_scalar_sim")
 
 QUIET = 0x1
 NOCHECK = 0x2
@@ -113,18 +116,45 @@ def load():
     L.sph_last_error.restype = ctypes.c_char_p
     L.sph_open_error.restype = ctypes.c_char_p
     L.sph_version_string.restype = ctypes.c_char_p
+    L.sph_open_batch.argtypes = [ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_uint, ctypes.POINTER(vp)]
+    L.sph_open_batch_mem.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_size_t),
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_uint,
+                                     ctypes.POINTER(vp)]
+    L.sph_num_sims.argtypes = [vp]
+    L.sph_write_bin_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+    L.sph_write_mem_sim.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t)]
+    L.sph_get_field_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t]
+    L.sph_set_field_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t]
+    L.sph_field_bytes_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    L.sph_field_bytes_sim.restype = ctypes.c_int64
+    L.sph_get_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_double)]
+    L.sph_set_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     _LIB = L
     return L
 
 
 class Context:
-    """One simulation resident on the GPU (a sph_ctx)."""
+    """Simulations resident on the GPU (a sph_ctx): one .bin image (or path),
+    or a batch of images stepped together.  Field and scalar accessors take
+    the simulation index `sim` (default 0)."""
 
-    def __init__(self, image=None, path=None, device=-1, quiet=True, check=True):
+    def __init__(self, image=None, path=None, device=-1, quiet=True, check=True, images=None,
+                 paths=None):
         L = load()
         h = ctypes.c_void_p()
         flags = (QUIET if quiet else 0) | (0 if check else NOCHECK)
-        if path is not None:
+        if paths is not None:
+            arr = (ctypes.c_char_p * len(paths))(*[p.encode() for p in paths])
+            rc = L.sph_open_batch(arr, len(paths), device, flags, ctypes.byref(h))
+        elif images is not None:
+            bufs = [ctypes.create_string_buffer(bytes(im), len(im)) for im in images]
+            ptrs = (ctypes.c_void_p * len(bufs))(*[ctypes.addressof(b) for b in bufs])
+            sizes = (ctypes.c_size_t * len(bufs))(*[len(im) for im in images])
+            rc = L.sph_open_batch_mem(ptrs, sizes, len(bufs), device, flags, ctypes.byref(h))
+        elif path is not None:
             rc = L.sph_open(path.encode(), device, flags, ctypes.byref(h))
         else:
             buf = ctypes.create_string_buffer(bytes(image), len(image))
@@ -133,6 +163,7 @@ class Context:
             raise EngineError(rc, L.sph_open_error().decode())
         self._h = h
         self._L = L
+        self.nsims = L.sph_num_sims(h)
 
     def _chk(self, rc):
         if rc != 0:
@@ -163,35 +194,37 @@ class Context:
     def sync(self):
         self._chk(self._L.sph_sync(self._h))
 
-    def get(self, name):
+    def get(self, name, sim=0):
         fid, dt, shape = FIELDS[name]
-        nbytes = self._L.sph_field_bytes(self._h, fid)
+        nbytes = self._L.sph_field_bytes_sim(self._h, sim, fid)
+        if nbytes < 0:
+            raise EngineError(int(nbytes), "bad simulation index or field")
         out = numpy.empty(nbytes // numpy.dtype(dt).itemsize, dtype=dt)
-        self._chk(self._L.sph_get_field(self._h, fid, out.ctypes.data, nbytes))
+        self._chk(self._L.sph_get_field_sim(self._h, sim, fid, out.ctypes.data, nbytes))
         return out.reshape(shape)
 
-    def set(self, name, value):
+    def set(self, name, value, sim=0):
         fid, dt, shape = FIELDS[name]
         a = numpy.ascontiguousarray(value, dtype=dt)
-        self._chk(self._L.sph_set_field(self._h, fid, a.ctypes.data, a.nbytes))
+        self._chk(self._L.sph_set_field_sim(self._h, sim, fid, a.ctypes.data, a.nbytes))
 
-    def scalar(self, name):
+    def scalar(self, name, sim=0):
         v = ctypes.c_double()
-        self._chk(self._L.sph_get_scalar(self._h, SCALARS[name], ctypes.byref(v)))
+        self._chk(self._L.sph_get_scalar_sim(self._h, sim, SCALARS[name], ctypes.byref(v)))
         return v.value
 
-    def set_scalar(self, name, value):
-        self._chk(self._L.sph_set_scalar(self._h, SCALARS[name], float(value)))
+    def set_scalar(self, name, value, sim=0):
+        self._chk(self._L.sph_set_scalar_sim(self._h, sim, SCALARS[name], float(value)))
 
-    def tobytes(self):
+    def tobytes(self, sim=0):
         n = ctypes.c_size_t()
-        self._chk(self._L.sph_write_mem(self._h, None, 0, ctypes.byref(n)))
+        self._chk(self._L.sph_write_mem_sim(self._h, sim, None, 0, ctypes.byref(n)))
         buf = ctypes.create_string_buffer(n.value)
-        self._chk(self._L.sph_write_mem(self._h, buf, n.value, ctypes.byref(n)))
+        self._chk(self._L.sph_write_mem_sim(self._h, sim, buf, n.value, ctypes.byref(n)))
         return buf.raw
 
-    def write_bin(self, path):
-        self._chk(self._L.sph_write_bin(self._h, path.encode()))
+    def write_bin(self, path, sim=0):
+        self._chk(self._L.sph_write_bin_sim(self._h, sim, path.encode()))
 
     def run(self, out_dir, sid):
         self._chk(self._L.sph_run(self._h, out_dir.encode(), sid.encode()))
