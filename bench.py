@@ -93,6 +93,86 @@ def build_workload(args):
     return img
 
 
+def batch_bench(args):
+    """S simulations of log-uniform size (granular gases collapsing in closed
+    boxes) advanced `steps` steps: one batched context (all of them per
+    launch) against the per-simulation loop (one context each, all enqueued
+    before one synchronisation, and one after another)."""
+    from sphere_amd import engine, scenes
+    sizes = scenes.batch_sizes(args.batch, seed=args.seed)
+    images = [scenes.lattice_box(int(n), seed=args.seed + k).tobytes()
+              for k, n in enumerate(sizes)]
+    total = int(sizes.sum())
+    steps = args.steps
+    log("%d simulations, %d particles in total (%d..%d)" % (len(images), total, sizes.min(),
+                                                             sizes.max()))
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    ctx = engine.Context(images=images)
+    if args.contact_lanes:
+        ctx.set_scalar("contact_lanes", args.contact_lanes)
+    ctx.step(args.warmup, sync=True)
+    t_batch = timed(lambda: ctx.step(steps, sync=True))
+    z = sum(int((ctx.get("contacts", sim=k) != sizes[k]).sum()) for k in range(len(images)))
+    ctx.close()
+    log("batched: %.4f s for %d steps" % (t_batch, steps))
+
+    ctxs = [engine.Context(im) for im in images]
+    for c in ctxs:
+        if args.contact_lanes:
+            c.set_scalar("contact_lanes", args.contact_lanes)
+        c.step(args.warmup, sync=False)
+    for c in ctxs:
+        c.sync()
+
+    def overlapped():
+        for c in ctxs:
+            c.step(steps, sync=False)
+        for c in ctxs:
+            c.sync()
+
+    def sequential():
+        for c in ctxs:
+            c.step(steps, sync=True)
+
+    t_ovl = timed(overlapped)
+    t_seq = timed(sequential)
+    for c in ctxs:
+        c.close()
+    log("per-simulation loop: %.4f s enqueued together, %.4f s one after another"
+        % (t_ovl, t_seq))
+    value = total * steps / t_batch
+    best_loop = total * steps / min(t_ovl, t_seq)
+    line = {
+        "metric": "particle-steps/s, batch of %d simulations FP64" % len(images),
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * t_batch / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "batched granular gases, lattice start, contactmodel 2",
+                   "nsims": len(images), "np_total": total, "np_min": int(sizes.min()),
+                   "np_max": int(sizes.max()), "seed": args.seed,
+                   "contacts_end": z},
+        "per_sim_loop": {"value": round(best_loop, 1),
+                         "enqueued_together_s": round(t_ovl, 5),
+                         "one_after_another_s": round(t_seq, 5)},
+        "speedup_vs_per_sim_loop": round(value / best_loop, 3),
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +194,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the "
                          "reference kernel for scaling HBM byte counters")
+    ap.add_argument("--batch", type=int, default=0, metavar="S",
+                    help="instead: S independent simulations (log-uniform 500..20000 "
+                         "particles) stepped as one batched context vs one context each")
     ap.add_argument("--save-image")
     ap.add_argument("--load-image")
     args = ap.parse_args()
@@ -129,6 +212,9 @@ def main():
         print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes,
                           "launches": 5}), flush=True)
         return 0
+
+    if args.batch:
+        return batch_bench(args)
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:

@@ -111,9 +111,11 @@ enum sph_scalar {
                                start, 1 = one workgroup at the end of the previous step
                                when some particle changed cell, 2 (default) = chosen
                                from the measured re-sort rate.  Identical results. */
-    SPH_S_RESORT_MERGE_MAX = 16 /* particles changing cell up to which the end-of-step
-                                   re-sort merges them into the previous order
-                                   (0..2048, default 2048); above it, it sorts anew. */
+    SPH_S_RESORT_MERGE_MAX = 16, /* particles changing cell up to which the end-of-step
+                                    re-sort merges them into the previous order
+                                    (0..2048, default 2048); above it, it sorts anew. */
+    SPH_S_GRAPH_STEPS = 17  /* > 0: sph_step replays steady-state chunks of this many
+                               steps from a captured HIP graph (default 0: launches). */
 };
 
 /* Open flags. */

@@ -135,6 +135,16 @@ struct sph_ctx {
     bool sample_pending = false;
     int64_t sample_iter0 = 0;
     uint64_t sample_sorts0 = 0;
+    // Steady-state step chunks replayed from captured HIP graphs (0 = off):
+    // one graph per (chunk length, re-sort placement, contact lanes).
+    int graph_steps = 0;
+    struct Graph {
+        int steps;
+        bool resort;
+        int lanes;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -900,6 +910,61 @@ int enqueue_step(sph_ctx* c) {
     return SPH_OK;
 }
 
+void drop_graphs(sph_ctx* c) {
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+}
+
+// Enqueue `n` steps as replays of a captured graph of c->graph_steps steps,
+// capturing it first if needed.  Only in the steady state: no hash pending,
+// no profiling, and the previous step's re-sort placement equal to the one
+// the captured steps use (so the first captured step behaves like the rest).
+// Returns the number of steps enqueued (0 when graphs do not apply).
+int enqueue_graph_steps(sph_ctx* c, uint64_t n, uint64_t* done) {
+    *done = 0;
+    const int G = c->graph_steps;
+    const bool resort = !c->any_adaptive && c->resort_at_end;
+    if (G <= 0 || n < (uint64_t)G || c->profile || c->hash_pending ||
+        c->prev_walls_resorted != resort)
+        return SPH_OK;
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+        if (g.steps == G && g.resort == resort && g.lanes == c->contact_lanes) exec = g.exec;
+    if (!exec) {
+        // Capturing must not advance the host mirror of the loop.
+        std::vector<std::pair<double, int64_t>> saved;
+        for (const Sim& sim : c->sims) saved.push_back({sim.h.t, sim.iter});
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "capture");
+        int rc = SPH_OK;
+        for (int k = 0; k < G && rc == SPH_OK; ++k) rc = enqueue_step(c);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        for (size_t k = 0; k < saved.size(); ++k) {
+            c->sims[k].h.t = saved[k].first;
+            c->sims[k].iter = saved[k].second;
+        }
+        if (rc) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return hip_fail(c, e, "capture");
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return hip_fail(c, e, "graph instantiate");
+        c->graphs.push_back({G, resort, c->contact_lanes, exec});
+    }
+    while (n - *done >= (uint64_t)G) {
+        HIPCHK(c, hipGraphLaunch(exec, c->stream), "graph launch");
+        for (Sim& sim : c->sims)
+            for (int k = 0; k < G; ++k) {
+                sim.h.t += sim.h.dt;
+                sim.iter += 1;
+            }
+        *done += (uint64_t)G;
+    }
+    return SPH_OK;
+}
+
 int read_loop(sph_ctx* c, int sim, DevLoop* L) {
     HIPCHK(c, hipMemcpyAsync(L, c->A.loop + sim, sizeof *L, hipMemcpyDeviceToHost, c->stream),
            "D2H");
@@ -1113,9 +1178,17 @@ int sph_num_sims(const sph_ctx* c) { return c ? (int)c->sims.size() : SPH_E_ARG;
 int sph_step(sph_ctx* c, uint64_t nsteps) {
     if (!c) return SPH_E_ARG;
     choose_sort_placement(c);
-    for (uint64_t k = 0; k < nsteps; ++k) {
-        int rc = enqueue_step(c);
+    uint64_t k = 0;
+    while (k < nsteps) {
+        uint64_t done = 0;
+        int rc = enqueue_graph_steps(c, nsteps - k, &done);
         if (rc) return rc;
+        if (done == 0) {
+            rc = enqueue_step(c);
+            if (rc) return rc;
+            done = 1;
+        }
+        k += done;
     }
     sample_loop(c);
     return SPH_OK;
@@ -1471,6 +1544,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_SORT_MODE: *v = c->sort_mode; return SPH_OK;
         case SPH_S_RESORT_MERGE_MAX: *v = S.P.mv_max; return SPH_OK;
         case SPH_S_NSIMS: *v = (double)c->sims.size(); return SPH_OK;
+        case SPH_S_GRAPH_STEPS: *v = c->graph_steps; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1511,6 +1585,11 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             if (rc) return rc;
             return set_loop_flag(c, sim, &DevLoop::force_rebuild);
         }
+        case SPH_S_GRAPH_STEPS:
+            if (!(v >= 0.0 && v <= 4096.0))
+                return fail(c, SPH_E_ARG, "graph steps must be in [0, 4096] (got %g)", v);
+            c->graph_steps = (int)v;
+            return SPH_OK;
         case SPH_S_CONTACT_LANES:
             if (v != 1.0 && v != 4.0 && v != 8.0 && v != 16.0)
                 return fail(c, SPH_E_ARG, "contact lanes must be 1, 4, 8 or 16 (got %g)", v);
@@ -1582,6 +1661,7 @@ int sph_profile_report(sph_ctx* c, char* buf, size_t cap) {
 void sph_close(sph_ctx* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graphs(c);
     if (c->sample_ev) (void)hipEventDestroy(c->sample_ev);
     if (c->loop_sample) (void)hipHostFree(c->loop_sample);
     if (c->prof) {

@@ -137,6 +137,44 @@ def settling_box(n=4096, seed=1001, box=(0.2, 0.2, 0.6), r_range=(0.005, 0.01), 
     return s
 
 
+def lattice_box(n=5000, seed=1, median=0.005, sigma=0.2, aspect=2.0, k_n=1.16e7,
+                damping_frac=0.5, mu=0.5, rho=2600.0, v0=0.2):
+    """A granular gas collapsing under gravity in a closed box: lognormal
+    radii on a jittered cubic lattice (spacing 2.2 r_max, so no initial
+    overlap), random velocities of about v0, side walls at 0 and L.  Built
+    in O(n) (no insertion search), for batches of many simulations."""
+    rng = numpy.random.default_rng(seed)
+    r = lognormal_radii(n, median, sigma, rng)
+    d = 2.2 * r.max()
+    nxy = max(3, int(math.ceil((n / aspect) ** (1.0 / 3.0))))
+    nz = max(3, int(math.ceil(n / (nxy * nxy))))
+    ix = numpy.arange(n) % nxy
+    iy = (numpy.arange(n) // nxy) % nxy
+    iz = numpy.arange(n) // (nxy * nxy)
+    jit = (d / 2.0 - r.max()) * rng.uniform(-0.9, 0.9, (n, 3))
+    s = sim(np=n, sid="lattice-%d" % n)
+    s.radius = r
+    s.x = numpy.stack([(ix + 0.5) * d, (iy + 0.5) * d, (iz + 0.5) * d], axis=1) + jit
+    s.vel[:, :3] = rng.normal(0.0, v0 / math.sqrt(3.0), (n, 3))
+    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
+    _set_contact_params(s, k_n, damping_frac, mu, r.min())
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.normalBoundariesXY()
+    s.defineWorldBoundaries(L=[nxy * d, nxy * d, nz * d * 1.05])
+    dt = dt_rule(r.min(), rho, k_n)
+    s.time_dt[0] = dt
+    s.time_current[0] = 0.0
+    s.time_total[0] = dt * 1.0e6
+    s.time_file_dt[0] = dt * 1.0e6
+    return s
+
+
+def batch_sizes(nsims=256, lo=500, hi=20000, seed=7):
+    """Log-uniform particle counts in [lo, hi] for a batch of simulations."""
+    rng = numpy.random.default_rng(seed)
+    return numpy.exp(rng.uniform(math.log(lo), math.log(hi), nsims)).astype(int)
+
+
 def lognormal_radii(n, median, sigma, rng):
     return median * numpy.exp(sigma * rng.standard_normal(n))
 
