@@ -173,6 +173,41 @@ def batch_bench(args):
     return 0
 
 
+def adjoint_bench(args, img):
+    """Wall time of sph_adjoint over K steps (record forward + reverse)
+    against K plain forward steps, from the same state."""
+    import numpy
+    from sphere_amd import engine
+    K = args.adjoint
+    ctx = engine.Context(img)
+    n = int(ctx.scalar("np"))
+    ctx.step(args.warmup, sync=True)
+    state = ctx.tobytes()
+    ctx.close()
+    fwd = engine.Context(state)
+    fwd.step(1, sync=True)
+    t0 = time.perf_counter()
+    fwd.step(K, sync=True)
+    t_fwd = time.perf_counter() - t0
+    fwd.close()
+    adj = engine.Context(state)
+    adj.step(1, sync=True)
+    rng = numpy.random.default_rng(args.seed)
+    seed_v = rng.standard_normal((n, 3))
+    t0 = time.perf_counter()
+    g = adj.adjoint(K, None, seed_v, None)
+    adj.sync()
+    t_adj = time.perf_counter() - t0
+    adj.close()
+    log("forward %d steps %.4f s, adjoint %.4f s (ratio %.2f); dL/dk_n %.4e"
+        % (K, t_fwd, t_adj, t_adj / t_fwd, g["k_n"]))
+    print(json.dumps({"metric": "adjoint cost / forward cost", "value": round(t_adj / t_fwd, 3),
+                      "unit": "ratio", "n_gpus": 1, "steps": K, "higher_is_better": False,
+                      "np": n, "forward_s": round(t_fwd, 5), "adjoint_s": round(t_adj, 5),
+                      "data": "synthetic"}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -197,6 +232,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0, metavar="S",
                     help="instead: S independent simulations (log-uniform 500..20000 "
                          "particles) stepped as one batched context vs one context each")
+    ap.add_argument("--adjoint", type=int, default=0, metavar="K",
+                    help="instead: time the adjoint of K steps against K forward steps")
     ap.add_argument("--save-image")
     ap.add_argument("--load-image")
     args = ap.parse_args()
@@ -226,6 +263,8 @@ def main():
     from sphere_amd import engine, scenes
 
     img = build_workload(args)
+    if args.adjoint:
+        return adjoint_bench(args, img)
     ctx = engine.Context(img)
     if args.contact_lanes:
         ctx.set_scalar("contact_lanes", args.contact_lanes)

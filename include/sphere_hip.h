@@ -182,6 +182,23 @@ int sph_get_contacts(sph_ctx* ctx, uint32_t* idx, double* delta_t, double* distm
 int sph_get_cells(sph_ctx* ctx, uint32_t* cell_id, uint32_t* perm,
                   uint32_t* cell_start, uint32_t* cell_end);
 
+/* Adjoint (reverse mode) of nsteps steps; no counterpart in the reference.
+ * Runs the steps forward from the current state, recording each step's
+ * inputs and contact rows, then backwards with that contact list and every
+ * branch frozen, and returns the gradient of
+ *     L = sum_i dL_dx[i] . x_i + dL_dv[i] . v_i + dL_dw[i] . w_i
+ * (state after the steps) with respect to the state before them (g_x0,
+ * g_v0, g_w0: np x 3 each) and to the parameters (g_params, 9 values:
+ * k_n, k_t, gamma_n, gamma_t, mu_s, mu_d, gamma_wn, gamma_wt, mu_wd).  Any
+ * pointer may be NULL.  Lengths are differentiated as exact roots, walls
+ * follow their recorded trajectories.  Covers contact model 2 without
+ * cohesion, bonds, E-scaled stiffness, wmode 3, adaptive grids or
+ * rate-state particles (SPH_E_UNSUPPORTED otherwise); single-simulation
+ * contexts.  The context is left at the state after the steps. */
+int sph_adjoint(sph_ctx* ctx, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
+                const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
+                double* g_params);
+
 /* Per-kernel timing: mean device microseconds of the last sph_step call's
  * kernels, measured with HIP events on the context stream
  * (the reference's PROFILING timers, src/device.cu:947-987,2711-2808). */

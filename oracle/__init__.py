@@ -94,6 +94,7 @@ def _lib():
         L.orc_tree_sum.restype = ctypes.c_double
         L.orc_tree_sum.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_set_exact_length.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -105,6 +106,12 @@ def set_threads(n):
 
 def get_threads():
     return _lib().orc_get_threads()
+
+
+def set_exact_length(on):
+    """Gradient-check aid: FP64 instead of FP32 lengths in every step that
+    follows (process-wide; the reference's FP32 lengths are the default)."""
+    _lib().orc_set_exact_length(1 if on else 0)
 
 
 class Oracle:

@@ -73,7 +73,10 @@ static inline R3 cross(R3 a, R3 b) {
 }
 // length(): FP32 square root of the FP64 dot, src/vector_arithmetic.h:317-324.
 static inline Real sqrt32(Real v) { return (Real)sqrtf((float)v); }
-static inline Real length(R3 v) { return sqrt32(dot(v, v)); }
+// A smooth variant for checking gradients by finite differences: lengths
+// as FP64 roots (off by default; the FP32 root is the reference's).
+static bool g_exact_length = false;
+static inline Real length(R3 v) { return g_exact_length ? sqrt(dot(v, v)) : sqrt32(dot(v, v)); }
 
 struct Params {
     Real g[3], k_n, k_t, k_r, E, gamma_n, gamma_t, gamma_r, mu_s, mu_d, mu_r;
@@ -1245,6 +1248,7 @@ int64_t orc_write_mem(const orc_sim* s, void* dst, size_t cap) {
 }
 
 double orc_sqrt32(double v) { return sqrt32(v); }
+void orc_set_exact_length(int on) { g_exact_length = on != 0; }
 
 double orc_tree_sum(const double* v, uint32_t n) {
     std::vector<Real> partial;

@@ -67,6 +67,8 @@ int64_t orc_iter(const orc_sim* s);
  * and the reference's wall-force reduction (256-wide block trees, then a
  * serial sum over blocks) of n values. */
 double orc_sqrt32(double v);
+/* Test aid for gradient checks: FP64 lengths instead of FP32 (default 0). */
+void orc_set_exact_length(int on);
 double orc_tree_sum(const double* v, uint32_t n);
 
 /* Number of OpenMP threads the per-particle stages use (0 = runtime default). */

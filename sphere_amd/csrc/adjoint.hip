@@ -1,0 +1,400 @@
+// Reverse mode (adjoint) of the DEM step for contact model 2, on a recorded
+// forward pass.  The reference has no counterpart.
+//
+// The forward pass records, per step, the state the step reads (positions,
+// velocities, angular velocities, the previous accelerations, the
+// tangential displacements and the walls) and the contact rows after the
+// interaction pass, which hold exactly the contacts that produced a force.
+// The reverse pass then runs the steps backwards with that contact list
+// frozen and every branch (normal-force clamp, stick or slip, wall
+// friction) taken as the forward took it, and propagates cotangents:
+//
+//   integrate (TY3)     x' = x + v dt + (2/3) a dt^2 - (1/6) a0 dt^2
+//                       v' = v + (3/2) a dt - (1/2) a0 dt,  a = F/m + g
+//   contactLinear       F, T and the new tangential displacement of each
+//                       contact, as functions of both particles' state, the
+//                       stored displacement and (k_n, k_t, gamma_n, gamma_t,
+//                       mu_s, mu_d)
+//   contactLinear_wall  F, T of each wall contact (k_n, gamma_wn, gamma_wt,
+//                       mu_wd)
+//
+// Lengths are differentiated as exact roots (the forward rounds them to
+// FP32, which is locally constant and has no useful derivative).  Walls move
+// along their recorded trajectories: the wall equation of motion is not
+// differentiated, which is exact for walls at fixed position or velocity.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "adjoint.hpp"
+#include "dem_math.hpp"
+
+namespace sphx {
+
+namespace {
+
+constexpr double PI = 3.14159265358979;  // src/constants.h:14
+
+__device__ __forceinline__ d3 zero3() { return mk3(0.0, 0.0, 0.0); }
+
+// Cotangents of the parameters, in the order of AdjParam.
+struct PGrad {
+    double v[ADJ_NPARAM];
+};
+
+// Reverse of contactLinear (src/contactmodels.cuh:244-411, kappa = 0,
+// E <= 0.001) for one contact of particle a with partner b.  Inputs are the
+// forward inputs; gF, gT are the cotangents of a's force and torque and gdt
+// that of the new tangential displacement.  Outputs are accumulated.
+__device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 wb, double ra,
+                                   double rb, d3 x, d3 dt0, d3 gF, d3 gT, d3 gdt, d3& gx,
+                                   d3& gva, d3& gwa, d3& gvb, d3& gwb, d3& gdt0, PGrad& gp) {
+    // forward (same expressions and FP32 lengths, so the same branches)
+    const double len = length(x);
+    const double delta = len - (ra + rb);
+    const d3 n = x / len;
+    const d3 vl = va - vb;
+    const double ca = ra + delta / 2.0, cb = rb + delta / 2.0;
+    const d3 na = cross(n, wa), nb = cross(n, wb);
+    const d3 vel = vl + ca * na + cb * nb;
+    const double vel_n = -dot(vl, n);
+    const double nv = dot(n, vel);
+    const d3 vel_t = vel - n * nv;
+    const double vtl = length(vel_t);
+    const double nd0 = dot(n, dt0);
+    const d3 dtp = dt0 - n * nd0;
+    const double dtpl = length(dtp);
+    const double cn_raw = -p.k_n * delta + p.gamma_n * vel_n;
+    const double cn = fmax(0.0, cn_raw);
+    const d3 f_n = cn * n;
+    const d3 f_c = -p.kappa * sqrt32(ra * rb) * n;
+    const double L = length(f_n - f_c);
+    const bool active = dtpl > 0.0 || vtl > 0.0;
+    d3 dtmp = zero3(), f_raw = zero3(), t = zero3(), f_t = zero3();
+    double frl = 0.0, limit = 0.0;
+    bool slip = false;
+    if (active) {
+        dtmp = dtp + vel_t * p.dt;
+        f_raw = -p.k_t * dtmp + -p.gamma_t * vel_t;
+        frl = length(f_raw);
+        limit = p.mu_s * L;
+        slip = frl > limit;
+        if (slip) {
+            t = f_raw / frl;
+            f_t = limit * t;
+        } else {
+            f_t = f_raw;
+        }
+    }
+    // reverse
+    d3 gn = zero3();
+    double gdelta = 0.0;
+    d3 gft = gF, gfn = gF;
+    const double cu = -(ra + delta * 0.5);
+    const d3 u = cu * n;
+    gft += cross(gT, u);
+    const d3 gu = cross(f_t, gT);
+    gn += cu * gu;
+    gdelta += -0.5 * dot(n, gu);
+    d3 gdtp = zero3(), gvel_t = zero3();
+    double gL = 0.0;
+    if (active) {
+        d3 gdtmp = zero3(), gfraw = zero3();
+        if (slip) {
+            // dt_new = -1/k_t (mu_d L t + gamma_t vel_t), f_t = mu_s L t
+            const d3 A = p.mu_d * L * t + p.gamma_t * vel_t;
+            const d3 gA = gdt * (-1.0 / p.k_t);
+            gp.v[ADJ_KT] += dot(gdt, A) / (p.k_t * p.k_t);
+            gp.v[ADJ_MUD] += L * dot(t, gA);
+            gL += p.mu_d * dot(t, gA);
+            d3 gt = (p.mu_d * L) * gA;
+            gp.v[ADJ_GT] += dot(vel_t, gA);
+            gvel_t += p.gamma_t * gA;
+            const double glim = dot(t, gft);
+            gt += limit * gft;
+            gp.v[ADJ_MUS] += L * glim;
+            gL += p.mu_s * glim;
+            gfraw = (gt - t * dot(t, gt)) / frl;
+        } else {
+            gdtmp = gdt;
+            gfraw = gft;
+        }
+        // f_raw = -k_t dtmp - gamma_t vel_t, dtmp = dtp + vel_t dt
+        gdtmp += -p.k_t * gfraw;
+        gp.v[ADJ_KT] += -dot(dtmp, gfraw);
+        gp.v[ADJ_GT] += -dot(vel_t, gfraw);
+        gvel_t += -p.gamma_t * gfraw;
+        gdtp += gdtmp;
+        gvel_t += p.dt * gdtmp;
+    } else {
+        gdtp += gdt;
+    }
+    if (L > 0.0) gfn += (gL / L) * (f_n - f_c);
+    const double gcn = dot(n, gfn);
+    gn += cn * gfn;
+    double gvel_n = 0.0;
+    if (cn_raw > 0.0) {
+        gp.v[ADJ_KN] += -delta * gcn;
+        gdelta += -p.k_n * gcn;
+        gp.v[ADJ_GN] += vel_n * gcn;
+        gvel_n = p.gamma_n * gcn;
+    }
+    // dtp = dt0 - n (n . dt0)
+    gdt0 += gdtp - n * dot(n, gdtp);
+    gn += -(nd0 * gdtp + dot(n, gdtp) * dt0);
+    // vel_t = vel - n (n . vel)
+    const d3 gvel = gvel_t - n * dot(n, gvel_t);
+    gn += -(nv * gvel_t + dot(n, gvel_t) * vel);
+    // vel_n = -(vl . n)
+    d3 gvl = -gvel_n * n;
+    gn += -gvel_n * vl;
+    // vel = vl + ca (n x wa) + cb (n x wb)
+    gvl += gvel;
+    gdelta += 0.5 * dot(gvel, na) + 0.5 * dot(gvel, nb);
+    const d3 gna = ca * gvel, gnb = cb * gvel;
+    gn += cross(wa, gna) + cross(wb, gnb);
+    gwa += cross(gna, n);
+    gwb += cross(gnb, n);
+    gva += gvl;
+    gvb += -gvl;
+    // n = x / |x|, delta = |x| - (ra + rb)
+    gx += (gn - n * dot(n, gn)) / len + n * gdelta;
+}
+
+// Reverse of contactLinear_wall (src/contactmodels.cuh:10-117) for one wall
+// with normal n, distance delta = e . x + const, wall speed wv.
+__device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3 n, d3 e,
+                                 double delta, double wv, d3 gF, d3 gT, d3& gx, d3& gva,
+                                 d3& gwa, PGrad& gp) {
+    const d3 vl = va;
+    const double c = ra + delta / 2.0;
+    const d3 nw = cross(n, wa);
+    const d3 vel = vl + c * nw + wv;
+    const double vel_n = dot(vl, n);
+    const double nv = dot(n, vel);
+    const d3 vel_t = vel - n * nv;
+    const double vtl = length(vel_t);
+    const double cn_raw = -p.k_n * delta - p.gamma_wn * vel_n;
+    const double cn = fmax(0.0, cn_raw);
+    const d3 f_n = cn * n;
+    const double fnl = length(f_n);
+    d3 f_t = zero3();
+    bool fric = vtl > 0.0 && p.gamma_wt > 0.0, stick = false;
+    double limit = 0.0;
+    if (fric) {
+        const d3 f_t_visc = p.gamma_wt * vel_t;
+        limit = p.mu_wd * fnl;
+        stick = length(f_t_visc) < limit;
+        f_t = stick ? -1.0 * f_t_visc : -limit * vel_t / vtl;
+    }
+    d3 gft = gF, gfn = gF;
+    double gdelta = 0.0;
+    const double cu = -(ra + delta * 0.5);
+    gft += cross(gT, cu * n);
+    gdelta += -0.5 * dot(n, cross(f_t, gT));
+    d3 gvel_t = zero3();
+    if (fric) {
+        if (stick) {
+            gvel_t += -p.gamma_wt * gft;
+            gp.v[ADJ_GWT] += -dot(vel_t, gft);
+        } else {
+            const d3 s = vel_t / vtl;
+            const double glim = -dot(s, gft);
+            const d3 gs = -limit * gft;
+            gvel_t += (gs - s * dot(s, gs)) / vtl;
+            gp.v[ADJ_MUWD] += fnl * glim;
+            if (fnl > 0.0) gfn += (p.mu_wd * glim / fnl) * f_n;
+        }
+    }
+    const double gcn = dot(n, gfn);
+    double gvel_n = 0.0;
+    if (cn_raw > 0.0) {
+        gp.v[ADJ_KN] += -delta * gcn;
+        gdelta += -p.k_n * gcn;
+        gp.v[ADJ_GWN] += -vel_n * gcn;
+        gvel_n = -p.gamma_wn * gcn;
+    }
+    const d3 gvel = gvel_t - n * dot(n, gvel_t);
+    d3 gvl = gvel_n * n + gvel;
+    gdelta += 0.5 * dot(gvel, nw);
+    gwa += cross(c * gvel, n);
+    gva += gvl;
+    gx += gdelta * e;
+}
+
+__device__ __forceinline__ void atomic_add3(double4* dst, d3 v) {
+    atomicAdd(&dst->x, v.x);
+    atomicAdd(&dst->y, v.y);
+    atomicAdd(&dst->z, v.z);
+}
+
+// Reverse of integrate for step k, in place on the cotangents: (Lx, Lv, Lw,
+// La, Laa) of the step's outputs (x', v', w', the new accelerations) become
+// those of its inputs, and gF, gT receive the force and torque cotangents.
+__global__ __launch_bounds__(256) void k_adj_integrate(const DevParams p, AdjStep st,
+                                                       AdjState g) {
+    const uint32_t o = blockIdx.x * 256 + threadIdx.x;
+    if (o >= p.np) return;
+    const double4 x = st.x[o], vel = st.v[o];
+    const double dt = p.dt;
+    const double r = x.w;
+    const double m = 4.0 / 3.0 * PI * r * r * r * p.rho;
+    const double I = 2.0 / 5.0 * m * r * r;
+    const double w = vel.w;  // fixvel
+    const double4 Lx = g.Lx[o], Lv = g.Lv[o], Lw = g.Lw[o], La = g.La[o], Laa = g.Laa[o];
+    // cotangents of acc and angacc (the new accelerations also feed the
+    // next step as acc0)
+    d3 ga = mk3(La.x + (2.0 / 3.0) * dt * dt * Lx.x + 1.5 * dt * Lv.x,
+                La.y + (2.0 / 3.0) * dt * dt * Lx.y + 1.5 * dt * Lv.y,
+                La.z + (2.0 / 3.0) * dt * dt * Lx.z + 1.5 * dt * Lv.z);
+    d3 gaa = mk3(Laa.x + 1.5 * dt * Lw.x, Laa.y + 1.5 * dt * Lw.y, Laa.z + 1.5 * dt * Lw.z);
+    // fixvel modes (src/integration.cuh): the masked components do not
+    // depend on the force.
+    if (w > 0.0001 && w < 10.0) {
+        ga.x = 0.0;
+        ga.y = 0.0;
+        gaa = zero3();
+    }
+    if (w >= 10.0) {
+        ga.y = 0.0;
+        gaa.x = 0.0;
+        gaa.z = 0.0;
+    }
+    if (w < -0.0001) {
+        ga = zero3();
+        gaa = zero3();
+    }
+    g.gF[o] = make_double4(ga.x / m, ga.y / m, ga.z / m, 0.0);
+    g.gT[o] = make_double4(gaa.x / I, gaa.y / I, gaa.z / I, 0.0);
+    const double c0 = st.first ? 0.0 : 1.0;  // acc0 is zero on the first step of a run
+    g.Lv[o] = make_double4(Lx.x * dt + Lv.x, Lx.y * dt + Lv.y, Lx.z * dt + Lv.z, 0.0);
+    g.La[o] = make_double4(c0 * (-(dt * dt / 6.0) * Lx.x - 0.5 * dt * Lv.x),
+                           c0 * (-(dt * dt / 6.0) * Lx.y - 0.5 * dt * Lv.y),
+                           c0 * (-(dt * dt / 6.0) * Lx.z - 0.5 * dt * Lv.z), 0.0);
+    g.Laa[o] = make_double4(c0 * -0.5 * dt * Lw.x, c0 * -0.5 * dt * Lw.y,
+                            c0 * -0.5 * dt * Lw.z, 0.0);
+}
+
+// Reverse of the contact and wall forces of step k, one work-item per
+// particle: cotangents of both partners' state (the partner's by atomics),
+// of the stored tangential displacements (handed to step k - 1 for contacts
+// that persisted), and of the parameters.
+__global__ __launch_bounds__(256) void k_adj_contact(const DevParams p, AdjStep st,
+                                                     AdjState g, double Lz) {
+    __shared__ double red[256];
+    const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+    PGrad gp;
+    for (int q = 0; q < ADJ_NPARAM; ++q) gp.v[q] = 0.0;
+    if (a < p.np) {
+        const uint32_t np = p.np;
+        const double4 xa4 = st.x[a];
+        const d3 xa = xyz(xa4), va = xyz(st.v[a]), wa = xyz(st.w[a]);
+        const double ra = xa4.w;
+        const d3 gF = xyz(g.gF[a]), gT = xyz(g.gT[a]);
+        d3 gxa = zero3(), gva = zero3(), gwa = zero3();
+        for (int s = 0; s < NC; ++s) {
+            const size_t mp = (size_t)a * NC + s;
+            const uint32_t wrd = st.rows[mp];
+            const uint32_t b = wrd & CW_IDX_MASK;
+            if (b >= np) {
+                g.Ldt_out[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
+                continue;
+            }
+            const uint32_t cx = (wrd >> CW_SHIFT_X) & 3u, cy = (wrd >> CW_SHIFT_Y) & 3u;
+            const d3 dm = mk3(cx == 1u ? 0.0 + p.L[0] : cx == 2u ? 0.0 - p.L[0] : 0.0,
+                              cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
+            const double4 xb4 = st.x[b];
+            const d3 xab = mk3(xa.x - xb4.x + dm.x, xa.y - xb4.y + dm.y, xa.z - xb4.z + dm.z);
+            d3 gx = zero3(), gvb = zero3(), gwb = zero3(), gdt0 = zero3();
+            adj_contact_linear(p, va, wa, xyz(st.v[b]), xyz(st.w[b]), ra, xb4.w, xab,
+                               xyz(st.dt0[mp]), gF, gT, xyz(g.Ldt_in[mp]), gx, gva, gwa, gvb,
+                               gwb, gdt0, gp);
+            gxa += gx;
+            atomic_add3(g.Lx + b, -gx);
+            atomic_add3(g.Lv + b, gvb);
+            atomic_add3(g.Lw + b, gwb);
+            // The stored displacement came from the same slot one step
+            // earlier only if the slot held this partner then.
+            const bool carried = st.prev_rows && st.prev_rows[mp] == wrd;
+            g.Ldt_out[mp] = carried ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
+                                    : make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+        // Walls, as in the forward pass: top (wall 0 or L_z), bottom, then
+        // the side walls by boundary mode.
+        {
+            double wpos = Lz, wv = 0.0;
+            d3 wn = mk3(0.0, 0.0, -1.0);
+            if (p.nw >= 1) {
+                wn = xyz(st.wall_nx[0]);
+                wpos = st.wall_nx[0].w;
+                wv = st.wall_mvfd[0].y;
+            }
+            const double dw = wpos - (xa.z + ra);
+            if (dw < 0.0)
+                adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 0.0, -1.0), dw, wv, gF, gT, gxa,
+                                 gva, gwa, gp);
+        }
+        {
+            const double dw = xa.z - ra - p.origo[2];
+            if (dw < 0.0)
+                adj_contact_wall(p, va, wa, ra, mk3(0.0, 0.0, 1.0), mk3(0.0, 0.0, 1.0), dw, 0.0,
+                                 gF, gT, gxa, gva, gwa, gp);
+        }
+        if (p.periodic == 0) {
+            double wpos = p.L[0], wv = 0.0;
+            d3 wn = mk3(-1.0, 0.0, 0.0);
+            if (p.nw >= 2) { wn = xyz(st.wall_nx[1]); wpos = st.wall_nx[1].w; wv = st.wall_mvfd[1].y; }
+            double dw = wpos - (xa.x + ra);
+            if (dw < 0.0)
+                adj_contact_wall(p, va, wa, ra, wn, mk3(-1.0, 0.0, 0.0), dw, wv, gF, gT, gxa, gva,
+                                 gwa, gp);
+            wpos = 0.0; wv = 0.0; wn = mk3(1.0, 0.0, 0.0);
+            if (p.nw >= 3) { wn = xyz(st.wall_nx[2]); wpos = st.wall_nx[2].w; wv = st.wall_mvfd[2].y; }
+            dw = xa.x - ra - wpos;
+            if (dw < 0.0)
+                adj_contact_wall(p, va, wa, ra, wn, mk3(1.0, 0.0, 0.0), dw, wv, gF, gT, gxa, gva,
+                                 gwa, gp);
+        }
+        if (p.periodic == 0 || p.periodic == 2) {
+            double wpos = p.L[1], wv = 0.0;
+            d3 wn = mk3(0.0, -1.0, 0.0);
+            if (p.nw >= 4) { wn = xyz(st.wall_nx[3]); wpos = st.wall_nx[3].w; wv = st.wall_mvfd[3].y; }
+            double dw = wpos - (xa.y + ra);
+            if (dw < 0.0)
+                adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, -1.0, 0.0), dw, wv, gF, gT, gxa, gva,
+                                 gwa, gp);
+            wpos = 0.0; wv = 0.0; wn = mk3(0.0, 1.0, 0.0);
+            if (p.nw >= 5) { wn = xyz(st.wall_nx[4]); wpos = st.wall_nx[4].w; wv = st.wall_mvfd[4].y; }
+            dw = xa.y - ra - wpos;
+            if (dw < 0.0)
+                adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 1.0, 0.0), dw, wv, gF, gT, gxa, gva,
+                                 gwa, gp);
+        }
+        atomic_add3(g.Lx + a, gxa);
+        atomic_add3(g.Lv + a, gva);
+        atomic_add3(g.Lw + a, gwa);
+    }
+    // Parameter cotangents: a workgroup sum, then one atomic per parameter.
+    for (int q = 0; q < ADJ_NPARAM; ++q) {
+        red[threadIdx.x] = gp.v[q];
+        __syncthreads();
+        for (unsigned h = 128; h > 0; h >>= 1) {
+            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) atomicAdd(&g.gparam[q], red[0]);
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
+                           hipStream_t s) {
+    const dim3 grid((p.np + 255) / 256), block(256);
+    hipLaunchKernelGGL(k_adj_integrate, grid, block, 0, s, p, st, g);
+    hipLaunchKernelGGL(k_adj_contact, grid, block, 0, s, p, st, g, Lz);
+    return hipGetLastError();
+}
+
+}  // namespace sphx

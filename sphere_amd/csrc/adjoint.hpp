@@ -1,0 +1,58 @@
+// Reverse-mode step (contact model 2) over a recorded forward pass.
+#ifndef SPHERE_AMD_ADJOINT_HPP_
+#define SPHERE_AMD_ADJOINT_HPP_
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dem_types.hpp"
+
+namespace sphx {
+
+// Parameters with a cotangent, in output order.
+enum AdjParam : int {
+    ADJ_KN = 0,    // k_n (contacts and walls)
+    ADJ_KT = 1,    // k_t
+    ADJ_GN = 2,    // gamma_n
+    ADJ_GT = 3,    // gamma_t
+    ADJ_MUS = 4,   // mu_s
+    ADJ_MUD = 5,   // mu_d
+    ADJ_GWN = 6,   // gamma_wn
+    ADJ_GWT = 7,   // gamma_wt
+    ADJ_MUWD = 8,  // mu_wd
+    ADJ_NPARAM = 9
+};
+
+// What the forward pass recorded for one step (original particle order).
+struct AdjStep {
+    const double4* x;          // positions at the step start (radius in w)
+    const double4* v;          // velocities (fixvel in w)
+    const double4* w;          // angular velocities
+    const double4* dt0;        // np x NC tangential displacements before the step
+    const uint32_t* rows;      // np x NC contact rows after the interaction pass
+    const uint32_t* prev_rows; // the previous step's rows, or null on the first step
+    const double4* wall_nx;    // walls at the step start
+    const double4* wall_mvfd;
+    int first;                 // the run's first step (acc0 = 0)
+};
+
+// Cotangents, updated step by step from the last to the first.
+struct AdjState {
+    double4* Lx;
+    double4* Lv;
+    double4* Lw;
+    double4* La;       // of the accelerations the step writes (next step's acc0)
+    double4* Laa;
+    double4* gF;       // scratch: force and torque cotangents of the step
+    double4* gT;
+    const double4* Ldt_in;  // np x NC: of the displacements the step writes
+    double4* Ldt_out;       // np x NC: of the displacements the step read
+    double* gparam;         // ADJ_NPARAM sums
+};
+
+hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
+                           hipStream_t s);
+
+}  // namespace sphx
+
+#endif

@@ -21,6 +21,7 @@
 
 #include "../../include/sphere_hip.h"
 #include "dem_arrays.hpp"
+#include "adjoint.hpp"
 #include "dem_kernels.hpp"
 #include "dem_types.hpp"
 
@@ -145,6 +146,9 @@ struct sph_ctx {
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
+    // While an adjoint forward pass records: where the contact rows after
+    // the interaction pass go.
+    uint32_t* tape_rows = nullptr;
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -884,6 +888,9 @@ int enqueue_step(sph_ctx* c) {
     pr.begin("contact");
     HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
     pr.end();
+    if (c->tape_rows)
+        HIPCHK(c, hipMemcpyAsync(c->tape_rows, A.crow, (size_t)c->sims[0].h.np * NC * 4,
+                                 hipMemcpyDeviceToDevice, s), "tape");
     if (A.nbond > 0) {
         pr.begin("bonds");
         HIPCHK(c, launch_bonds(A, s), "bonds");
@@ -1631,6 +1638,166 @@ int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_
     if (!rc && perm) rc = sph_get_field(c, SPH_F_PERM, perm, np * 4);
     if (!rc && cell_start) rc = sph_get_field(c, SPH_F_CELLSTART, cell_start, nc * 4);
     if (!rc && cell_end) rc = sph_get_field(c, SPH_F_CELLEND, cell_end, nc * 4);
+    return rc;
+}
+
+int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
+                const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
+                double* g_params) {
+    if (!c) return SPH_E_ARG;
+    if (c->sims.size() != 1)
+        return fail(c, SPH_E_UNSUPPORTED, "the adjoint runs on a single-simulation context");
+    Sim& S = c->sims[0];
+    const Host& h = S.h;
+    if (h.contactmodel != 2 || h.kappa != 0.0 || h.E > 0.001 || h.nb0 > 0 || c->shear_stress ||
+        c->any_adaptive)
+        return fail(c, SPH_E_UNSUPPORTED,
+                    "the adjoint covers contact model 2 without cohesion, E-scaled stiffness, "
+                    "bonds, a wmode-3 wall or an adaptive grid");
+    for (uint32_t i = 0; i < h.np; ++i)
+        if (h.vel[4 * i + 3] > 5.0 && h.vel[4 * i + 3] < 10.0)
+            return fail(c, SPH_E_UNSUPPORTED, "the adjoint excludes rate-state particles");
+    if (nsteps == 0) return SPH_OK;
+    const size_t np = h.np, nw = h.nw, nc = np * NC;
+    const size_t per_step = np * 3 * sizeof(double4) + nc * (sizeof(double4) + 4) +
+                            2 * nw * sizeof(double4);
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t state_b = np * 7 * sizeof(double4) + 2 * nc * sizeof(double4) + 1024;
+    if ((double)per_step * (double)nsteps + (double)state_b > 0.9 * (double)free_b)
+        return fail(c, SPH_E_ARG, "recording %llu steps needs %.1f GB of device memory",
+                    (unsigned long long)nsteps, (per_step * (double)nsteps + state_b) / 1e9);
+    const DevArrays& A = c->A;
+    hipStream_t s = c->stream;
+    std::vector<void*> bufs;
+    auto dalloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return p;
+    };
+    auto release = [&]() {
+        (void)hipStreamSynchronize(s);
+        for (void* p : bufs) (void)hipFree(p);
+    };
+    char* tape = static_cast<char*>(dalloc(per_step * nsteps));
+    double4* st_buf = static_cast<double4*>(dalloc(state_b));
+    if (!tape || !st_buf) {
+        (void)hipGetLastError();
+        release();
+        return fail(c, SPH_E_DEVICE, "hipMalloc failed for the adjoint tape");
+    }
+    struct Rec {
+        double4 *x, *v, *w, *dt0, *wnx, *wmv;
+        uint32_t* rows;
+    };
+    std::vector<Rec> rec(nsteps);
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        char* q = tape + k * per_step;
+        Rec& r = rec[k];
+        r.x = reinterpret_cast<double4*>(q); q += np * 32;
+        r.v = reinterpret_cast<double4*>(q); q += np * 32;
+        r.w = reinterpret_cast<double4*>(q); q += np * 32;
+        r.dt0 = reinterpret_cast<double4*>(q); q += nc * 32;
+        r.wnx = reinterpret_cast<double4*>(q); q += nw * 32;
+        r.wmv = reinterpret_cast<double4*>(q); q += nw * 32;
+        r.rows = reinterpret_cast<uint32_t*>(q);
+    }
+    const int64_t iter0 = S.iter;
+    const DevParams P = S.P;
+    // Forward pass, recording each step's inputs and contact rows.
+#define D2D(dst, src, bytes)                                                            \
+    do {                                                                                \
+        hipError_t e_ = hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToDevice, s); \
+        if (e_ != hipSuccess) {                                                         \
+            release();                                                                  \
+            return hip_fail(c, e_, "tape");                                             \
+        }                                                                               \
+    } while (0)
+    for (uint64_t k = 0; k < nsteps; ++k) {
+        const Rec& r = rec[k];
+        D2D(r.x, A.x, np * 32);
+        D2D(r.v, A.vel, np * 32);
+        D2D(r.w, A.angvel, np * 32);
+        D2D(r.dt0, A.delta_t, nc * 32);
+        if (nw) {
+            D2D(r.wnx, A.w_nx, nw * 32);
+            D2D(r.wmv, A.w_mvfd, nw * 32);
+        }
+        c->tape_rows = r.rows;
+        int rc = enqueue_step(c);
+        c->tape_rows = nullptr;
+        if (rc) {
+            release();
+            return rc;
+        }
+    }
+#undef D2D
+    // Reverse pass.
+    double4* q = st_buf;
+    AdjState g;
+    g.Lx = q; q += np;
+    g.Lv = q; q += np;
+    g.Lw = q; q += np;
+    g.La = q; q += np;
+    g.Laa = q; q += np;
+    g.gF = q; q += np;
+    g.gT = q; q += np;
+    double4* ldt[2] = {q, q + nc};
+    q += 2 * nc;
+    g.gparam = reinterpret_cast<double*>(q);
+    std::vector<double> init(np * 4, 0.0);
+    const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
+    double4* dst[3] = {g.Lx, g.Lv, g.Lw};
+    int rc = SPH_OK;
+    for (int f = 0; f < 3 && !rc; ++f) {
+        for (size_t i = 0; i < np; ++i)
+            for (int d = 0; d < 3; ++d) init[4 * i + d] = seeds[f] ? seeds[f][3 * i + d] : 0.0;
+        rc = h2d(c, dst[f], init.data(), np * 32);
+    }
+    if (rc) {
+        release();
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(g.La, 0, 2 * np * 32, s);
+    if (e == hipSuccess) e = hipMemsetAsync(ldt[0], 0, nc * 32, s);
+    if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8, s);
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(c, e, "memset");
+    }
+    int cur = 0;
+    for (uint64_t kk = nsteps; kk-- > 0;) {
+        const Rec& r = rec[kk];
+        AdjStep st;
+        st.x = r.x;
+        st.v = r.v;
+        st.w = r.w;
+        st.dt0 = r.dt0;
+        st.rows = r.rows;
+        st.prev_rows = kk > 0 ? rec[kk - 1].rows : nullptr;
+        st.wall_nx = r.wnx;
+        st.wall_mvfd = r.wmv;
+        st.first = (iter0 + (int64_t)kk == 0) ? 1 : 0;
+        g.Ldt_in = ldt[cur];
+        g.Ldt_out = ldt[cur ^ 1];
+        e = launch_adj_step(P, st, g, P.L[2], s);
+        if (e != hipSuccess) {
+            release();
+            return hip_fail(c, e, "adjoint step");
+        }
+        cur ^= 1;
+    }
+    std::vector<double> out(np * 4);
+    double* outs[3] = {g_x0, g_v0, g_w0};
+    for (int f = 0; f < 3 && !rc; ++f) {
+        if (!outs[f]) continue;
+        rc = d2h(c, out.data(), dst[f], np * 32);
+        for (size_t i = 0; !rc && i < np; ++i)
+            for (int d = 0; d < 3; ++d) outs[f][3 * i + d] = out[4 * i + d];
+    }
+    if (!rc && g_params) rc = d2h(c, g_params, g.gparam, ADJ_NPARAM * 8);
+    release();
     return rc;
 }
 

@@ -73,7 +73,11 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_open_error", "sph_version_string", "sph_open_batch", "sph_open_batch_mem",
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
-           "sph_set_scalar_sim")
+           "sph_set_scalar_sim", "sph_adjoint")
+
+# Parameters of sph_adjoint's g_params, in order.
+ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
+                  "mu_wd")
 
 QUIET = 0x1
 NOCHECK = 0x2
@@ -133,6 +137,7 @@ def load():
     L.sph_get_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_double)]
     L.sph_set_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    L.sph_adjoint.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp, vp]
     _LIB = L
     return L
 
@@ -229,6 +234,30 @@ class Context:
 
     def run(self, out_dir, sid):
         self._chk(self._L.sph_run(self._h, out_dir.encode(), sid.encode()))
+
+    def adjoint(self, nsteps, dL_dx=None, dL_dv=None, dL_dw=None):
+        """Gradient of L = sum dL_dx.x + dL_dv.v + dL_dw.w after `nsteps` steps
+        with respect to the state before them and to the contact parameters
+        (sph_adjoint).  Advances the simulation by nsteps.  Returns a dict
+        with "x", "v", "w" (np x 3) and one entry per ADJOINT_PARAMS name."""
+        n = int(self.scalar("np"))
+
+        def seed(a):
+            if a is None:
+                return None
+            a = numpy.ascontiguousarray(a, dtype=numpy.float64).reshape(n, 3)
+            return a
+
+        seeds = [seed(a) for a in (dL_dx, dL_dv, dL_dw)]
+        outs = [numpy.zeros((n, 3)) for _ in range(3)]
+        gp = numpy.zeros(len(ADJOINT_PARAMS))
+        ptr = [None if a is None else a.ctypes.data for a in seeds]
+        self._chk(self._L.sph_adjoint(self._h, int(nsteps), ptr[0], ptr[1], ptr[2],
+                                      outs[0].ctypes.data, outs[1].ctypes.data,
+                                      outs[2].ctypes.data, gp.ctypes.data))
+        res = {"x": outs[0], "v": outs[1], "w": outs[2]}
+        res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp)})
+        return res
 
     def profile(self, enable=True):
         self._chk(self._L.sph_profile(self._h, 1 if enable else 0))

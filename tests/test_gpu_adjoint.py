@@ -1,0 +1,147 @@
+"""The adjoint (sph_adjoint) against central finite differences of the
+oracle.  The adjoint differentiates lengths as exact roots, so the
+differences are taken on the oracle with FP64 lengths (oracle.set_exact_length);
+walls in these cases do not move with the particles (wmode 0 or none), the
+case the adjoint's frozen wall trajectories reproduce exactly."""
+
+import numpy
+import pytest
+
+import oracle
+from sphere_amd import engine
+from sphere_amd.sim import sim
+
+from . import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _loss_parts(o):
+    return o.get("x")[:, :3], o.get("vel")[:, :3], o.get("angvel")
+
+
+def _fd_dir(image, steps, perturb, h, seeds):
+    """Central difference of L = sum c . state along `perturb` (which edits
+    a sim copy by +-h), on the oracle with exact lengths."""
+    outs = []
+    for sign in (1.0, -1.0):
+        s = sim().frombytes(image)
+        perturb(s, sign * h)
+        o = oracle.Oracle(s.tobytes())
+        o.step(steps)
+        outs.append(_loss_parts(o))
+    return sum(float(numpy.sum(c * (a - b))) for c, a, b in zip(seeds, outs[0], outs[1])) / (2 * h)
+
+
+def _adjoint(image, steps, seeds):
+    ctx = engine.Context(image)
+    g = ctx.adjoint(steps, *seeds)
+    ctx.close()
+    return g
+
+
+@pytest.fixture(autouse=True)
+def exact_lengths():
+    oracle.set_exact_length(True)
+    yield
+    oracle.set_exact_length(False)
+
+
+def test_equal_mass_head_on_collision_swaps_velocities():
+    """Two equal elastic spheres: after the collision v1' = v2 and v2' = v1,
+    so d v2'_z / d v1_z = 1 and d v2'_z / d v2_z = 0 (closed form)."""
+    s = sim(np=2, sid="adj2")
+    s.radius[:] = 1.0
+    s.x[0, :] = [5.0, 5.0, 2.0]
+    s.x[1, :] = [5.0, 5.0, 4.05]
+    s.vel[0, 2] = 1.0
+    s.defineWorldBoundaries(L=[10, 10, 10])
+    s.initTemporal(total=1.0, file_dt=0.1)
+    s.g[:] = 0.0
+    s.gamma_n[0] = 0.0
+    s.gamma_t[0] = 0.0
+    dt = float(s.time_dt[0])
+    steps = int(0.08 / dt)
+    seeds = [numpy.zeros((2, 3)), numpy.zeros((2, 3)), numpy.zeros((2, 3))]
+    seeds[1][1, 2] = 1.0
+    g = _adjoint(s.tobytes(), steps, seeds)
+    assert abs(g["v"][0, 2] - 1.0) < 1e-3
+    assert abs(g["v"][1, 2]) < 1e-3
+
+
+def _check_params(image, steps, names, seeds, rtol):
+    g = _adjoint(image, steps, seeds)
+    scale = max(abs(g[n]) for n in names)
+    bad = []
+    for name in names:
+        v0 = float(getattr(sim().frombytes(image), name)[0])
+
+        def pert(s, d, name=name, v0=v0):
+            getattr(s, name)[0] = v0 * (1.0 + d)
+
+        fd = _fd_dir(image, steps, pert, 1e-6, seeds) / v0
+        if abs(g[name] - fd) > rtol * max(abs(fd), 1e-3 * scale):
+            bad.append("%s adjoint %.9e fd %.9e" % (name, g[name], fd))
+    return bad, g
+
+
+def _seeds(n, seed):
+    rng = numpy.random.default_rng(seed)
+    return [rng.standard_normal((n, 3)) for _ in range(3)]
+
+
+def test_settling_box_parameter_gradients():
+    s = cases.settling_case(300, 11)
+    s.gamma_wt[0] = 0.5 * float(s.gamma_wn[0])
+    image = s.tobytes()
+    bad, g = _check_params(image, 40, ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d",
+                                       "gamma_wn", "gamma_wt", "mu_wd"), _seeds(300, 1), 1e-4)
+    assert bad == []
+
+
+def test_settling_box_state_gradients():
+    s = cases.settling_case(300, 11)
+    image = s.tobytes()
+    steps = 40
+    seeds = _seeds(300, 2)
+    g = _adjoint(image, steps, seeds)
+    rng = numpy.random.default_rng(3)
+    for key, field, h in (("x", "x", 1e-8), ("v", "vel", 1e-6)):
+        d = rng.standard_normal((300, 3))
+
+        def pert(s2, eps, field=field, d=d):
+            getattr(s2, field)[:, :3] += eps * d
+
+        fd = _fd_dir(image, steps, pert, h, seeds)
+        ad = float(numpy.sum(g[key] * d))
+        assert abs(ad - fd) <= 1e-4 * abs(fd), (key, ad, fd)
+
+
+def test_sheared_bed_parameter_gradients():
+    """Periodic images, fixed-velocity layers, a fixed top wall."""
+    s = cases.fixture("shear1k.bin")
+    s.wmode[0] = 0
+    bad, g = _check_params(s.tobytes(), 30, ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s",
+                                             "mu_d"), _seeds(1000, 4), 1e-4)
+    assert bad == []
+
+
+def test_adjoint_leaves_the_forward_state():
+    s = cases.settling_case(300, 11)
+    image = s.tobytes()
+    ctx = engine.Context(image)
+    ctx.adjoint(25, numpy.ones((300, 3)))
+    oracle.set_exact_length(False)
+    ref = oracle.Oracle(image)
+    ref.step(25)
+    assert numpy.array_equal(ctx.get("x"), ref.get("x"))
+    ctx.close()
+
+
+def test_unsupported_cases_are_refused():
+    s = cases.fixture("shear1k.bin")
+    s.contactmodel[0] = 3
+    ctx = engine.Context(s.tobytes())
+    with pytest.raises(engine.EngineError):
+        ctx.adjoint(5, numpy.ones((1000, 3)))
+    ctx.close()
