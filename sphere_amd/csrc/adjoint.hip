@@ -315,7 +315,9 @@ __global__ __launch_bounds__(256) void k_adj_contact(const DevParams p, AdjStep 
             atomic_add3(g.Lw + b, gwb);
             // The stored displacement came from the same slot one step
             // earlier only if the slot held this partner then.
-            const bool carried = st.prev_rows && st.prev_rows[mp] == wrd;
+            // (The image code may change while the contact persists.)
+            const bool carried =
+                st.prev_rows && (st.prev_rows[mp] & CW_IDX_MASK) == (wrd & CW_IDX_MASK);
             g.Ldt_out[mp] = carried ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
                                     : make_double4(0.0, 0.0, 0.0, 0.0);
         }

@@ -57,6 +57,12 @@ struct DevArrays {
     double4* ws;         // which keeps those gathers local
     uint32_t* rank;      // original index -> sorted index (inverse of perm)
     uint32_t* movers;    // per simulation, MV_CAP indices whose key changed (unordered)
+    // The order a step used, kept when the step's end already re-sorted for
+    // the next one (what the reference's arrays hold after that step).
+    uint32_t* skey_bk;
+    uint32_t* perm_bk;
+    uint32_t* cs_bk;
+    uint32_t* ce_bk;
     // candidate lists (sorted-index rows of NV entries)
     uint2* vlist;
     uint32_t* vcount;    // entries per row, or VL_OVERFLOW

@@ -1552,6 +1552,16 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
     // The radix sort orders by the low 8 * passes bits of the key.
     const uint32_t kmask = passes >= 4 ? 0xffffffffu : ((1u << (8 * passes)) - 1u);
     const bool merge = M <= p.mv_max && M <= MV_CAP;
+    // Keep the order and cell ranges the finished step used.
+    for (uint32_t j = tid; j < N; j += BLOCK) {
+        a.skey_bk[B + j] = a.skey[B + j];
+        a.perm_bk[B + j] = a.perm[B + j];
+    }
+    for (uint32_t c = tid; c < p.ncells; c += BLOCK) {
+        a.cs_bk[p.cell_base + c] = a.cell_start[p.cell_base + c];
+        a.ce_bk[p.cell_base + c] = a.cell_end[p.cell_base + c];
+    }
+    __syncthreads();
     if (merge) {
         uint32_t n2 = 1;
         while (n2 < M) n2 <<= 1;
@@ -1655,6 +1665,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     if (do_sort) resort_one_block(p, a, passes, sorted_bits);
     if (threadIdx.x != 0) return;
     if (do_sort) lp->sorts += 1u;
+    lp->resorted_last = do_sort;
     const double t_now = lp->t;
     const double dt = p.dt;
     if (p.adaptive == 1 && p.nw > 0 && p.np > 0) lp->grid_Lz = a.w_nx[p.wall_base].w;

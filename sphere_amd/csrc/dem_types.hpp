@@ -111,6 +111,8 @@ struct DevLoop {
     uint32_t vl_builds;         // number of list rebuilds so far (diagnostic)
     uint32_t sorts;             // number of re-sorts so far (diagnostic)
     uint32_t nmovers;           // particles whose key changed for the next step
+    uint32_t resorted_last;     // the last step ended with the next step's re-sort
+    uint32_t pad1;
 };
 
 // Batch-wide state.

@@ -490,6 +490,8 @@ int allocate(sph_ctx* c) {
     ALLOC(A.vcount, np);
     ALLOC(A.x_build, np);
     ALLOC(A.movers, ns * MV_CAP);
+    ALLOC(A.skey_bk, np); ALLOC(A.perm_bk, np);
+    ALLOC(A.cs_bk, nkeys); ALLOC(A.ce_bk, nkeys);
     ALLOC(A.crow, np * NC);
     ALLOC(A.delta_t, np * NC);
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
@@ -1364,25 +1366,31 @@ int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
             }
             return SPH_OK;
         }
-        case SPH_F_CELLID: {
-            int rc = d2h(c, dst, A.skey + b0, bytes);
-            uint32_t* w = static_cast<uint32_t*>(dst);
-            for (size_t i = 0; !rc && i < np; ++i) w[i] -= S.P.cell_base;
-            return rc;
-        }
-        case SPH_F_PERM: {
-            int rc = d2h(c, dst, A.perm + b0, bytes);
-            uint32_t* w = static_cast<uint32_t*>(dst);
-            for (size_t i = 0; !rc && i < np; ++i) w[i] -= (uint32_t)b0;
-            return rc;
-        }
+        case SPH_F_CELLID:
+        case SPH_F_PERM:
         case SPH_F_CELLSTART:
         case SPH_F_CELLEND: {
-            const uint32_t* src = (f == SPH_F_CELLSTART ? A.cell_start : A.cell_end);
-            int rc = d2h(c, dst, src + S.P.cell_base, bytes);
+            // After a step that ended by re-sorting for the next one, the
+            // order the step itself used is in the backup arrays.
+            DevLoop L;
+            int rc = read_loop(c, sim, &L);
+            if (rc) return rc;
+            const bool bk = L.resorted_last != 0u;
+            const uint32_t* src = nullptr;
+            size_t off = 0;
+            if (f == SPH_F_CELLID) { src = bk ? A.skey_bk : A.skey; off = b0; }
+            if (f == SPH_F_PERM) { src = bk ? A.perm_bk : A.perm; off = b0; }
+            if (f == SPH_F_CELLSTART) { src = bk ? A.cs_bk : A.cell_start; off = S.P.cell_base; }
+            if (f == SPH_F_CELLEND) { src = bk ? A.ce_bk : A.cell_end; off = S.P.cell_base; }
+            rc = d2h(c, dst, src + off, bytes);
             uint32_t* w = static_cast<uint32_t*>(dst);
-            for (size_t i = 0; !rc && i < S.P.ncells; ++i)
-                if (w[i] != 0xffffffffu) w[i] -= (uint32_t)b0;
+            if (f == SPH_F_CELLID)
+                for (size_t i = 0; !rc && i < np; ++i) w[i] -= S.P.cell_base;
+            if (f == SPH_F_PERM)
+                for (size_t i = 0; !rc && i < np; ++i) w[i] -= (uint32_t)b0;
+            if (f == SPH_F_CELLSTART || f == SPH_F_CELLEND)
+                for (size_t i = 0; !rc && i < S.P.ncells; ++i)
+                    if (w[i] != 0xffffffffu) w[i] -= (uint32_t)b0;
             return rc;
         }
         case SPH_F_WALLS_NX: return d2h(c, dst, A.w_nx + w0, bytes);

@@ -49,7 +49,9 @@ def exact_lengths():
 
 def test_equal_mass_head_on_collision_swaps_velocities():
     """Two equal elastic spheres: after the collision v1' = v2 and v2' = v1,
-    so d v2'_z / d v1_z = 1 and d v2'_z / d v2_z = 0 (closed form)."""
+    so d v2'_z / d v1_z = 1 and d v2'_z / d v2_z = 0 in continuous time (the
+    discrete collision, about 30 steps long, departs by a few per cent);
+    and the adjoint matches central differences of the oracle."""
     s = sim(np=2, sid="adj2")
     s.radius[:] = 1.0
     s.x[0, :] = [5.0, 5.0, 2.0]
@@ -64,9 +66,15 @@ def test_equal_mass_head_on_collision_swaps_velocities():
     steps = int(0.08 / dt)
     seeds = [numpy.zeros((2, 3)), numpy.zeros((2, 3)), numpy.zeros((2, 3))]
     seeds[1][1, 2] = 1.0
-    g = _adjoint(s.tobytes(), steps, seeds)
-    assert abs(g["v"][0, 2] - 1.0) < 1e-3
-    assert abs(g["v"][1, 2]) < 1e-3
+    image = s.tobytes()
+    g = _adjoint(image, steps, seeds)
+    assert abs(g["v"][0, 2] - 1.0) < 0.05
+    assert abs(g["v"][1, 2]) < 0.05
+    for i in (0, 1):
+        def pert(s2, eps, i=i):
+            s2.vel[i, 2] += eps
+        fd = _fd_dir(image, steps, pert, 1e-7, seeds)
+        assert abs(g["v"][i, 2] - fd) < 1e-5, (i, g["v"][i, 2], fd)
 
 
 def _check_params(image, steps, names, seeds, rtol):

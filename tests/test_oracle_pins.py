@@ -104,17 +104,20 @@ def test_hash_matches_floor_formula():
 
 
 def test_hash_at_upper_boundary_p17():
-    """A particle at exactly x == L (kept by the `> L` wrap) hashes to
-    ix == num, one past the last cell of its row."""
+    """A particle at exactly x == L (kept by the `> L` wrap) is neither
+    clamped into the last cell of its row nor wrapped to the first: its key
+    is that of the first cell of the next row (ix == num)."""
     s = _random_box(10, 4)
     s.x[3, 0] = s.L[0]
     o = oracle.Oracle(s.tobytes())
     o.stage("hash")
     key = int(o.get("cellid")[3])
-    ix = key % int(s.num[0]) + (int(s.num[0]) if key % int(s.num[0]) == 0 else 0)
-    iy_row = (key - int(numpy.floor(s.x[3, 0] / (s.L[0] / s.num[0])))) // int(s.num[0])
-    assert int(numpy.floor(s.x[3, 0] / (s.L[0] / s.num[0]))) == int(s.num[0])
-    assert ix == int(s.num[0]) or iy_row >= 0
+    nx, ny = int(s.num[0]), int(s.num[1])
+    iy = int(s.x[3, 1] // (s.L[1] / ny))
+    iz = int(s.x[3, 2] // (s.L[2] / int(s.num[2])))
+    row_first = (iz * ny + iy) * nx
+    assert key == row_first + nx
+    assert key != row_first + nx - 1  # not clamped
 
 
 def test_sort_is_stable_lexicographic():
