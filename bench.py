@@ -22,6 +22,7 @@ import hashlib
 import json
 import os
 import platform
+import struct
 import subprocess
 import sys
 import time
@@ -93,19 +94,55 @@ def build_workload(args):
     return img
 
 
-def batch_bench(args):
-    """S simulations of log-uniform size (granular gases collapsing in closed
-    boxes) advanced `steps` steps: one batched context (all of them per
-    launch) against the per-simulation loop (one context each, all enqueued
-    before one synchronisation, and one after another)."""
+def apply_settings(ctx, args):
+    for item in args.set:
+        name, _, val = item.partition("=")
+        ctx.set_scalar(name.strip(), float(val))
+        log("set %s = %g" % (name.strip(), float(val)))
+
+
+def batch_images(args):
+    """The batch workload: `args.batch` simulations of log-uniform size in
+    500..20000 particles.  "shear" (default): direct-shear beds, each a
+    settled 500-particle periodic cell (one of four seeds) tiled in x and y
+    to its size, with its own friction (0.3..0.7) and normal stress
+    (5..20 kPa); "gas": granular gases collapsing in closed boxes from a
+    lattice start (few contacts within a short run)."""
     from sphere_amd import engine, scenes
     sizes = scenes.batch_sizes(args.batch, seed=args.seed)
-    images = [scenes.lattice_box(int(n), seed=args.seed + k).tobytes()
-              for k, n in enumerate(sizes)]
-    total = int(sizes.sum())
+    if args.batch_kind == "gas":
+        images = [scenes.lattice_box(int(n), seed=args.seed + k).tobytes()
+                  for k, n in enumerate(sizes)]
+        return images, "batched granular gases, lattice start, contactmodel 2"
+    t0 = time.time()
+
+    def opener(img):
+        return engine.Context(img)
+
+    bases = [scenes.settled_shear_cell(500, args.seed + k, open_stepper=opener,
+                                       settle_steps=args.settle_steps,
+                                       consolidate_steps=args.consolidate_steps)
+             for k in range(4)]
+    cells = scenes.batch_shear_cells(sizes, bases, seed=args.seed)
+    log("batch of sheared beds prepacked in %.1f s" % (time.time() - t0))
+    return ([c.tobytes() for c in cells],
+            "batched direct-shear beds (tiled settled 500-particle cells), contactmodel 2, "
+            "per-simulation mu 0.3..0.7 and normal stress 5..20 kPa")
+
+
+def batch_bench(args):
+    """S simulations of log-uniform size advanced `steps` steps: one batched
+    context (all of them per launch) against the per-simulation loop (one
+    context each, all enqueued before one synchronisation, and one after
+    another)."""
+    from sphere_amd import engine
+    images, workload = batch_images(args)
+    # np is the u32 after the f64 version and the i32 nd (src/file_io.cpp:430-433).
+    sizes = [struct.unpack_from("<I", im, 12)[0] for im in images]
+    total = int(sum(sizes))
     steps = args.steps
-    log("%d simulations, %d particles in total (%d..%d)" % (len(images), total, sizes.min(),
-                                                             sizes.max()))
+    log("%d simulations, %d particles in total (%d..%d)" % (len(images), total, min(sizes),
+                                                             max(sizes)))
 
     def timed(fn):
         t0 = time.perf_counter()
@@ -115,16 +152,23 @@ def batch_bench(args):
     ctx = engine.Context(images=images)
     if args.contact_lanes:
         ctx.set_scalar("contact_lanes", args.contact_lanes)
+    apply_settings(ctx, args)
     ctx.step(args.warmup, sync=True)
     t_batch = timed(lambda: ctx.step(steps, sync=True))
     z = sum(int((ctx.get("contacts", sim=k) != sizes[k]).sum()) for k in range(len(images)))
-    ctx.close()
     log("batched: %.4f s for %d steps" % (t_batch, steps))
+    ctx.profile(True)
+    ctx.step(min(steps, 50), sync=True)
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    ctx.close()
+    log("batched kernel means (us): " + ", ".join("%s %.2f" % (k, v[0]) for k, v in rep.items()))
 
     ctxs = [engine.Context(im) for im in images]
     for c in ctxs:
         if args.contact_lanes:
             c.set_scalar("contact_lanes", args.contact_lanes)
+        apply_settings(c, args)
         c.step(args.warmup, sync=False)
     for c in ctxs:
         c.sync()
@@ -146,7 +190,8 @@ def batch_bench(args):
     log("per-simulation loop: %.4f s enqueued together, %.4f s one after another"
         % (t_ovl, t_seq))
     value = total * steps / t_batch
-    best_loop = total * steps / min(t_ovl, t_seq)
+    plain_loop = total * steps / t_seq
+    overlapped_loop = total * steps / t_ovl
     line = {
         "metric": "particle-steps/s, batch of %d simulations FP64" % len(images),
         "value": round(value, 1),
@@ -160,22 +205,27 @@ def batch_bench(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "batched granular gases, lattice start, contactmodel 2",
-                   "nsims": len(images), "np_total": total, "np_min": int(sizes.min()),
-                   "np_max": int(sizes.max()), "seed": args.seed,
-                   "contacts_end": z},
-        "per_sim_loop": {"value": round(best_loop, 1),
-                         "enqueued_together_s": round(t_ovl, 5),
-                         "one_after_another_s": round(t_seq, 5)},
-        "speedup_vs_per_sim_loop": round(value / best_loop, 3),
+        "config": {"workload": workload,
+                   "nsims": len(images), "np_total": total, "np_min": int(min(sizes)),
+                   "np_max": int(max(sizes)), "seed": args.seed,
+                   "contacts_per_particle_end": round(z / total, 3)},
+        "kernel_us": {k: round(v[0], 3) for k, v in rep.items()},
+        "per_sim_loop": {"value": round(plain_loop, 1), "s": round(t_seq, 5),
+                         "note": "one context per simulation, stepped one after another"},
+        "overlapped_loop": {"value": round(overlapped_loop, 1), "s": round(t_ovl, 5),
+                            "note": "one context (and stream) per simulation, all enqueued "
+                                    "before one synchronisation"},
+        "speedup_vs_per_sim_loop": round(value / plain_loop, 3),
+        "speedup_vs_overlapped_loop": round(value / overlapped_loop, 3),
     }
     print(json.dumps(line), flush=True)
     return 0
 
 
 def adjoint_bench(args, img):
-    """Wall time of sph_adjoint over K steps (record forward + reverse)
-    against K plain forward steps, from the same state."""
+    """Wall time of sph_adjoint over K steps (record forward + reverse, with
+    the seed upload and the gradient download) against K plain forward
+    steps, on a context that already holds its tape."""
     import numpy
     from sphere_amd import engine
     K = args.adjoint
@@ -190,20 +240,37 @@ def adjoint_bench(args, img):
     fwd.step(K, sync=True)
     t_fwd = time.perf_counter() - t0
     fwd.close()
-    adj = engine.Context(state)
-    adj.step(1, sync=True)
     rng = numpy.random.default_rng(args.seed)
     seed_v = rng.standard_normal((n, 3))
-    t0 = time.perf_counter()
-    g = adj.adjoint(K, None, seed_v, None)
-    adj.sync()
-    t_adj = time.perf_counter() - t0
-    adj.close()
-    log("forward %d steps %.4f s, adjoint %.4f s (ratio %.2f); dL/dk_n %.4e"
-        % (K, t_fwd, t_adj, t_adj / t_fwd, g["k_n"]))
+
+    def run_adjoint(k):
+        # The first call allocates the context's tape; the second, timed,
+        # continues from the state the first one left.
+        adj = engine.Context(state)
+        adj.step(1, sync=True)
+        adj.adjoint(k, None, seed_v, None)
+        adj.sync()
+        t0 = time.perf_counter()
+        g = adj.adjoint(k, None, seed_v, None)
+        adj.sync()
+        t = time.perf_counter() - t0
+        adj.close()
+        return t, g
+
+    t_adj, g = run_adjoint(K)
+    k_short = max(1, K // 10)
+    t_short, _ = run_adjoint(k_short)
+    per_step = (t_adj - t_short) / (K - k_short) if K > k_short else float("nan")
+    fixed = t_adj - K * per_step
+    log("forward %d steps %.4f s, adjoint %.4f s (ratio %.2f; %.1f us per step + %.2f ms "
+        "per call); dL/dk_n %.4e"
+        % (K, t_fwd, t_adj, t_adj / t_fwd, per_step * 1e6, fixed * 1e3, g["k_n"]))
     print(json.dumps({"metric": "adjoint cost / forward cost", "value": round(t_adj / t_fwd, 3),
                       "unit": "ratio", "n_gpus": 1, "steps": K, "higher_is_better": False,
                       "np": n, "forward_s": round(t_fwd, 5), "adjoint_s": round(t_adj, 5),
+                      "adjoint_us_per_step": round(per_step * 1e6, 2),
+                      "adjoint_ms_per_call": round(fixed * 1e3, 3),
+                      "forward_us_per_step": round(t_fwd / K * 1e6, 2),
                       "data": "synthetic"}), flush=True)
     return 0
 
@@ -226,9 +293,13 @@ def main():
     ap.add_argument("--sort-mode", type=int, default=-1,
                     help="0: sort launches every step, 1: end-of-step one-workgroup "
                          "re-sort, 2: automatic (default)")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="set an engine scalar before timing (e.g. xcd_order=0, "
+                         "graph_steps=50); repeatable")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the "
                          "reference kernel for scaling HBM byte counters")
+    ap.add_argument("--batch-kind", choices=("shear", "gas"), default="shear")
     ap.add_argument("--batch", type=int, default=0, metavar="S",
                     help="instead: S independent simulations (log-uniform 500..20000 "
                          "particles) stepped as one batched context vs one context each")
@@ -270,6 +341,7 @@ def main():
         ctx.set_scalar("contact_lanes", args.contact_lanes)
     if args.sort_mode >= 0:
         ctx.set_scalar("sort_mode", args.sort_mode)
+    apply_settings(ctx, args)
     n = int(ctx.scalar("np"))
     nw = int(ctx.scalar("nw"))
     dt = ctx.scalar("dt")

@@ -114,8 +114,12 @@ enum sph_scalar {
     SPH_S_RESORT_MERGE_MAX = 16, /* particles changing cell up to which the end-of-step
                                     re-sort merges them into the previous order
                                     (0..2048, default 2048); above it, it sorts anew. */
-    SPH_S_GRAPH_STEPS = 17  /* > 0: sph_step replays steady-state chunks of this many
+    SPH_S_GRAPH_STEPS = 17, /* > 0: sph_step replays steady-state chunks of this many
                                steps from a captured HIP graph (default 0: launches). */
+    SPH_S_XCD_ORDER = 18    /* 1: the gathering kernels hand each XCD one contiguous
+                               range of workgroups; 0 (default): hardware order
+                               (round-robin over the XCDs), measured faster on the
+                               sheared bed.  Identical results. */
 };
 
 /* Open flags. */
@@ -210,6 +214,12 @@ const char* sph_last_error(const sph_ctx* ctx);
 /* Error message of the last failed sph_open / sph_open_mem on this thread. */
 const char* sph_open_error(void);
 const char* sph_version_string(void);
+
+/* Diagnostic, no reference counterpart: runs the engine's FP32 square root
+ * (the reference's sqrtf in length(), src/vector_arithmetic.h) on device
+ * `device` for every FP32 bit pattern in [lo, hi) (hi <= 2^32) and counts
+ * the results that differ from the correctly rounded root. */
+int sph_selftest_sqrt32(int device, uint64_t lo, uint64_t hi, uint64_t* mismatches);
 
 #ifdef __cplusplus
 }

@@ -228,76 +228,104 @@ __device__ __forceinline__ void atomic_add3(double4* dst, d3 v) {
     atomicAdd(&dst->z, v.z);
 }
 
-// Reverse of integrate for step k, in place on the cotangents: (Lx, Lv, Lw,
-// La, Laa) of the step's outputs (x', v', w', the new accelerations) become
-// those of its inputs, and gF, gT receive the force and torque cotangents.
-__global__ __launch_bounds__(256) void k_adj_integrate(const DevParams p, AdjStep st,
-                                                       AdjState g) {
-    const uint32_t o = blockIdx.x * 256 + threadIdx.x;
-    if (o >= p.np) return;
-    const double4 x = st.x[o], vel = st.v[o];
-    const double dt = p.dt;
-    const double r = x.w;
-    const double m = 4.0 / 3.0 * PI * r * r * r * p.rho;
-    const double I = 2.0 / 5.0 * m * r * r;
-    const double w = vel.w;  // fixvel
-    const double4 Lx = g.Lx[o], Lv = g.Lv[o], Lw = g.Lw[o], La = g.La[o], Laa = g.Laa[o];
-    // cotangents of acc and angacc (the new accelerations also feed the
-    // next step as acc0)
-    d3 ga = mk3(La.x + (2.0 / 3.0) * dt * dt * Lx.x + 1.5 * dt * Lv.x,
-                La.y + (2.0 / 3.0) * dt * dt * Lx.y + 1.5 * dt * Lv.y,
-                La.z + (2.0 / 3.0) * dt * dt * Lx.z + 1.5 * dt * Lv.z);
-    d3 gaa = mk3(Laa.x + 1.5 * dt * Lw.x, Laa.y + 1.5 * dt * Lw.y, Laa.z + 1.5 * dt * Lw.z);
-    // fixvel modes (src/integration.cuh): the masked components do not
-    // depend on the force.
-    if (w > 0.0001 && w < 10.0) {
-        ga.x = 0.0;
-        ga.y = 0.0;
-        gaa = zero3();
-    }
-    if (w >= 10.0) {
-        ga.y = 0.0;
-        gaa.x = 0.0;
-        gaa.z = 0.0;
-    }
-    if (w < -0.0001) {
-        ga = zero3();
-        gaa = zero3();
-    }
-    g.gF[o] = make_double4(ga.x / m, ga.y / m, ga.z / m, 0.0);
-    g.gT[o] = make_double4(gaa.x / I, gaa.y / I, gaa.z / I, 0.0);
-    const double c0 = st.first ? 0.0 : 1.0;  // acc0 is zero on the first step of a run
-    g.Lv[o] = make_double4(Lx.x * dt + Lv.x, Lx.y * dt + Lv.y, Lx.z * dt + Lv.z, 0.0);
-    g.La[o] = make_double4(c0 * (-(dt * dt / 6.0) * Lx.x - 0.5 * dt * Lv.x),
-                           c0 * (-(dt * dt / 6.0) * Lx.y - 0.5 * dt * Lv.y),
-                           c0 * (-(dt * dt / 6.0) * Lx.z - 0.5 * dt * Lv.z), 0.0);
-    g.Laa[o] = make_double4(c0 * -0.5 * dt * Lw.x, c0 * -0.5 * dt * Lw.y,
-                            c0 * -0.5 * dt * Lw.z, 0.0);
+constexpr int ALPP = 4;               // work-items per particle
+constexpr int ASPL = NC / ALPP;       // contact slots per work-item
+constexpr int APPB = 256 / ALPP;      // particles per workgroup
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int o = 1; o < ALPP; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
 }
 
-// Reverse of the contact and wall forces of step k, one work-item per
-// particle: cotangents of both partners' state (the partner's by atomics),
-// of the stored tangential displacements (handed to step k - 1 for contacts
-// that persisted), and of the parameters.
-__global__ __launch_bounds__(256) void k_adj_contact(const DevParams p, AdjStep st,
-                                                     AdjState g, double Lz) {
-    __shared__ double red[256];
-    const uint32_t a = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ d3 group_sum(d3 v) {
+    return mk3(group_sum(v.x), group_sum(v.y), group_sum(v.z));
+}
+
+// Reverse of one whole step, ALPP work-items per particle (in the order
+// `order`, the last sorted order, so that a workgroup's partners are near
+// each other):
+//  1. integrate (TY3): the cotangents of the step's outputs (Lout: x', v',
+//     w'; La, Laa: the new accelerations, which also feed the next step as
+//     acc0) give the force and torque cotangents gF, gT and the integrate
+//     part of the input cotangents;
+//  2. contacts: each work-item takes slots q, q + ALPP, ... of the row the
+//     step left, reverses contactLinear for each (the partner's cotangents
+//     by atomics) and hands the cotangent of the stored displacement to the
+//     step before when that slot held the same partner then;
+//  3. walls, on work-item 0;
+//  4. the particle's own input cotangents are added to Lin (other particles
+//     add to it concurrently); Lout was cleared for reuse after step 1.
+__global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st, AdjState g,
+                                                  double Lz) {
+    __shared__ double red[ADJ_NPARAM][4];
+    const uint32_t q = threadIdx.x % ALPP;
+    const uint32_t ia = blockIdx.x * APPB + threadIdx.x / ALPP;
     PGrad gp;
-    for (int q = 0; q < ADJ_NPARAM; ++q) gp.v[q] = 0.0;
-    if (a < p.np) {
+    for (int k = 0; k < ADJ_NPARAM; ++k) gp.v[k] = 0.0;
+    if (ia < p.np) {
         const uint32_t np = p.np;
-        const double4 xa4 = st.x[a];
-        const d3 xa = xyz(xa4), va = xyz(st.v[a]), wa = xyz(st.w[a]);
+        const uint32_t a = g.order ? g.order[ia] : ia;
+        const double4 xa4 = st.x[a], va4 = st.v[a];
+        const d3 xa = xyz(xa4), va = xyz(va4), wa = xyz(st.w[a]);
         const double ra = xa4.w;
-        const d3 gF = xyz(g.gF[a]), gT = xyz(g.gT[a]);
+        // 1. integrate
+        const double dt = p.dt;
+        const double m = 4.0 / 3.0 * PI * ra * ra * ra * p.rho;
+        const double I = 2.0 / 5.0 * m * ra * ra;
+        const double fv = va4.w;  // fixvel
+        const double4 Lx = g.Lout[0][a], Lv = g.Lout[1][a], Lw = g.Lout[2][a];
+        const double4 La = g.La[a], Laa = g.Laa[a];
+        d3 ga = mk3(La.x + (2.0 / 3.0) * dt * dt * Lx.x + 1.5 * dt * Lv.x,
+                    La.y + (2.0 / 3.0) * dt * dt * Lx.y + 1.5 * dt * Lv.y,
+                    La.z + (2.0 / 3.0) * dt * dt * Lx.z + 1.5 * dt * Lv.z);
+        d3 gaa = mk3(Laa.x + 1.5 * dt * Lw.x, Laa.y + 1.5 * dt * Lw.y, Laa.z + 1.5 * dt * Lw.z);
+        // fixvel modes (src/integration.cuh): the masked components do not
+        // depend on the force.
+        if (fv > 0.0001 && fv < 10.0) {
+            ga.x = 0.0;
+            ga.y = 0.0;
+            gaa = zero3();
+        }
+        if (fv >= 10.0) {
+            ga.y = 0.0;
+            gaa.x = 0.0;
+            gaa.z = 0.0;
+        }
+        if (fv < -0.0001) {
+            ga = zero3();
+            gaa = zero3();
+        }
+        const d3 gF = mk3(ga.x / m, ga.y / m, ga.z / m);
+        const d3 gT = mk3(gaa.x / I, gaa.y / I, gaa.z / I);
+        // The integrate part of the input cotangents (x' = x + ..., v' = v +
+        // ..., w' = w + ...) starts work-item 0's sums; the acceleration
+        // cotangents are replaced, and Lout is cleared for reuse.
         d3 gxa = zero3(), gva = zero3(), gwa = zero3();
-        for (int s = 0; s < NC; ++s) {
-            const size_t mp = (size_t)a * NC + s;
+        if (q == 0) {
+            gxa = mk3(Lx.x, Lx.y, Lx.z);
+            gva = mk3(Lx.x * dt + Lv.x, Lx.y * dt + Lv.y, Lx.z * dt + Lv.z);
+            gwa = mk3(Lw.x, Lw.y, Lw.z);
+            const double c0 = st.first ? 0.0 : 1.0;  // acc0 is zero on a run's first step
+            g.La[a] = make_double4(c0 * (-(dt * dt / 6.0) * Lx.x - 0.5 * dt * Lv.x),
+                                   c0 * (-(dt * dt / 6.0) * Lx.y - 0.5 * dt * Lv.y),
+                                   c0 * (-(dt * dt / 6.0) * Lx.z - 0.5 * dt * Lv.z), 0.0);
+            g.Laa[a] = make_double4(c0 * -0.5 * dt * Lw.x, c0 * -0.5 * dt * Lw.y,
+                                    c0 * -0.5 * dt * Lw.z, 0.0);
+            const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
+            g.Lout[0][a] = z;
+            g.Lout[1][a] = z;
+            g.Lout[2][a] = z;
+        }
+        // 2. contacts
+#pragma unroll 1
+        for (int j = 0; j < ASPL; ++j) {
+            const size_t mp = (size_t)a * NC + q + ALPP * j;
             const uint32_t wrd = st.rows[mp];
             const uint32_t b = wrd & CW_IDX_MASK;
+            const uint32_t pb = st.prev_rows ? (st.prev_rows[mp] & CW_IDX_MASK) : EMPTY;
             if (b >= np) {
-                g.Ldt_out[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (pb < np) g.Ldt_out[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
                 continue;
             }
             const uint32_t cx = (wrd >> CW_SHIFT_X) & 3u, cy = (wrd >> CW_SHIFT_Y) & 3u;
@@ -305,87 +333,97 @@ __global__ __launch_bounds__(256) void k_adj_contact(const DevParams p, AdjStep 
                               cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
             const double4 xb4 = st.x[b];
             const d3 xab = mk3(xa.x - xb4.x + dm.x, xa.y - xb4.y + dm.y, xa.z - xb4.z + dm.z);
+            const d3 gdt = st.last ? zero3() : xyz(g.Ldt_in[mp]);
             d3 gx = zero3(), gvb = zero3(), gwb = zero3(), gdt0 = zero3();
             adj_contact_linear(p, va, wa, xyz(st.v[b]), xyz(st.w[b]), ra, xb4.w, xab,
-                               xyz(st.dt0[mp]), gF, gT, xyz(g.Ldt_in[mp]), gx, gva, gwa, gvb,
-                               gwb, gdt0, gp);
+                               xyz(st.dt0[mp]), gF, gT, gdt, gx, gva, gwa, gvb, gwb, gdt0, gp);
             gxa += gx;
-            atomic_add3(g.Lx + b, -gx);
-            atomic_add3(g.Lv + b, gvb);
-            atomic_add3(g.Lw + b, gwb);
+            atomic_add3(g.Lin[0] + b, -gx);
+            atomic_add3(g.Lin[1] + b, gvb);
+            atomic_add3(g.Lin[2] + b, gwb);
             // The stored displacement came from the same slot one step
-            // earlier only if the slot held this partner then.
-            // (The image code may change while the contact persists.)
-            const bool carried =
-                st.prev_rows && (st.prev_rows[mp] & CW_IDX_MASK) == (wrd & CW_IDX_MASK);
-            g.Ldt_out[mp] = carried ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
-                                    : make_double4(0.0, 0.0, 0.0, 0.0);
+            // earlier only if the slot held this partner then (the image
+            // code may change while the contact persists).
+            if (pb < np)
+                g.Ldt_out[mp] = pb == b ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
+                                        : make_double4(0.0, 0.0, 0.0, 0.0);
         }
-        // Walls, as in the forward pass: top (wall 0 or L_z), bottom, then
-        // the side walls by boundary mode.
-        {
-            double wpos = Lz, wv = 0.0;
-            d3 wn = mk3(0.0, 0.0, -1.0);
-            if (p.nw >= 1) {
-                wn = xyz(st.wall_nx[0]);
-                wpos = st.wall_nx[0].w;
-                wv = st.wall_mvfd[0].y;
+        // 3. walls, as in the forward pass: top (wall 0 or L_z), bottom,
+        // then the side walls by boundary mode.
+        if (q == 0) {
+            {
+                double wpos = Lz, wv = 0.0;
+                d3 wn = mk3(0.0, 0.0, -1.0);
+                if (p.nw >= 1) {
+                    wn = xyz(st.wall_nx[0]);
+                    wpos = st.wall_nx[0].w;
+                    wv = st.wall_mvfd[0].y;
+                }
+                const double dw = wpos - (xa.z + ra);
+                if (dw < 0.0)
+                    adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 0.0, -1.0), dw, wv, gF, gT, gxa,
+                                     gva, gwa, gp);
             }
-            const double dw = wpos - (xa.z + ra);
-            if (dw < 0.0)
-                adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 0.0, -1.0), dw, wv, gF, gT, gxa,
-                                 gva, gwa, gp);
+            {
+                const double dw = xa.z - ra - p.origo[2];
+                if (dw < 0.0)
+                    adj_contact_wall(p, va, wa, ra, mk3(0.0, 0.0, 1.0), mk3(0.0, 0.0, 1.0), dw,
+                                     0.0, gF, gT, gxa, gva, gwa, gp);
+            }
+            if (p.periodic == 0) {
+                double wpos = p.L[0], wv = 0.0;
+                d3 wn = mk3(-1.0, 0.0, 0.0);
+                if (p.nw >= 2) { wn = xyz(st.wall_nx[1]); wpos = st.wall_nx[1].w; wv = st.wall_mvfd[1].y; }
+                double dw = wpos - (xa.x + ra);
+                if (dw < 0.0)
+                    adj_contact_wall(p, va, wa, ra, wn, mk3(-1.0, 0.0, 0.0), dw, wv, gF, gT, gxa,
+                                     gva, gwa, gp);
+                wpos = 0.0; wv = 0.0; wn = mk3(1.0, 0.0, 0.0);
+                if (p.nw >= 3) { wn = xyz(st.wall_nx[2]); wpos = st.wall_nx[2].w; wv = st.wall_mvfd[2].y; }
+                dw = xa.x - ra - wpos;
+                if (dw < 0.0)
+                    adj_contact_wall(p, va, wa, ra, wn, mk3(1.0, 0.0, 0.0), dw, wv, gF, gT, gxa,
+                                     gva, gwa, gp);
+            }
+            if (p.periodic == 0 || p.periodic == 2) {
+                double wpos = p.L[1], wv = 0.0;
+                d3 wn = mk3(0.0, -1.0, 0.0);
+                if (p.nw >= 4) { wn = xyz(st.wall_nx[3]); wpos = st.wall_nx[3].w; wv = st.wall_mvfd[3].y; }
+                double dw = wpos - (xa.y + ra);
+                if (dw < 0.0)
+                    adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, -1.0, 0.0), dw, wv, gF, gT, gxa,
+                                     gva, gwa, gp);
+                wpos = 0.0; wv = 0.0; wn = mk3(0.0, 1.0, 0.0);
+                if (p.nw >= 5) { wn = xyz(st.wall_nx[4]); wpos = st.wall_nx[4].w; wv = st.wall_mvfd[4].y; }
+                dw = xa.y - ra - wpos;
+                if (dw < 0.0)
+                    adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 1.0, 0.0), dw, wv, gF, gT, gxa,
+                                     gva, gwa, gp);
+            }
         }
-        {
-            const double dw = xa.z - ra - p.origo[2];
-            if (dw < 0.0)
-                adj_contact_wall(p, va, wa, ra, mk3(0.0, 0.0, 1.0), mk3(0.0, 0.0, 1.0), dw, 0.0,
-                                 gF, gT, gxa, gva, gwa, gp);
+        // 4. the particle's own input cotangents
+        gxa = group_sum(gxa);
+        gva = group_sum(gva);
+        gwa = group_sum(gwa);
+        if (q == 0) {
+            atomic_add3(g.Lin[0] + a, gxa);
+            atomic_add3(g.Lin[1] + a, gva);
+            atomic_add3(g.Lin[2] + a, gwa);
         }
-        if (p.periodic == 0) {
-            double wpos = p.L[0], wv = 0.0;
-            d3 wn = mk3(-1.0, 0.0, 0.0);
-            if (p.nw >= 2) { wn = xyz(st.wall_nx[1]); wpos = st.wall_nx[1].w; wv = st.wall_mvfd[1].y; }
-            double dw = wpos - (xa.x + ra);
-            if (dw < 0.0)
-                adj_contact_wall(p, va, wa, ra, wn, mk3(-1.0, 0.0, 0.0), dw, wv, gF, gT, gxa, gva,
-                                 gwa, gp);
-            wpos = 0.0; wv = 0.0; wn = mk3(1.0, 0.0, 0.0);
-            if (p.nw >= 3) { wn = xyz(st.wall_nx[2]); wpos = st.wall_nx[2].w; wv = st.wall_mvfd[2].y; }
-            dw = xa.x - ra - wpos;
-            if (dw < 0.0)
-                adj_contact_wall(p, va, wa, ra, wn, mk3(1.0, 0.0, 0.0), dw, wv, gF, gT, gxa, gva,
-                                 gwa, gp);
-        }
-        if (p.periodic == 0 || p.periodic == 2) {
-            double wpos = p.L[1], wv = 0.0;
-            d3 wn = mk3(0.0, -1.0, 0.0);
-            if (p.nw >= 4) { wn = xyz(st.wall_nx[3]); wpos = st.wall_nx[3].w; wv = st.wall_mvfd[3].y; }
-            double dw = wpos - (xa.y + ra);
-            if (dw < 0.0)
-                adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, -1.0, 0.0), dw, wv, gF, gT, gxa, gva,
-                                 gwa, gp);
-            wpos = 0.0; wv = 0.0; wn = mk3(0.0, 1.0, 0.0);
-            if (p.nw >= 5) { wn = xyz(st.wall_nx[4]); wpos = st.wall_nx[4].w; wv = st.wall_mvfd[4].y; }
-            dw = xa.y - ra - wpos;
-            if (dw < 0.0)
-                adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 1.0, 0.0), dw, wv, gF, gT, gxa, gva,
-                                 gwa, gp);
-        }
-        atomic_add3(g.Lx + a, gxa);
-        atomic_add3(g.Lv + a, gva);
-        atomic_add3(g.Lw + a, gwa);
     }
-    // Parameter cotangents: a workgroup sum, then one atomic per parameter.
-    for (int q = 0; q < ADJ_NPARAM; ++q) {
-        red[threadIdx.x] = gp.v[q];
-        __syncthreads();
-        for (unsigned h = 128; h > 0; h >>= 1) {
-            if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) atomicAdd(&g.gparam[q], red[0]);
-        __syncthreads();
+    // Parameter cotangents: wavefront sums, then the four wavefronts' sums,
+    // then one atomic per parameter and workgroup.
+    const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    for (int k = 0; k < ADJ_NPARAM; ++k) {
+        double v = gp.v[k];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) red[k][wave] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < ADJ_NPARAM) {
+        const int k = threadIdx.x;
+        atomicAdd(&g.gparam[k], red[k][0] + red[k][1] + red[k][2] + red[k][3]);
     }
 }
 
@@ -393,9 +431,8 @@ __global__ __launch_bounds__(256) void k_adj_contact(const DevParams p, AdjStep 
 
 hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
                            hipStream_t s) {
-    const dim3 grid((p.np + 255) / 256), block(256);
-    hipLaunchKernelGGL(k_adj_integrate, grid, block, 0, s, p, st, g);
-    hipLaunchKernelGGL(k_adj_contact, grid, block, 0, s, p, st, g, Lz);
+    const dim3 grid((p.np + APPB - 1) / APPB), block(256);
+    hipLaunchKernelGGL(k_adj_step, grid, block, 0, s, p, st, g, Lz);
     return hipGetLastError();
 }
 

@@ -34,20 +34,20 @@ struct AdjStep {
     const double4* wall_nx;    // walls at the step start
     const double4* wall_mvfd;
     int first;                 // the run's first step (acc0 = 0)
+    int last;                  // the recorded pass's last step (no displacement cotangent)
 };
 
 // Cotangents, updated step by step from the last to the first.
 struct AdjState {
-    double4* Lx;
-    double4* Lv;
-    double4* Lw;
-    double4* La;       // of the accelerations the step writes (next step's acc0)
-    double4* Laa;
-    double4* gF;       // scratch: force and torque cotangents of the step
-    double4* gT;
+    double4* Lout[3];  // of the step's outputs x', v', w' (read, then cleared)
+    double4* Lin[3];   // of its inputs x, v, w (zero on entry, accumulated)
+    double4* La;       // of the accelerations the step writes (next step's acc0),
+    double4* Laa;      //   replaced by those of the accelerations it read
     const double4* Ldt_in;  // np x NC: of the displacements the step writes
-    double4* Ldt_out;       // np x NC: of the displacements the step read
+    double4* Ldt_out;       // np x NC: of the displacements the step read, at the
+                            //   slots occupied in prev_rows
     double* gparam;         // ADJ_NPARAM sums
+    const uint32_t* order;  // particle order of the work (a permutation), or null
 };
 
 hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,

@@ -26,7 +26,8 @@ struct DevArrays {
     uint32_t nchunk;
     uint32_t nbond;
     uint32_t nkeys;             // global cell keys (length of cell_start/cell_end)
-    uint32_t pad0;
+    uint32_t xcd_order;         // 1: contiguous workgroup ranges per XCD (kernels that gather),
+                                // 0: hardware order
     // kinematics, 32 B per particle each (x,y,z,w)
     double4* x;        // position, radius in w
     double4* vel;      // velocity, fixvel in w
@@ -70,6 +71,10 @@ struct DevArrays {
     // contact history (NC words per particle, original-index rows)
     uint32_t* crow;      // packed partner index + periodic image code
     double4* delta_t;    // tangential displacement per slot (w unused)
+    // Recording for the adjoint (null otherwise): the contact pass writes
+    // its rows, and each contact's displacement before the step.
+    uint32_t* tape_rows;
+    double4* tape_dt0;
     // walls
     double4* w_nx;       // normal (xyz) and position (w)
     double4* w_mvfd;     // mass, velocity, force, sigma0

@@ -58,11 +58,16 @@ __device__ __forceinline__ const DevParams& chunk_params(const DevArrays& a, uin
 }
 
 // Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD
-// b % 8), and each XCD has its own L2.  Renumbering so that each XCD takes
-// one contiguous range of workgroups keeps the neighbours a workgroup
-// gathers (close in sorted order) in the L2 of the XCD that reads them.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+// b % 8), and each XCD has its own L2.  With xcd_order set, renumbering
+// gives each XCD one contiguous range of workgroups, so that the neighbours
+// a workgroup gathers (close in sorted order) tend to sit in the L2 of the
+// XCD that reads them.  On the 1e5-particle sheared bed the contact pass
+// measured 62.8 us in hardware order against 73.4 us renumbered (the
+// renumbered order runs the chip's concurrent workgroups over eight far
+// apart ranges), so hardware order is the default.
+__device__ __forceinline__ uint32_t xcd_block(const DevArrays& a, uint32_t b, uint32_t n) {
     constexpr uint32_t X = 8;
+    if (!a.xcd_order) return b;
     const uint32_t per = n / X, rem = n % X;
     const uint32_t x = b % X, k = b / X;
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
@@ -747,7 +752,7 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
 template <int CM>
 __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const DevArrays a) {
     __shared__ uint32_t srow[BLOCK * ROW_STRIDE];
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t blk = xcd_block(a, blockIdx.x, gridDim.x);
     const DevParams& p = chunk_params(a, blk);
     const uint32_t ia = blk * BLOCK + threadIdx.x;
     if (ia - p.base >= p.np) return;
@@ -880,6 +885,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     const double4 vel_b = a.vs[rb_s];
                     const d3 angvel_b = xyz(a.ws[rb_s]);
                     const double4 dt0 = a.delta_t[mp];
+                    if (a.tape_dt0) a.tape_dt0[mp] = dt0;
                     d3 nd;
                     Contrib C;
                     contrib_init(C);
@@ -906,6 +912,12 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
 #pragma unroll
         for (int q = 0; q < NC / 4; ++q)
             wrow[q] = make_uint4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
+        if (a.tape_rows) {
+            uint4* trow = reinterpret_cast<uint4*>(a.tape_rows + (size_t)ia_orig * NC);
+#pragma unroll
+            for (int q = 0; q < NC / 4; ++q)
+                trow[q] = make_uint4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
+        }
     } else if constexpr (CM == 1) {
         // findAndProcessContactsInCell, src/contactsearch.cuh:91-186.
         for (int dz = -1; dz < 2; ++dz)
@@ -981,7 +993,34 @@ struct LaneGroup {
     __device__ __forceinline__ d3 from(d3 v, uint32_t src) const {
         return mk3(from(v.x, src), from(v.y, src), from(v.z, src));
     }
+    // Group lane T's value, T fixed at compile time: for groups of four (a
+    // DPP quad) one quad_perm move per dword, with no LDS round trip.
+    template <int T>
+    __device__ __forceinline__ uint32_t bc(uint32_t v) const {
+        if constexpr (LPP == 4) {
+            constexpr int ctrl = T | (T << 2) | (T << 4) | (T << 6);
+            return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, ctrl, 0xf, 0xf, false);
+        } else {
+            return from(v, (uint32_t)T);
+        }
+    }
+    template <int T>
+    __device__ __forceinline__ double bc(double v) const {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint32_t lo = bc<T>((uint32_t)u), hi = bc<T>((uint32_t)(u >> 32));
+        return __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
+    }
+    template <int T>
+    __device__ __forceinline__ d3 bc(d3 v) const {
+        return mk3(bc<T>(v.x), bc<T>(v.y), bc<T>(v.z));
+    }
 };
+
+// Index of the (t+1)-th set bit of m (m has more than t bits set).
+__device__ __forceinline__ uint32_t nth_bit(uint32_t m, int t) {
+    for (int k = 0; k < t; ++k) m &= m - 1u;
+    return (uint32_t)(__ffs(m) - 1);
+}
 
 // The contact-slot policy (src/contactsearch.cuh:265-299) over the slot
 // registers of a group: the first slot holding the partner, else the last
@@ -989,7 +1028,8 @@ struct LaneGroup {
 // kernel).  Returns true when nc exceeds NC, which ends the current cell.
 template <int LPP>
 __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t (&slot)[NC / LPP],
-                                               uint32_t word, int& nc, DevLoop* loop) {
+                                               uint32_t (&srt)[NC / LPP], uint32_t word,
+                                               uint32_t sib, int& nc, DevLoop* loop) {
     constexpr int SPL = NC / LPP;
     const uint32_t ib_orig = word & CW_IDX_MASK;
     uint32_t match = 0, empty = 0;
@@ -1005,7 +1045,10 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
     if (cpos < NC && (uint32_t)cpos % LPP == g.q) {
 #pragma unroll
         for (int j = 0; j < SPL; ++j)
-            if (j == cpos / LPP) slot[j] = word;
+            if (j == cpos / LPP) {
+                slot[j] = word;
+                srt[j] = sib;
+            }
     }
     ++nc;
     if (nc > NC) {
@@ -1015,15 +1058,47 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
     return false;
 }
 
+// The interaction round's contributions, added on every lane of the group in
+// slot order (group lane T evaluated the T-th slot of the round); the slots
+// whose contact ended are marked in `drop` (bit j: this lane's slot j).
+template <int LPP, int T>
+__device__ __forceinline__ void replay_round(const LaneGroup<LPP>& g, int take, uint32_t tm,
+                                             uint32_t cm, uint32_t rem, const Contrib& C,
+                                             Acc& A, uint32_t& drop) {
+    if constexpr (T < LPP) {
+        if (T >= take) return;
+        if ((tm >> T) & 1u) {
+            Contrib c;
+            c.F = g.template bc<T>(C.F);
+            c.T = g.template bc<T>(C.T);
+            c.p = g.template bc<T>(C.p);
+            const uint32_t nn = g.template bc<T>((uint32_t)C.nev | ((uint32_t)C.nes << 8));
+            c.nev = (int)(nn & 0xffu);
+            c.nes = (int)(nn >> 8);
+            c.ev0 = g.template bc<T>(C.ev0);
+            c.ev1 = g.template bc<T>(C.ev1);
+            c.es = g.template bc<T>(C.es);
+            apply(A, c);
+        }
+        if ((cm >> T) & 1u) {
+            const uint32_t st = nth_bit(rem, T);
+            if (st % LPP == g.q) drop |= 1u << (st / LPP);
+        }
+        replay_round<LPP, T + 1>(g, take, tm, cm, rem, C, A, drop);
+    }
+}
+
 template <int CM, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     static_assert(CM == 2 || CM == 3, "history models only");
     constexpr int SPL = NC / LPP;       // slots per lane
     constexpr int PPB = BLOCK / LPP;    // particles per workgroup
     const LaneGroup<LPP> g;
-    const uint32_t ia = xcd_block(blockIdx.x, gridDim.x) * PPB + threadIdx.x / LPP;
-    // A workgroup's particles lie in one 256-particle chunk.
-    const DevParams& p = chunk_params(a, ia / BLOCK);
+    const uint32_t blk = xcd_block(a, blockIdx.x, gridDim.x);
+    const uint32_t ia = blk * PPB + threadIdx.x / LPP;
+    // A workgroup's particles lie in one 256-particle chunk (computed from
+    // the workgroup index alone, so that the parameter loads are scalar).
+    const DevParams& p = chunk_params(a, blk * PPB / BLOCK);
     if (ia - p.base >= p.np) return;  // whole groups leave together
     DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
@@ -1034,8 +1109,14 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     const double4 av_a = a.ws[ia];
     const d3 angvel_a = xyz(av_a);
     uint32_t slot[SPL];
+    // Sorted index of each slot's partner, where this step's search found
+    // it; NO_SORTED (a contact the search did not meet again) is looked up.
+    uint32_t srt[SPL];
 #pragma unroll
-    for (int j = 0; j < SPL; ++j) slot[j] = a.crow[(size_t)ia_orig * NC + g.q + LPP * j];
+    for (int j = 0; j < SPL; ++j) {
+        slot[j] = a.crow[(size_t)ia_orig * NC + g.q + LPP * j];
+        srt[j] = NO_SORTED;
+    }
 
     // topology: the candidate sequence of the 27-cell scan, LPP at a time.
     int nc = 0;
@@ -1099,7 +1180,8 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
                         while (hm && !cell_done) {
                             const uint32_t b = __ffs(hm) - 1;
                             hm &= hm - 1u;
-                            cell_done = group_register(g, slot, g.from(word, b), nc, lp);
+                            cell_done = group_register(g, slot, srt, g.from(word, b),
+                                                       r + b, nc, lp);
                         }
                     }
                 }
@@ -1140,7 +1222,8 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
                     cell_done = false;
                 }
                 if (cell_done) continue;
-                cell_done = group_register(g, slot, g.from(e.y, b), nc, lp);
+                cell_done = group_register(g, slot, srt, g.from(e.y, b),
+                                           g.from(e.x, b) & VL_IB_MASK, nc, lp);
             }
         }
     }
@@ -1171,11 +1254,15 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
         const int take = __popc(rem) < LPP ? __popc(rem) : LPP;
         const bool mine = m != 0u && (int)g.q < take;
         const uint32_t sidx = mine ? (uint32_t)(__ffs(m) - 1) : 0u;
-        uint32_t wrd = 0;
+        uint32_t wrd = 0, sib = NO_SORTED;
 #pragma unroll
         for (int j = 0; j < SPL; ++j) {
             const uint32_t v = g.from(slot[j], sidx % LPP);
-            if ((uint32_t)j == sidx / LPP) wrd = v;
+            const uint32_t u = g.from(srt[j], sidx % LPP);
+            if ((uint32_t)j == sidx / LPP) {
+                wrd = v;
+                sib = u;
+            }
         }
         Contrib C;
         contrib_init(C);
@@ -1190,7 +1277,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
             const size_t mp = (size_t)ia_orig * NC + sidx;
             const uint32_t ib_orig = wrd & CW_IDX_MASK;
             const d3 dm = shift_of(p, word_code(wrd));
-            const uint32_t rb_s = a.rank[ib_orig];
+            const uint32_t rb_s = sib != NO_SORTED ? sib : a.rank[ib_orig];
             const double4 xb = a.xs[rb_s];
             const double rb = xb.w;
             const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
@@ -1201,6 +1288,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
                 const double4 vel_b = a.vs[rb_s];
                 const d3 angvel_b = xyz(a.ws[rb_s]);
                 const double4 dt0 = a.delta_t[mp];
+                if (a.tape_dt0) a.tape_dt0[mp] = dt0;
                 d3 nd;
                 if constexpr (CM == 2)
                     nd = contact_linear(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab, len,
@@ -1216,31 +1304,20 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
         }
         const uint32_t tm = g.bits(touching);
         const uint32_t cm = g.bits(cleared);
-        for (int t = 0; t < take; ++t) {
-            const uint32_t st = g.from(sidx, (uint32_t)t);
-            if ((tm >> t) & 1u) {
-                Contrib c;
-                c.F = g.from(C.F, (uint32_t)t);
-                c.T = g.from(C.T, (uint32_t)t);
-                c.p = g.from(C.p, (uint32_t)t);
-                c.nev = (int)g.from((uint32_t)C.nev, (uint32_t)t);
-                c.nes = (int)g.from((uint32_t)C.nes, (uint32_t)t);
-                c.ev0 = g.from(C.ev0, (uint32_t)t);
-                c.ev1 = g.from(C.ev1, (uint32_t)t);
-                c.es = g.from(C.es, (uint32_t)t);
-                apply(A, c);
-            }
-            if (((cm >> t) & 1u) && st % LPP == g.q) {
+        uint32_t drop = 0;
+        replay_round<LPP, 0>(g, take, tm, cm, rem, C, A, drop);
 #pragma unroll
-                for (int j = 0; j < SPL; ++j)
-                    if ((uint32_t)j == st / LPP) slot[j] = EMPTY;
-            }
-        }
+        for (int j = 0; j < SPL; ++j)
+            if ((drop >> j) & 1u) slot[j] = EMPTY;
         // Drop the round's slots from the remaining set.
         for (int t = 0; t < take; ++t) rem &= rem - 1u;
     }
 #pragma unroll
     for (int j = 0; j < SPL; ++j) a.crow[(size_t)ia_orig * NC + g.q + LPP * j] = slot[j];
+    if (a.tape_rows) {
+#pragma unroll
+        for (int j = 0; j < SPL; ++j) a.tape_rows[(size_t)ia_orig * NC + g.q + LPP * j] = slot[j];
+    }
     walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q == 0);
 }
 
@@ -1809,6 +1886,21 @@ __global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevArrays a) {
     }
 }
 
+// Self-check of sqrt_rn_f32: the FP32 values with bit patterns in [lo, hi)
+// against the reference root; mismatches (other than two NaNs) are counted.
+__global__ __launch_bounds__(BLOCK) void k_check_sqrt32(unsigned long long* bad, uint64_t lo,
+                                                        uint64_t hi) {
+    unsigned long long n = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t u = lo + (uint64_t)blockIdx.x * BLOCK + threadIdx.x; u < hi; u += stride) {
+        const float x = __uint_as_float((uint32_t)u);
+        const float y = sqrt_rn_f32(x), r = sqrt_rn_f32_ref(x);
+        const bool nan2 = y != y && r != r;
+        n += (!nan2 && __float_as_uint(y) != __float_as_uint(r)) ? 1ull : 0ull;
+    }
+    if (n) atomicAdd(bad, n);
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1898,6 +1990,11 @@ hipError_t launch_integrate(const DevArrays& a, int hash_next, hipStream_t s) {
 hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_walls, dim3(a.nsim), dim3(BLOCK), 0, s, a, resort, passes, sorted_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_check_sqrt32(unsigned long long* bad, uint64_t lo, uint64_t hi, hipStream_t s) {
+    hipLaunchKernelGGL(k_check_sqrt32, dim3(4096), dim3(BLOCK), 0, s, bad, lo, hi);
     return hipGetLastError();
 }
 

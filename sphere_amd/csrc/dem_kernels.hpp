@@ -46,6 +46,7 @@ hipError_t launch_integrate(const DevArrays& a, int hash_next, hipStream_t s);
 // (src/integration.cuh:344-450, src/device.cu:2500-2538).  With resort != 0
 // it also re-sorts the simulation for the next step, in one workgroup, when
 // the integration pass (launched with hash_next) found a cell change.
+hipError_t launch_check_sqrt32(unsigned long long* bad, uint64_t lo, uint64_t hi, hipStream_t s);
 hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
                         hipStream_t s);
 // Fill n words with v.

@@ -40,31 +40,49 @@ __device__ __forceinline__ d3 cross(d3 a, d3 b) {
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 
-// Correctly rounded FP32 square root.  The candidate comes from the FP64
-// square root rounded to FP32; it is then checked against the exact
-// midpoints to its FP32 neighbours.  For y with a 24-bit significand the
-// midpoints carry at most 26 bits, so their squares (<= 52 bits) are exact
-// in FP64, and a square root never ties.  The result does not depend on the
-// accuracy of the hardware FP64 square root beyond one FP32 ulp.
-__device__ __forceinline__ float sqrt_rn_f32(float x) {
-    float y = (float)__builtin_sqrt((double)x);
-    if (x > 0.0f && x < __builtin_huge_valf()) {
-        const uint32_t b = __float_as_uint(y);
-        const float yu = __uint_as_float(b + 1u);
-        const float yd = __uint_as_float(b - 1u);
-        const double xd = (double)x;
-        const double mu = 0.5 * ((double)y + (double)yu);
-        const double md = 0.5 * ((double)y + (double)yd);
-        if (xd > mu * mu) {
-            y = yu;
-        } else if (xd < md * md) {
-            y = yd;
-        }
-    }
+// Correctly rounded FP32 square root.  A candidate y is checked against the
+// exact midpoints to its FP32 neighbours and moved by one ulp when the root
+// lies beyond one: for y with a 24-bit significand the midpoints carry at
+// most 26 bits, so their squares (<= 52 bits) are exact in FP64, and a
+// square root never ties.  This gives the correctly rounded root whenever
+// the candidate is within one FP32 ulp of it.
+__device__ __forceinline__ float sqrt_rn_fix(float x, float y) {
+    const uint32_t b = __float_as_uint(y);
+    const float yu = __uint_as_float(b + 1u);
+    const float yd = __uint_as_float(b - 1u);
+    const double xd = (double)x;
+    const double mu = 0.5 * ((double)y + (double)yu);
+    const double md = 0.5 * ((double)y + (double)yd);
+    if (xd > mu * mu) return yu;
+    if (xd < md * md) return yd;
     return y;
 }
 
-// The reference's length(): FP32 root of the FP64 dot, widened back.
+// The candidate: the hardware FP64 reciprocal square root r0 (relative
+// error below 2^-22) gives y0 = x r0, and one Newton step on the residual,
+// y1 = y0 + (x - y0^2) r0 / 2, has a relative error below 2^-43, so y1
+// rounded to FP32 is within one ulp of the correctly rounded root.
+// Zero, negative, infinite and NaN inputs follow IEEE sqrt.  Checked on
+// the device against the correctly rounded FP64 root rounded to FP32 (which
+// is correctly rounded: 53 >= 2 * 24 + 2) for every positive FP32 value by
+// sph_selftest_sqrt32.
+__device__ __forceinline__ float sqrt_rn_f32(float x) {
+    if (!(x > 0.0f && x < __builtin_huge_valf()))
+        return x == 0.0f ? x : (x < 0.0f ? __builtin_nanf("") : x);
+    const double xd = (double)x;
+    const double r0 = __builtin_amdgcn_rsq(xd);
+    const double y0 = xd * r0;
+    const double e = __builtin_fma(-y0, y0, xd);
+    const double y1 = __builtin_fma(e, 0.5 * r0, y0);
+    return sqrt_rn_fix(x, (float)y1);
+}
+
+// The reference for that check: correctly rounded FP64 root (llvm.sqrt.f64
+// without fast-math flags), rounded to FP32.
+__device__ __forceinline__ float sqrt_rn_f32_ref(float x) {
+    return (float)__builtin_sqrt((double)x);
+}
+
 __device__ __forceinline__ double sqrt32(double v) { return (double)sqrt_rn_f32((float)v); }
 __device__ __forceinline__ double length(d3 v) { return sqrt32(dot(v, v)); }
 

@@ -31,6 +31,8 @@ constexpr int NV = 32;                   // entries per particle
 constexpr uint32_t VL_OVERFLOW = 0xffffffffu;
 constexpr uint32_t VL_IB_MASK = (1u << 27) - 1u;
 constexpr uint32_t VL_CELL_SHIFT = 27;
+// A contact slot whose partner's sorted index the step has not seen.
+constexpr uint32_t NO_SORTED = 0xffffffffu;
 
 // Particles that changed cell, listed by the integration pass for the
 // one-workgroup re-sort at the end of the step.

@@ -148,7 +148,9 @@ struct sph_ctx {
     std::vector<Graph> graphs;
     // While an adjoint forward pass records: where the contact rows after
     // the interaction pass go.
-    uint32_t* tape_rows = nullptr;
+    // The adjoint's tape and reverse state, kept between calls.
+    void* adj_buf = nullptr;
+    size_t adj_bytes = 0;
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -442,6 +444,7 @@ void layout(sph_ctx* c) {
     A.nchunk = base / BLOCK;
     A.nbond = bonds;
     A.nkeys = keys;
+    A.xcd_order = 0;
     c->sorted_bits = std::max(1, bit_length(keys));
     if (c->sorted_bits > 32) c->sorted_bits = 32;
     c->passes = (c->sorted_bits + 7) / 8;
@@ -890,9 +893,6 @@ int enqueue_step(sph_ctx* c) {
     pr.begin("contact");
     HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
     pr.end();
-    if (c->tape_rows)
-        HIPCHK(c, hipMemcpyAsync(c->tape_rows, A.crow, (size_t)c->sims[0].h.np * NC * 4,
-                                 hipMemcpyDeviceToDevice, s), "tape");
     if (A.nbond > 0) {
         pr.begin("bonds");
         HIPCHK(c, launch_bonds(A, s), "bonds");
@@ -1560,6 +1560,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_RESORT_MERGE_MAX: *v = S.P.mv_max; return SPH_OK;
         case SPH_S_NSIMS: *v = (double)c->sims.size(); return SPH_OK;
         case SPH_S_GRAPH_STEPS: *v = c->graph_steps; return SPH_OK;
+        case SPH_S_XCD_ORDER: *v = c->A.xcd_order; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1604,6 +1605,12 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             if (!(v >= 0.0 && v <= 4096.0))
                 return fail(c, SPH_E_ARG, "graph steps must be in [0, 4096] (got %g)", v);
             c->graph_steps = (int)v;
+            return SPH_OK;
+        case SPH_S_XCD_ORDER:
+            if (v != 0.0 && v != 1.0)
+                return fail(c, SPH_E_ARG, "XCD order must be 0 or 1 (got %g)", v);
+            drop_graphs(c);
+            c->A.xcd_order = (uint32_t)v;
             return SPH_OK;
         case SPH_S_CONTACT_LANES:
             if (v != 1.0 && v != 4.0 && v != 8.0 && v != 16.0)
@@ -1667,34 +1674,37 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
             return fail(c, SPH_E_UNSUPPORTED, "the adjoint excludes rate-state particles");
     if (nsteps == 0) return SPH_OK;
     const size_t np = h.np, nw = h.nw, nc = np * NC;
-    const size_t per_step = np * 3 * sizeof(double4) + nc * (sizeof(double4) + 4) +
-                            2 * nw * sizeof(double4);
-    size_t free_b = 0, total_b = 0;
-    HIPCHK(c, hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const size_t state_b = np * 7 * sizeof(double4) + 2 * nc * sizeof(double4) + 1024;
-    if ((double)per_step * (double)nsteps + (double)state_b > 0.9 * (double)free_b)
-        return fail(c, SPH_E_ARG, "recording %llu steps needs %.1f GB of device memory",
-                    (unsigned long long)nsteps, (per_step * (double)nsteps + state_b) / 1e9);
-    const DevArrays& A = c->A;
-    hipStream_t s = c->stream;
-    std::vector<void*> bufs;
-    auto dalloc = [&](size_t bytes) -> void* {
-        void* p = nullptr;
-        if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
-        bufs.push_back(p);
-        return p;
-    };
-    auto release = [&]() {
-        (void)hipStreamSynchronize(s);
-        for (void* p : bufs) (void)hipFree(p);
-    };
-    char* tape = static_cast<char*>(dalloc(per_step * nsteps));
-    double4* st_buf = static_cast<double4*>(dalloc(state_b));
-    if (!tape || !st_buf) {
-        (void)hipGetLastError();
-        release();
-        return fail(c, SPH_E_DEVICE, "hipMalloc failed for the adjoint tape");
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    // Tape per step: x, v, w at the step start, the displacements of the
+    // step's contacts before it (written by the contact pass, at their slots),
+    // the walls, and the rows the contact pass left.
+    const size_t per_step = up(np * 3 * sizeof(double4)) + up(nc * sizeof(double4)) +
+                            up(2 * nw * sizeof(double4)) + up(nc * 4);
+    // Reverse state: cotangents of x, v, w (two sets), of the two
+    // accelerations, of the displacements (two sets), and the parameters.
+    const size_t state_b = up(np * 8 * sizeof(double4)) + up(2 * nc * sizeof(double4)) + 256;
+    const double need = (double)per_step * (double)nsteps + (double)state_b;
+    if (need > (double)c->adj_bytes) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(c, hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        if (need > 0.9 * (double)(free_b + c->adj_bytes))
+            return fail(c, SPH_E_ARG, "recording %llu steps needs %.1f GB of device memory",
+                        (unsigned long long)nsteps, need / 1e9);
+        HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
+        if (c->adj_buf) (void)hipFree(c->adj_buf);
+        c->adj_buf = nullptr;
+        c->adj_bytes = 0;
+        if (hipMalloc(&c->adj_buf, (size_t)need) != hipSuccess) {
+            (void)hipGetLastError();
+            c->adj_buf = nullptr;
+            return fail(c, SPH_E_DEVICE, "hipMalloc failed for the adjoint tape (%.1f GB)",
+                        need / 1e9);
+        }
+        c->adj_bytes = (size_t)need;
     }
+    DevArrays& A = c->A;
+    hipStream_t s = c->stream;
+    char* tape = static_cast<char*>(c->adj_buf);
     struct Rec {
         double4 *x, *v, *w, *dt0, *wnx, *wmv;
         uint32_t* rows;
@@ -1703,77 +1713,71 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     for (uint64_t k = 0; k < nsteps; ++k) {
         char* q = tape + k * per_step;
         Rec& r = rec[k];
-        r.x = reinterpret_cast<double4*>(q); q += np * 32;
-        r.v = reinterpret_cast<double4*>(q); q += np * 32;
-        r.w = reinterpret_cast<double4*>(q); q += np * 32;
-        r.dt0 = reinterpret_cast<double4*>(q); q += nc * 32;
-        r.wnx = reinterpret_cast<double4*>(q); q += nw * 32;
-        r.wmv = reinterpret_cast<double4*>(q); q += nw * 32;
+        r.x = reinterpret_cast<double4*>(q);
+        r.v = r.x + np;
+        r.w = r.v + np;
+        q += up(np * 3 * sizeof(double4));
+        r.dt0 = reinterpret_cast<double4*>(q);
+        q += up(nc * sizeof(double4));
+        r.wnx = reinterpret_cast<double4*>(q);
+        r.wmv = r.wnx + nw;
+        q += up(2 * nw * sizeof(double4));
         r.rows = reinterpret_cast<uint32_t*>(q);
     }
     const int64_t iter0 = S.iter;
     const DevParams P = S.P;
-    // Forward pass, recording each step's inputs and contact rows.
+    // Forward pass, recording each step's inputs; the contact pass writes
+    // the rows and the displacements it read.
 #define D2D(dst, src, bytes)                                                            \
     do {                                                                                \
         hipError_t e_ = hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToDevice, s); \
-        if (e_ != hipSuccess) {                                                         \
-            release();                                                                  \
-            return hip_fail(c, e_, "tape");                                             \
-        }                                                                               \
+        if (e_ != hipSuccess) return hip_fail(c, e_, "tape");                           \
     } while (0)
     for (uint64_t k = 0; k < nsteps; ++k) {
         const Rec& r = rec[k];
         D2D(r.x, A.x, np * 32);
         D2D(r.v, A.vel, np * 32);
         D2D(r.w, A.angvel, np * 32);
-        D2D(r.dt0, A.delta_t, nc * 32);
         if (nw) {
             D2D(r.wnx, A.w_nx, nw * 32);
             D2D(r.wmv, A.w_mvfd, nw * 32);
         }
-        c->tape_rows = r.rows;
+        A.tape_rows = r.rows;
+        A.tape_dt0 = r.dt0;
         int rc = enqueue_step(c);
-        c->tape_rows = nullptr;
-        if (rc) {
-            release();
-            return rc;
-        }
+        A.tape_rows = nullptr;
+        A.tape_dt0 = nullptr;
+        if (rc) return rc;
     }
 #undef D2D
     // Reverse pass.
-    double4* q = st_buf;
+    double4* q = reinterpret_cast<double4*>(tape + nsteps * per_step);
+    double4* L[2][3];
+    for (int b = 0; b < 2; ++b)
+        for (int f = 0; f < 3; ++f) {
+            L[b][f] = q;
+            q += np;
+        }
     AdjState g;
-    g.Lx = q; q += np;
-    g.Lv = q; q += np;
-    g.Lw = q; q += np;
     g.La = q; q += np;
     g.Laa = q; q += np;
-    g.gF = q; q += np;
-    g.gT = q; q += np;
     double4* ldt[2] = {q, q + nc};
     q += 2 * nc;
     g.gparam = reinterpret_cast<double*>(q);
+    g.order = A.perm;  // the last sorted order (one simulation: base 0)
     std::vector<double> init(np * 4, 0.0);
     const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
-    double4* dst[3] = {g.Lx, g.Lv, g.Lw};
     int rc = SPH_OK;
     for (int f = 0; f < 3 && !rc; ++f) {
         for (size_t i = 0; i < np; ++i)
             for (int d = 0; d < 3; ++d) init[4 * i + d] = seeds[f] ? seeds[f][3 * i + d] : 0.0;
-        rc = h2d(c, dst[f], init.data(), np * 32);
+        rc = h2d(c, L[0][f], init.data(), np * 32);
     }
-    if (rc) {
-        release();
-        return rc;
-    }
-    hipError_t e = hipMemsetAsync(g.La, 0, 2 * np * 32, s);
-    if (e == hipSuccess) e = hipMemsetAsync(ldt[0], 0, nc * 32, s);
+    if (rc) return rc;
+    hipError_t e = hipMemsetAsync(L[1][0], 0, 3 * np * 32, s);
+    if (e == hipSuccess) e = hipMemsetAsync(g.La, 0, 2 * np * 32, s);
     if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8, s);
-    if (e != hipSuccess) {
-        release();
-        return hip_fail(c, e, "memset");
-    }
+    if (e != hipSuccess) return hip_fail(c, e, "memset");
     int cur = 0;
     for (uint64_t kk = nsteps; kk-- > 0;) {
         const Rec& r = rec[kk];
@@ -1787,15 +1791,18 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         st.wall_nx = r.wnx;
         st.wall_mvfd = r.wmv;
         st.first = (iter0 + (int64_t)kk == 0) ? 1 : 0;
+        st.last = kk + 1 == nsteps ? 1 : 0;
+        for (int f = 0; f < 3; ++f) {
+            g.Lout[f] = L[cur][f];
+            g.Lin[f] = L[cur ^ 1][f];
+        }
         g.Ldt_in = ldt[cur];
         g.Ldt_out = ldt[cur ^ 1];
         e = launch_adj_step(P, st, g, P.L[2], s);
-        if (e != hipSuccess) {
-            release();
-            return hip_fail(c, e, "adjoint step");
-        }
+        if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
         cur ^= 1;
     }
+    double4* dst[3] = {L[cur][0], L[cur][1], L[cur][2]};
     std::vector<double> out(np * 4);
     double* outs[3] = {g_x0, g_v0, g_w0};
     for (int f = 0; f < 3 && !rc; ++f) {
@@ -1805,7 +1812,6 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
             for (int d = 0; d < 3; ++d) outs[f][3 * i + d] = out[4 * i + d];
     }
     if (!rc && g_params) rc = d2h(c, g_params, g.gparam, ADJ_NPARAM * 8);
-    release();
     return rc;
 }
 
@@ -1845,6 +1851,7 @@ void sph_close(sph_ctx* c) {
         delete c->prof;
     }
     for (void* p : c->allocs) (void)hipFree(p);
+    if (c->adj_buf) (void)hipFree(c->adj_buf);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1852,5 +1859,20 @@ void sph_close(sph_ctx* c) {
 const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : g_open_error.c_str(); }
 const char* sph_open_error(void) { return g_open_error.c_str(); }
 const char* sph_version_string(void) { return "sphere_hip 0.1 (.bin 2.15, gfx950)"; }
+
+int sph_selftest_sqrt32(int device, uint64_t lo, uint64_t hi, uint64_t* mismatches) {
+    if (!mismatches || lo > hi || hi > (1ull << 32)) return SPH_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SPH_E_DEVICE;
+    unsigned long long* d = nullptr;
+    if (hipMalloc(&d, sizeof *d) != hipSuccess) return SPH_E_DEVICE;
+    unsigned long long n = 0;
+    hipError_t e = hipMemcpy(d, &n, sizeof n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_check_sqrt32(d, lo, hi, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(&n, d, sizeof n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return SPH_E_DEVICE;
+    *mismatches = n;
+    return SPH_OK;
+}
 
 }  // extern "C"

@@ -63,7 +63,7 @@ FIELDS = {
 SCALARS = {"time": 0, "time_total": 1, "dt": 2, "file_dt": 3, "iter": 4, "step_count": 5,
            "np": 6, "nw": 7, "nb0": 8, "ncells": 9, "vl_skin": 10, "sorts": 11, "vl_builds": 12,
            "nsims": 13, "contact_lanes": 14, "sort_mode": 15, "resort_merge_max": 16,
-           "graph_steps": 17}
+           "graph_steps": 17, "xcd_order": 18}
 
 # Exported symbols of the C ABI (include/sphere_hip.h).
 SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_write_bin",
@@ -73,7 +73,7 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_open_error", "sph_version_string", "sph_open_batch", "sph_open_batch_mem",
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
-           "sph_set_scalar_sim", "sph_adjoint")
+           "sph_set_scalar_sim", "sph_adjoint", "sph_selftest_sqrt32")
 
 # Parameters of sph_adjoint's g_params, in order.
 ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
@@ -138,6 +138,8 @@ def load():
                                      ctypes.POINTER(ctypes.c_double)]
     L.sph_set_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.sph_adjoint.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp, vp]
+    L.sph_selftest_sqrt32.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
+                                      ctypes.POINTER(ctypes.c_uint64)]
     _LIB = L
     return L
 
@@ -289,3 +291,14 @@ def run_file(path, out_dir, sid, verbose=True, dry=False, device=-1):
         return e.code
     finally:
         ctx.close()
+
+
+def selftest_sqrt32(lo=0, hi=1 << 32, device=0):
+    """Mismatches of the engine's FP32 square root against the correctly
+    rounded one over the FP32 bit patterns [lo, hi)."""
+    L = load()
+    n = ctypes.c_uint64()
+    rc = L.sph_selftest_sqrt32(device, lo, hi, ctypes.byref(n))
+    if rc:
+        raise EngineError(rc, "sph_selftest_sqrt32 failed (%d)" % rc)
+    return n.value

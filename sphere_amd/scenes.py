@@ -12,6 +12,7 @@ rules follow python/sphere.py:3762-3873 (defaultParams) and 3307-3376
 * the sorting cell is 2.1 r_max (python/sphere.py:2632,2642-2644).
 """
 
+import copy
 import math
 
 import numpy
@@ -173,6 +174,69 @@ def batch_sizes(nsims=256, lo=500, hi=20000, seed=7):
     """Log-uniform particle counts in [lo, hi] for a batch of simulations."""
     rng = numpy.random.default_rng(seed)
     return numpy.exp(rng.uniform(math.log(lo), math.log(hi), nsims)).astype(int)
+
+
+_PER_PARTICLE = ("x", "radius", "xyzsum", "vel", "fixvel", "force", "angpos", "angvel",
+                 "torque", "es_dot", "es", "ev_dot", "ev", "p", "color")
+
+
+def tile_xy(s, mx, my):
+    """mx x my copies of a cell periodic in x and y, side by side: a cell
+    of mx L_x by my L_y with the same bed (the periodic images of the
+    original become real neighbours).  The grid keeps its cell size, and
+    the walls' masses scale with the area, so that a stress-controlled wall
+    moves as before.  Contact histories are not carried (the .bin holds
+    none)."""
+    if int(s.periodic[0]) != 1:
+        raise ValueError("tile_xy needs a cell periodic in x and y")
+    if int(s.nb0[0]):
+        raise ValueError("tile_xy does not copy bonds")
+    n = int(s.np[0])
+    L0, L1 = float(s.L[0]), float(s.L[1])
+    t = copy.deepcopy(s)
+    shift = numpy.array([[i * L0, j * L1, 0.0] for j in range(my) for i in range(mx)])
+    for name in _PER_PARTICLE:
+        a = numpy.asarray(getattr(s, name))
+        setattr(t, name, numpy.concatenate([a] * (mx * my), axis=0))
+    t.x = t.x + numpy.repeat(shift, n, axis=0)
+    t.np[0] = n * mx * my
+    t.L[0], t.L[1] = L0 * mx, L1 * my
+    t.num[0], t.num[1] = int(s.num[0]) * mx, int(s.num[1]) * my
+    t.w_m = numpy.asarray(s.w_m, dtype=numpy.float64) * (mx * my)
+    t.w_force = numpy.asarray(s.w_force, dtype=numpy.float64) * (mx * my)
+    t.sid = "%s-x%dy%d" % (s.sid, mx, my)
+    return t
+
+
+def near_square(m):
+    """The factor pair (a, b) of m with a >= b and a - b smallest."""
+    b = int(math.isqrt(m))
+    while m % b:
+        b -= 1
+    return m // b, b
+
+
+def batch_shear_cells(sizes, bases, seed=7, mu_range=(0.3, 0.7),
+                      stress_range=(5.0e3, 2.0e4)):
+    """A batch of sheared beds for many-simulation runs: simulation k is
+    base cell k % len(bases) (settled direct-shear cells, periodic in x and
+    y) tiled to the nearest multiple of its size in `sizes`, with its own
+    Coulomb friction and normal stress drawn from the ranges."""
+    rng = numpy.random.default_rng(seed)
+    out = []
+    for k, n in enumerate(sizes):
+        b = bases[k % len(bases)]
+        nb = int(b.np[0])
+        m = max(1, int(round(float(n) / nb)))
+        mx, my = near_square(m)
+        t = tile_xy(b, mx, my)
+        mu = rng.uniform(*mu_range)
+        t.mu_s[0] = mu
+        t.mu_d[0] = mu
+        t.w_sigma0 = numpy.asarray(t.w_sigma0, dtype=numpy.float64).copy()
+        t.w_sigma0[0] = rng.uniform(*stress_range)
+        out.append(t)
+    return out
 
 
 def lognormal_radii(n, median, sigma, rng):

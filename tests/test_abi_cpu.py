@@ -23,7 +23,7 @@ HEADER = os.path.join(ROOT, "include", "sphere_hip.h")
 
 def declared_symbols():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(sph_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(sph_[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -86,3 +86,38 @@ def test_capillary_case_builds_and_bridges():
     o.step(2)
     f = o.get("force")
     assert numpy.isfinite(f).all()
+
+
+def test_tiled_shear_cell_moves_like_its_copies():
+    """A periodic cell tiled 3 x 2 is the same bed: after a few oracle steps
+    every copy's particles are where the single cell's are (to the rounding
+    of the shifted coordinates), and the contact count scales by six."""
+    s = cases.fixture("shear1k.bin")
+    n = int(s.np[0])
+    t = scenes.tile_xy(s, 3, 2)
+    assert int(t.np[0]) == 6 * n
+    a, b = oracle.Oracle(s.tobytes()), oracle.Oracle(t.tobytes())
+    a.step(5)
+    b.step(5)
+    xa, xb = a.get("x")[:, :3], b.get("x")[:, :3]
+    L = numpy.asarray(s.L, dtype=numpy.float64)
+    for k in range(6):
+        i, j = k % 3, k // 3
+        d = xb[k * n:(k + 1) * n] - (xa + [i * L[0], j * L[1], 0.0])
+        assert numpy.abs(d).max() < 1e-9
+    va, vb = a.get("vel")[:, :3], b.get("vel")[:, :3]
+    assert numpy.abs(vb - numpy.tile(va, (6, 1))).max() < 1e-6 * numpy.abs(va).max()
+    za = int((a.get("contacts") != n).sum())
+    zb = int((b.get("contacts") != 6 * n).sum())
+    assert zb == 6 * za
+
+
+def test_batch_shear_cells_sizes_and_parameters():
+    s = cases.fixture("shear1k.bin")
+    sizes = [1000, 3000, 12000, 20000]
+    cells = scenes.batch_shear_cells(sizes, [s], seed=3)
+    assert [int(c.np[0]) for c in cells] == sizes
+    mus = [float(c.mu_s[0]) for c in cells]
+    assert len(set(mus)) == len(mus) and all(0.3 <= m <= 0.7 for m in mus)
+    assert all(5e3 <= float(c.w_sigma0[0]) <= 2e4 for c in cells)
+    assert float(s.w_sigma0[0]) == 1e4  # the base cell is untouched

@@ -187,6 +187,23 @@ def test_contact_lanes_bit_exact(case, lanes):
     ctx.close()
 
 
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_xcd_workgroup_order_bit_exact(lanes):
+    """The contact kernel with each XCD given one contiguous range of
+    workgroups gives the oracle's rows and state as well."""
+    image = cases.fixture("shear1k.bin").tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    assert ctx.scalar("xcd_order") == 0
+    ctx.set_scalar("contact_lanes", lanes)
+    ctx.set_scalar("xcd_order", 1)
+    assert ctx.scalar("xcd_order") == 1
+    ref.step(100)
+    ctx.step(100)
+    assert _compare(ctx, ref) == []
+    ctx.close()
+
+
 @pytest.mark.parametrize("mode,merge_max", [(0, 2048), (1, 2048), (1, 3), (1, 0), (2, 2048)])
 @pytest.mark.parametrize("case", ["settling", "shear1k"])
 def test_resort_placement_bit_exact(case, mode, merge_max):
