@@ -37,9 +37,12 @@ constexpr double PI = 3.14159265358979;  // src/constants.h:14
 
 __device__ __forceinline__ d3 zero3() { return mk3(0.0, 0.0, 0.0); }
 
-// Cotangents of the parameters, in the order of AdjParam.
+// Cotangents of the parameters, in the order of AdjParam: one work-item's
+// running sums, kept in LDS (stride 256 doubles between parameters) to
+// spare registers.
 struct PGrad {
-    double v[ADJ_NPARAM];
+    double* s;
+    __device__ __forceinline__ void add(int k, double x) const { s[k * 256] += x; }
 };
 
 // Reverse of contactLinear (src/contactmodels.cuh:244-411, kappa = 0,
@@ -66,9 +69,8 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
     const double dtpl = length(dtp);
     const double cn_raw = -p.k_n * delta + p.gamma_n * vel_n;
     const double cn = fmax(0.0, cn_raw);
-    const d3 f_n = cn * n;
-    const d3 f_c = -p.kappa * sqrt32(ra * rb) * n;
-    const double L = length(f_n - f_c);
+    // kappa = 0: the capillary term is a signed zero and |f_n - f_c| = |f_n|.
+    const double L = length(cn * n);
     const bool active = dtpl > 0.0 || vtl > 0.0;
     d3 dtmp = zero3(), f_raw = zero3(), t = zero3(), f_t = zero3();
     double frl = 0.0, limit = 0.0;
@@ -104,15 +106,15 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
             // dt_new = -1/k_t (mu_d L t + gamma_t vel_t), f_t = mu_s L t
             const d3 A = p.mu_d * L * t + p.gamma_t * vel_t;
             const d3 gA = gdt * (-1.0 / p.k_t);
-            gp.v[ADJ_KT] += dot(gdt, A) / (p.k_t * p.k_t);
-            gp.v[ADJ_MUD] += L * dot(t, gA);
+            gp.add(ADJ_KT, dot(gdt, A) / (p.k_t * p.k_t));
+            gp.add(ADJ_MUD, L * dot(t, gA));
             gL += p.mu_d * dot(t, gA);
             d3 gt = (p.mu_d * L) * gA;
-            gp.v[ADJ_GT] += dot(vel_t, gA);
+            gp.add(ADJ_GT, dot(vel_t, gA));
             gvel_t += p.gamma_t * gA;
             const double glim = dot(t, gft);
             gt += limit * gft;
-            gp.v[ADJ_MUS] += L * glim;
+            gp.add(ADJ_MUS, L * glim);
             gL += p.mu_s * glim;
             gfraw = (gt - t * dot(t, gt)) / frl;
         } else {
@@ -121,22 +123,22 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
         }
         // f_raw = -k_t dtmp - gamma_t vel_t, dtmp = dtp + vel_t dt
         gdtmp += -p.k_t * gfraw;
-        gp.v[ADJ_KT] += -dot(dtmp, gfraw);
-        gp.v[ADJ_GT] += -dot(vel_t, gfraw);
+        gp.add(ADJ_KT, -dot(dtmp, gfraw));
+        gp.add(ADJ_GT, -dot(vel_t, gfraw));
         gvel_t += -p.gamma_t * gfraw;
         gdtp += gdtmp;
         gvel_t += p.dt * gdtmp;
     } else {
         gdtp += gdt;
     }
-    if (L > 0.0) gfn += (gL / L) * (f_n - f_c);
+    if (L > 0.0) gfn += (gL / L) * (cn * n);
     const double gcn = dot(n, gfn);
     gn += cn * gfn;
     double gvel_n = 0.0;
     if (cn_raw > 0.0) {
-        gp.v[ADJ_KN] += -delta * gcn;
+        gp.add(ADJ_KN, -delta * gcn);
         gdelta += -p.k_n * gcn;
-        gp.v[ADJ_GN] += vel_n * gcn;
+        gp.add(ADJ_GN, vel_n * gcn);
         gvel_n = p.gamma_n * gcn;
     }
     // dtp = dt0 - n (n . dt0)
@@ -196,22 +198,22 @@ __device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3
     if (fric) {
         if (stick) {
             gvel_t += -p.gamma_wt * gft;
-            gp.v[ADJ_GWT] += -dot(vel_t, gft);
+            gp.add(ADJ_GWT, -dot(vel_t, gft));
         } else {
             const d3 s = vel_t / vtl;
             const double glim = -dot(s, gft);
             const d3 gs = -limit * gft;
             gvel_t += (gs - s * dot(s, gs)) / vtl;
-            gp.v[ADJ_MUWD] += fnl * glim;
+            gp.add(ADJ_MUWD, fnl * glim);
             if (fnl > 0.0) gfn += (p.mu_wd * glim / fnl) * f_n;
         }
     }
     const double gcn = dot(n, gfn);
     double gvel_n = 0.0;
     if (cn_raw > 0.0) {
-        gp.v[ADJ_KN] += -delta * gcn;
+        gp.add(ADJ_KN, -delta * gcn);
         gdelta += -p.k_n * gcn;
-        gp.v[ADJ_GWN] += -vel_n * gcn;
+        gp.add(ADJ_GWN, -vel_n * gcn);
         gvel_n = -p.gamma_wn * gcn;
     }
     const d3 gvel = gvel_t - n * dot(n, gvel_t);
@@ -242,9 +244,9 @@ __device__ __forceinline__ d3 group_sum(d3 v) {
     return mk3(group_sum(v.x), group_sum(v.y), group_sum(v.z));
 }
 
-// Reverse of one whole step, ALPP work-items per particle (in the order
-// `order`, the last sorted order, so that a workgroup's partners are near
-// each other):
+// Reverse of one whole step, ALPP work-items per particle, in the step's
+// sorted order (a workgroup's partners are then near each other in the
+// recorded state):
 //  1. integrate (TY3): the cotangents of the step's outputs (Lout: x', v',
 //     w'; La, Laa: the new accelerations, which also feed the next step as
 //     acc0) give the force and torque cotangents gF, gT and the integrate
@@ -259,15 +261,17 @@ __device__ __forceinline__ d3 group_sum(d3 v) {
 __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st, AdjState g,
                                                   double Lz) {
     __shared__ double red[ADJ_NPARAM][4];
+    __shared__ double psum[ADJ_NPARAM * 256];
     const uint32_t q = threadIdx.x % ALPP;
     const uint32_t ia = blockIdx.x * APPB + threadIdx.x / ALPP;
     PGrad gp;
-    for (int k = 0; k < ADJ_NPARAM; ++k) gp.v[k] = 0.0;
+    gp.s = psum + threadIdx.x;
+    for (int k = 0; k < ADJ_NPARAM; ++k) gp.s[k * 256] = 0.0;
     if (ia < p.np) {
         const uint32_t np = p.np;
-        const uint32_t a = g.order ? g.order[ia] : ia;
-        const double4 xa4 = st.x[a], va4 = st.v[a];
-        const d3 xa = xyz(xa4), va = xyz(va4), wa = xyz(st.w[a]);
+        const uint32_t a = st.perm[ia];
+        const double4 xa4 = st.x[ia], va4 = st.v[ia];
+        const d3 xa = xyz(xa4), va = xyz(va4), wa = xyz(st.w[ia]);
         const double ra = xa4.w;
         // 1. integrate
         const double dt = p.dt;
@@ -317,25 +321,63 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
             g.Lout[1][a] = z;
             g.Lout[2][a] = z;
         }
-        // 2. contacts
-#pragma unroll 1
+        // 2. contacts.  This work-item's row words (and the previous step's)
+        // are loaded at once; the occupied slots go into a small queue held
+        // in registers (shifted, never indexed), so the only dependent loads
+        // left are each contact's partner state.
+        uint32_t wv[ASPL], pv[ASPL];
+#pragma unroll
         for (int j = 0; j < ASPL; ++j) {
             const size_t mp = (size_t)a * NC + q + ALPP * j;
-            const uint32_t wrd = st.rows[mp];
-            const uint32_t b = wrd & CW_IDX_MASK;
-            const uint32_t pb = st.prev_rows ? (st.prev_rows[mp] & CW_IDX_MASK) : EMPTY;
-            if (b >= np) {
-                if (pb < np) g.Ldt_out[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
-                continue;
+            wv[j] = st.rows[mp];
+            pv[j] = st.prev_rows ? st.prev_rows[mp] : EMPTY;
+        }
+        // Queue entry: the row word, and j | previous slot occupied << 3 |
+        // same partner then << 4.
+        uint32_t qw[ASPL], qj[ASPL];
+        int nq = 0;
+#pragma unroll
+        for (int j = 0; j < ASPL; ++j) {
+            qw[j] = EMPTY;
+            qj[j] = 0;
+        }
+#pragma unroll
+        for (int j = 0; j < ASPL; ++j) {
+            const bool occ = (wv[j] & CW_IDX_MASK) < np;
+            if (!occ && (pv[j] & CW_IDX_MASK) < np)
+                g.Ldt_out[(size_t)a * NC + q + ALPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (occ) {
+#pragma unroll
+                for (int k = ASPL - 1; k > 0; --k) {
+                    qw[k] = qw[k - 1];
+                    qj[k] = qj[k - 1];
+                }
+                const uint32_t pb = pv[j] & CW_IDX_MASK;
+                qw[0] = wv[j];
+                qj[0] = (uint32_t)j | (pb < np ? 8u : 0u) |
+                        (pb == (wv[j] & CW_IDX_MASK) ? 16u : 0u);
+                ++nq;
             }
+        }
+#pragma unroll 1
+        for (int k = 0; k < nq; ++k) {
+            const uint32_t wrd = qw[0], fl = qj[0];
+            const size_t mp = (size_t)a * NC + q + ALPP * (fl & 7u);
+#pragma unroll
+            for (int kk = 0; kk + 1 < ASPL; ++kk) {
+                qw[kk] = qw[kk + 1];
+                qj[kk] = qj[kk + 1];
+            }
+            const uint32_t b = wrd & CW_IDX_MASK;
             const uint32_t cx = (wrd >> CW_SHIFT_X) & 3u, cy = (wrd >> CW_SHIFT_Y) & 3u;
             const d3 dm = mk3(cx == 1u ? 0.0 + p.L[0] : cx == 2u ? 0.0 - p.L[0] : 0.0,
                               cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
-            const double4 xb4 = st.x[b];
+            const uint32_t bs = st.srt[mp];  // the partner's sorted index
+            const double4 xb4 = st.x[bs];
             const d3 xab = mk3(xa.x - xb4.x + dm.x, xa.y - xb4.y + dm.y, xa.z - xb4.z + dm.z);
             const d3 gdt = st.last ? zero3() : xyz(g.Ldt_in[mp]);
             d3 gx = zero3(), gvb = zero3(), gwb = zero3(), gdt0 = zero3();
-            adj_contact_linear(p, va, wa, xyz(st.v[b]), xyz(st.w[b]), ra, xb4.w, xab,
+            adj_contact_linear(p, va, wa, xyz(st.v[bs]), xyz(st.w[bs]), ra, xb4.w, xab,
                                xyz(st.dt0[mp]), gF, gT, gdt, gx, gva, gwa, gvb, gwb, gdt0, gp);
             gxa += gx;
             atomic_add3(g.Lin[0] + b, -gx);
@@ -344,9 +386,9 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
             // The stored displacement came from the same slot one step
             // earlier only if the slot held this partner then (the image
             // code may change while the contact persists).
-            if (pb < np)
-                g.Ldt_out[mp] = pb == b ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
-                                        : make_double4(0.0, 0.0, 0.0, 0.0);
+            if (fl & 8u)
+                g.Ldt_out[mp] = (fl & 16u) ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
+                                           : make_double4(0.0, 0.0, 0.0, 0.0);
         }
         // 3. walls, as in the forward pass: top (wall 0 or L_z), bottom,
         // then the side walls by boundary mode.
@@ -415,7 +457,7 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
     // then one atomic per parameter and workgroup.
     const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     for (int k = 0; k < ADJ_NPARAM; ++k) {
-        double v = gp.v[k];
+        double v = gp.s[k * 256];
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
         if (lane == 0) red[k][wave] = v;
@@ -427,7 +469,33 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
     }
 }
 
+// Three-component host layout <-> the engine's four-component one.
+__global__ __launch_bounds__(256) void k_adj_expand(double4* dst, const double* src, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) dst[i] = make_double4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_adj_pack(double* dst, const double4* src, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        const double4 v = src[i];
+        dst[3 * i] = v.x;
+        dst[3 * i + 1] = v.y;
+        dst[3 * i + 2] = v.z;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_adj_expand(double4* dst, const double* src, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_expand, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_pack(double* dst, const double4* src, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_pack, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
                            hipStream_t s) {

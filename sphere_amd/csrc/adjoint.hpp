@@ -23,12 +23,15 @@ enum AdjParam : int {
     ADJ_NPARAM = 9
 };
 
-// What the forward pass recorded for one step (original particle order).
+// What the forward pass recorded for one step.  The state is in the step's
+// sorted order; rows and displacements are original-index rows.
 struct AdjStep {
     const double4* x;          // positions at the step start (radius in w)
     const double4* v;          // velocities (fixvel in w)
     const double4* w;          // angular velocities
+    const uint32_t* perm;      // sorted index -> original index
     const double4* dt0;        // np x NC tangential displacements before the step
+    const uint32_t* srt;       // np x NC the partner's sorted index, per contact
     const uint32_t* rows;      // np x NC contact rows after the interaction pass
     const uint32_t* prev_rows; // the previous step's rows, or null on the first step
     const double4* wall_nx;    // walls at the step start
@@ -47,9 +50,11 @@ struct AdjState {
     double4* Ldt_out;       // np x NC: of the displacements the step read, at the
                             //   slots occupied in prev_rows
     double* gparam;         // ADJ_NPARAM sums
-    const uint32_t* order;  // particle order of the work (a permutation), or null
 };
 
+// (n x 3) doubles <-> n double4 (w = 0).
+hipError_t launch_adj_expand(double4* dst, const double* src, uint32_t n, hipStream_t s);
+hipError_t launch_adj_pack(double* dst, const double4* src, uint32_t n, hipStream_t s);
 hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
                            hipStream_t s);
 

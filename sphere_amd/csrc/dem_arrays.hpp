@@ -71,10 +71,20 @@ struct DevArrays {
     // contact history (NC words per particle, original-index rows)
     uint32_t* crow;      // packed partner index + periodic image code
     double4* delta_t;    // tangential displacement per slot (w unused)
-    // Recording for the adjoint (null otherwise): the contact pass writes
-    // its rows, and each contact's displacement before the step.
+    // Recording for the adjoint (null otherwise; one simulation): the
+    // contact pass writes its rows and, per contact, the displacement
+    // before the step and the partner's sorted index (original-index rows);
+    // the state the step starts from and the order, in sorted order; the
+    // walls.
     uint32_t* tape_rows;
     double4* tape_dt0;
+    uint32_t* tape_srt;
+    uint32_t* tape_perm;
+    double4* tape_x;
+    double4* tape_v;
+    double4* tape_w;
+    double4* tape_wnx;
+    double4* tape_wmv;
     // walls
     double4* w_nx;       // normal (xyz) and position (w)
     double4* w_mvfd;     // mass, velocity, force, sigma0

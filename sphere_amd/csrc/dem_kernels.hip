@@ -747,6 +747,24 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
     if (p.nw > 4) a.force_pp[(size_t)np * 4 + o] = wf4;
 }
 
+// The adjoint's record of the step's starting state (one simulation):
+// each particle's and the order, at its sorted index, and the walls by the
+// work-item of original particle 0.
+__device__ __forceinline__ void tape_start(const DevArrays& a, const DevParams& p, uint32_t ia,
+                                           uint32_t ia_orig, double4 x, double4 v, double4 w,
+                                           bool writer) {
+    if (!a.tape_x || !writer) return;
+    a.tape_x[ia] = x;
+    a.tape_v[ia] = v;
+    a.tape_w[ia] = w;
+    a.tape_perm[ia] = ia_orig;
+    if (ia_orig == p.base)
+        for (uint32_t k = 0; k < p.nw; ++k) {
+            a.tape_wnx[k] = a.w_nx[p.wall_base + k];
+            a.tape_wmv[k] = a.w_mvfd[p.wall_base + k];
+        }
+}
+
 // CM is the contact model (1, 2 or 3); one instantiation per model keeps
 // only that model's registers live.
 template <int CM>
@@ -765,6 +783,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
     const double4 vel_a = a.vs[ia];
     const double4 av_a = a.ws[ia];
     const d3 angvel_a = xyz(av_a);
+    tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, true);
     Acc A;
     A.F = mk3(0.0, 0.0, 0.0);
     A.T = mk3(0.0, 0.0, 0.0);
@@ -885,7 +904,10 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     const double4 vel_b = a.vs[rb_s];
                     const d3 angvel_b = xyz(a.ws[rb_s]);
                     const double4 dt0 = a.delta_t[mp];
-                    if (a.tape_dt0) a.tape_dt0[mp] = dt0;
+                    if (a.tape_dt0) {
+                        a.tape_dt0[mp] = dt0;
+                        a.tape_srt[mp] = rb_s;
+                    }
                     d3 nd;
                     Contrib C;
                     contrib_init(C);
@@ -1108,6 +1130,7 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     const double4 vel_a = a.vs[ia];
     const double4 av_a = a.ws[ia];
     const d3 angvel_a = xyz(av_a);
+    tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
     uint32_t slot[SPL];
     // Sorted index of each slot's partner, where this step's search found
     // it; NO_SORTED (a contact the search did not meet again) is looked up.
@@ -1288,7 +1311,10 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
                 const double4 vel_b = a.vs[rb_s];
                 const d3 angvel_b = xyz(a.ws[rb_s]);
                 const double4 dt0 = a.delta_t[mp];
-                if (a.tape_dt0) a.tape_dt0[mp] = dt0;
+                if (a.tape_dt0) {
+                    a.tape_dt0[mp] = dt0;
+                    a.tape_srt[mp] = rb_s;
+                }
                 d3 nd;
                 if constexpr (CM == 2)
                     nd = contact_linear(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab, len,

@@ -1675,14 +1675,16 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     if (nsteps == 0) return SPH_OK;
     const size_t np = h.np, nw = h.nw, nc = np * NC;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    // Tape per step: x, v, w at the step start, the displacements of the
-    // step's contacts before it (written by the contact pass, at their slots),
-    // the walls, and the rows the contact pass left.
-    const size_t per_step = up(np * 3 * sizeof(double4)) + up(nc * sizeof(double4)) +
-                            up(2 * nw * sizeof(double4)) + up(nc * 4);
+    // Tape per step: x, v, w at the step start and the order (sorted), the
+    // displacements of the step's contacts before it and their partners'
+    // sorted indices (written by the contact pass, at their slots), the
+    // walls, and the rows the contact pass left.
+    const size_t per_step = up(np * 3 * sizeof(double4)) + up(np * 4) + up(nc * sizeof(double4)) +
+                            up(nc * 4) + up(2 * nw * sizeof(double4)) + up(nc * 4);
     // Reverse state: cotangents of x, v, w (two sets), of the two
     // accelerations, of the displacements (two sets), and the parameters.
-    const size_t state_b = up(np * 8 * sizeof(double4)) + up(2 * nc * sizeof(double4)) + 256;
+    const size_t state_b = up(np * 8 * sizeof(double4)) + up(2 * nc * sizeof(double4)) +
+                           up(np * 3 * sizeof(double)) + 256;
     const double need = (double)per_step * (double)nsteps + (double)state_b;
     if (need > (double)c->adj_bytes) {
         size_t free_b = 0, total_b = 0;
@@ -1707,7 +1709,7 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     char* tape = static_cast<char*>(c->adj_buf);
     struct Rec {
         double4 *x, *v, *w, *dt0, *wnx, *wmv;
-        uint32_t* rows;
+        uint32_t *perm, *srt, *rows;
     };
     std::vector<Rec> rec(nsteps);
     for (uint64_t k = 0; k < nsteps; ++k) {
@@ -1717,8 +1719,12 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         r.v = r.x + np;
         r.w = r.v + np;
         q += up(np * 3 * sizeof(double4));
+        r.perm = reinterpret_cast<uint32_t*>(q);
+        q += up(np * 4);
         r.dt0 = reinterpret_cast<double4*>(q);
         q += up(nc * sizeof(double4));
+        r.srt = reinterpret_cast<uint32_t*>(q);
+        q += up(nc * 4);
         r.wnx = reinterpret_cast<double4*>(q);
         r.wmv = r.wnx + nw;
         q += up(2 * nw * sizeof(double4));
@@ -1726,31 +1732,8 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     }
     const int64_t iter0 = S.iter;
     const DevParams P = S.P;
-    // Forward pass, recording each step's inputs; the contact pass writes
-    // the rows and the displacements it read.
-#define D2D(dst, src, bytes)                                                            \
-    do {                                                                                \
-        hipError_t e_ = hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToDevice, s); \
-        if (e_ != hipSuccess) return hip_fail(c, e_, "tape");                           \
-    } while (0)
-    for (uint64_t k = 0; k < nsteps; ++k) {
-        const Rec& r = rec[k];
-        D2D(r.x, A.x, np * 32);
-        D2D(r.v, A.vel, np * 32);
-        D2D(r.w, A.angvel, np * 32);
-        if (nw) {
-            D2D(r.wnx, A.w_nx, nw * 32);
-            D2D(r.wmv, A.w_mvfd, nw * 32);
-        }
-        A.tape_rows = r.rows;
-        A.tape_dt0 = r.dt0;
-        int rc = enqueue_step(c);
-        A.tape_rows = nullptr;
-        A.tape_dt0 = nullptr;
-        if (rc) return rc;
-    }
-#undef D2D
-    // Reverse pass.
+    // Reverse-pass state, and the seeds (uploaded before the forward pass,
+    // so that only the gradients' download waits for the device).
     double4* q = reinterpret_cast<double4*>(tape + nsteps * per_step);
     double4* L[2][3];
     for (int b = 0; b < 2; ++b)
@@ -1763,18 +1746,44 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     g.Laa = q; q += np;
     double4* ldt[2] = {q, q + nc};
     q += 2 * nc;
-    g.gparam = reinterpret_cast<double*>(q);
-    g.order = A.perm;  // the last sorted order (one simulation: base 0)
-    std::vector<double> init(np * 4, 0.0);
+    double* stage = reinterpret_cast<double*>(q);  // np x 3 host-layout staging
+    g.gparam = reinterpret_cast<double*>(reinterpret_cast<char*>(q) + up(np * 3 * sizeof(double)));
     const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
     int rc = SPH_OK;
+    hipError_t e = hipSuccess;
     for (int f = 0; f < 3 && !rc; ++f) {
-        for (size_t i = 0; i < np; ++i)
-            for (int d = 0; d < 3; ++d) init[4 * i + d] = seeds[f] ? seeds[f][3 * i + d] : 0.0;
-        rc = h2d(c, L[0][f], init.data(), np * 32);
+        if (!seeds[f]) {
+            e = hipMemsetAsync(L[0][f], 0, np * 32, s);
+            if (e != hipSuccess) return hip_fail(c, e, "memset");
+            continue;
+        }
+        rc = h2d(c, stage, seeds[f], np * 24);
+        if (!rc) {
+            e = launch_adj_expand(L[0][f], stage, (uint32_t)np, s);
+            if (e != hipSuccess) return hip_fail(c, e, "seed");
+        }
     }
     if (rc) return rc;
-    hipError_t e = hipMemsetAsync(L[1][0], 0, 3 * np * 32, s);
+    // Forward pass: the contact pass of each step records the state and
+    // walls the step starts from, its rows and the displacements it read.
+    for (uint64_t k = 0; k < nsteps && !rc; ++k) {
+        const Rec& r = rec[k];
+        A.tape_rows = r.rows;
+        A.tape_dt0 = r.dt0;
+        A.tape_srt = r.srt;
+        A.tape_perm = r.perm;
+        A.tape_x = r.x;
+        A.tape_v = r.v;
+        A.tape_w = r.w;
+        A.tape_wnx = r.wnx;
+        A.tape_wmv = r.wmv;
+        rc = enqueue_step(c);
+        A.tape_rows = A.tape_srt = A.tape_perm = nullptr;
+        A.tape_dt0 = A.tape_x = A.tape_v = A.tape_w = A.tape_wnx = A.tape_wmv = nullptr;
+    }
+    if (rc) return rc;
+    // Reverse pass.
+    e = hipMemsetAsync(L[1][0], 0, 3 * np * 32, s);
     if (e == hipSuccess) e = hipMemsetAsync(g.La, 0, 2 * np * 32, s);
     if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8, s);
     if (e != hipSuccess) return hip_fail(c, e, "memset");
@@ -1785,7 +1794,9 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         st.x = r.x;
         st.v = r.v;
         st.w = r.w;
+        st.perm = r.perm;
         st.dt0 = r.dt0;
+        st.srt = r.srt;
         st.rows = r.rows;
         st.prev_rows = kk > 0 ? rec[kk - 1].rows : nullptr;
         st.wall_nx = r.wnx;
@@ -1802,14 +1813,12 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
         cur ^= 1;
     }
-    double4* dst[3] = {L[cur][0], L[cur][1], L[cur][2]};
-    std::vector<double> out(np * 4);
     double* outs[3] = {g_x0, g_v0, g_w0};
     for (int f = 0; f < 3 && !rc; ++f) {
         if (!outs[f]) continue;
-        rc = d2h(c, out.data(), dst[f], np * 32);
-        for (size_t i = 0; !rc && i < np; ++i)
-            for (int d = 0; d < 3; ++d) outs[f][3 * i + d] = out[4 * i + d];
+        e = launch_adj_pack(stage, L[cur][f], (uint32_t)np, s);
+        if (e != hipSuccess) return hip_fail(c, e, "pack");
+        rc = d2h(c, outs[f], stage, np * 24);
     }
     if (!rc && g_params) rc = d2h(c, g_params, g.gparam, ADJ_NPARAM * 8);
     return rc;
