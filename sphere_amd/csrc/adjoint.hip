@@ -49,6 +49,7 @@ struct PGrad {
 // E <= 0.001) for one contact of particle a with partner b.  Inputs are the
 // forward inputs; gF, gT are the cotangents of a's force and torque and gdt
 // that of the new tangential displacement.  Outputs are accumulated.
+template <bool PARAMS>
 __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 wb, double ra,
                                    double rb, d3 x, d3 dt0, d3 gF, d3 gT, d3 gdt, d3& gx,
                                    d3& gva, d3& gwa, d3& gvb, d3& gwb, d3& gdt0, PGrad& gp) {
@@ -106,15 +107,15 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
             // dt_new = -1/k_t (mu_d L t + gamma_t vel_t), f_t = mu_s L t
             const d3 A = p.mu_d * L * t + p.gamma_t * vel_t;
             const d3 gA = gdt * (-1.0 / p.k_t);
-            gp.add(ADJ_KT, dot(gdt, A) / (p.k_t * p.k_t));
-            gp.add(ADJ_MUD, L * dot(t, gA));
+            if constexpr (PARAMS) gp.add(ADJ_KT, dot(gdt, A) / (p.k_t * p.k_t));
+            if constexpr (PARAMS) gp.add(ADJ_MUD, L * dot(t, gA));
             gL += p.mu_d * dot(t, gA);
             d3 gt = (p.mu_d * L) * gA;
-            gp.add(ADJ_GT, dot(vel_t, gA));
+            if constexpr (PARAMS) gp.add(ADJ_GT, dot(vel_t, gA));
             gvel_t += p.gamma_t * gA;
             const double glim = dot(t, gft);
             gt += limit * gft;
-            gp.add(ADJ_MUS, L * glim);
+            if constexpr (PARAMS) gp.add(ADJ_MUS, L * glim);
             gL += p.mu_s * glim;
             gfraw = (gt - t * dot(t, gt)) / frl;
         } else {
@@ -123,8 +124,8 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
         }
         // f_raw = -k_t dtmp - gamma_t vel_t, dtmp = dtp + vel_t dt
         gdtmp += -p.k_t * gfraw;
-        gp.add(ADJ_KT, -dot(dtmp, gfraw));
-        gp.add(ADJ_GT, -dot(vel_t, gfraw));
+        if constexpr (PARAMS) gp.add(ADJ_KT, -dot(dtmp, gfraw));
+        if constexpr (PARAMS) gp.add(ADJ_GT, -dot(vel_t, gfraw));
         gvel_t += -p.gamma_t * gfraw;
         gdtp += gdtmp;
         gvel_t += p.dt * gdtmp;
@@ -136,9 +137,9 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
     gn += cn * gfn;
     double gvel_n = 0.0;
     if (cn_raw > 0.0) {
-        gp.add(ADJ_KN, -delta * gcn);
+        if constexpr (PARAMS) gp.add(ADJ_KN, -delta * gcn);
         gdelta += -p.k_n * gcn;
-        gp.add(ADJ_GN, vel_n * gcn);
+        if constexpr (PARAMS) gp.add(ADJ_GN, vel_n * gcn);
         gvel_n = p.gamma_n * gcn;
     }
     // dtp = dt0 - n (n . dt0)
@@ -224,12 +225,6 @@ __device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3
     gx += gdelta * e;
 }
 
-__device__ __forceinline__ void atomic_add3(double4* dst, d3 v) {
-    atomicAdd(&dst->x, v.x);
-    atomicAdd(&dst->y, v.y);
-    atomicAdd(&dst->z, v.z);
-}
-
 constexpr int ALPP = 4;               // work-items per particle
 constexpr int ASPL = NC / ALPP;       // contact slots per work-item
 constexpr int APPB = 256 / ALPP;      // particles per workgroup
@@ -244,20 +239,65 @@ __device__ __forceinline__ d3 group_sum(d3 v) {
     return mk3(group_sum(v.x), group_sum(v.y), group_sum(v.z));
 }
 
+// Force and torque cotangents of one particle (reverse of TY3 for the
+// accelerations the step computes from them): from the cotangents of its
+// new position, velocity and angular velocity and of the new accelerations
+// (which also feed the next step as acc0).
+__device__ __forceinline__ void force_cotangents(const DevParams& p, double r, double fv,
+                                                 double4 Lx, double4 Lv, double4 Lw, double4 La,
+                                                 double4 Laa, d3& gF, d3& gT) {
+    const double dt = p.dt;
+    const double m = 4.0 / 3.0 * PI * r * r * r * p.rho;
+    const double I = 2.0 / 5.0 * m * r * r;
+    d3 ga = mk3(La.x + (2.0 / 3.0) * dt * dt * Lx.x + 1.5 * dt * Lv.x,
+                La.y + (2.0 / 3.0) * dt * dt * Lx.y + 1.5 * dt * Lv.y,
+                La.z + (2.0 / 3.0) * dt * dt * Lx.z + 1.5 * dt * Lv.z);
+    d3 gaa = mk3(Laa.x + 1.5 * dt * Lw.x, Laa.y + 1.5 * dt * Lw.y, Laa.z + 1.5 * dt * Lw.z);
+    // fixvel modes (src/integration.cuh): the masked components do not
+    // depend on the force.
+    if (fv > 0.0001 && fv < 10.0) {
+        ga.x = 0.0;
+        ga.y = 0.0;
+        gaa = zero3();
+    }
+    if (fv >= 10.0) {
+        ga.y = 0.0;
+        gaa.x = 0.0;
+        gaa.z = 0.0;
+    }
+    if (fv < -0.0001) {
+        ga = zero3();
+        gaa = zero3();
+    }
+    gF = mk3(ga.x / m, ga.y / m, ga.z / m);
+    gT = mk3(gaa.x / I, gaa.y / I, gaa.z / I);
+}
+
+__device__ __forceinline__ d3 image_shift(const DevParams& p, uint32_t wrd) {
+    const uint32_t cx = (wrd >> CW_SHIFT_X) & 3u, cy = (wrd >> CW_SHIFT_Y) & 3u;
+    return mk3(cx == 1u ? 0.0 + p.L[0] : cx == 2u ? 0.0 - p.L[0] : 0.0,
+               cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
+}
+
 // Reverse of one whole step, ALPP work-items per particle, in the step's
-// sorted order (a workgroup's partners are then near each other in the
-// recorded state):
-//  1. integrate (TY3): the cotangents of the step's outputs (Lout: x', v',
-//     w'; La, Laa: the new accelerations, which also feed the next step as
-//     acc0) give the force and torque cotangents gF, gT and the integrate
-//     part of the input cotangents;
+// sorted order (partner gathers then stay near each other in the recorded
+// state).  Every particle's input cotangent is gathered by its own group,
+// with no atomics:
+//  1. integrate (TY3): from the cotangents of the step's outputs (Lout,
+//     La/Laa out) the force and torque cotangents and the integrate part
+//     of the input cotangents; the acceleration cotangents of the step's
+//     inputs go to La/Laa in;
 //  2. contacts: each work-item takes slots q, q + ALPP, ... of the row the
-//     step left, reverses contactLinear for each (the partner's cotangents
-//     by atomics) and hands the cotangent of the stored displacement to the
-//     step before when that slot held the same partner then;
+//     step left.  For contact a -> b it reverses a's contactLinear (a's
+//     part, the parameters, and the stored displacement's cotangent, handed
+//     to the step before where that slot held the same partner then), and
+//     finds a in b's row to reverse b's contactLinear against a for the
+//     part that falls on a (b's force cotangents recomputed from b's Lout).
+//     Contact rows are symmetric unless a row overflowed; a contact whose
+//     partner does not list it is counted in `asym` (the host then refuses
+//     the result);
 //  3. walls, on work-item 0;
-//  4. the particle's own input cotangents are added to Lin (other particles
-//     add to it concurrently); Lout was cleared for reuse after step 1.
+//  4. the group's sums are written to Lin.
 __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st, AdjState g,
                                                   double Lz) {
     __shared__ double red[ADJ_NPARAM][4];
@@ -273,58 +313,30 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
         const double4 xa4 = st.x[ia], va4 = st.v[ia];
         const d3 xa = xyz(xa4), va = xyz(va4), wa = xyz(st.w[ia]);
         const double ra = xa4.w;
-        // 1. integrate
         const double dt = p.dt;
-        const double m = 4.0 / 3.0 * PI * ra * ra * ra * p.rho;
-        const double I = 2.0 / 5.0 * m * ra * ra;
-        const double fv = va4.w;  // fixvel
+        // 1. integrate
         const double4 Lx = g.Lout[0][a], Lv = g.Lout[1][a], Lw = g.Lout[2][a];
-        const double4 La = g.La[a], Laa = g.Laa[a];
-        d3 ga = mk3(La.x + (2.0 / 3.0) * dt * dt * Lx.x + 1.5 * dt * Lv.x,
-                    La.y + (2.0 / 3.0) * dt * dt * Lx.y + 1.5 * dt * Lv.y,
-                    La.z + (2.0 / 3.0) * dt * dt * Lx.z + 1.5 * dt * Lv.z);
-        d3 gaa = mk3(Laa.x + 1.5 * dt * Lw.x, Laa.y + 1.5 * dt * Lw.y, Laa.z + 1.5 * dt * Lw.z);
-        // fixvel modes (src/integration.cuh): the masked components do not
-        // depend on the force.
-        if (fv > 0.0001 && fv < 10.0) {
-            ga.x = 0.0;
-            ga.y = 0.0;
-            gaa = zero3();
+        d3 gF, gT;
+        {
+            const double4 La = g.La_out[a], Laa = g.Laa_out[a];
+            force_cotangents(p, ra, va4.w, Lx, Lv, Lw, La, Laa, gF, gT);
         }
-        if (fv >= 10.0) {
-            ga.y = 0.0;
-            gaa.x = 0.0;
-            gaa.z = 0.0;
-        }
-        if (fv < -0.0001) {
-            ga = zero3();
-            gaa = zero3();
-        }
-        const d3 gF = mk3(ga.x / m, ga.y / m, ga.z / m);
-        const d3 gT = mk3(gaa.x / I, gaa.y / I, gaa.z / I);
-        // The integrate part of the input cotangents (x' = x + ..., v' = v +
-        // ..., w' = w + ...) starts work-item 0's sums; the acceleration
-        // cotangents are replaced, and Lout is cleared for reuse.
         d3 gxa = zero3(), gva = zero3(), gwa = zero3();
         if (q == 0) {
+            // x' = x + ..., v' = v + ..., w' = w + ...
             gxa = mk3(Lx.x, Lx.y, Lx.z);
             gva = mk3(Lx.x * dt + Lv.x, Lx.y * dt + Lv.y, Lx.z * dt + Lv.z);
             gwa = mk3(Lw.x, Lw.y, Lw.z);
             const double c0 = st.first ? 0.0 : 1.0;  // acc0 is zero on a run's first step
-            g.La[a] = make_double4(c0 * (-(dt * dt / 6.0) * Lx.x - 0.5 * dt * Lv.x),
-                                   c0 * (-(dt * dt / 6.0) * Lx.y - 0.5 * dt * Lv.y),
-                                   c0 * (-(dt * dt / 6.0) * Lx.z - 0.5 * dt * Lv.z), 0.0);
-            g.Laa[a] = make_double4(c0 * -0.5 * dt * Lw.x, c0 * -0.5 * dt * Lw.y,
-                                    c0 * -0.5 * dt * Lw.z, 0.0);
-            const double4 z = make_double4(0.0, 0.0, 0.0, 0.0);
-            g.Lout[0][a] = z;
-            g.Lout[1][a] = z;
-            g.Lout[2][a] = z;
+            g.La_in[a] = make_double4(c0 * (-(dt * dt / 6.0) * Lx.x - 0.5 * dt * Lv.x),
+                                      c0 * (-(dt * dt / 6.0) * Lx.y - 0.5 * dt * Lv.y),
+                                      c0 * (-(dt * dt / 6.0) * Lx.z - 0.5 * dt * Lv.z), 0.0);
+            g.Laa_in[a] = make_double4(c0 * -0.5 * dt * Lw.x, c0 * -0.5 * dt * Lw.y,
+                                       c0 * -0.5 * dt * Lw.z, 0.0);
         }
         // 2. contacts.  This work-item's row words (and the previous step's)
         // are loaded at once; the occupied slots go into a small queue held
-        // in registers (shifted, never indexed), so the only dependent loads
-        // left are each contact's partner state.
+        // in registers (shifted, never indexed).
         uint32_t wv[ASPL], pv[ASPL];
 #pragma unroll
         for (int j = 0; j < ASPL; ++j) {
@@ -369,26 +381,54 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
                 qj[kk] = qj[kk + 1];
             }
             const uint32_t b = wrd & CW_IDX_MASK;
-            const uint32_t cx = (wrd >> CW_SHIFT_X) & 3u, cy = (wrd >> CW_SHIFT_Y) & 3u;
-            const d3 dm = mk3(cx == 1u ? 0.0 + p.L[0] : cx == 2u ? 0.0 - p.L[0] : 0.0,
-                              cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
             const uint32_t bs = st.srt[mp];  // the partner's sorted index
-            const double4 xb4 = st.x[bs];
-            const d3 xab = mk3(xa.x - xb4.x + dm.x, xa.y - xb4.y + dm.y, xa.z - xb4.z + dm.z);
-            const d3 gdt = st.last ? zero3() : xyz(g.Ldt_in[mp]);
-            d3 gx = zero3(), gvb = zero3(), gwb = zero3(), gdt0 = zero3();
-            adj_contact_linear(p, va, wa, xyz(st.v[bs]), xyz(st.w[bs]), ra, xb4.w, xab,
-                               xyz(st.dt0[mp]), gF, gT, gdt, gx, gva, gwa, gvb, gwb, gdt0, gp);
-            gxa += gx;
-            atomic_add3(g.Lin[0] + b, -gx);
-            atomic_add3(g.Lin[1] + b, gvb);
-            atomic_add3(g.Lin[2] + b, gwb);
-            // The stored displacement came from the same slot one step
-            // earlier only if the slot held this partner then (the image
-            // code may change while the contact persists).
-            if (fl & 8u)
-                g.Ldt_out[mp] = (fl & 16u) ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
-                                           : make_double4(0.0, 0.0, 0.0, 0.0);
+            const double4 xb4 = st.x[bs], vb4 = st.v[bs];
+            const d3 xb = xyz(xb4), vb = xyz(vb4), wb = xyz(st.w[bs]);
+            const double rb = xb4.w;
+            // a's contact with b: a's part, parameters, displacement.
+            {
+                const d3 dm = image_shift(p, wrd);
+                const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
+                const d3 gdt = st.last ? zero3() : xyz(g.Ldt_in[mp]);
+                d3 gx = zero3(), gvb = zero3(), gwb = zero3(), gdt0 = zero3();
+                adj_contact_linear<true>(p, va, wa, vb, wb, ra, rb, xab, xyz(st.dt0[mp]), gF, gT,
+                                         gdt, gx, gva, gwa, gvb, gwb, gdt0, gp);
+                gxa += gx;
+                if (fl & 8u)
+                    g.Ldt_out[mp] = (fl & 16u) ? make_double4(gdt0.x, gdt0.y, gdt0.z, 0.0)
+                                               : make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+            // b's contact with a: the part that falls on a.
+            uint32_t t = NC;
+            const uint4* brow = reinterpret_cast<const uint4*>(st.rows + (size_t)b * NC);
+#pragma unroll
+            for (int h = 0; h < NC / 4; ++h) {
+                const uint4 u = brow[h];
+                if (t == NC) {
+                    if ((u.x & CW_IDX_MASK) == a) t = 4 * h;
+                    else if ((u.y & CW_IDX_MASK) == a) t = 4 * h + 1;
+                    else if ((u.z & CW_IDX_MASK) == a) t = 4 * h + 2;
+                    else if ((u.w & CW_IDX_MASK) == a) t = 4 * h + 3;
+                }
+            }
+            if (t == NC) {
+                atomicAdd(g.asym, 1u);
+                continue;
+            }
+            const size_t mpb = (size_t)b * NC + t;
+            d3 gFb, gTb;
+            force_cotangents(p, rb, vb4.w, g.Lout[0][b], g.Lout[1][b], g.Lout[2][b],
+                             g.La_out[b], g.Laa_out[b], gFb, gTb);
+            const d3 dmb = image_shift(p, st.rows[mpb]);
+            const d3 xba = mk3(xb.x - xa.x + dmb.x, xb.y - xa.y + dmb.y, xb.z - xa.z + dmb.z);
+            const d3 gdtb = st.last ? zero3() : xyz(g.Ldt_in[mpb]);
+            d3 gx = zero3(), gvb_own = zero3(), gwb_own = zero3(), gva_part = zero3(),
+               gwa_part = zero3(), gdt0 = zero3();
+            adj_contact_linear<false>(p, vb, wb, va, wa, rb, ra, xba, xyz(st.dt0[mpb]), gFb, gTb,
+                                      gdtb, gx, gvb_own, gwb_own, gva_part, gwa_part, gdt0, gp);
+            gxa += -gx;
+            gva += gva_part;
+            gwa += gwa_part;
         }
         // 3. walls, as in the forward pass: top (wall 0 or L_z), bottom,
         // then the side walls by boundary mode.
@@ -443,14 +483,14 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
                                      gva, gwa, gp);
             }
         }
-        // 4. the particle's own input cotangents
+        // 4. the particle's input cotangents
         gxa = group_sum(gxa);
         gva = group_sum(gva);
         gwa = group_sum(gwa);
         if (q == 0) {
-            atomic_add3(g.Lin[0] + a, gxa);
-            atomic_add3(g.Lin[1] + a, gva);
-            atomic_add3(g.Lin[2] + a, gwa);
+            g.Lin[0][a] = make_double4(gxa.x, gxa.y, gxa.z, 0.0);
+            g.Lin[1][a] = make_double4(gva.x, gva.y, gva.z, 0.0);
+            g.Lin[2][a] = make_double4(gwa.x, gwa.y, gwa.z, 0.0);
         }
     }
     // Parameter cotangents: wavefront sums, then the four wavefronts' sums,

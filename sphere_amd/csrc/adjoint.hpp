@@ -41,15 +41,21 @@ struct AdjStep {
 };
 
 // Cotangents, updated step by step from the last to the first.
+// Cotangents, updated step by step from the last to the first; the "out"
+// arrays (of the step's outputs) are only read, the "in" arrays (of its
+// inputs) written, and the two swap between steps.
 struct AdjState {
-    double4* Lout[3];  // of the step's outputs x', v', w' (read, then cleared)
-    double4* Lin[3];   // of its inputs x, v, w (zero on entry, accumulated)
-    double4* La;       // of the accelerations the step writes (next step's acc0),
-    double4* Laa;      //   replaced by those of the accelerations it read
-    const double4* Ldt_in;  // np x NC: of the displacements the step writes
-    double4* Ldt_out;       // np x NC: of the displacements the step read, at the
-                            //   slots occupied in prev_rows
-    double* gparam;         // ADJ_NPARAM sums
+    const double4* Lout[3];  // of the step's outputs x', v', w'
+    double4* Lin[3];         // of its inputs x, v, w
+    const double4* La_out;   // of the accelerations the step writes (next step's acc0)
+    const double4* Laa_out;
+    double4* La_in;          // of the accelerations it read (acc0)
+    double4* Laa_in;
+    const double4* Ldt_in;   // np x NC: of the displacements the step writes
+    double4* Ldt_out;        // np x NC: of the displacements the step read, at the
+                             //   slots occupied in prev_rows
+    double* gparam;          // ADJ_NPARAM sums
+    uint32_t* asym;          // contacts missing from their partner's row
 };
 
 // (n x 3) doubles <-> n double4 (w = 0).

@@ -1683,7 +1683,7 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
                             up(nc * 4) + up(2 * nw * sizeof(double4)) + up(nc * 4);
     // Reverse state: cotangents of x, v, w (two sets), of the two
     // accelerations, of the displacements (two sets), and the parameters.
-    const size_t state_b = up(np * 8 * sizeof(double4)) + up(2 * nc * sizeof(double4)) +
+    const size_t state_b = up(np * 10 * sizeof(double4)) + up(2 * nc * sizeof(double4)) +
                            up(np * 3 * sizeof(double)) + 256;
     const double need = (double)per_step * (double)nsteps + (double)state_b;
     if (need > (double)c->adj_bytes) {
@@ -1736,18 +1736,23 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     // so that only the gradients' download waits for the device).
     double4* q = reinterpret_cast<double4*>(tape + nsteps * per_step);
     double4* L[2][3];
-    for (int b = 0; b < 2; ++b)
+    double4 *LA[2], *LAA[2];
+    for (int b = 0; b < 2; ++b) {
         for (int f = 0; f < 3; ++f) {
             L[b][f] = q;
             q += np;
         }
+        LA[b] = q;
+        q += np;
+        LAA[b] = q;
+        q += np;
+    }
     AdjState g;
-    g.La = q; q += np;
-    g.Laa = q; q += np;
     double4* ldt[2] = {q, q + nc};
     q += 2 * nc;
     double* stage = reinterpret_cast<double*>(q);  // np x 3 host-layout staging
     g.gparam = reinterpret_cast<double*>(reinterpret_cast<char*>(q) + up(np * 3 * sizeof(double)));
+    g.asym = reinterpret_cast<uint32_t*>(g.gparam + ADJ_NPARAM);
     const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
     int rc = SPH_OK;
     hipError_t e = hipSuccess;
@@ -1783,9 +1788,8 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     }
     if (rc) return rc;
     // Reverse pass.
-    e = hipMemsetAsync(L[1][0], 0, 3 * np * 32, s);
-    if (e == hipSuccess) e = hipMemsetAsync(g.La, 0, 2 * np * 32, s);
-    if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8, s);
+    e = hipMemsetAsync(LA[0], 0, 2 * np * 32, s);  // LA[0], LAA[0]
+    if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8 + 8, s);
     if (e != hipSuccess) return hip_fail(c, e, "memset");
     int cur = 0;
     for (uint64_t kk = nsteps; kk-- > 0;) {
@@ -1807,6 +1811,10 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
             g.Lout[f] = L[cur][f];
             g.Lin[f] = L[cur ^ 1][f];
         }
+        g.La_out = LA[cur];
+        g.Laa_out = LAA[cur];
+        g.La_in = LA[cur ^ 1];
+        g.Laa_in = LAA[cur ^ 1];
         g.Ldt_in = ldt[cur];
         g.Ldt_out = ldt[cur ^ 1];
         e = launch_adj_step(P, st, g, P.L[2], s);
@@ -1820,6 +1828,13 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         if (e != hipSuccess) return hip_fail(c, e, "pack");
         rc = d2h(c, outs[f], stage, np * 24);
     }
+    uint32_t asym = 0;
+    if (!rc) rc = d2h(c, &asym, g.asym, 4);
+    if (!rc && asym)
+        return fail(c, SPH_E_UNSUPPORTED,
+                    "%u recorded contacts were missing from their partner's row (a contact row "
+                    "overflowed NC = %d); the gradients are incomplete",
+                    asym, NC);
     if (!rc && g_params) rc = d2h(c, g_params, g.gparam, ADJ_NPARAM * 8);
     return rc;
 }
