@@ -1110,6 +1110,72 @@ __device__ __forceinline__ void replay_round(const LaneGroup<LPP>& g, int take, 
     }
 }
 
+// The common case of a round of list entries: every overlapping entry of
+// the round (bit t of hm: group lane t's entry e) is a partner that exactly
+// one slot of the row holds, the cell is not done and no overflow is
+// possible.  The slot policy then only rewrites each matched slot with the
+// entry's word (its image code may have changed) in entry order, and counts
+// nc up: done here at once, and true returned.  Otherwise nothing changes
+// and false is returned (the entries are then registered one by one).
+// Groups of four only (DPP quad broadcasts).
+template <int LPP>
+__device__ __forceinline__ bool group_register_known(const LaneGroup<LPP>& g,
+                                                     uint32_t (&slot)[NC / LPP],
+                                                     uint32_t (&srt)[NC / LPP], uint2 e,
+                                                     uint32_t hm, bool cell_done, int& nc,
+                                                     uint32_t& last_cell) {
+    if constexpr (LPP != 4) {
+        return false;
+    } else {
+        constexpr int SPL = NC / LPP;
+        const int nh = __popc(hm);
+        if (cell_done || nc + nh > NC) return false;
+        uint32_t idx[4];
+        idx[0] = g.template bc<0>(e.y) & CW_IDX_MASK;
+        idx[1] = g.template bc<1>(e.y) & CW_IDX_MASK;
+        idx[2] = g.template bc<2>(e.y) & CW_IDX_MASK;
+        idx[3] = g.template bc<3>(e.y) & CW_IDX_MASK;
+        // mm bit t * SPL + j: this lane's slot j holds hit t's partner.
+        uint32_t mm = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < SPL; ++j)
+                if (((hm >> t) & 1u) && (slot[j] & CW_IDX_MASK) == idx[t])
+                    mm |= 1u << (t * SPL + j);
+        // Byte t: the group's number of slots holding hit t's partner.
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            cnt += (uint32_t)__popc((mm >> (t * SPL)) & ((1u << SPL) - 1u)) << (8 * t);
+        cnt += (uint32_t)__shfl_xor((int)cnt, 1, 64);
+        cnt += (uint32_t)__shfl_xor((int)cnt, 2, 64);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            if (((cnt >> (8 * t)) & 0xffu) != ((hm >> t) & 1u)) return false;
+        uint32_t w[4], sb[4];
+        w[0] = g.template bc<0>(e.y);
+        w[1] = g.template bc<1>(e.y);
+        w[2] = g.template bc<2>(e.y);
+        w[3] = g.template bc<3>(e.y);
+        sb[0] = g.template bc<0>(e.x) & VL_IB_MASK;
+        sb[1] = g.template bc<1>(e.x) & VL_IB_MASK;
+        sb[2] = g.template bc<2>(e.x) & VL_IB_MASK;
+        sb[3] = g.template bc<3>(e.x) & VL_IB_MASK;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int j = 0; j < SPL; ++j)
+                if ((mm >> (t * SPL + j)) & 1u) {
+                    slot[j] = w[t];
+                    srt[j] = sb[t];
+                }
+        nc += nh;
+        last_cell = g.from(e.x, 31u - (uint32_t)__clz((int)hm)) >> VL_CELL_SHIFT;
+        return true;
+    }
+}
+
 template <int CM, int LPP>
 __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     static_assert(CM == 2 || CM == 3, "history models only");
@@ -1213,29 +1279,46 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
             a.x_build[ia_orig] = xa;
         }
     } else {
+        // Test every listed candidate first (lane q takes entries q, q +
+        // LPP, ...; all their loads are in flight together), then register
+        // the overlapping ones in list (= scan) order.
+        constexpr int RMAX = NV / LPP;
         const uint2* vrow = a.vlist + (size_t)ia * NV;
-        uint32_t last_cell = 0xffffffffu;
-        bool cell_done = false;
-        for (uint32_t r = 0; r < vcnt; r += LPP) {
-            const uint32_t k = r + g.q;
-            bool hit = false;
-            uint2 e = make_uint2(0u, 0u);
-            if (k < vcnt) {
-                e = vrow[k];
-                const double4 xb = a.xs[e.x & VL_IB_MASK];
+        uint2 ev[RMAX];
+#pragma unroll
+        for (int i = 0; i < RMAX; ++i) {
+            ev[i] = make_uint2(0u, 0u);
+            if (g.q + LPP * i < vcnt) ev[i] = vrow[g.q + LPP * i];
+        }
+        uint32_t hits = 0;  // bit i: entry g.q + LPP i overlaps
+#pragma unroll
+        for (int i = 0; i < RMAX; ++i) {
+            if (g.q + LPP * i < vcnt) {
+                const double4 xb = a.xs[ev[i].x & VL_IB_MASK];
                 const double rb = xb.w;
                 d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
-                xab += shift_of(p, word_code(e.y));
+                xab += shift_of(p, word_code(ev[i].y));
                 const double dd = dot(xab, xab);
                 const double S = ra + rb;
                 if (!(dd >= S * S * 1.0000152587890625)) {
                     const double len = sqrt32(dd);
-                    hit = len - S < 0.0;
+                    if (len - S < 0.0) hits |= 1u << i;
                 }
             }
-            // Only overlapping candidates register; the cells of the others
-            // cannot matter, as each cell's entries are contiguous.
-            uint32_t hm = g.bits(hit);
+        }
+        // Only overlapping candidates register; the cells of the others
+        // cannot matter, as each cell's entries are contiguous.  The
+        // entries are shifted down a round at a time (never indexed).
+        uint32_t last_cell = 0xffffffffu;
+        bool cell_done = false;
+        for (uint32_t r = 0; r < vcnt; r += LPP) {
+            uint32_t hm = g.bits(hits & 1u);
+            const uint2 e = ev[0];
+#pragma unroll
+            for (int i = 0; i + 1 < RMAX; ++i) ev[i] = ev[i + 1];
+            hits >>= 1;
+            if (hm && group_register_known<LPP>(g, slot, srt, e, hm, cell_done, nc, last_cell))
+                continue;
             while (hm) {
                 const uint32_t b = __ffs(hm) - 1;
                 hm &= hm - 1u;

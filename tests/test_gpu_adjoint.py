@@ -171,3 +171,19 @@ def test_unsupported_cases_are_refused():
     with pytest.raises(engine.EngineError):
         ctx.adjoint(5, numpy.ones((1000, 3)))
     ctx.close()
+
+
+@pytest.mark.gpu
+def test_overflowing_contact_row_is_refused():
+    """A particle overlapped by more spheres than a row holds: some of them
+    list it while it cannot list them, and the adjoint (which gathers each
+    contact's partner part from the mirror slot) refuses the result."""
+    s = cases.crowded_case()
+    s.contactmodel[0] = 2
+    n = int(s.np[0])
+    ctx = engine.Context(s.tobytes())
+    with pytest.raises(engine.EngineError) as e:
+        ctx.adjoint(2, None, numpy.ones((n, 3)), None)
+    assert e.value.code == -6  # SPH_E_UNSUPPORTED
+    assert "partner's row" in str(e.value)
+    ctx.close()
