@@ -1038,11 +1038,27 @@ struct LaneGroup {
     }
 };
 
-// Index of the (t+1)-th set bit of m (m has more than t bits set).
-__device__ __forceinline__ uint32_t nth_bit(uint32_t m, int t) {
-    for (int k = 0; k < t; ++k) m &= m - 1u;
-    return (uint32_t)(__ffs(m) - 1);
+// Copy of n contiguous words (n = 2, 4 or 8; both sides 8-byte aligned,
+// 16-byte aligned for n >= 4).
+template <int N>
+__device__ __forceinline__ void copy_words(uint32_t* dst, const uint32_t* src) {
+    if constexpr (N % 4 == 0) {
+#pragma unroll
+        for (int k = 0; k < N / 4; ++k)
+            reinterpret_cast<uint4*>(dst)[k] = reinterpret_cast<const uint4*>(src)[k];
+    } else {
+        static_assert(N == 2, "word count");
+        *reinterpret_cast<uint2*>(dst) = *reinterpret_cast<const uint2*>(src);
+    }
 }
+
+// Orders the wavefront's LDS accesses around it (the lanes of a group
+// exchange row words through LDS; a group never spans wavefronts).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 
 // The contact-slot policy (src/contactsearch.cuh:265-299) over the slot
 // registers of a group: the first slot holding the partner, else the last
@@ -1081,12 +1097,10 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
 }
 
 // The interaction round's contributions, added on every lane of the group in
-// slot order (group lane T evaluated the T-th slot of the round); the slots
-// whose contact ended are marked in `drop` (bit j: this lane's slot j).
+// slot order (group lane T evaluated the T-th slot of the round).
 template <int LPP, int T>
 __device__ __forceinline__ void replay_round(const LaneGroup<LPP>& g, int take, uint32_t tm,
-                                             uint32_t cm, uint32_t rem, const Contrib& C,
-                                             Acc& A, uint32_t& drop) {
+                                             const Contrib& C, Acc& A) {
     if constexpr (T < LPP) {
         if (T >= take) return;
         if ((tm >> T) & 1u) {
@@ -1102,11 +1116,7 @@ __device__ __forceinline__ void replay_round(const LaneGroup<LPP>& g, int take, 
             c.es = g.template bc<T>(C.es);
             apply(A, c);
         }
-        if ((cm >> T) & 1u) {
-            const uint32_t st = nth_bit(rem, T);
-            if (st % LPP == g.q) drop |= 1u << (st / LPP);
-        }
-        replay_round<LPP, T + 1>(g, take, tm, cm, rem, C, A, drop);
+        replay_round<LPP, T + 1>(g, take, tm, C, A);
     }
 }
 
@@ -1176,8 +1186,12 @@ __device__ __forceinline__ bool group_register_known(const LaneGroup<LPP>& g,
     }
 }
 
+// Four wavefronts per SIMD (128 VGPRs): the compiler's own allocation takes
+// 131 and three; capping it spills 12 bytes per lane and measured 51.3
+// against 54.9 us on the 1e5-particle sheared bed.
 template <int CM, int LPP>
-__global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_g(
+    const DevArrays a) {
     static_assert(CM == 2 || CM == 3, "history models only");
     constexpr int SPL = NC / LPP;       // slots per lane
     constexpr int PPB = BLOCK / LPP;    // particles per workgroup
@@ -1197,13 +1211,23 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     const double4 av_a = a.ws[ia];
     const d3 angvel_a = xyz(av_a);
     tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
+    // The row passes through LDS: read and written as contiguous words per
+    // lane, held LPP-strided in registers during the search, and back in
+    // LDS (with the partners' sorted indices) for the interaction.
+    __shared__ uint32_t s_slot[PPB * NC];
+    __shared__ uint32_t s_srt[PPB * NC];
+    uint32_t* gs = s_slot + (threadIdx.x / LPP) * NC;
+    uint32_t* gr = s_srt + (threadIdx.x / LPP) * NC;
+    uint32_t* const crow_a = a.crow + (size_t)ia_orig * NC;
+    copy_words<SPL>(gs + g.q * SPL, crow_a + g.q * SPL);
+    wave_sync();
     uint32_t slot[SPL];
     // Sorted index of each slot's partner, where this step's search found
     // it; NO_SORTED (a contact the search did not meet again) is looked up.
     uint32_t srt[SPL];
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
-        slot[j] = a.crow[(size_t)ia_orig * NC + g.q + LPP * j];
+        slot[j] = gs[g.q + LPP * j];
         srt[j] = NO_SORTED;
     }
 
@@ -1339,13 +1363,17 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
     // then added in slot order on every lane of the group.
     const bool zero_empty = lp->zero_empty != 0u;
     uint32_t occ = 0;
+    wave_sync();
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
         const bool o = (slot[j] & CW_IDX_MASK) != EMPTY;
         occ |= g.bits(o) << (LPP * j);
+        gs[g.q + LPP * j] = slot[j];
+        gr[g.q + LPP * j] = srt[j];
         if (!o && zero_empty)
             a.delta_t[(size_t)ia_orig * NC + g.q + LPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
+    wave_sync();
     Acc A;
     A.F = mk3(0.0, 0.0, 0.0);
     A.T = mk3(0.0, 0.0, 0.0);
@@ -1361,14 +1389,9 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
         const bool mine = m != 0u && (int)g.q < take;
         const uint32_t sidx = mine ? (uint32_t)(__ffs(m) - 1) : 0u;
         uint32_t wrd = 0, sib = NO_SORTED;
-#pragma unroll
-        for (int j = 0; j < SPL; ++j) {
-            const uint32_t v = g.from(slot[j], sidx % LPP);
-            const uint32_t u = g.from(srt[j], sidx % LPP);
-            if ((uint32_t)j == sidx / LPP) {
-                wrd = v;
-                sib = u;
-            }
+        if (mine) {
+            wrd = gs[sidx];
+            sib = gr[sidx];
         }
         Contrib C;
         contrib_init(C);
@@ -1407,26 +1430,19 @@ __global__ __launch_bounds__(BLOCK) void k_contact_g(const DevArrays a) {
                                        dn, dt0);
                 a.delta_t[mp] = make_double4(nd.x, nd.y, nd.z, 0.0);
             } else {
-                cleared = true;
+                // The contact ended: its slot empties (src/contactsearch.cuh:540-548).
+                gs[sidx] = EMPTY;
                 a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
             }
         }
         const uint32_t tm = g.bits(touching);
-        const uint32_t cm = g.bits(cleared);
-        uint32_t drop = 0;
-        replay_round<LPP, 0>(g, take, tm, cm, rem, C, A, drop);
-#pragma unroll
-        for (int j = 0; j < SPL; ++j)
-            if ((drop >> j) & 1u) slot[j] = EMPTY;
+        replay_round<LPP, 0>(g, take, tm, C, A);
         // Drop the round's slots from the remaining set.
         for (int t = 0; t < take; ++t) rem &= rem - 1u;
     }
-#pragma unroll
-    for (int j = 0; j < SPL; ++j) a.crow[(size_t)ia_orig * NC + g.q + LPP * j] = slot[j];
-    if (a.tape_rows) {
-#pragma unroll
-        for (int j = 0; j < SPL; ++j) a.tape_rows[(size_t)ia_orig * NC + g.q + LPP * j] = slot[j];
-    }
+    wave_sync();
+    copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
+    if (a.tape_rows) copy_words<SPL>(a.tape_rows + (size_t)ia_orig * NC + g.q * SPL, gs + g.q * SPL);
     walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q == 0);
 }
 
