@@ -105,9 +105,10 @@ def batch_images(args):
     """The batch workload: `args.batch` simulations of log-uniform size in
     500..20000 particles.  "shear" (default): direct-shear beds, each a
     settled 500-particle periodic cell (one of four seeds) tiled in x and y
-    to its size, with its own friction (0.3..0.7) and normal stress
-    (5..20 kPa); "gas": granular gases collapsing in closed boxes from a
-    lattice start (few contacts within a short run)."""
+    to its size, with its own friction (0.3..0.7), normal stress
+    (5..160 kPa), stiffness (log-uniform 1e6..1e8 N/m) and dt; "gas":
+    granular gases collapsing in closed boxes from a lattice start (few
+    contacts within a short run)."""
     from sphere_amd import engine, scenes
     sizes = scenes.batch_sizes(args.batch, seed=args.seed)
     if args.batch_kind == "gas":
@@ -123,11 +124,12 @@ def batch_images(args):
                                        settle_steps=args.settle_steps,
                                        consolidate_steps=args.consolidate_steps)
              for k in range(4)]
-    cells = scenes.batch_shear_cells(sizes, bases, seed=args.seed)
+    cells = scenes.batch_shear_cells(sizes, bases, seed=args.seed, kn_range=(1.0e6, 1.0e8))
     log("batch of sheared beds prepacked in %.1f s" % (time.time() - t0))
     return ([c.tobytes() for c in cells],
             "batched direct-shear beds (tiled settled 500-particle cells), contactmodel 2, "
-            "per-simulation mu 0.3..0.7 and normal stress 5..20 kPa")
+            "per-simulation mu 0.3..0.7, normal stress 5..160 kPa, k_n log-uniform "
+            "1e6..1e8 N/m with its own critical damping and dt")
 
 
 def batch_bench(args):
@@ -294,8 +296,11 @@ def main():
                     help="0: sort launches every step, 1: end-of-step one-workgroup "
                          "re-sort, 2: automatic (default)")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
-                    help="set an engine scalar before timing (e.g. xcd_order=0, "
-                         "graph_steps=50); repeatable")
+                    help="set an engine scalar before timing (e.g. xcd_order=1, "
+                         "graph_steps=0); repeatable")
+    ap.add_argument("--graph-steps", type=int, default=100,
+                    help="steady-state chunk replayed from a captured HIP graph "
+                         "(identical results; 0 = plain launches)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the "
                          "reference kernel for scaling HBM byte counters")
@@ -341,6 +346,7 @@ def main():
         ctx.set_scalar("contact_lanes", args.contact_lanes)
     if args.sort_mode >= 0:
         ctx.set_scalar("sort_mode", args.sort_mode)
+    ctx.set_scalar("graph_steps", args.graph_steps)
     apply_settings(ctx, args)
     n = int(ctx.scalar("np"))
     nw = int(ctx.scalar("nw"))
@@ -362,9 +368,10 @@ def main():
     sorts = int(ctx.scalar("sorts") - sorts0)
     builds = int(ctx.scalar("vl_builds") - builds0)
     log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f, "
-        "%d re-sorts, %d list rebuilds, contact lanes %d, sort mode %d)"
+        "%d re-sorts, %d list rebuilds, contact lanes %d, sort mode %d, graph steps %d)"
         % (args.steps, elapsed, value, z, z_end, sorts, builds,
-           int(ctx.scalar("contact_lanes")), int(ctx.scalar("sort_mode"))))
+           int(ctx.scalar("contact_lanes")), int(ctx.scalar("sort_mode")),
+           int(ctx.scalar("graph_steps"))))
 
     # Kernel timing pass with HIP events on the context stream.
     ctx.profile(True)
@@ -425,7 +432,8 @@ def main():
         "config": {"workload": "direct shear, contactmodel 2, settled lognormal bed",
                    "np": n, "nw": nw, "periodic": "xy", "normal_stress_Pa": 1.0e4,
                    "shear_velocity_m_s": 1.0e-3, "k_n": 1.16e7, "mu": 0.5, "dt_s": dt,
-                   "contacts_per_particle": round(zz, 3), "seed": args.seed},
+                   "contacts_per_particle": round(zz, 3), "seed": args.seed,
+                   "graph_steps": int(ctx.scalar("graph_steps"))},
         "roofline": roofline,
         "step_roofline": {"bytes_per_particle_step": round(step_bytes, 1),
                           "achieved_GBs": round(step_gbs, 2),

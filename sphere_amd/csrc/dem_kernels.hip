@@ -1450,16 +1450,29 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 // Block tree sum of the reference's `summation` kernel (one element per
 // work-item, then the halving tree, src/integration.cuh:309-341).
 
+// The halving steps of stride 128 and 64 cross wavefronts and go through
+// LDS; those of stride 32..1 stay in wavefront 0 and use shuffles.  Every
+// element receives the same additions in the same order as in the LDS tree.
+// The sum is valid on work-item 0.
 __device__ __forceinline__ double block_tree_sum(double v, double* cache) {
+    static_assert(BLOCK == 256, "tree of 256");
     double temp = 0.0;
     temp += v;
     cache[threadIdx.x] = temp;
     __syncthreads();
-    for (unsigned i = BLOCK / 2; i != 0; i /= 2) {
-        if (threadIdx.x < i) cache[threadIdx.x] += cache[threadIdx.x + i];
-        __syncthreads();
+    if (threadIdx.x < 128) cache[threadIdx.x] += cache[threadIdx.x + 128];
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x < 64) {
+        s = cache[threadIdx.x] + cache[threadIdx.x + 64];
+#pragma unroll
+        for (int i = 32; i > 0; i >>= 1) {
+            const double o = __shfl_down(s, (unsigned)i, 64);
+            if ((int)threadIdx.x < i) s += o;
+        }
     }
-    return cache[0];
+    __syncthreads();  // cache may be reused by the caller
+    return s;
 }
 
 // The serial left-to-right sum of n partials (integrateWalls'

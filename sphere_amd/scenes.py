@@ -217,11 +217,14 @@ def near_square(m):
 
 
 def batch_shear_cells(sizes, bases, seed=7, mu_range=(0.3, 0.7),
-                      stress_range=(5.0e3, 2.0e4)):
+                      stress_range=(5.0e3, 1.6e5), kn_range=None, damping_frac=0.5):
     """A batch of sheared beds for many-simulation runs: simulation k is
     base cell k % len(bases) (settled direct-shear cells, periodic in x and
     y) tiled to the nearest multiple of its size in `sizes`, with its own
-    Coulomb friction and normal stress drawn from the ranges."""
+    Coulomb friction and normal stress drawn uniformly from the ranges and,
+    with `kn_range`, its own stiffness drawn log-uniformly (k_t = k_n,
+    damping_frac of critical damping, and its own dt = 0.075
+    sqrt(m_min / k_n))."""
     rng = numpy.random.default_rng(seed)
     out = []
     for k, n in enumerate(sizes):
@@ -231,6 +234,11 @@ def batch_shear_cells(sizes, bases, seed=7, mu_range=(0.3, 0.7),
         mx, my = near_square(m)
         t = tile_xy(b, mx, my)
         mu = rng.uniform(*mu_range)
+        if kn_range is not None:
+            k_n = math.exp(rng.uniform(math.log(kn_range[0]), math.log(kn_range[1])))
+            r_min = float(numpy.min(t.radius))
+            _set_contact_params(t, k_n, damping_frac, mu, r_min)
+            t.time_dt[0] = dt_rule(r_min, float(t.rho[0]), k_n)
         t.mu_s[0] = mu
         t.mu_d[0] = mu
         t.w_sigma0 = numpy.asarray(t.w_sigma0, dtype=numpy.float64).copy()

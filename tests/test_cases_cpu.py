@@ -119,5 +119,22 @@ def test_batch_shear_cells_sizes_and_parameters():
     assert [int(c.np[0]) for c in cells] == sizes
     mus = [float(c.mu_s[0]) for c in cells]
     assert len(set(mus)) == len(mus) and all(0.3 <= m <= 0.7 for m in mus)
-    assert all(5e3 <= float(c.w_sigma0[0]) <= 2e4 for c in cells)
+    assert all(5e3 <= float(c.w_sigma0[0]) <= 1.6e5 for c in cells)
     assert float(s.w_sigma0[0]) == 1e4  # the base cell is untouched
+
+
+def test_batch_shear_cells_with_own_stiffness_stay_stable():
+    """The stiffest and the softest draw of the batch's stiffness range, each
+    with its own dt, step stably on the oracle from the tiled bed."""
+    s = cases.fixture("shear1k.bin")
+    for kn in (1.0e6, 1.0e8):
+        c = scenes.batch_shear_cells([2000], [s], seed=5, kn_range=(kn, kn),
+                                     stress_range=(1.6e5, 1.6e5))[0]
+        assert float(c.k_n[0]) == pytest.approx(kn)
+        assert float(c.time_dt[0]) == pytest.approx(scenes.dt_rule(float(c.radius.min()),
+                                                                   2600.0, kn))
+        o = oracle.Oracle(c.tobytes())
+        o.step(200)
+        x = o.get("x")[:, :3]
+        assert numpy.isfinite(x).all()
+        assert (x[:, 2] > 0.0).all() and (x[:, 2] < float(c.L[2])).all()
