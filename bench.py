@@ -348,6 +348,7 @@ def main():
         ctx.set_scalar("sort_mode", args.sort_mode)
     ctx.set_scalar("graph_steps", args.graph_steps)
     apply_settings(ctx, args)
+    graph_steps = int(ctx.scalar("graph_steps"))
     n = int(ctx.scalar("np"))
     nw = int(ctx.scalar("nw"))
     dt = ctx.scalar("dt")
@@ -370,8 +371,7 @@ def main():
     log("timed %d steps in %.4f s: %.4e particle-steps/s (contacts per particle %.2f -> %.2f, "
         "%d re-sorts, %d list rebuilds, contact lanes %d, sort mode %d, graph steps %d)"
         % (args.steps, elapsed, value, z, z_end, sorts, builds,
-           int(ctx.scalar("contact_lanes")), int(ctx.scalar("sort_mode")),
-           int(ctx.scalar("graph_steps"))))
+           int(ctx.scalar("contact_lanes")), int(ctx.scalar("sort_mode")), graph_steps))
 
     # Kernel timing pass with HIP events on the context stream.
     ctx.profile(True)
@@ -433,7 +433,7 @@ def main():
                    "np": n, "nw": nw, "periodic": "xy", "normal_stress_Pa": 1.0e4,
                    "shear_velocity_m_s": 1.0e-3, "k_n": 1.16e7, "mu": 0.5, "dt_s": dt,
                    "contacts_per_particle": round(zz, 3), "seed": args.seed,
-                   "graph_steps": int(ctx.scalar("graph_steps"))},
+                   "graph_steps": graph_steps},
         "roofline": roofline,
         "step_roofline": {"bytes_per_particle_step": round(step_bytes, 1),
                           "achieved_GBs": round(step_gbs, 2),

@@ -125,6 +125,11 @@ enum sph_scalar {
 /* Open flags. */
 #define SPH_OPEN_QUIET 0x1u      /* no stdout progress lines                     */
 #define SPH_OPEN_NOCHECK 0x2u    /* skip checkValues (src/sphere.cpp:145-345)     */
+#define SPH_OPEN_SIDECAR 0x4u    /* restart sidecars: sph_open reads "<name>.state"
+                                    next to "<name>.bin" when present, and sph_run
+                                    writes one next to every output .bin (no
+                                    counterpart in the reference, whose restarts
+                                    lose the contact history and acc0) */
 
 typedef struct sph_ctx sph_ctx;
 
@@ -202,6 +207,18 @@ int sph_get_cells(sph_ctx* ctx, uint32_t* cell_id, uint32_t* perm,
 int sph_adjoint(sph_ctx* ctx, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
                 const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
                 double* g_params);
+
+/* Restart sidecar (no counterpart in the reference: src/file_io.cpp:420-731
+ * writes none of this, so a run resumed from an output .bin starts with
+ * empty contact rows, zero displacements and acc0 = 0).  Holds the contact
+ * rows with their periodic images, the tangential displacements, the
+ * accelerations of the last step, the walls' accelerations and the loop
+ * counter, so that a run resumed from an output .bin and its sidecar is
+ * bit-identical to the uninterrupted one.  Reading checks np, nw and NC. */
+int sph_write_state(sph_ctx* ctx, const char* path);
+int sph_read_state(sph_ctx* ctx, const char* path);
+int sph_write_state_sim(sph_ctx* ctx, int sim, const char* path);
+int sph_read_state_sim(sph_ctx* ctx, int sim, const char* path);
 
 /* Per-kernel timing: mean device microseconds of the last sph_step call's
  * kernels, measured with HIP events on the context stream

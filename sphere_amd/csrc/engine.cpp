@@ -1125,6 +1125,15 @@ int write_status(sph_ctx* c, const std::string& fn) {
 }  // namespace
 
 // ===========================================================================
+namespace {
+// "<dir>/<name>.bin" -> "<dir>/<name>.state"
+std::string state_path(const std::string& bin) {
+    const size_t n = bin.size();
+    if (n >= 4 && bin.compare(n - 4, 4, ".bin") == 0) return bin.substr(0, n - 4) + ".state";
+    return bin + ".state";
+}
+}  // namespace
+
 extern "C" {
 
 int sph_open_mem(const void* bytes, size_t nbytes, int device, unsigned flags, sph_ctx** out) {
@@ -1153,7 +1162,19 @@ int sph_open(const char* bin_path, int device, unsigned flags, sph_ctx** out) {
     std::vector<unsigned char> buf;
     int rc = read_file(bin_path, buf);
     if (rc) return rc;
-    return open_images({{buf.data(), buf.size()}}, device, flags, out);
+    rc = open_images({{buf.data(), buf.size()}}, device, flags, out);
+    if (rc || !(flags & SPH_OPEN_SIDECAR)) return rc;
+    const std::string sp = state_path(bin_path);
+    FILE* fp = fopen(sp.c_str(), "rb");
+    if (!fp) return SPH_OK;  // no sidecar: the reference's fresh start
+    fclose(fp);
+    rc = sph_read_state(*out, sp.c_str());
+    if (rc) {
+        g_open_error = sph_last_error(*out);
+        sph_close(*out);
+        *out = nullptr;
+    }
+    return rc;
 }
 
 int sph_open_batch_mem(const void* const* images, const size_t* sizes, int nsims, int device,
@@ -1244,6 +1265,136 @@ int sph_write_bin_sim(sph_ctx* c, int sim, const char* path) {
 
 int sph_write_bin(sph_ctx* c, const char* path) { return sph_write_bin_sim(c, 0, path); }
 
+// Restart sidecar: the device state a .bin does not carry (src/file_io.cpp
+// writes none of it, so the reference restarts with empty contact rows,
+// zero displacements and acc0 = 0).  Little-endian, no padding:
+//   "SPHSTATE" | u32 version (1) | u32 np | u32 nw | u32 NC | i64 iter |
+//   u32 partner[np*NC] (np = empty) | u8 image[np*NC] (x code | y code << 2,
+//   codes 0: none, 1: +L, 2: -L) | f64 delta_t[np*NC*3] | f64 acc[np*3] |
+//   f64 angacc[np*3] | f64 walls_acc[nw]
+namespace {
+constexpr char STATE_MAGIC[8] = {'S', 'P', 'H', 'S', 'T', 'A', 'T', 'E'};
+constexpr uint32_t STATE_VERSION = 1;
+int set_loop_flag(sph_ctx* c, int sim, uint32_t DevLoop::*flag);
+}  // namespace
+
+int sph_write_state_sim(sph_ctx* c, int sim, const char* path) {
+    CHECK_SIM(c, sim);
+    if (!path) return SPH_E_ARG;
+    const Sim& S = c->sims[sim];
+    const size_t np = S.h.np, nw = S.h.nw, b0 = S.P.base;
+    DevLoop L;
+    int rc = read_loop(c, sim, &L);
+    std::vector<uint32_t> rows(np * NC);
+    std::vector<double> dt(np * NC * 3), acc(np * 3), angacc(np * 3), wacc(nw);
+    if (!rc) rc = d2h(c, rows.data(), c->A.crow + b0 * NC, np * NC * 4);
+    if (!rc) rc = sph_get_field_sim(c, sim, SPH_F_DELTA_T, dt.data(), dt.size() * 8);
+    if (!rc) rc = sph_get_field_sim(c, sim, SPH_F_ACC, acc.data(), acc.size() * 8);
+    if (!rc) rc = sph_get_field_sim(c, sim, SPH_F_ANGACC, angacc.data(), angacc.size() * 8);
+    if (!rc && nw) rc = sph_get_field_sim(c, sim, SPH_F_WALLS_ACC, wacc.data(), nw * 8);
+    if (rc) return rc;
+    Writer w;
+    w.b.insert(w.b.end(), STATE_MAGIC, STATE_MAGIC + 8);
+    w.put(STATE_VERSION);
+    w.put((uint32_t)np);
+    w.put((uint32_t)nw);
+    w.put((uint32_t)NC);
+    w.put((int64_t)L.iter);
+    for (uint32_t v : rows) {
+        const uint32_t idx = v & CW_IDX_MASK;
+        w.put(idx == EMPTY ? (uint32_t)np : idx - (uint32_t)b0);
+    }
+    for (uint32_t v : rows) {
+        const uint32_t cx = (v >> CW_SHIFT_X) & 3u, cy = (v >> CW_SHIFT_Y) & 3u;
+        w.b.push_back((char)(cx | (cy << 2)));
+    }
+    for (double v : dt) w.put(v);
+    for (double v : acc) w.put(v);
+    for (double v : angacc) w.put(v);
+    for (double v : wacc) w.put(v);
+    FILE* fp = fopen(path, "wb");
+    if (!fp) return fail(c, SPH_E_IO, "could not create state file '%s'", path);
+    const size_t k = fwrite(w.b.data(), 1, w.b.size(), fp);
+    fclose(fp);
+    if (k != w.b.size()) return fail(c, SPH_E_IO, "short write to '%s'", path);
+    return SPH_OK;
+}
+
+int sph_write_state(sph_ctx* c, const char* path) { return sph_write_state_sim(c, 0, path); }
+
+int sph_read_state_sim(sph_ctx* c, int sim, const char* path) {
+    CHECK_SIM(c, sim);
+    if (!path) return SPH_E_ARG;
+    Sim& S = c->sims[sim];
+    const size_t np = S.h.np, nw = S.h.nw, b0 = S.P.base;
+    FILE* fp = fopen(path, "rb");
+    if (!fp) return fail(c, SPH_E_IO, "could not open state file '%s'", path);
+    std::vector<char> buf;
+    char tmp[1 << 16];
+    size_t k;
+    while ((k = fread(tmp, 1, sizeof tmp, fp)) > 0) buf.insert(buf.end(), tmp, tmp + k);
+    fclose(fp);
+    const size_t need = 8 + 4 * 4 + 8 + np * NC * (4 + 1 + 24) + np * 48 + nw * 8;
+    if (buf.size() < 28 || memcmp(buf.data(), STATE_MAGIC, 8) != 0)
+        return fail(c, SPH_E_INPUT, "'%s' is not a state file", path);
+    uint32_t hdr[4];
+    memcpy(hdr, buf.data() + 8, 16);
+    if (hdr[0] != STATE_VERSION)
+        return fail(c, SPH_E_INPUT, "'%s': state file version %u (expected %u)", path, hdr[0],
+                    STATE_VERSION);
+    if (hdr[1] != np || hdr[2] != nw || hdr[3] != (uint32_t)NC)
+        return fail(c, SPH_E_INPUT,
+                    "'%s' holds np %u, nw %u, NC %u; the simulation has np %zu, nw %zu, NC %d",
+                    path, hdr[1], hdr[2], hdr[3], np, nw, NC);
+    if (buf.size() != need)
+        return fail(c, SPH_E_INPUT, "'%s': %zu bytes, expected %zu", path, buf.size(), need);
+    const char* q = buf.data() + 24;
+    int64_t iter;
+    memcpy(&iter, q, 8);
+    q += 8;
+    if (iter < 0) return fail(c, SPH_E_INPUT, "'%s': negative iteration count", path);
+    std::vector<uint32_t> rows(np * NC);
+    for (size_t i = 0; i < np * NC; ++i) {
+        uint32_t v;
+        memcpy(&v, q + 4 * i, 4);
+        if (v > np) return fail(c, SPH_E_INPUT, "'%s': contact index out of range", path);
+        rows[i] = v == np ? EMPTY : v + (uint32_t)b0;
+    }
+    q += np * NC * 4;
+    for (size_t i = 0; i < np * NC; ++i) {
+        const uint32_t img = (uint8_t)q[i], cx = img & 3u, cy = (img >> 2) & 3u;
+        if (cx == 3u || cy == 3u || img > 15u)
+            return fail(c, SPH_E_INPUT, "'%s': bad image code", path);
+        if (rows[i] != EMPTY) rows[i] |= (cx << CW_SHIFT_X) | (cy << CW_SHIFT_Y);
+    }
+    q += np * NC;
+    std::vector<double> dt(np * NC * 3), acc(np * 3), angacc(np * 3), wacc(nw);
+    memcpy(dt.data(), q, dt.size() * 8);
+    q += dt.size() * 8;
+    memcpy(acc.data(), q, acc.size() * 8);
+    q += acc.size() * 8;
+    memcpy(angacc.data(), q, angacc.size() * 8);
+    q += angacc.size() * 8;
+    memcpy(wacc.data(), q, nw * 8);
+    int rc = h2d(c, c->A.crow + b0 * NC, rows.data(), rows.size() * 4);
+    if (!rc) rc = sph_set_field_sim(c, sim, SPH_F_DELTA_T, dt.data(), dt.size() * 8);
+    if (!rc) rc = sph_set_field_sim(c, sim, SPH_F_ACC, acc.data(), acc.size() * 8);
+    if (!rc) rc = sph_set_field_sim(c, sim, SPH_F_ANGACC, angacc.data(), angacc.size() * 8);
+    if (!rc && nw) rc = sph_set_field_sim(c, sim, SPH_F_WALLS_ACC, wacc.data(), nw * 8);
+    if (!rc) rc = sph_set_scalar_sim(c, sim, SPH_S_ITER, (double)iter);
+    if (!rc) rc = set_loop_flag(c, sim, &DevLoop::force_rebuild);
+    return rc;
+}
+
+int sph_read_state(sph_ctx* c, const char* path) { return sph_read_state_sim(c, 0, path); }
+
+// An output .bin, and with SPH_OPEN_SIDECAR its restart sidecar.
+static int write_output(sph_ctx* c, const char* file) {
+    int rc = sph_write_bin(c, file);
+    if (rc || !(c->flags & SPH_OPEN_SIDECAR)) return rc;
+    return sph_write_state(c, state_path(file).c_str());
+}
+
 int sph_run(sph_ctx* c, const char* out_dir, const char* sid) {
     if (!c || !out_dir || !sid) return SPH_E_ARG;
     if (c->sims.size() != 1)
@@ -1258,7 +1409,7 @@ int sph_run(sph_ctx* c, const char* out_dir, const char* sid) {
     HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
     if (h.step_count == 0) {
         snprintf(file, sizeof file, "%s/%s.output00000.bin", dir.c_str(), sid);
-        if ((rc = sph_write_bin(c, file))) return rc;
+        if ((rc = write_output(c, file))) return rc;
     }
     double filetimeclock = 0.0;
     const std::string status = dir + "/" + sid + ".status.dat";
@@ -1295,7 +1446,7 @@ int sph_run(sph_ctx* c, const char* out_dir, const char* sid) {
             if ((rc = sph_sync(c))) return rc;
             h.step_count += 1;
             snprintf(file, sizeof file, "%s/%s.output%05d.bin", dir.c_str(), sid, h.step_count);
-            if ((rc = sph_write_bin(c, file))) return rc;
+            if ((rc = write_output(c, file))) return rc;
             if ((rc = write_status(c, status))) return rc;
             filetimeclock = 0.0;
         }

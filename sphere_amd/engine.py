@@ -73,7 +73,8 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_open_error", "sph_version_string", "sph_open_batch", "sph_open_batch_mem",
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
-           "sph_set_scalar_sim", "sph_adjoint", "sph_selftest_sqrt32")
+           "sph_set_scalar_sim", "sph_adjoint", "sph_selftest_sqrt32", "sph_write_state",
+           "sph_read_state", "sph_write_state_sim", "sph_read_state_sim")
 
 # Parameters of sph_adjoint's g_params, in order.
 ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
@@ -81,6 +82,7 @@ ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn"
 
 QUIET = 0x1
 NOCHECK = 0x2
+SIDECAR = 0x4
 
 
 def load():
@@ -128,6 +130,10 @@ def load():
                                      ctypes.POINTER(vp)]
     L.sph_num_sims.argtypes = [vp]
     L.sph_write_bin_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+    L.sph_write_state.argtypes = [vp, ctypes.c_char_p]
+    L.sph_read_state.argtypes = [vp, ctypes.c_char_p]
+    L.sph_write_state_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+    L.sph_read_state_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
     L.sph_write_mem_sim.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t,
                                     ctypes.POINTER(ctypes.c_size_t)]
     L.sph_get_field_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, ctypes.c_size_t]
@@ -150,10 +156,10 @@ class Context:
     the simulation index `sim` (default 0)."""
 
     def __init__(self, image=None, path=None, device=-1, quiet=True, check=True, images=None,
-                 paths=None):
+                 paths=None, sidecar=False):
         L = load()
         h = ctypes.c_void_p()
-        flags = (QUIET if quiet else 0) | (0 if check else NOCHECK)
+        flags = (QUIET if quiet else 0) | (0 if check else NOCHECK) | (SIDECAR if sidecar else 0)
         if paths is not None:
             arr = (ctypes.c_char_p * len(paths))(*[p.encode() for p in paths])
             rc = L.sph_open_batch(arr, len(paths), device, flags, ctypes.byref(h))
@@ -234,6 +240,14 @@ class Context:
     def write_bin(self, path, sim=0):
         self._chk(self._L.sph_write_bin_sim(self._h, sim, path.encode()))
 
+    def write_state(self, path, sim=0):
+        """Restart sidecar (sph_write_state): the contact rows, displacements,
+        accelerations, wall accelerations and loop counter a .bin lacks."""
+        self._chk(self._L.sph_write_state_sim(self._h, sim, path.encode()))
+
+    def read_state(self, path, sim=0):
+        self._chk(self._L.sph_read_state_sim(self._h, sim, path.encode()))
+
     def run(self, out_dir, sid):
         self._chk(self._L.sph_run(self._h, out_dir.encode(), sid.encode()))
 
@@ -274,10 +288,12 @@ class Context:
         return rep
 
 
-def run_file(path, out_dir, sid, verbose=True, dry=False, device=-1):
-    """Open a .bin and run DEM::startTime's loop; returns 0 or a negative code."""
+def run_file(path, out_dir, sid, verbose=True, dry=False, device=-1, sidecar=False):
+    """Open a .bin and run DEM::startTime's loop; returns 0 or a negative code.
+    With `sidecar`, a restart sidecar next to the input is read and one is
+    written next to every output."""
     try:
-        ctx = Context(path=path, device=device, quiet=not verbose)
+        ctx = Context(path=path, device=device, quiet=not verbose, sidecar=sidecar)
     except EngineError as e:
         print(str(e))
         return e.code

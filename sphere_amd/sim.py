@@ -17,6 +17,7 @@ path.  What differs by design:
 
 import math
 import os
+import shutil
 
 import numpy
 
@@ -825,21 +826,31 @@ class sim:
 
     # -- running -------------------------------------------------------------
     def run(self, verbose=True, hideinputfile=False, dry=False, valgrind=False,
-            cudamemcheck=False, device=-1, root=None):
+            cudamemcheck=False, device=-1, root=None, sidecar=False, restart_state=None):
         """Run the simulation (python/sphere.py:4482-4540).
 
         Writes <root>/input/<sid>.bin, then runs the DEM loop in-process on the
         GPU and writes <root>/output/<sid>.output%05d.bin + status.dat.
         Returns 0 on success; on failure prints the reference's warning and
-        returns the (negative) library status."""
+        returns the (negative) library status.  Not in the reference:
+        `sidecar` also writes a restart sidecar (.state) next to every
+        output, and `restart_state` (the sidecar of the output this sim was
+        read from) makes the run continue bit for bit where that one left
+        off."""
         from . import engine
         rt = _root(root)
         fn = self.writebin(os.path.join(rt, "input"), verbose=False)
+        state_in = fn[:-4] + ".state"
+        if restart_state is not None:
+            shutil.copyfile(restart_state, state_in)
+            sidecar = True
+        elif os.path.exists(state_in):
+            os.remove(state_in)  # a sidecar left by an earlier resumed run
         if valgrind or cudamemcheck:
             print("Warning: valgrind/cuda-memcheck wrappers are not available; "
                   "running normally")
         status_ = engine.run_file(fn, os.path.join(rt, "output"), self.sid,
-                                  verbose=verbose, dry=dry, device=device)
+                                  verbose=verbose, dry=dry, device=device, sidecar=sidecar)
         if status_ != 0:
             print("Warning: the sphere run returned with status " + str(status_))
         return status_
