@@ -287,7 +287,8 @@ def main():
     ap.add_argument("--settle-steps", type=int, default=40000)
     ap.add_argument("--consolidate-steps", type=int, default=10000)
     ap.add_argument("--profile-steps", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=1000,
+                    help="oracle steps of the CPU baseline sample (about 15 s on 16 threads)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--contact-lanes", type=int, default=0,
