@@ -1672,7 +1672,10 @@ __device__ __forceinline__ double4 integrate_particle(const DevParams& p, const 
 // next step needs no hash launch; with an adaptive grid (L[2] follows the
 // top wall, which moves after this pass) hash_next is 0.
 
-__global__ __launch_bounds__(BLOCK) void k_integrate(const DevArrays a, int hash_next) {
+// Five wavefronts per SIMD (96 VGPRs, no spills; the compiler's own choice
+// is 100 and four): more loads in flight where the grid fills the chip.
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void k_integrate(
+    const DevArrays a, int hash_next) {
     __shared__ double cache[BLOCK];
     const DevParams& p = chunk_params(a, blockIdx.x);
     DevLoop* lp = a.loop + p.sim;

@@ -111,3 +111,20 @@ def test_graph_replay_bit_exact():
             bad += _compare_sim(ctx, k, r, EXACT_FIELDS)
         assert bad == [], chunk
     ctx.close()
+
+
+def test_batch_of_sheared_beds_with_own_stiffness_and_dt():
+    """The benchmark's batch workload at small sizes: tiled sheared beds,
+    each with its own friction, normal stress, stiffness, damping and dt,
+    stepped together (with graph replay), each bit-exact with the oracle."""
+    from sphere_amd import scenes as sc
+    base = cases.fixture("shear1k.bin")
+    cells = sc.batch_shear_cells([1000, 2000, 3000, 1000, 4000, 2000], [base], seed=11,
+                                 kn_range=(1.0e6, 1.0e8))
+    assert len({float(c.time_dt[0]) for c in cells}) == len(cells)
+    ctx, refs = _run_batch(cells, 150, graph_steps=25)
+    bad = []
+    for k, r in enumerate(refs):
+        bad += _compare_sim(ctx, k, r, EXACT_FIELDS + WALL_FIELDS)
+    assert bad == []
+    ctx.close()
