@@ -249,6 +249,7 @@ def adjoint_bench(args, img):
         # The first call allocates the context's tape; the second, timed,
         # continues from the state the first one left.
         adj = engine.Context(state)
+        apply_settings(adj, args)
         adj.step(1, sync=True)
         adj.adjoint(k, None, seed_v, None)
         adj.sync()

@@ -116,10 +116,17 @@ enum sph_scalar {
                                     (0..2048, default 2048); above it, it sorts anew. */
     SPH_S_GRAPH_STEPS = 17, /* > 0: sph_step replays steady-state chunks of this many
                                steps from a captured HIP graph (default 0: launches). */
-    SPH_S_XCD_ORDER = 18    /* 1: the gathering kernels hand each XCD one contiguous
+    SPH_S_XCD_ORDER = 18,   /* 1: the gathering kernels hand each XCD one contiguous
                                range of workgroups; 0 (default): hardware order
                                (round-robin over the XCDs), measured faster on the
                                sheared bed.  Identical results. */
+    SPH_S_ADJ_CHECKPOINT = 19 /* sph_adjoint: record the forward pass in segments of
+                                 this many steps from checkpoints of the whole
+                                 context, recomputing each segment before its
+                                 reverse (memory for K steps plus one checkpoint
+                                 per segment); 0 (default): record every step,
+                                 or segments if that does not fit in memory.
+                                 Identical gradients either way. */
 };
 
 /* Open flags. */

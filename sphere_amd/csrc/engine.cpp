@@ -106,6 +106,7 @@ struct sph_ctx {
     DevArrays A;
     DevParams* d_ptab = nullptr;  // device copy of the parameter blocks
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;  // size of each allocation, in order
     hipStream_t stream = nullptr;
     int device = 0;
     unsigned flags = 0;
@@ -148,9 +149,11 @@ struct sph_ctx {
     std::vector<Graph> graphs;
     // While an adjoint forward pass records: where the contact rows after
     // the interaction pass go.
-    // The adjoint's tape and reverse state, kept between calls.
+    // The adjoint's tape and reverse state, kept between calls, and its
+    // segment length (0: automatic).
     void* adj_buf = nullptr;
     size_t adj_bytes = 0;
+    uint64_t adj_checkpoint = 0;
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -459,6 +462,7 @@ template <class T> int dev_alloc(sph_ctx* c, T** p, size_t count) {
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) return hip_fail(c, e, "hipMalloc");
     c->allocs.push_back((void*)*p);
+    c->alloc_bytes.push_back(count * sizeof(T));
     return SPH_OK;
 }
 
@@ -1712,6 +1716,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_NSIMS: *v = (double)c->sims.size(); return SPH_OK;
         case SPH_S_GRAPH_STEPS: *v = c->graph_steps; return SPH_OK;
         case SPH_S_XCD_ORDER: *v = c->A.xcd_order; return SPH_OK;
+        case SPH_S_ADJ_CHECKPOINT: *v = (double)c->adj_checkpoint; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1756,6 +1761,12 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             if (!(v >= 0.0 && v <= 4096.0))
                 return fail(c, SPH_E_ARG, "graph steps must be in [0, 4096] (got %g)", v);
             c->graph_steps = (int)v;
+            return SPH_OK;
+        case SPH_S_ADJ_CHECKPOINT:
+            if (!(v >= 0.0 && v <= 1e9) || v != (double)(uint64_t)v)
+                return fail(c, SPH_E_ARG, "checkpoint interval must be a whole number >= 0 "
+                            "(got %g)", v);
+            c->adj_checkpoint = (uint64_t)v;
             return SPH_OK;
         case SPH_S_XCD_ORDER:
             if (v != 0.0 && v != 1.0)
@@ -1836,13 +1847,40 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     // accelerations, of the displacements (two sets), and the parameters.
     const size_t state_b = up(np * 10 * sizeof(double4)) + up(2 * nc * sizeof(double4)) +
                            up(np * 3 * sizeof(double)) + 256;
-    const double need = (double)per_step * (double)nsteps + (double)state_b;
+    // A checkpoint: every device allocation of the context.
+    size_t snap_b = 0;
+    for (size_t b : c->alloc_bytes) snap_b += up(b);
+    // Segment length K: the whole run, unless set or unless the whole run's
+    // tape does not fit; then the longest segments that fit.
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(c, hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const double avail = 0.9 * (double)(free_b + c->adj_bytes);
+    auto need_for = [&](uint64_t K) {
+        const uint64_t nseg = (nsteps + K - 1) / K;
+        return (double)per_step * (double)K + (double)state_b +
+               (nseg > 1 ? (double)(nseg + 1) * (double)snap_b : 0.0);
+    };
+    uint64_t K = c->adj_checkpoint > 0 ? std::min<uint64_t>(c->adj_checkpoint, nsteps) : nsteps;
+    if (c->adj_checkpoint == 0 && need_for(K) > avail) {
+        K = 0;
+        for (uint64_t k = nsteps; k >= 1; k = k > 1 ? k * 3 / 4 : 0) {
+            if (need_for(k) <= avail) {
+                K = k;
+                break;
+            }
+            if (k == 1) break;
+        }
+        if (K == 0)
+            return fail(c, SPH_E_ARG, "the adjoint of %llu steps does not fit in device memory",
+                        (unsigned long long)nsteps);
+    }
+    const uint64_t nseg = (nsteps + K - 1) / K;
+    const double need = need_for(K);
+    if (need > avail)
+        return fail(c, SPH_E_ARG, "recording %llu steps in segments of %llu needs %.1f GB of "
+                    "device memory", (unsigned long long)nsteps, (unsigned long long)K,
+                    need / 1e9);
     if (need > (double)c->adj_bytes) {
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(c, hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-        if (need > 0.9 * (double)(free_b + c->adj_bytes))
-            return fail(c, SPH_E_ARG, "recording %llu steps needs %.1f GB of device memory",
-                        (unsigned long long)nsteps, need / 1e9);
         HIPCHK(c, hipStreamSynchronize(c->stream), "sync");
         if (c->adj_buf) (void)hipFree(c->adj_buf);
         c->adj_buf = nullptr;
@@ -1862,8 +1900,8 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         double4 *x, *v, *w, *dt0, *wnx, *wmv;
         uint32_t *perm, *srt, *rows;
     };
-    std::vector<Rec> rec(nsteps);
-    for (uint64_t k = 0; k < nsteps; ++k) {
+    std::vector<Rec> rec(K);
+    for (uint64_t k = 0; k < K; ++k) {
         char* q = tape + k * per_step;
         Rec& r = rec[k];
         r.x = reinterpret_cast<double4*>(q);
@@ -1885,7 +1923,7 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     const DevParams P = S.P;
     // Reverse-pass state, and the seeds (uploaded before the forward pass,
     // so that only the gradients' download waits for the device).
-    double4* q = reinterpret_cast<double4*>(tape + nsteps * per_step);
+    double4* q = reinterpret_cast<double4*>(tape + K * per_step);
     double4* L[2][3];
     double4 *LA[2], *LAA[2];
     for (int b = 0; b < 2; ++b) {
@@ -1904,6 +1942,7 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
     double* stage = reinterpret_cast<double*>(q);  // np x 3 host-layout staging
     g.gparam = reinterpret_cast<double*>(reinterpret_cast<char*>(q) + up(np * 3 * sizeof(double)));
     g.asym = reinterpret_cast<uint32_t*>(g.gparam + ADJ_NPARAM);
+    char* snaps = tape + K * per_step + state_b;  // (nseg + 1) checkpoints when nseg > 1
     const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
     int rc = SPH_OK;
     hipError_t e = hipSuccess;
@@ -1920,58 +1959,125 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
         }
     }
     if (rc) return rc;
-    // Forward pass: the contact pass of each step records the state and
-    // walls the step starts from, its rows and the displacements it read.
-    for (uint64_t k = 0; k < nsteps && !rc; ++k) {
-        const Rec& r = rec[k];
-        A.tape_rows = r.rows;
-        A.tape_dt0 = r.dt0;
-        A.tape_srt = r.srt;
-        A.tape_perm = r.perm;
-        A.tape_x = r.x;
-        A.tape_v = r.v;
-        A.tape_w = r.w;
-        A.tape_wnx = r.wnx;
-        A.tape_wmv = r.wmv;
-        rc = enqueue_step(c);
-        A.tape_rows = A.tape_srt = A.tape_perm = nullptr;
-        A.tape_dt0 = A.tape_x = A.tape_v = A.tape_w = A.tape_wnx = A.tape_wmv = nullptr;
-    }
-    if (rc) return rc;
-    // Reverse pass.
     e = hipMemsetAsync(LA[0], 0, 2 * np * 32, s);  // LA[0], LAA[0]
     if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8 + 8, s);
     if (e != hipSuccess) return hip_fail(c, e, "memset");
-    int cur = 0;
-    for (uint64_t kk = nsteps; kk-- > 0;) {
-        const Rec& r = rec[kk];
-        AdjStep st;
-        st.x = r.x;
-        st.v = r.v;
-        st.w = r.w;
-        st.perm = r.perm;
-        st.dt0 = r.dt0;
-        st.srt = r.srt;
-        st.rows = r.rows;
-        st.prev_rows = kk > 0 ? rec[kk - 1].rows : nullptr;
-        st.wall_nx = r.wnx;
-        st.wall_mvfd = r.wmv;
-        st.first = (iter0 + (int64_t)kk == 0) ? 1 : 0;
-        st.last = kk + 1 == nsteps ? 1 : 0;
-        for (int f = 0; f < 3; ++f) {
-            g.Lout[f] = L[cur][f];
-            g.Lin[f] = L[cur ^ 1][f];
+
+    // Checkpoints: the device allocations, and the host state enqueue_step
+    // reads (the loop mirror and the hash/re-sort bookkeeping).
+    struct HostMark {
+        double t;
+        int64_t iter;
+        bool hash_pending, prev_walls_resorted;
+    };
+    std::vector<HostMark> marks(nseg > 1 ? nseg + 1 : 0);
+    size_t crow_off = 0;
+    for (size_t k = 0, off = 0; k < c->allocs.size(); off += up(c->alloc_bytes[k]), ++k)
+        if (c->allocs[k] == (void*)A.crow) crow_off = off;
+    auto checkpoint = [&](uint64_t m, bool save) -> int {
+        char* base = snaps + m * snap_b;
+        size_t off = 0;
+        for (size_t k = 0; k < c->allocs.size(); ++k) {
+            void* dev = c->allocs[k];
+            hipError_t e_ = save ? hipMemcpyAsync(base + off, dev, c->alloc_bytes[k],
+                                                  hipMemcpyDeviceToDevice, s)
+                                 : hipMemcpyAsync(dev, base + off, c->alloc_bytes[k],
+                                                  hipMemcpyDeviceToDevice, s);
+            if (e_ != hipSuccess) return hip_fail(c, e_, "checkpoint");
+            off += up(c->alloc_bytes[k]);
         }
-        g.La_out = LA[cur];
-        g.Laa_out = LAA[cur];
-        g.La_in = LA[cur ^ 1];
-        g.Laa_in = LAA[cur ^ 1];
-        g.Ldt_in = ldt[cur];
-        g.Ldt_out = ldt[cur ^ 1];
-        e = launch_adj_step(P, st, g, P.L[2], s);
-        if (e != hipSuccess) return hip_fail(c, e, "adjoint step");
-        cur ^= 1;
+        HostMark& hm = marks[m];
+        if (save) {
+            hm = {S.h.t, S.iter, c->hash_pending, c->prev_walls_resorted};
+        } else {
+            S.h.t = hm.t;
+            S.iter = hm.iter;
+            c->hash_pending = hm.hash_pending;
+            c->prev_walls_resorted = hm.prev_walls_resorted;
+        }
+        return SPH_OK;
+    };
+    // Forward steps [k0, k0 + n) with the tape; the contact pass of each step
+    // records the state and walls the step starts from, its rows and the
+    // displacements it read.
+    auto record = [&](uint64_t n) -> int {
+        int rc_ = SPH_OK;
+        for (uint64_t k = 0; k < n && !rc_; ++k) {
+            const Rec& r = rec[k];
+            A.tape_rows = r.rows;
+            A.tape_dt0 = r.dt0;
+            A.tape_srt = r.srt;
+            A.tape_perm = r.perm;
+            A.tape_x = r.x;
+            A.tape_v = r.v;
+            A.tape_w = r.w;
+            A.tape_wnx = r.wnx;
+            A.tape_wmv = r.wmv;
+            rc_ = enqueue_step(c);
+            A.tape_rows = A.tape_srt = A.tape_perm = nullptr;
+            A.tape_dt0 = A.tape_x = A.tape_v = A.tape_w = A.tape_wnx = A.tape_wmv = nullptr;
+        }
+        return rc_;
+    };
+    // Reverse of the recorded steps [k0, k0 + n), last first.  prev0: the
+    // rows before step k0 (null for the run's first step).
+    int cur = 0;
+    auto reverse = [&](uint64_t k0, uint64_t n, const uint32_t* prev0) -> int {
+        for (uint64_t kk = n; kk-- > 0;) {
+            const Rec& r = rec[kk];
+            const uint64_t gk = k0 + kk;
+            AdjStep st;
+            st.x = r.x;
+            st.v = r.v;
+            st.w = r.w;
+            st.perm = r.perm;
+            st.dt0 = r.dt0;
+            st.srt = r.srt;
+            st.rows = r.rows;
+            st.prev_rows = kk > 0 ? rec[kk - 1].rows : prev0;
+            st.wall_nx = r.wnx;
+            st.wall_mvfd = r.wmv;
+            st.first = (iter0 + (int64_t)gk == 0) ? 1 : 0;
+            st.last = gk + 1 == nsteps ? 1 : 0;
+            for (int f = 0; f < 3; ++f) {
+                g.Lout[f] = L[cur][f];
+                g.Lin[f] = L[cur ^ 1][f];
+            }
+            g.La_out = LA[cur];
+            g.Laa_out = LAA[cur];
+            g.La_in = LA[cur ^ 1];
+            g.Laa_in = LAA[cur ^ 1];
+            g.Ldt_in = ldt[cur];
+            g.Ldt_out = ldt[cur ^ 1];
+            hipError_t e_ = launch_adj_step(P, st, g, P.L[2], s);
+            if (e_ != hipSuccess) return hip_fail(c, e_, "adjoint step");
+            cur ^= 1;
+        }
+        return SPH_OK;
+    };
+    if (nseg == 1) {
+        rc = record(nsteps);
+        if (!rc) rc = reverse(0, nsteps, nullptr);
+    } else {
+        // Checkpoint m holds the state before step m K (m = nseg: the end).
+        for (uint64_t m = 0; m < nseg && !rc; ++m) {
+            rc = checkpoint(m, true);
+            const uint64_t n = std::min<uint64_t>(K, nsteps - m * K);
+            for (uint64_t k = 0; k < n && !rc; ++k) rc = enqueue_step(c);
+        }
+        if (!rc) rc = checkpoint(nseg, true);
+        for (uint64_t m = nseg; m-- > 0 && !rc;) {
+            const uint64_t n = std::min<uint64_t>(K, nsteps - m * K);
+            rc = checkpoint(m, false);
+            if (!rc) rc = record(n);
+            // The rows before step m K: those the checkpoint's contact rows hold.
+            const uint32_t* prev0 =
+                m > 0 ? reinterpret_cast<const uint32_t*>(snaps + m * snap_b + crow_off) : nullptr;
+            if (!rc) rc = reverse(m * K, n, prev0);
+        }
+        if (!rc) rc = checkpoint(nseg, false);
     }
+    if (rc) return rc;
     double* outs[3] = {g_x0, g_v0, g_w0};
     for (int f = 0; f < 3 && !rc; ++f) {
         if (!outs[f]) continue;

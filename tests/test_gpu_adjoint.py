@@ -187,3 +187,34 @@ def test_overflowing_contact_row_is_refused():
     assert e.value.code == -6  # SPH_E_UNSUPPORTED
     assert "partner's row" in str(e.value)
     ctx.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case,steps,K", [("settling", 40, 7), ("shear1k", 30, 10)])
+def test_checkpointed_adjoint_matches_recorded(case, steps, K):
+    """Segments recomputed from whole-context checkpoints give the gradients
+    of recording every step (state cotangents bit for bit; parameter sums
+    up to the order of their workgroup atomics), and leave the context at
+    the same final state."""
+    if case == "settling":
+        s = cases.settling_case(1200, 12)
+    else:
+        s = cases.fixture("shear1k.bin")
+    image = s.tobytes()
+    n = int(s.np[0])
+    rng = numpy.random.default_rng(3)
+    seed_x, seed_v = rng.standard_normal((n, 3)), rng.standard_normal((n, 3))
+    out = []
+    for k in (0, K):
+        ctx = engine.Context(image)
+        ctx.set_scalar("adj_checkpoint", k)
+        g = ctx.adjoint(steps, seed_x, seed_v, None)
+        out.append((g, ctx.get("x"), ctx.get("vel"), ctx.get("delta_t")))
+        ctx.close()
+    (ga, xa, va, da), (gb, xb, vb, db) = out
+    for f in ("x", "v", "w"):
+        assert numpy.array_equal(ga[f], gb[f]), f
+    for name in engine.ADJOINT_PARAMS:
+        assert gb[name] == pytest.approx(ga[name], rel=1e-11, abs=1e-300), name
+    assert numpy.array_equal(xa, xb) and numpy.array_equal(va, vb)
+    assert numpy.array_equal(da, db)
