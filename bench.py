@@ -382,13 +382,17 @@ def main():
     ctx.profile(False)
     log("kernel means (us): " + ", ".join("%s %.2f" % (k, v[0]) for k, v in rep.items()))
     zz = 0.5 * (z + z_end)
-    # Algorithmic bytes of the fused contact kernel per particle: the search
-    # (36 + 128 + 36 Z) plus the interaction (84 + 128 + 72 Z + 104 + 8 nw).
-    b_contact = 480.0 + 108.0 * zz + 8.0 * nw
-    t_contact = rep.get("contact", (float("nan"), 0))[0] * 1e-6
+    # Algorithmic bytes of the contact kernel per particle: the search
+    # (36 + 128 + 36 Z) plus the interaction (84 + 128 + 72 Z + 104 + 8 nw);
+    # when it also integrates, plus the integration's 256 + 184.
+    fused = "contact_integrate" in rep
+    b_contact = 480.0 + 108.0 * zz + 8.0 * nw + (440.0 if fused else 0.0)
+    t_contact = rep.get("contact_integrate" if fused else "contact",
+                        (float("nan"), 0))[0] * 1e-6
     achieved = n * b_contact / t_contact / 1e9
     traffic = measured_traffic(zz, nw)
-    roofline = {"bound": "hbm", "kernel": "k_contact", "achieved": round(achieved, 2),
+    roofline = {"bound": "hbm", "kernel": "k_contact_g<2, 4, true>" if fused else "k_contact_g",
+                "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "algorithmic_bytes_per_launch": round(n * b_contact),
                 "mean_launch_us": round(t_contact * 1e6, 3),

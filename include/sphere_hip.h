@@ -120,13 +120,20 @@ enum sph_scalar {
                                range of workgroups; 0 (default): hardware order
                                (round-robin over the XCDs), measured faster on the
                                sheared bed.  Identical results. */
-    SPH_S_ADJ_CHECKPOINT = 19 /* sph_adjoint: record the forward pass in segments of
+    SPH_S_ADJ_CHECKPOINT = 19, /* sph_adjoint: record the forward pass in segments of
                                  this many steps from checkpoints of the whole
                                  context, recomputing each segment before its
                                  reverse (memory for K steps plus one checkpoint
                                  per segment); 0 (default): record every step,
                                  or segments if that does not fit in memory.
                                  Identical gradients either way. */
+    SPH_S_FUSED = 20          /* 1: where the step allows (fixed grid height, no
+                                 bonds, no wmode-3 wall, 4 contact lanes, model 2
+                                 or 3) the contact kernel also integrates each
+                                 particle (one vector component per lane) and a
+                                 small pass forms the wall-force partials; 0
+                                 (default, measured faster): separate integration
+                                 kernel.  Identical results. */
 };
 
 /* Open flags. */

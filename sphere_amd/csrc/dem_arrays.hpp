@@ -53,9 +53,15 @@ struct DevArrays {
     uint32_t* rhist;     // radix digit histograms, 256 x tiles
     uint32_t* cell_start;
     uint32_t* cell_end;
-    double4* xs;         // sorted copies of x, vel and angvel: the contact kernel
-    double4* vs;         // reads its own and its partners' state in sorted order,
-    double4* ws;         // which keeps those gathers local
+    // Sorted copies of x, vel and angvel: the contact kernel reads its own
+    // and its partners' state in sorted order, which keeps those gathers
+    // local.  Two sets, by loop-counter parity: step k reads set k & 1 and
+    // whatever writes the state for step k + 1 (the integration, a re-sort
+    // at the end of step k) writes set (k + 1) & 1, so that a kernel may
+    // integrate particles while others still read their step-k state.
+    double4* xs[2];
+    double4* vs[2];
+    double4* ws[2];
     uint32_t* rank;      // original index -> sorted index (inverse of perm)
     uint32_t* movers;    // per simulation, MV_CAP indices whose key changed (unordered)
     // The order a step used, kept when the step's end already re-sorted for

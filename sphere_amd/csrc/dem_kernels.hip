@@ -52,6 +52,11 @@ __device__ __forceinline__ bool sort_needed(const DevGlobal* g, const DevLoop* l
     return g->changed[step_slot(lp0)] != 0u;
 }
 
+// The set of sorted copies the current step reads (loop-counter parity).
+__device__ __forceinline__ uint32_t cur_set(const DevLoop* lp) {
+    return (uint32_t)(lp->iter & 1);
+}
+
 // The simulation that owns a 256-particle chunk.
 __device__ __forceinline__ const DevParams& chunk_params(const DevArrays& a, uint32_t chunk) {
     return a.ptab[a.chunk_sim[chunk]];
@@ -315,7 +320,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(const DevGlobal* __restrict_
 // global (cell_base + the simulation's own key); only keys of real cells
 // get a range (a key beyond the grid, or of a padding particle, does not).
 __device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a, uint32_t i,
-                                          int sorted_bits) {
+                                          int sorted_bits, uint32_t par) {
     const uint32_t key = a.skey[i];
     if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
         atomicOr(&a.loop[p.sim].error_bits, (uint32_t)DFLAG_KEY_RANGE);
@@ -329,9 +334,9 @@ __device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a
     }
     if (i == p.base + p.np - 1 && lk < p.ncells) a.cell_end[key] = i + 1;
     const uint32_t o = a.perm[i];
-    a.xs[i] = a.x[o];
-    a.vs[i] = a.vel[o];
-    a.ws[i] = a.angvel[o];
+    a.xs[par][i] = a.x[o];
+    a.vs[par][i] = a.vel[o];
+    a.ws[par][i] = a.angvel[o];
     a.rank[o] = i;
     a.key_prev[o] = key;
 }
@@ -345,7 +350,7 @@ __global__ __launch_bounds__(BLOCK) void k_cells(const DevArrays a, int sorted_b
     const DevParams& p = chunk_params(a, blockIdx.x);
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i == p.base) atomicAdd(&a.loop[p.sim].sorts, 1u);
-    cells_one(p, a, i, sorted_bits);
+    cells_one(p, a, i, sorted_bits, cur_set(a.loop + p.sim));
 }
 
 // ---------------------------------------------------------------------------
@@ -777,11 +782,15 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
     uint32_t* row = srow + threadIdx.x * ROW_STRIDE;
     DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
-    const double4 xa = a.xs[ia];
+    const uint32_t cs = cur_set(lp);
+    const double4* __restrict__ xs = a.xs[cs];
+    const double4* __restrict__ vs = a.vs[cs];
+    const double4* __restrict__ ws = a.ws[cs];
+    const double4 xa = xs[ia];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
-    const double4 vel_a = a.vs[ia];
-    const double4 av_a = a.ws[ia];
+    const double4 vel_a = vs[ia];
+    const double4 av_a = ws[ia];
     const d3 angvel_a = xyz(av_a);
     tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, true);
     Acc A;
@@ -831,7 +840,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                         bool cell_done = false;
                         for (uint32_t ib = start; ib < end; ++ib) {
                             if (ib == ia) continue;
-                            const double4 xb = a.xs[ib];
+                            const double4 xb = xs[ib];
                             const double rb = xb.w;
                             d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
                             xab += dm;
@@ -873,7 +882,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     cell_done = false;
                 }
                 if (cell_done) continue;
-                const double4 xb = a.xs[e.x & VL_IB_MASK];
+                const double4 xb = xs[e.x & VL_IB_MASK];
                 const double rb = xb.w;
                 const uint32_t code = word_code(e.y);
                 d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
@@ -895,14 +904,14 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
             if (ib_orig != EMPTY) {
                 const d3 dm = shift_of(p, word_code(wrd));
                 const uint32_t rb_s = a.rank[ib_orig];
-                const double4 xb = a.xs[rb_s];
+                const double4 xb = xs[rb_s];
                 const double rb = xb.w;
                 const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
                 const double len = length(xab);
                 const double dn = len - (ra + rb);
                 if (dn < 0.0) {
-                    const double4 vel_b = a.vs[rb_s];
-                    const d3 angvel_b = xyz(a.ws[rb_s]);
+                    const double4 vel_b = vs[rb_s];
+                    const d3 angvel_b = xyz(ws[rb_s]);
                     const double4 dt0 = a.delta_t[mp];
                     if (a.tape_dt0) {
                         a.tape_dt0[mp] = dt0;
@@ -957,7 +966,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     const d3 dm = shift_of(p, (uint32_t)code);
                     for (uint32_t ib = start; ib < end; ++ib) {
                         if (ib == ia) continue;
-                        const double4 xb = a.xs[ib];
+                        const double4 xb = xs[ib];
                         const double rb = xb.w;
                         const double kappa = p.kappa;
                         d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
@@ -965,8 +974,8 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                         const double len = length(xab);
                         const double delta = len - (ra + rb);
                         if (delta < 0.0) {
-                            contact_linear_viscous(p, A, vel_a, angvel_a, a.vs[ib],
-                                                   xyz(a.ws[ib]), ra, rb, xab, len,
+                            contact_linear_viscous(p, A, vel_a, angvel_a, vs[ib],
+                                                   xyz(ws[ib]), ra, rb, xab, len,
                                                    delta, kappa);
                         } else if (delta < p.db) {
                             capillary_cohesion_exp(p, A, ra, rb, delta, xab, len, kappa);
@@ -975,6 +984,237 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                 }
     }
     walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, true);
+}
+
+// ---------------------------------------------------------------------------
+// integrate (TY3), src/integration.cuh:14-305, one work-item per particle in
+// original order (the reference's sorted-order pass reads copies of the
+// same values; every output is per particle).
+
+// One particle's step from its force, torque and state (x, vel, angvel
+// carry radius and fixvel in w); writes the new state at its original index
+// and the next step's sorted copies at sorted index `r`.
+__device__ __forceinline__ double4 integrate_particle(const DevParams& p, const DevArrays& a,
+                                                      uint32_t orig, const double4 force,
+                                                      const double4 torque, const double4 x,
+                                                      const double4 vel, const double4 angvel,
+                                                      uint32_t r) {
+    const DevLoop* lp = a.loop + p.sim;
+    const double t_now = lp->t;
+    const int64_t iter = lp->iter;
+    // Rate-state switch decided by the host loop before integrate
+    // (src/device.cu:2454-2462).
+    int cvs = 0;
+    if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) cvs = 1;
+    else if (t_now >= 15.0 && lp->velocity_state == 2) cvs = -1;
+    const double Lz = grid_Lz(p, a);
+    const double4 angpos = a.angpos[orig];
+    double4 xyzsum = a.xyzsum[orig];
+    const int wall0mode = p.nw > 0 ? p.wmode[0] : 0;
+    const double wall0mass = p.nw > 0 ? a.w_mvfd[p.wall_base].x : 0.0;
+    double tau_eff_x = 0.0;
+    if (p.nw > 0 && wall0mode == 3) tau_eff_x = a.tau_eff[p.sim];
+    double4 acc0, angacc0;
+    if (iter == 0) {
+        acc0 = make_double4(0.0, 0.0, 0.0, 0.0);
+        angacc0 = make_double4(0.0, 0.0, 0.0, 0.0);
+    } else {
+        acc0 = a.acc[orig];
+        angacc0 = a.angacc[orig];
+    }
+    const double radius = x.w;
+    const double dt = p.dt;
+    const double m = 4.0 / 3.0 * PI * radius * radius * radius * p.rho;
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    acc.x = force.x / m + p.g[0];
+    acc.y = force.y / m + p.g[1];
+    acc.z = force.z / m + p.g[2];
+    double4 angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    angacc.x = torque.x * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    angacc.y = torque.y * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    angacc.z = torque.z * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    if (wall0mode == 3 && vel.w > 0.0001 && x.z > Lz * 0.5) {
+        const double f_tau_x = (p.tau_x + tau_eff_x) * p.L[0] * p.L[1];
+        acc.x = f_tau_x / wall0mass;
+        acc.y = 0.0;
+        acc.z -= p.g[2];
+        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    } else if (vel.w > 0.0001 && vel.w < 10.0) {
+        acc.x = 0.0;
+        acc.y = 0.0;
+        acc.z -= p.g[2];
+        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    if (vel.w >= 10.0) {
+        acc.y = 0.0;
+        angacc.x = 0.0;
+        angacc.z = 0.0;
+    }
+    if (vel.w < -0.0001) {
+        acc.x = 0.0;
+        acc.y = 0.0;
+        acc.z = 0.0;
+        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
+    }
+    const d3 dacc_dt = mk3((acc.x - acc0.x) / dt, (acc.y - acc0.y) / dt, (acc.z - acc0.z) / dt);
+    const d3 dangacc_dt = mk3((angacc.x - angacc0.x) / dt, (angacc.y - angacc0.y) / dt,
+                              (angacc.z - angacc0.z) / dt);
+    double4 x_new, vel_new, angpos_new, angvel_new;
+    x_new.x = x.x + vel.x * dt + 0.5 * acc.x * dt * dt + 1.0 / 6.0 * dacc_dt.x * dt * dt * dt;
+    x_new.y = x.y + vel.y * dt + 0.5 * acc.y * dt * dt + 1.0 / 6.0 * dacc_dt.y * dt * dt * dt;
+    x_new.z = x.z + vel.z * dt + 0.5 * acc.z * dt * dt + 1.0 / 6.0 * dacc_dt.z * dt * dt * dt;
+    x_new.w = x.w;
+    vel_new.x = vel.x + acc.x * dt + 0.5 * dacc_dt.x * dt * dt;
+    vel_new.y = vel.y + acc.y * dt + 0.5 * dacc_dt.y * dt * dt;
+    vel_new.z = vel.z + acc.z * dt + 0.5 * dacc_dt.z * dt * dt;
+    vel_new.w = vel.w;
+    angpos_new.x = angpos.x + angvel.x * dt + 0.5 * angacc.x * dt * dt +
+                   1.0 / 6.0 * dangacc_dt.x * dt * dt * dt;
+    angpos_new.y = angpos.y + angvel.y * dt + 0.5 * angacc.y * dt * dt +
+                   1.0 / 6.0 * dangacc_dt.y * dt * dt * dt;
+    angpos_new.z = angpos.z + angvel.z * dt + 0.5 * angacc.z * dt * dt +
+                   1.0 / 6.0 * dangacc_dt.z * dt * dt * dt;
+    angpos_new.w = angpos.w;
+    angvel_new.x = angvel.x + angacc.x * dt + 0.5 * dangacc_dt.x * dt * dt;
+    angvel_new.y = angvel.y + angacc.y * dt + 0.5 * dangacc_dt.y * dt * dt;
+    angvel_new.z = angvel.z + angacc.z * dt + 0.5 * dangacc_dt.z * dt * dt;
+    angvel_new.w = angvel.w;
+    xyzsum.x += vel.x * dt + 0.5 * acc.x * dt * dt + 1.0 / 6.0 * dacc_dt.x * dt * dt * dt;
+    xyzsum.y += vel.y * dt + 0.5 * acc.y * dt * dt + 1.0 / 6.0 * dacc_dt.y * dt * dt * dt;
+    xyzsum.z += vel.z * dt + 0.5 * acc.z * dt * dt + 1.0 / 6.0 * dacc_dt.z * dt * dt * dt;
+    // Periodic wrap, src/integration.cuh:267-283 (note `> L`, P17).
+    if (p.periodic == 1) {
+        if (x_new.x < p.origo[0]) x_new.x += p.L[0];
+        if (x_new.x > p.L[0]) x_new.x -= p.L[0];
+        if (x_new.y < p.origo[1]) x_new.y += p.L[1];
+        if (x_new.y > p.L[1]) x_new.y -= p.L[1];
+    } else if (p.periodic == 2) {
+        if (x_new.x < p.origo[0]) x_new.x += p.L[0];
+        if (x_new.x > p.L[0]) x_new.x -= p.L[0];
+    }
+    if (vel.w > 5.0 && vel.w < 10.0) {
+        if (cvs == 1) vel_new.x *= 10.0;
+        else if (cvs == -1) vel_new.x /= 10.0;
+    }
+    a.xyzsum[orig] = xyzsum;
+    a.acc[orig] = acc;
+    a.angacc[orig] = angacc;
+    a.angvel[orig] = angvel_new;
+    a.vel[orig] = vel_new;
+    a.angpos[orig] = angpos_new;
+    a.x[orig] = x_new;
+    // Keep the sorted copies current: valid for the next step whenever the
+    // order does not change (otherwise the next sort rebuilds them).
+    const uint32_t nxt = cur_set(lp) ^ 1u;
+    a.xs[nxt][r] = x_new;
+    a.vs[nxt][r] = vel_new;
+    a.ws[nxt][r] = angvel_new;
+    return x_new;
+}
+
+// integrate_particle for the four lanes of a contact group, one vector
+// component per lane (lane q: x, y, z; lane 3 carries the w components of
+// the next step's sorted copies): the same expressions per component, the
+// reads and writes of the particle's vectors spread over the group.  For a
+// fixed grid height without a wmode-3 wall (the fused pass).  Returns the
+// new position's component q (w: the radius).
+__device__ __forceinline__ double integrate_component(const DevParams& p, const DevArrays& a,
+                                                      const DevLoop* lp, uint32_t orig, uint32_t q,
+                                                      double force_q, double torque_q,
+                                                      const double4 x, const double4 vel,
+                                                      const double4 angvel, uint32_t r) {
+    const uint32_t nxt = cur_set(lp) ^ 1u;
+    if (q == 3) {
+        a.xs[nxt][r].w = x.w;
+        a.vs[nxt][r].w = vel.w;
+        a.ws[nxt][r].w = angvel.w;
+        return x.w;
+    }
+    const double t_now = lp->t;
+    const int64_t iter = lp->iter;
+    int cvs = 0;
+    if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) cvs = 1;
+    else if (t_now >= 15.0 && lp->velocity_state == 2) cvs = -1;
+    const double xq = q == 0 ? x.x : q == 1 ? x.y : x.z;
+    const double vq = q == 0 ? vel.x : q == 1 ? vel.y : vel.z;
+    const double wq = q == 0 ? angvel.x : q == 1 ? angvel.y : angvel.z;
+    double* const angpos_c = reinterpret_cast<double*>(a.angpos + orig) + q;
+    double* const xyzsum_c = reinterpret_cast<double*>(a.xyzsum + orig) + q;
+    double* const acc_c = reinterpret_cast<double*>(a.acc + orig) + q;
+    double* const angacc_c = reinterpret_cast<double*>(a.angacc + orig) + q;
+    const double angpos_q = *angpos_c;
+    double xyzsum_q = *xyzsum_c;
+    double acc0 = 0.0, angacc0 = 0.0;
+    if (iter != 0) {
+        acc0 = *acc_c;
+        angacc0 = *angacc_c;
+    }
+    const double radius = x.w;
+    const double dt = p.dt;
+    const double m = 4.0 / 3.0 * PI * radius * radius * radius * p.rho;
+    double acc = force_q / m + p.g[q];
+    double angacc = torque_q * 1.0 / (2.0 / 5.0 * m * radius * radius);
+    if (vel.w > 0.0001 && vel.w < 10.0) {
+        if (q < 2) acc = 0.0;
+        else acc -= p.g[2];
+        angacc = 0.0;
+    }
+    if (vel.w >= 10.0) {
+        if (q == 1) acc = 0.0;
+        if (q != 1) angacc = 0.0;
+    }
+    if (vel.w < -0.0001) {
+        acc = 0.0;
+        angacc = 0.0;
+    }
+    const double dacc_dt = (acc - acc0) / dt;
+    const double dangacc_dt = (angacc - angacc0) / dt;
+    double x_new = xq + vq * dt + 0.5 * acc * dt * dt + 1.0 / 6.0 * dacc_dt * dt * dt * dt;
+    double vel_new = vq + acc * dt + 0.5 * dacc_dt * dt * dt;
+    const double angpos_new =
+        angpos_q + wq * dt + 0.5 * angacc * dt * dt + 1.0 / 6.0 * dangacc_dt * dt * dt * dt;
+    const double angvel_new = wq + angacc * dt + 0.5 * dangacc_dt * dt * dt;
+    xyzsum_q += vq * dt + 0.5 * acc * dt * dt + 1.0 / 6.0 * dacc_dt * dt * dt * dt;
+    // Periodic wrap, src/integration.cuh:267-283 (note `> L`, P17).
+    if ((p.periodic == 1 && q < 2) || (p.periodic == 2 && q == 0)) {
+        if (x_new < p.origo[q]) x_new += p.L[q];
+        if (x_new > p.L[q]) x_new -= p.L[q];
+    }
+    if (q == 0 && vel.w > 5.0 && vel.w < 10.0) {
+        if (cvs == 1) vel_new *= 10.0;
+        else if (cvs == -1) vel_new /= 10.0;
+    }
+    *xyzsum_c = xyzsum_q;
+    *acc_c = acc;
+    *angacc_c = angacc;
+    reinterpret_cast<double*>(a.angvel + orig)[q] = angvel_new;
+    reinterpret_cast<double*>(a.vel + orig)[q] = vel_new;
+    *angpos_c = angpos_new;
+    reinterpret_cast<double*>(a.x + orig)[q] = x_new;
+    reinterpret_cast<double*>(a.xs[nxt] + r)[q] = x_new;
+    reinterpret_cast<double*>(a.vs[nxt] + r)[q] = vel_new;
+    reinterpret_cast<double*>(a.ws[nxt] + r)[q] = angvel_new;
+    return x_new;
+}
+
+// The next step's hash flags of a wavefront, and the particles that changed
+// cell listed for the end-of-step re-sort (one atomic per wavefront; the
+// list is unordered, the count keeps growing past MV_CAP so that an
+// overfull list is recognised).  Every lane of a wavefront that has any
+// particle left calls it (f is empty for lanes without one).
+__device__ __forceinline__ void publish_next(const DevParams& p, const DevArrays& a, DevLoop* lp,
+                                             const HashFlags& f, uint32_t orig) {
+    publish_hash(lp, a.glob, step_slot(lp) ^ 1u, f);
+    const unsigned long long mv = __ballot(f.changed);
+    if (mv) {
+        const uint32_t lane = threadIdx.x & 63u;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&lp->nmovers, (uint32_t)__popcll(mv));
+        base = (uint32_t)__shfl((int)base, 0, 64);
+        const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const uint32_t pos = base + (uint32_t)__popcll(mv & lt);
+        if (f.changed && pos < MV_CAP) a.movers[(size_t)p.sim * MV_CAP + pos] = orig;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1189,7 +1429,7 @@ __device__ __forceinline__ bool group_register_known(const LaneGroup<LPP>& g,
 // Four wavefronts per SIMD (128 VGPRs): the compiler's own allocation takes
 // 131 and three; capping it spills 12 bytes per lane and measured 51.3
 // against 54.9 us on the 1e5-particle sheared bed.
-template <int CM, int LPP>
+template <int CM, int LPP, bool FUSED>
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_g(
     const DevArrays a) {
     static_assert(CM == 2 || CM == 3, "history models only");
@@ -1204,11 +1444,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (ia - p.base >= p.np) return;  // whole groups leave together
     DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
-    const double4 xa = a.xs[ia];
+    const uint32_t cs = cur_set(lp);
+    const double4* __restrict__ xs = a.xs[cs];
+    const double4* __restrict__ vs = a.vs[cs];
+    const double4* __restrict__ ws = a.ws[cs];
+    const double4 xa = xs[ia];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
-    const double4 vel_a = a.vs[ia];
-    const double4 av_a = a.ws[ia];
+    const double4 vel_a = vs[ia];
+    const double4 av_a = ws[ia];
     const d3 angvel_a = xyz(av_a);
     tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
     // The row passes through LDS: read and written as contiguous words per
@@ -1264,7 +1508,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                         bool hit = false, near = false;
                         uint32_t word = 0;
                         if (ib < end && ib != ia) {
-                            const double4 xb = a.xs[ib];
+                            const double4 xb = xs[ib];
                             const double rb = xb.w;
                             d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
                             xab += dm;
@@ -1318,7 +1562,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 #pragma unroll
         for (int i = 0; i < RMAX; ++i) {
             if (g.q + LPP * i < vcnt) {
-                const double4 xb = a.xs[ev[i].x & VL_IB_MASK];
+                const double4 xb = xs[ev[i].x & VL_IB_MASK];
                 const double rb = xb.w;
                 d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
                 xab += shift_of(p, word_code(ev[i].y));
@@ -1407,15 +1651,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             const uint32_t ib_orig = wrd & CW_IDX_MASK;
             const d3 dm = shift_of(p, word_code(wrd));
             const uint32_t rb_s = sib != NO_SORTED ? sib : a.rank[ib_orig];
-            const double4 xb = a.xs[rb_s];
+            const double4 xb = xs[rb_s];
             const double rb = xb.w;
             const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
             const double len = length(xab);
             const double dn = len - (ra + rb);
             if (dn < 0.0) {
                 touching = true;
-                const double4 vel_b = a.vs[rb_s];
-                const d3 angvel_b = xyz(a.ws[rb_s]);
+                const double4 vel_b = vs[rb_s];
+                const d3 angvel_b = xyz(ws[rb_s]);
                 const double4 dt0 = a.delta_t[mp];
                 if (a.tape_dt0) {
                     a.tape_dt0[mp] = dt0;
@@ -1444,6 +1688,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
     if (a.tape_rows) copy_words<SPL>(a.tape_rows + (size_t)ia_orig * NC + g.q * SPL, gs + g.q * SPL);
     walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q == 0);
+    if constexpr (FUSED) {
+        // The particle's force and torque are final (its own row and the
+        // walls): integrate it now (src/integration.cuh:14-305), from the
+        // step's sorted copies, into the next step's set, and hash it for
+        // the next step.  (Fixed grid height, no bonds, no wmode-3 wall.)
+        static_assert(!FUSED || LPP == 4, "one vector component per lane");
+        const double fq = g.q == 0 ? A.F.x : g.q == 1 ? A.F.y : A.F.z;
+        const double tq = g.q == 0 ? A.T.x : g.q == 1 ? A.T.y : A.T.z;
+        const double xn = integrate_component(p, a, lp, ia_orig, g.q, fq, tq, xa, vel_a, av_a, ia);
+        const double4 x_new = make_double4(g.template bc<0>(xn), g.template bc<1>(xn),
+                                           g.template bc<2>(xn), g.template bc<3>(xn));
+        HashFlags f;
+        if (g.q == 0) hash_particle(p, a, ia_orig, p.L[2], x_new, f);
+        publish_next(p, a, lp, f, ia_orig);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1539,131 +1798,6 @@ __global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevArrays a) {
 }
 
 // ---------------------------------------------------------------------------
-// integrate (TY3), src/integration.cuh:14-305, one work-item per particle in
-// original order (the reference's sorted-order pass reads copies of the
-// same values; every output is per particle).
-
-__device__ __forceinline__ double4 integrate_particle(const DevParams& p, const DevArrays& a,
-                                                      uint32_t orig) {
-    const DevLoop* lp = a.loop + p.sim;
-    const double t_now = lp->t;
-    const int64_t iter = lp->iter;
-    // Rate-state switch decided by the host loop before integrate
-    // (src/device.cu:2454-2462).
-    int cvs = 0;
-    if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) cvs = 1;
-    else if (t_now >= 15.0 && lp->velocity_state == 2) cvs = -1;
-    const double Lz = grid_Lz(p, a);
-    const double4 force = a.force[orig];
-    const double4 torque = a.torque[orig];
-    const double4 angpos = a.angpos[orig];
-    const double4 x = a.x[orig];
-    const double4 vel = a.vel[orig];
-    const double4 angvel = a.angvel[orig];
-    double4 xyzsum = a.xyzsum[orig];
-    const int wall0mode = p.nw > 0 ? p.wmode[0] : 0;
-    const double wall0mass = p.nw > 0 ? a.w_mvfd[p.wall_base].x : 0.0;
-    double tau_eff_x = 0.0;
-    if (p.nw > 0 && wall0mode == 3) tau_eff_x = a.tau_eff[p.sim];
-    double4 acc0, angacc0;
-    if (iter == 0) {
-        acc0 = make_double4(0.0, 0.0, 0.0, 0.0);
-        angacc0 = make_double4(0.0, 0.0, 0.0, 0.0);
-    } else {
-        acc0 = a.acc[orig];
-        angacc0 = a.angacc[orig];
-    }
-    const double radius = x.w;
-    const double dt = p.dt;
-    const double m = 4.0 / 3.0 * PI * radius * radius * radius * p.rho;
-    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-    acc.x = force.x / m + p.g[0];
-    acc.y = force.y / m + p.g[1];
-    acc.z = force.z / m + p.g[2];
-    double4 angacc = make_double4(0.0, 0.0, 0.0, 0.0);
-    angacc.x = torque.x * 1.0 / (2.0 / 5.0 * m * radius * radius);
-    angacc.y = torque.y * 1.0 / (2.0 / 5.0 * m * radius * radius);
-    angacc.z = torque.z * 1.0 / (2.0 / 5.0 * m * radius * radius);
-    if (wall0mode == 3 && vel.w > 0.0001 && x.z > Lz * 0.5) {
-        const double f_tau_x = (p.tau_x + tau_eff_x) * p.L[0] * p.L[1];
-        acc.x = f_tau_x / wall0mass;
-        acc.y = 0.0;
-        acc.z -= p.g[2];
-        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
-    } else if (vel.w > 0.0001 && vel.w < 10.0) {
-        acc.x = 0.0;
-        acc.y = 0.0;
-        acc.z -= p.g[2];
-        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
-    }
-    if (vel.w >= 10.0) {
-        acc.y = 0.0;
-        angacc.x = 0.0;
-        angacc.z = 0.0;
-    }
-    if (vel.w < -0.0001) {
-        acc.x = 0.0;
-        acc.y = 0.0;
-        acc.z = 0.0;
-        angacc = make_double4(0.0, 0.0, 0.0, 0.0);
-    }
-    const d3 dacc_dt = mk3((acc.x - acc0.x) / dt, (acc.y - acc0.y) / dt, (acc.z - acc0.z) / dt);
-    const d3 dangacc_dt = mk3((angacc.x - angacc0.x) / dt, (angacc.y - angacc0.y) / dt,
-                              (angacc.z - angacc0.z) / dt);
-    double4 x_new, vel_new, angpos_new, angvel_new;
-    x_new.x = x.x + vel.x * dt + 0.5 * acc.x * dt * dt + 1.0 / 6.0 * dacc_dt.x * dt * dt * dt;
-    x_new.y = x.y + vel.y * dt + 0.5 * acc.y * dt * dt + 1.0 / 6.0 * dacc_dt.y * dt * dt * dt;
-    x_new.z = x.z + vel.z * dt + 0.5 * acc.z * dt * dt + 1.0 / 6.0 * dacc_dt.z * dt * dt * dt;
-    x_new.w = x.w;
-    vel_new.x = vel.x + acc.x * dt + 0.5 * dacc_dt.x * dt * dt;
-    vel_new.y = vel.y + acc.y * dt + 0.5 * dacc_dt.y * dt * dt;
-    vel_new.z = vel.z + acc.z * dt + 0.5 * dacc_dt.z * dt * dt;
-    vel_new.w = vel.w;
-    angpos_new.x = angpos.x + angvel.x * dt + 0.5 * angacc.x * dt * dt +
-                   1.0 / 6.0 * dangacc_dt.x * dt * dt * dt;
-    angpos_new.y = angpos.y + angvel.y * dt + 0.5 * angacc.y * dt * dt +
-                   1.0 / 6.0 * dangacc_dt.y * dt * dt * dt;
-    angpos_new.z = angpos.z + angvel.z * dt + 0.5 * angacc.z * dt * dt +
-                   1.0 / 6.0 * dangacc_dt.z * dt * dt * dt;
-    angpos_new.w = angpos.w;
-    angvel_new.x = angvel.x + angacc.x * dt + 0.5 * dangacc_dt.x * dt * dt;
-    angvel_new.y = angvel.y + angacc.y * dt + 0.5 * dangacc_dt.y * dt * dt;
-    angvel_new.z = angvel.z + angacc.z * dt + 0.5 * dangacc_dt.z * dt * dt;
-    angvel_new.w = angvel.w;
-    xyzsum.x += vel.x * dt + 0.5 * acc.x * dt * dt + 1.0 / 6.0 * dacc_dt.x * dt * dt * dt;
-    xyzsum.y += vel.y * dt + 0.5 * acc.y * dt * dt + 1.0 / 6.0 * dacc_dt.y * dt * dt * dt;
-    xyzsum.z += vel.z * dt + 0.5 * acc.z * dt * dt + 1.0 / 6.0 * dacc_dt.z * dt * dt * dt;
-    // Periodic wrap, src/integration.cuh:267-283 (note `> L`, P17).
-    if (p.periodic == 1) {
-        if (x_new.x < p.origo[0]) x_new.x += p.L[0];
-        if (x_new.x > p.L[0]) x_new.x -= p.L[0];
-        if (x_new.y < p.origo[1]) x_new.y += p.L[1];
-        if (x_new.y > p.L[1]) x_new.y -= p.L[1];
-    } else if (p.periodic == 2) {
-        if (x_new.x < p.origo[0]) x_new.x += p.L[0];
-        if (x_new.x > p.L[0]) x_new.x -= p.L[0];
-    }
-    if (vel.w > 5.0 && vel.w < 10.0) {
-        if (cvs == 1) vel_new.x *= 10.0;
-        else if (cvs == -1) vel_new.x /= 10.0;
-    }
-    a.xyzsum[orig] = xyzsum;
-    a.acc[orig] = acc;
-    a.angacc[orig] = angacc;
-    a.angvel[orig] = angvel_new;
-    a.vel[orig] = vel_new;
-    a.angpos[orig] = angpos_new;
-    a.x[orig] = x_new;
-    // Keep the sorted copies current: valid for the next step whenever the
-    // order does not change (otherwise the next sort rebuilds them).
-    const uint32_t r = a.rank[orig];
-    a.xs[r] = x_new;
-    a.vs[r] = vel_new;
-    a.ws[r] = angvel_new;
-    return x_new;
-}
-
-// ---------------------------------------------------------------------------
 // integrate + the next step's hash + the wall force block partials.  The
 // grid is the summation kernel's (256 work-items per block over np, so the
 // partials are the reference's, src/integration.cuh:309-341).  With a fixed
@@ -1683,30 +1817,30 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
     const bool in = idx - p.base < p.np;
     HashFlags f;
     if (in) {
-        const double4 x_new = integrate_particle(p, a, idx);
+        const double4 x_new = integrate_particle(p, a, idx, a.force[idx], a.torque[idx], a.x[idx],
+                                                 a.vel[idx], a.angvel[idx], a.rank[idx]);
         if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
     }
-    if (hash_next) {
-        publish_hash(lp, a.glob, step_slot(lp) ^ 1u, f);
-        // List the particles that changed cell for the end-of-step re-sort
-        // (one atomic per wavefront; the list is unordered, the count keeps
-        // growing past MV_CAP so that an overfull list is recognised).
-        const unsigned long long mv = __ballot(f.changed);
-        if (mv) {
-            const uint32_t lane = threadIdx.x & 63u;
-            uint32_t base = 0;
-            if (lane == 0) base = atomicAdd(&lp->nmovers, (uint32_t)__popcll(mv));
-            base = (uint32_t)__shfl((int)base, 0, 64);
-            const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-            const uint32_t pos = base + (uint32_t)__popcll(mv & lt);
-            if (f.changed && pos < MV_CAP) a.movers[(size_t)p.sim * MV_CAP + pos] = idx;
-        }
-    }
+    if (hash_next) publish_next(p, a, lp, f, idx);
     if (p.nw > 0) {
         const double v = in ? a.force_pp[idx] : 0.0;
         const double s = block_tree_sum(v, cache);
         if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
     }
+}
+
+// The wall-force block partials alone (the reference's summation kernel,
+// src/integration.cuh:309-341), after the fused contact-and-integration
+// pass: 256 original indices per workgroup.
+__global__ __launch_bounds__(BLOCK) void k_wall_partials(const DevArrays a) {
+    __shared__ double cache[BLOCK];
+    const DevParams& p = chunk_params(a, blockIdx.x);
+    if (p.nw == 0) return;
+    const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
+    const bool in = idx - p.base < p.np;
+    const double v = in ? a.force_pp[idx] : 0.0;
+    const double s = block_tree_sum(v, cache);
+    if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -1861,7 +1995,9 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
     }
     for (uint32_t c = tid; c < p.ncells; c += BLOCK) a.cell_start[p.cell_base + c] = 0xffffffffu;
     __syncthreads();
-    for (uint32_t j = tid; j < N; j += BLOCK) cells_one(p, a, B + j, sorted_bits);
+    // For the next step: the loop counter advances after this re-sort.
+    const uint32_t nxt = cur_set(a.loop + p.sim) ^ 1u;
+    for (uint32_t j = tid; j < N; j += BLOCK) cells_one(p, a, B + j, sorted_bits, nxt);
     __syncthreads();
 }
 
@@ -2088,13 +2224,13 @@ hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipSt
     if (lanes > 1 && (contactmodel == 2 || contactmodel == 3)) {
         const dim3 g(a.nchunk * (unsigned)lanes);  // BLOCK / lanes particles per workgroup
         if (contactmodel == 2) {
-            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<2, 4>), g, b, 0, s, a);
-            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<2, 8>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_contact_g<2, 16>), g, b, 0, s, a);
+            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<2, 4, false>), g, b, 0, s, a);
+            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<2, 8, false>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_contact_g<2, 16, false>), g, b, 0, s, a);
         } else {
-            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<3, 4>), g, b, 0, s, a);
-            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<3, 8>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_contact_g<3, 16>), g, b, 0, s, a);
+            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<3, 4, false>), g, b, 0, s, a);
+            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<3, 8, false>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_contact_g<3, 16, false>), g, b, 0, s, a);
         }
         return hipGetLastError();
     }
@@ -2107,6 +2243,20 @@ hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipSt
         hipLaunchKernelGGL(k_contact<2>, g, b, 0, s, a);
     else
         hipLaunchKernelGGL(k_contact<0>, g, b, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_contact_integrate(const DevArrays& a, int contactmodel, hipStream_t s) {
+    const dim3 b(BLOCK), g(a.nchunk * 4u);
+    if (contactmodel == 2)
+        hipLaunchKernelGGL((k_contact_g<2, 4, true>), g, b, 0, s, a);
+    else
+        hipLaunchKernelGGL((k_contact_g<3, 4, true>), g, b, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_wall_partials(const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_wall_partials, dim3(a.nchunk), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

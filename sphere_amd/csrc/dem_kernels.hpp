@@ -35,6 +35,11 @@ hipError_t launch_cells(const DevArrays& a, int sorted_bits, hipStream_t s);
 hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipStream_t s);
 // bondsLinear per bond, then a per-particle gather in bond order
 // (src/cohesion.cuh:8-233).
+// Contact pass (models 2 and 3, four lanes per particle) that also
+// integrates each particle and hashes it for the next step; then the wall
+// force partials.  For a fixed grid height, without bonds or a wmode-3 wall.
+hipError_t launch_contact_integrate(const DevArrays& a, int contactmodel, hipStream_t s);
+hipError_t launch_wall_partials(const DevArrays& a, hipStream_t s);
 hipError_t launch_bonds(const DevArrays& a, hipStream_t s);
 // findShearStressOnFixedMovingParticles + summation + serial sum
 // (src/integration.cuh:457-484,309-341,64-68), wmode 3 only.

@@ -154,6 +154,10 @@ struct sph_ctx {
     void* adj_buf = nullptr;
     size_t adj_bytes = 0;
     uint64_t adj_checkpoint = 0;
+    // Contact and integration in one pass where the step allows it (off by
+    // default: measured slower than the two kernels, 72.1 + 6.2 us for the
+    // fused pass and its wall partials against 53.3 + 19.6 us).
+    bool fused = false;
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -489,9 +493,11 @@ int allocate(sph_ctx* c) {
     // The batch-wide sort and the per-simulation one-workgroup sorts.
     ALLOC(A.rhist, std::max<size_t>(256 * (size_t)n_tiles((unsigned)np), rh));
     ALLOC(A.cell_start, nkeys); ALLOC(A.cell_end, nkeys);
-    ALLOC(A.xs, np);
-    ALLOC(A.vs, np);
-    ALLOC(A.ws, np);
+    for (int k = 0; k < 2; ++k) {
+        ALLOC(A.xs[k], np);
+        ALLOC(A.vs[k], np);
+        ALLOC(A.ws[k], np);
+    }
     ALLOC(A.rank, np);
     ALLOC(A.vlist, np * NV);
     ALLOC(A.vcount, np);
@@ -554,9 +560,11 @@ int upload_state(sph_ctx* c) {
     for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemsetAsync(e[k], 0, N * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.delta_t, 0, N * NC * sizeof(double4), s), "memset");
     HIPCHK(c, hipMemsetAsync(A.vcount, 0, N * 4, s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.xs, 0, N * 32, s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.vs, 0, N * 32, s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.ws, 0, N * 32, s), "memset");
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(c, hipMemsetAsync(A.xs[k], 0, N * 32, s), "memset");
+        HIPCHK(c, hipMemsetAsync(A.vs[k], 0, N * 32, s), "memset");
+        HIPCHK(c, hipMemsetAsync(A.ws[k], 0, N * 32, s), "memset");
+    }
     HIPCHK(c, hipMemsetAsync(A.x_build, 0, N * 32, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.force_pp, 0, (size_t)MAXWALLS * N * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.tau_pp, 0, N * 8, s), "memset");
@@ -894,22 +902,42 @@ int enqueue_step(sph_ctx* c) {
         HIPCHK(c, launch_cells(A, c->sorted_bits, s), "cells");
         pr.end();
     }
-    pr.begin("contact");
-    HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
-    pr.end();
-    if (A.nbond > 0) {
+    // Contact and integration in one pass where each particle's force is
+    // final after its own contact row: a fixed grid height, no bonds, no
+    // wmode-3 wall, four lanes per particle.
+    const bool fused = c->fused && hash_in_integrate && A.nbond == 0 && !c->shear_stress &&
+                       c->contact_lanes == 4 && (c->contactmodel == 2 || c->contactmodel == 3);
+    if (fused) {
+        pr.begin("contact_integrate");
+        HIPCHK(c, launch_contact_integrate(A, (int)c->contactmodel, s), "contact");
+        pr.end();
+        bool walls = false;
+        for (const Sim& sim : c->sims) walls = walls || sim.h.nw > 0;
+        if (walls) {
+            pr.begin("wall_partials");
+            HIPCHK(c, launch_wall_partials(A, s), "partials");
+            pr.end();
+        }
+    } else {
+        pr.begin("contact");
+        HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
+        pr.end();
+    }
+    if (!fused && A.nbond > 0) {
         pr.begin("bonds");
         HIPCHK(c, launch_bonds(A, s), "bonds");
         pr.end();
     }
-    if (c->shear_stress) {
+    if (!fused && c->shear_stress) {
         pr.begin("shear_stress");
         HIPCHK(c, launch_shear_stress(A, s), "shear_stress");
         pr.end();
     }
-    pr.begin("integrate");
-    HIPCHK(c, launch_integrate(A, hash_in_integrate ? 1 : 0, s), "integrate");
-    pr.end();
+    if (!fused) {
+        pr.begin("integrate");
+        HIPCHK(c, launch_integrate(A, hash_in_integrate ? 1 : 0, s), "integrate");
+        pr.end();
+    }
     const bool resort = hash_in_integrate && c->resort_at_end;
     pr.begin("walls");
     HIPCHK(c, launch_walls(A, resort ? 1 : 0, c->passes, c->sorted_bits, s), "walls");
@@ -1717,6 +1745,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_GRAPH_STEPS: *v = c->graph_steps; return SPH_OK;
         case SPH_S_XCD_ORDER: *v = c->A.xcd_order; return SPH_OK;
         case SPH_S_ADJ_CHECKPOINT: *v = (double)c->adj_checkpoint; return SPH_OK;
+        case SPH_S_FUSED: *v = c->fused ? 1.0 : 0.0; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1743,6 +1772,16 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             // them, and the next step hashes again.
             const int64_t it = (int64_t)v;
             c->hash_pending = true;
+            // The sorted copies the next step reads follow the counter's
+            // parity: move them along when it changes.
+            if (((it ^ c->sims[0].iter) & 1) != 0) {
+                const size_t bytes = (size_t)c->A.ntot * 32;
+                const int from = (int)(c->sims[0].iter & 1), to = from ^ 1;
+                double4* const* sets[3] = {c->A.xs, c->A.vs, c->A.ws};
+                for (auto set : sets)
+                    HIPCHK(c, hipMemcpyAsync(set[to], set[from], bytes, hipMemcpyDeviceToDevice,
+                                             c->stream), "copy");
+            }
             for (size_t k = 0; k < c->sims.size(); ++k) {
                 c->sims[k].iter = it;
                 int rc = h2d(c, &c->A.loop[k].iter, &it, 8);
@@ -1761,6 +1800,12 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             if (!(v >= 0.0 && v <= 4096.0))
                 return fail(c, SPH_E_ARG, "graph steps must be in [0, 4096] (got %g)", v);
             c->graph_steps = (int)v;
+            return SPH_OK;
+        case SPH_S_FUSED:
+            if (v != 0.0 && v != 1.0)
+                return fail(c, SPH_E_ARG, "fused must be 0 or 1 (got %g)", v);
+            drop_graphs(c);
+            c->fused = v != 0.0;
             return SPH_OK;
         case SPH_S_ADJ_CHECKPOINT:
             if (!(v >= 0.0 && v <= 1e9) || v != (double)(uint64_t)v)

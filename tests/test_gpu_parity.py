@@ -187,6 +187,28 @@ def test_contact_lanes_bit_exact(case, lanes):
     ctx.close()
 
 
+@pytest.mark.parametrize("case", ["settling", "shear1k", "hertz"])
+def test_fused_contact_integration_bit_exact(case):
+    """The contact kernel that also integrates (one vector component per
+    lane) gives the oracle's state, order and rows."""
+    if case == "settling":
+        s, steps = cases.settling_case(1200, 12), 200
+    else:
+        s, steps = cases.fixture("shear1k.bin"), 150
+        if case == "hertz":
+            s.contactmodel[0] = 3
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    ctx.set_scalar("fused", 1)
+    assert ctx.scalar("fused") == 1
+    for _ in range(2):
+        ref.step(steps // 2)
+        ctx.step(steps // 2)
+    assert _compare(ctx, ref) == []
+    ctx.close()
+
+
 @pytest.mark.parametrize("lanes", [1, 4])
 def test_xcd_workgroup_order_bit_exact(lanes):
     """The contact kernel with each XCD given one contiguous range of
