@@ -127,13 +127,20 @@ enum sph_scalar {
                                  per segment); 0 (default): record every step,
                                  or segments if that does not fit in memory.
                                  Identical gradients either way. */
-    SPH_S_FUSED = 20          /* 1: where the step allows (fixed grid height, no
+    SPH_S_FUSED = 20,         /* 1: where the step allows (fixed grid height, no
                                  bonds, no wmode-3 wall, 4 contact lanes, model 2
                                  or 3) the contact kernel also integrates each
                                  particle (one vector component per lane) and a
                                  small pass forms the wall-force partials; 0
                                  (default, measured faster): separate integration
                                  kernel.  Identical results. */
+    SPH_S_WALL_SUMS = 21      /* per simulation.  0 (default, the reference,
+                                 src/device.cu:2504-2505): every dynamic wall
+                                 integrates wall 0's force sum; 1: wall w
+                                 integrates the sum of its own per-particle
+                                 forces (same tree and serial order) - the
+                                 correction multi-wall setups (triaxial, nw = 5)
+                                 need.  No counterpart in the reference. */
 };
 
 /* Open flags. */

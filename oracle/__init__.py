@@ -95,6 +95,7 @@ def _lib():
         L.orc_tree_sum.argtypes = [ctypes.c_void_p, ctypes.c_uint32]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_exact_length.argtypes = [ctypes.c_int]
+        L.orc_set_wall_sums_per_wall.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -112,6 +113,12 @@ def set_exact_length(on):
     """Gradient-check aid: FP64 instead of FP32 lengths in every step that
     follows (process-wide; the reference's FP32 lengths are the default)."""
     _lib().orc_set_exact_length(1 if on else 0)
+
+
+def set_wall_sums_per_wall(on):
+    """Each dynamic wall integrates its own force sum (the engine's
+    SPH_S_WALL_SUMS = 1) instead of wall 0's (the reference)."""
+    _lib().orc_set_wall_sums_per_wall(1 if on else 0)
 
 
 class Oracle:

@@ -76,6 +76,7 @@ static inline Real sqrt32(Real v) { return (Real)sqrtf((float)v); }
 // A smooth variant for checking gradients by finite differences: lengths
 // as FP64 roots (off by default; the FP32 root is the reference's).
 static bool g_exact_length = false;
+static bool g_wall_sums_per_wall = false;
 static inline Real length(R3 v) { return g_exact_length ? sqrt(dot(v, v)) : sqrt32(dot(v, v)); }
 
 struct Params {
@@ -1036,20 +1037,29 @@ void stage_integrate(orc_sim* s) {
 }
 
 // summation + integrateWalls, src/integration.cuh:309-450 and
-// src/device.cu:2500-2527.  Every dynamic wall integrates wall 0's sum (P8).
+// src/device.cu:2500-2527.  Every dynamic wall integrates wall 0's sum (P8);
+// with g_wall_sums_per_wall (the engine's optional correction, no reference
+// counterpart) wall w integrates the same tree-and-serial sum of its own
+// per-particle forces.
 void stage_walls(orc_sim* s) {
     if (s->nw == 0) return;
     block_tree_sum(s->force_pp.data(), s->np, s->force_partial);
     const Params& P = s->P;
     const Real dt = s->dt;
+    std::vector<Real> own;
     for (uint32_t w = 0; w < s->nw; ++w) {
         R4 nx = s->w_nx[w], mvfd = s->w_mvfd[w];
         const int wmode = s->wmode[w];
         if (wmode == 0) continue;
         Real acc0;
         if (s->iter == 0) acc0 = 0.0; else acc0 = s->walls_acc[w];
+        const Real* part = s->force_partial.data();
+        if (g_wall_sums_per_wall && w > 0) {
+            block_tree_sum(s->force_pp.data() + (size_t)w * s->np, s->np, own);
+            part = own.data();
+        }
         mvfd.z = 0.0;
-        for (uint32_t i = 0; i < s->blocks(); ++i) mvfd.z += s->force_partial[i];
+        for (uint32_t i = 0; i < s->blocks(); ++i) mvfd.z += part[i];
         const Real sigma_0 = mvfd.w + P.sigma0_A * sin(2.0 * PI * P.sigma0_f * s->t);
         const Real N = -sigma_0 * s->L[0] * s->L[1];
         Real acc = (mvfd.z + N) / mvfd.x;
@@ -1249,6 +1259,7 @@ int64_t orc_write_mem(const orc_sim* s, void* dst, size_t cap) {
 
 double orc_sqrt32(double v) { return sqrt32(v); }
 void orc_set_exact_length(int on) { g_exact_length = on != 0; }
+void orc_set_wall_sums_per_wall(int on) { g_wall_sums_per_wall = on != 0; }
 
 double orc_tree_sum(const double* v, uint32_t n) {
     std::vector<Real> partial;

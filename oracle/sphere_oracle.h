@@ -69,6 +69,9 @@ int64_t orc_iter(const orc_sim* s);
 double orc_sqrt32(double v);
 /* Test aid for gradient checks: FP64 lengths instead of FP32 (default 0). */
 void orc_set_exact_length(int on);
+/* Each dynamic wall integrates its own force sum instead of wall 0's (the
+ * engine's optional correction of the reference's P8 behaviour). */
+void orc_set_wall_sums_per_wall(int on);
 double orc_tree_sum(const double* v, uint32_t n);
 
 /* Number of OpenMP threads the per-particle stages use (0 = runtime default). */

@@ -1822,10 +1822,13 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
         if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
     }
     if (hash_next) publish_next(p, a, lp, f, idx);
-    if (p.nw > 0) {
-        const double v = in ? a.force_pp[idx] : 0.0;
+    // Wall-force block partials: wall 0's, and with per-wall sums every
+    // wall's (wall-major, nchunk per wall).
+    const uint32_t nsum = p.nw == 0 ? 0u : (p.wall_sums ? p.nw : 1u);
+    for (uint32_t w = 0; w < nsum; ++w) {
+        const double v = in ? a.force_pp[(size_t)w * a.ntot + idx] : 0.0;
         const double s = block_tree_sum(v, cache);
-        if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+        if (threadIdx.x == 0) a.partial[(size_t)w * a.nchunk + blockIdx.x] = s;
     }
 }
 
@@ -1835,12 +1838,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(5))) void
 __global__ __launch_bounds__(BLOCK) void k_wall_partials(const DevArrays a) {
     __shared__ double cache[BLOCK];
     const DevParams& p = chunk_params(a, blockIdx.x);
-    if (p.nw == 0) return;
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool in = idx - p.base < p.np;
-    const double v = in ? a.force_pp[idx] : 0.0;
-    const double s = block_tree_sum(v, cache);
-    if (threadIdx.x == 0) a.partial[blockIdx.x] = s;
+    const uint32_t nsum = p.nw == 0 ? 0u : (p.wall_sums ? p.nw : 1u);
+    for (uint32_t w = 0; w < nsum; ++w) {
+        const double v = in ? a.force_pp[(size_t)w * a.ntot + idx] : 0.0;
+        const double s = block_tree_sum(v, cache);
+        if (threadIdx.x == 0) a.partial[(size_t)w * a.nchunk + blockIdx.x] = s;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2014,7 +2019,14 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     DevLoop* lp = a.loop + p.sim;
     if (threadIdx.x == 0) do_sort = (resort && lp->changed[step_slot(lp) ^ 1u]) ? 1u : 0u;
     const bool walls = p.nw > 0 && p.np > 0;
-    const double fsum = walls ? serial_sum(a.partial + p.chunk_base, p.nblocks, stage) : 0.0;
+    // The serial sums of the block partials: wall 0's (every wall integrates
+    // it, P8), or with per-wall sums each wall's.
+    double fsum[MAXWALLS];
+    const uint32_t nsum = !walls ? 0u : (p.wall_sums ? p.nw : 1u);
+    for (uint32_t w = 0; w < MAXWALLS; ++w)
+        fsum[w] = w < nsum ? serial_sum(a.partial + (size_t)w * a.nchunk + p.chunk_base,
+                                        p.nblocks, stage)
+                           : 0.0;
     __syncthreads();
     if (do_sort) resort_one_block(p, a, passes, sorted_bits);
     if (threadIdx.x != 0) return;
@@ -2031,7 +2043,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
         double4 w_mvfd = a.w_mvfd[wi];
         double acc0;
         if (lp->iter == 0) acc0 = 0.0; else acc0 = a.w_acc[wi];
-        w_mvfd.z = fsum;  // 0.0 + partial[0] + partial[1] + ... in order
+        w_mvfd.z = p.wall_sums ? fsum[w] : fsum[0];  // 0.0 + partial[0] + partial[1] + ...
         const double sigma_0 = w_mvfd.w + p.sigma0_A * sin(2.0 * PI * p.sigma0_f * t_now);
         const double N = -sigma_0 * p.L[0] * p.L[1];
         double acc = (w_mvfd.z + N) / w_mvfd.x;

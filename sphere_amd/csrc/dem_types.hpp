@@ -91,7 +91,7 @@ struct DevParams {
     uint32_t wall_base;   // first wall of the simulation
     uint32_t bond_base;   // first bond of the simulation
     uint32_t rhist_base;  // radix histogram space of the one-workgroup re-sort
-    uint32_t pad2;
+    uint32_t wall_sums;   // 1: wall w integrates its own force sum; 0: wall 0's (P8)
 };
 
 // Loop state kept on the device so that captured step sequences can be

@@ -510,7 +510,8 @@ int allocate(sph_ctx* c) {
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
     ALLOC(A.force_pp, (size_t)MAXWALLS * np);
     ALLOC(A.tau_pp, np);
-    ALLOC(A.partial, A.nchunk); ALLOC(A.tau_partial, A.nchunk); ALLOC(A.tau_eff, ns);
+    ALLOC(A.partial, (size_t)MAXWALLS * A.nchunk);
+    ALLOC(A.tau_partial, A.nchunk); ALLOC(A.tau_eff, ns);
     ALLOC(A.bonds, 2 * nb); ALLOC(A.bonds_delta, nb); ALLOC(A.bonds_omega, nb);
     ALLOC(A.bond_f, nb); ALLOC(A.bond_t, nb); ALLOC(A.bond_ok, nb);
     ALLOC(A.badj_off, np + 1);
@@ -1746,6 +1747,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_XCD_ORDER: *v = c->A.xcd_order; return SPH_OK;
         case SPH_S_ADJ_CHECKPOINT: *v = (double)c->adj_checkpoint; return SPH_OK;
         case SPH_S_FUSED: *v = c->fused ? 1.0 : 0.0; return SPH_OK;
+        case SPH_S_WALL_SUMS: *v = S.P.wall_sums; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1801,6 +1803,11 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
                 return fail(c, SPH_E_ARG, "graph steps must be in [0, 4096] (got %g)", v);
             c->graph_steps = (int)v;
             return SPH_OK;
+        case SPH_S_WALL_SUMS:
+            if (v != 0.0 && v != 1.0)
+                return fail(c, SPH_E_ARG, "wall sums must be 0 or 1 (got %g)", v);
+            S.P.wall_sums = (uint32_t)v;
+            return upload_params(c);
         case SPH_S_FUSED:
             if (v != 0.0 && v != 1.0)
                 return fail(c, SPH_E_ARG, "fused must be 0 or 1 (got %g)", v);

@@ -251,3 +251,32 @@ def test_resort_placement_bit_exact(case, mode, merge_max):
     assert _compare(ctx, ref) == []
     _occupied_cells_equal(ctx, ref)
     ctx.close()
+
+
+@pytest.mark.parametrize("per_wall", [False, True])
+def test_triaxial_walls_bit_exact(per_wall):
+    """Five dynamic walls (triaxial: wall 0 at constant velocity, the side
+    walls under a normal stress), with the reference's wall-0 force sum for
+    every wall, and with each wall's own sum (SPH_S_WALL_SUMS = 1)."""
+    s = cases.settling_case(1200, 12)
+    s.triaxial(wvel=-0.01, normal_stress=2.0e3)
+    image = s.tobytes()
+    try:
+        oracle.set_wall_sums_per_wall(per_wall)
+        ref = oracle.Oracle(image)
+        ctx = engine.Context(image)
+        ctx.set_scalar("wall_sums", 1 if per_wall else 0)
+        for _ in range(2):
+            ref.step(80)
+            ctx.step(80)
+    finally:
+        oracle.set_wall_sums_per_wall(False)
+    bad = _compare(ctx, ref)
+    for f in ("walls_nx", "walls_mvfd", "walls_acc"):
+        if not numpy.array_equal(ctx.get(f), ref.get(f)):
+            bad.append(f)
+    assert bad == []
+    mv = ctx.get("walls_mvfd").reshape(-1, 4)
+    if per_wall:
+        assert len({float(v) for v in mv[1:, 2]}) > 1  # the side walls' own forces
+    ctx.close()

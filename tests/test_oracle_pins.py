@@ -626,3 +626,50 @@ def test_codec_round_trip_through_oracle():
     s.bond(5, 9)
     img = s.tobytes()
     assert oracle.Oracle(img).tobytes() == img
+
+
+def _wall1_only_case():
+    """Two resting spheres in a triaxial box of five dynamic walls; sphere 0
+    presses on side wall 1 (normal -x) and touches nothing else."""
+    from sphere_amd.sim import sim
+    s = sim(np=2, sid="wall1")
+    s.radius[:] = 0.01
+    s.x[0, :] = [0.0897, 0.05, 0.05]
+    s.x[1, :] = [0.05, 0.05, 0.05]
+    s.defineWorldBoundaries(L=[0.1, 0.1, 0.1])
+    s.initTemporal(total=1.0, file_dt=0.1)
+    s.g[:] = 0.0
+    s.triaxial(wvel=0.0, normal_stress=1.0e3)
+    # the box walls back at the domain faces, wall 1 just inside sphere 0
+    s.w_x[:] = [0.1, 0.0995, 0.0, 0.1, 0.0]
+    s.w_m[:] = 1.0
+    return s
+
+
+def test_per_wall_sums_integrate_each_walls_own_force():
+    """Reference behaviour (P8): every dynamic wall integrates wall 0's sum,
+    here zero, so side wall 1 feels nothing.  With per-wall sums, wall 1's
+    force is the sum of its own per-particle forces: one nonzero leaf, so
+    exactly that particle's wall-1 force; the other walls stay at zero."""
+    s = _wall1_only_case()
+    img = s.tobytes()
+    try:
+        oracle.set_wall_sums_per_wall(False)
+        ref = oracle.Oracle(img)
+        ref.step(1)
+        oracle.set_wall_sums_per_wall(True)
+        fix = oracle.Oracle(img)
+        fix.step(1)
+    finally:
+        oracle.set_wall_sums_per_wall(False)
+    n = 2
+    fpp = fix.get("walls_force_pp").reshape(5, n)
+    assert fpp[1, 0] != 0.0 and fpp[1, 1] == 0.0
+    assert (fpp[[0, 2, 3, 4]] == 0.0).all()
+    mref = ref.get("walls_mvfd").reshape(-1, 4)
+    mfix = fix.get("walls_mvfd").reshape(-1, 4)
+    assert (mref[:, 2] == 0.0).all()
+    assert mfix[1, 2] == fpp[1, 0]
+    assert mfix[0, 2] == 0.0 and (mfix[2:, 2] == 0.0).all()
+    # wall 1 now moves under its own force (it did not before)
+    assert fix.get("walls_acc")[1] != ref.get("walls_acc")[1]
