@@ -253,6 +253,19 @@ const char* sph_last_error(const sph_ctx* ctx);
 const char* sph_open_error(void);
 const char* sph_version_string(void);
 
+/* Output-side readers of the current state, on the GPU.
+ * DEM::findOverlaps (src/sphere.cpp:674-724, printed by `sphere --contacts`,
+ * src/sphere.cpp:904-913): every pair i < j (.bin indices, ascending i then
+ * j) whose FP64 distance is below r_i + r_j, no periodic images, both with
+ * fixvel 0; pairs[2k], pairs[2k + 1] and overlaps[k] = distance - (r_i + r_j)
+ * for the first min(total, cap) pairs; *count = total.
+ * DEM::porosity (src/sphere.cpp:486-592, `sphere --porosity`): z_slices
+ * slices from origo to the top wall (or L_z); SPH_E_INPUT when the slices
+ * are thinner than twice the last particle's radius (the reference exits). */
+int sph_find_overlaps(sph_ctx* ctx, int sim, uint32_t* pairs, double* overlaps, uint64_t cap,
+                      uint64_t* count);
+int sph_porosity(sph_ctx* ctx, int sim, int z_slices, double* z_pos, double* porosity);
+
 /* Diagnostic, no reference counterpart: runs the engine's FP32 square root
  * (the reference's sqrtf in length(), src/vector_arithmetic.h) on device
  * `device` for every FP32 bit pattern in [lo, hi) (hi <= 2^32) and counts

@@ -96,6 +96,11 @@ def _lib():
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_set_exact_length.argtypes = [ctypes.c_int]
         L.orc_set_wall_sums_per_wall.argtypes = [ctypes.c_int]
+        L.orc_find_overlaps.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]
+        L.orc_find_overlaps.restype = None
+        L.orc_porosity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                   ctypes.c_void_p]
         L.orc_get_threads.restype = ctypes.c_int
         _LIB = L
     return _LIB
@@ -181,6 +186,26 @@ class Oracle:
         rc = _lib().orc_set_field(self._h, fid, a.ctypes.data, a.nbytes)
         if rc != 0:
             raise RuntimeError("orc_set_field(%s) failed: %d" % (name, rc))
+
+    def find_overlaps(self):
+        """DEM::findOverlaps: (pairs (k, 2) i < j, overlaps (k,))."""
+        L = _lib()
+        n = ctypes.c_uint64()
+        L.orc_find_overlaps(self._h, None, None, 0, ctypes.byref(n))
+        pairs = numpy.zeros((n.value, 2), dtype=numpy.uint32)
+        delta = numpy.zeros(n.value)
+        L.orc_find_overlaps(self._h, pairs.ctypes.data, delta.ctypes.data, n.value,
+                            ctypes.byref(n))
+        return pairs, delta
+
+    def porosity(self, z_slices=10):
+        """DEM::porosity: (z positions, porosities) of z_slices slices."""
+        z = numpy.zeros(z_slices)
+        por = numpy.zeros(z_slices)
+        rc = _lib().orc_porosity(self._h, int(z_slices), z.ctypes.data, por.ctypes.data)
+        if rc:
+            raise ValueError("The number of z-slices is too high.")
+        return z, por
 
     def tobytes(self):
         n = _lib().orc_write_mem(self._h, None, 0)

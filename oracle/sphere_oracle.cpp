@@ -1288,4 +1288,71 @@ int orc_get_threads(void) {
 #endif
 }
 
+// ---------------------------------------------------------------------------
+// Output-side readers.
+
+// DEM::findOverlaps, src/sphere.cpp:674-724: every pair i < j (index order)
+// with len(x_j - x_i) - (r_i + r_j) < 0 (FP64 root, no periodic images),
+// both particles with fixvel exactly 0.  Writes at most `cap` pairs; returns
+// the total in *count.
+void orc_find_overlaps(const orc_sim* s, uint32_t* pairs, double* delta, uint64_t cap,
+                       uint64_t* count) {
+    uint64_t n = 0;
+    for (uint32_t i = 0; i < s->np; ++i)
+        for (uint32_t j = i + 1; j < s->np; ++j) {
+            const R4 xi = s->x[i], xj = s->x[j];
+            const Real dx = xj.x - xi.x, dy = xj.y - xi.y, dz = xj.z - xi.z;
+            const Real len = sqrt(dx * dx + dy * dy + dz * dz);
+            const Real dn = len - (xi.w + xj.w);
+            if (dn < 0.0 && (s->vel[i].w == 0 && s->vel[j].w == 0)) {
+                if (n < cap) {
+                    pairs[2 * n] = i;
+                    pairs[2 * n + 1] = j;
+                    delta[n] = dn;
+                }
+                ++n;
+            }
+        }
+    *count = n;
+}
+
+// DEM::porosity, src/sphere.cpp:486-592 (with sphericalCap, :468-471, and
+// r_max, :474-484, which returns the last particle's radius).  Returns -1
+// when the slices are thinner than that radius times two (the reference
+// exits), else 0.
+int orc_porosity(const orc_sim* s, int z_slices, double* z_pos, double* porosity) {
+    const double pi = 3.14159265358979323846;
+    auto cap = [&](double h, double r) { return pi * h * h / 3.0 * (3.0 * r - h); };
+    const double top = s->nw > 0 ? s->w_nx[0].w : s->L[2];
+    const double h_slice = (top - s->origo[2]) / (double)z_slices;
+    const double r_last = s->np > 0 ? s->x[s->np - 1].w : 0.0;
+    if (h_slice <= r_last * 2.0) return -1;
+    const double V_slice = h_slice * s->L[0] * s->L[1];
+    for (int iz = 0; iz < z_slices; ++iz) {
+        double V_void = V_slice;
+        const double lo = iz * h_slice;
+        const double hi = lo + h_slice;
+        for (uint32_t i = 0; i < s->np; ++i) {
+            const double zc = s->x[i].z, r = s->x[i].w;
+            const double zl = zc - r, zh = zc + r;
+            const double V_sphere = 4.0 / 3.0 * pi * r * r * r;
+            if (lo <= zl && zh <= hi) {
+                V_void -= V_sphere;
+            } else {
+                if (lo >= zc && lo <= zh)
+                    V_void -= cap(zh - lo, r);
+                else if (lo < zc && lo > zl)
+                    V_void -= V_sphere + cap(lo - zl, r);
+                if (hi <= zc && hi >= zl)
+                    V_void -= cap(hi - zl, r);
+                else if (hi > zc && hi < zh)
+                    V_void -= V_sphere + cap(zh - hi, r);
+            }
+        }
+        z_pos[iz] = lo + 0.5 * h_slice;
+        porosity[iz] = V_void / V_slice;
+    }
+    return 0;
+}
+
 }  // extern "C"

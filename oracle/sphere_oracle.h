@@ -73,6 +73,11 @@ void orc_set_exact_length(int on);
  * engine's optional correction of the reference's P8 behaviour). */
 void orc_set_wall_sums_per_wall(int on);
 double orc_tree_sum(const double* v, uint32_t n);
+/* Output-side readers: DEM::findOverlaps (pairs i < j, FP64 lengths, no
+ * periodic images, fixvel 0) and DEM::porosity (z slices, -1 when too thin). */
+void orc_find_overlaps(const orc_sim* s, uint32_t* pairs, double* delta, uint64_t cap,
+                       uint64_t* count);
+int orc_porosity(const orc_sim* s, int z_slices, double* z_pos, double* porosity);
 
 /* Number of OpenMP threads the per-particle stages use (0 = runtime default). */
 void orc_set_threads(int n);

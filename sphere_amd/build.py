@@ -25,7 +25,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function",
 ]
 
-KERNEL_SOURCES = ["dem_kernels.hip", "adjoint.hip"]
+KERNEL_SOURCES = ["dem_kernels.hip", "adjoint.hip", "analysis.hip"]
 HOST_SOURCES = ["engine.cpp"]
 
 

@@ -22,6 +22,7 @@
 #include "../../include/sphere_hip.h"
 #include "dem_arrays.hpp"
 #include "adjoint.hpp"
+#include "analysis.hpp"
 #include "dem_kernels.hpp"
 #include "dem_types.hpp"
 
@@ -2192,6 +2193,98 @@ void sph_close(sph_ctx* c) {
 const char* sph_last_error(const sph_ctx* c) { return c ? c->err.c_str() : g_open_error.c_str(); }
 const char* sph_open_error(void) { return g_open_error.c_str(); }
 const char* sph_version_string(void) { return "sphere_hip 0.1 (.bin 2.15, gfx950)"; }
+
+int sph_find_overlaps(sph_ctx* c, int sim, uint32_t* pairs, double* overlaps, uint64_t cap,
+                      uint64_t* count) {
+    CHECK_SIM(c, sim);
+    if (!count || (cap && (!pairs || !overlaps))) return SPH_E_ARG;
+    const Sim& S = c->sims[sim];
+    const uint32_t np = S.h.np, b0 = S.P.base;
+    hipStream_t s = c->stream;
+    uint32_t* counts = nullptr;
+    uint64_t* offs = nullptr;
+    uint32_t* dpairs = nullptr;
+    double* ddelta = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(counts);
+        (void)hipFree(offs);
+        (void)hipFree(dpairs);
+        (void)hipFree(ddelta);
+    };
+#define AFAIL(e_, what)                        \
+    do {                                       \
+        hipError_t ee_ = (e_);                 \
+        if (ee_ != hipSuccess) {               \
+            release();                         \
+            return hip_fail(c, ee_, what);     \
+        }                                      \
+    } while (0)
+    AFAIL(hipMalloc(&counts, (size_t)np * 4 + 4), "hipMalloc");
+    AFAIL(hipMalloc(&offs, (size_t)np * 8 + 8), "hipMalloc");
+    AFAIL(launch_overlaps(c->A.x + b0, c->A.vel + b0, np, counts, nullptr, nullptr, nullptr, s),
+          "overlaps");
+    std::vector<uint32_t> hc(np);
+    std::vector<uint64_t> ho(np);
+    AFAIL(hipMemcpyAsync(hc.data(), counts, (size_t)np * 4, hipMemcpyDeviceToHost, s), "D2H");
+    AFAIL(hipStreamSynchronize(s), "sync");
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < np; ++i) {
+        ho[i] = total;
+        total += hc[i];
+    }
+    *count = total;
+    const uint64_t take = std::min<uint64_t>(total, cap);
+    if (take > 0) {
+        AFAIL(hipMemcpyAsync(offs, ho.data(), (size_t)np * 8, hipMemcpyHostToDevice, s), "H2D");
+        AFAIL(hipMalloc(&dpairs, total * 8), "hipMalloc");
+        AFAIL(hipMalloc(&ddelta, total * 8), "hipMalloc");
+        AFAIL(launch_overlaps(c->A.x + b0, c->A.vel + b0, np, counts, offs, dpairs, ddelta, s),
+              "overlaps");
+        AFAIL(hipMemcpyAsync(pairs, dpairs, take * 8, hipMemcpyDeviceToHost, s), "D2H");
+        AFAIL(hipMemcpyAsync(overlaps, ddelta, take * 8, hipMemcpyDeviceToHost, s), "D2H");
+    }
+#undef AFAIL
+    release();
+    return SPH_OK;
+}
+
+int sph_porosity(sph_ctx* c, int sim, int z_slices, double* z_pos, double* porosity) {
+    CHECK_SIM(c, sim);
+    if (z_slices <= 0 || !z_pos || !porosity) return SPH_E_ARG;
+    const Sim& S = c->sims[sim];
+    const uint32_t np = S.h.np, b0 = S.P.base;
+    double top = S.h.L[2];
+    if (S.h.nw > 0) {
+        double4 w;
+        int rc = d2h(c, &w, c->A.w_nx + S.P.wall_base, 32);
+        if (rc) return rc;
+        top = w.w;
+    }
+    const double h_slice = (top - S.h.origo[2]) / (double)z_slices;
+    double4 last = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (np > 0) {
+        int rc = d2h(c, &last, c->A.x + b0 + np - 1, 32);
+        if (rc) return rc;
+    }
+    // The reference checks the slice height against "r_max()", which
+    // returns the last particle's radius (src/sphere.cpp:474-484).
+    if (h_slice <= last.w * 2.0)
+        return fail(c, SPH_E_INPUT, "Error! The number of z-slices is too high.");
+    const double V_slice = h_slice * S.h.L[0] * S.h.L[1];
+    double* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, (size_t)z_slices * 16), "hipMalloc");
+    hipError_t e = launch_porosity(c->A.x + b0, np, z_slices, h_slice, V_slice, d, d + z_slices,
+                                   c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(z_pos, d, (size_t)z_slices * 8,
+                                            hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(porosity, d + z_slices, (size_t)z_slices * 8,
+                                            hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "porosity");
+    return SPH_OK;
+}
 
 int sph_selftest_sqrt32(int device, uint64_t lo, uint64_t hi, uint64_t* mismatches) {
     if (!mismatches || lo > hi || hi > (1ull << 32)) return SPH_E_ARG;

@@ -75,7 +75,8 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
            "sph_set_scalar_sim", "sph_adjoint", "sph_selftest_sqrt32", "sph_write_state",
-           "sph_read_state", "sph_write_state_sim", "sph_read_state_sim")
+           "sph_read_state", "sph_write_state_sim", "sph_read_state_sim", "sph_find_overlaps",
+           "sph_porosity")
 
 # Parameters of sph_adjoint's g_params, in order.
 ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
@@ -131,6 +132,9 @@ def load():
                                      ctypes.POINTER(vp)]
     L.sph_num_sims.argtypes = [vp]
     L.sph_write_bin_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+    L.sph_find_overlaps.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_uint64)]
+    L.sph_porosity.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     L.sph_write_state.argtypes = [vp, ctypes.c_char_p]
     L.sph_read_state.argtypes = [vp, ctypes.c_char_p]
     L.sph_write_state_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
@@ -240,6 +244,25 @@ class Context:
 
     def write_bin(self, path, sim=0):
         self._chk(self._L.sph_write_bin_sim(self._h, sim, path.encode()))
+
+    def find_overlaps(self, sim=0):
+        """DEM::findOverlaps on the GPU (sph_find_overlaps): (pairs (k, 2),
+        i < j, ascending; overlaps (k,), negative)."""
+        n = ctypes.c_uint64()
+        self._chk(self._L.sph_find_overlaps(self._h, sim, None, None, 0, ctypes.byref(n)))
+        pairs = numpy.zeros((n.value, 2), dtype=numpy.uint32)
+        delta = numpy.zeros(n.value)
+        self._chk(self._L.sph_find_overlaps(self._h, sim, pairs.ctypes.data, delta.ctypes.data,
+                                            n.value, ctypes.byref(n)))
+        return pairs, delta
+
+    def porosity(self, z_slices=10, sim=0):
+        """DEM::porosity on the GPU (sph_porosity): (z positions, porosities)."""
+        z = numpy.zeros(int(z_slices))
+        por = numpy.zeros(int(z_slices))
+        self._chk(self._L.sph_porosity(self._h, sim, int(z_slices), z.ctypes.data,
+                                       por.ctypes.data))
+        return z, por
 
     def write_state(self, path, sim=0):
         """Restart sidecar (sph_write_state): the contact rows, displacements,

@@ -673,3 +673,71 @@ def test_per_wall_sums_integrate_each_walls_own_force():
     assert mfix[0, 2] == 0.0 and (mfix[2:, 2] == 0.0).all()
     # wall 1 now moves under its own force (it did not before)
     assert fix.get("walls_acc")[1] != ref.get("walls_acc")[1]
+
+
+def _three_spheres(fix2=False):
+    from sphere_amd.sim import sim
+    s = sim(np=3)
+    s.radius[:] = [0.01, 0.01, 0.02]
+    s.x[0] = [0.05, 0.05, 0.05]
+    s.x[1] = [0.068, 0.05, 0.05]
+    s.x[2] = [0.05, 0.05, 0.079]
+    if fix2:
+        s.fixvel[2] = 1.0
+    s.defineWorldBoundaries(L=[0.1, 0.1, 0.2])
+    s.initTemporal(total=1.0, file_dt=0.1)
+    return s
+
+
+def test_find_overlaps_pairs_against_kd_tree_and_exclusions():
+    """DEM::findOverlaps: the pair set equals an independent k-d tree search
+    (pairs closer than r_i + r_j), the overlaps equal the distances less the
+    radius sums, and pairs with a fixed particle are left out."""
+    from scipy.spatial import cKDTree
+    pairs, delta = oracle.Oracle(_three_spheres().tobytes()).find_overlaps()
+    assert pairs.tolist() == [[0, 1], [0, 2]]
+    assert delta == pytest.approx([-0.002, -0.001], rel=1e-12)
+    pairs, _ = oracle.Oracle(_three_spheres(fix2=True).tobytes()).find_overlaps()
+    assert pairs.tolist() == [[0, 1]]
+    from . import cases
+    s = cases.settling_case(300, 11)
+    x = numpy.asarray(s.x)[:, :3]
+    r = numpy.asarray(s.radius)
+    s.fixvel[:] = 0.0
+    pairs, delta = oracle.Oracle(s.tobytes()).find_overlaps()
+    cand = cKDTree(x).query_pairs(2.0 * r.max(), output_type="ndarray")
+    d = numpy.linalg.norm(x[cand[:, 1]] - x[cand[:, 0]], axis=1) - (r[cand[:, 0]] + r[cand[:, 1]])
+    want = cand[d < 0]
+    want = want[numpy.lexsort((want[:, 1], want[:, 0]))]
+    assert numpy.array_equal(pairs, want)
+    dd = d[d < 0][numpy.lexsort((cand[d < 0][:, 1], cand[d < 0][:, 0]))]
+    assert delta == pytest.approx(dd, rel=1e-9, abs=1e-15)
+
+
+def test_porosity_closed_forms_and_the_references_cap_rule():
+    """DEM::porosity: a sphere inside a slice removes its volume; a sphere
+    centred on a slice face removes a half (the cap of height r) from each
+    side; a sphere whose centre lies inside a slice and crosses its lower
+    face has the reference's rule V_sphere + cap(z_low - (z_c - r))
+    subtracted (src/sphere.cpp:565-570), kept as it is."""
+    from sphere_amd.sim import sim
+    o = oracle.Oracle(_three_spheres().tobytes())
+    z, por = o.porosity(4)
+    V = lambda r: 4.0 / 3.0 * math.pi * r ** 3
+    Vs = 0.05 * 0.1 * 0.1
+    assert z.tolist() == pytest.approx([0.025, 0.075, 0.125, 0.175])
+    # spheres 0 and 1 centred on the face z = 0.05: half each side
+    assert por[0] == pytest.approx(1.0 - V(0.01) / Vs, rel=1e-13)
+    assert por[1] == pytest.approx(1.0 - (V(0.01) + V(0.02)) / Vs, rel=1e-13)
+    assert por[2] == 1.0 and por[3] == 1.0
+    s = sim(np=1)
+    s.radius[:] = 0.01
+    s.x[0] = [0.05, 0.05, 0.055]
+    s.defineWorldBoundaries(L=[0.1, 0.1, 0.2])
+    s.initTemporal(total=1.0, file_dt=0.1)
+    z, por = oracle.Oracle(s.tobytes()).porosity(4)
+    cap = lambda h, r: math.pi * h * h / 3.0 * (3.0 * r - h)
+    assert por[1] == pytest.approx(1.0 - (V(0.01) + cap(0.005, 0.01)) / Vs, rel=1e-13)
+    assert por[0] == pytest.approx(1.0 - cap(0.005, 0.01) / Vs, rel=1e-13)
+    with pytest.raises(ValueError):
+        oracle.Oracle(s.tobytes()).porosity(11)  # slices of 0.018 m < 2 r
