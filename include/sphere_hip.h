@@ -134,13 +134,16 @@ enum sph_scalar {
                                  small pass forms the wall-force partials; 0
                                  (default, measured faster): separate integration
                                  kernel.  Identical results. */
-    SPH_S_WALL_SUMS = 21      /* per simulation.  0 (default, the reference,
+    SPH_S_WALL_SUMS = 21,     /* per simulation.  0 (default, the reference,
                                  src/device.cu:2504-2505): every dynamic wall
                                  integrates wall 0's force sum; 1: wall w
                                  integrates the sum of its own per-particle
                                  forces (same tree and serial order) - the
                                  correction multi-wall setups (triaxial, nw = 5)
                                  need.  No counterpart in the reference. */
+    SPH_S_K_N = 22            /* per simulation, read-only: the normal stiffness
+                                 params.k_n, which DEM::forcechains scales the
+                                 overlaps by (src/sphere.cpp:789). */
 };
 
 /* Open flags. */

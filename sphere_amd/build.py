@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsphere_hip.so")
 CLI = os.path.join(HERE, "bin", "sphere")
+FORCECHAINS = os.path.join(HERE, "bin", "forcechains")
 
 ARCH = os.environ.get("SPHERE_HIP_ARCH", "gfx950")
 
@@ -68,8 +69,9 @@ def build(verbose=False):
     _run([cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs, verbose)
     os.replace(tmp, LIB)
     os.makedirs(os.path.dirname(CLI), exist_ok=True)
-    _run([cc, "-O2", "-std=c++17", os.path.join(CSRC, "main.cpp"), "-o", CLI,
-          "-L" + HERE, "-lsphere_hip", "-Wl,-rpath,$ORIGIN/.."], verbose)
+    for src, exe in (("main.cpp", CLI), ("forcechains.cpp", FORCECHAINS)):
+        _run([cc, "-O2", "-std=c++17", os.path.join(CSRC, src), "-o", exe,
+              "-L" + HERE, "-lsphere_hip", "-Wl,-rpath,$ORIGIN/.."], verbose)
     return LIB
 
 

@@ -1749,6 +1749,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_ADJ_CHECKPOINT: *v = (double)c->adj_checkpoint; return SPH_OK;
         case SPH_S_FUSED: *v = c->fused ? 1.0 : 0.0; return SPH_OK;
         case SPH_S_WALL_SUMS: *v = S.P.wall_sums; return SPH_OK;
+        case SPH_S_K_N: *v = S.P.k_n; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -2200,17 +2201,23 @@ int sph_find_overlaps(sph_ctx* c, int sim, uint32_t* pairs, double* overlaps, ui
     if (!count || (cap && (!pairs || !overlaps))) return SPH_E_ARG;
     const Sim& S = c->sims[sim];
     const uint32_t np = S.h.np, b0 = S.P.base;
+    const double4* x = c->A.x + b0;
+    const double4* vel = c->A.vel + b0;
     hipStream_t s = c->stream;
     uint32_t* counts = nullptr;
     uint64_t* offs = nullptr;
     uint32_t* dpairs = nullptr;
     double* ddelta = nullptr;
+    double* dbounds = nullptr;
+    uint32_t* key = nullptr;
+    uint32_t* cells = nullptr;
+    uint32_t* cursor = nullptr;
+    uint32_t* order = nullptr;
     auto release = [&]() {
         (void)hipStreamSynchronize(s);
-        (void)hipFree(counts);
-        (void)hipFree(offs);
-        (void)hipFree(dpairs);
-        (void)hipFree(ddelta);
+        for (void* p : {(void*)counts, (void*)offs, (void*)dpairs, (void*)ddelta, (void*)dbounds,
+                        (void*)key, (void*)cells, (void*)cursor, (void*)order})
+            (void)hipFree(p);
     };
 #define AFAIL(e_, what)                        \
     do {                                       \
@@ -2222,8 +2229,70 @@ int sph_find_overlaps(sph_ctx* c, int sim, uint32_t* pairs, double* overlaps, ui
     } while (0)
     AFAIL(hipMalloc(&counts, (size_t)np * 4 + 4), "hipMalloc");
     AFAIL(hipMalloc(&offs, (size_t)np * 8 + 8), "hipMalloc");
-    AFAIL(launch_overlaps(c->A.x + b0, c->A.vel + b0, np, counts, nullptr, nullptr, nullptr, s),
-          "overlaps");
+
+    // A uniform grid whose cells are at least the largest contact distance
+    // (2 r_max, with a margin for the rounding of the cell coordinates)
+    // holds every overlapping pair within adjacent cells.  The grid is
+    // coarsened until it has at most max(2 np, 4096) cells.  Positions that
+    // are not finite fall back to the all-pairs search.
+    double bounds[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (np >= 2) {
+        AFAIL(hipMalloc(&dbounds, 64), "hipMalloc");
+        AFAIL(launch_overlap_bounds(x, np, dbounds, s), "overlap bounds");
+        AFAIL(hipMemcpyAsync(bounds, dbounds, 64, hipMemcpyDeviceToHost, s), "D2H");
+        AFAIL(hipStreamSynchronize(s), "sync");
+    }
+    const bool use_grid = np >= 2 && bounds[7] == 1.0;
+    OverlapGrid g{};
+    if (use_grid) {
+        double maxabs = 0.0, emax = 0.0, ext[3];
+        for (int d = 0; d < 3; ++d) {
+            maxabs = std::max(maxabs, std::max(std::fabs(bounds[d]), std::fabs(bounds[3 + d])));
+            ext[d] = bounds[3 + d] - bounds[d];
+            emax = std::max(emax, ext[d]);
+            g.lo[d] = bounds[d];
+        }
+        double h = 2.0 * bounds[6] * (1.0 + 1e-6) + 1e-12 * maxabs;
+        if (!(h > 0.0)) h = emax > 0.0 ? emax : 1.0;
+        const uint64_t limit = std::max<uint64_t>(2ull * np, 4096);
+        for (;;) {
+            uint64_t total = 1;
+            for (int d = 0; d < 3; ++d) {
+                g.n[d] = (int)std::min(ext[d] / h, (double)(1 << 20)) + 1;
+                total *= (uint64_t)g.n[d];
+            }
+            if (total <= limit) break;
+            h *= 1.5;
+        }
+        g.inv_h = 1.0 / h;
+        const size_t ncells = (size_t)g.n[0] * g.n[1] * g.n[2];
+        AFAIL(hipMalloc(&key, (size_t)np * 4), "hipMalloc");
+        AFAIL(hipMalloc(&order, (size_t)np * 4), "hipMalloc");
+        AFAIL(hipMalloc(&cells, (ncells + 1) * 4), "hipMalloc");
+        AFAIL(hipMalloc(&cursor, ncells * 4), "hipMalloc");
+        AFAIL(hipMemsetAsync(cursor, 0, ncells * 4, s), "memset");
+        AFAIL(launch_overlap_bin(x, np, g, key, cursor, s), "overlap bin");
+        std::vector<uint32_t> start(ncells + 1);
+        AFAIL(hipMemcpyAsync(start.data(), cursor, ncells * 4, hipMemcpyDeviceToHost, s), "D2H");
+        AFAIL(hipStreamSynchronize(s), "sync");
+        uint32_t run = 0;
+        for (size_t k = 0; k < ncells; ++k) {
+            const uint32_t n = start[k];
+            start[k] = run;
+            run += n;
+        }
+        start[ncells] = run;
+        AFAIL(hipMemcpyAsync(cells, start.data(), (ncells + 1) * 4, hipMemcpyHostToDevice, s),
+              "H2D");
+        AFAIL(hipMemcpyAsync(cursor, cells, ncells * 4, hipMemcpyDeviceToDevice, s), "D2D");
+        AFAIL(launch_overlap_scatter(key, np, cursor, order, s), "overlap scatter");
+        AFAIL(hipStreamSynchronize(s), "sync");
+    }
+    auto search = [&](const uint64_t* o, uint32_t* p, double* dl) {
+        return use_grid ? launch_overlaps_grid(x, vel, np, g, cells, order, counts, o, p, dl, s)
+                        : launch_overlaps(x, vel, np, counts, o, p, dl, s);
+    };
+    AFAIL(search(nullptr, nullptr, nullptr), "overlaps");
     std::vector<uint32_t> hc(np);
     std::vector<uint64_t> ho(np);
     AFAIL(hipMemcpyAsync(hc.data(), counts, (size_t)np * 4, hipMemcpyDeviceToHost, s), "D2H");
@@ -2239,8 +2308,7 @@ int sph_find_overlaps(sph_ctx* c, int sim, uint32_t* pairs, double* overlaps, ui
         AFAIL(hipMemcpyAsync(offs, ho.data(), (size_t)np * 8, hipMemcpyHostToDevice, s), "H2D");
         AFAIL(hipMalloc(&dpairs, total * 8), "hipMalloc");
         AFAIL(hipMalloc(&ddelta, total * 8), "hipMalloc");
-        AFAIL(launch_overlaps(c->A.x + b0, c->A.vel + b0, np, counts, offs, dpairs, ddelta, s),
-              "overlaps");
+        AFAIL(search(offs, dpairs, ddelta), "overlaps");
         AFAIL(hipMemcpyAsync(pairs, dpairs, take * 8, hipMemcpyDeviceToHost, s), "D2H");
         AFAIL(hipMemcpyAsync(overlaps, ddelta, take * 8, hipMemcpyDeviceToHost, s), "D2H");
     }

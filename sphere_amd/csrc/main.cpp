@@ -2,15 +2,17 @@
 //
 // Flags, messages and exit statuses follow src/main.cpp:28-223.  Like the
 // reference it is run from the project root: `./sphere input/<sid>.bin`
-// writes output/<sid>.output%05d.bin and output/<sid>.status.dat.  Fluid
-// (-f), rendering (-r, -m) and the contact listing (-c) belong to parts of
-// the reference this engine does not carry; they exit with status 1.
+// writes output/<sid>.output%05d.bin and output/<sid>.status.dat; `-c`
+// prints the particle overlaps of the input file instead.  Fluid (-f) and
+// rendering (-r, -m) belong to parts of the reference this engine does not
+// carry; they exit with status 1.
 
 #include <stdio.h>
 #include <stdlib.h>
 
 #include <iostream>
 #include <string>
+#include <vector>
 
 #include "../../include/sphere_hip.h"
 
@@ -44,7 +46,7 @@ static void report_values(sph_ctx* c) {
 }
 
 int main(int argc, const char* argv[]) {
-    int verbose = 1, check = 1, dry = 0, device = -1, nfiles = 0;
+    int verbose = 1, check = 1, dry = 0, device = -1, nfiles = 0, contacts = 0;
     for (int i = 1; i < argc; ++i) {
         const std::string a = argv[i];
         if (a == "-h" || a == "--help") {
@@ -57,6 +59,7 @@ int main(int argc, const char* argv[]) {
                          "-d <device>\t\texecute on device with specified id\n"
                          "-n, --dry\t\tshow key experiment parameters and quit\n"
                          "-dc, --dont-check\tdon't check values before running\n"
+                         "-c, --contacts\t\tlist particle overlaps (i, j, overlap) and quit\n"
                       << std::endl;
             return 0;
         } else if (a == "-V" || a == "--version") {
@@ -80,19 +83,41 @@ int main(int argc, const char* argv[]) {
                 return 1;
             }
             ++i;
+        } else if (a == "-c" || a == "--contacts") {
+            contacts = 1;
         } else if (a == "-f" || a == "--fluid" || a == "-r" || a == "--render" || a == "-m" ||
-                   a == "--method" || a == "-c" || a == "--contacts") {
+                   a == "--method") {
             std::cerr << "Error: option " << a << " is not supported by this engine." << std::endl;
             return 1;
         } else {
             ++nfiles;
             if (verbose) std::cout << argv[0] << ": processing input file: " << a << std::endl;
-            unsigned flags = (verbose ? 0u : SPH_OPEN_QUIET) | (check ? 0u : SPH_OPEN_NOCHECK);
+            // `--contacts` opens without the input checks and prints every
+            // overlap, as DEM::printContacts (src/sphere.cpp:904-913).
+            unsigned flags = (verbose ? 0u : SPH_OPEN_QUIET) |
+                             (check && !contacts ? 0u : SPH_OPEN_NOCHECK);
             sph_ctx* c = nullptr;
             int rc = sph_open(a.c_str(), device, flags, &c);
             if (rc) {
                 std::cerr << sph_open_error() << std::endl;
                 return 1;
+            }
+            if (contacts) {
+                uint64_t n = 0;
+                rc = sph_find_overlaps(c, 0, nullptr, nullptr, 0, &n);
+                std::vector<uint32_t> pairs(2 * n);
+                std::vector<double> delta(n);
+                if (!rc) rc = sph_find_overlaps(c, 0, pairs.data(), delta.data(), n, &n);
+                if (rc) {
+                    std::cerr << sph_last_error(c) << std::endl;
+                    sph_close(c);
+                    return 1;
+                }
+                for (uint64_t k = 0; k < n; ++k)
+                    std::cout << pairs[2 * k] << '\t' << pairs[2 * k + 1] << '\t' << delta[k]
+                              << std::endl;
+                sph_close(c);
+                return 0;
             }
             if (dry) {
                 report_values(c);

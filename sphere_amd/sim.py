@@ -816,10 +816,76 @@ class sim:
             return force[0] / (self.L[0] * self.L[1])
         raise Exception("Shear stress type " + type + " not understood")
 
-    def findOverlaps(self):
-        """Pairwise overlaps (python/sphere.py:4868-4883 via DEM::findOverlaps).
-        Returns (pairs (n,2) with i<j, overlaps, distances) from a cell search."""
-        return find_overlaps(self)
+    def findOverlaps(self, device=-1):
+        """Pairwise overlaps (python/sphere.py:4868-4883): DEM::findOverlaps on
+        the GPU (sph_find_overlaps; the reference runs `sphere --contacts`
+        and reads its printout back).  Sets ``self.pairs`` (2 x k, int32,
+        i < j) and ``self.overlaps`` (k), as the reference does."""
+        from . import engine
+        with engine.Context(self.tobytes(), device=device, check=False) as ctx:
+            pairs, overlaps = ctx.find_overlaps()
+        self.pairs = numpy.ascontiguousarray(pairs.T.astype(numpy.int32))
+        self.overlaps = overlaps
+
+    def porosity(self, slices=10, verbose=False, device=-1):
+        """Porosity in horizontal slabs (python/sphere.py:4437-4480):
+        DEM::porosity on the GPU (sph_porosity) where the reference runs
+        its `porosity` program.  Returns (porosities, slab depths)."""
+        from . import engine
+        with engine.Context(self.tobytes(), device=device, check=False) as ctx:
+            depth, por = ctx.porosity(int(slices))
+        return por, depth
+
+    # Consumers of findOverlaps' pair list (python/sphere.py:4885-5017).
+    def findCoordinationNumber(self):
+        """Contacts per particle from self.pairs."""
+        self.coordinationnumber = numpy.bincount(
+            self.pairs.reshape(-1).astype(numpy.int64), minlength=int(self.np[0]))
+        return self.coordinationnumber
+
+    def findMeanCoordinationNumber(self):
+        return numpy.mean(self.coordinationnumber)
+
+    def findNormalForces(self):
+        """Elastic normal force magnitude k_n |delta| per overlapping pair."""
+        self.findOverlaps()
+        self.f_n_magn = self.k_n * numpy.abs(self.overlaps)
+
+    def contactSurfaceArea(self, i, j, overlap):
+        """Area of the lens where spheres i and j (centre distance
+        r_i + r_j + overlap) intersect: pi a^2, with a the radius of the
+        intersection circle from the triangle of sides d, r_i, r_j."""
+        ri = self.radius[i]
+        rj = self.radius[j]
+        d = ri + rj + overlap
+        a = numpy.sqrt((-d + ri - rj) * (-d - ri + rj) * (-d + ri + rj) * (d + ri + rj)) / (2.0 * d)
+        return numpy.pi * a ** 2
+
+    def contactParticleArea(self, i, j):
+        """Cross-section of the pair's mean radius."""
+        return numpy.pi * ((self.radius[i] + self.radius[j]) * 0.5) ** 2
+
+    def findAllContactSurfaceAreas(self):
+        return self.contactSurfaceArea(self.pairs[0, :], self.pairs[1, :], self.overlaps)
+
+    def findAllAverageParticlePairAreas(self):
+        return self.contactParticleArea(self.pairs[0, :], self.pairs[1, :])
+
+    def findContactStresses(self, area="average"):
+        """Normal stress per pair: k_n |delta| over the mean-radius section
+        ('average') or the lens area ('contact')."""
+        self.findNormalForces()
+        if area == "average":
+            areas = self.findAllAverageParticlePairAreas()
+        elif area == "contact":
+            areas = self.findAllContactSurfaceAreas()
+        else:
+            raise Exception('Contact area type "' + area + '" not understood')
+        self.sigma_contacts = self.f_n_magn / areas
+
+    def findLoadedContacts(self, threshold):
+        self.findContactStresses()
+        return numpy.nonzero(self.sigma_contacts >= threshold)
 
     def dry(self):
         self.fluid = False
@@ -857,43 +923,6 @@ class sim:
 
     def cleanup(self, root=None):
         cleanup(self, root=root)
-
-
-def find_overlaps(s):
-    """All overlapping pairs (i < j) by a uniform-grid search on the host.
-
-    Distances use the FP64 norm; this is an analysis helper, not the step."""
-    x = numpy.asarray(s.x, dtype=numpy.float64)
-    r = numpy.asarray(s.radius, dtype=numpy.float64)
-    n = x.shape[0]
-    if n == 0:
-        return numpy.zeros((0, 2), dtype=numpy.int64), numpy.zeros(0), numpy.zeros(0)
-    cell = 2.0 * r.max()
-    key = numpy.floor((x - x.min(axis=0)) / cell).astype(numpy.int64)
-    from collections import defaultdict
-    buckets = defaultdict(list)
-    for i, k in enumerate(map(tuple, key)):
-        buckets[k].append(i)
-    pairs, ov, dist = [], [], []
-    for k, members in buckets.items():
-        cand = []
-        for dz in (-1, 0, 1):
-            for dy in (-1, 0, 1):
-                for dx in (-1, 0, 1):
-                    cand.extend(buckets.get((k[0] + dx, k[1] + dy, k[2] + dz), ()))
-        cand = numpy.array(sorted(cand))
-        for i in members:
-            c = cand[cand > i]
-            if c.size == 0:
-                continue
-            d = numpy.linalg.norm(x[c] - x[i], axis=1)
-            o = d - (r[c] + r[i])
-            m = o < 0.0
-            for j, oo, dd in zip(c[m], o[m], d[m]):
-                pairs.append((i, j))
-                ov.append(oo)
-                dist.append(dd)
-    return numpy.array(pairs, dtype=numpy.int64).reshape(-1, 2), numpy.array(ov), numpy.array(dist)
 
 
 def status(project, root=None):

@@ -112,6 +112,20 @@ def test_cli_help_version_and_missing_input():
     assert r.returncode == 1 and "Could not read input binary file" in r.stderr
 
 
+def test_forcechains_help_version_and_missing_input():
+    """src/forcechains.cpp:16-103: help and version exit 0, no input file
+    exits 1 with the reference's message."""
+    exe = os.path.join(ROOT, "sphere_amd", "bin", "forcechains")
+    r = subprocess.run([exe, "--help"], capture_output=True, text=True)
+    assert r.returncode == 0 and "force chain visualizer" in r.stdout
+    r = subprocess.run([exe, "-V"], capture_output=True, text=True)
+    assert r.returncode == 0 and r.stdout == "Force chain calculator, sphere version 2.15\n"
+    r = subprocess.run([exe, "-f", "txt"], capture_output=True, text=True)
+    assert r.returncode == 1 and "missing input binary file" in r.stderr
+    r = subprocess.run([exe, "-lc"], capture_output=True, text=True)
+    assert r.returncode == 1 and "needs a value" in r.stderr
+
+
 def test_reference_fixture_decodes_and_upgrades():
     """output/cube-init.output00029.bin (v2.11): header values and the
     upgrade to a v2.15 image of the expected size."""
