@@ -1451,10 +1451,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const double4 xa = xs[ia];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
-    const double4 vel_a = vs[ia];
-    const double4 av_a = ws[ia];
-    const d3 angvel_a = xyz(av_a);
-    tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
     // The row passes through LDS: read and written as contiguous words per
     // lane, held LPP-strided in registers during the search, and back in
     // LDS (with the partners' sorted indices) for the interaction.
@@ -1463,6 +1459,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     uint32_t* gs = s_slot + (threadIdx.x / LPP) * NC;
     uint32_t* gr = s_srt + (threadIdx.x / LPP) * NC;
     uint32_t* const crow_a = a.crow + (size_t)ia_orig * NC;
+    // The list length is read ahead of the row staging, so that the list
+    // entries' loads need not wait for it.
+    const bool rebuild =
+        !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
+    const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
     copy_words<SPL>(gs + g.q * SPL, crow_a + g.q * SPL);
     wave_sync();
     uint32_t slot[SPL];
@@ -1477,9 +1478,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
 
     // topology: the candidate sequence of the 27-cell scan, LPP at a time.
     int nc = 0;
-    const bool rebuild =
-        !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
-    const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
     const uint32_t below = (1u << g.q) - 1u;
     if (rebuild || vcnt == VL_OVERFLOW) {
         const bool build = p.vl_enabled && rebuild;
@@ -1602,6 +1600,14 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         }
     }
 
+    // The particle's velocities are loaded only now: the candidate tests
+    // above hold many positions in flight, and registers are what bound the
+    // number of wavefronts per SIMD here.
+    const double4 vel_a = vs[ia];
+    const double4 av_a = ws[ia];
+    const d3 angvel_a = xyz(av_a);
+    tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
+
     // interact (src/contactsearch.cuh:484-559): the occupied slots, LPP at a
     // time in ascending slot order, each on its own lane; contributions are
     // then added in slot order on every lane of the group.
@@ -1651,16 +1657,19 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             const uint32_t ib_orig = wrd & CW_IDX_MASK;
             const d3 dm = shift_of(p, word_code(wrd));
             const uint32_t rb_s = sib != NO_SORTED ? sib : a.rank[ib_orig];
+            // A slot's contact has almost always persisted: its partner's
+            // velocities and the displacement are loaded with its position.
             const double4 xb = xs[rb_s];
+            const double4 vel_b = vs[rb_s];
+            const double4 av_b = ws[rb_s];
+            const double4 dt0 = a.delta_t[mp];
             const double rb = xb.w;
             const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
             const double len = length(xab);
             const double dn = len - (ra + rb);
             if (dn < 0.0) {
                 touching = true;
-                const double4 vel_b = vs[rb_s];
-                const d3 angvel_b = xyz(ws[rb_s]);
-                const double4 dt0 = a.delta_t[mp];
+                const d3 angvel_b = xyz(av_b);
                 if (a.tape_dt0) {
                     a.tape_dt0[mp] = dt0;
                     a.tape_srt[mp] = rb_s;
@@ -2018,36 +2027,42 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     const DevParams& p = a.ptab[blockIdx.x];  // one workgroup per simulation
     DevLoop* lp = a.loop + p.sim;
     if (threadIdx.x == 0) do_sort = (resort && lp->changed[step_slot(lp) ^ 1u]) ? 1u : 0u;
+    __shared__ double s_fsum[MAXWALLS];
     const bool walls = p.nw > 0 && p.np > 0;
+    // Work-item w integrates wall w (src/integration.cuh:344-450).  Its
+    // state, the clock and the normal load are read and formed before the
+    // force sums, so that their latency hides behind the sums'.
+    const uint32_t w = threadIdx.x;
+    const bool mine = walls && w < p.nw && p.wmode[w] != 0;
+    const double t_now = lp->t;
+    const int64_t iter = lp->iter;
+    const double dt = p.dt;
+    double4 w_nx = make_double4(0.0, 0.0, 0.0, 0.0), w_mvfd = w_nx;
+    double acc0 = 0.0, N = 0.0;
+    if (mine) {
+        w_nx = a.w_nx[p.wall_base + w];
+        w_mvfd = a.w_mvfd[p.wall_base + w];
+        if (iter != 0) acc0 = a.w_acc[p.wall_base + w];
+        const double sigma_0 = w_mvfd.w + p.sigma0_A * sin(2.0 * PI * p.sigma0_f * t_now);
+        N = -sigma_0 * p.L[0] * p.L[1];
+    }
+    const double grid_top = (p.adaptive == 1 && walls && threadIdx.x == 0)
+                                ? a.w_nx[p.wall_base].w : 0.0;
     // The serial sums of the block partials: wall 0's (every wall integrates
     // it, P8), or with per-wall sums each wall's.
-    double fsum[MAXWALLS];
     const uint32_t nsum = !walls ? 0u : (p.wall_sums ? p.nw : 1u);
-    for (uint32_t w = 0; w < MAXWALLS; ++w)
-        fsum[w] = w < nsum ? serial_sum(a.partial + (size_t)w * a.nchunk + p.chunk_base,
-                                        p.nblocks, stage)
-                           : 0.0;
+    for (uint32_t k = 0; k < nsum; ++k) {
+        const double fs = serial_sum(a.partial + (size_t)k * a.nchunk + p.chunk_base, p.nblocks,
+                                     stage);
+        if (threadIdx.x == 0) s_fsum[k] = fs;
+    }
     __syncthreads();
     if (do_sort) resort_one_block(p, a, passes, sorted_bits);
-    if (threadIdx.x != 0) return;
-    if (do_sort) lp->sorts += 1u;
-    lp->resorted_last = do_sort;
-    const double t_now = lp->t;
-    const double dt = p.dt;
-    if (p.adaptive == 1 && p.nw > 0 && p.np > 0) lp->grid_Lz = a.w_nx[p.wall_base].w;
-    for (uint32_t w = 0; walls && w < p.nw; ++w) {
-        const int wmode = p.wmode[w];
-        if (wmode == 0) continue;
+    if (mine) {
         const uint32_t wi = p.wall_base + w;
-        double4 w_nx = a.w_nx[wi];
-        double4 w_mvfd = a.w_mvfd[wi];
-        double acc0;
-        if (lp->iter == 0) acc0 = 0.0; else acc0 = a.w_acc[wi];
-        w_mvfd.z = p.wall_sums ? fsum[w] : fsum[0];  // 0.0 + partial[0] + partial[1] + ...
-        const double sigma_0 = w_mvfd.w + p.sigma0_A * sin(2.0 * PI * p.sigma0_f * t_now);
-        const double N = -sigma_0 * p.L[0] * p.L[1];
+        w_mvfd.z = s_fsum[p.wall_sums ? w : 0u];  // 0.0 + partial[0] + partial[1] + ...
         double acc = (w_mvfd.z + N) / w_mvfd.x;
-        if (wmode == 2) acc = 0.0;
+        if (p.wmode[w] == 2) acc = 0.0;
         const double dacc_dt = (acc - acc0) / dt;
         w_nx.w += w_mvfd.y * dt + 0.5 * acc * dt * dt + 1.0 / 6.0 * dacc_dt * dt * dt * dt;
         w_mvfd.y += acc * dt + 0.5 * dacc_dt * dt * dt;
@@ -2055,6 +2070,10 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
         a.w_mvfd[wi] = w_mvfd;
         a.w_acc[wi] = acc;
     }
+    if (threadIdx.x != 0) return;
+    if (do_sort) lp->sorts += 1u;
+    lp->resorted_last = do_sort;
+    if (p.adaptive == 1 && walls) lp->grid_Lz = grid_top;
     if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
     else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
     // Retire this step's flag slot; the next step's slot was filled by the
