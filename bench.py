@@ -328,15 +328,15 @@ def main():
                           "launches": 5}), flush=True)
         return 0
 
-    if args.batch:
-        return batch_bench(args)
-
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py: this engine runs the DEM step on one GPU; --gpus %d is not supported "
-              "(see DESIGN.md for how the path would shard)." % max(args.gpus, world),
-              file=sys.stderr)
+        print("bench.py: this engine runs the DEM step on one GPU; --gpus %d is not supported: "
+              "a batch of simulations or one large bed is measured on a single GPU."
+              % max(args.gpus, world), file=sys.stderr)
         return 2
+
+    if args.batch:
+        return batch_bench(args)
 
     from sphere_amd import engine, scenes
 

@@ -49,3 +49,63 @@ def crowded_case(n_small=44, seed=9):
     s.defineWorldBoundaries(L=[6.0, 6.0, 6.0])
     s.initTemporal(total=1.0, file_dt=0.1)
     return s
+
+
+def lattice_bed(n, seed, median, sigma, k_n=1.16e7, normal_stress=10.0e3, shear_vel=1.0e-3,
+                bond_frac=0.0, aspect=2.0, rho=2600.0, mu=0.5):
+    """A direct-shear bed of any size, built in O(n): lognormal radii
+    (median, sigma), clipped at 0.55 of the lattice spacing 2 median, on a
+    jittered cubic lattice periodic in x and y, so that a particle touches at
+    most its six lattice neighbours (about half of them do); the top wall at
+    `normal_stress` (consolidate) and the top layer sheared at `shear_vel`
+    (shear).  `bond_frac` of the touching lattice pairs are bonded, breaking
+    at a stress of k_n * 0.05 * median (a bond stretched by 5% of a median
+    radius)."""
+    import math
+    rng = numpy.random.default_rng(seed)
+    d = 2.0 * median
+    r = numpy.minimum(scenes.lognormal_radii(n, median, sigma, rng), 0.55 * d)
+    nxy = int(math.ceil((n * aspect) ** (1.0 / 3.0)))
+    nz = int(math.ceil(n / float(nxy * nxy)))
+    i = numpy.arange(n)
+    ix, iy, iz = i % nxy, (i // nxy) % nxy, i // (nxy * nxy)
+    x = numpy.stack([ix, iy, iz], axis=1) * d + 0.5 * d + 0.02 * d * rng.uniform(-1.0, 1.0,
+                                                                                (n, 3))
+    s = sim(np=n, sid="bed-%d" % n)
+    s.radius = r
+    s.x = x
+    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
+    scenes._set_contact_params(s, k_n, 0.5, mu, float(r.min()))
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.periodicBoundariesXY()
+    s.defineWorldBoundaries(L=[nxy * d, nxy * d, (nz + 1) * d])
+    dt = scenes.dt_rule(float(r.min()), rho, k_n)
+    s.time_dt[0] = dt
+    s.time_current[0] = 0.0
+    s.time_total[0] = 1.0e3
+    s.time_file_dt[0] = 1.0e3
+    s.consolidate(normal_stress)
+    z_min = float(numpy.min(s.x[:, 2] - s.radius))
+    z_max = float(numpy.max(s.x[:, 2] + s.radius))
+    s.shear(shear_strain_rate=shear_vel / (z_max - z_min))
+    if bond_frac > 0.0:
+        pairs = []
+        for step, ok in ((1, ix < nxy - 1), (nxy, iy < nxy - 1), (nxy * nxy, iz < nz - 1)):
+            a = i[ok & (i + step < n)]
+            b = a + step
+            dist = numpy.linalg.norm(s.x[b] - s.x[a], axis=1)
+            pairs.append(numpy.stack([a, b], axis=1)[dist < r[a] + r[b]])
+        pairs = numpy.concatenate(pairs)
+        pick = numpy.sort(rng.choice(len(pairs), size=max(1, int(bond_frac * len(pairs))),
+                                     replace=False))
+        nb = len(pick)
+        s.bonds = pairs[pick].astype(numpy.uint32)
+        s.nb0[0] = nb
+        s.lambda_bar[0] = 1.0
+        s.bonds_delta_n = numpy.zeros(nb)
+        s.bonds_delta_t = numpy.zeros((nb, 3))
+        s.bonds_omega_n = numpy.zeros(nb)
+        s.bonds_omega_t = numpy.zeros((nb, 3))
+        s.sigma_b[0] = k_n * 0.05 * median
+        s.tau_b[0] = k_n * 0.05 * median
+    return s

@@ -138,3 +138,24 @@ def test_batch_shear_cells_with_own_stiffness_stay_stable():
         x = o.get("x")[:, :3]
         assert numpy.isfinite(x).all()
         assert (x[:, 2] > 0.0).all() and (x[:, 2] < float(c.L[2])).all()
+
+
+def test_lattice_bed_is_a_sheared_bonded_bed():
+    """The full-size bed builder at a small size: every bond joins two
+    touching lattice neighbours, radii stay below 0.55 of the spacing, the
+    top wall is stress-controlled and the top layer moves."""
+    s = cases.lattice_bed(6000, seed=3, median=0.005, sigma=0.25, k_n=1.0e8,
+                          normal_stress=20.0e3, shear_vel=5.0e-4, bond_frac=0.1)
+    n = int(s.np[0])
+    assert n == 6000 and int(s.periodic[0]) == 1 and list(s.wmode) == [1]
+    assert float(s.w_sigma0[0]) == 20.0e3
+    assert s.radius.max() <= 0.55 * 0.01
+    nb = int(s.nb0[0])
+    assert nb == len(s.bonds) > 50
+    a, b = s.bonds[:, 0].astype(int), s.bonds[:, 1].astype(int)
+    dist = numpy.linalg.norm(s.x[b] - s.x[a], axis=1)
+    assert numpy.all(dist < s.radius[a] + s.radius[b])
+    assert numpy.all(numpy.isin(b - a, [1, 23, 23 * 23]))  # lattice neighbours (23 per row)
+    top = s.fixvel > 0
+    assert numpy.any(s.vel[top, 0] > 0.0) and numpy.all(s.vel[~top] == 0.0)
+    assert s.sigma_b[0] == s.tau_b[0] == 1.0e8 * 0.05 * 0.005

@@ -1,0 +1,85 @@
+"""The full-size workloads: a 1e5-particle direct-shear bed (the benchmark's
+size and stiffness) and a 1e6-particle bonded shear bed (2% of the touching
+pairs bonded, 20 kPa, 5e-4 m/s), stepped by the engine and by the oracle
+from the same state and compared bit for bit; then the properties that do
+not depend on size, at full size: the same result for every contact lane
+count, with graph replay, and for a batch holding the bed twice."""
+
+import numpy
+import pytest
+
+import oracle
+from sphere_amd import engine
+
+from . import cases
+from .test_gpu_parity import EXACT_FIELDS, _compare
+
+pytestmark = pytest.mark.gpu
+
+WALL_FIELDS = ("walls_nx", "walls_mvfd", "walls_acc", "walls_force_pp")
+STATE = ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t", "es", "ev", "p")
+
+
+def _same_state(a, b, fields=STATE):
+    return [f for f in fields if not numpy.array_equal(a.get(f), b.get(f))]
+
+
+def test_shear_bed_1e5_bit_exact():
+    s = cases.lattice_bed(100000, seed=2001, median=0.01, sigma=0.2, k_n=1.16e7,
+                          normal_stress=10.0e3, shear_vel=1.0e-3)
+    image = s.tobytes()
+    steps = 20
+    print("1e5 bed: %d particles, %d steps" % (int(s.np[0]), steps), flush=True)
+    ref = oracle.Oracle(image)
+    ref.step(steps)
+    ctx = engine.Context(image)
+    ctx.step(steps)
+    bad = _compare(ctx, ref) + _compare(ctx, ref, WALL_FIELDS)
+    assert bad == []
+    n = int(s.np[0])
+    assert (ctx.get("contacts") != n).sum() > 2 * n
+    # size-independent: every lane count, and graph replay, give the same
+    for lanes in (1, 8, 16):
+        with engine.Context(image) as other:
+            other.set_scalar("contact_lanes", lanes)
+            other.step(steps)
+            assert _same_state(other, ctx) == [], lanes
+    with engine.Context(image) as other:
+        other.set_scalar("graph_steps", 5)
+        other.step(steps)
+        assert _same_state(other, ctx) == []
+    ctx.close()
+
+
+def test_bonded_shear_bed_1e6_bit_exact():
+    s = cases.lattice_bed(1000000, seed=5001, median=0.005, sigma=0.25, k_n=1.0e8,
+                          normal_stress=20.0e3, shear_vel=5.0e-4, bond_frac=0.02)
+    image = s.tobytes()
+    steps = 4
+    print("1e6 bonded bed: %d particles, %d bonds, %d steps" % (int(s.np[0]), int(s.nb0[0]),
+                                                                 steps), flush=True)
+    ref = oracle.Oracle(image)
+    ref.step(steps)
+    ctx = engine.Context(image)
+    ctx.step(steps)
+    bad = _compare(ctx, ref) + _compare(ctx, ref, WALL_FIELDS)
+    bad += _compare(ctx, ref, ("bonds", "bonds_delta", "bonds_omega"))
+    assert bad == []
+    n = int(s.np[0])
+    assert (ctx.get("contacts") != n).sum() > 2 * n
+    assert int(s.nb0[0]) > 20000
+    ctx.close()
+
+
+def test_batch_of_two_full_beds_matches_single():
+    """A batch holding the 1e5 bed twice steps each copy exactly as a
+    context of its own."""
+    s = cases.lattice_bed(100000, seed=2001, median=0.01, sigma=0.2, k_n=1.16e7,
+                          normal_stress=10.0e3, shear_vel=1.0e-3)
+    image = s.tobytes()
+    with engine.Context(image) as one, engine.Context(images=[image, image]) as two:
+        one.step(10)
+        two.step(10)
+        for k in (0, 1):
+            for f in ("x", "vel", "angvel", "force", "contacts"):
+                assert numpy.array_equal(two.get(f, sim=k), one.get(f)), (k, f)
