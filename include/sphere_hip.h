@@ -232,6 +232,19 @@ int sph_adjoint(sph_ctx* ctx, uint64_t nsteps, const double* dL_dx, const double
                 const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
                 double* g_params);
 
+/* Adjoint of a trajectory loss; no counterpart in the reference.  Runs
+ * every * nsamples steps forward from the current state like sph_adjoint
+ * and returns the gradient of
+ *     L = sum_k sum_i dL_dx[k][i] . x_i(k) + dL_dv[k][i] . v_i(k)
+ * where x(k), v(k) is the state after (k + 1) * every steps; dL_dx and dL_dv
+ * are nsamples x np x 3 (either may be NULL).  Each term enters the reverse
+ * pass at its step, so the cotangents of the accelerations and of the
+ * contact displacements carry across the sampling times.  Same coverage,
+ * outputs and final state as sph_adjoint. */
+int sph_adjoint_samples(sph_ctx* ctx, uint64_t every, uint64_t nsamples, const double* dL_dx,
+                        const double* dL_dv, double* g_x0, double* g_v0, double* g_w0,
+                        double* g_params);
+
 /* Restart sidecar (no counterpart in the reference: src/file_io.cpp:420-731
  * writes none of this, so a run resumed from an output .bin starts with
  * empty contact rows, zero displacements and acc0 = 0).  Holds the contact

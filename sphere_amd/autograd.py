@@ -1,0 +1,143 @@
+"""A DEM trajectory as a differentiable torch function.
+
+`Trajectory(s, every, nsamples)` maps contact parameters (and optionally the
+initial positions and velocities) to the particle positions sampled every
+`every` steps, nsamples times, running the HIP engine forward; its backward
+is the engine's adjoint of a trajectory loss (sph_adjoint_samples), so that
+any torch loss on the sampled positions has gradients with respect to k_n,
+gamma_n, mu, x0 and v0.  No counterpart in the reference.
+
+The forward pass keeps one checkpoint of the whole simulation state at its
+start (the .bin image and the restart sidecar: contact rows,
+displacements, accelerations, loop counter).  The backward pass restores it
+and runs sph_adjoint_samples over the whole run with the loss gradient of
+every sample: each enters the reverse pass at its step, so the cotangents
+of the accelerations and contact displacements carry across sampling
+times.  The engine records the run in segments from checkpoints of its own
+when the whole tape does not fit (SPH_S_ADJ_CHECKPOINT sets the segment
+length).  Lengths are differentiated as exact roots and the contact list and
+branches of each step are frozen, as in sph_adjoint.
+"""
+
+import os
+import shutil
+import tempfile
+
+import numpy
+import torch
+
+from . import engine
+
+# How each named parameter enters the simulation: a value sets every listed
+# field of the .bin, and its gradient is the sum of theirs.  The contact
+# laws take k_t = k_n and mu_s = mu_d in the workloads here.
+TIES = {
+    "k_n": ("k_n", "k_t"),
+    "k_t": ("k_t",),
+    "gamma_n": ("gamma_n",),
+    "gamma_t": ("gamma_t",),
+    "mu": ("mu_s", "mu_d"),
+    "mu_s": ("mu_s",),
+    "mu_d": ("mu_d",),
+}
+
+
+class _Start:
+    """The forward pass's starting state; its sidecar file goes with it."""
+
+    def __init__(self, ctx):
+        self.dir = tempfile.mkdtemp(prefix="sphere-traj-")
+        self.state = os.path.join(self.dir, "start.state")
+        ctx.write_state(self.state)
+        self.image = ctx.tobytes()
+
+    def __del__(self):
+        shutil.rmtree(self.dir, ignore_errors=True)
+
+
+class Trajectory:
+    """Positions sampled along a run of the engine, differentiable.
+
+    `s` is a sphere_amd.sim.sim holding the initial state and the other
+    parameters; `params` names the differentiable parameters (keys of
+    `ties`).  Calling the object with a 1-D tensor theta (one value per
+    name) and optional x0, v0 (np x 3 tensors) returns a float64 tensor of
+    shape (nsamples, np, 3): positions after every, 2 every, ... steps."""
+
+    def __init__(self, s, every, nsamples, params=("k_n", "gamma_n", "mu"), ties=None,
+                 device=-1):
+        if every <= 0 or nsamples <= 0:
+            raise ValueError("every and nsamples must be positive")
+        self.template = s
+        self.every = int(every)
+        self.nsamples = int(nsamples)
+        self.params = tuple(params)
+        self.ties = dict(TIES if ties is None else ties)
+        for p in self.params:
+            if p not in self.ties:
+                raise ValueError("unknown parameter %r" % p)
+        self.device = device
+        self.np = int(s.np[0])
+
+    def image(self, theta, x0=None, v0=None):
+        """The .bin image of the template with theta (and x0, v0) applied."""
+        from .sim import sim
+        s = sim().frombytes(self.template.tobytes())
+        for name, value in zip(self.params, numpy.asarray(theta, dtype=numpy.float64)):
+            for field in self.ties[name]:
+                getattr(s, field)[0] = float(value)
+        if x0 is not None:
+            s.x[:, :3] = numpy.asarray(x0, dtype=numpy.float64).reshape(self.np, 3)
+        if v0 is not None:
+            s.vel[:, :3] = numpy.asarray(v0, dtype=numpy.float64).reshape(self.np, 3)
+        return s.tobytes()
+
+    def __call__(self, theta, x0=None, v0=None):
+        if x0 is None:
+            x0 = torch.as_tensor(self.template.x[:, :3], dtype=torch.float64)
+        if v0 is None:
+            v0 = torch.as_tensor(self.template.vel[:, :3], dtype=torch.float64)
+        return _TrajectoryFn.apply(theta, x0, v0, self)
+
+    # -- the two passes ---------------------------------------------------
+    def run_forward(self, theta, x0, v0):
+        """Samples (nsamples, np, 3) and the start state for run_backward."""
+        samples = numpy.zeros((self.nsamples, self.np, 3))
+        ctx = engine.Context(self.image(theta, x0, v0), device=self.device)
+        try:
+            start = _Start(ctx)
+            for k in range(self.nsamples):
+                ctx.step(self.every)
+                samples[k] = ctx.get("x")[:, :3]
+        finally:
+            ctx.close()
+        return samples, start
+
+    def run_backward(self, start, grad_samples):
+        """Gradients (theta, x0, v0) of sum_k grad_samples[k] . sample_k."""
+        ctx = engine.Context(start.image, device=self.device)
+        try:
+            ctx.read_state(start.state)
+            g = ctx.adjoint_samples(self.every, self.nsamples, grad_samples, None)
+        finally:
+            ctx.close()
+        gtheta = numpy.array([sum(g[f] for f in self.ties[name]) for name in self.params])
+        return gtheta, g["x"], g["v"]
+
+
+class _TrajectoryFn(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, theta, x0, v0, traj):
+        samples, saved = traj.run_forward(theta.detach().cpu().numpy(),
+                                          x0.detach().cpu().numpy(), v0.detach().cpu().numpy())
+        fctx.traj = traj
+        fctx.saved = saved
+        fctx.dev = theta.device
+        return torch.from_numpy(samples).to(fctx.dev)
+
+    @staticmethod
+    def backward(fctx, grad):
+        gtheta, gx, gv = fctx.traj.run_backward(fctx.saved, grad.detach().cpu().numpy())
+        dev = fctx.dev
+        return (torch.from_numpy(gtheta).to(dev), torch.from_numpy(gx).to(dev),
+                torch.from_numpy(gv).to(dev), None)

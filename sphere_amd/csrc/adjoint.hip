@@ -525,7 +525,25 @@ __global__ __launch_bounds__(256) void k_adj_pack(double* dst, const double4* sr
     }
 }
 
+// dst[i].xyz += src[3 i .. 3 i + 2] (a loss term on an intermediate state).
+__global__ __launch_bounds__(256) void k_adj_accumulate(double4* dst, const double* src,
+                                                        uint32_t n) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        double4 v = dst[i];
+        v.x += src[3 * i];
+        v.y += src[3 * i + 1];
+        v.z += src[3 * i + 2];
+        dst[i] = v;
+    }
+}
+
 }  // namespace
+
+hipError_t launch_adj_accumulate(double4* dst, const double* src, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_adj_expand(double4* dst, const double* src, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_adj_expand, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);

@@ -61,6 +61,8 @@ struct AdjState {
 // (n x 3) doubles <-> n double4 (w = 0).
 hipError_t launch_adj_expand(double4* dst, const double* src, uint32_t n, hipStream_t s);
 hipError_t launch_adj_pack(double* dst, const double4* src, uint32_t n, hipStream_t s);
+// dst[i].xyz += the i-th row of the (n x 3) src.
+hipError_t launch_adj_accumulate(double4* dst, const double* src, uint32_t n, hipStream_t s);
 hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
                            hipStream_t s);
 

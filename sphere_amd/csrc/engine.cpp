@@ -1872,9 +1872,13 @@ int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_
     return rc;
 }
 
-int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
-                const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
-                double* g_params) {
+// The adjoint of nsteps steps.  Seeds: dL_dx/dv/dw on the final state, and
+// with every > 0 the (nsteps / every) x np x 3 arrays sx / sv on the
+// positions / velocities after every, 2 every, ... steps (either may be null).
+static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
+                       const double* dL_dw, uint64_t every, const double* sx,
+                       const double* sv, double* g_x0, double* g_v0, double* g_w0,
+                       double* g_params) {
     if (!c) return SPH_E_ARG;
     if (c->sims.size() != 1)
         return fail(c, SPH_E_UNSUPPORTED, "the adjoint runs on a single-simulation context");
@@ -2093,6 +2097,19 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
             st.wall_mvfd = r.wmv;
             st.first = (iter0 + (int64_t)gk == 0) ? 1 : 0;
             st.last = gk + 1 == nsteps ? 1 : 0;
+            // A loss term on the state after this step: added to its
+            // cotangent before the step is reversed.
+            if (every > 0 && (gk + 1) % every == 0) {
+                const size_t m = (size_t)((gk + 1) / every - 1);
+                const double* samp[2] = {sx, sv};
+                for (int f = 0; f < 2; ++f) {
+                    if (!samp[f]) continue;
+                    const int rc_ = h2d(c, stage, samp[f] + m * np * 3, np * 24);
+                    if (rc_) return rc_;
+                    hipError_t ea = launch_adj_accumulate(L[cur][f], stage, (uint32_t)np, s);
+                    if (ea != hipSuccess) return hip_fail(c, ea, "sample seed");
+                }
+            }
             for (int f = 0; f < 3; ++f) {
                 g.Lout[f] = L[cur][f];
                 g.Lin[f] = L[cur ^ 1][f];
@@ -2148,6 +2165,24 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
                     asym, NC);
     if (!rc && g_params) rc = d2h(c, g_params, g.gparam, ADJ_NPARAM * 8);
     return rc;
+}
+
+int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
+                const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
+                double* g_params) {
+    return adjoint_run(c, nsteps, dL_dx, dL_dv, dL_dw, 0, nullptr, nullptr, g_x0, g_v0, g_w0,
+                       g_params);
+}
+
+int sph_adjoint_samples(sph_ctx* c, uint64_t every, uint64_t nsamples, const double* dL_dx,
+                        const double* dL_dv, double* g_x0, double* g_v0, double* g_w0,
+                        double* g_params) {
+    if (!c) return SPH_E_ARG;
+    if (every == 0 || nsamples == 0 || nsamples > (uint64_t)1e12 / every)
+        return fail(c, SPH_E_ARG, "sampling every %llu steps, %llu times: both must be "
+                    "positive", (unsigned long long)every, (unsigned long long)nsamples);
+    return adjoint_run(c, every * nsamples, nullptr, nullptr, nullptr, every, dL_dx, dL_dv,
+                       g_x0, g_v0, g_w0, g_params);
 }
 
 int sph_profile(sph_ctx* c, int enable) {

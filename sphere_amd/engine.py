@@ -74,8 +74,8 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_open_error", "sph_version_string", "sph_open_batch", "sph_open_batch_mem",
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
-           "sph_set_scalar_sim", "sph_adjoint", "sph_selftest_sqrt32", "sph_write_state",
-           "sph_read_state", "sph_write_state_sim", "sph_read_state_sim", "sph_find_overlaps",
+           "sph_set_scalar_sim", "sph_adjoint", "sph_adjoint_samples", "sph_selftest_sqrt32",
+           "sph_write_state", "sph_read_state", "sph_write_state_sim", "sph_read_state_sim", "sph_find_overlaps",
            "sph_porosity")
 
 # Parameters of sph_adjoint's g_params, in order.
@@ -149,6 +149,8 @@ def load():
                                      ctypes.POINTER(ctypes.c_double)]
     L.sph_set_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.sph_adjoint.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp, vp]
+    L.sph_adjoint_samples.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp, vp,
+                                      vp]
     L.sph_selftest_sqrt32.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_uint64)]
     _LIB = L
@@ -295,6 +297,30 @@ class Context:
         self._chk(self._L.sph_adjoint(self._h, int(nsteps), ptr[0], ptr[1], ptr[2],
                                       outs[0].ctypes.data, outs[1].ctypes.data,
                                       outs[2].ctypes.data, gp.ctypes.data))
+        res = {"x": outs[0], "v": outs[1], "w": outs[2]}
+        res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp)})
+        return res
+
+    def adjoint_samples(self, every, nsamples, dL_dx=None, dL_dv=None):
+        """Gradient of L = sum_k dL_dx[k] . x(k) + dL_dv[k] . v(k), x(k) and
+        v(k) the state after (k + 1) * every steps (sph_adjoint_samples),
+        with respect to the state now and to the contact parameters.
+        Advances the simulation by every * nsamples steps; returns the same
+        dict as adjoint()."""
+        n = int(self.scalar("np"))
+
+        def seeds(a):
+            if a is None:
+                return None
+            return numpy.ascontiguousarray(a, dtype=numpy.float64).reshape(int(nsamples), n, 3)
+
+        sx, sv = seeds(dL_dx), seeds(dL_dv)
+        outs = [numpy.zeros((n, 3)) for _ in range(3)]
+        gp = numpy.zeros(len(ADJOINT_PARAMS))
+        self._chk(self._L.sph_adjoint_samples(
+            self._h, int(every), int(nsamples), None if sx is None else sx.ctypes.data,
+            None if sv is None else sv.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data,
+            outs[2].ctypes.data, gp.ctypes.data))
         res = {"x": outs[0], "v": outs[1], "w": outs[2]}
         res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp)})
         return res

@@ -170,6 +170,59 @@ def lattice_box(n=5000, seed=1, median=0.005, sigma=0.2, aspect=2.0, k_n=1.16e7,
     return s
 
 
+def collapse_column(n=2000, seed=3001, r_range=(0.008, 0.012), footprint=0.3, k_n=1.0e7,
+                    damping_frac=0.2, mu=0.5, rho=2600.0, v_std=0.01, open_stepper=None,
+                    settle_steps=100000, settle_damping=1.0, spread=3.0):
+    """A granular column released onto a frictional floor: n spheres with
+    radii uniform in `r_range` on a jittered cubic lattice (spacing 2.02
+    r_max, no overlaps) in a footprint x footprint box with frictional side
+    walls, settled under gravity for `settle_steps` steps with
+    `open_stepper` (at `settle_damping` times critical damping) and brought
+    to rest, then placed at the centre of a floor `spread` times wider (the
+    side walls move out: the column is released).  Initial velocities
+    N(0, v_std) per component, seeded.  The contact parameters of the result
+    are k_n = k_t, gamma_n = gamma_t = damping_frac times the critical
+    damping of the smallest pair, mu_s = mu_d = mu.  (2,000 spheres of these
+    radii fill a 0.3 m footprint to about 0.3 m once settled.)"""
+    rng = numpy.random.default_rng(seed)
+    r = rng.uniform(r_range[0], r_range[1], n)
+    d = 2.02 * r.max()
+    m = max(2, int(footprint // d))
+    nz = int(math.ceil(n / float(m * m)))
+    i = numpy.arange(n)
+    ix, iy, iz = i % m, (i // m) % m, i // (m * m)
+    x0 = 0.5 * (footprint - m * d)
+    jit = (d / 2.0 - r.max()) * rng.uniform(-0.9, 0.9, (n, 3))
+    s = sim(np=n, sid="column-%d" % n)
+    s.radius = r
+    s.x = numpy.stack([x0 + (ix + 0.5) * d, x0 + (iy + 0.5) * d, (iz + 0.5) * d], axis=1) + jit
+    s.defaultParams(mu_s=mu, mu_d=mu, rho=rho, k_n=k_n, k_t=k_n)
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.normalBoundariesXY()
+    height = (nz + 1) * d
+    s.defineWorldBoundaries(L=[footprint, footprint, height])
+    dt = dt_rule(float(r.min()), rho, k_n)
+    s.time_dt[0] = dt
+    s.time_current[0] = 0.0
+    s.time_total[0] = 1.0e3
+    s.time_file_dt[0] = 1.0e3
+    if open_stepper is not None and settle_steps > 0:
+        _set_contact_params(s, k_n, settle_damping, mu, float(r.min()))
+        st = open_stepper(s.tobytes())
+        _relax(st, settle_steps, 0)
+        _quench(st)
+        s.frombytes(st.tobytes())
+    _set_contact_params(s, k_n, damping_frac, mu, float(r.min()))
+    wide = spread * footprint
+    s.x[:, 0] += 0.5 * (wide - footprint)
+    s.x[:, 1] += 0.5 * (wide - footprint)
+    s.defineWorldBoundaries(L=[wide, wide, height])
+    s.time_current[0] = 0.0
+    s.vel[:, :3] = rng.normal(0.0, v_std, (n, 3))
+    s.angvel[:] = 0.0
+    return s
+
+
 def batch_sizes(nsims=256, lo=500, hi=20000, seed=7):
     """Log-uniform particle counts in [lo, hi] for a batch of simulations."""
     rng = numpy.random.default_rng(seed)
