@@ -12,6 +12,7 @@ import numpy
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
+_DUAL = None
 
 NC = 32
 
@@ -52,23 +53,23 @@ STAGES = {"hash": 0, "sort": 1, "reorder": 2, "topology": 3, "interact": 4,
           "bonds": 5, "integrate": 6, "walls": 7, "advance": 8}
 
 
-def lib_path():
-    return os.path.join(_HERE, "liboracle.so")
+def lib_path(dual=False):
+    return os.path.join(_HERE, "liboracle_dual.so" if dual else "liboracle.so")
 
 
 def build(force=False):
     """Compile liboracle.so with the committed Makefile."""
     import subprocess
-    if force or not os.path.exists(lib_path()):
+    if force or not os.path.exists(lib_path()) or not os.path.exists(lib_path(True)):
         subprocess.check_call(["make", "-s", "-B" if force else "all"], cwd=_HERE)
 
 
-def _lib():
-    global _LIB
-    if _LIB is None:
-        if not os.path.exists(lib_path()):
+def _lib(dual=False):
+    global _LIB, _DUAL
+    if (_DUAL if dual else _LIB) is None:
+        if not os.path.exists(lib_path(dual)):
             build()
-        L = ctypes.CDLL(lib_path())
+        L = ctypes.CDLL(lib_path(dual))
         L.orc_open_mem.restype = ctypes.c_void_p
         L.orc_open_mem.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p,
                                    ctypes.c_size_t]
@@ -102,8 +103,14 @@ def _lib():
         L.orc_porosity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                    ctypes.c_void_p]
         L.orc_get_threads.restype = ctypes.c_int
-        _LIB = L
-    return _LIB
+        if dual:
+            L.orc_dual_seed_param.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+            L.orc_dual_seed_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+            L.orc_dual_get_tangent.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+            _DUAL = L
+        else:
+            _LIB = L
+    return _DUAL if dual else _LIB
 
 
 def set_threads(n):
@@ -129,8 +136,10 @@ def set_wall_sums_per_wall(on):
 class Oracle:
     """One oracle simulation, created from a v2.15 .bin image."""
 
+    DUAL = False
+
     def __init__(self, image):
-        L = _lib()
+        L = self._L = _lib(self.DUAL)
         buf = ctypes.create_string_buffer(bytes(image), len(image))
         err = ctypes.create_string_buffer(256)
         h = L.orc_open_mem(buf, len(image), err, 256)
@@ -140,7 +149,7 @@ class Oracle:
 
     def close(self):
         if self._h:
-            _lib().orc_close(self._h)
+            self._L.orc_close(self._h)
             self._h = None
 
     def __del__(self):
@@ -151,31 +160,31 @@ class Oracle:
 
     @property
     def np(self):
-        return int(_lib().orc_np(self._h))
+        return int(self._L.orc_np(self._h))
 
     @property
     def time(self):
-        return float(_lib().orc_time(self._h))
+        return float(self._L.orc_time(self._h))
 
     @property
     def iter(self):
-        return int(_lib().orc_iter(self._h))
+        return int(self._L.orc_iter(self._h))
 
     def step(self, n=1):
-        rc = _lib().orc_step(self._h, int(n))
+        rc = self._L.orc_step(self._h, int(n))
         if rc != 0:
             raise RuntimeError("orc_step failed: %d" % rc)
 
     def stage(self, name):
-        rc = _lib().orc_stage(self._h, STAGES[name])
+        rc = self._L.orc_stage(self._h, STAGES[name])
         if rc != 0:
             raise RuntimeError("orc_stage failed: %d" % rc)
 
     def get(self, name):
         fid, dt, shape = FIELDS[name]
-        nbytes = _lib().orc_field_bytes(self._h, fid)
+        nbytes = self._L.orc_field_bytes(self._h, fid)
         out = numpy.empty(nbytes // numpy.dtype(dt).itemsize, dtype=dt)
-        rc = _lib().orc_get_field(self._h, fid, out.ctypes.data, nbytes)
+        rc = self._L.orc_get_field(self._h, fid, out.ctypes.data, nbytes)
         if rc != 0:
             raise RuntimeError("orc_get_field(%s) failed: %d" % (name, rc))
         return out.reshape(shape)
@@ -183,13 +192,13 @@ class Oracle:
     def set(self, name, value):
         fid, dt, shape = FIELDS[name]
         a = numpy.ascontiguousarray(value, dtype=dt)
-        rc = _lib().orc_set_field(self._h, fid, a.ctypes.data, a.nbytes)
+        rc = self._L.orc_set_field(self._h, fid, a.ctypes.data, a.nbytes)
         if rc != 0:
             raise RuntimeError("orc_set_field(%s) failed: %d" % (name, rc))
 
     def find_overlaps(self):
         """DEM::findOverlaps: (pairs (k, 2) i < j, overlaps (k,))."""
-        L = _lib()
+        L = self._L
         n = ctypes.c_uint64()
         L.orc_find_overlaps(self._h, None, None, 0, ctypes.byref(n))
         pairs = numpy.zeros((n.value, 2), dtype=numpy.uint32)
@@ -202,16 +211,45 @@ class Oracle:
         """DEM::porosity: (z positions, porosities) of z_slices slices."""
         z = numpy.zeros(z_slices)
         por = numpy.zeros(z_slices)
-        rc = _lib().orc_porosity(self._h, int(z_slices), z.ctypes.data, por.ctypes.data)
+        rc = self._L.orc_porosity(self._h, int(z_slices), z.ctypes.data, por.ctypes.data)
         if rc:
             raise ValueError("The number of z-slices is too high.")
         return z, por
 
     def tobytes(self):
-        n = _lib().orc_write_mem(self._h, None, 0)
+        n = self._L.orc_write_mem(self._h, None, 0)
         buf = ctypes.create_string_buffer(n)
-        _lib().orc_write_mem(self._h, buf, n)
+        self._L.orc_write_mem(self._h, buf, n)
         return buf.raw
+
+
+class TangentOracle(Oracle):
+    """The oracle's dual-number build (liboracle_dual.so): the same values
+    bit for bit, each carrying its derivative along one seeded direction of
+    the parameters and the initial state.  Reversing the engine's adjoint
+    gives the same directional derivatives (forward mode vs reverse mode)."""
+
+    DUAL = True
+
+    def seed_param(self, name, d=1.0):
+        """Direction component of a parameter (ADJOINT_PARAMS names)."""
+        i = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
+             "mu_wd").index(name)
+        if self._L.orc_dual_seed_param(self._h, i, float(d)) != 0:
+            raise RuntimeError("orc_dual_seed_param failed")
+
+    def seed_field(self, name, direction):
+        """Direction component of the state "x", "vel" or "angvel" (np x 3)."""
+        a = numpy.ascontiguousarray(direction, dtype=numpy.float64)
+        if self._L.orc_dual_seed_field(self._h, FIELDS[name][0], a.ctypes.data) != 0:
+            raise RuntimeError("orc_dual_seed_field failed")
+
+    def tangent(self, name):
+        """Directional derivative of "x", "vel" or "angvel" (np x 3)."""
+        out = numpy.zeros((self.np, 3))
+        if self._L.orc_dual_get_tangent(self._h, FIELDS[name][0], out.ctypes.data) != 0:
+            raise RuntimeError("orc_dual_get_tangent failed")
+        return out
 
 
 def sqrt32(v):

@@ -7,7 +7,10 @@
 //
 // Arithmetic contract (the reference's, made explicit):
 //  * Real is double; no FMA contraction (build with -ffp-contract=off,
-//    matching nvcc --fmad=false in src/CMakeLists.txt:25).
+//    matching nvcc --fmad=false in src/CMakeLists.txt:25).  Built with
+//    -DORACLE_DUAL, Real is the dual number of dual.h (the tangent build:
+//    the same values, each with a directional derivative; the orc_dual_*
+//    functions seed a direction and read the derivatives).
 //  * length(v) = sqrtf(dot(v,v)): the double dot is narrowed to float, the
 //    float square root is correctly rounded, the result widened back
 //    (src/vector_arithmetic.h:317-324).
@@ -44,9 +47,19 @@
 #include <omp.h>
 #endif
 
+#ifdef ORACLE_DUAL
+#include "dual.h"
+#endif
+
 namespace {
 
+#ifdef ORACLE_DUAL
+typedef Dual Real;
+static inline double val(const Dual& x) { return x.v; }
+#else
 typedef double Real;
+#endif
+static inline double val(double x) { return x; }
 static const int NC = SPH_NC;
 static const Real PI = 3.14159265358979;  // src/constants.h:14
 
@@ -72,7 +85,15 @@ static inline R3 cross(R3 a, R3 b) {
     return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
 // length(): FP32 square root of the FP64 dot, src/vector_arithmetic.h:317-324.
+#ifdef ORACLE_DUAL
+// The FP32 value; the derivative of the exact root.
+static inline Real sqrt32(Real v) {
+    const double e = ::sqrt(v.v);
+    return Dual((double)sqrtf((float)v.v), e > 0.0 ? v.d / (2.0 * e) : 0.0);
+}
+#else
 static inline Real sqrt32(Real v) { return (Real)sqrtf((float)v); }
+#endif
 // A smooth variant for checking gradients by finite differences: lengths
 // as FP64 roots (off by default; the FP32 root is the reference's).
 static bool g_exact_length = false;
@@ -154,6 +175,9 @@ struct Writer {
         const unsigned char* q = reinterpret_cast<const unsigned char*>(&v);
         b.insert(b.end(), q, q + sizeof(T));
     }
+#ifdef ORACLE_DUAL
+    void put(const Dual& v) { put(v.v); }
+#endif
 };
 
 bool parse_bin(orc_sim* s, const void* bytes, size_t nbytes, std::string& err) {
@@ -1075,7 +1099,7 @@ void stage_walls(orc_sim* s) {
 
 // DEM::updateGridSize, src/device.cu:363-393 (adaptive grid, nw > 0).
 void stage_adaptive(orc_sim* s) {
-    if (s->adaptive == 1 && s->nw > 0) s->L[2] = s->w_nx[0].w;
+    if (s->adaptive == 1 && s->nw > 0) s->L[2] = val(s->w_nx[0].w);
 }
 
 void advance(orc_sim* s) {
@@ -1126,9 +1150,33 @@ int64_t field_bytes(const orc_sim* s, int f) {
 template <class T> void copy_out(const std::vector<T>& v, void* dst, size_t n) {
     if (n) memcpy(dst, v.data(), n);
 }
+template <class T> void copy_in(std::vector<T>& v, const void* src, size_t n) {
+    if (n) memcpy(v.data(), src, n);
+}
+#ifdef ORACLE_DUAL
+// The tangent build stores Duals: fields cross the C ABI as their values.
+void copy_out(const std::vector<Real>& v, void* dst, size_t) {
+    double* d = static_cast<double*>(dst);
+    for (size_t i = 0; i < v.size(); ++i) d[i] = v[i].v;
+}
+void copy_out(const std::vector<R4>& v, void* dst, size_t) {
+    double* d = static_cast<double*>(dst);
+    for (size_t i = 0; i < v.size(); ++i) {
+        d[4 * i] = v[i].x.v; d[4 * i + 1] = v[i].y.v; d[4 * i + 2] = v[i].z.v; d[4 * i + 3] = v[i].w.v;
+    }
+}
+void copy_in(std::vector<Real>& v, const void* src, size_t) {
+    const double* d = static_cast<const double*>(src);
+    for (size_t i = 0; i < v.size(); ++i) v[i] = d[i];
+}
+void copy_in(std::vector<R4>& v, const void* src, size_t) {
+    const double* d = static_cast<const double*>(src);
+    for (size_t i = 0; i < v.size(); ++i) v[i] = mk4(d[4 * i], d[4 * i + 1], d[4 * i + 2], d[4 * i + 3]);
+}
+#endif
 void copy3_out(const std::vector<R4>& v, void* dst) {
     double* d = static_cast<double*>(dst);
-    for (size_t i = 0; i < v.size(); ++i) { d[3 * i] = v[i].x; d[3 * i + 1] = v[i].y; d[3 * i + 2] = v[i].z; }
+    for (size_t i = 0; i < v.size(); ++i) { d[3 * i] = val(v[i].x); d[3 * i + 1] = val(v[i].y); d[3 * i + 2] = val(v[i].z); }
 }
 void copy3_in(std::vector<R4>& v, const void* src) {
     const double* d = static_cast<const double*>(src);
@@ -1222,8 +1270,8 @@ int orc_set_field(orc_sim* s, int f, const void* src, size_t bytes) {
     const int64_t need = field_bytes(s, f);
     if (need < 0 || (size_t)need != bytes) return SPH_E_ARG;
     switch (f) {
-        case SPH_F_X: memcpy(s->x.data(), src, bytes); return 0;
-        case SPH_F_VEL: memcpy(s->vel.data(), src, bytes); return 0;
+        case SPH_F_X: copy_in(s->x, src, bytes); return 0;
+        case SPH_F_VEL: copy_in(s->vel, src, bytes); return 0;
         case SPH_F_ANGVEL: copy3_in(s->angvel, src); return 0;
         case SPH_F_ANGPOS: copy3_in(s->angpos, src); return 0;
         case SPH_F_FORCE: copy3_in(s->force, src); return 0;
@@ -1231,21 +1279,21 @@ int orc_set_field(orc_sim* s, int f, const void* src, size_t bytes) {
         case SPH_F_XYZSUM: copy3_in(s->xyzsum, src); return 0;
         case SPH_F_ACC: copy3_in(s->acc, src); return 0;
         case SPH_F_ANGACC: copy3_in(s->angacc, src); return 0;
-        case SPH_F_ES_DOT: memcpy(s->es_dot.data(), src, bytes); return 0;
-        case SPH_F_ES: memcpy(s->es.data(), src, bytes); return 0;
-        case SPH_F_EV_DOT: memcpy(s->ev_dot.data(), src, bytes); return 0;
-        case SPH_F_EV: memcpy(s->ev.data(), src, bytes); return 0;
-        case SPH_F_P: memcpy(s->p.data(), src, bytes); return 0;
-        case SPH_F_CONTACTS: memcpy(s->contacts.data(), src, bytes); return 0;
+        case SPH_F_ES_DOT: copy_in(s->es_dot, src, bytes); return 0;
+        case SPH_F_ES: copy_in(s->es, src, bytes); return 0;
+        case SPH_F_EV_DOT: copy_in(s->ev_dot, src, bytes); return 0;
+        case SPH_F_EV: copy_in(s->ev, src, bytes); return 0;
+        case SPH_F_P: copy_in(s->p, src, bytes); return 0;
+        case SPH_F_CONTACTS: copy_in(s->contacts, src, bytes); return 0;
         case SPH_F_DELTA_T: copy3_in(s->delta_t, src); return 0;
-        case SPH_F_DISTMOD: memcpy(s->distmod.data(), src, bytes); return 0;
-        case SPH_F_WALLS_NX: memcpy(s->w_nx.data(), src, bytes); return 0;
-        case SPH_F_WALLS_MVFD: memcpy(s->w_mvfd.data(), src, bytes); return 0;
-        case SPH_F_WALLS_ACC: memcpy(s->walls_acc.data(), src, bytes); return 0;
-        case SPH_F_BONDS: memcpy(s->bonds.data(), src, bytes); return 0;
-        case SPH_F_BONDS_DELTA: memcpy(s->bonds_delta.data(), src, bytes); return 0;
-        case SPH_F_BONDS_OMEGA: memcpy(s->bonds_omega.data(), src, bytes); return 0;
-        case SPH_F_COLOR: memcpy(s->color.data(), src, bytes); return 0;
+        case SPH_F_DISTMOD: copy_in(s->distmod, src, bytes); return 0;
+        case SPH_F_WALLS_NX: copy_in(s->w_nx, src, bytes); return 0;
+        case SPH_F_WALLS_MVFD: copy_in(s->w_mvfd, src, bytes); return 0;
+        case SPH_F_WALLS_ACC: copy_in(s->walls_acc, src, bytes); return 0;
+        case SPH_F_BONDS: copy_in(s->bonds, src, bytes); return 0;
+        case SPH_F_BONDS_DELTA: copy_in(s->bonds_delta, src, bytes); return 0;
+        case SPH_F_BONDS_OMEGA: copy_in(s->bonds_omega, src, bytes); return 0;
+        case SPH_F_COLOR: copy_in(s->color, src, bytes); return 0;
         default: return SPH_E_ARG;
     }
 }
@@ -1257,16 +1305,52 @@ int64_t orc_write_mem(const orc_sim* s, void* dst, size_t cap) {
     return (int64_t)w.b.size();
 }
 
-double orc_sqrt32(double v) { return sqrt32(v); }
+double orc_sqrt32(double v) { return val(sqrt32(v)); }
+
+#ifdef ORACLE_DUAL
+// Tangent build: seed a direction, step, read the directional derivatives.
+// Parameters in the order k_n, k_t, gamma_n, gamma_t, mu_s, mu_d, gamma_wn,
+// gamma_wt, mu_wd (the engine's adjoint parameters).
+int orc_dual_seed_param(orc_sim* s, int which, double d) {
+    Real* p[9] = {&s->P.k_n, &s->P.k_t, &s->P.gamma_n, &s->P.gamma_t, &s->P.mu_s,
+                  &s->P.mu_d, &s->P.gamma_wn, &s->P.gamma_wt, &s->P.mu_wd};
+    if (which < 0 || which > 8) return SPH_E_ARG;
+    p[which]->d = d;
+    return 0;
+}
+// field: SPH_F_X, SPH_F_VEL or SPH_F_ANGVEL; dir: np x 3.
+int orc_dual_seed_field(orc_sim* s, int field, const double* dir) {
+    std::vector<R4>* v = field == SPH_F_X ? &s->x : field == SPH_F_VEL ? &s->vel
+                       : field == SPH_F_ANGVEL ? &s->angvel : nullptr;
+    if (!v) return SPH_E_ARG;
+    for (size_t i = 0; i < v->size(); ++i) {
+        (*v)[i].x.d = dir[3 * i];
+        (*v)[i].y.d = dir[3 * i + 1];
+        (*v)[i].z.d = dir[3 * i + 2];
+    }
+    return 0;
+}
+int orc_dual_get_tangent(const orc_sim* s, int field, double* out) {
+    const std::vector<R4>* v = field == SPH_F_X ? &s->x : field == SPH_F_VEL ? &s->vel
+                             : field == SPH_F_ANGVEL ? &s->angvel : nullptr;
+    if (!v) return SPH_E_ARG;
+    for (size_t i = 0; i < v->size(); ++i) {
+        out[3 * i] = (*v)[i].x.d;
+        out[3 * i + 1] = (*v)[i].y.d;
+        out[3 * i + 2] = (*v)[i].z.d;
+    }
+    return 0;
+}
+#endif
 void orc_set_exact_length(int on) { g_exact_length = on != 0; }
 void orc_set_wall_sums_per_wall(int on) { g_wall_sums_per_wall = on != 0; }
 
 double orc_tree_sum(const double* v, uint32_t n) {
-    std::vector<Real> partial;
-    block_tree_sum(v, n, partial);
+    std::vector<Real> in(v, v + n), partial;
+    block_tree_sum(in.data(), n, partial);
     Real sum = 0.0;
     for (size_t i = 0; i < partial.size(); ++i) sum += partial[i];
-    return sum;
+    return val(sum);
 }
 
 uint32_t orc_np(const orc_sim* s) { return s->np; }
@@ -1308,7 +1392,7 @@ void orc_find_overlaps(const orc_sim* s, uint32_t* pairs, double* delta, uint64_
                 if (n < cap) {
                     pairs[2 * n] = i;
                     pairs[2 * n + 1] = j;
-                    delta[n] = dn;
+                    delta[n] = val(dn);
                 }
                 ++n;
             }
@@ -1323,9 +1407,9 @@ void orc_find_overlaps(const orc_sim* s, uint32_t* pairs, double* delta, uint64_
 int orc_porosity(const orc_sim* s, int z_slices, double* z_pos, double* porosity) {
     const double pi = 3.14159265358979323846;
     auto cap = [&](double h, double r) { return pi * h * h / 3.0 * (3.0 * r - h); };
-    const double top = s->nw > 0 ? s->w_nx[0].w : s->L[2];
+    const double top = s->nw > 0 ? val(s->w_nx[0].w) : s->L[2];
     const double h_slice = (top - s->origo[2]) / (double)z_slices;
-    const double r_last = s->np > 0 ? s->x[s->np - 1].w : 0.0;
+    const double r_last = s->np > 0 ? val(s->x[s->np - 1].w) : 0.0;
     if (h_slice <= r_last * 2.0) return -1;
     const double V_slice = h_slice * s->L[0] * s->L[1];
     for (int iz = 0; iz < z_slices; ++iz) {
@@ -1333,7 +1417,7 @@ int orc_porosity(const orc_sim* s, int z_slices, double* z_pos, double* porosity
         const double lo = iz * h_slice;
         const double hi = lo + h_slice;
         for (uint32_t i = 0; i < s->np; ++i) {
-            const double zc = s->x[i].z, r = s->x[i].w;
+            const double zc = val(s->x[i].z), r = val(s->x[i].w);
             const double zl = zc - r, zh = zc + r;
             const double V_sphere = 4.0 / 3.0 * pi * r * r * r;
             if (lo <= zl && zh <= hi) {
