@@ -300,6 +300,9 @@ def main():
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="set an engine scalar before timing (e.g. xcd_order=1, "
                          "graph_steps=0); repeatable")
+    ap.add_argument("--autotune", type=int, default=0, metavar="K",
+                    help="before the warm-up, run sph_autotune with K steps per trial "
+                         "(contact lanes, workgroup order; identical results)")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="steady-state chunk replayed from a captured HIP graph "
                          "(identical results; 0 = plain launches)")
@@ -355,6 +358,10 @@ def main():
     nw = int(ctx.scalar("nw"))
     dt = ctx.scalar("dt")
     log("np %d, nw %d, ncells %d, dt %.4g s" % (n, nw, int(ctx.scalar("ncells")), dt))
+    if args.autotune:
+        us = ctx.autotune(args.autotune)
+        log("autotune: %.2f us per step with contact lanes %d, xcd order %d" % (
+            us, int(ctx.scalar("contact_lanes")), int(ctx.scalar("xcd_order"))))
     ctx.step(args.warmup, sync=True)
     z = scenes.contact_number(ctx, n)
 

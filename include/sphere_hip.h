@@ -260,6 +260,16 @@ int sph_read_state_sim(sph_ctx* ctx, int sim, const char* path);
 /* Per-kernel timing: mean device microseconds of the last sph_step call's
  * kernels, measured with HIP events on the context stream
  * (the reference's PROFILING timers, src/device.cu:947-987,2711-2808). */
+/* Online autotuner; no counterpart in the reference.  Times steps_per_trial
+ * steps under each setting of a coordinate search - contact lanes (4, 8,
+ * 16; contact models 2 and 3), then the workgroup order (SPH_S_XCD_ORDER) -
+ * each trial after steps_per_trial / 2 untimed steps, and keeps a setting
+ * that is at least 3% faster.  Every setting gives the same results bit for
+ * bit, so the trials are ordinary steps: the simulation advances by 1.5 x
+ * steps_per_trial steps per trial (4 trials, or 2 for contact model 1).
+ * *us_per_step (may be NULL) receives the best wall time per step. */
+int sph_autotune(sph_ctx* ctx, uint64_t steps_per_trial, double* us_per_step);
+
 int sph_profile(sph_ctx* ctx, int enable);
 int sph_profile_report(sph_ctx* ctx, char* buf, size_t cap);
 

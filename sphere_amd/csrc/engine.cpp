@@ -16,6 +16,7 @@
 #include <time.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -1831,11 +1832,13 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
         case SPH_S_CONTACT_LANES:
             if (v != 1.0 && v != 4.0 && v != 8.0 && v != 16.0)
                 return fail(c, SPH_E_ARG, "contact lanes must be 1, 4, 8 or 16 (got %g)", v);
+            if (c->contact_lanes != (int)v) drop_graphs(c);  // captured with the old kernel
             c->contact_lanes = (int)v;
             return SPH_OK;
         case SPH_S_SORT_MODE:
             if (v != 0.0 && v != 1.0 && v != 2.0)
                 return fail(c, SPH_E_ARG, "sort mode must be 0, 1 or 2 (got %g)", v);
+            if (c->sort_mode != (int)v) drop_graphs(c);
             c->sort_mode = (int)v;
             if (c->sort_mode != 2) c->resort_at_end = c->sort_mode == 1;
             return SPH_OK;
@@ -2183,6 +2186,60 @@ int sph_adjoint_samples(sph_ctx* c, uint64_t every, uint64_t nsamples, const dou
                     "positive", (unsigned long long)every, (unsigned long long)nsamples);
     return adjoint_run(c, every * nsamples, nullptr, nullptr, nullptr, every, dL_dx, dL_dv,
                        g_x0, g_v0, g_w0, g_params);
+}
+
+int sph_autotune(sph_ctx* c, uint64_t steps_per_trial, double* us_per_step) {
+    if (!c || steps_per_trial == 0) return SPH_E_ARG;
+    // Every setting tried gives the same results bit for bit, so the trials
+    // are ordinary steps of the simulation: only their wall time differs.
+    // A trial first runs half as many untimed steps (graph capture, the
+    // list rebuild a new skin forces), and a setting replaces the incumbent
+    // only when at least 3% faster (re-sorts and rebuilds make short trials
+    // noisy).
+    const uint64_t warm = std::max<uint64_t>(1, steps_per_trial / 2);
+    auto trial = [&](double* us) -> int {
+        int rc = sph_step(c, warm);
+        if (!rc) rc = sph_sync(c);
+        if (rc) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = sph_step(c, steps_per_trial);
+        if (!rc) rc = sph_sync(c);
+        const auto t1 = std::chrono::steady_clock::now();
+        *us = std::chrono::duration<double, std::micro>(t1 - t0).count() / (double)steps_per_trial;
+        return rc;
+    };
+    const bool history = c->sims[0].h.contactmodel == 2 || c->sims[0].h.contactmodel == 3;
+    double best = 0.0;
+    int rc = trial(&best);  // the current settings
+    // Coordinate search: contact lanes, then the workgroup order.  (The
+    // candidate-list skin is not tuned here: its cost is the rebuild rate,
+    // which short trials do not see.)
+    if (!rc && history) {
+        int best_lanes = c->contact_lanes;
+        for (int lanes : {4, 8, 16}) {
+            if (lanes == best_lanes || rc) continue;
+            const int keep = c->contact_lanes;
+            rc = sph_set_scalar(c, SPH_S_CONTACT_LANES, lanes);
+            double t = 0.0;
+            if (!rc) rc = trial(&t);
+            if (!rc && t < 0.97 * best) {
+                best = t;
+                best_lanes = lanes;
+            } else if (!rc) {
+                rc = sph_set_scalar(c, SPH_S_CONTACT_LANES, keep);
+            }
+        }
+    }
+    if (!rc) {
+        const uint32_t keep = c->A.xcd_order;
+        rc = sph_set_scalar(c, SPH_S_XCD_ORDER, keep ? 0.0 : 1.0);
+        double t = 0.0;
+        if (!rc) rc = trial(&t);
+        if (!rc && t < 0.97 * best) best = t;
+        else if (!rc) rc = sph_set_scalar(c, SPH_S_XCD_ORDER, keep);
+    }
+    if (!rc && us_per_step) *us_per_step = best;
+    return rc;
 }
 
 int sph_profile(sph_ctx* c, int enable) {

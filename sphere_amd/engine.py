@@ -75,8 +75,8 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
            "sph_set_scalar_sim", "sph_adjoint", "sph_adjoint_samples", "sph_selftest_sqrt32",
-           "sph_write_state", "sph_read_state", "sph_write_state_sim", "sph_read_state_sim", "sph_find_overlaps",
-           "sph_porosity")
+           "sph_write_state", "sph_read_state", "sph_write_state_sim", "sph_read_state_sim",
+           "sph_find_overlaps", "sph_porosity", "sph_autotune")
 
 # Parameters of sph_adjoint's g_params, in order.
 ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
@@ -149,6 +149,7 @@ def load():
                                      ctypes.POINTER(ctypes.c_double)]
     L.sph_set_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.sph_adjoint.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp, vp]
+    L.sph_autotune.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.sph_adjoint_samples.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp, vp,
                                       vp]
     L.sph_selftest_sqrt32.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
@@ -324,6 +325,14 @@ class Context:
         res = {"x": outs[0], "v": outs[1], "w": outs[2]}
         res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp)})
         return res
+
+    def autotune(self, steps_per_trial=50):
+        """sph_autotune: choose the contact lanes and the workgroup order by
+        timing steps of this simulation (identical results for every
+        setting; the simulation advances).  Returns the best us per step."""
+        us = ctypes.c_double()
+        self._chk(self._L.sph_autotune(self._h, int(steps_per_trial), ctypes.byref(us)))
+        return us.value
 
     def profile(self, enable=True):
         self._chk(self._L.sph_profile(self._h, 1 if enable else 0))

@@ -280,3 +280,29 @@ def test_triaxial_walls_bit_exact(per_wall):
     if per_wall:
         assert len({float(v) for v in mv[1:, 2]}) > 1  # the side walls' own forces
     ctx.close()
+
+
+@pytest.mark.parametrize("case", ["shear1k", "batch"])
+def test_autotune_keeps_results(case):
+    """sph_autotune's trials are ordinary steps under other settings: the
+    simulation it leaves, and every step after it, are bit-exact with the
+    oracle's; and it leaves valid settings."""
+    a = cases.fixture("shear1k.bin")
+    if case == "shear1k":
+        images = [a.tobytes()]
+        ctx = engine.Context(images[0])
+    else:
+        images = [a.tobytes(), cases.settling_case(1200, 12).tobytes()]
+        ctx = engine.Context(images=images)
+    us = ctx.autotune(20)
+    assert us > 0.0
+    assert ctx.scalar("contact_lanes") in (4, 8, 16)
+    done = int(ctx.scalar("iter"))
+    assert done == 4 * 30
+    ctx.step(30)
+    for k, image in enumerate(images):
+        ref = oracle.Oracle(image)
+        ref.step(done + 30)
+        for name in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t"):
+            assert numpy.array_equal(ctx.get(name, sim=k), ref.get(name)), (k, name)
+    ctx.close()
