@@ -278,6 +278,57 @@ def adjoint_bench(args, img):
     return 0
 
 
+def collapse_bench(args):
+    """The differentiable column collapse: 2,000 spheres (radii uniform in
+    [0.008, 0.012] m) settled in a 0.3 m column and released onto a
+    frictional floor; 500 steps sampled every 50; the MSE of positions
+    against a target made with k_n* = 2e7, gamma_n* = 0.3 critical, from
+    k_n = 1e7, gamma_n = 0.2 critical.  Times the torch forward (the
+    trajectory) against forward + backward (the loss gradient with respect
+    to k_n, gamma_n, mu and the initial velocities)."""
+    import numpy
+    import torch
+    from sphere_amd import autograd, engine, scenes
+    s = scenes.collapse_column(2000, seed=args.seed, k_n=1.0e7, damping_frac=0.2,
+                               open_stepper=lambda img: engine.Context(img))
+    traj = autograd.Trajectory(s, 50, 10, params=("k_n", "gamma_n", "mu"))
+    r_min, rho = float(s.radius.min()), float(s.rho[0])
+
+    def crit(k, f):
+        return f * scenes.critical_damping(r_min, rho, k)
+
+    with torch.no_grad():
+        target = traj(torch.tensor([2.0e7, crit(2.0e7, 0.3), 0.5], dtype=torch.float64))
+    theta0 = [1.0e7, crit(1.0e7, 0.2), 0.5]
+    v0 = torch.as_tensor(s.vel[:, :3].copy())
+    reps = 5
+    t_fwd = t_all = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            loss = ((traj(torch.tensor(theta0, dtype=torch.float64), v0=v0) - target) ** 2).mean()
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+        theta = torch.tensor(theta0, dtype=torch.float64, requires_grad=True)
+        v = v0.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        loss = ((traj(theta, v0=v) - target) ** 2).mean()
+        loss.backward()
+        t_all = min(t_all, time.perf_counter() - t0)
+    g = theta.grad.numpy()
+    log("column collapse: forward %.4f s, forward + backward %.4f s (ratio %.2f); loss %.4e, "
+        "dL/dk_n %.3e, dL/dgamma_n %.3e, dL/dmu %.3e, |dL/dv0| %.3e"
+        % (t_fwd, t_all, t_all / t_fwd, float(loss.detach()), g[0], g[1], g[2],
+           float(v.grad.norm())))
+    print(json.dumps({"metric": "trajectory-loss gradient cost / forward cost", "value":
+                      round(t_all / t_fwd, 3), "unit": "ratio", "n_gpus": 1, "steps": 500,
+                      "higher_is_better": False, "np": 2000, "forward_s": round(t_fwd, 5),
+                      "forward_backward_s": round(t_all, 5), "data": "synthetic",
+                      "config": {"workload": "granular column collapse, 2000 spheres, 500 "
+                                 "steps sampled every 50, MSE vs k_n*=2e7, gamma_n*=0.3 crit",
+                                 "params": ["k_n", "gamma_n", "mu", "v0"]}}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,6 +351,9 @@ def main():
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
                     help="set an engine scalar before timing (e.g. xcd_order=1, "
                          "graph_steps=0); repeatable")
+    ap.add_argument("--collapse", action="store_true",
+                    help="instead: time the column-collapse calibration gradient (torch "
+                         "autograd over the trajectory adjoint) against its forward pass")
     ap.add_argument("--autotune", type=int, default=0, metavar="K",
                     help="before the warm-up, run sph_autotune with K steps per trial "
                          "(contact lanes, workgroup order; identical results)")
@@ -340,6 +394,8 @@ def main():
 
     if args.batch:
         return batch_bench(args)
+    if args.collapse:
+        return collapse_bench(args)
 
     from sphere_amd import engine, scenes
 

@@ -137,13 +137,13 @@ def test_column_collapse_gradient_descends():
     loss.backward()
     g = theta.grad.numpy()
     print("column: loss %.6e, dL/dk_n %.3e, dL/dgamma_n %.3e, dL/dmu %.3e, |dL/dv0| %.3e" % (
-        float(loss), g[0], g[1], g[2], float(v0.grad.norm())), flush=True)
+        float(loss.detach()), g[0], g[1], g[2], float(v0.grad.norm())), flush=True)
     assert numpy.all(numpy.isfinite(g)) and numpy.all(numpy.isfinite(v0.grad.numpy()))
-    assert float(loss) > 0.0 and g[0] != 0.0 and g[1] != 0.0
+    assert float(loss.detach()) > 0.0 and g[0] != 0.0 and g[1] != 0.0
     # a 1% step in log-parameters against the gradient lowers the loss
     glog = g * theta0.numpy()
     step = 0.01 / numpy.abs(glog).max()
     theta1 = theta0 * torch.as_tensor(numpy.exp(-step * glog))
     with torch.no_grad():
         loss1 = ((traj(theta1) - target) ** 2).mean()
-    assert float(loss1) < float(loss), (float(loss1), float(loss))
+    assert float(loss1) < float(loss.detach()), (float(loss1), float(loss.detach()))
