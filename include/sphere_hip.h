@@ -141,9 +141,15 @@ enum sph_scalar {
                                  forces (same tree and serial order) - the
                                  correction multi-wall setups (triaxial, nw = 5)
                                  need.  No counterpart in the reference. */
-    SPH_S_K_N = 22            /* per simulation, read-only: the normal stiffness
+    SPH_S_K_N = 22,           /* per simulation, read-only: the normal stiffness
                                  params.k_n, which DEM::forcechains scales the
                                  overlaps by (src/sphere.cpp:789). */
+    SPH_S_INTEGRATE_LANES = 23 /* work-items per particle in the integration
+                                 pass: 1 (default) or 4 (one vector component
+                                 each, then a wall-partials pass; used with a
+                                 fixed grid height and no wmode-3 wall;
+                                 measured slower on the sheared bed, 22 + 6
+                                 us against 18).  Identical results. */
 };
 
 /* Open flags. */

@@ -1842,6 +1842,34 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     }
 }
 
+// The integration with one vector component per work-item (four per
+// particle, original order): integrate_component's expressions, the same
+// as k_integrate's per component, with each particle's vectors read and
+// written by its four work-items together.  The next step's hash is taken
+// on component lane 0 from the position the quad assembles by DPP.  The
+// wall-force block partials follow in k_wall_partials.  For a fixed grid
+// height without a wmode-3 wall.
+__global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
+    constexpr uint32_t PPB = BLOCK / 4;  // particles per workgroup
+    const LaneGroup<4> g;
+    const uint32_t idx = blockIdx.x * PPB + threadIdx.x / 4;
+    const DevParams& p = chunk_params(a, blockIdx.x / (BLOCK / PPB));
+    DevLoop* lp = a.loop + p.sim;
+    const bool in = idx - p.base < p.np;  // the same for a particle's four lanes
+    HashFlags f;
+    if (in) {
+        const uint32_t c = g.q < 3 ? g.q : 0u;
+        const double fq = reinterpret_cast<const double*>(a.force + idx)[c];
+        const double tq = reinterpret_cast<const double*>(a.torque + idx)[c];
+        const double xn = integrate_component(p, a, lp, idx, g.q, fq, tq, a.x[idx], a.vel[idx],
+                                              a.angvel[idx], a.rank[idx]);
+        const double4 x_new = make_double4(g.template bc<0>(xn), g.template bc<1>(xn),
+                                           g.template bc<2>(xn), g.template bc<3>(xn));
+        if (g.q == 0) hash_particle(p, a, idx, p.L[2], x_new, f);
+    }
+    publish_next(p, a, lp, f, idx);
+}
+
 // The wall-force block partials alone (the reference's summation kernel,
 // src/integration.cuh:309-341), after the fused contact-and-integration
 // pass: 256 original indices per workgroup.
@@ -2307,6 +2335,11 @@ hipError_t launch_shear_stress(const DevArrays& a, hipStream_t s) {
 
 hipError_t launch_integrate(const DevArrays& a, int hash_next, hipStream_t s) {
     hipLaunchKernelGGL(k_integrate, dim3(a.nchunk), dim3(BLOCK), 0, s, a, hash_next);
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate_q(const DevArrays& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_integrate_q, dim3(a.nchunk * 4), dim3(BLOCK), 0, s, a);
     return hipGetLastError();
 }
 

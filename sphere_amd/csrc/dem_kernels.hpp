@@ -47,6 +47,10 @@ hipError_t launch_shear_stress(const DevArrays& a, hipStream_t s);
 // integrate (TY3), src/integration.cuh:14-305, the wall force block
 // partials, and (hash_next != 0) the next step's cell keys.
 hipError_t launch_integrate(const DevArrays& a, int hash_next, hipStream_t s);
+// Four work-items per particle (one vector component each), hashing for the
+// next step; the wall partials then need launch_wall_partials.  Fixed grid
+// height, no wmode-3 wall.
+hipError_t launch_integrate_q(const DevArrays& a, hipStream_t s);
 // Per simulation: serial sum of the partials + integrateWalls + loop advance
 // (src/integration.cuh:344-450, src/device.cu:2500-2538).  With resort != 0
 // it also re-sorts the simulation for the next step, in one workgroup, when

@@ -160,6 +160,7 @@ struct sph_ctx {
     // default: measured slower than the two kernels, 72.1 + 6.2 us for the
     // fused pass and its wall partials against 53.3 + 19.6 us).
     bool fused = false;
+    int integrate_lanes = 1;  // SPH_S_INTEGRATE_LANES
     // per-kernel profile (stage name, total ms, launches)
     std::vector<std::string> prof_names;
     std::vector<double> prof_ms;
@@ -936,7 +937,18 @@ int enqueue_step(sph_ctx* c) {
         HIPCHK(c, launch_shear_stress(A, s), "shear_stress");
         pr.end();
     }
-    if (!fused) {
+    if (!fused && c->integrate_lanes == 4 && hash_in_integrate && !c->shear_stress) {
+        pr.begin("integrate");
+        HIPCHK(c, launch_integrate_q(A, s), "integrate");
+        pr.end();
+        bool walls = false;
+        for (const Sim& sim : c->sims) walls = walls || sim.h.nw > 0;
+        if (walls) {
+            pr.begin("wall_partials");
+            HIPCHK(c, launch_wall_partials(A, s), "partials");
+            pr.end();
+        }
+    } else if (!fused) {
         pr.begin("integrate");
         HIPCHK(c, launch_integrate(A, hash_in_integrate ? 1 : 0, s), "integrate");
         pr.end();
@@ -1751,6 +1763,7 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_FUSED: *v = c->fused ? 1.0 : 0.0; return SPH_OK;
         case SPH_S_WALL_SUMS: *v = S.P.wall_sums; return SPH_OK;
         case SPH_S_K_N: *v = S.P.k_n; return SPH_OK;
+        case SPH_S_INTEGRATE_LANES: *v = c->integrate_lanes; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
 }
@@ -1811,6 +1824,12 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
                 return fail(c, SPH_E_ARG, "wall sums must be 0 or 1 (got %g)", v);
             S.P.wall_sums = (uint32_t)v;
             return upload_params(c);
+        case SPH_S_INTEGRATE_LANES:
+            if (v != 1.0 && v != 4.0)
+                return fail(c, SPH_E_ARG, "integrate lanes must be 1 or 4 (got %g)", v);
+            if (c->integrate_lanes != (int)v) drop_graphs(c);
+            c->integrate_lanes = (int)v;
+            return SPH_OK;
         case SPH_S_FUSED:
             if (v != 0.0 && v != 1.0)
                 return fail(c, SPH_E_ARG, "fused must be 0 or 1 (got %g)", v);

@@ -64,7 +64,7 @@ SCALARS = {"time": 0, "time_total": 1, "dt": 2, "file_dt": 3, "iter": 4, "step_c
            "np": 6, "nw": 7, "nb0": 8, "ncells": 9, "vl_skin": 10, "sorts": 11, "vl_builds": 12,
            "nsims": 13, "contact_lanes": 14, "sort_mode": 15, "resort_merge_max": 16,
            "graph_steps": 17, "xcd_order": 18, "adj_checkpoint": 19, "fused": 20,
-           "wall_sums": 21, "k_n": 22}
+           "wall_sums": 21, "k_n": 22, "integrate_lanes": 23}
 
 # Exported symbols of the C ABI (include/sphere_hip.h).
 SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_write_bin",

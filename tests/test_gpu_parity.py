@@ -306,3 +306,32 @@ def test_autotune_keeps_results(case):
         for name in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t"):
             assert numpy.array_equal(ctx.get(name, sim=k), ref.get(name)), (k, name)
     ctx.close()
+
+
+@pytest.mark.parametrize("case", ["shear1k", "settling", "bonded", "batch"])
+def test_integrate_lanes_bit_exact(case):
+    """The integration with four work-items per particle (one vector
+    component each, SPH_S_INTEGRATE_LANES = 4) and its wall-partials pass:
+    bit-exact with the oracle, walls included, single and batched."""
+    from .test_gpu_bonds import _bonded
+    if case == "shear1k":
+        images = [cases.fixture("shear1k.bin").tobytes()]
+    elif case == "settling":
+        images = [cases.settling_case(1200, 12).tobytes()]
+    elif case == "bonded":
+        images = [_bonded(frac=0.3, seed=4, sigma_b=300.0).tobytes()]
+    else:
+        images = [cases.fixture("shear1k.bin").tobytes(), cases.settling_case(300, 11).tobytes()]
+    ctx = engine.Context(images=images) if len(images) > 1 else engine.Context(images[0])
+    ctx.set_scalar("integrate_lanes", 4)
+    ctx.step(120)
+    for k, image in enumerate(images):
+        ref = oracle.Oracle(image)
+        ref.step(120)
+        bad = []
+        for name in EXACT_FIELDS + ("walls_nx", "walls_mvfd", "walls_acc"):
+            a, b = ctx.get(name, sim=k), ref.get(name)
+            if a.shape != b.shape or not numpy.array_equal(a, b):
+                bad.append(name)
+        assert bad == [], (k, bad)
+    ctx.close()
