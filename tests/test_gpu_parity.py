@@ -335,3 +335,54 @@ def test_integrate_lanes_bit_exact(case):
                 bad.append(name)
         assert bad == [], (k, bad)
     ctx.close()
+
+
+def _tiny(n, seed=3):
+    """n spheres (n may be 0) falling in a closed 0.1 m box: for the edge
+    sizes, the empty simulation (which the reference runs: checkValues only
+    notes np = 0, src/sphere.cpp:168-172) and one particle."""
+    from sphere_amd.sim import sim
+    s = sim(np=n, sid="tiny%d" % n)
+    rng = numpy.random.default_rng(seed)
+    if n:
+        s.radius[:] = rng.uniform(0.004, 0.006, n)
+        s.x[:] = rng.uniform(0.01, 0.09, (n, 3))
+        s.vel[:, :3] = rng.normal(0.0, 0.1, (n, 3))
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.origo[:] = 0.0
+    s.L[:] = [0.1, 0.1, 0.1]
+    s.num[:] = [8, 8, 8]
+    s.time_dt[0] = 1.0e-5
+    s.time_total[0] = 1.0
+    s.time_file_dt[0] = 0.1
+    return s
+
+
+@pytest.mark.parametrize("n", [0, 1])
+def test_edge_sizes_bit_exact(n):
+    s = _tiny(n)
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    ctx.step(3000)
+    ref.step(3000)
+    assert ctx.tobytes() == ref.tobytes()
+    if n:
+        assert _compare(ctx, ref) == []
+        assert ctx.get("x")[0, 2] < s.x[0, 2]  # it fell
+    ctx.close()
+
+
+def test_ragged_batch_bit_exact():
+    """Simulation sizes that are not multiples of a workgroup, one of a single
+    particle, batched together."""
+    images = [_tiny(1).tobytes(), cases.settling_case(257, 5).tobytes(),
+              _tiny(7, seed=9).tobytes(), cases.settling_case(300, 11).tobytes()]
+    ctx = engine.Context(images=images)
+    ctx.step(200)
+    for k, image in enumerate(images):
+        ref = oracle.Oracle(image)
+        ref.step(200)
+        for name in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t"):
+            assert numpy.array_equal(ctx.get(name, sim=k), ref.get(name)), (k, name)
+    ctx.close()
