@@ -1,9 +1,10 @@
-"""The full-size workloads: a 1e5-particle direct-shear bed (the benchmark's
-size and stiffness) and a 1e6-particle bonded shear bed (2% of the touching
-pairs bonded, 20 kPa, 5e-4 m/s), stepped by the engine and by the oracle
-from the same state and compared bit for bit; then the properties that do
-not depend on size, at full size: the same result for every contact lane
-count, with graph replay, and for a batch holding the bed twice."""
+"""The full-size workloads: the 4,096-sphere gravity settling box, a
+1e5-particle direct-shear bed (the benchmark's size and stiffness) and a
+1e6-particle bonded shear bed (2% of the touching pairs bonded, 20 kPa,
+5e-4 m/s), stepped by the engine and by the oracle from the same state and
+compared bit for bit; then the properties that do not depend on size, at
+full size: the same result for every contact lane count, with graph
+replay, and for a batch holding the bed twice."""
 
 import numpy
 import pytest
@@ -48,6 +49,22 @@ def test_shear_bed_1e5_bit_exact():
         other.set_scalar("graph_steps", 5)
         other.step(steps)
         assert _same_state(other, ctx) == []
+    ctx.close()
+
+
+def test_gravity_settling_4096_bit_exact():
+    """The 4,096-sphere gravity settling workload (radii uniform in
+    [0.005, 0.01] m, seeded random sequential insertion in a 0.2 x 0.2 x
+    0.6 m box with frictional walls), 3,000 steps, every field."""
+    from sphere_amd import scenes
+    s = scenes.settling_box(4096, seed=1001)
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    for _ in range(3):
+        ref.step(1000)
+        ctx.step(1000)
+        assert _compare(ctx, ref) == []
     ctx.close()
 
 
