@@ -100,3 +100,30 @@ def test_batch_of_two_full_beds_matches_single():
         for k in (0, 1):
             for f in ("x", "vel", "angvel", "force", "contacts"):
                 assert numpy.array_equal(two.get(f, sim=k), one.get(f)), (k, f)
+
+
+def test_batch_of_256_beds_bit_exact():
+    """The batched sweep at full size: 256 sheared beds of log-uniform size
+    in [500, 20000] (1.36e6 particles), each with its own stiffness
+    (log-uniform 1e6..1e8 N/m, its own dt), friction (0.3..0.7) and normal
+    stress (5..160 kPa), in one context; every simulation bit-exact with
+    its own oracle after 20 steps."""
+    from sphere_amd import scenes
+    sizes = scenes.batch_sizes(256, seed=7)
+    rng = numpy.random.default_rng(7)
+    images = []
+    for k, n in enumerate(sizes):
+        s = cases.lattice_bed(int(n), seed=100 + k, median=0.0075, sigma=0.2,
+                              k_n=float(10.0 ** rng.uniform(6.0, 8.0)),
+                              normal_stress=float(rng.uniform(5.0e3, 160.0e3)),
+                              mu=float(rng.uniform(0.3, 0.7)))
+        images.append(s.tobytes())
+    print("batch: %d simulations, %d particles" % (len(images), int(sum(sizes))), flush=True)
+    with engine.Context(images=images) as ctx:
+        ctx.step(20)
+        for k, image in enumerate(images):
+            ref = oracle.Oracle(image)
+            ref.step(20)
+            for f in ("x", "vel", "angvel", "force", "contacts", "delta_t"):
+                assert numpy.array_equal(ctx.get(f, sim=k), ref.get(f)), (k, f)
+            assert numpy.array_equal(ctx.get("walls_nx", sim=k), ref.get("walls_nx")), k
