@@ -103,6 +103,9 @@ def _lib(dual=False):
         L.orc_porosity.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                                    ctypes.c_void_p]
         L.orc_get_threads.restype = ctypes.c_int
+        L.orc_set_contact_log.argtypes = [ctypes.c_int]
+        L.orc_contact_log_width.restype = ctypes.c_int
+        L.orc_get_contact_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         if dual:
             L.orc_dual_seed_param.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
             L.orc_dual_seed_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
@@ -125,6 +128,18 @@ def set_exact_length(on):
     """Gradient-check aid: FP64 instead of FP32 lengths in every step that
     follows (process-wide; the reference's FP32 lengths are the default)."""
     _lib().orc_set_exact_length(1 if on else 0)
+
+
+def set_contact_log(on):
+    """Test aid: record every evaluated contact of the following interact
+    stages (contact models 2 and 3; process-wide, off by default)."""
+    _lib().orc_set_contact_log(1 if on else 0)
+
+
+# Columns of one contact record (Oracle.contact_log).
+CLOG = {"active": 0, "slip": 1, "delta": 2, "n": slice(3, 6), "f_n": slice(6, 9),
+        "f_t": slice(9, 12), "f_c": slice(12, 15), "vel_t": slice(15, 18),
+        "delta_t": slice(18, 21), "delta_t0": slice(21, 24), "k_t": 24}
 
 
 def set_wall_sums_per_wall(on):
@@ -195,6 +210,15 @@ class Oracle:
         rc = self._L.orc_set_field(self._h, fid, a.ctypes.data, a.nbytes)
         if rc != 0:
             raise RuntimeError("orc_set_field(%s) failed: %d" % (name, rc))
+
+    def contact_log(self):
+        """The contact records of the last interact stage (np x NC x width,
+        original-index rows; columns in CLOG), when set_contact_log is on."""
+        w = self._L.orc_contact_log_width()
+        out = numpy.zeros(self.np * NC * w)
+        if self._L.orc_get_contact_log(self._h, out.ctypes.data, out.size) != 0:
+            raise RuntimeError("no contact log (set_contact_log(True) before stepping)")
+        return out.reshape(self.np, NC, w)
 
     def find_overlaps(self):
         """DEM::findOverlaps: (pairs (k, 2) i < j, overlaps (k,))."""

@@ -98,6 +98,11 @@ static inline Real sqrt32(Real v) { return (Real)sqrtf((float)v); }
 // as FP64 roots (off by default; the FP32 root is the reference's).
 static bool g_exact_length = false;
 static bool g_wall_sums_per_wall = false;
+// Test-only: record every evaluated contact's forces and displacement.
+static bool g_contact_log = false;
+// Record layout: active, slip, delta, n[3], f_n[3], f_t[3], f_c[3], vel_t[3],
+// delta_t (stored)[3], delta_t0 (projected)[3], k_t.
+static const int CLOG_W = 25;
 static inline Real length(R3 v) { return g_exact_length ? sqrt(dot(v, v)) : sqrt32(dot(v, v)); }
 
 struct Params {
@@ -144,6 +149,9 @@ struct orc_sim {
     // loop state (src/device.cu:919-993)
     int64_t iter;
     int velocity_state, change_velocity_state;
+    // Test-only per-contact record of the last interact stage (models 2 and
+    // 3), np x NC x CLOG_W doubles, filled when g_contact_log is set.
+    std::vector<double> clog;
 
     uint32_t ncells() const { return num[0] * num[1] * num[2]; }
     uint32_t blocks() const { return (np + 255u) / 256u; }
@@ -598,6 +606,28 @@ void capillary_cohesion_exp(const orc_sim* s, Acc& A, Real ra, Real rb, Real del
     A.F += f_c;
 }
 
+static inline void log_put3(double* r, int at, const R3& v) {
+    r[at] = val(v.x); r[at + 1] = val(v.y); r[at + 2] = val(v.z);
+}
+
+// Test-only record of one evaluated contact (see CLOG_W).
+static inline void log_contact(orc_sim* s, size_t mempos, bool slip, Real delta, R3 n, R3 f_n,
+                               R3 f_t, R3 f_c, R3 vel_t, R3 delta_t, R3 delta_t0, Real k_t) {
+    if (!g_contact_log) return;
+    double* r = &s->clog[mempos * CLOG_W];
+    r[0] = 1.0;
+    r[1] = slip ? 1.0 : 0.0;
+    r[2] = val(delta);
+    log_put3(r, 3, n);
+    log_put3(r, 6, f_n);
+    log_put3(r, 9, f_t);
+    log_put3(r, 12, f_c);
+    log_put3(r, 15, vel_t);
+    log_put3(r, 18, delta_t);
+    log_put3(r, 21, delta_t0);
+    r[24] = val(k_t);
+}
+
 // contactLinear (model 2), src/contactmodels.cuh:244-411.
 void contact_linear(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 angvel_a,
                     Real ra, Real rb, R3 x, Real x_length, Real delta, size_t mempos) {
@@ -628,6 +658,7 @@ void contact_linear(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 an
     A.ev_dot += 0.5 * P.gamma_n * vel_n * vel_n;
     f_c = -P.kappa * sqrt32(ra * rb) * n;
     f_t = mk3(0.0, 0.0, 0.0);
+    bool slip = false;
     if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
         delta_t = delta_t0 + vel_t * s->dt;
         const R3 f_t_elast = -k_t * delta_t;
@@ -637,6 +668,7 @@ void contact_linear(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 an
         const Real f_t_limit = P.mu_s * length(f_n - f_c);
         A.ev_dot += 0.5 * P.gamma_t * vel_t_length * vel_t_length;
         if (f_t_length > f_t_limit) {
+            slip = true;
             const R3 t = f_t / length(f_t);
             f_t = f_t_limit * t;
             delta_t = -1.0 / k_t * (P.mu_d * length(f_n - f_c) * t + P.gamma_t * vel_t);
@@ -647,6 +679,7 @@ void contact_linear(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 an
     A.T += cross(-(ra + delta * 0.5) * n, f_t);
     A.p += f_n_length / (4.0 * PI * ra * ra);
     s->delta_t[mempos] = mk4(delta_t.x, delta_t.y, delta_t.z, 0.0);
+    log_contact(s, mempos, slip, delta, n, f_n, f_t, f_c, vel_t, delta_t, delta_t0, k_t);
 }
 
 // contactHertz (model 3), src/contactmodels.cuh:417-600, with the documented
@@ -687,6 +720,7 @@ void contact_hertz(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 ang
     f_c = -P.kappa * sqrt32(ra * rb) * n_ab;
     f_t = mk3(0.0, 0.0, 0.0);
     T_res = mk3(0.0, 0.0, 0.0);
+    bool slip = false;
     if (delta_t0_length > 0.0 || vel_t_ab_length > 0.0) {
         const R3 f_t_elast = -k_t * p12 * delta_t0;
         const R3 f_t_visc = -P.gamma_t * p14 * vel_t_ab;
@@ -698,6 +732,7 @@ void contact_hertz(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 ang
         f_t = f_t_elast + f_t_visc;
         const Real f_t_length = length(f_t);
         if (f_t_length > f_t_limit) {
+            slip = true;
             f_t *= f_t_limit / f_t_length;
             delta_t = (f_t + P.gamma_t * p14 * vel_t_ab) / (-k_t * p12);
             A.es_dot += length(delta_t0 - delta_t) * k_t / s->dt;
@@ -713,6 +748,7 @@ void contact_hertz(orc_sim* s, Acc& A, uint32_t ib_orig, const R4& vel_a, R3 ang
     A.T += -(ra + delta / 2.0) * cross(n_ab, f_t) + T_res;
     A.p += f_n_length / (4.0 * PI * ra * ra);
     s->delta_t[mempos] = mk4(delta_t.x, delta_t.y, delta_t.z, 0.0);
+    log_contact(s, mempos, slip, delta, n_ab, f_n, f_t, f_c, vel_t_ab, delta_t, delta_t0, k_t);
 }
 
 // findAndProcessContactsInCell (model 1), src/contactsearch.cuh:91-186.
@@ -753,6 +789,7 @@ void stage_interact(orc_sim* s) {
                 mk4(0, 0, 0, 0)};
     for (uint32_t w = 0; w < nw && w < 5; ++w) { wn[w] = s->w_nx[w]; wm[w] = s->w_mvfd[w]; }
     const uint32_t cm = s->P.contactmodel;
+    if (g_contact_log) s->clog.assign((size_t)np * NC * CLOG_W, 0.0);
 #pragma omp parallel for schedule(dynamic, 256) if (np > 1024)
     for (int64_t ia64 = 0; ia64 < np; ++ia64) {
         const uint32_t ia = (uint32_t)ia64;
@@ -1343,6 +1380,14 @@ int orc_dual_get_tangent(const orc_sim* s, int field, double* out) {
 }
 #endif
 void orc_set_exact_length(int on) { g_exact_length = on != 0; }
+void orc_set_contact_log(int on) { g_contact_log = on != 0; }
+int orc_contact_log_width(void) { return CLOG_W; }
+int orc_get_contact_log(const orc_sim* s, double* dst, size_t n) {
+    const size_t want = (size_t)s->np * NC * CLOG_W;
+    if (n != want || s->clog.size() != want) return -1;
+    memcpy(dst, s->clog.data(), want * sizeof(double));
+    return 0;
+}
 void orc_set_wall_sums_per_wall(int on) { g_wall_sums_per_wall = on != 0; }
 
 double orc_tree_sum(const double* v, uint32_t n) {

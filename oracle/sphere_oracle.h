@@ -69,6 +69,13 @@ int64_t orc_iter(const orc_sim* s);
 double orc_sqrt32(double v);
 /* Test aid for gradient checks: FP64 lengths instead of FP32 (default 0). */
 void orc_set_exact_length(int on);
+/* Test aid: record, in every following interact stage (contact models 2
+ * and 3), each evaluated contact's overlap, normal, forces, tangential
+ * velocity and displacements (np x NC records of orc_contact_log_width()
+ * doubles, original-index rows; see CLOG_W in sphere_oracle.cpp). */
+void orc_set_contact_log(int on);
+int orc_contact_log_width(void);
+int orc_get_contact_log(const orc_sim* s, double* dst, size_t n);
 /* Each dynamic wall integrates its own force sum instead of wall 0's (the
  * engine's optional correction of the reference's P8 behaviour). */
 void orc_set_wall_sums_per_wall(int on);

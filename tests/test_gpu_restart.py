@@ -1,12 +1,14 @@
 """Restart sidecars (no counterpart in the reference, whose resumed runs
 start with empty contact rows and acc0 = 0): a run resumed from an output
-.bin and its sidecar is bit-identical to the uninterrupted run."""
+.bin and its sidecar is bit-identical to the uninterrupted run, and to the
+oracle run uninterrupted from the same image."""
 
 import os
 
 import numpy
 import pytest
 
+import oracle
 from sphere_amd import engine
 
 from . import cases
@@ -45,8 +47,15 @@ def test_resumed_run_is_bit_identical(tmp_path, case):
 
     resumed = engine.Context(path=binp, sidecar=True)
     resumed.step(n2)
-    assert _same(_state(resumed), ref) == []
+    got = _state(resumed)
+    assert _same(got, ref) == []
     resumed.close()
+
+    # The anchor: the oracle stepped n1 + n2 steps from the same image
+    # without interruption.
+    orc = oracle.Oracle(image)
+    orc.step(n1 + n2)
+    assert [f for f in FIELDS if not numpy.array_equal(got[f], orc.get(f))] == []
 
     # Without the sidecar the resumed run is the reference's: it differs.
     plain = engine.Context(path=binp)
@@ -125,3 +134,19 @@ def test_sim_run_resumes_from_output_and_sidecar(tmp_path):
     for f in ("x", "vel", "angvel", "angpos", "xyzsum", "es", "ev", "w_x", "w_vel"):
         assert numpy.array_equal(numpy.asarray(getattr(again, f)),
                                  numpy.asarray(getattr(whole, f))), f
+    # And the oracle, uninterrupted from the input over as many steps as the
+    # host loop takes to write file 2 (its own double additions of t and of
+    # the file clock).
+    t, ftc, nfile, steps = t0, 0.0, 0, 0
+    while nfile < 2:
+        t += dt
+        ftc += dt
+        steps += 1
+        if ftc >= float(s.time_file_dt[0]):
+            nfile += 1
+            ftc = 0.0
+    orc = oracle.Oracle(s.tobytes())
+    orc.step(steps)
+    want = sim().frombytes(orc.tobytes())
+    again.time_step_count[0] = want.time_step_count[0]
+    assert again.compare(want) == []
