@@ -15,6 +15,14 @@ under a 10 kPa stress-controlled top wall, then sheared with the top layer at
 1e-3 m/s; linear visco-elastic contacts (contactmodel 2) with Coulomb
 mu = 0.5, k_n = k_t = 1.16e7 N/m, gamma = 0.5 critical, dt = 0.075
 sqrt(m_min/k_n).
+
+The same line also carries the metric's other two parts, measured in the
+same run: "batched", the batched sweep (256 consolidation beds of
+log-uniform size in [500, 20000], radii uniform in [0.005, 0.01] m, each
+with its own normal stress, stiffness, friction and dt) as one batched
+context against the plain loop over per-simulation contexts; and
+"adjoint", the column-collapse calibration gradient (2,000 spheres, 500
+steps sampled every 50) as forward + backward against the forward.
 """
 
 import argparse
@@ -39,6 +47,14 @@ def log(msg):
 
 
 def cpu_model():
+    """The CPU's full model name (/proc/cpuinfo, else lscpu)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
     try:
         out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, text=True).stdout
         for line in out.splitlines():
@@ -47,6 +63,18 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def cpu_threads():
+    """The threads this process may use: OMP_NUM_THREADS when set (the
+    machine's share for this job), else the CPUs in its affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.strip().isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def kernel_source_hash():
@@ -58,8 +86,9 @@ def kernel_source_hash():
 
 
 def measured_traffic(z, nw):
-    """HBM bytes per launch of the contact kernel from the committed PMC
-    summary, when it was measured on the current kernel sources."""
+    """(HBM bytes per launch of the contact kernel, the command that measured
+    them) from the committed PMC summary, when it was measured on the
+    current kernel sources; else None.  Not measured in this run."""
     fn = os.path.join(ROOT, "profiles", "traffic_contact.json")
     if not os.path.exists(fn):
         return None
@@ -70,7 +99,7 @@ def measured_traffic(z, nw):
         return None
     if t.get("kernel_source_sha256") != kernel_source_hash():
         return None
-    return t.get("bytes_per_launch")
+    return t.get("bytes_per_launch"), t.get("command", "")
 
 
 def build_workload(args):
@@ -101,16 +130,17 @@ def apply_settings(ctx, args):
         log("set %s = %g" % (name.strip(), float(val)))
 
 
-def batch_images(args):
-    """The batch workload: `args.batch` simulations of log-uniform size in
-    500..20000 particles.  "shear" (default): direct-shear beds, each a
-    settled 500-particle periodic cell (one of four seeds) tiled in x and y
-    to its size, with its own friction (0.3..0.7), normal stress
-    (5..160 kPa), stiffness (log-uniform 1e6..1e8 N/m) and dt; "gas":
-    granular gases collapsing in closed boxes from a lattice start (few
-    contacts within a short run)."""
+def batch_images(args, nsims):
+    """The batched sweep: `nsims` consolidation beds of log-uniform size in
+    [500, 20000] particles, each a settled 500-particle cell (radii uniform
+    in [0.005, 0.01] m, one of four seeds, periodic in x and y, under a
+    stress-controlled top wall) tiled in x and y to its size, with its own
+    normal stress (uniform in [5, 160] kPa), stiffness (log-uniform in
+    [1e6, 1e8] N/m, with its own critical damping and dt) and friction
+    (uniform in [0.3, 0.7]).  kind "gas": granular gases collapsing in
+    closed boxes from a lattice start."""
     from sphere_amd import engine, scenes
-    sizes = scenes.batch_sizes(args.batch, seed=args.seed)
+    sizes = scenes.batch_sizes(nsims, seed=args.seed)
     if args.batch_kind == "gas":
         images = [scenes.lattice_box(int(n), seed=args.seed + k).tobytes()
                   for k, n in enumerate(sizes)]
@@ -121,57 +151,53 @@ def batch_images(args):
         return engine.Context(img)
 
     bases = [scenes.settled_shear_cell(500, args.seed + k, open_stepper=opener,
-                                       settle_steps=args.settle_steps,
-                                       consolidate_steps=args.consolidate_steps)
+                                       settle_steps=args.cell_settle_steps,
+                                       consolidate_steps=args.cell_consolidate_steps,
+                                       r_uniform=(0.005, 0.01), shear=False)
              for k in range(4)]
     cells = scenes.batch_shear_cells(sizes, bases, seed=args.seed, kn_range=(1.0e6, 1.0e8))
-    log("batch of sheared beds prepacked in %.1f s" % (time.time() - t0))
+    log("batch of %d consolidation beds prepacked in %.1f s" % (nsims, time.time() - t0))
     return ([c.tobytes() for c in cells],
-            "batched direct-shear beds (tiled settled 500-particle cells), contactmodel 2, "
-            "per-simulation mu 0.3..0.7, normal stress 5..160 kPa, k_n log-uniform "
-            "1e6..1e8 N/m with its own critical damping and dt")
+            "batched consolidation beds (tiled settled 500-particle cells, radii U[0.005, "
+            "0.01] m), contactmodel 2, per-simulation normal stress U[5, 160] kPa, k_n "
+            "log-uniform [1e6, 1e8] N/m with its own critical damping and dt, mu U[0.3, 0.7]")
 
 
-def batch_bench(args):
-    """S simulations of log-uniform size advanced `steps` steps: one batched
-    context (all of them per launch) against the per-simulation loop (one
-    context each, all enqueued before one synchronisation, and one after
-    another)."""
+def measure_batch(args, images, steps, warmup, graph_steps):
+    """One batched context against the plain per-simulation loop (one
+    context per simulation, stepped one after another) and against one
+    context and stream per simulation all enqueued before one
+    synchronisation; the same engine settings for all."""
     from sphere_amd import engine
-    images, workload = batch_images(args)
     # np is the u32 after the f64 version and the i32 nd (src/file_io.cpp:430-433).
     sizes = [struct.unpack_from("<I", im, 12)[0] for im in images]
     total = int(sum(sizes))
-    steps = args.steps
-    log("%d simulations, %d particles in total (%d..%d)" % (len(images), total, min(sizes),
-                                                             max(sizes)))
 
     def timed(fn):
         t0 = time.perf_counter()
         fn()
         return time.perf_counter() - t0
 
+    def setup(c):
+        if args.contact_lanes:
+            c.set_scalar("contact_lanes", args.contact_lanes)
+        c.set_scalar("graph_steps", graph_steps)
+        apply_settings(c, args)
+
     ctx = engine.Context(images=images)
-    if args.contact_lanes:
-        ctx.set_scalar("contact_lanes", args.contact_lanes)
-    apply_settings(ctx, args)
-    ctx.step(args.warmup, sync=True)
+    setup(ctx)
+    ctx.step(warmup, sync=True)
     t_batch = timed(lambda: ctx.step(steps, sync=True))
     z = sum(int((ctx.get("contacts", sim=k) != sizes[k]).sum()) for k in range(len(images)))
-    log("batched: %.4f s for %d steps" % (t_batch, steps))
     ctx.profile(True)
     ctx.step(min(steps, 50), sync=True)
     rep = ctx.profile_report()
     ctx.profile(False)
     ctx.close()
-    log("batched kernel means (us): " + ", ".join("%s %.2f" % (k, v[0]) for k, v in rep.items()))
-
     ctxs = [engine.Context(im) for im in images]
     for c in ctxs:
-        if args.contact_lanes:
-            c.set_scalar("contact_lanes", args.contact_lanes)
-        apply_settings(c, args)
-        c.step(args.warmup, sync=False)
+        setup(c)
+        c.step(warmup, sync=False)
     for c in ctxs:
         c.sync()
 
@@ -189,36 +215,34 @@ def batch_bench(args):
     t_seq = timed(sequential)
     for c in ctxs:
         c.close()
-    log("per-simulation loop: %.4f s enqueued together, %.4f s one after another"
-        % (t_ovl, t_seq))
     value = total * steps / t_batch
-    plain_loop = total * steps / t_seq
-    overlapped_loop = total * steps / t_ovl
+    log("batch of %d (%d particles): batched %.4f s, per-simulation loop %.4f s, enqueued "
+        "together %.4f s for %d steps" % (len(images), total, t_batch, t_seq, t_ovl, steps))
+    return {
+        "value": round(value, 1), "unit": "particle-steps/s", "steps": steps,
+        "nsims": len(images), "np_total": total, "np_min": int(min(sizes)),
+        "np_max": int(max(sizes)), "batched_s": round(t_batch, 5),
+        "per_sim_loop_s": round(t_seq, 5), "overlapped_loop_s": round(t_ovl, 5),
+        "speedup_vs_per_sim_loop": round(t_seq / t_batch, 3),
+        "speedup_vs_overlapped_loop": round(t_ovl / t_batch, 3),
+        "contacts_per_particle_end": round(z / total, 3), "graph_steps": graph_steps,
+        "kernel_us": {k: round(v[0], 3) for k, v in rep.items()},
+    }
+
+
+def batch_bench(args):
+    """The batched sweep alone, as its own JSON line."""
+    images, workload = batch_images(args, args.batch)
+    r = measure_batch(args, images, args.steps, args.warmup, args.graph_steps)
     line = {
         "metric": "particle-steps/s, batch of %d simulations FP64" % len(images),
-        "value": round(value, 1),
-        "unit": "particle-steps/s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * t_batch / steps, 5),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
+        "value": r["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * r["batched_s"] / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": workload,
-                   "nsims": len(images), "np_total": total, "np_min": int(min(sizes)),
-                   "np_max": int(max(sizes)), "seed": args.seed,
-                   "contacts_per_particle_end": round(z / total, 3)},
-        "kernel_us": {k: round(v[0], 3) for k, v in rep.items()},
-        "per_sim_loop": {"value": round(plain_loop, 1), "s": round(t_seq, 5),
-                         "note": "one context per simulation, stepped one after another"},
-        "overlapped_loop": {"value": round(overlapped_loop, 1), "s": round(t_ovl, 5),
-                            "note": "one context (and stream) per simulation, all enqueued "
-                                    "before one synchronisation"},
-        "speedup_vs_per_sim_loop": round(value / plain_loop, 3),
-        "speedup_vs_overlapped_loop": round(value / overlapped_loop, 3),
+        "config": {"workload": workload, "nsims": r["nsims"], "np_total": r["np_total"],
+                   "np_min": r["np_min"], "np_max": r["np_max"], "seed": args.seed},
+        "batched": r,
     }
     print(json.dumps(line), flush=True)
     return 0
@@ -278,19 +302,24 @@ def adjoint_bench(args, img):
     return 0
 
 
-def collapse_bench(args):
+def collapse_column(args, n, seed):
+    from sphere_amd import engine, scenes
+    return scenes.collapse_column(n, seed=seed, k_n=1.0e7, damping_frac=0.2,
+                                  settle_steps=args.column_settle_steps,
+                                  open_stepper=lambda img: engine.Context(img))
+
+
+def measure_collapse(args, reps=5):
     """The differentiable column collapse: 2,000 spheres (radii uniform in
     [0.008, 0.012] m) settled in a 0.3 m column and released onto a
     frictional floor; 500 steps sampled every 50; the MSE of positions
     against a target made with k_n* = 2e7, gamma_n* = 0.3 critical, from
-    k_n = 1e7, gamma_n = 0.2 critical.  Times the torch forward (the
-    trajectory) against forward + backward (the loss gradient with respect
-    to k_n, gamma_n, mu and the initial velocities)."""
-    import numpy
+    k_n = 1e7, gamma_n = 0.2 critical.  The torch forward (the trajectory)
+    against forward + backward (the loss gradient with respect to k_n,
+    gamma_n, mu and the initial velocities), best of `reps`."""
     import torch
-    from sphere_amd import autograd, engine, scenes
-    s = scenes.collapse_column(2000, seed=args.seed, k_n=1.0e7, damping_frac=0.2,
-                               open_stepper=lambda img: engine.Context(img))
+    from sphere_amd import autograd, scenes
+    s = collapse_column(args, 2000, args.seed)
     traj = autograd.Trajectory(s, 50, 10, params=("k_n", "gamma_n", "mu"))
     r_min, rho = float(s.radius.min()), float(s.rho[0])
 
@@ -301,7 +330,6 @@ def collapse_bench(args):
         target = traj(torch.tensor([2.0e7, crit(2.0e7, 0.3), 0.5], dtype=torch.float64))
     theta0 = [1.0e7, crit(1.0e7, 0.2), 0.5]
     v0 = torch.as_tensor(s.vel[:, :3].copy())
-    reps = 5
     t_fwd = t_all = float("inf")
     for _ in range(reps):
         t0 = time.perf_counter()
@@ -319,14 +347,97 @@ def collapse_bench(args):
         "dL/dk_n %.3e, dL/dgamma_n %.3e, dL/dmu %.3e, |dL/dv0| %.3e"
         % (t_fwd, t_all, t_all / t_fwd, float(loss.detach()), g[0], g[1], g[2],
            float(v.grad.norm())))
-    print(json.dumps({"metric": "trajectory-loss gradient cost / forward cost", "value":
-                      round(t_all / t_fwd, 3), "unit": "ratio", "n_gpus": 1, "steps": 500,
-                      "higher_is_better": False, "np": 2000, "forward_s": round(t_fwd, 5),
-                      "forward_backward_s": round(t_all, 5), "data": "synthetic",
-                      "config": {"workload": "granular column collapse, 2000 spheres, 500 "
-                                 "steps sampled every 50, MSE vs k_n*=2e7, gamma_n*=0.3 crit",
+    return {"value": round(t_all / t_fwd, 3), "unit": "(forward + backward) / forward",
+            "higher_is_better": False, "forward_s": round(t_fwd, 5),
+            "forward_backward_s": round(t_all, 5), "np": 2000, "steps": 500,
+            "workload": "granular column collapse, 2000 spheres, 500 steps sampled every 50, "
+                        "MSE vs k_n*=2e7, gamma_n*=0.3 crit; gradient w.r.t. k_n, gamma_n, "
+                        "mu, v0"}
+
+
+def collapse_bench(args):
+    r = measure_collapse(args)
+    print(json.dumps({"metric": "trajectory-loss gradient cost / forward cost",
+                      "value": r["value"], "unit": "ratio", "n_gpus": 1, "steps": 500,
+                      "higher_is_better": False, "np": 2000, "forward_s": r["forward_s"],
+                      "forward_backward_s": r["forward_backward_s"], "data": "synthetic",
+                      "config": {"workload": r["workload"],
                                  "params": ["k_n", "gamma_n", "mu", "v0"]}}), flush=True)
     return 0
+
+
+def measure_batch_calibration(args, nsims=64, n=4000, steps=300, every=50):
+    """Many calibrations at once: `nsims` collapse columns of `n` spheres
+    (four settled columns, each simulation with its own seeded initial
+    velocities N(0, 0.01 m/s) and its own target parameters k_n* log-uniform
+    in [1e7, 4e7] N/m, gamma_n* in [0.2, 0.4] critical, mu* in [0.3, 0.7]),
+    `steps` steps sampled every `every`, the MSE of positions of each
+    against its own target; one gradient step of all the (k_n, gamma_n,
+    mu) fits as a batched trajectory + batched adjoint, against the plain
+    loop of one trajectory + adjoint per simulation."""
+    import numpy
+    import torch
+    from sphere_amd import autograd, scenes
+    from sphere_amd.sim import sim
+    bases = [collapse_column(args, n, args.seed + 10 * k) for k in range(4)]
+    rng = numpy.random.default_rng(args.seed)
+    sims, stars = [], []
+    for k in range(nsims):
+        s = sim().frombytes(bases[k % 4].tobytes())
+        s.vel[:, :3] = rng.normal(0.0, 0.01, (n, 3))
+        r_min, rho = float(s.radius.min()), float(s.rho[0])
+        ks = float(numpy.exp(rng.uniform(numpy.log(1.0e7), numpy.log(4.0e7))))
+        stars.append([ks, rng.uniform(0.2, 0.4) * scenes.critical_damping(r_min, rho, ks),
+                      rng.uniform(0.3, 0.7)])
+        sims.append(s)
+    nsamp = steps // every
+    batch = autograd.BatchTrajectory(sims, every, nsamp, params=("k_n", "gamma_n", "mu"))
+    with torch.no_grad():
+        target = batch(torch.tensor(stars, dtype=torch.float64))
+    theta0 = numpy.array([[float(s.k_n[0]), float(s.gamma_n[0]), float(s.mu_s[0])]
+                          for s in sims])
+
+    # Each simulation's loss is its own MSE; their sum has, for each
+    # simulation's parameters, that simulation's gradient.
+    scale = 1.0 / (nsamp * n * 3)
+
+    def batched():
+        th = torch.tensor(theta0, requires_grad=True)
+        (((batch(th) - target) ** 2).sum() * scale).backward()
+        return th.grad.numpy().copy()
+
+    def forward_only():
+        with torch.no_grad():
+            batch(torch.tensor(theta0))
+
+    singles = [autograd.Trajectory(s, every, nsamp, params=("k_n", "gamma_n", "mu"))
+               for s in sims]
+
+    def loop():
+        out = []
+        for k, t in enumerate(singles):
+            th = torch.tensor(theta0[k], requires_grad=True)
+            a, b = batch.offsets[k], batch.offsets[k + 1]
+            (((t(th) - target[:, a:b]) ** 2).sum() * scale).backward()
+            out.append(th.grad.numpy().copy())
+        return numpy.array(out)
+
+    def timed(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return time.perf_counter() - t0, r
+
+    batched()  # warm-up (allocations)
+    t_b, g_b = timed(batched)
+    t_f, _ = timed(forward_only)
+    t_l, g_l = timed(loop)
+    same = bool(numpy.array_equal(g_b, g_l))
+    log("batched calibration of %d x %d: batched %.3f s (forward %.3f s), loop %.3f s, "
+        "gradients identical: %s" % (nsims, n, t_b, t_f, t_l, same))
+    return {"nsims": nsims, "np_each": n, "steps": steps, "every": every,
+            "batched_s": round(t_b, 4), "batched_forward_s": round(t_f, 4),
+            "per_sim_loop_s": round(t_l, 4), "speedup_vs_per_sim_loop": round(t_l / t_b, 3),
+            "adjoint_ratio": round(t_b / t_f, 3), "gradients_bit_identical": same}
 
 
 def main():
@@ -338,11 +449,22 @@ def main():
     ap.add_argument("--seed", type=int, default=2001)
     ap.add_argument("--settle-steps", type=int, default=40000)
     ap.add_argument("--consolidate-steps", type=int, default=10000)
+    ap.add_argument("--cell-settle-steps", type=int, default=20000,
+                    help="settling steps of the batched sweep's 500-particle base cells")
+    ap.add_argument("--cell-consolidate-steps", type=int, default=5000)
+    ap.add_argument("--column-settle-steps", type=int, default=100000)
     ap.add_argument("--profile-steps", type=int, default=200)
-    ap.add_argument("--cpu-steps", type=int, default=1000,
-                    help="oracle steps of the CPU baseline sample (about 15 s on 16 threads)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=600,
+                    help="oracle steps of the all-threads CPU baseline sample")
+    ap.add_argument("--cpu-steps-1t", type=int, default=40,
+                    help="oracle steps of the single-thread CPU baseline sample")
+    ap.add_argument("--check-steps", type=int, default=20,
+                    help="steps of the full-size bit-exact check of the timed state")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="leave the batched and adjoint parts out of the default line")
+    ap.add_argument("--batch-steps", type=int, default=100,
+                    help="steps of the batched-sweep part of the default line")
     ap.add_argument("--contact-lanes", type=int, default=0,
                     help="work-items per particle in the contact kernel (1, 4, 8, 16)")
     ap.add_argument("--sort-mode", type=int, default=-1,
@@ -354,16 +476,18 @@ def main():
     ap.add_argument("--collapse", action="store_true",
                     help="instead: time the column-collapse calibration gradient (torch "
                          "autograd over the trajectory adjoint) against its forward pass")
+    ap.add_argument("--batch-calibration", type=int, default=0, metavar="S",
+                    help="instead: S batched calibrations of 4000-sphere collapse columns "
+                         "over 300 steps against the per-simulation loop")
     ap.add_argument("--autotune", type=int, default=0, metavar="K",
-                    help="before the warm-up, run sph_autotune with K steps per trial "
-                         "(contact lanes, workgroup order; identical results)")
+                    help="before the warm-up, run sph_autotune with K steps per trial")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="steady-state chunk replayed from a captured HIP graph "
                          "(identical results; 0 = plain launches)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (5 times): the "
                          "reference kernel for scaling HBM byte counters")
-    ap.add_argument("--batch-kind", choices=("shear", "gas"), default="shear")
+    ap.add_argument("--batch-kind", choices=("consolidation", "gas"), default="consolidation")
     ap.add_argument("--batch", type=int, default=0, metavar="S",
                     help="instead: S independent simulations (log-uniform 500..20000 "
                          "particles) stepped as one batched context vs one context each")
@@ -396,6 +520,13 @@ def main():
         return batch_bench(args)
     if args.collapse:
         return collapse_bench(args)
+    if args.batch_calibration:
+        r = measure_batch_calibration(args, nsims=args.batch_calibration)
+        print(json.dumps({"metric": "batched calibration speedup vs per-simulation loop",
+                          "value": r["speedup_vs_per_sim_loop"], "unit": "ratio", "n_gpus": 1,
+                          "higher_is_better": True, "data": "synthetic", "config": r}),
+              flush=True)
+        return 0
 
     from sphere_amd import engine, scenes
 
@@ -457,33 +588,76 @@ def main():
     roofline = {"bound": "hbm", "kernel": "k_contact_g<2, 4, true>" if fused else "k_contact_g",
                 "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "algorithmic_bytes_per_launch": round(n * b_contact),
+                "traffic": traffic[0] if traffic else None,
+                "traffic_source": ("not measured in this run: rocprofv3 --pmc FETCH_SIZE and "
+                                   "WRITE_SIZE of the same kernel sources, "
+                                   "profiles/traffic_contact.json" if traffic else None),
+                "algorithmic_bytes_per_launch": round(n * b_contact),
                 "mean_launch_us": round(t_contact * 1e6, 3),
+                "mean_launch_us_source": "HIP events around each launch on the context stream",
                 "contacts_per_particle": round(zz, 3)}
     # Whole-step algorithmic floor, B_min = 528 + 72 Z bytes per particle-step.
     step_bytes = 528.0 + 72.0 * zz
     step_gbs = step_bytes * value / 1e9
+    state = ctx.tobytes()
+    ctx.close()
 
     cpu = None
+    check = None
     if not args.no_cpu:
+        import numpy
         import oracle
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        threads = cpu_threads()
+        model = cpu_model()
+
+        def cpu_rate(nthreads, steps):
+            oracle.set_threads(nthreads)
+            o = oracle.Oracle(state)
+            o.step(1)
+            c0 = time.perf_counter()
+            o.step(steps)
+            c1 = time.perf_counter()
+            return n * steps / (c1 - c0), c1 - c0, o
+
+        rate_all, s_all, o = cpu_rate(threads, args.cpu_steps)
+        rate_1, s_1, _ = cpu_rate(1, args.cpu_steps_1t)
         oracle.set_threads(threads)
-        state = ctx.tobytes()
-        o = oracle.Oracle(state)
-        o.step(1)
-        c0 = time.perf_counter()
-        o.step(args.cpu_steps)
-        c1 = time.perf_counter()
-        cpu_rate = n * args.cpu_steps / (c1 - c0)
-        cpu = {"value": round(cpu_rate, 1), "unit": "particle-steps/s", "cores": threads,
+        cpu = {"value": round(rate_all, 1), "unit": "particle-steps/s", "cores": threads,
                "kind": "port",
+               "single_thread": {"value": round(rate_1, 1), "cores": 1,
+                                 "steps": args.cpu_steps_1t, "s": round(s_1, 2)},
+               "cpu": model,
                "sample": "CPU oracle (oracle/sphere_oracle.cpp, g++ -O3 -fopenmp "
-                         "-ffp-contract=off), %d steps of the same %d-particle shear state "
-                         "after the GPU timed region, %.1f s, CPU: %s"
-                         % (args.cpu_steps, n, c1 - c0, cpu_model())}
-        log("cpu baseline: %.4e particle-steps/s on %d threads" % (cpu_rate, threads))
-    ctx.close()
+                         "-ffp-contract=off) from the %d-particle shear state after the GPU "
+                         "timed region: %d steps on %d threads (%.1f s) and %d steps on one "
+                         "thread (%.1f s)" % (n, args.cpu_steps, threads, s_all,
+                                              args.cpu_steps_1t, s_1)}
+        log("cpu baseline: %.4e particle-steps/s on %d threads, %.4e on one (%s)"
+            % (rate_all, threads, rate_1, model))
+        # The timed bed at full size: a fresh GPU context and the oracle from
+        # the same image, the same steps, every field compared bit for bit.
+        ref = oracle.Oracle(state)
+        ref.step(args.check_steps)
+        fields = ("perm", "cellid", "contacts", "x", "vel", "angvel", "force", "torque",
+                  "delta_t", "walls_nx", "walls_mvfd")
+        with engine.Context(state) as chk:
+            chk.step(args.check_steps)
+            diff = [f for f in fields if not numpy.array_equal(chk.get(f), ref.get(f))]
+        check = {"steps": args.check_steps, "np": n, "fields": list(fields),
+                 "bit_exact": not diff, "differing": diff}
+        log("full-size check of the timed bed over %d steps: %s"
+            % (args.check_steps, "bit-exact" if not diff else "DIFFERS in %s" % diff))
+
+    batched = adjoint = None
+    if not args.no_extras:
+        images, _ = batch_images(args, 256)
+        batched = measure_batch(args, images, max(args.batch_steps, 50), 100, 50)
+        batched["workload"] = ("256 consolidation beds of log-uniform size in [500, 20000] "
+                               "(radii U[0.005, 0.01] m, normal stress U[5, 160] kPa, k_n "
+                               "log-uniform [1e6, 1e8] N/m, mu U[0.3, 0.7], own dt each); "
+                               "one batched context vs one context per simulation stepped "
+                               "one after another")
+        adjoint = measure_collapse(args, reps=3)
 
     line = {
         "metric": "particle-steps/s at 1e5 particles FP64",
@@ -508,6 +682,9 @@ def main():
                           "achieved_GBs": round(step_gbs, 2),
                           "frac": round(step_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
+        "state_check": check,
+        "batched": batched,
+        "adjoint": adjoint,
         "kernel_us": {k: v[0] for k, v in rep.items()},
         "timed_events": {"re_sorts": sorts, "list_rebuilds": builds},
     }

@@ -144,12 +144,24 @@ enum sph_scalar {
     SPH_S_K_N = 22,           /* per simulation, read-only: the normal stiffness
                                  params.k_n, which DEM::forcechains scales the
                                  overlaps by (src/sphere.cpp:789). */
-    SPH_S_INTEGRATE_LANES = 23 /* work-items per particle in the integration
+    SPH_S_INTEGRATE_LANES = 23,/* work-items per particle in the integration
                                  pass: 1 (default) or 4 (one vector component
                                  each, then a wall-partials pass; used with a
                                  fixed grid height and no wmode-3 wall;
                                  measured slower on the sheared bed, 22 + 6
                                  us against 18).  Identical results. */
+    SPH_S_BOND_K = 24,        /* per simulation.  0 (default, the reference,
+                                 src/cohesion.cuh:124,177): bonds take k_n A and
+                                 k_t A as stiffness; > 0: this bond stiffness
+                                 in N/m for both (bending and twisting
+                                 stiffness per unit area bond_k / A).  Not in
+                                 the .bin (fixed layout): set after opening. */
+    SPH_S_BOND_BREAK_EXT = 25 /* per simulation.  0 (default, the reference,
+                                 src/cohesion.cuh:193-206): bonds break on the
+                                 stress criterion (sigma_b, tau_b); > 0: a bond
+                                 breaks when its normal extension reaches this
+                                 fraction of the smaller radius.  Not in the
+                                 .bin: set after opening. */
 };
 
 /* Open flags. */
@@ -223,17 +235,24 @@ int sph_get_cells(sph_ctx* ctx, uint32_t* cell_id, uint32_t* perm,
 
 /* Adjoint (reverse mode) of nsteps steps; no counterpart in the reference.
  * Runs the steps forward from the current state, recording each step's
- * inputs and contact rows, then backwards with that contact list and every
- * branch frozen, and returns the gradient of
+ * inputs, contact rows and bonds, then backwards with that contact list and
+ * every branch (force clamps, stick or slip, wall friction, bond breakage)
+ * frozen, and returns the gradient of
  *     L = sum_i dL_dx[i] . x_i + dL_dv[i] . v_i + dL_dw[i] . w_i
  * (state after the steps) with respect to the state before them (g_x0,
  * g_v0, g_w0: np x 3 each) and to the parameters (g_params, 9 values:
  * k_n, k_t, gamma_n, gamma_t, mu_s, mu_d, gamma_wn, gamma_wt, mu_wd).  Any
- * pointer may be NULL.  Lengths are differentiated as exact roots, walls
- * follow their recorded trajectories.  Covers contact model 2 without
- * cohesion, bonds, E-scaled stiffness, wmode 3, adaptive grids or
- * rate-state particles (SPH_E_UNSUPPORTED otherwise); single-simulation
- * contexts.  The context is left at the state after the steps. */
+ * pointer may be NULL.  Lengths are differentiated as exact roots.  The
+ * reverse covers contactLinear (src/contactmodels.cuh:244-411),
+ * contactLinear_wall (:10-117), bondsLinear (src/cohesion.cuh:8-233), the
+ * particles' TY3 integration (src/integration.cuh:14-305) and the walls'
+ * equation of motion for fixed, stress-controlled and velocity-controlled
+ * walls (wmode 0, 1, 2; src/integration.cuh:344-450).  Not covered
+ * (SPH_E_UNSUPPORTED): contact models 1 and 3, cohesion, E-scaled
+ * stiffness, wmode 3, adaptive grids, rate-state particles.  Parameter
+ * sums are formed in a fixed order: gradients are reproducible bit for
+ * bit.  Single-simulation contexts (see sph_adjoint_batch).  The context
+ * is left at the state after the steps. */
 int sph_adjoint(sph_ctx* ctx, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
                 const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
                 double* g_params);
@@ -250,6 +269,21 @@ int sph_adjoint(sph_ctx* ctx, uint64_t nsteps, const double* dL_dx, const double
 int sph_adjoint_samples(sph_ctx* ctx, uint64_t every, uint64_t nsamples, const double* dL_dx,
                         const double* dL_dv, double* g_x0, double* g_v0, double* g_w0,
                         double* g_params);
+
+/* The same two adjoints over every simulation of a context (a batch of
+ * independent simulations, e.g. one calibration per simulation); no
+ * counterpart in the reference.  Particle arrays (dL_dx, dL_dv, dL_dw and
+ * g_x0, g_v0, g_w0; for the samples variant nsamples of them) concatenate
+ * the simulations' particles in simulation order (sum of np rows x 3);
+ * g_params receives nsims x 9 values, simulation by simulation, each the
+ * gradient with respect to that simulation's own parameters.  On a
+ * single-simulation context they equal sph_adjoint / sph_adjoint_samples. */
+int sph_adjoint_batch(sph_ctx* ctx, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
+                      const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
+                      double* g_params);
+int sph_adjoint_samples_batch(sph_ctx* ctx, uint64_t every, uint64_t nsamples,
+                              const double* dL_dx, const double* dL_dv, double* g_x0,
+                              double* g_v0, double* g_w0, double* g_params);
 
 /* Restart sidecar (no counterpart in the reference: src/file_io.cpp:420-731
  * writes none of this, so a run resumed from an output .bin starts with

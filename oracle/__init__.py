@@ -104,6 +104,8 @@ def _lib(dual=False):
                                    ctypes.c_void_p]
         L.orc_get_threads.restype = ctypes.c_int
         L.orc_set_contact_log.argtypes = [ctypes.c_int]
+        L.orc_set_bond_model.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.orc_set_bond_model.restype = None
         L.orc_contact_log_width.restype = ctypes.c_int
         L.orc_get_contact_log.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
         if dual:
@@ -210,6 +212,12 @@ class Oracle:
         rc = self._L.orc_set_field(self._h, fid, a.ctypes.data, a.nbytes)
         if rc != 0:
             raise RuntimeError("orc_set_field(%s) failed: %d" % (name, rc))
+
+    def set_bond_model(self, bond_k=0.0, bond_break_ext=0.0):
+        """The engine's bond options for this simulation: a bond stiffness in
+        N/m and a break at a normal extension of bond_break_ext times the
+        smaller radius (0 = the reference's law)."""
+        self._L.orc_set_bond_model(self._h, float(bond_k), float(bond_break_ext))
 
     def contact_log(self):
         """The contact records of the last interact stage (np x NC x width,

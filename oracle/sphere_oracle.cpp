@@ -152,6 +152,9 @@ struct orc_sim {
     // Test-only per-contact record of the last interact stage (models 2 and
     // 3), np x NC x CLOG_W doubles, filled when g_contact_log is set.
     std::vector<double> clog;
+    // The engine's bond options (SPH_S_BOND_K, SPH_S_BOND_BREAK_EXT); 0 =
+    // the reference's bondsLinear.
+    double bond_k = 0.0, bond_break_ext = 0.0;
 
     uint32_t ncells() const { return num[0] * num[1] * num[2]; }
     uint32_t blocks() const { return (np + 255u) / 256u; }
@@ -900,8 +903,12 @@ void stage_bonds(orc_sim* s) {
         const Real delta_n = delta0_n + ddelta_n;
         const R3 ddelta_t = ddelta - n * dot(n, ddelta);
         const R3 delta_t = delta0_t + ddelta_t;
-        const R3 f_n = (P.k_n * A * delta_n + P.gamma_n * ddelta_n / dt) * n;
-        const R3 f_t = -P.k_t * A * delta_t - P.gamma_t * ddelta_t / dt;
+        // Stiffness per unit area: k_n, k_t, or the engine's bond option
+        // (a stiffness in N/m, bond_k / A per unit area).
+        const Real kn = s->bond_k > 0.0 ? s->bond_k / A : P.k_n;
+        const Real kt = s->bond_k > 0.0 ? s->bond_k / A : P.k_t;
+        const R3 f_n = (kn * A * delta_n + P.gamma_n * ddelta_n / dt) * n;
+        const R3 f_t = -kt * A * delta_t - P.gamma_t * ddelta_t / dt;
         const R3 f = f_n + f_t;
         const R3 mn = -n;
         omega0_t = omega0_t - mn * dot(omega0_t, mn);
@@ -911,12 +918,20 @@ void stage_bonds(orc_sim* s) {
         const Real omega_n = omega0_n + domega_n;
         const R3 domega_t = domega - domega_n * mn;
         const R3 omega_t = omega0_t + domega_t;
-        const R3 t_n = -P.k_t * J * omega_n * mn;
-        const R3 t_t = -P.k_n * I * omega_t;
+        const R3 t_n = -kt * J * omega_n * mn;
+        const R3 t_t = -kn * I * omega_t;
         const R3 t_j = t_n + t_t;
-        const Real sigma_max = length(f_n) / A + length(t_t) * R_bar / I;
-        const Real tau_max = length(f_t) / A + length(t_n) * R_bar / J;
-        if (sigma_max >= P.sigma_b || tau_max >= P.tau_b) {
+        bool breaks;
+        if (s->bond_break_ext > 0.0) {
+            // The engine's option: normal extension (delta_n < 0 when the
+            // pair separates) of bond_break_ext times the smaller radius.
+            breaks = -delta_n >= s->bond_break_ext * fmin(x_i.w, x_j.w);
+        } else {
+            const Real sigma_max = length(f_n) / A + length(t_t) * R_bar / I;
+            const Real tau_max = length(f_t) / A + length(t_n) * R_bar / J;
+            breaks = sigma_max >= P.sigma_b || tau_max >= P.tau_b;
+        }
+        if (breaks) {
             s->bonds[2 * b] = P.nb0;  // src/cohesion.cuh:205 (sets x to nb0)
             continue;
         }
@@ -1381,6 +1396,10 @@ int orc_dual_get_tangent(const orc_sim* s, int field, double* out) {
 #endif
 void orc_set_exact_length(int on) { g_exact_length = on != 0; }
 void orc_set_contact_log(int on) { g_contact_log = on != 0; }
+void orc_set_bond_model(orc_sim* s, double bond_k, double bond_break_ext) {
+    s->bond_k = bond_k;
+    s->bond_break_ext = bond_break_ext;
+}
 int orc_contact_log_width(void) { return CLOG_W; }
 int orc_get_contact_log(const orc_sim* s, double* dst, size_t n) {
     const size_t want = (size_t)s->np * NC * CLOG_W;

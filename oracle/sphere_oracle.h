@@ -76,6 +76,10 @@ void orc_set_exact_length(int on);
 void orc_set_contact_log(int on);
 int orc_contact_log_width(void);
 int orc_get_contact_log(const orc_sim* s, double* dst, size_t n);
+/* The engine's bond options (SPH_S_BOND_K in N/m, SPH_S_BOND_BREAK_EXT as
+ * a fraction of the smaller radius; 0 = the reference's law) for this
+ * simulation. */
+void orc_set_bond_model(orc_sim* s, double bond_k, double bond_break_ext);
 /* Each dynamic wall integrates its own force sum instead of wall 0's (the
  * engine's optional correction of the reference's P8 behaviour). */
 void orc_set_wall_sums_per_wall(int on);

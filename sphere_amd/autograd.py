@@ -5,7 +5,9 @@ initial positions and velocities) to the particle positions sampled every
 `every` steps, nsamples times, running the HIP engine forward; its backward
 is the engine's adjoint of a trajectory loss (sph_adjoint_samples), so that
 any torch loss on the sampled positions has gradients with respect to k_n,
-gamma_n, mu, x0 and v0.  No counterpart in the reference.
+gamma_n, mu, x0 and v0.  `BatchTrajectory(sims, every, nsamples)` does the
+same for many simulations at once, each with its own parameters, in one
+batched context and one batched adjoint.  No counterpart in the reference.
 
 The forward pass keeps one checkpoint of the whole simulation state at its
 start (the .bin image and the restart sidecar: contact rows,
@@ -125,6 +127,94 @@ class Trajectory:
         return gtheta, g["x"], g["v"]
 
 
+class _BatchStart:
+    """The batched forward pass's starting state: one image and one
+    sidecar per simulation."""
+
+    def __init__(self, ctx):
+        self.dir = tempfile.mkdtemp(prefix="sphere-batch-")
+        self.images, self.states = [], []
+        for k in range(ctx.nsims):
+            path = os.path.join(self.dir, "start%04d.state" % k)
+            ctx.write_state(path, sim=k)
+            self.states.append(path)
+            self.images.append(ctx.tobytes(sim=k))
+
+    def __del__(self):
+        shutil.rmtree(self.dir, ignore_errors=True)
+
+
+class BatchTrajectory:
+    """Many independent trajectories (one per simulation) in one batched
+    context, differentiable: each simulation has its own parameters.
+
+    `sims` is a list of sphere_amd.sim.sim.  Calling the object with theta
+    of shape (nsims, len(params)) and optional x0, v0 of shape (rows, 3)
+    (the simulations' particles concatenated in order) returns a float64
+    tensor of shape (nsamples, rows, 3).  Its backward runs one batched
+    adjoint (sph_adjoint_samples_batch): each simulation's parameter
+    gradient comes from its own rows of the loss gradient."""
+
+    def __init__(self, sims, every, nsamples, params=("k_n", "gamma_n", "mu"), ties=None,
+                 device=-1):
+        if every <= 0 or nsamples <= 0 or not sims:
+            raise ValueError("every, nsamples and the batch must be non-empty")
+        self.parts = [Trajectory(s, every, nsamples, params=params, ties=ties, device=device)
+                      for s in sims]
+        self.every = int(every)
+        self.nsamples = int(nsamples)
+        self.params = tuple(params)
+        self.ties = self.parts[0].ties
+        self.device = device
+        self.sizes = [t.np for t in self.parts]
+        self.offsets = numpy.concatenate([[0], numpy.cumsum(self.sizes)]).astype(int)
+        self.rows = int(self.offsets[-1])
+
+    def images(self, theta, x0=None, v0=None):
+        theta = numpy.asarray(theta, dtype=numpy.float64).reshape(len(self.parts), -1)
+        out = []
+        for k, t in enumerate(self.parts):
+            a, b = self.offsets[k], self.offsets[k + 1]
+            out.append(t.image(theta[k], None if x0 is None else numpy.asarray(x0)[a:b],
+                               None if v0 is None else numpy.asarray(v0)[a:b]))
+        return out
+
+    def __call__(self, theta, x0=None, v0=None):
+        if x0 is None:
+            x0 = torch.as_tensor(numpy.concatenate([t.template.x[:, :3] for t in self.parts]),
+                                 dtype=torch.float64)
+        if v0 is None:
+            v0 = torch.as_tensor(numpy.concatenate([t.template.vel[:, :3] for t in self.parts]),
+                                 dtype=torch.float64)
+        return _TrajectoryFn.apply(theta, x0, v0, self)
+
+    def run_forward(self, theta, x0, v0):
+        samples = numpy.zeros((self.nsamples, self.rows, 3))
+        ctx = engine.Context(images=self.images(theta, x0, v0), device=self.device)
+        try:
+            start = _BatchStart(ctx)
+            for k in range(self.nsamples):
+                ctx.step(self.every)
+                for j in range(ctx.nsims):
+                    a, b = self.offsets[j], self.offsets[j + 1]
+                    samples[k, a:b] = ctx.get("x", sim=j)[:, :3]
+        finally:
+            ctx.close()
+        return samples, start
+
+    def run_backward(self, start, grad_samples):
+        ctx = engine.Context(images=start.images, device=self.device)
+        try:
+            for j, path in enumerate(start.states):
+                ctx.read_state(path, sim=j)
+            g = ctx.adjoint_samples(self.every, self.nsamples, grad_samples, None)
+        finally:
+            ctx.close()
+        gtheta = numpy.stack([sum(numpy.asarray(g[f]) for f in self.ties[name])
+                              for name in self.params], axis=1)
+        return gtheta, g["x"], g["v"]
+
+
 class _TrajectoryFn(torch.autograd.Function):
     @staticmethod
     def forward(fctx, theta, x0, v0, traj):
@@ -139,5 +229,5 @@ class _TrajectoryFn(torch.autograd.Function):
     def backward(fctx, grad):
         gtheta, gx, gv = fctx.traj.run_backward(fctx.saved, grad.detach().cpu().numpy())
         dev = fctx.dev
-        return (torch.from_numpy(gtheta).to(dev), torch.from_numpy(gx).to(dev),
-                torch.from_numpy(gv).to(dev), None)
+        return (torch.from_numpy(numpy.ascontiguousarray(gtheta)).to(dev),
+                torch.from_numpy(gx).to(dev), torch.from_numpy(gv).to(dev), None)

@@ -1,13 +1,14 @@
-// Reverse mode (adjoint) of the DEM step for contact model 2, on a recorded
-// forward pass.  The reference has no counterpart.
+// Reverse mode (adjoint) of the DEM step for contact model 2, with bonds and
+// walls, on a recorded forward pass, for every simulation of a context.  The
+// reference has no counterpart.
 //
 // The forward pass records, per step, the state the step reads (positions,
-// velocities, angular velocities, the previous accelerations, the
-// tangential displacements and the walls) and the contact rows after the
-// interaction pass, which hold exactly the contacts that produced a force.
-// The reverse pass then runs the steps backwards with that contact list
-// frozen and every branch (normal-force clamp, stick or slip, wall
-// friction) taken as the forward took it, and propagates cotangents:
+// velocities, angular velocities, the tangential displacements, the walls,
+// and each bond's inputs) and the contact rows after the interaction pass,
+// which hold exactly the contacts that produced a force.  The reverse pass
+// then runs the steps backwards with that contact list frozen and every
+// branch (normal-force clamp, stick or slip, wall friction, bond breakage)
+// taken as the forward took it, and propagates cotangents:
 //
 //   integrate (TY3)     x' = x + v dt + (2/3) a dt^2 - (1/6) a0 dt^2
 //                       v' = v + (3/2) a dt - (1/2) a0 dt,  a = F/m + g
@@ -15,13 +16,23 @@
 //                       contact, as functions of both particles' state, the
 //                       stored displacement and (k_n, k_t, gamma_n, gamma_t,
 //                       mu_s, mu_d)
-//   contactLinear_wall  F, T of each wall contact (k_n, gamma_wn, gamma_wt,
-//                       mu_wd)
+//   contactLinear_wall  F, T of each wall contact and the force it puts on
+//                       the wall (k_n, gamma_wn, gamma_wt, mu_wd), as
+//                       functions of the particle's and the wall's state
+//   bondsLinear         the bond's force and torque and its new
+//                       displacements and rotations (k_n, k_t, gamma_n,
+//                       gamma_t)
+//   integrateWalls      p' = p + u dt + (2/3) a dt^2 - (1/6) a0 dt^2,
+//                       u' = u + (3/2) a dt - (1/2) a0 dt, with
+//                       a = (sum of wall forces - sigma0 Lx Ly) / m for a
+//                       stress-controlled wall (wmode 1) and a = 0 for a
+//                       velocity-controlled one (wmode 2)
 //
 // Lengths are differentiated as exact roots (the forward rounds them to
-// FP32, which is locally constant and has no useful derivative).  Walls move
-// along their recorded trajectories: the wall equation of motion is not
-// differentiated, which is exact for walls at fixed position or velocity.
+// FP32, which is locally constant and has no useful derivative).  Sums
+// that span particles (parameter cotangents, the walls' position and
+// velocity cotangents) are formed per workgroup and then per simulation in
+// a fixed order, so gradients are reproducible bit for bit.
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -165,10 +176,13 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
 }
 
 // Reverse of contactLinear_wall (src/contactmodels.cuh:10-117) for one wall
-// with normal n, distance delta = e . x + const, wall speed wv.
-__device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3 n, d3 e,
-                                 double delta, double wv, d3 gF, d3 gT, d3& gx, d3& gva,
-                                 d3& gwa, PGrad& gp) {
+// with normal n at distance delta from the particle (delta = e . x + const,
+// with the wall position entering const), moving at speed wv, whose return
+// value dot(f_n, n) (the force on the wall) has cotangent gret.  Adds to the
+// particle's cotangents; returns the cotangents of delta and of wv.
+__device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3 n, double delta,
+                                 double wv, d3 gF, d3 gT, double gret, d3& gva, d3& gwa,
+                                 double& gdelta, double& gwv, PGrad& gp) {
     const d3 vl = va;
     const double c = ra + delta / 2.0;
     const d3 nw = cross(n, wa);
@@ -190,8 +204,8 @@ __device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3
         stick = length(f_t_visc) < limit;
         f_t = stick ? -1.0 * f_t_visc : -limit * vel_t / vtl;
     }
-    d3 gft = gF, gfn = gF;
-    double gdelta = 0.0;
+    d3 gft = gF, gfn = gF + gret * n;
+    gdelta = 0.0;
     const double cu = -(ra + delta * 0.5);
     gft += cross(gT, cu * n);
     gdelta += -0.5 * dot(n, cross(f_t, gT));
@@ -222,12 +236,13 @@ __device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3
     gdelta += 0.5 * dot(gvel, nw);
     gwa += cross(c * gvel, n);
     gva += gvl;
-    gx += gdelta * e;
+    // vel = vl + c (n x w) + wv: the wall speed is added to every component.
+    gwv = gvel.x + gvel.y + gvel.z;
 }
 
 constexpr int ALPP = 4;               // work-items per particle
 constexpr int ASPL = NC / ALPP;       // contact slots per work-item
-constexpr int APPB = 256 / ALPP;      // particles per workgroup
+static_assert(ADJ_PPB * ALPP == 256, "four work-items per particle");
 
 __device__ __forceinline__ double group_sum(double v) {
 #pragma unroll
@@ -279,6 +294,36 @@ __device__ __forceinline__ d3 image_shift(const DevParams& p, uint32_t wrd) {
                cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
 }
 
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// One list wall's contact (walls 0..nw-1 of the simulation): the particle's
+// part, the wall's position and velocity cotangents (sign: how the contact
+// distance changes with the wall position), the force sum's cotangent gS.
+__device__ __forceinline__ void wall_contact(const DevParams& p, const AdjStep& st,
+                                             const AdjState& g, int w, d3 va, d3 wa, double ra,
+                                             d3 e, double sign, double dw, d3 wn, double wvel,
+                                             d3 gF, d3 gT, d3& gxa, d3& gva, d3& gwa,
+                                             double (&cwp)[MAXWALLS], double (&cwu)[MAXWALLS],
+                                             PGrad& gp) {
+    const bool listed = (uint32_t)w < p.nw;
+    const double gS = listed ? g.wall[(p.wall_base + w) * ADJ_WALL + 3] : 0.0;
+    double gdel = 0.0, gwv = 0.0;
+    adj_contact_wall(p, va, wa, ra, wn, dw, wvel, gF, gT, gS, gva, gwa, gdel, gwv, gp);
+    gxa += gdel * e;
+    if (listed) {
+#pragma unroll
+        for (int k = 0; k < MAXWALLS; ++k)
+            if (k == w) {
+                cwp[k] += sign * gdel;
+                cwu[k] += gwv;
+            }
+    }
+}
+
 // Reverse of one whole step, ALPP work-items per particle, in the step's
 // sorted order (partner gathers then stay near each other in the recorded
 // state).  Every particle's input cotangent is gathered by its own group,
@@ -296,19 +341,24 @@ __device__ __forceinline__ d3 image_shift(const DevParams& p, uint32_t wrd) {
 //     Contact rows are symmetric unless a row overflowed; a contact whose
 //     partner does not list it is counted in `asym` (the host then refuses
 //     the result);
-//  3. walls, on work-item 0;
-//  4. the group's sums are written to Lin.
-__global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st, AdjState g,
-                                                  double Lz) {
-    __shared__ double red[ADJ_NPARAM][4];
+//  3. walls, on work-item 0, including the cotangent of each wall force
+//     the particle puts on a list wall (from the wall's equation of
+//     motion) and the cotangents of the walls' position and velocity;
+//  4. the group's sums are written to Lin; the workgroup's partial sums of
+//     the parameter and wall cotangents to part.
+__global__ __launch_bounds__(256) void k_adj_step(AdjStep st, AdjState g) {
+    __shared__ double red[ADJ_NPART][4];
     __shared__ double psum[ADJ_NPARAM * 256];
+    const DevParams& p = st.ptab[st.chunk_sim[blockIdx.x / (256 / ADJ_PPB)]];
     const uint32_t q = threadIdx.x % ALPP;
-    const uint32_t ia = blockIdx.x * APPB + threadIdx.x / ALPP;
+    const uint32_t ia = blockIdx.x * ADJ_PPB + threadIdx.x / ALPP;
     PGrad gp;
     gp.s = psum + threadIdx.x;
     for (int k = 0; k < ADJ_NPARAM; ++k) gp.s[k * 256] = 0.0;
-    if (ia < p.np) {
-        const uint32_t np = p.np;
+    double cwp[MAXWALLS], cwu[MAXWALLS];
+#pragma unroll
+    for (int k = 0; k < MAXWALLS; ++k) cwp[k] = cwu[k] = 0.0;
+    if (ia - p.base < p.np) {
         const uint32_t a = st.perm[ia];
         const double4 xa4 = st.x[ia], va4 = st.v[ia];
         const d3 xa = xyz(xa4), va = xyz(va4), wa = xyz(st.w[ia]);
@@ -355,8 +405,9 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
         }
 #pragma unroll
         for (int j = 0; j < ASPL; ++j) {
-            const bool occ = (wv[j] & CW_IDX_MASK) < np;
-            if (!occ && (pv[j] & CW_IDX_MASK) < np)
+            const uint32_t wb = wv[j] & CW_IDX_MASK, pb = pv[j] & CW_IDX_MASK;
+            const bool occ = wb != EMPTY;
+            if (!occ && pb != EMPTY)
                 g.Ldt_out[(size_t)a * NC + q + ALPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
             if (occ) {
 #pragma unroll
@@ -364,10 +415,8 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
                     qw[k] = qw[k - 1];
                     qj[k] = qj[k - 1];
                 }
-                const uint32_t pb = pv[j] & CW_IDX_MASK;
                 qw[0] = wv[j];
-                qj[0] = (uint32_t)j | (pb < np ? 8u : 0u) |
-                        (pb == (wv[j] & CW_IDX_MASK) ? 16u : 0u);
+                qj[0] = (uint32_t)j | (pb != EMPTY ? 8u : 0u) | (pb == wb ? 16u : 0u);
                 ++nq;
             }
         }
@@ -430,57 +479,56 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
             gva += gva_part;
             gwa += gwa_part;
         }
-        // 3. walls, as in the forward pass: top (wall 0 or L_z), bottom,
-        // then the side walls by boundary mode.
+        // 3. walls, as in the forward pass (src/contactsearch.cuh:602-701):
+        // top (wall 0, or the implicit wall at L_z), bottom, then the side
+        // walls by boundary mode; walls beyond nw are the implicit ones.
         if (q == 0) {
+            const double4* wnx = st.wall_nx + p.wall_base;
+            const double4* wmv = st.wall_mvfd + p.wall_base;
             {
-                double wpos = Lz, wv = 0.0;
+                double wpos = p.L[2], wvel = 0.0;
                 d3 wn = mk3(0.0, 0.0, -1.0);
-                if (p.nw >= 1) {
-                    wn = xyz(st.wall_nx[0]);
-                    wpos = st.wall_nx[0].w;
-                    wv = st.wall_mvfd[0].y;
-                }
+                if (p.nw >= 1) { wn = xyz(wnx[0]); wpos = wnx[0].w; wvel = wmv[0].y; }
                 const double dw = wpos - (xa.z + ra);
                 if (dw < 0.0)
-                    adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 0.0, -1.0), dw, wv, gF, gT, gxa,
-                                     gva, gwa, gp);
+                    wall_contact(p, st, g, 0, va, wa, ra, mk3(0.0, 0.0, -1.0), 1.0, dw, wn, wvel,
+                                 gF, gT, gxa, gva, gwa, cwp, cwu, gp);
             }
             {
                 const double dw = xa.z - ra - p.origo[2];
                 if (dw < 0.0)
-                    adj_contact_wall(p, va, wa, ra, mk3(0.0, 0.0, 1.0), mk3(0.0, 0.0, 1.0), dw,
-                                     0.0, gF, gT, gxa, gva, gwa, gp);
+                    wall_contact(p, st, g, -1, va, wa, ra, mk3(0.0, 0.0, 1.0), 0.0, dw,
+                                 mk3(0.0, 0.0, 1.0), 0.0, gF, gT, gxa, gva, gwa, cwp, cwu, gp);
             }
             if (p.periodic == 0) {
-                double wpos = p.L[0], wv = 0.0;
+                double wpos = p.L[0], wvel = 0.0;
                 d3 wn = mk3(-1.0, 0.0, 0.0);
-                if (p.nw >= 2) { wn = xyz(st.wall_nx[1]); wpos = st.wall_nx[1].w; wv = st.wall_mvfd[1].y; }
+                if (p.nw >= 2) { wn = xyz(wnx[1]); wpos = wnx[1].w; wvel = wmv[1].y; }
                 double dw = wpos - (xa.x + ra);
                 if (dw < 0.0)
-                    adj_contact_wall(p, va, wa, ra, wn, mk3(-1.0, 0.0, 0.0), dw, wv, gF, gT, gxa,
-                                     gva, gwa, gp);
-                wpos = 0.0; wv = 0.0; wn = mk3(1.0, 0.0, 0.0);
-                if (p.nw >= 3) { wn = xyz(st.wall_nx[2]); wpos = st.wall_nx[2].w; wv = st.wall_mvfd[2].y; }
+                    wall_contact(p, st, g, 1, va, wa, ra, mk3(-1.0, 0.0, 0.0), 1.0, dw, wn, wvel,
+                                 gF, gT, gxa, gva, gwa, cwp, cwu, gp);
+                wpos = 0.0; wvel = 0.0; wn = mk3(1.0, 0.0, 0.0);
+                if (p.nw >= 3) { wn = xyz(wnx[2]); wpos = wnx[2].w; wvel = wmv[2].y; }
                 dw = xa.x - ra - wpos;
                 if (dw < 0.0)
-                    adj_contact_wall(p, va, wa, ra, wn, mk3(1.0, 0.0, 0.0), dw, wv, gF, gT, gxa,
-                                     gva, gwa, gp);
+                    wall_contact(p, st, g, 2, va, wa, ra, mk3(1.0, 0.0, 0.0), -1.0, dw, wn, wvel,
+                                 gF, gT, gxa, gva, gwa, cwp, cwu, gp);
             }
             if (p.periodic == 0 || p.periodic == 2) {
-                double wpos = p.L[1], wv = 0.0;
+                double wpos = p.L[1], wvel = 0.0;
                 d3 wn = mk3(0.0, -1.0, 0.0);
-                if (p.nw >= 4) { wn = xyz(st.wall_nx[3]); wpos = st.wall_nx[3].w; wv = st.wall_mvfd[3].y; }
+                if (p.nw >= 4) { wn = xyz(wnx[3]); wpos = wnx[3].w; wvel = wmv[3].y; }
                 double dw = wpos - (xa.y + ra);
                 if (dw < 0.0)
-                    adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, -1.0, 0.0), dw, wv, gF, gT, gxa,
-                                     gva, gwa, gp);
-                wpos = 0.0; wv = 0.0; wn = mk3(0.0, 1.0, 0.0);
-                if (p.nw >= 5) { wn = xyz(st.wall_nx[4]); wpos = st.wall_nx[4].w; wv = st.wall_mvfd[4].y; }
+                    wall_contact(p, st, g, 3, va, wa, ra, mk3(0.0, -1.0, 0.0), 1.0, dw, wn, wvel,
+                                 gF, gT, gxa, gva, gwa, cwp, cwu, gp);
+                wpos = 0.0; wvel = 0.0; wn = mk3(0.0, 1.0, 0.0);
+                if (p.nw >= 5) { wn = xyz(wnx[4]); wpos = wnx[4].w; wvel = wmv[4].y; }
                 dw = xa.y - ra - wpos;
                 if (dw < 0.0)
-                    adj_contact_wall(p, va, wa, ra, wn, mk3(0.0, 1.0, 0.0), dw, wv, gF, gT, gxa,
-                                     gva, gwa, gp);
+                    wall_contact(p, st, g, 4, va, wa, ra, mk3(0.0, 1.0, 0.0), -1.0, dw, wn, wvel,
+                                 gF, gT, gxa, gva, gwa, cwp, cwu, gp);
             }
         }
         // 4. the particle's input cotangents
@@ -493,72 +541,330 @@ __global__ __launch_bounds__(256) void k_adj_step(const DevParams p, AdjStep st,
             g.Lin[2][a] = make_double4(gwa.x, gwa.y, gwa.z, 0.0);
         }
     }
-    // Parameter cotangents: wavefront sums, then the four wavefronts' sums,
-    // then one atomic per parameter and workgroup.
+    // Partial sums of the workgroup: wavefront butterflies, then the four
+    // wavefronts in order (a fixed order: reproducible bit for bit).
     const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
     for (int k = 0; k < ADJ_NPARAM; ++k) {
-        double v = gp.s[k * 256];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        const double v = wave_sum(gp.s[k * 256]);
         if (lane == 0) red[k][wave] = v;
     }
+#pragma unroll
+    for (int k = 0; k < MAXWALLS; ++k) {
+        const double vp = wave_sum(cwp[k]), vu = wave_sum(cwu[k]);
+        if (lane == 0) {
+            red[ADJ_NPARAM + 2 * k][wave] = vp;
+            red[ADJ_NPARAM + 2 * k + 1][wave] = vu;
+        }
+    }
     __syncthreads();
-    if (threadIdx.x < ADJ_NPARAM) {
+    if (threadIdx.x < ADJ_NPART) {
         const int k = threadIdx.x;
-        atomicAdd(&g.gparam[k], red[k][0] + red[k][1] + red[k][2] + red[k][3]);
+        g.part[(size_t)blockIdx.x * ADJ_NPART + k] = red[k][0] + red[k][1] + red[k][2] + red[k][3];
     }
 }
 
-// Three-component host layout <-> the engine's four-component one.
-__global__ __launch_bounds__(256) void k_adj_expand(double4* dst, const double* src, uint32_t n) {
-    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) dst[i] = make_double4(src[3 * i], src[3 * i + 1], src[3 * i + 2], 0.0);
+// Reverse of bondsLinear (src/cohesion.cuh:8-233) for the bonds that applied
+// forces in the step, from the recorded inputs: the cotangents of the
+// particles' forces and torques (recomputed from their output cotangents)
+// and of the bond's new displacements and rotations.  Writes the
+// cotangents of both particles' x, v, w to bcontrib, of the displacements
+// and rotations it read to Lb_in, and its parameter cotangents to bpart.
+// A bond that applied no force (broken, or broken in this step) leaves its
+// displacements and rotations as they were: their cotangents pass through.
+__global__ __launch_bounds__(256) void k_adj_bonds(AdjStep st, AdjState g) {
+    const uint32_t b = blockIdx.x * 256 + threadIdx.x;
+    if (b >= st.nbond) return;
+    const DevParams& p = st.ptab[st.bond_sim[b]];
+    const uint4 idx = st.bond_idx[b];
+    double4* co = g.bcontrib + (size_t)6 * b;
+    double* bp = g.bpart + (size_t)ADJ_NBPARAM * b;
+    const double4 z4 = make_double4(0.0, 0.0, 0.0, 0.0);
+    const double4 Ld = st.last ? z4 : g.Lb_out[2 * b], Lo = st.last ? z4 : g.Lb_out[2 * b + 1];
+    if (!idx.z) {
+        g.Lb_in[2 * b] = Ld;
+        g.Lb_in[2 * b + 1] = Lo;
+        for (int k = 0; k < 6; ++k) co[k] = z4;
+        for (int k = 0; k < ADJ_NBPARAM; ++k) bp[k] = 0.0;
+        return;
+    }
+    const uint32_t bi = idx.x, bj = idx.y;
+    const double4* in = st.bond_in + (size_t)8 * b;
+    const double4 x_i = in[0], x_j = in[1], v_i4 = in[2], v_j4 = in[3];
+    const d3 w_i = xyz(in[4]), w_j = xyz(in[5]);
+    const double4 d0 = in[6], o0 = in[7];
+    const d3 v_i = xyz(v_i4), v_j = xyz(v_j4);
+    const double dt = p.dt;
+    // cotangents of the bond's force on i and torque t_j (force_i += f,
+    // force_j -= f, torque_i -= t_j, torque_j += t_j)
+    d3 gFi, gTi, gFj, gTj;
+    force_cotangents(p, x_i.w, v_i4.w, g.Lout[0][bi], g.Lout[1][bi], g.Lout[2][bi],
+                     g.La_out[bi], g.Laa_out[bi], gFi, gTi);
+    force_cotangents(p, x_j.w, v_j4.w, g.Lout[0][bj], g.Lout[1][bj], g.Lout[2][bj],
+                     g.La_out[bj], g.Laa_out[bj], gFj, gTj);
+    const d3 gf = gFi - gFj, gtj = gTj - gTi;
+    // forward
+    const double R_bar = p.lambda_bar * fmin(x_i.w, x_j.w);
+    const double A = PI * R_bar * R_bar;
+    const double I = 0.25 * PI * R_bar * R_bar * R_bar * R_bar;
+    const double J = I * 2.0;
+    const d3 x = mk3(x_i.x - x_j.x, x_i.y - x_j.y, x_i.z - x_j.z);
+    const double len = length(x);
+    const double sep = len - (x_i.w + x_j.w);
+    const double overlap = fmin(0.0, sep);
+    const d3 n = x / len;
+    const d3 dt0 = xyz(d0), om0 = xyz(o0);
+    const double nd0 = dot(n, dt0);
+    const d3 dt0p = dt0 - n * nd0;
+    const double ci = x_i.w + overlap / 2.0, cj = x_j.w + overlap / 2.0;
+    const d3 ni = cross(n, w_i), nj = cross(n, w_j);
+    const d3 dd = ((v_i - v_j) + ci * ni + cj * nj) * dt;
+    const double ddn = -dot(dd, n);
+    const double dn = d0.w + ddn;
+    const double ndd = dot(n, dd);
+    const d3 ddt = dd - n * ndd;
+    const d3 dtv = dt0p + ddt;
+    // Stiffness per unit area (the bond option replaces k_n, k_t here; its
+    // own cotangent is not formed, and then k_n, k_t get none from bonds).
+    const bool ref_k = !(p.bond_k > 0.0);
+    const double kn = ref_k ? p.k_n : p.bond_k / A;
+    const double kt = ref_k ? p.k_t : p.bond_k / A;
+    const double cfn = kn * A * dn + p.gamma_n * ddn / dt;
+    const double no0 = dot(n, om0);
+    const d3 dom = (w_j - w_i) * dt;
+    const double ndo = dot(n, dom);
+    const double don = -ndo;
+    const double omn = o0.w + don;
+    const d3 omt = (om0 - n * no0) + (dom - n * ndo);
+    // reverse
+    const d3 gdtv = xyz(Ld), gomt_o = xyz(Lo);
+    double gdn = Ld.w, gomn = Lo.w;
+    d3 gn = zero3();
+    // t_j = t_n + t_t; t_n = k_t J omn n; t_t = -k_n I omt
+    const d3 gomt = gomt_o + (-kn * I) * gtj;
+    bp[0] = -I * dot(omt, gtj);                       // k_n
+    const double ngt = dot(n, gtj);
+    gomn += kt * J * ngt;
+    bp[1] = J * omn * ngt;                            // k_t
+    gn += (kt * J * omn) * gtj;
+    // omt = om0 - n (n . om0) + dom - n (n . dom); omn = o0.w - n . dom
+    d3 gom0 = gomt - n * dot(n, gomt);
+    gn += -(no0 * gomt + dot(n, gomt) * om0);
+    d3 gdom = gomt - n * dot(n, gomt);
+    gn += -(ndo * gomt + dot(n, gomt) * dom);
+    const double gom0n = gomn;
+    gdom += -gomn * n;
+    gn += -gomn * dom;
+    d3 gwi = -dt * gdom, gwj = dt * gdom;
+    // f = f_n + f_t; f_n = cfn n; f_t = -k_t A dtv - gamma_t ddt / dt
+    const double gcfn = dot(n, gf);
+    gn += cfn * gf;
+    gdn += kn * A * gcfn;
+    bp[0] += A * dn * gcfn;
+    double gddn = p.gamma_n / dt * gcfn;
+    bp[2] = ddn / dt * gcfn;                          // gamma_n
+    const d3 gdtv_tot = gdtv + (-kt * A) * gf;
+    bp[1] += -A * dot(dtv, gf);
+    if (!ref_k) bp[0] = bp[1] = 0.0;
+    d3 gddt = (-p.gamma_t / dt) * gf;
+    bp[3] = -dot(ddt, gf) / dt;                       // gamma_t
+    // dtv = dt0p + ddt; dn = d0.w + ddn
+    const d3 gdt0p = gdtv_tot;
+    gddt += gdtv_tot;
+    const double gd0n = gdn;
+    gddn += gdn;
+    // ddt = dd - n (n . dd); ddn = -(dd . n)
+    d3 gdd = gddt - n * dot(n, gddt);
+    gn += -(ndd * gddt + dot(n, gddt) * dd);
+    gdd += -gddn * n;
+    gn += -gddn * dd;
+    // dd = ((v_i - v_j) + ci (n x w_i) + cj (n x w_j)) dt
+    const d3 gw = dt * gdd;
+    const d3 gvi = gw, gvj = -gw;
+    gn += cross(w_i, ci * gw) + cross(w_j, cj * gw);
+    gwi += cross(ci * gw, n);
+    gwj += cross(cj * gw, n);
+    const double gov = 0.5 * (dot(gw, ni) + dot(gw, nj));
+    // dt0p = dt0 - n (n . dt0)
+    const d3 gdt0 = gdt0p - n * dot(n, gdt0p);
+    gn += -(nd0 * gdt0p + dot(n, gdt0p) * dt0);
+    // overlap = fmin(0, len - R); n = x / len
+    const double glen = sep < 0.0 ? gov : 0.0;
+    const d3 gx = (gn - n * dot(n, gn)) / len + n * glen;
+    g.Lb_in[2 * b] = make_double4(gdt0.x, gdt0.y, gdt0.z, gd0n);
+    g.Lb_in[2 * b + 1] = make_double4(gom0.x, gom0.y, gom0.z, gom0n);
+    co[0] = make_double4(gx.x, gx.y, gx.z, 0.0);
+    co[1] = make_double4(gvi.x, gvi.y, gvi.z, 0.0);
+    co[2] = make_double4(gwi.x, gwi.y, gwi.z, 0.0);
+    co[3] = make_double4(-gx.x, -gx.y, -gx.z, 0.0);
+    co[4] = make_double4(gvj.x, gvj.y, gvj.z, 0.0);
+    co[5] = make_double4(gwj.x, gwj.y, gwj.z, 0.0);
 }
 
-__global__ __launch_bounds__(256) void k_adj_pack(double* dst, const double4* src, uint32_t n) {
+// Adds each particle's share of its bonds' cotangents to its input
+// cotangents, in ascending bond id over the bonds the forward gather
+// visited (the particle's own CSR list, or the whole list for particle nb0
+// after a broken bond was retargeted onto it).
+__global__ __launch_bounds__(256) void k_adj_bond_gather(AdjStep st, AdjState g) {
+    const DevParams& p = st.ptab[st.chunk_sim[blockIdx.x]];
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) {
-        const double4 v = src[i];
-        dst[3 * i] = v.x;
-        dst[3 * i + 1] = v.y;
-        dst[3 * i + 2] = v.z;
+    if (i - p.base >= p.np) return;
+    const bool full = (i - p.base == p.nb0) && st.bond_full[p.sim] != 0u;
+    const uint32_t b0 = full ? p.bond_base : st.badj_off[i];
+    const uint32_t b1 = full ? p.bond_base + p.nb0 : st.badj_off[i + 1];
+    if (b0 == b1) return;
+    double4 gx = g.Lin[0][i], gv = g.Lin[1][i], gw = g.Lin[2][i];
+    for (uint32_t k = b0; k < b1; ++k) {
+        const uint32_t b = full ? k : st.badj[k];
+        const uint4 idx = st.bond_idx[b];
+        if (!idx.z) continue;
+        const double4* co = g.bcontrib + (size_t)6 * b;
+        for (int side = 0; side < 2; ++side) {
+            if ((side == 0 ? idx.x : idx.y) != i) continue;
+            const double4 cx = co[3 * side], cv = co[3 * side + 1], cw = co[3 * side + 2];
+            gx.x += cx.x; gx.y += cx.y; gx.z += cx.z;
+            gv.x += cv.x; gv.y += cv.y; gv.z += cv.z;
+            gw.x += cw.x; gw.y += cw.y; gw.z += cw.z;
+        }
+    }
+    g.Lin[0][i] = gx;
+    g.Lin[1][i] = gv;
+    g.Lin[2][i] = gw;
+}
+
+// Fixed-order sum of n values v[k * stride] over one workgroup of 256: each
+// work-item sums its strided share, then a tree.  Result on every work-item.
+__device__ double block_sum(const double* v, uint32_t n, uint32_t stride, double* cache) {
+    double t = 0.0;
+    for (uint32_t k = threadIdx.x; k < n; k += 256) t += v[(size_t)k * stride];
+    cache[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) cache[threadIdx.x] += cache[threadIdx.x + h];
+        __syncthreads();
+    }
+    const double r = cache[0];
+    __syncthreads();
+    return r;
+}
+
+// Per simulation (one workgroup each): the step's parameter cotangents
+// (particle workgroups, then bonds) added to the running sums, and the
+// reverse of integrateWalls (src/integration.cuh:344-450) for the walls:
+// from the cotangents of the position, velocity and acceleration the step
+// left (P, U, A) and of the wall contacts' position and velocity (cp, cu),
+// those of the position, velocity and acceleration the step started from;
+// then the force sum's cotangent for the step before (the wall's
+// acceleration there is (sum + N) / m for wmode 1, zero for wmode 2, and
+// wall w integrates wall 0's sum unless each wall has its own).
+__global__ __launch_bounds__(256) void k_adj_reduce(AdjStep st, AdjState g) {
+    __shared__ double cache[256];
+    __shared__ double tot[ADJ_NPART];
+    const DevParams& p = st.ptab[blockIdx.x];
+    const uint32_t wb0 = p.base / ADJ_PPB, nwb = p.npad / ADJ_PPB;
+    for (int k = 0; k < ADJ_NPART; ++k) {
+        const double v = block_sum(g.part + (size_t)wb0 * ADJ_NPART + k, nwb, ADJ_NPART, cache);
+        if (threadIdx.x == 0) tot[k] = v;
+    }
+    double bsum[ADJ_NBPARAM] = {0.0, 0.0, 0.0, 0.0};
+    if (st.nbond && p.nb0)
+        for (int k = 0; k < ADJ_NBPARAM; ++k)
+            bsum[k] = block_sum(g.bpart + (size_t)p.bond_base * ADJ_NBPARAM + k, p.nb0,
+                                ADJ_NBPARAM, cache);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    double* gp = g.gparam + (size_t)p.sim * ADJ_NPARAM;
+    for (int k = 0; k < ADJ_NPARAM; ++k) gp[k] += tot[k] + (k < ADJ_NBPARAM ? bsum[k] : 0.0);
+    const double dt = p.dt;
+    double* W = g.wall + (size_t)p.wall_base * ADJ_WALL;
+    for (uint32_t w = 0; w < p.nw; ++w) {
+        double* s = W + w * ADJ_WALL;
+        const double P = s[0], U = s[1], A = s[2];
+        const double cp = tot[ADJ_NPARAM + 2 * w], cu = tot[ADJ_NPARAM + 2 * w + 1];
+        if (p.wmode[w] == 0) {
+            s[0] = P + cp;
+            s[1] = U + cu;
+        } else {
+            s[0] = P + cp;
+            s[1] = dt * P + U + cu;
+            s[2] = st.first ? 0.0 : -(dt * dt / 6.0) * P - 0.5 * dt * U;
+        }
+        s[3] = 0.0;
+    }
+    for (uint32_t w = 0; w < p.nw; ++w) {
+        if (p.wmode[w] != 1) continue;
+        const double* s = W + w * ADJ_WALL;
+        const double ga = s[2] + (2.0 / 3.0) * dt * dt * s[0] + 1.5 * dt * s[1];
+        const uint32_t src = p.wall_sums ? w : 0u;
+        W[src * ADJ_WALL + 3] += ga / st.wall_mvfd[p.wall_base + w].x;
     }
 }
 
-// dst[i].xyz += src[3 i .. 3 i + 2] (a loss term on an intermediate state).
+// Host rows <-> batch arrays.
+__device__ __forceinline__ bool host_row(const AdjStep& st, uint32_t i, uint32_t& h) {
+    const DevParams& p = st.ptab[st.chunk_sim[i / 256]];
+    const uint32_t local = i - p.base;
+    h = p.hbase + local;
+    return local < p.np;
+}
+
+__global__ __launch_bounds__(256) void k_adj_expand(double4* dst, const double* src, AdjStep st) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= st.ntot) return;
+    uint32_t h;
+    dst[i] = host_row(st, i, h) ? make_double4(src[3 * h], src[3 * h + 1], src[3 * h + 2], 0.0)
+                                : make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void k_adj_pack(double* dst, const double4* src, AdjStep st) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    uint32_t h;
+    if (i >= st.ntot || !host_row(st, i, h)) return;
+    const double4 v = src[i];
+    dst[3 * h] = v.x;
+    dst[3 * h + 1] = v.y;
+    dst[3 * h + 2] = v.z;
+}
+
+// dst[i].xyz += the host row of particle i (a loss term on an intermediate
+// state).
 __global__ __launch_bounds__(256) void k_adj_accumulate(double4* dst, const double* src,
-                                                        uint32_t n) {
+                                                        AdjStep st) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) {
-        double4 v = dst[i];
-        v.x += src[3 * i];
-        v.y += src[3 * i + 1];
-        v.z += src[3 * i + 2];
-        dst[i] = v;
-    }
+    uint32_t h;
+    if (i >= st.ntot || !host_row(st, i, h)) return;
+    double4 v = dst[i];
+    v.x += src[3 * h];
+    v.y += src[3 * h + 1];
+    v.z += src[3 * h + 2];
+    dst[i] = v;
 }
 
 }  // namespace
 
-hipError_t launch_adj_accumulate(double4* dst, const double* src, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+hipError_t launch_adj_accumulate(double4* dst, const double* src, const AdjStep& st,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_accumulate, dim3((st.ntot + 255) / 256), dim3(256), 0, s, dst, src,
+                       st);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_expand(double4* dst, const double* src, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_expand, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+hipError_t launch_adj_expand(double4* dst, const double* src, const AdjStep& st, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_expand, dim3((st.ntot + 255) / 256), dim3(256), 0, s, dst, src, st);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_pack(double* dst, const double4* src, uint32_t n, hipStream_t s) {
-    hipLaunchKernelGGL(k_adj_pack, dim3((n + 255) / 256), dim3(256), 0, s, dst, src, n);
+hipError_t launch_adj_pack(double* dst, const double4* src, const AdjStep& st, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_pack, dim3((st.ntot + 255) / 256), dim3(256), 0, s, dst, src, st);
     return hipGetLastError();
 }
 
-hipError_t launch_adj_step(const DevParams& p, const AdjStep& st, const AdjState& g, double Lz,
-                           hipStream_t s) {
-    const dim3 grid((p.np + APPB - 1) / APPB), block(256);
-    hipLaunchKernelGGL(k_adj_step, grid, block, 0, s, p, st, g, Lz);
+hipError_t launch_adj_step(const AdjStep& st, const AdjState& g, hipStream_t s) {
+    hipLaunchKernelGGL(k_adj_step, dim3(st.ntot / ADJ_PPB), dim3(256), 0, s, st, g);
+    if (st.nbond) {
+        hipLaunchKernelGGL(k_adj_bonds, dim3((st.nbond + 255) / 256), dim3(256), 0, s, st, g);
+        hipLaunchKernelGGL(k_adj_bond_gather, dim3(st.ntot / 256), dim3(256), 0, s, st, g);
+    }
+    hipLaunchKernelGGL(k_adj_reduce, dim3(st.nsim), dim3(256), 0, s, st, g);
     return hipGetLastError();
 }
 

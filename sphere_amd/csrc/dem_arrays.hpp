@@ -77,11 +77,13 @@ struct DevArrays {
     // contact history (NC words per particle, original-index rows)
     uint32_t* crow;      // packed partner index + periodic image code
     double4* delta_t;    // tangential displacement per slot (w unused)
-    // Recording for the adjoint (null otherwise; one simulation): the
-    // contact pass writes its rows and, per contact, the displacement
-    // before the step and the partner's sorted index (original-index rows);
-    // the state the step starts from and the order, in sorted order; the
-    // walls.
+    // Recording for the adjoint (null otherwise): the contact pass writes
+    // its rows and, per contact, the displacement before the step and the
+    // partner's sorted index (original-index rows); the state the step
+    // starts from and the order, in sorted order; the walls.  The bond pass
+    // writes each bond's inputs (x, v, w of both particles, displacement
+    // and rotation; 8 per bond), its particles and whether it applied
+    // forces, and the bond gather whether particle nb0 scanned every bond.
     uint32_t* tape_rows;
     double4* tape_dt0;
     uint32_t* tape_srt;
@@ -91,6 +93,9 @@ struct DevArrays {
     double4* tape_w;
     double4* tape_wnx;
     double4* tape_wmv;
+    double4* tape_bond_in;
+    uint4* tape_bond_idx;
+    uint32_t* tape_bond_full;
     // walls
     double4* w_nx;       // normal (xyz) and position (w)
     double4* w_mvfd;     // mass, velocity, force, sigma0

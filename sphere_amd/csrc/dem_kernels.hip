@@ -752,9 +752,9 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
     if (p.nw > 4) a.force_pp[(size_t)np * 4 + o] = wf4;
 }
 
-// The adjoint's record of the step's starting state (one simulation):
-// each particle's and the order, at its sorted index, and the walls by the
-// work-item of original particle 0.
+// The adjoint's record of the step's starting state: each particle's and
+// the order, at its sorted index, and each simulation's walls by the
+// work-item of its first particle.
 __device__ __forceinline__ void tape_start(const DevArrays& a, const DevParams& p, uint32_t ia,
                                            uint32_t ia_orig, double4 x, double4 v, double4 w,
                                            bool writer) {
@@ -764,9 +764,9 @@ __device__ __forceinline__ void tape_start(const DevArrays& a, const DevParams& 
     a.tape_w[ia] = w;
     a.tape_perm[ia] = ia_orig;
     if (ia_orig == p.base)
-        for (uint32_t k = 0; k < p.nw; ++k) {
-            a.tape_wnx[k] = a.w_nx[p.wall_base + k];
-            a.tape_wmv[k] = a.w_mvfd[p.wall_base + k];
+        for (uint32_t k = p.wall_base; k < p.wall_base + p.nw; ++k) {
+            a.tape_wnx[k] = a.w_nx[k];
+            a.tape_wmv[k] = a.w_mvfd[k];
         }
 }
 
@@ -2131,9 +2131,22 @@ __global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevArrays a) {
     const DevParams& p = a.ptab[a.bond_sim[b]];
     const uint32_t bx = a.bonds[2 * b], by = a.bonds[2 * b + 1];
     a.bond_ok[b] = 0u;
+    if (a.tape_bond_idx) a.tape_bond_idx[b] = make_uint4(bx, by, 0u, 0u);
     if (!(bx - p.base < p.np)) return;
     const double dt = p.dt;
     const double4 d0 = a.bonds_delta[b], o0 = a.bonds_omega[b];
+    if (a.tape_bond_in) {
+        // The adjoint's record of the bond's inputs.
+        double4* t = a.tape_bond_in + (size_t)8 * b;
+        t[0] = a.x[bx];
+        t[1] = a.x[by];
+        t[2] = a.vel[bx];
+        t[3] = a.vel[by];
+        t[4] = a.angvel[bx];
+        t[5] = a.angvel[by];
+        t[6] = d0;
+        t[7] = o0;
+    }
     d3 delta0_t = mk3(d0.x, d0.y, d0.z);
     const double delta0_n = d0.w;
     d3 omega0_t = mk3(o0.x, o0.y, o0.z);
@@ -2157,8 +2170,11 @@ __global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevArrays a) {
     const double delta_n = delta0_n + ddelta_n;
     const d3 ddelta_t = ddelta - n * dot(n, ddelta);
     const d3 delta_t = delta0_t + ddelta_t;
-    const d3 f_n = (p.k_n * A * delta_n + p.gamma_n * ddelta_n / dt) * n;
-    const d3 f_t = -p.k_t * A * delta_t - p.gamma_t * ddelta_t / dt;
+    // Stiffness per unit area: the contact's k_n, k_t, or the bond option.
+    const double kn = p.bond_k > 0.0 ? p.bond_k / A : p.k_n;
+    const double kt = p.bond_k > 0.0 ? p.bond_k / A : p.k_t;
+    const d3 f_n = (kn * A * delta_n + p.gamma_n * ddelta_n / dt) * n;
+    const d3 f_t = -kt * A * delta_t - p.gamma_t * ddelta_t / dt;
     const d3 f = f_n + f_t;
     const d3 mn = -n;
     omega0_t = omega0_t - mn * dot(omega0_t, mn);
@@ -2168,12 +2184,20 @@ __global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevArrays a) {
     const double omega_n = omega0_n + domega_n;
     const d3 domega_t = domega - domega_n * mn;
     const d3 omega_t = omega0_t + domega_t;
-    const d3 t_n = -p.k_t * J * omega_n * mn;
-    const d3 t_t = -p.k_n * I * omega_t;
+    const d3 t_n = -kt * J * omega_n * mn;
+    const d3 t_t = -kn * I * omega_t;
     const d3 t_j = t_n + t_t;
-    const double sigma_max = length(f_n) / A + length(t_t) * R_bar / I;
-    const double tau_max = length(f_t) / A + length(t_n) * R_bar / J;
-    if (sigma_max >= p.sigma_b || tau_max >= p.tau_b) {
+    bool breaks;
+    if (p.bond_break_ext > 0.0) {
+        // The option: a normal extension (delta_n < 0 when the pair
+        // separates) of bond_break_ext times the smaller radius.
+        breaks = -delta_n >= p.bond_break_ext * fmin(x_i.w, x_j.w);
+    } else {
+        const double sigma_max = length(f_n) / A + length(t_t) * R_bar / I;
+        const double tau_max = length(f_t) / A + length(t_n) * R_bar / J;
+        breaks = sigma_max >= p.sigma_b || tau_max >= p.tau_b;
+    }
+    if (breaks) {
         a.bonds[2 * b] = p.base + p.nb0;  // src/cohesion.cuh:205
         if (p.nb0 < p.np) atomicOr(&a.loop[p.sim].bond_retargeted, 1u);
         return;
@@ -2183,6 +2207,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevArrays a) {
     a.bond_f[b] = make_double4(f.x, f.y, f.z, 0.0);
     a.bond_t[b] = make_double4(t_j.x, t_j.y, t_j.z, 0.0);
     a.bond_ok[b] = 1u;
+    if (a.tape_bond_idx) a.tape_bond_idx[b] = make_uint4(bx, by, 1u, 0u);
 }
 
 // Per-particle gather over the bonds that touch the particle, in ascending
@@ -2211,6 +2236,7 @@ __global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevArrays a) {
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i - p.base >= p.np) return;
     const bool full = (i - p.base == p.nb0) && a.loop[p.sim].bond_retargeted != 0u;
+    if (a.tape_bond_full && i - p.base == p.nb0) a.tape_bond_full[p.sim] = full ? 1u : 0u;
     const uint32_t b0 = full ? p.bond_base : a.badj_off[i];
     const uint32_t b1 = full ? p.bond_base + p.nb0 : a.badj_off[i + 1];
     if (b0 == b1) return;

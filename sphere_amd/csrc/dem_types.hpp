@@ -92,6 +92,15 @@ struct DevParams {
     uint32_t bond_base;   // first bond of the simulation
     uint32_t rhist_base;  // radix histogram space of the one-workgroup re-sort
     uint32_t wall_sums;   // 1: wall w integrates its own force sum; 0: wall 0's (P8)
+    uint32_t hbase;       // first row of the simulation in host arrays that
+                          // concatenate the simulations' particles
+    // Bond options (0 = the reference's bondsLinear): a bond stiffness in
+    // N/m in place of k_n A and k_t A (the bending and twisting stiffness
+    // per unit area becoming bond_k / A), and a break at a normal
+    // extension of bond_break_ext times the smaller radius in place of the
+    // stress criterion.
+    double bond_k;
+    double bond_break_ext;
 };
 
 // Loop state kept on the device so that captured step sequences can be

@@ -424,7 +424,7 @@ void fill_params(Sim& sim) {
 // own keys (up to ncells + num[0] + num[0]*num[1]: a particle at exactly
 // x == L or y == L survives the periodic wrap, P17) and its padding key.
 void layout(sph_ctx* c) {
-    uint32_t base = 0, walls = 0, bonds = 0, keys = 0, rh = 0;
+    uint32_t base = 0, walls = 0, bonds = 0, keys = 0, rh = 0, hrows = 0;
     bool shear = false, adaptive = false;
     for (size_t k = 0; k < c->sims.size(); ++k) {
         Sim& sim = c->sims[k];
@@ -440,6 +440,8 @@ void layout(sph_ctx* c) {
         P.wall_base = walls;
         P.bond_base = bonds;
         P.rhist_base = rh;
+        P.hbase = hrows;
+        hrows += h.np;
         base += P.npad;
         keys += kmax + 2;
         walls += h.nw;
@@ -1763,6 +1765,8 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_FUSED: *v = c->fused ? 1.0 : 0.0; return SPH_OK;
         case SPH_S_WALL_SUMS: *v = S.P.wall_sums; return SPH_OK;
         case SPH_S_K_N: *v = S.P.k_n; return SPH_OK;
+        case SPH_S_BOND_K: *v = S.P.bond_k; return SPH_OK;
+        case SPH_S_BOND_BREAK_EXT: *v = S.P.bond_break_ext; return SPH_OK;
         case SPH_S_INTEGRATE_LANES: *v = c->integrate_lanes; return SPH_OK;
         default: return fail(c, SPH_E_ARG, "unknown scalar %d", which);
     }
@@ -1823,6 +1827,17 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             if (v != 0.0 && v != 1.0)
                 return fail(c, SPH_E_ARG, "wall sums must be 0 or 1 (got %g)", v);
             S.P.wall_sums = (uint32_t)v;
+            return upload_params(c);
+        case SPH_S_BOND_K:
+            if (!(v >= 0.0 && v < 1e300))
+                return fail(c, SPH_E_ARG, "bond stiffness must be finite and >= 0 (got %g)", v);
+            S.P.bond_k = v;
+            return upload_params(c);
+        case SPH_S_BOND_BREAK_EXT:
+            if (!(v >= 0.0 && v < 1e300))
+                return fail(c, SPH_E_ARG, "bond break extension must be finite and >= 0 "
+                            "(got %g)", v);
+            S.P.bond_break_ext = v;
             return upload_params(c);
         case SPH_S_INTEGRATE_LANES:
             if (v != 1.0 && v != 4.0)
@@ -1894,39 +1909,63 @@ int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_
     return rc;
 }
 
-// The adjoint of nsteps steps.  Seeds: dL_dx/dv/dw on the final state, and
-// with every > 0 the (nsteps / every) x np x 3 arrays sx / sv on the
-// positions / velocities after every, 2 every, ... steps (either may be null).
+// The adjoint of nsteps steps, over every simulation of the context.  Host
+// arrays concatenate the simulations' particles (rows hbase .. hbase + np
+// of simulation s).  Seeds: dL_dx/dv/dw on the final state, and with
+// every > 0 the (nsteps / every) x rows x 3 arrays sx / sv on the positions
+// / velocities after every, 2 every, ... steps (any may be null).
+// g_params receives nsims x ADJ_NPARAM values.
 static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
                        const double* dL_dw, uint64_t every, const double* sx,
                        const double* sv, double* g_x0, double* g_v0, double* g_w0,
                        double* g_params) {
     if (!c) return SPH_E_ARG;
-    if (c->sims.size() != 1)
-        return fail(c, SPH_E_UNSUPPORTED, "the adjoint runs on a single-simulation context");
-    Sim& S = c->sims[0];
-    const Host& h = S.h;
-    if (h.contactmodel != 2 || h.kappa != 0.0 || h.E > 0.001 || h.nb0 > 0 || c->shear_stress ||
-        c->any_adaptive)
+    const size_t nsim = c->sims.size();
+    size_t rows = 0, nwt = 0;
+    for (const Sim& S : c->sims) {
+        const Host& h = S.h;
+        if (h.contactmodel != 2 || h.kappa != 0.0 || h.E > 0.001)
+            return fail(c, SPH_E_UNSUPPORTED,
+                        "the adjoint covers contact model 2 without cohesion or E-scaled "
+                        "stiffness");
+        for (uint32_t w = 0; w < h.nw; ++w)
+            if (h.wmode[w] < 0 || h.wmode[w] > 2)
+                return fail(c, SPH_E_UNSUPPORTED,
+                            "the adjoint covers fixed, stress-controlled and velocity-controlled "
+                            "walls (wmode 0, 1, 2), not wmode %d", h.wmode[w]);
+        for (uint32_t i = 0; i < h.np; ++i)
+            if (h.vel[4 * i + 3] > 5.0 && h.vel[4 * i + 3] < 10.0)
+                return fail(c, SPH_E_UNSUPPORTED, "the adjoint excludes rate-state particles");
+        rows += h.np;
+        nwt += h.nw;
+    }
+    if (c->shear_stress || c->any_adaptive)
         return fail(c, SPH_E_UNSUPPORTED,
-                    "the adjoint covers contact model 2 without cohesion, E-scaled stiffness, "
-                    "bonds, a wmode-3 wall or an adaptive grid");
-    for (uint32_t i = 0; i < h.np; ++i)
-        if (h.vel[4 * i + 3] > 5.0 && h.vel[4 * i + 3] < 10.0)
-            return fail(c, SPH_E_UNSUPPORTED, "the adjoint excludes rate-state particles");
+                    "the adjoint excludes a wmode-3 wall and an adaptive grid");
     if (nsteps == 0) return SPH_OK;
-    const size_t np = h.np, nw = h.nw, nc = np * NC;
+    DevArrays& A = c->A;
+    const size_t ntot = A.ntot, nc = ntot * NC, nb = A.nbond;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     // Tape per step: x, v, w at the step start and the order (sorted), the
     // displacements of the step's contacts before it and their partners'
     // sorted indices (written by the contact pass, at their slots), the
-    // walls, and the rows the contact pass left.
-    const size_t per_step = up(np * 3 * sizeof(double4)) + up(np * 4) + up(nc * sizeof(double4)) +
-                            up(nc * 4) + up(2 * nw * sizeof(double4)) + up(nc * 4);
+    // walls, the rows the contact pass left, and the bonds' inputs.
+    const size_t per_step = up(ntot * 3 * sizeof(double4)) + up(ntot * 4) +
+                            up(nc * sizeof(double4)) + up(nc * 4) +
+                            up(2 * nwt * sizeof(double4)) + up(nc * 4) +
+                            (nb ? up(nb * 8 * sizeof(double4)) + up(nb * sizeof(uint4)) +
+                                      up(nsim * 4)
+                                : 0);
     // Reverse state: cotangents of x, v, w (two sets), of the two
-    // accelerations, of the displacements (two sets), and the parameters.
-    const size_t state_b = up(np * 10 * sizeof(double4)) + up(2 * nc * sizeof(double4)) +
-                           up(np * 3 * sizeof(double)) + 256;
+    // accelerations, of the displacements (two sets), of the bond state
+    // (two sets), the bonds' contributions and parameter shares, the
+    // workgroup partials, the parameter sums, the walls, the staging rows.
+    const size_t nwg = ntot / ADJ_PPB;
+    const size_t state_b = up(ntot * 10 * sizeof(double4)) + up(2 * nc * sizeof(double4)) +
+                           up(4 * nb * sizeof(double4)) + up(6 * nb * sizeof(double4)) +
+                           up(nb * ADJ_NBPARAM * 8) + up(nwg * ADJ_NPART * 8) +
+                           up(nsim * ADJ_NPARAM * 8) + up(nwt * ADJ_WALL * 8 + 8) +
+                           up(rows * 3 * sizeof(double)) + 256;
     // A checkpoint: every device allocation of the context.
     size_t snap_b = 0;
     for (size_t b : c->alloc_bytes) snap_b += up(b);
@@ -1973,81 +2012,117 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
         }
         c->adj_bytes = (size_t)need;
     }
-    DevArrays& A = c->A;
     hipStream_t s = c->stream;
     char* tape = static_cast<char*>(c->adj_buf);
     struct Rec {
-        double4 *x, *v, *w, *dt0, *wnx, *wmv;
-        uint32_t *perm, *srt, *rows;
+        double4 *x, *v, *w, *dt0, *wnx, *wmv, *bond_in;
+        uint32_t *perm, *srt, *rows, *bond_full;
+        uint4* bond_idx;
     };
     std::vector<Rec> rec(K);
     for (uint64_t k = 0; k < K; ++k) {
         char* q = tape + k * per_step;
         Rec& r = rec[k];
         r.x = reinterpret_cast<double4*>(q);
-        r.v = r.x + np;
-        r.w = r.v + np;
-        q += up(np * 3 * sizeof(double4));
+        r.v = r.x + ntot;
+        r.w = r.v + ntot;
+        q += up(ntot * 3 * sizeof(double4));
         r.perm = reinterpret_cast<uint32_t*>(q);
-        q += up(np * 4);
+        q += up(ntot * 4);
         r.dt0 = reinterpret_cast<double4*>(q);
         q += up(nc * sizeof(double4));
         r.srt = reinterpret_cast<uint32_t*>(q);
         q += up(nc * 4);
         r.wnx = reinterpret_cast<double4*>(q);
-        r.wmv = r.wnx + nw;
-        q += up(2 * nw * sizeof(double4));
+        r.wmv = r.wnx + nwt;
+        q += up(2 * nwt * sizeof(double4));
         r.rows = reinterpret_cast<uint32_t*>(q);
+        q += up(nc * 4);
+        r.bond_in = nullptr;
+        r.bond_idx = nullptr;
+        r.bond_full = nullptr;
+        if (nb) {
+            r.bond_in = reinterpret_cast<double4*>(q);
+            q += up(nb * 8 * sizeof(double4));
+            r.bond_idx = reinterpret_cast<uint4*>(q);
+            q += up(nb * sizeof(uint4));
+            r.bond_full = reinterpret_cast<uint32_t*>(q);
+        }
     }
-    const int64_t iter0 = S.iter;
-    const DevParams P = S.P;
+    const int64_t iter0 = c->sims[0].iter;
     // Reverse-pass state, and the seeds (uploaded before the forward pass,
     // so that only the gradients' download waits for the device).
-    double4* q = reinterpret_cast<double4*>(tape + K * per_step);
+    char* q = tape + K * per_step;
+    auto take = [&](size_t bytes) {
+        char* r = q;
+        q += up(bytes);
+        return r;
+    };
     double4* L[2][3];
     double4 *LA[2], *LAA[2];
-    for (int b = 0; b < 2; ++b) {
-        for (int f = 0; f < 3; ++f) {
-            L[b][f] = q;
-            q += np;
+    {
+        double4* base = reinterpret_cast<double4*>(take(ntot * 10 * sizeof(double4)));
+        for (int b = 0; b < 2; ++b) {
+            for (int f = 0; f < 3; ++f) {
+                L[b][f] = base;
+                base += ntot;
+            }
+            LA[b] = base;
+            base += ntot;
+            LAA[b] = base;
+            base += ntot;
         }
-        LA[b] = q;
-        q += np;
-        LAA[b] = q;
-        q += np;
     }
     AdjState g;
-    double4* ldt[2] = {q, q + nc};
-    q += 2 * nc;
-    double* stage = reinterpret_cast<double*>(q);  // np x 3 host-layout staging
-    g.gparam = reinterpret_cast<double*>(reinterpret_cast<char*>(q) + up(np * 3 * sizeof(double)));
-    g.asym = reinterpret_cast<uint32_t*>(g.gparam + ADJ_NPARAM);
+    double4* ldt0 = reinterpret_cast<double4*>(take(2 * nc * sizeof(double4)));
+    double4* ldt[2] = {ldt0, ldt0 + nc};
+    double4* lb0 = reinterpret_cast<double4*>(take(4 * nb * sizeof(double4)));
+    double4* lb[2] = {lb0, lb0 + 2 * nb};
+    g.bcontrib = reinterpret_cast<double4*>(take(6 * nb * sizeof(double4)));
+    g.bpart = reinterpret_cast<double*>(take(nb * ADJ_NBPARAM * 8));
+    g.part = reinterpret_cast<double*>(take(nwg * ADJ_NPART * 8));
+    g.gparam = reinterpret_cast<double*>(take(nsim * ADJ_NPARAM * 8));
+    g.wall = reinterpret_cast<double*>(take(nwt * ADJ_WALL * 8 + 8));
+    g.asym = reinterpret_cast<uint32_t*>(g.wall + nwt * ADJ_WALL);
+    double* stage = reinterpret_cast<double*>(take(rows * 3 * sizeof(double)));
     char* snaps = tape + K * per_step + state_b;  // (nseg + 1) checkpoints when nseg > 1
+    AdjStep base_st;
+    memset(&base_st, 0, sizeof base_st);
+    base_st.ptab = c->d_ptab;
+    base_st.chunk_sim = A.chunk_sim;
+    base_st.nsim = (uint32_t)nsim;
+    base_st.ntot = (uint32_t)ntot;
+    base_st.nbond = (uint32_t)nb;
+    base_st.bond_sim = A.bond_sim;
+    base_st.badj_off = A.badj_off;
+    base_st.badj = A.badj;
     const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
     int rc = SPH_OK;
     hipError_t e = hipSuccess;
     for (int f = 0; f < 3 && !rc; ++f) {
         if (!seeds[f]) {
-            e = hipMemsetAsync(L[0][f], 0, np * 32, s);
+            e = hipMemsetAsync(L[0][f], 0, ntot * 32, s);
             if (e != hipSuccess) return hip_fail(c, e, "memset");
             continue;
         }
-        rc = h2d(c, stage, seeds[f], np * 24);
+        if (rows) rc = h2d(c, stage, seeds[f], rows * 24);
         if (!rc) {
-            e = launch_adj_expand(L[0][f], stage, (uint32_t)np, s);
+            e = launch_adj_expand(L[0][f], stage, base_st, s);
             if (e != hipSuccess) return hip_fail(c, e, "seed");
         }
     }
     if (rc) return rc;
-    e = hipMemsetAsync(LA[0], 0, 2 * np * 32, s);  // LA[0], LAA[0]
-    if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, ADJ_NPARAM * 8 + 8, s);
+    e = hipMemsetAsync(LA[0], 0, 2 * ntot * 32, s);  // LA[0], LAA[0]
+    if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, nsim * ADJ_NPARAM * 8, s);
+    if (e == hipSuccess) e = hipMemsetAsync(g.wall, 0, nwt * ADJ_WALL * 8 + 8, s);
+    if (e == hipSuccess && nb) e = hipMemsetAsync(lb0, 0, 4 * nb * sizeof(double4), s);
     if (e != hipSuccess) return hip_fail(c, e, "memset");
 
     // Checkpoints: the device allocations, and the host state enqueue_step
-    // reads (the loop mirror and the hash/re-sort bookkeeping).
+    // reads (the loop mirrors and the hash/re-sort bookkeeping).
     struct HostMark {
-        double t;
-        int64_t iter;
+        std::vector<double> t;
+        std::vector<int64_t> iter;
         bool hash_pending, prev_walls_resorted;
     };
     std::vector<HostMark> marks(nseg > 1 ? nseg + 1 : 0);
@@ -2055,23 +2130,32 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     for (size_t k = 0, off = 0; k < c->allocs.size(); off += up(c->alloc_bytes[k]), ++k)
         if (c->allocs[k] == (void*)A.crow) crow_off = off;
     auto checkpoint = [&](uint64_t m, bool save) -> int {
-        char* base = snaps + m * snap_b;
+        char* cbase = snaps + m * snap_b;
         size_t off = 0;
         for (size_t k = 0; k < c->allocs.size(); ++k) {
             void* dev = c->allocs[k];
-            hipError_t e_ = save ? hipMemcpyAsync(base + off, dev, c->alloc_bytes[k],
+            hipError_t e_ = save ? hipMemcpyAsync(cbase + off, dev, c->alloc_bytes[k],
                                                   hipMemcpyDeviceToDevice, s)
-                                 : hipMemcpyAsync(dev, base + off, c->alloc_bytes[k],
+                                 : hipMemcpyAsync(dev, cbase + off, c->alloc_bytes[k],
                                                   hipMemcpyDeviceToDevice, s);
             if (e_ != hipSuccess) return hip_fail(c, e_, "checkpoint");
             off += up(c->alloc_bytes[k]);
         }
         HostMark& hm = marks[m];
         if (save) {
-            hm = {S.h.t, S.iter, c->hash_pending, c->prev_walls_resorted};
+            hm.t.resize(nsim);
+            hm.iter.resize(nsim);
+            for (size_t k = 0; k < nsim; ++k) {
+                hm.t[k] = c->sims[k].h.t;
+                hm.iter[k] = c->sims[k].iter;
+            }
+            hm.hash_pending = c->hash_pending;
+            hm.prev_walls_resorted = c->prev_walls_resorted;
         } else {
-            S.h.t = hm.t;
-            S.iter = hm.iter;
+            for (size_t k = 0; k < nsim; ++k) {
+                c->sims[k].h.t = hm.t[k];
+                c->sims[k].iter = hm.iter[k];
+            }
             c->hash_pending = hm.hash_pending;
             c->prev_walls_resorted = hm.prev_walls_resorted;
         }
@@ -2079,7 +2163,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     };
     // Forward steps [k0, k0 + n) with the tape; the contact pass of each step
     // records the state and walls the step starts from, its rows and the
-    // displacements it read.
+    // displacements it read, the bond pass the bonds' inputs.
     auto record = [&](uint64_t n) -> int {
         int rc_ = SPH_OK;
         for (uint64_t k = 0; k < n && !rc_; ++k) {
@@ -2093,9 +2177,14 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
             A.tape_w = r.w;
             A.tape_wnx = r.wnx;
             A.tape_wmv = r.wmv;
+            A.tape_bond_in = r.bond_in;
+            A.tape_bond_idx = r.bond_idx;
+            A.tape_bond_full = r.bond_full;
             rc_ = enqueue_step(c);
-            A.tape_rows = A.tape_srt = A.tape_perm = nullptr;
+            A.tape_rows = A.tape_srt = A.tape_perm = A.tape_bond_full = nullptr;
             A.tape_dt0 = A.tape_x = A.tape_v = A.tape_w = A.tape_wnx = A.tape_wmv = nullptr;
+            A.tape_bond_in = nullptr;
+            A.tape_bond_idx = nullptr;
         }
         return rc_;
     };
@@ -2106,7 +2195,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
         for (uint64_t kk = n; kk-- > 0;) {
             const Rec& r = rec[kk];
             const uint64_t gk = k0 + kk;
-            AdjStep st;
+            AdjStep st = base_st;
             st.x = r.x;
             st.v = r.v;
             st.w = r.w;
@@ -2117,6 +2206,9 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
             st.prev_rows = kk > 0 ? rec[kk - 1].rows : prev0;
             st.wall_nx = r.wnx;
             st.wall_mvfd = r.wmv;
+            st.bond_in = r.bond_in;
+            st.bond_idx = r.bond_idx;
+            st.bond_full = r.bond_full;
             st.first = (iter0 + (int64_t)gk == 0) ? 1 : 0;
             st.last = gk + 1 == nsteps ? 1 : 0;
             // A loss term on the state after this step: added to its
@@ -2125,10 +2217,10 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
                 const size_t m = (size_t)((gk + 1) / every - 1);
                 const double* samp[2] = {sx, sv};
                 for (int f = 0; f < 2; ++f) {
-                    if (!samp[f]) continue;
-                    const int rc_ = h2d(c, stage, samp[f] + m * np * 3, np * 24);
+                    if (!samp[f] || !rows) continue;
+                    const int rc_ = h2d(c, stage, samp[f] + m * rows * 3, rows * 24);
                     if (rc_) return rc_;
-                    hipError_t ea = launch_adj_accumulate(L[cur][f], stage, (uint32_t)np, s);
+                    hipError_t ea = launch_adj_accumulate(L[cur][f], stage, st, s);
                     if (ea != hipSuccess) return hip_fail(c, ea, "sample seed");
                 }
             }
@@ -2142,7 +2234,9 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
             g.Laa_in = LAA[cur ^ 1];
             g.Ldt_in = ldt[cur];
             g.Ldt_out = ldt[cur ^ 1];
-            hipError_t e_ = launch_adj_step(P, st, g, P.L[2], s);
+            g.Lb_out = lb[cur];
+            g.Lb_in = lb[cur ^ 1];
+            hipError_t e_ = launch_adj_step(st, g, s);
             if (e_ != hipSuccess) return hip_fail(c, e_, "adjoint step");
             cur ^= 1;
         }
@@ -2173,10 +2267,10 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     if (rc) return rc;
     double* outs[3] = {g_x0, g_v0, g_w0};
     for (int f = 0; f < 3 && !rc; ++f) {
-        if (!outs[f]) continue;
-        e = launch_adj_pack(stage, L[cur][f], (uint32_t)np, s);
+        if (!outs[f] || !rows) continue;
+        e = launch_adj_pack(stage, L[cur][f], base_st, s);
         if (e != hipSuccess) return hip_fail(c, e, "pack");
-        rc = d2h(c, outs[f], stage, np * 24);
+        rc = d2h(c, outs[f], stage, rows * 24);
     }
     uint32_t asym = 0;
     if (!rc) rc = d2h(c, &asym, g.asym, 4);
@@ -2185,13 +2279,31 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
                     "%u recorded contacts were missing from their partner's row (a contact row "
                     "overflowed NC = %d); the gradients are incomplete",
                     asym, NC);
-    if (!rc && g_params) rc = d2h(c, g_params, g.gparam, ADJ_NPARAM * 8);
+    if (!rc && g_params) rc = d2h(c, g_params, g.gparam, nsim * ADJ_NPARAM * 8);
     return rc;
+}
+
+static int single_sim_adjoint(sph_ctx* c) {
+    if (!c) return SPH_E_ARG;
+    if (c->sims.size() != 1)
+        return fail(c, SPH_E_UNSUPPORTED,
+                    "sph_adjoint / sph_adjoint_samples act on a single-simulation context; "
+                    "use sph_adjoint_batch / sph_adjoint_samples_batch for a batch");
+    return SPH_OK;
+}
+
+static int check_samples(sph_ctx* c, uint64_t every, uint64_t nsamples) {
+    if (every == 0 || nsamples == 0 || nsamples > (uint64_t)1e12 / every)
+        return fail(c, SPH_E_ARG, "sampling every %llu steps, %llu times: both must be "
+                    "positive", (unsigned long long)every, (unsigned long long)nsamples);
+    return SPH_OK;
 }
 
 int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
                 const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
                 double* g_params) {
+    const int rc = single_sim_adjoint(c);
+    if (rc) return rc;
     return adjoint_run(c, nsteps, dL_dx, dL_dv, dL_dw, 0, nullptr, nullptr, g_x0, g_v0, g_w0,
                        g_params);
 }
@@ -2199,10 +2311,27 @@ int sph_adjoint(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* 
 int sph_adjoint_samples(sph_ctx* c, uint64_t every, uint64_t nsamples, const double* dL_dx,
                         const double* dL_dv, double* g_x0, double* g_v0, double* g_w0,
                         double* g_params) {
+    int rc = single_sim_adjoint(c);
+    if (!rc) rc = check_samples(c, every, nsamples);
+    if (rc) return rc;
+    return adjoint_run(c, every * nsamples, nullptr, nullptr, nullptr, every, dL_dx, dL_dv,
+                       g_x0, g_v0, g_w0, g_params);
+}
+
+int sph_adjoint_batch(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
+                      const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
+                      double* g_params) {
     if (!c) return SPH_E_ARG;
-    if (every == 0 || nsamples == 0 || nsamples > (uint64_t)1e12 / every)
-        return fail(c, SPH_E_ARG, "sampling every %llu steps, %llu times: both must be "
-                    "positive", (unsigned long long)every, (unsigned long long)nsamples);
+    return adjoint_run(c, nsteps, dL_dx, dL_dv, dL_dw, 0, nullptr, nullptr, g_x0, g_v0, g_w0,
+                       g_params);
+}
+
+int sph_adjoint_samples_batch(sph_ctx* c, uint64_t every, uint64_t nsamples,
+                              const double* dL_dx, const double* dL_dv, double* g_x0,
+                              double* g_v0, double* g_w0, double* g_params) {
+    if (!c) return SPH_E_ARG;
+    const int rc = check_samples(c, every, nsamples);
+    if (rc) return rc;
     return adjoint_run(c, every * nsamples, nullptr, nullptr, nullptr, every, dL_dx, dL_dv,
                        g_x0, g_v0, g_w0, g_params);
 }

@@ -64,7 +64,8 @@ SCALARS = {"time": 0, "time_total": 1, "dt": 2, "file_dt": 3, "iter": 4, "step_c
            "np": 6, "nw": 7, "nb0": 8, "ncells": 9, "vl_skin": 10, "sorts": 11, "vl_builds": 12,
            "nsims": 13, "contact_lanes": 14, "sort_mode": 15, "resort_merge_max": 16,
            "graph_steps": 17, "xcd_order": 18, "adj_checkpoint": 19, "fused": 20,
-           "wall_sums": 21, "k_n": 22, "integrate_lanes": 23}
+           "wall_sums": 21, "k_n": 22, "integrate_lanes": 23, "bond_k": 24,
+           "bond_break_ext": 25}
 
 # Exported symbols of the C ABI (include/sphere_hip.h).
 SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_write_bin",
@@ -74,7 +75,8 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_open_error", "sph_version_string", "sph_open_batch", "sph_open_batch_mem",
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
-           "sph_set_scalar_sim", "sph_adjoint", "sph_adjoint_samples", "sph_selftest_sqrt32",
+           "sph_set_scalar_sim", "sph_adjoint", "sph_adjoint_samples", "sph_adjoint_batch",
+           "sph_adjoint_samples_batch", "sph_selftest_sqrt32",
            "sph_write_state", "sph_read_state", "sph_write_state_sim", "sph_read_state_sim",
            "sph_find_overlaps", "sph_porosity", "sph_autotune")
 
@@ -149,6 +151,9 @@ def load():
                                      ctypes.POINTER(ctypes.c_double)]
     L.sph_set_scalar_sim.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_double]
     L.sph_adjoint.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp, vp]
+    L.sph_adjoint_batch.argtypes = [vp, ctypes.c_uint64, vp, vp, vp, vp, vp, vp, vp]
+    L.sph_adjoint_samples_batch.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp,
+                                            vp, vp, vp]
     L.sph_autotune.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.sph_adjoint_samples.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp, vp,
                                       vp]
@@ -278,37 +283,49 @@ class Context:
     def run(self, out_dir, sid):
         self._chk(self._L.sph_run(self._h, out_dir.encode(), sid.encode()))
 
+    def rows(self):
+        """Particle rows of the host arrays the adjoint takes and returns:
+        every simulation's particles, in simulation order."""
+        return int(sum(self.scalar("np", sim=k) for k in range(self.nsims)))
+
+    def _adjoint_result(self, outs, gp):
+        res = {"x": outs[0], "v": outs[1], "w": outs[2]}
+        if self.nsims == 1:
+            res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp[0])})
+        else:
+            res.update({name: gp[:, k].copy() for k, name in enumerate(ADJOINT_PARAMS)})
+        return res
+
     def adjoint(self, nsteps, dL_dx=None, dL_dv=None, dL_dw=None):
         """Gradient of L = sum dL_dx.x + dL_dv.v + dL_dw.w after `nsteps` steps
         with respect to the state before them and to the contact parameters
-        (sph_adjoint).  Advances the simulation by nsteps.  Returns a dict
-        with "x", "v", "w" (np x 3) and one entry per ADJOINT_PARAMS name."""
-        n = int(self.scalar("np"))
+        (sph_adjoint; sph_adjoint_batch for a batch, whose particle arrays
+        concatenate the simulations).  Advances the simulation by nsteps.
+        Returns a dict with "x", "v", "w" (rows x 3) and one entry per
+        ADJOINT_PARAMS name (a float, or one value per simulation)."""
+        n = self.rows()
 
         def seed(a):
             if a is None:
                 return None
-            a = numpy.ascontiguousarray(a, dtype=numpy.float64).reshape(n, 3)
-            return a
+            return numpy.ascontiguousarray(a, dtype=numpy.float64).reshape(n, 3)
 
         seeds = [seed(a) for a in (dL_dx, dL_dv, dL_dw)]
         outs = [numpy.zeros((n, 3)) for _ in range(3)]
-        gp = numpy.zeros(len(ADJOINT_PARAMS))
+        gp = numpy.zeros((self.nsims, len(ADJOINT_PARAMS)))
         ptr = [None if a is None else a.ctypes.data for a in seeds]
-        self._chk(self._L.sph_adjoint(self._h, int(nsteps), ptr[0], ptr[1], ptr[2],
-                                      outs[0].ctypes.data, outs[1].ctypes.data,
-                                      outs[2].ctypes.data, gp.ctypes.data))
-        res = {"x": outs[0], "v": outs[1], "w": outs[2]}
-        res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp)})
-        return res
+        fn = self._L.sph_adjoint if self.nsims == 1 else self._L.sph_adjoint_batch
+        self._chk(fn(self._h, int(nsteps), ptr[0], ptr[1], ptr[2], outs[0].ctypes.data,
+                     outs[1].ctypes.data, outs[2].ctypes.data, gp.ctypes.data))
+        return self._adjoint_result(outs, gp)
 
     def adjoint_samples(self, every, nsamples, dL_dx=None, dL_dv=None):
         """Gradient of L = sum_k dL_dx[k] . x(k) + dL_dv[k] . v(k), x(k) and
-        v(k) the state after (k + 1) * every steps (sph_adjoint_samples),
-        with respect to the state now and to the contact parameters.
-        Advances the simulation by every * nsamples steps; returns the same
-        dict as adjoint()."""
-        n = int(self.scalar("np"))
+        v(k) the state after (k + 1) * every steps (sph_adjoint_samples, or
+        sph_adjoint_samples_batch for a batch), with respect to the state
+        now and to the contact parameters.  Advances the simulation by
+        every * nsamples steps; returns the same dict as adjoint()."""
+        n = self.rows()
 
         def seeds(a):
             if a is None:
@@ -317,14 +334,13 @@ class Context:
 
         sx, sv = seeds(dL_dx), seeds(dL_dv)
         outs = [numpy.zeros((n, 3)) for _ in range(3)]
-        gp = numpy.zeros(len(ADJOINT_PARAMS))
-        self._chk(self._L.sph_adjoint_samples(
-            self._h, int(every), int(nsamples), None if sx is None else sx.ctypes.data,
-            None if sv is None else sv.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data,
-            outs[2].ctypes.data, gp.ctypes.data))
-        res = {"x": outs[0], "v": outs[1], "w": outs[2]}
-        res.update({name: float(v) for name, v in zip(ADJOINT_PARAMS, gp)})
-        return res
+        gp = numpy.zeros((self.nsims, len(ADJOINT_PARAMS)))
+        fn = (self._L.sph_adjoint_samples if self.nsims == 1
+              else self._L.sph_adjoint_samples_batch)
+        self._chk(fn(self._h, int(every), int(nsamples), None if sx is None else sx.ctypes.data,
+                     None if sv is None else sv.ctypes.data, outs[0].ctypes.data,
+                     outs[1].ctypes.data, outs[2].ctypes.data, gp.ctypes.data))
+        return self._adjoint_result(outs, gp)
 
     def autotune(self, steps_per_trial=50):
         """sph_autotune: choose the contact lanes and the workgroup order by
@@ -347,16 +363,20 @@ class Context:
         return rep
 
 
-def run_file(path, out_dir, sid, verbose=True, dry=False, device=-1, sidecar=False):
+def run_file(path, out_dir, sid, verbose=True, dry=False, device=-1, sidecar=False,
+             scalars=None):
     """Open a .bin and run DEM::startTime's loop; returns 0 or a negative code.
     With `sidecar`, a restart sidecar next to the input is read and one is
-    written next to every output."""
+    written next to every output.  `scalars` ({name: value}, SCALARS names)
+    are set after opening (options the .bin cannot carry)."""
     try:
         ctx = Context(path=path, device=device, quiet=not verbose, sidecar=sidecar)
     except EngineError as e:
         print(str(e))
         return e.code
     try:
+        for name, value in (scalars or {}).items():
+            ctx.set_scalar(name, value)
         if dry:
             return 0
         ctx.run(out_dir, sid)

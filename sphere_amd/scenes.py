@@ -538,7 +538,8 @@ def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, depth=0.35, ph
                        shear_vel=1.0e-3, rho=2600.0, open_stepper=None, gravity_factor=30.0,
                        settle_steps=40000, ramp_stages=8, ramp_steps=2000, quench_every=500,
                        consolidate_steps=10000, free_steps=2000, chunk=2000,
-                       grow_to_phi=0.5, grow_stages=30, grow_steps=100, log=None):
+                       grow_to_phi=0.5, grow_stages=30, grow_steps=100, log=None,
+                       r_uniform=None, shear=True):
     """The direct-shear workload, prepacked by seeded gravity settling.
 
     Lognormal radii (median, sigma) are inserted without overlap at fraction
@@ -557,9 +558,15 @@ def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, depth=0.35, ph
     With `grow_to_phi`, the particles are first inserted at `phi_rsa` into a
     box of that fraction and grown to their true radii (grow_pack) before
     the box top is lifted and the bed settles.
-    `open_stepper(image)` runs the DEM steps (GPU context or oracle)."""
+    `open_stepper(image)` runs the DEM steps (GPU context or oracle).
+    With `r_uniform` = (lo, hi) the radii are uniform in [lo, hi] instead;
+    with shear=False the bed is left consolidated under the wall (no
+    sheared layers)."""
     rng = numpy.random.default_rng(seed)
-    r = lognormal_radii(n, median, sigma, rng)
+    if r_uniform is not None:
+        r = rng.uniform(r_uniform[0], r_uniform[1], n)
+    else:
+        r = lognormal_radii(n, median, sigma, rng)
     vol = float(numpy.sum(4.0 / 3.0 * math.pi * r ** 3))
     if lxy is None:
         # A wide, shallow cell: `depth` of settled bed at a fraction near 0.56.
@@ -627,6 +634,8 @@ def settled_shear_cell(n, seed, median=0.01, sigma=0.2, lxy=None, depth=0.35, ph
                 done, consolidate_steps, contact_number(st, n)))
     s.frombytes(st.tobytes())
     s.time_current[0] = 0.0
+    if not shear:
+        return s
     z_min = float(numpy.min(s.x[:, 2] - s.radius))
     z_max = float(numpy.max(s.x[:, 2] + s.radius))
     s.shear(shear_strain_rate=shear_vel / (z_max - z_min))

@@ -892,7 +892,8 @@ class sim:
 
     # -- running -------------------------------------------------------------
     def run(self, verbose=True, hideinputfile=False, dry=False, valgrind=False,
-            cudamemcheck=False, device=-1, root=None, sidecar=False, restart_state=None):
+            cudamemcheck=False, device=-1, root=None, sidecar=False, restart_state=None,
+            bond_k=0.0, bond_break_ext=0.0):
         """Run the simulation (python/sphere.py:4482-4540).
 
         Writes <root>/input/<sid>.bin, then runs the DEM loop in-process on the
@@ -902,7 +903,10 @@ class sim:
         `sidecar` also writes a restart sidecar (.state) next to every
         output, and `restart_state` (the sidecar of the output this sim was
         read from) makes the run continue bit for bit where that one left
-        off."""
+        off.  `bond_k` (N/m) and `bond_break_ext` (fraction of the smaller
+        radius) select the engine's bond options (SPH_S_BOND_K,
+        SPH_S_BOND_BREAK_EXT; 0 keeps the reference's bond law), which the
+        .bin layout cannot carry."""
         from . import engine
         rt = _root(root)
         fn = self.writebin(os.path.join(rt, "input"), verbose=False)
@@ -915,8 +919,14 @@ class sim:
         if valgrind or cudamemcheck:
             print("Warning: valgrind/cuda-memcheck wrappers are not available; "
                   "running normally")
+        opts = {}
+        if bond_k:
+            opts["bond_k"] = float(bond_k)
+        if bond_break_ext:
+            opts["bond_break_ext"] = float(bond_break_ext)
         status_ = engine.run_file(fn, os.path.join(rt, "output"), self.sid,
-                                  verbose=verbose, dry=dry, device=device, sidecar=sidecar)
+                                  verbose=verbose, dry=dry, device=device, sidecar=sidecar,
+                                  scalars=opts)
         if status_ != 0:
             print("Warning: the sphere run returned with status " + str(status_))
         return status_

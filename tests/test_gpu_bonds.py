@@ -50,6 +50,33 @@ def test_bond_breakage_bit_exact():
     ctx.close()
 
 
+@pytest.mark.parametrize("bond_k,ext", [(1.0e5, 0.0), (0.0, 2.0e-3), (1.0e5, 2.0e-3)])
+def test_bond_options_bit_exact(bond_k, ext):
+    """The bond options (SPH_S_BOND_K, SPH_S_BOND_BREAK_EXT): an explicit
+    bond stiffness in N/m and a break at a normal extension, on the engine
+    and the oracle; with the extension break some bonds break."""
+    s = _bonded(frac=0.3, seed=4)
+    rng = numpy.random.default_rng(6)
+    s.vel[:, :3] += rng.normal(0.0, 0.01, (int(s.np[0]), 3))
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ref.set_bond_model(bond_k, ext)
+    with engine.Context(image) as ctx:
+        ctx.set_scalar("bond_k", bond_k)
+        ctx.set_scalar("bond_break_ext", ext)
+        assert ctx.scalar("bond_k") == bond_k and ctx.scalar("bond_break_ext") == ext
+        for _ in range(4):
+            ref.step(30)
+            ctx.step(30)
+            bad = _compare(ctx, ref, EXACT_FIELDS + ("bonds", "bonds_delta", "bonds_omega"))
+            assert bad == []
+    broken = int((ref.get("bonds")[:, 0] == int(s.nb0[0])).sum())
+    if ext > 0.0:
+        assert broken > 0
+    else:
+        assert broken == 0
+
+
 def test_wmode3_shear_stress_bit_exact():
     s = cases.fixture("shear1k.bin")
     s.shear(shear_stress=2.0e3)

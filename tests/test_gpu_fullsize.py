@@ -88,6 +88,37 @@ def test_bonded_shear_bed_1e6_bit_exact():
     ctx.close()
 
 
+def test_bonded_shear_bed_1e6_breaking_bit_exact():
+    """The large bonded shear bed with its configured bonds: 1e6 spheres
+    (lognormal radii, median 0.005 m, sigma 0.25), 20 kPa, 5e-4 m/s, 2% of
+    the touching pairs bonded with a stiffness of 1e8 N/m that break at a
+    normal extension of 0.05 of the smaller radius (SPH_S_BOND_K,
+    SPH_S_BOND_BREAK_EXT).  Bonds break throughout the 300 steps; every
+    field and every bond bit-exact with the oracle at steps 100, 200, 300."""
+    s = cases.lattice_bed(1000000, seed=5001, median=0.005, sigma=0.25, k_n=1.0e8,
+                          normal_stress=20.0e3, shear_vel=5.0e-4, bond_frac=0.02)
+    image = s.tobytes()
+    nb = int(s.nb0[0])
+    ref = oracle.Oracle(image)
+    ref.set_bond_model(1.0e8, 0.05)
+    ctx = engine.Context(image)
+    ctx.set_scalar("bond_k", 1.0e8)
+    ctx.set_scalar("bond_break_ext", 0.05)
+    broken = []
+    for k in range(3):
+        print("1e6 bonded bed with breaking bonds: steps %d..%d" % (100 * k, 100 * (k + 1)),
+              flush=True)
+        ref.step(100)
+        ctx.step(100)
+        bad = _compare(ctx, ref) + _compare(ctx, ref, WALL_FIELDS)
+        bad += _compare(ctx, ref, ("bonds", "bonds_delta", "bonds_omega"))
+        assert bad == [], (k, bad)
+        broken.append(int((ref.get("bonds")[:, 0] == nb).sum()))
+    print("bonds broken after 100, 200, 300 steps: %s of %d" % (broken, nb), flush=True)
+    assert 0 < broken[0] < broken[1] < broken[2] < nb
+    ctx.close()
+
+
 def test_batch_of_two_full_beds_matches_single():
     """A batch holding the 1e5 bed twice steps each copy exactly as a
     context of its own."""

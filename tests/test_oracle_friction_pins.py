@@ -713,6 +713,67 @@ def test_bond_breaks_at_the_first_step_reaching_the_strength(which):
             assert (o.get("force")[0] != 0.0).any()
 
 
+@pytest.mark.parametrize("mode", ["axial", "bend"])
+def test_bond_stiffness_option_sets_the_frequency(mode):
+    """The engine's bond option SPH_S_BOND_K (here on the oracle): a bond
+    stiffness k_b in N/m in place of k_n A, so the axial mode oscillates at
+    omega^2 = 2 k_b / m and the bend mode at 2 (k_b / A) I / I_p, whatever
+    the contact stiffness."""
+    s, _ = bonded_pair(mode, k=1.16e9)
+    k_b = 1.0e8
+    r = s.radius[0]
+    m = _mass(r)
+    A = math.pi * r ** 2
+    I = 0.25 * math.pi * r ** 4
+    omega = (math.sqrt(2.0 * k_b / m) if mode == "axial"
+             else math.sqrt(2.0 * (k_b / A) * I / (0.4 * m * r * r)))
+    period = 2.0 * math.pi / omega
+    s.time_dt[0] = period / 2000.0
+    o = oracle.Oracle(s.tobytes())
+    o.set_bond_model(bond_k=k_b)
+    rel, t = [], []
+    for k in range(int(3 * 2000)):
+        o.step(1)
+        if mode == "axial":
+            v = o.get("vel")
+            rel.append(v[1, 0] - v[0, 0])
+        else:
+            w = o.get("angvel")
+            rel.append(w[1, 1] - w[0, 1])
+        t.append((k + 1) * s.time_dt[0])
+    rel, t = numpy.array(rel), numpy.array(t)
+    crossings = numpy.nonzero(numpy.diff(numpy.sign(rel)) != 0)[0]
+    half = numpy.diff(t[crossings]).mean()
+    assert abs(half - period / 2.0) / (period / 2.0) < 0.01
+
+
+def test_bond_extension_break_option():
+    """SPH_S_BOND_BREAK_EXT: the bond breaks in the first step whose normal
+    extension (-delta_n) reaches the fraction of the smaller radius, taken
+    from the same pair stepped without the option; the stress strengths do
+    not matter then."""
+    s, _ = bonded_pair("axial", k=1.0e8)
+    s.vel[1, 0] = 0.05
+    s.radius[1] = 0.4
+    s.x[1, 0] = s.x[0, 0] + 0.5 + 0.4 + 0.1
+    rmin = 0.4
+    o = oracle.Oracle(s.tobytes())
+    ext = []
+    for _ in range(60):
+        o.step(1)
+        ext.append(-o.get("bonds_delta")[0, 3])
+    ext = numpy.array(ext)
+    assert (numpy.diff(ext[:20]) > 0).all()
+    frac = 0.5 * (ext[14] + ext[15]) / rmin
+    s.sigma_b[0] = s.tau_b[0] = 0.0  # the stress criterion would break it at once
+    o = oracle.Oracle(s.tobytes())
+    o.set_bond_model(bond_break_ext=frac)
+    for k_step in range(1, 18):
+        o.step(1)
+        broken = int(o.get("bonds")[0, 0]) == s.nb0[0]
+        assert broken == (k_step >= 16), k_step
+
+
 # ---------------------------------------------------------------------------
 # Boundary conditions around the step
 
