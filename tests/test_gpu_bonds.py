@@ -50,14 +50,16 @@ def test_bond_breakage_bit_exact():
     ctx.close()
 
 
-@pytest.mark.parametrize("bond_k,ext", [(1.0e5, 0.0), (0.0, 2.0e-3), (1.0e5, 2.0e-3)])
+@pytest.mark.parametrize("bond_k,ext", [(1.0e5, 0.0), (0.0, 1.0e-4), (1.0e5, 1.0e-4)])
 def test_bond_options_bit_exact(bond_k, ext):
     """The bond options (SPH_S_BOND_K, SPH_S_BOND_BREAK_EXT): an explicit
     bond stiffness in N/m and a break at a normal extension, on the engine
-    and the oracle; with the extension break some bonds break."""
+    and the oracle; with the extension break (1e-4 of the smaller radius)
+    some bonds break within the 120 steps (about 35 of 530 on the oracle)."""
     s = _bonded(frac=0.3, seed=4)
     rng = numpy.random.default_rng(6)
     s.vel[:, :3] += rng.normal(0.0, 0.01, (int(s.np[0]), 3))
+    first = s.bonds[:, 0].copy()
     image = s.tobytes()
     ref = oracle.Oracle(image)
     ref.set_bond_model(bond_k, ext)
@@ -70,7 +72,10 @@ def test_bond_options_bit_exact(bond_k, ext):
             ctx.step(30)
             bad = _compare(ctx, ref, EXACT_FIELDS + ("bonds", "bonds_delta", "bonds_omega"))
             assert bad == []
-    broken = int((ref.get("bonds")[:, 0] == int(s.nb0[0])).sum())
+    # A broken bond's first index becomes nb0 (src/cohesion.cuh:205); a
+    # bond whose first particle is nb0 from the start is not counted.
+    nb0 = int(s.nb0[0])
+    broken = int(((ref.get("bonds")[:, 0] == nb0) & (first != nb0)).sum())
     if ext > 0.0:
         assert broken > 0
     else:
