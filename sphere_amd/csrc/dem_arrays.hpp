@@ -72,10 +72,11 @@ struct DevArrays {
     uint32_t* ce_bk;
     // candidate lists (sorted-index rows of NV entries)
     uint2* vlist;
-    uint32_t* vcount;    // entries per row, or VL_OVERFLOW
+    uint2* vcount;       // {entries per row | flags, or VL_OVERFLOW; hit mask}
     double4* x_build;    // position at the last list build, original index
     // contact history (NC words per particle, original-index rows)
-    uint32_t* crow;      // packed partner index + periodic image code
+    uint32_t* crow;      // packed partner index (in perm_rows' order) + image code
+    uint32_t* perm_rows; // the order the row words index: sorted index -> original
     double4* delta_t;    // tangential displacement per slot (w unused)
     // Recording for the adjoint (null otherwise): the contact pass writes
     // its rows and, per contact, the displacement before the step and the

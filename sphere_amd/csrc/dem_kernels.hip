@@ -349,7 +349,10 @@ __global__ __launch_bounds__(BLOCK) void k_cells(const DevArrays a, int sorted_b
     if (!sort_needed(a.glob, a.loop)) return;
     const DevParams& p = chunk_params(a, blockIdx.x);
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i == p.base) atomicAdd(&a.loop[p.sim].sorts, 1u);
+    if (i == p.base) {
+        atomicAdd(&a.loop[p.sim].sorts, 1u);
+        a.loop[p.sim].order_changed = 1u;
+    }
     cells_one(p, a, i, sorted_bits, cur_set(a.loop + p.sim));
 }
 
@@ -644,6 +647,22 @@ __device__ __forceinline__ uint32_t word_code(uint32_t w) {
            (cy == 2u ? 8u : 0u);
 }
 
+// A row word written in the order of perm_rows, mapped to the current
+// sorted order through the partner's original index (after a re-sort).
+__device__ __forceinline__ uint32_t word_to_current(const DevArrays& a, uint32_t w) {
+    const uint32_t idx = w & CW_IDX_MASK;
+    if (idx == EMPTY) return w;
+    return (w & ~CW_IDX_MASK) | a.rank[a.perm_rows[idx]];
+}
+
+// A row word of the current sorted order mapped to the partner's original
+// index (the adjoint's tape keeps original indices).
+__device__ __forceinline__ uint32_t word_to_orig(const DevArrays& a, uint32_t w) {
+    const uint32_t idx = w & CW_IDX_MASK;
+    if (idx == EMPTY) return w;
+    return (w & ~CW_IDX_MASK) | a.perm[idx];
+}
+
 // ---------------------------------------------------------------------------
 // topology + interact, one work-item per particle in sorted order
 // (src/contactsearch.cuh:329-371 then :385-726).  The particle's contact
@@ -658,21 +677,21 @@ constexpr int ROW_STRIDE = NC + 1;
 // partner (src/contactsearch.cuh:265-299): the last empty slot, else the
 // partner's own slot, else slot nc.  Returns true when nc exceeds NC, which
 // ends the scan of the current cell.
-__device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t ib_orig,
-                                                 uint32_t code, int& nc, DevLoop* loop) {
+__device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t ib, uint32_t code,
+                                                 int& nc, DevLoop* loop) {
     int cpos = nc;
     for (int i = 0; i < NC; ++i) {
         const uint32_t cidx = row[i] & CW_IDX_MASK;
         if (cidx == EMPTY) {
             cpos = i;
-        } else if (cidx == ib_orig) {
+        } else if (cidx == ib) {
             cpos = i;
             break;
         }
     }
     // cpos == NC would write into the next particle's row
     // (src/contactsearch.cuh:286); that write is suppressed.
-    if (cpos < NC) row[cpos] = pack_word(ib_orig, code);
+    if (cpos < NC) row[cpos] = pack_word(ib, code);
     ++nc;
     if (nc > NC) {
         atomicOr(&loop->error_bits, (uint32_t)DFLAG_CONTACT_OVERFLOW);
@@ -810,6 +829,9 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
             const uint4 v = grow[q];
             row[4 * q] = v.x; row[4 * q + 1] = v.y; row[4 * q + 2] = v.z; row[4 * q + 3] = v.w;
         }
+        // After a re-sort the words are mapped to the new order.
+        if (lp->order_changed != 0u)
+            for (int i = 0; i < NC; ++i) row[i] = word_to_current(a, row[i]);
         // topology (findContactsInCell over the 27 cells, z-major, each cell
         // in ascending sorted index).  Three ways to visit the same candidate
         // sequence: the full scan that also rebuilds this particle's
@@ -817,7 +839,11 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
         int nc = 0;
         const bool rebuild =
             !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
-        const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
+        const uint2 vmeta = rebuild ? make_uint2(0u, 0u) : a.vcount[ia];
+        const uint32_t vcnt = vmeta.x == VL_OVERFLOW ? VL_OVERFLOW : (vmeta.x & VC_COUNT_MASK);
+        // This kernel keeps no hit mask: a valid one would go stale.
+        if (!rebuild && vcnt != VL_OVERFLOW && (vmeta.x & VC_HP_VALID) != 0u)
+            a.vcount[ia] = make_uint2(vcnt, 0u);
         if (rebuild || vcnt == VL_OVERFLOW) {
             const bool build = p.vl_enabled && rebuild;
             uint32_t nl = 0;
@@ -851,7 +877,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                                 if (dd < Sv * Sv) {
                                     if (nl < (uint32_t)NV)
                                         vrow[nl] = make_uint2(ib | (celloff << VL_CELL_SHIFT),
-                                                              pack_word(a.perm[ib], (uint32_t)code));
+                                                              pack_word(ib, (uint32_t)code));
                                     ++nl;
                                 }
                             }
@@ -862,12 +888,11 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                             if (dd >= S * S * 1.0000152587890625) continue;
                             const double len = sqrt32(dd);
                             if (len - S < 0.0)
-                                cell_done = register_contact(row, a.perm[ib], (uint32_t)code,
-                                                             nc, lp);
+                                cell_done = register_contact(row, ib, (uint32_t)code, nc, lp);
                         }
                     }
             if (build) {
-                a.vcount[ia] = nl <= (uint32_t)NV ? nl : VL_OVERFLOW;
+                a.vcount[ia] = make_uint2(nl <= (uint32_t)NV ? nl : VL_OVERFLOW, 0u);
                 a.x_build[ia_orig] = xa;
             }
         } else {
@@ -900,10 +925,9 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
         for (int i = 0; i < NC; ++i) {
             const size_t mp = (size_t)ia_orig * NC + i;
             const uint32_t wrd = row[i];
-            const uint32_t ib_orig = wrd & CW_IDX_MASK;
-            if (ib_orig != EMPTY) {
+            const uint32_t rb_s = wrd & CW_IDX_MASK;
+            if (rb_s != EMPTY) {
                 const d3 dm = shift_of(p, word_code(wrd));
-                const uint32_t rb_s = a.rank[ib_orig];
                 const double4 xb = xs[rb_s];
                 const double rb = xb.w;
                 const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
@@ -943,12 +967,9 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
 #pragma unroll
         for (int q = 0; q < NC / 4; ++q)
             wrow[q] = make_uint4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
-        if (a.tape_rows) {
-            uint4* trow = reinterpret_cast<uint4*>(a.tape_rows + (size_t)ia_orig * NC);
-#pragma unroll
-            for (int q = 0; q < NC / 4; ++q)
-                trow[q] = make_uint4(row[4 * q], row[4 * q + 1], row[4 * q + 2], row[4 * q + 3]);
-        }
+        if (a.tape_rows)
+            for (int i = 0; i < NC; ++i)
+                a.tape_rows[(size_t)ia_orig * NC + i] = word_to_orig(a, row[i]);
     } else if constexpr (CM == 1) {
         // findAndProcessContactsInCell, src/contactsearch.cuh:91-186.
         for (int dz = -1; dz < 2; ++dz)
@@ -1303,30 +1324,29 @@ __device__ __forceinline__ void wave_sync() {
 // The contact-slot policy (src/contactsearch.cuh:265-299) over the slot
 // registers of a group: the first slot holding the partner, else the last
 // empty slot, else slot nc (whose write is suppressed, as in the one-lane
-// kernel).  Returns true when nc exceeds NC, which ends the current cell.
+// kernel; `clobber` is then set).  Returns true when nc exceeds NC, which
+// ends the current cell.
 template <int LPP>
 __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t (&slot)[NC / LPP],
-                                               uint32_t (&srt)[NC / LPP], uint32_t word,
-                                               uint32_t sib, int& nc, DevLoop* loop) {
+                                               uint32_t word, int& nc, bool& clobber,
+                                               DevLoop* loop) {
     constexpr int SPL = NC / LPP;
-    const uint32_t ib_orig = word & CW_IDX_MASK;
+    const uint32_t ib = word & CW_IDX_MASK;
     uint32_t match = 0, empty = 0;
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
         const uint32_t cidx = slot[j] & CW_IDX_MASK;
-        match |= g.bits(cidx == ib_orig) << (LPP * j);
+        match |= g.bits(cidx == ib) << (LPP * j);
         empty |= g.bits(cidx == EMPTY) << (LPP * j);
     }
     int cpos = nc;
     if (match) cpos = __ffs(match) - 1;
     else if (empty) cpos = 31 - __clz((int)empty);
+    else clobber = true;
     if (cpos < NC && (uint32_t)cpos % LPP == g.q) {
 #pragma unroll
         for (int j = 0; j < SPL; ++j)
-            if (j == cpos / LPP) {
-                slot[j] = word;
-                srt[j] = sib;
-            }
+            if (j == cpos / LPP) slot[j] = word;
     }
     ++nc;
     if (nc > NC) {
@@ -1334,6 +1354,14 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
         return true;
     }
     return false;
+}
+
+// OR of a value over the lanes of a group.
+template <int LPP>
+__device__ __forceinline__ uint32_t group_or(uint32_t v) {
+#pragma unroll
+    for (int m = 1; m < LPP; m <<= 1) v |= (uint32_t)__shfl_xor((int)v, m, 64);
+    return v;
 }
 
 // The interaction round's contributions, added on every lane of the group in
@@ -1370,8 +1398,7 @@ __device__ __forceinline__ void replay_round(const LaneGroup<LPP>& g, int take, 
 // Groups of four only (DPP quad broadcasts).
 template <int LPP>
 __device__ __forceinline__ bool group_register_known(const LaneGroup<LPP>& g,
-                                                     uint32_t (&slot)[NC / LPP],
-                                                     uint32_t (&srt)[NC / LPP], uint2 e,
+                                                     uint32_t (&slot)[NC / LPP], uint2 e,
                                                      uint32_t hm, bool cell_done, int& nc,
                                                      uint32_t& last_cell) {
     if constexpr (LPP != 4) {
@@ -1403,23 +1430,16 @@ __device__ __forceinline__ bool group_register_known(const LaneGroup<LPP>& g,
 #pragma unroll
         for (int t = 0; t < 4; ++t)
             if (((cnt >> (8 * t)) & 0xffu) != ((hm >> t) & 1u)) return false;
-        uint32_t w[4], sb[4];
+        uint32_t w[4];
         w[0] = g.template bc<0>(e.y);
         w[1] = g.template bc<1>(e.y);
         w[2] = g.template bc<2>(e.y);
         w[3] = g.template bc<3>(e.y);
-        sb[0] = g.template bc<0>(e.x) & VL_IB_MASK;
-        sb[1] = g.template bc<1>(e.x) & VL_IB_MASK;
-        sb[2] = g.template bc<2>(e.x) & VL_IB_MASK;
-        sb[3] = g.template bc<3>(e.x) & VL_IB_MASK;
 #pragma unroll
         for (int t = 0; t < 4; ++t)
 #pragma unroll
             for (int j = 0; j < SPL; ++j)
-                if ((mm >> (t * SPL + j)) & 1u) {
-                    slot[j] = w[t];
-                    srt[j] = sb[t];
-                }
+                if ((mm >> (t * SPL + j)) & 1u) slot[j] = w[t];
         nc += nh;
         last_cell = g.from(e.x, 31u - (uint32_t)__clz((int)hm)) >> VL_CELL_SHIFT;
         return true;
@@ -1453,35 +1473,38 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     const double Lz = grid_Lz(p, a);
     // The row passes through LDS: read and written as contiguous words per
     // lane, held LPP-strided in registers during the search, and back in
-    // LDS (with the partners' sorted indices) for the interaction.
+    // LDS for the interaction.
     __shared__ uint32_t s_slot[PPB * NC];
-    __shared__ uint32_t s_srt[PPB * NC];
     uint32_t* gs = s_slot + (threadIdx.x / LPP) * NC;
-    uint32_t* gr = s_srt + (threadIdx.x / LPP) * NC;
     uint32_t* const crow_a = a.crow + (size_t)ia_orig * NC;
-    // The list length is read ahead of the row staging, so that the list
-    // entries' loads need not wait for it.
+    // The list length and hit mask are read ahead of the row staging, so
+    // that the list entries' loads need not wait for them.
     const bool rebuild =
         !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
-    const uint32_t vcnt = rebuild ? 0u : a.vcount[ia];
+    const uint2 vmeta = rebuild ? make_uint2(0u, 0u) : a.vcount[ia];
+    const uint32_t vcnt = vmeta.x == VL_OVERFLOW ? VL_OVERFLOW : (vmeta.x & VC_COUNT_MASK);
+    const bool hp_valid = vmeta.x != VL_OVERFLOW && (vmeta.x & VC_HP_VALID) != 0u;
     copy_words<SPL>(gs + g.q * SPL, crow_a + g.q * SPL);
     wave_sync();
+    // Slot registers; after a re-sort the words are mapped to the new order.
     uint32_t slot[SPL];
-    // Sorted index of each slot's partner, where this step's search found
-    // it; NO_SORTED (a contact the search did not meet again) is looked up.
-    uint32_t srt[SPL];
+    const bool remap = lp->order_changed != 0u;
 #pragma unroll
     for (int j = 0; j < SPL; ++j) {
         slot[j] = gs[g.q + LPP * j];
-        srt[j] = NO_SORTED;
+        if (remap) slot[j] = word_to_current(a, slot[j]);
     }
 
     // topology: the candidate sequence of the 27-cell scan, LPP at a time.
     int nc = 0;
     const uint32_t below = (1u << g.q) - 1u;
+    // In the list path: whether the row after this step holds exactly the
+    // partners of the overlapping entries (hp_new, VC_HP_VALID).
+    bool quiet = false;
     if (rebuild || vcnt == VL_OVERFLOW) {
         const bool build = p.vl_enabled && rebuild;
         uint32_t nl = 0;
+        bool clobber = false;
         uint2* vrow = a.vlist + (size_t)ia * NV;
         const int gx = (int)floor((xa.x - p.origo[0]) / (p.L[0] / p.num[0]));
         const int gy = (int)floor((xa.y - p.origo[1]) / (p.L[1] / p.num[1]));
@@ -1522,7 +1545,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                                 const double len = sqrt32(dd);
                                 hit = len - S < 0.0;
                             }
-                            if (hit || near) word = pack_word(a.perm[ib], (uint32_t)code);
+                            if (hit || near) word = pack_word(ib, (uint32_t)code);
                         }
                         if (build) {
                             const uint32_t nm = g.bits(near);
@@ -1535,19 +1558,25 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                         while (hm && !cell_done) {
                             const uint32_t b = __ffs(hm) - 1;
                             hm &= hm - 1u;
-                            cell_done = group_register(g, slot, srt, g.from(word, b),
-                                                       r + b, nc, lp);
+                            cell_done = group_register(g, slot, g.from(word, b), nc, clobber, lp);
                         }
                     }
                 }
         if (build && g.q == 0) {
-            a.vcount[ia] = nl <= (uint32_t)NV ? nl : VL_OVERFLOW;
+            // A new list starts without a hit mask.
+            a.vcount[ia] = make_uint2(nl <= (uint32_t)NV ? nl : VL_OVERFLOW, 0u);
             a.x_build[ia_orig] = xa;
         }
     } else {
-        // Test every listed candidate first (lane q takes entries q, q +
-        // LPP, ...; all their loads are in flight together), then register
-        // the overlapping ones in list (= scan) order.
+        // Every listed candidate is tested first (lane q takes entries q,
+        // q + LPP, ...; all their loads are in flight together).  With a
+        // valid hit mask only the entries that held no contact are tested:
+        // when none of them overlaps, the slot policy would only re-register
+        // each held partner in its own slot with the word it holds, so the
+        // row stays as it is, and a held contact that ended is emptied by
+        // the interaction below exactly as without registration.  Otherwise
+        // the held entries are tested too and the overlapping entries
+        // register in list (= scan) order.
         constexpr int RMAX = NV / LPP;
         const uint2* vrow = a.vlist + (size_t)ia * NV;
         uint2 ev[RMAX];
@@ -1556,10 +1585,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             ev[i] = make_uint2(0u, 0u);
             if (g.q + LPP * i < vcnt) ev[i] = vrow[g.q + LPP * i];
         }
+        const uint32_t held = hp_valid ? (vmeta.y >> (RMAX * g.q)) & ((1u << RMAX) - 1u) : 0u;
         uint32_t hits = 0;  // bit i: entry g.q + LPP i overlaps
 #pragma unroll
         for (int i = 0; i < RMAX; ++i) {
-            if (g.q + LPP * i < vcnt) {
+            if (g.q + LPP * i < vcnt && !((held >> i) & 1u)) {
                 const double4 xb = xs[ev[i].x & VL_IB_MASK];
                 const double rb = xb.w;
                 d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
@@ -1572,31 +1602,61 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 }
             }
         }
-        // Only overlapping candidates register; the cells of the others
-        // cannot matter, as each cell's entries are contiguous.  The
-        // entries are shifted down a round at a time (never indexed).
-        uint32_t last_cell = 0xffffffffu;
-        bool cell_done = false;
-        for (uint32_t r = 0; r < vcnt; r += LPP) {
-            uint32_t hm = g.bits(hits & 1u);
-            const uint2 e = ev[0];
+        quiet = hp_valid && g.bits(hits != 0u) == 0u;
+        if (!quiet) {
 #pragma unroll
-            for (int i = 0; i + 1 < RMAX; ++i) ev[i] = ev[i + 1];
-            hits >>= 1;
-            if (hm && group_register_known<LPP>(g, slot, srt, e, hm, cell_done, nc, last_cell))
-                continue;
-            while (hm) {
-                const uint32_t b = __ffs(hm) - 1;
-                hm &= hm - 1u;
-                const uint32_t celloff = g.from(e.x, b) >> VL_CELL_SHIFT;
-                if (celloff != last_cell) {
-                    last_cell = celloff;
-                    cell_done = false;
+            for (int i = 0; i < RMAX; ++i) {
+                if (g.q + LPP * i < vcnt && ((held >> i) & 1u)) {
+                    const double4 xb = xs[ev[i].x & VL_IB_MASK];
+                    const double rb = xb.w;
+                    d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
+                    xab += shift_of(p, word_code(ev[i].y));
+                    const double dd = dot(xab, xab);
+                    const double S = ra + rb;
+                    if (!(dd >= S * S * 1.0000152587890625)) {
+                        const double len = sqrt32(dd);
+                        if (len - S < 0.0) hits |= 1u << i;
+                    }
                 }
-                if (cell_done) continue;
-                cell_done = group_register(g, slot, srt, g.from(e.y, b),
-                                           g.from(e.x, b) & VL_IB_MASK, nc, lp);
             }
+            const uint32_t hits0 = hits;
+            // Only overlapping candidates register; the cells of the others
+            // cannot matter, as each cell's entries are contiguous.  The
+            // entries are shifted down a round at a time (never indexed).
+            uint32_t last_cell = 0xffffffffu;
+            bool cell_done = false, clobber = false, overflow = false;
+            for (uint32_t r = 0; r < vcnt; r += LPP) {
+                uint32_t hm = g.bits(hits & 1u);
+                const uint2 e = ev[0];
+#pragma unroll
+                for (int i = 0; i + 1 < RMAX; ++i) ev[i] = ev[i + 1];
+                hits >>= 1;
+                if (hm && group_register_known<LPP>(g, slot, e, hm, cell_done, nc, last_cell))
+                    continue;
+                while (hm) {
+                    const uint32_t b = __ffs(hm) - 1;
+                    hm &= hm - 1u;
+                    const uint32_t celloff = g.from(e.x, b) >> VL_CELL_SHIFT;
+                    if (celloff != last_cell) {
+                        last_cell = celloff;
+                        cell_done = false;
+                    }
+                    if (cell_done) {
+                        overflow = true;
+                        continue;
+                    }
+                    cell_done = group_register(g, slot, g.from(e.y, b), nc, clobber, lp);
+                    overflow = overflow || cell_done;
+                }
+            }
+            // The row now holds exactly the overlapping entries' partners
+            // unless a registration found neither its partner nor an empty
+            // slot, or the count passed NC; the interaction keeps exactly
+            // those (it applies the same overlap test).
+            const uint32_t hp_new = group_or<LPP>(hits0 << (RMAX * g.q));
+            const bool exact = g.bits(clobber || overflow) == 0u;
+            if (g.q == 0)
+                a.vcount[ia] = make_uint2(vcnt | (exact ? VC_HP_VALID : 0u), hp_new);
         }
     }
 
@@ -1619,7 +1679,6 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const bool o = (slot[j] & CW_IDX_MASK) != EMPTY;
         occ |= g.bits(o) << (LPP * j);
         gs[g.q + LPP * j] = slot[j];
-        gr[g.q + LPP * j] = srt[j];
         if (!o && zero_empty)
             a.delta_t[(size_t)ia_orig * NC + g.q + LPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
@@ -1630,6 +1689,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     A.es_dot = 0.0;
     A.ev_dot = 0.0;
     A.p = 0.0;
+    bool ended = false;  // a held contact ended (the hit mask goes stale)
     uint32_t rem = occ;
     while (rem) {
         // This lane takes the q-th occupied slot of the round.
@@ -1638,11 +1698,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         const int take = __popc(rem) < LPP ? __popc(rem) : LPP;
         const bool mine = m != 0u && (int)g.q < take;
         const uint32_t sidx = mine ? (uint32_t)(__ffs(m) - 1) : 0u;
-        uint32_t wrd = 0, sib = NO_SORTED;
-        if (mine) {
-            wrd = gs[sidx];
-            sib = gr[sidx];
-        }
+        const uint32_t wrd = mine ? gs[sidx] : 0u;
         Contrib C;
         contrib_init(C);
         C.F = mk3(0.0, 0.0, 0.0);
@@ -1651,12 +1707,11 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         C.ev0 = 0.0;
         C.ev1 = 0.0;
         C.es = 0.0;
-        bool touching = false, cleared = false;
+        bool touching = false;
         if (mine) {
             const size_t mp = (size_t)ia_orig * NC + sidx;
-            const uint32_t ib_orig = wrd & CW_IDX_MASK;
+            const uint32_t rb_s = wrd & CW_IDX_MASK;
             const d3 dm = shift_of(p, word_code(wrd));
-            const uint32_t rb_s = sib != NO_SORTED ? sib : a.rank[ib_orig];
             // A slot's contact has almost always persisted: its partner's
             // velocities and the displacement are loaded with its position.
             const double4 xb = xs[rb_s];
@@ -1686,6 +1741,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 // The contact ended: its slot empties (src/contactsearch.cuh:540-548).
                 gs[sidx] = EMPTY;
                 a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
+                ended = true;
             }
         }
         const uint32_t tm = g.bits(touching);
@@ -1693,9 +1749,15 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         // Drop the round's slots from the remaining set.
         for (int t = 0; t < take; ++t) rem &= rem - 1u;
     }
+    // Without registration the hit mask stays true until a contact ends.
+    if (quiet && g.bits(ended) != 0u && g.q == 0) a.vcount[ia] = make_uint2(vcnt, 0u);
     wave_sync();
     copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
-    if (a.tape_rows) copy_words<SPL>(a.tape_rows + (size_t)ia_orig * NC + g.q * SPL, gs + g.q * SPL);
+    if (a.tape_rows) {
+#pragma unroll
+        for (int k = 0; k < SPL; ++k)
+            a.tape_rows[(size_t)ia_orig * NC + g.q * SPL + k] = word_to_orig(a, gs[g.q * SPL + k]);
+    }
     walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q == 0);
     if constexpr (FUSED) {
         // The particle's force and torque are final (its own row and the
@@ -1825,6 +1887,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     DevLoop* lp = a.loop + p.sim;
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool in = idx - p.base < p.np;
+    // The contact pass has mapped its rows to the current order.
+    if (lp->order_changed) a.perm_rows[idx] = a.perm[idx];
     HashFlags f;
     if (in) {
         const double4 x_new = integrate_particle(p, a, idx, a.force[idx], a.torque[idx], a.x[idx],
@@ -1856,6 +1920,8 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
     const DevParams& p = chunk_params(a, blockIdx.x / (BLOCK / PPB));
     DevLoop* lp = a.loop + p.sim;
     const bool in = idx - p.base < p.np;  // the same for a particle's four lanes
+    // The contact pass has mapped its rows to the current order.
+    if (lp->order_changed && g.q == 0) a.perm_rows[idx] = a.perm[idx];
     HashFlags f;
     if (in) {
         const uint32_t c = g.q < 3 ? g.q : 0u;
@@ -1878,6 +1944,8 @@ __global__ __launch_bounds__(BLOCK) void k_wall_partials(const DevArrays a) {
     const DevParams& p = chunk_params(a, blockIdx.x);
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool in = idx - p.base < p.np;
+    // After the fused pass, whose rows are in the current order.
+    if (a.loop[p.sim].order_changed) a.perm_rows[idx] = a.perm[idx];
     const uint32_t nsum = p.nw == 0 ? 0u : (p.wall_sums ? p.nw : 1u);
     for (uint32_t w = 0; w < nsum; ++w) {
         const double v = in ? a.force_pp[(size_t)w * a.ntot + idx] : 0.0;
@@ -2102,6 +2170,8 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     if (threadIdx.x != 0) return;
     if (do_sort) lp->sorts += 1u;
     lp->resorted_last = do_sort;
+    // The next step's contact pass maps its rows to the new order.
+    lp->order_changed = do_sort;
     if (p.adaptive == 1 && walls) lp->grid_Lz = grid_top;
     if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
     else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
@@ -2266,6 +2336,17 @@ __global__ __launch_bounds__(BLOCK) void k_check_sqrt32(unsigned long long* bad,
 
 }  // namespace
 
+// Contact rows whose words index the order of perm_rows, rewritten with the
+// partners' original indices.
+__global__ __launch_bounds__(BLOCK) void k_rows_to_orig(const uint32_t* __restrict__ rows,
+                                                       const uint32_t* __restrict__ perm_rows,
+                                                       uint32_t* __restrict__ out, uint32_t n) {
+    const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = rows[i], idx = w & CW_IDX_MASK;
+    out[i] = idx == EMPTY ? w : (w & ~CW_IDX_MASK) | perm_rows[idx];
+}
+
 // ---------------------------------------------------------------------------
 // Launch wrappers
 
@@ -2372,6 +2453,14 @@ hipError_t launch_integrate_q(const DevArrays& a, hipStream_t s) {
 hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
                         hipStream_t s) {
     hipLaunchKernelGGL(k_walls, dim3(a.nsim), dim3(BLOCK), 0, s, a, resort, passes, sorted_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_to_orig(const uint32_t* rows, const uint32_t* perm_rows, uint32_t* out,
+                               uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_rows_to_orig, dim3(n_blocks(n)), dim3(BLOCK), 0, s, rows, perm_rows,
+                       out, n);
     return hipGetLastError();
 }
 

@@ -60,6 +60,10 @@ hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_b
                         hipStream_t s);
 // Fill n words with v.
 hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s);
+// n contact-row words indexing the order of perm_rows, rewritten with the
+// partners' original indices into out.
+hipError_t launch_rows_to_orig(const uint32_t* rows, const uint32_t* perm_rows, uint32_t* out,
+                               uint32_t n, hipStream_t s);
 
 }  // namespace sphx
 

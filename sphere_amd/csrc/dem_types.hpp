@@ -14,10 +14,16 @@ constexpr int NC = 32;          // contacts per particle, src/constants.h:21
 constexpr int MAXWALLS = 6;     // src/constants.h:12
 constexpr int BLOCK = 256;      // particles per workgroup in the per-particle kernels
 
-// A contact-row word packs the partner's original index with the periodic
-// image it was found through.  The image shift components are exactly
-// 0.0, +L or -L (src/contactsearch.cuh:17-58 adds/subtracts L to a zero
-// vector), so two bits per periodic axis reproduce them bit for bit.
+// A contact-row word packs the partner's index with the periodic image it
+// was found through.  On the device the index is the partner's position in
+// the order the rows were last written in (the sorted order of the step
+// that wrote them, `perm_rows`): the contact kernel reads its partner's
+// state there without a lookup, and a step that follows a re-sort maps
+// each word to the new order through the original index.  The accessors
+// translate to the reference's original indices.  The image shift
+// components are exactly 0.0, +L or -L (src/contactsearch.cuh:17-58
+// adds/subtracts L to a zero vector), so two bits per periodic axis
+// reproduce them bit for bit.
 constexpr uint32_t CW_IDX_BITS = 28;
 constexpr uint32_t CW_IDX_MASK = (1u << CW_IDX_BITS) - 1u;
 constexpr uint32_t CW_SHIFT_X = 28;   // 0: none, 1: +L[0], 2: -L[0]
@@ -27,12 +33,17 @@ constexpr uint32_t MAX_NP = 1u << 27; // the reference's own u32 row index limit
 // Candidate (Verlet) lists: per particle, in sorted order, the neighbours
 // within r_a + r_b + skin found by the 27-cell scan, kept in scan order.
 // An entry is {sorted index | cell offset << 27, packed contact word}.
+// Per particle the list keeps {count | flags, hit mask}: the hit mask has
+// bit RMAX q + i set when entry q + LPP i (the entry lane q of a group of
+// LPP lanes tests in its round i, RMAX = NV / LPP) held a contact after the
+// last step, and VC_HP_VALID says that the row then held exactly those
+// partners.
 constexpr int NV = 32;                   // entries per particle
 constexpr uint32_t VL_OVERFLOW = 0xffffffffu;
 constexpr uint32_t VL_IB_MASK = (1u << 27) - 1u;
 constexpr uint32_t VL_CELL_SHIFT = 27;
-// A contact slot whose partner's sorted index the step has not seen.
-constexpr uint32_t NO_SORTED = 0xffffffffu;
+constexpr uint32_t VC_COUNT_MASK = 0xffffu;
+constexpr uint32_t VC_HP_VALID = 1u << 16;
 
 // Particles that changed cell, listed by the integration pass for the
 // one-workgroup re-sort at the end of the step.
@@ -123,7 +134,7 @@ struct DevLoop {
     uint32_t sorts;             // number of re-sorts so far (diagnostic)
     uint32_t nmovers;           // particles whose key changed for the next step
     uint32_t resorted_last;     // the last step ended with the next step's re-sort
-    uint32_t pad1;
+    uint32_t order_changed;     // the order changed since the contact rows were written
 };
 
 // Batch-wide state.

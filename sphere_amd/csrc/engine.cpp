@@ -511,6 +511,7 @@ int allocate(sph_ctx* c) {
     ALLOC(A.skey_bk, np); ALLOC(A.perm_bk, np);
     ALLOC(A.cs_bk, nkeys); ALLOC(A.ce_bk, nkeys);
     ALLOC(A.crow, np * NC);
+    ALLOC(A.perm_rows, np);
     ALLOC(A.delta_t, np * NC);
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
     ALLOC(A.force_pp, (size_t)MAXWALLS * np);
@@ -565,7 +566,7 @@ int upload_state(sph_ctx* c) {
     double* e[5] = {A.es_dot, A.es, A.ev_dot, A.ev, A.p};
     for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemsetAsync(e[k], 0, N * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.delta_t, 0, N * NC * sizeof(double4), s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.vcount, 0, N * 4, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.vcount, 0, N * 8, s), "memset");
     for (int k = 0; k < 2; ++k) {
         HIPCHK(c, hipMemsetAsync(A.xs[k], 0, N * 32, s), "memset");
         HIPCHK(c, hipMemsetAsync(A.vs[k], 0, N * 32, s), "memset");
@@ -584,6 +585,7 @@ int upload_state(sph_ctx* c) {
     for (size_t i = 0; i < N; ++i) ident[i] = (uint32_t)i;
     H2D(A.perm, ident.data(), N * 4);
     H2D(A.rank, ident.data(), N * 4);
+    H2D(A.perm_rows, ident.data(), N * 4);
     H2D(A.key_prev, ff.data(), N * 4);
     std::vector<uint32_t> cs(A.nkeys, 0xffffffffu);
     H2D(A.cell_start, cs.data(), (size_t)A.nkeys * 4);
@@ -917,13 +919,10 @@ int enqueue_step(sph_ctx* c) {
         pr.begin("contact_integrate");
         HIPCHK(c, launch_contact_integrate(A, (int)c->contactmodel, s), "contact");
         pr.end();
-        bool walls = false;
-        for (const Sim& sim : c->sims) walls = walls || sim.h.nw > 0;
-        if (walls) {
-            pr.begin("wall_partials");
-            HIPCHK(c, launch_wall_partials(A, s), "partials");
-            pr.end();
-        }
+        // Also after a re-sort: the order the fused pass left its rows in.
+        pr.begin("wall_partials");
+        HIPCHK(c, launch_wall_partials(A, s), "partials");
+        pr.end();
     } else {
         pr.begin("contact");
         HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
@@ -1322,9 +1321,49 @@ int sph_write_bin(sph_ctx* c, const char* path) { return sph_write_bin_sim(c, 0,
 //   codes 0: none, 1: +L, 2: -L) | f64 delta_t[np*NC*3] | f64 acc[np*3] |
 //   f64 angacc[np*3] | f64 walls_acc[nw]
 namespace {
+// A simulation's contact rows with the partners' original (batch) indices.
+// The device words index the order of perm_rows (the sorted order of the
+// step that wrote them).
+int rows_get(sph_ctx* c, int sim, std::vector<uint32_t>& rows) {
+    const Sim& S = c->sims[sim];
+    const size_t np = S.h.np, b0 = S.P.base, npad = S.P.npad;
+    rows.resize(np * NC);
+    std::vector<uint32_t> pr(npad);
+    int rc = d2h(c, rows.data(), c->A.crow + b0 * NC, np * NC * 4);
+    if (!rc) rc = d2h(c, pr.data(), c->A.perm_rows + b0, npad * 4);
+    if (rc) return rc;
+    for (uint32_t& w : rows) {
+        const uint32_t idx = w & CW_IDX_MASK;
+        if (idx != EMPTY && idx - b0 < npad) w = (w & ~CW_IDX_MASK) | pr[idx - b0];
+    }
+    return SPH_OK;
+}
+
+int set_loop_flag(sph_ctx* c, int sim, uint32_t DevLoop::*flag);
+
+// Writes rows given with original (batch) indices, in the order of
+// perm_rows, and has the next step rebuild the candidate lists (their hit
+// masks describe the rows they saw).
+int rows_put(sph_ctx* c, int sim, std::vector<uint32_t>& rows) {
+    const Sim& S = c->sims[sim];
+    const size_t np = S.h.np, b0 = S.P.base, npad = S.P.npad;
+    std::vector<uint32_t> pr(npad), inv(npad);
+    int rc = d2h(c, pr.data(), c->A.perm_rows + b0, npad * 4);
+    if (rc) return rc;
+    for (size_t k = 0; k < npad; ++k) inv[pr[k] - b0] = (uint32_t)(b0 + k);
+    for (uint32_t& w : rows) {
+        const uint32_t idx = w & CW_IDX_MASK;
+        if (idx != EMPTY && idx - b0 < npad) w = (w & ~CW_IDX_MASK) | inv[idx - b0];
+    }
+    rc = h2d(c, c->A.crow + b0 * NC, rows.data(), np * NC * 4);
+    if (!rc) rc = set_loop_flag(c, sim, &DevLoop::force_rebuild);
+    return rc;
+}
+}  // namespace
+
+namespace {
 constexpr char STATE_MAGIC[8] = {'S', 'P', 'H', 'S', 'T', 'A', 'T', 'E'};
 constexpr uint32_t STATE_VERSION = 1;
-int set_loop_flag(sph_ctx* c, int sim, uint32_t DevLoop::*flag);
 }  // namespace
 
 int sph_write_state_sim(sph_ctx* c, int sim, const char* path) {
@@ -1334,9 +1373,9 @@ int sph_write_state_sim(sph_ctx* c, int sim, const char* path) {
     const size_t np = S.h.np, nw = S.h.nw, b0 = S.P.base;
     DevLoop L;
     int rc = read_loop(c, sim, &L);
-    std::vector<uint32_t> rows(np * NC);
+    std::vector<uint32_t> rows;
     std::vector<double> dt(np * NC * 3), acc(np * 3), angacc(np * 3), wacc(nw);
-    if (!rc) rc = d2h(c, rows.data(), c->A.crow + b0 * NC, np * NC * 4);
+    if (!rc) rc = rows_get(c, sim, rows);
     if (!rc) rc = sph_get_field_sim(c, sim, SPH_F_DELTA_T, dt.data(), dt.size() * 8);
     if (!rc) rc = sph_get_field_sim(c, sim, SPH_F_ACC, acc.data(), acc.size() * 8);
     if (!rc) rc = sph_get_field_sim(c, sim, SPH_F_ANGACC, angacc.data(), angacc.size() * 8);
@@ -1425,7 +1464,7 @@ int sph_read_state_sim(sph_ctx* c, int sim, const char* path) {
     memcpy(angacc.data(), q, angacc.size() * 8);
     q += angacc.size() * 8;
     memcpy(wacc.data(), q, nw * 8);
-    int rc = h2d(c, c->A.crow + b0 * NC, rows.data(), rows.size() * 4);
+    int rc = rows_put(c, sim, rows);
     if (!rc) rc = sph_set_field_sim(c, sim, SPH_F_DELTA_T, dt.data(), dt.size() * 8);
     if (!rc) rc = sph_set_field_sim(c, sim, SPH_F_ACC, acc.data(), acc.size() * 8);
     if (!rc) rc = sph_set_field_sim(c, sim, SPH_F_ANGACC, angacc.data(), angacc.size() * 8);
@@ -1537,11 +1576,12 @@ int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
         case SPH_F_P: return d2h(c, dst, A.p + b0, bytes);
         case SPH_F_CONTACTS: {
             // The reference's row format: partner's .bin index, np when empty.
-            int rc = d2h(c, dst, A.crow + b0 * NC, bytes);
+            std::vector<uint32_t> rows;
+            int rc = rows_get(c, sim, rows);
             if (rc) return rc;
             uint32_t* w = static_cast<uint32_t*>(dst);
             for (size_t i = 0; i < np * NC; ++i) {
-                const uint32_t idx = w[i] & CW_IDX_MASK;
+                const uint32_t idx = rows[i] & CW_IDX_MASK;
                 w[i] = idx == EMPTY ? (uint32_t)np : idx - (uint32_t)b0;
             }
             return SPH_OK;
@@ -1550,9 +1590,9 @@ int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
         case SPH_F_DISTMOD: {
             // Reconstructed from the packed image code and the partner's
             // radius; meaningful where the slot is occupied.
-            std::vector<uint32_t> rows(np * NC);
+            std::vector<uint32_t> rows;
             std::vector<double> x(np * 4);
-            int rc = d2h(c, rows.data(), A.crow + b0 * NC, np * NC * 4);
+            int rc = rows_get(c, sim, rows);
             if (!rc) rc = d2h(c, x.data(), A.x + b0, np * 32);
             if (rc) return rc;
             double* d = static_cast<double*>(dst);
@@ -1667,8 +1707,8 @@ int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes)
         case SPH_F_P: return h2d(c, A.p + b0, src, bytes);
         case SPH_F_CONTACTS: {
             // Keep each slot's image code; replace the partner index.
-            std::vector<uint32_t> rows(np * NC);
-            int rc = d2h(c, rows.data(), A.crow + b0 * NC, bytes);
+            std::vector<uint32_t> rows;
+            int rc = rows_get(c, sim, rows);
             if (rc) return rc;
             const uint32_t* s = static_cast<const uint32_t*>(src);
             for (size_t k = 0; k < np * NC; ++k) {
@@ -1676,7 +1716,7 @@ int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes)
                 const uint32_t idx = s[k] == np ? EMPTY : s[k] + (uint32_t)b0;
                 rows[k] = (rows[k] & ~CW_IDX_MASK) | idx;
             }
-            rc = h2d(c, A.crow + b0 * NC, rows.data(), bytes);
+            rc = rows_put(c, sim, rows);
             if (rc) return rc;
             return set_loop_flag(c, sim, &DevLoop::zero_empty);
         }
@@ -1686,8 +1726,8 @@ int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes)
             return set_loop_flag(c, sim, &DevLoop::zero_empty);
         }
         case SPH_F_DISTMOD: {
-            std::vector<uint32_t> rows(np * NC);
-            int rc = d2h(c, rows.data(), A.crow + b0 * NC, np * NC * 4);
+            std::vector<uint32_t> rows;
+            int rc = rows_get(c, sim, rows);
             if (rc) return rc;
             const double* d = static_cast<const double*>(src);
             for (size_t k = 0; k < np * NC; ++k) {
@@ -1702,7 +1742,7 @@ int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes)
                 if (d[4 * k + 2] != 0.0) return fail(c, SPH_E_ARG, "distmod z must be 0");
                 rows[k] = (rows[k] & CW_IDX_MASK) | code;
             }
-            return h2d(c, A.crow + b0 * NC, rows.data(), np * NC * 4);
+            return rows_put(c, sim, rows);
         }
         case SPH_F_WALLS_NX: {
             int rc = h2d(c, A.w_nx + w0, src, bytes);
@@ -1965,7 +2005,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
                            up(4 * nb * sizeof(double4)) + up(6 * nb * sizeof(double4)) +
                            up(nb * ADJ_NBPARAM * 8) + up(nwg * ADJ_NPART * 8) +
                            up(nsim * ADJ_NPARAM * 8) + up(nwt * ADJ_WALL * 8 + 8) +
-                           up(rows * 3 * sizeof(double)) + 256;
+                           up(rows * 3 * sizeof(double)) + up(nc * 4) + 256;
     // A checkpoint: every device allocation of the context.
     size_t snap_b = 0;
     for (size_t b : c->alloc_bytes) snap_b += up(b);
@@ -2085,6 +2125,8 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     g.wall = reinterpret_cast<double*>(take(nwt * ADJ_WALL * 8 + 8));
     g.asym = reinterpret_cast<uint32_t*>(g.wall + nwt * ADJ_WALL);
     double* stage = reinterpret_cast<double*>(take(rows * 3 * sizeof(double)));
+    // A checkpoint's rows with the partners' original indices (the tape's).
+    uint32_t* prev_orig = reinterpret_cast<uint32_t*>(take(nc * 4));
     char* snaps = tape + K * per_step + state_b;  // (nseg + 1) checkpoints when nseg > 1
     AdjStep base_st;
     memset(&base_st, 0, sizeof base_st);
@@ -2126,9 +2168,11 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
         bool hash_pending, prev_walls_resorted;
     };
     std::vector<HostMark> marks(nseg > 1 ? nseg + 1 : 0);
-    size_t crow_off = 0;
-    for (size_t k = 0, off = 0; k < c->allocs.size(); off += up(c->alloc_bytes[k]), ++k)
+    size_t crow_off = 0, prow_off = 0;
+    for (size_t k = 0, off = 0; k < c->allocs.size(); off += up(c->alloc_bytes[k]), ++k) {
         if (c->allocs[k] == (void*)A.crow) crow_off = off;
+        if (c->allocs[k] == (void*)A.perm_rows) prow_off = off;
+    }
     auto checkpoint = [&](uint64_t m, bool save) -> int {
         char* cbase = snaps + m * snap_b;
         size_t off = 0;
@@ -2257,9 +2301,17 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
             const uint64_t n = std::min<uint64_t>(K, nsteps - m * K);
             rc = checkpoint(m, false);
             if (!rc) rc = record(n);
-            // The rows before step m K: those the checkpoint's contact rows hold.
-            const uint32_t* prev0 =
-                m > 0 ? reinterpret_cast<const uint32_t*>(snaps + m * snap_b + crow_off) : nullptr;
+            // The rows before step m K: those the checkpoint's contact rows
+            // hold, with original indices.
+            const uint32_t* prev0 = nullptr;
+            if (m > 0 && !rc) {
+                const char* ck = snaps + m * snap_b;
+                e = launch_rows_to_orig(reinterpret_cast<const uint32_t*>(ck + crow_off),
+                                        reinterpret_cast<const uint32_t*>(ck + prow_off),
+                                        prev_orig, (uint32_t)nc, s);
+                if (e != hipSuccess) return hip_fail(c, e, "checkpoint rows");
+                prev0 = prev_orig;
+            }
             if (!rc) rc = reverse(m * K, n, prev0);
         }
         if (!rc) rc = checkpoint(nseg, false);
