@@ -428,6 +428,19 @@ __device__ __forceinline__ double contact_linear_wall(const DevParams& p, Acc& A
     return dot(f_n, n);
 }
 
+// The factor -kappa sqrt32(r_a r_b) of the contact laws' capillary term
+// f_c = -kappa sqrt32(r_a r_b) n.  Without cohesion (kappa == 0) and with
+// r_a r_b > 0 finite in FP32 the factor is a zero of the sign of -kappa
+// whatever the root is, so the root is skipped (`zero` set); then every
+// component of f_n - f_c equals f_n's up to the sign of a zero, which the
+// squares in length() drop: |f_n - f_c| = |f_n|.
+__device__ __forceinline__ double capillary_factor(double kappa, double ra, double rb,
+                                                   bool& zero) {
+    const double rr = ra * rb;
+    zero = kappa == 0.0 && rr > 0.0 && rr <= 3.4028234663852886e38;
+    return -kappa * (zero ? 1.0 : sqrt32(rr));
+}
+
 // contactLinear (model 2), src/contactmodels.cuh:244-411.  Returns the new
 // tangential displacement (stored by the caller).
 __device__ __forceinline__ d3 contact_linear(const DevParams& p, Contrib& C, const double4 vel_a,
@@ -443,7 +456,9 @@ __device__ __forceinline__ d3 contact_linear(const DevParams& p, Contrib& C, con
     const d3 vel_t = vel - n * dot(n, vel);
     const double vel_t_length = length(vel_t);
     const d3 delta_t0 = delta_t0_uncor - (n * dot(n, delta_t0_uncor));
-    const double delta_t0_length = length(delta_t0);
+    // length(delta_t0) > 0 exactly when the FP32 square is (the root of a
+    // positive FP32 value is positive, of zero zero, of NaN NaN).
+    const bool delta_t0_nonzero = (float)dot(delta_t0, delta_t0) > 0.0f;
     d3 delta_t = delta_t0;  // the value stored when neither branch assigns it
     double k_n = p.k_n, k_t = p.k_t;
     if (p.E > .001) {
@@ -453,20 +468,22 @@ __device__ __forceinline__ d3 contact_linear(const DevParams& p, Contrib& C, con
     const d3 f_n = fmax(0.0, -k_n * delta + p.gamma_n * vel_n) * n;
     const double f_n_length = length(f_n);
     add_ev(C, 0.5 * p.gamma_n * vel_n * vel_n);
-    const d3 f_c = -p.kappa * sqrt32(ra * rb) * n;
+    bool fc_zero;
+    const d3 f_c = capillary_factor(p.kappa, ra, rb, fc_zero) * n;
     d3 f_t = mk3(0.0, 0.0, 0.0);
-    if (delta_t0_length > 0.0 || vel_t_length > 0.0) {
+    if (delta_t0_nonzero || vel_t_length > 0.0) {
         delta_t = delta_t0 + vel_t * p.dt;
         const d3 f_t_elast = -k_t * delta_t;
         const d3 f_t_visc = -p.gamma_t * vel_t;
         f_t = f_t_elast + f_t_visc;
         const double f_t_length = length(f_t);
-        const double f_t_limit = p.mu_s * length(f_n - f_c);
+        const double fnc_length = fc_zero ? f_n_length : length(f_n - f_c);
+        const double f_t_limit = p.mu_s * fnc_length;
         add_ev(C, 0.5 * p.gamma_t * vel_t_length * vel_t_length);
         if (f_t_length > f_t_limit) {
-            const d3 t = f_t / length(f_t);
+            const d3 t = f_t / f_t_length;
             f_t = f_t_limit * t;
-            delta_t = -1.0 / k_t * (p.mu_d * length(f_n - f_c) * t + p.gamma_t * vel_t);
+            delta_t = -1.0 / k_t * (p.mu_d * fnc_length * t + p.gamma_t * vel_t);
             add_es(C, 0.5 * length(length(f_t) * vel_t * p.dt) / p.dt);
         }
     }
@@ -494,7 +511,7 @@ __device__ __forceinline__ d3 contact_hertz(const DevParams& p, Contrib& C, cons
     const d3 vel_t_ab = vel_ab - (n_ab * dot(vel_ab, n_ab));
     const double vel_t_ab_length = length(vel_t_ab);
     const d3 delta_t0 = delta_t0_uncor - (n_ab * dot(delta_t0_uncor, n_ab));
-    const double delta_t0_length = length(delta_t0);
+    const bool delta_t0_nonzero = (float)dot(delta_t0, delta_t0) > 0.0f;  // length > 0
     d3 delta_t = delta_t0;
     double k_n = p.k_n, k_t = p.k_t;
     if (p.E > .001) {
@@ -509,17 +526,19 @@ __device__ __forceinline__ d3 contact_hertz(const DevParams& p, Contrib& C, cons
     add_ev(C, p.gamma_n * vel_n_ab * vel_n_ab);
     if (dot(f_n, n_ab) < 0.0) f_n = mk3(0.0, 0.0, 0.0);
     const double f_n_length = length(f_n);
-    const d3 f_c = -p.kappa * sqrt32(ra * rb) * n_ab;
+    bool fc_zero;
+    const d3 f_c = capillary_factor(p.kappa, ra, rb, fc_zero) * n_ab;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     d3 T_res = mk3(0.0, 0.0, 0.0);
-    if (delta_t0_length > 0.0 || vel_t_ab_length > 0.0) {
+    if (delta_t0_nonzero || vel_t_ab_length > 0.0) {
         const d3 f_t_elast = -k_t * p12 * delta_t0;
         const d3 f_t_visc = -p.gamma_t * p14 * vel_t_ab;
+        const double fnc_length = fc_zero ? f_n_length : length(f_n - f_c);
         double f_t_limit;
         if (vel_t_ab_length > (double)0.001f)
-            f_t_limit = p.mu_d * length(f_n - f_c);
+            f_t_limit = p.mu_d * fnc_length;
         else
-            f_t_limit = p.mu_s * length(f_n - f_c);
+            f_t_limit = p.mu_s * fnc_length;
         f_t = f_t_elast + f_t_visc;
         const double f_t_length = length(f_t);
         if (f_t_length > f_t_limit) {
@@ -559,15 +578,17 @@ __device__ __forceinline__ void contact_linear_viscous(const DevParams& p, Acc& 
     d3 f_n = fmax(0.0, -k_n * delta - p.gamma_n * vel_n_ab) * n_ab;
     if (dot(f_n, n_ab) < 0.0) f_n = mk3(0.0, 0.0, 0.0);
     const double f_n_length = length(f_n);
-    const d3 f_c = -kappa * sqrt32(ra * rb) * n_ab;
+    bool fc_zero;
+    const d3 f_c = capillary_factor(kappa, ra, rb, fc_zero) * n_ab;
     d3 f_t = mk3(0.0, 0.0, 0.0);
     if (vel_t_ab_length > 0.0) {
         const double f_t_visc = p.gamma_t * vel_t_ab_length;
+        const double fnc_length = fc_zero ? f_n_length : length(f_n - f_c);
         double f_t_limit;
         if (vel_t_ab_length > (double)0.001f)
-            f_t_limit = p.mu_d * length(f_n - f_c);
+            f_t_limit = p.mu_d * fnc_length;
         else
-            f_t_limit = p.mu_s * length(f_n - f_c);
+            f_t_limit = p.mu_s * fnc_length;
         if (f_t_visc < f_t_limit)
             f_t = -f_t_visc * vel_t_ab / vel_t_ab_length;
         else
