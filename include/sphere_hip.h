@@ -332,6 +332,19 @@ int sph_find_overlaps(sph_ctx* ctx, int sim, uint32_t* pairs, double* overlaps, 
                       uint64_t* count);
 int sph_porosity(sph_ctx* ctx, int sim, int z_slices, double* z_pos, double* porosity);
 
+/* No counterpart in the reference.  sph_porosity for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the per-slice results are staged
+ * in a fixed device array of 2 x SPH_POROSITY_NOALLOC_MAX_SLICES doubles
+ * (1 MiB) that the library holds on each GPU it runs on until the process
+ * ends, not in a buffer allocated per call.  Same results and errors as
+ * sph_porosity; SPH_E_ARG when z_slices exceeds the maximum.  Every call on
+ * one GPU uses that GPU's array: calls that overlap on one GPU (from
+ * several host threads, or contexts on different streams) race. */
+#define SPH_POROSITY_NOALLOC_MAX_SLICES 65536
+int sph_porosity_noalloc(sph_ctx* ctx, int sim, int z_slices, double* z_pos,
+                         double* porosity);
+
 /* Diagnostic, no reference counterpart: runs the engine's FP32 square root
  * (the reference's sqrtf in length(), src/vector_arithmetic.h) on device
  * `device` for every FP32 bit pattern in [lo, hi) (hi <= 2^32) and counts

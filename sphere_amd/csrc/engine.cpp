@@ -27,6 +27,14 @@
 #include "dem_kernels.hpp"
 #include "dem_types.hpp"
 
+#include <mutex>
+
+// The staging array of sph_porosity_noalloc: z positions then porosities of
+// up to SPH_POROSITY_NOALLOC_MAX_SLICES slices, one copy on every GPU the
+// library runs on, held until the process ends.  Host code uses only the
+// device address that hipGetSymbolAddress returns for it (per device).
+static __device__ __align__(256) double scratch[2 * SPH_POROSITY_NOALLOC_MAX_SLICES];
+
 using namespace sphx;
 
 namespace {
@@ -680,6 +688,29 @@ int upload_state(sph_ctx* c) {
 
 // readbin + checkValues for every image, then one set of device arrays for
 // all of them.
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (at the first context opened on it).
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[SCRATCH_MAX_DEVICES] = {};
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return hipSuccess;
+    double* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = ptr;
+    return hipSuccess;
+}
+
+double* scratch_of(int device) {
+    if (device < 0 || device >= SCRATCH_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    return g_scratch_dev[device];
+}
+
 int open_images(const std::vector<std::pair<const void*, size_t>>& images, int device,
                 unsigned flags, sph_ctx** out) {
     *out = nullptr;
@@ -737,6 +768,12 @@ int open_images(const std::vector<std::pair<const void*, size_t>>& images, int d
         }
     }
     (void)hipGetDevice(&c->device);
+    e = scratch_setup(c->device);
+    if (e != hipSuccess) {
+        g_open_error = std::string("hipGetSymbolAddress: ") + hipGetErrorString(e);
+        delete c;
+        return SPH_E_DEVICE;
+    }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         g_open_error = std::string("hipStreamCreate: ") + hipGetErrorString(e);
@@ -2642,6 +2679,46 @@ int sph_porosity(sph_ctx* c, int sim, int z_slices, double* z_pos, double* poros
                                             hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "porosity");
+    return SPH_OK;
+}
+
+int sph_porosity_noalloc(sph_ctx* c, int sim, int z_slices, double* z_pos, double* porosity) {
+    CHECK_SIM(c, sim);
+    if (z_slices <= 0 || !z_pos || !porosity) return SPH_E_ARG;
+    if (z_slices > SPH_POROSITY_NOALLOC_MAX_SLICES)
+        return fail(c, SPH_E_ARG, "z_slices %d exceeds the maximum of %d for "
+                    "sph_porosity_noalloc", z_slices, SPH_POROSITY_NOALLOC_MAX_SLICES);
+    double* d = scratch_of(c->device);
+    if (!d) return fail(c, SPH_E_DEVICE, "no staging array on device %d", c->device);
+    const Sim& S = c->sims[sim];
+    const uint32_t np = S.h.np, b0 = S.P.base;
+    double top = S.h.L[2];
+    if (S.h.nw > 0) {
+        double4 w;
+        int rc = d2h(c, &w, c->A.w_nx + S.P.wall_base, 32);
+        if (rc) return rc;
+        top = w.w;
+    }
+    const double h_slice = (top - S.h.origo[2]) / (double)z_slices;
+    double4 last = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (np > 0) {
+        int rc = d2h(c, &last, c->A.x + b0 + np - 1, 32);
+        if (rc) return rc;
+    }
+    // src/sphere.cpp:474-484, as in sph_porosity.
+    if (h_slice <= last.w * 2.0)
+        return fail(c, SPH_E_INPUT, "Error! The number of z-slices is too high.");
+    const double V_slice = h_slice * S.h.L[0] * S.h.L[1];
+    // Every slice's two results are written by the kernel: nothing in the
+    // array needs clearing first.
+    hipError_t e = launch_porosity(c->A.x + b0, np, z_slices, h_slice, V_slice, d, d + z_slices,
+                                   c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(z_pos, d, (size_t)z_slices * 8,
+                                            hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(porosity, d + z_slices, (size_t)z_slices * 8,
+                                            hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return hip_fail(c, e, "porosity");
     return SPH_OK;
 }

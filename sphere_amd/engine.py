@@ -78,7 +78,7 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_set_scalar_sim", "sph_adjoint", "sph_adjoint_samples", "sph_adjoint_batch",
            "sph_adjoint_samples_batch", "sph_selftest_sqrt32",
            "sph_write_state", "sph_read_state", "sph_write_state_sim", "sph_read_state_sim",
-           "sph_find_overlaps", "sph_porosity", "sph_autotune")
+           "sph_find_overlaps", "sph_porosity", "sph_autotune", "sph_porosity_noalloc")
 
 # Parameters of sph_adjoint's g_params, in order.
 ADJOINT_PARAMS = ("k_n", "k_t", "gamma_n", "gamma_t", "mu_s", "mu_d", "gamma_wn", "gamma_wt",
@@ -137,6 +137,7 @@ def load():
     L.sph_find_overlaps.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.c_uint64,
                                     ctypes.POINTER(ctypes.c_uint64)]
     L.sph_porosity.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
+    L.sph_porosity_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp]
     L.sph_write_state.argtypes = [vp, ctypes.c_char_p]
     L.sph_read_state.argtypes = [vp, ctypes.c_char_p]
     L.sph_write_state_sim.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
