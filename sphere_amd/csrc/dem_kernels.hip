@@ -1377,6 +1377,49 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
     return false;
 }
 
+// Overlap tests of a lane's list entries (entry g.q + LPP i in ev[i]) whose
+// bit is set in `want`, in chunks of TCHUNK: each chunk's partner positions
+// are loaded together (an entry not tested loads the particle's own
+// position) before any of them is used, so that a chunk costs one memory
+// round trip.  The second chunk runs only when some lane of the wavefront
+// lists entries there (`tail`).  Returns bit i set where entry i overlaps:
+// the same test as the full scan's, len32 - (r_a + r_b) < 0 after the
+// conservative reject dd >= S^2 (1 + 2^-16).
+constexpr int TCHUNK = 4;
+template <int LPP>
+__device__ __forceinline__ uint32_t test_entries(const DevParams& p,
+                                                 const double4* __restrict__ xs,
+                                                 const double4 xa, uint32_t ia,
+                                                 const uint2 (&ev)[NV / LPP], uint32_t want,
+                                                 bool tail) {
+    constexpr int RMAX = NV / LPP;
+    uint32_t hits = 0;
+#pragma unroll
+    for (int c = 0; c < RMAX; c += TCHUNK) {
+        if (c > 0 && !tail) break;
+        double4 xb[TCHUNK];
+#pragma unroll
+        for (int k = 0; k < TCHUNK; ++k)
+            if (c + k < RMAX)
+                xb[k] = xs[((want >> (c + k)) & 1u) ? (ev[c + k].x & VL_IB_MASK) : ia];
+#pragma unroll
+        for (int k = 0; k < TCHUNK; ++k) {
+            if (c + k >= RMAX) continue;
+            const int i = c + k;
+            const double rb = xb[k].w;
+            d3 xab = mk3(xa.x - xb[k].x, xa.y - xb[k].y, xa.z - xb[k].z);
+            xab += shift_of(p, word_code(ev[i].y));
+            const double dd = dot(xab, xab);
+            const double S = xa.w + rb;
+            if (((want >> i) & 1u) && !(dd >= S * S * 1.0000152587890625)) {
+                const double len = sqrt32(dd);
+                if (len - S < 0.0) hits |= 1u << i;
+            }
+        }
+    }
+    return hits;
+}
+
 // OR of a value over the lanes of a group.
 template <int LPP>
 __device__ __forceinline__ uint32_t group_or(uint32_t v) {
@@ -1607,39 +1650,16 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             if (g.q + LPP * i < vcnt) ev[i] = vrow[g.q + LPP * i];
         }
         const uint32_t held = hp_valid ? (vmeta.y >> (RMAX * g.q)) & ((1u << RMAX) - 1u) : 0u;
-        uint32_t hits = 0;  // bit i: entry g.q + LPP i overlaps
-#pragma unroll
-        for (int i = 0; i < RMAX; ++i) {
-            if (g.q + LPP * i < vcnt && !((held >> i) & 1u)) {
-                const double4 xb = xs[ev[i].x & VL_IB_MASK];
-                const double rb = xb.w;
-                d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
-                xab += shift_of(p, word_code(ev[i].y));
-                const double dd = dot(xab, xab);
-                const double S = ra + rb;
-                if (!(dd >= S * S * 1.0000152587890625)) {
-                    const double len = sqrt32(dd);
-                    if (len - S < 0.0) hits |= 1u << i;
-                }
-            }
-        }
+        // The lane's listed entries as a bit mask, and whether any lane of
+        // the wavefront has entries beyond the first chunk of tests.
+        const uint32_t listed =
+            vcnt > g.q ? ((1u << ((vcnt - g.q + LPP - 1) / LPP)) - 1u) & ((1u << RMAX) - 1u) : 0u;
+        const bool tail = __ballot(listed >> TCHUNK) != 0ull;
+        // Entries not held: their tests decide whether registration is needed.
+        uint32_t hits = test_entries<LPP>(p, xs, xa, ia, ev, listed & ~held, tail);
         quiet = hp_valid && g.bits(hits != 0u) == 0u;
         if (!quiet) {
-#pragma unroll
-            for (int i = 0; i < RMAX; ++i) {
-                if (g.q + LPP * i < vcnt && ((held >> i) & 1u)) {
-                    const double4 xb = xs[ev[i].x & VL_IB_MASK];
-                    const double rb = xb.w;
-                    d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
-                    xab += shift_of(p, word_code(ev[i].y));
-                    const double dd = dot(xab, xab);
-                    const double S = ra + rb;
-                    if (!(dd >= S * S * 1.0000152587890625)) {
-                        const double len = sqrt32(dd);
-                        if (len - S < 0.0) hits |= 1u << i;
-                    }
-                }
-            }
+            hits |= test_entries<LPP>(p, xs, xa, ia, ev, listed & held, tail);
             const uint32_t hits0 = hits;
             // Only overlapping candidates register; the cells of the others
             // cannot matter, as each cell's entries are contiguous.  The
@@ -1770,10 +1790,12 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         // Drop the round's slots from the remaining set.
         for (int t = 0; t < take; ++t) rem &= rem - 1u;
     }
-    // Without registration the hit mask stays true until a contact ends.
-    if (quiet && g.bits(ended) != 0u && g.q == 0) a.vcount[ia] = make_uint2(vcnt, 0u);
+    // Without registration the hit mask stays true until a contact ends,
+    // and the row changes only where a contact ended or the order changed.
+    const bool any_ended = g.bits(ended) != 0u;
+    if (quiet && any_ended && g.q == 0) a.vcount[ia] = make_uint2(vcnt, 0u);
     wave_sync();
-    copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
+    if (!quiet || any_ended || remap) copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
     if (a.tape_rows) {
 #pragma unroll
         for (int k = 0; k < SPL; ++k)
