@@ -15,6 +15,12 @@ namespace sphx {
 // All simulations of a context share these arrays: simulation s owns the
 // particle range [ptab[s].base, ptab[s].base + ptab[s].npad), which is a
 // whole number of 256-particle chunks, its walls, bonds and cells.
+// Parts of a particle's output record: force and es, torque and ev,
+// es_dot, ev_dot, p and its force on wall 0, its forces on walls 1-4.
+constexpr uint32_t OUT_F = 0, OUT_T = 1, OUT_E = 2, OUT_W = 3;
+// The walls with a force per particle (top, two x walls, two y walls).
+constexpr uint32_t OUT_NWALL = 5;
+
 struct DevArrays {
     // batch layout
     const DevParams* ptab;      // per-simulation parameters
@@ -36,13 +42,10 @@ struct DevArrays {
     double4* xyzsum;
     double4* acc;      // previous-step acceleration (TY3 acc0)
     double4* angacc;
-    double4* force;
-    double4* torque;
-    double* es_dot;
-    double* es;
-    double* ev_dot;
-    double* ev;
-    double* p;
+    // The contact pass's outputs: one 128-byte record (four double4 parts,
+    // OUT_*) per particle, written whole by the lanes of a contact group
+    // (scattered by original index, each record is one cache line).
+    double4* out;
     // sorting
     uint32_t* key;       // cell key per original index (hash output)
     uint32_t* key_prev;  // key of the last sorted step, per original index
@@ -101,7 +104,6 @@ struct DevArrays {
     double4* w_nx;       // normal (xyz) and position (w)
     double4* w_mvfd;     // mass, velocity, force, sigma0
     double* w_acc;
-    double* force_pp;    // MAXWALLS x ntot wall force per particle
     double* tau_pp;      // np shear stress per particle (wmode 3)
     double* partial;     // block sums, one per 256-particle chunk
     double* tau_partial;

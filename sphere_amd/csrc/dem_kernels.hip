@@ -728,7 +728,7 @@ __device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t ib, uin
 __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevArrays& a, Acc& A,
                                                   const double4 xa, const double4 vel_a,
                                                   const double4 av_a, double Lz, uint32_t o,
-                                                  bool write) {
+                                                  int part) {
     const double ra = xa.w;
     double wf0 = 0.0, wf1 = 0.0, wf2 = 0.0, wf3 = 0.0, wf4 = 0.0;
     {
@@ -775,21 +775,30 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
             if (dw < 0.0) wf4 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
     }
-    // Outputs at the original index, src/contactsearch.cuh:706-724.
-    if (!write) return;
-    const size_t np = a.ntot;  // wall-major stride of force_pp
-    a.force[o] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
-    a.torque[o] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
-    a.es_dot[o] = A.es_dot;
-    a.ev_dot[o] = A.ev_dot;
-    a.es[o] += A.es_dot * p.dt;
-    a.ev[o] += A.ev_dot * p.dt;
-    a.p[o] = A.p;
-    if (p.nw > 0) a.force_pp[o] = wf0;
-    if (p.nw > 1) a.force_pp[(size_t)np + o] = wf1;
-    if (p.nw > 2) a.force_pp[(size_t)np * 2 + o] = wf2;
-    if (p.nw > 3) a.force_pp[(size_t)np * 3 + o] = wf3;
-    if (p.nw > 4) a.force_pp[(size_t)np * 4 + o] = wf4;
+    // Outputs at the original index, src/contactsearch.cuh:706-724: part
+    // `part` of the particle's output record (all four with part == 4; -1
+    // writes nothing).  es and ev accumulate, es += es_dot dt.
+    double4* const rec = a.out + (size_t)o * 4;
+    if (part == 0 || part == 4) {
+        const double es = rec[OUT_F].w;
+        rec[OUT_F] = make_double4(A.F.x, A.F.y, A.F.z, es + A.es_dot * p.dt);
+    }
+    if (part == 1 || part == 4) {
+        const double ev = rec[OUT_T].w;
+        rec[OUT_T] = make_double4(A.T.x, A.T.y, A.T.z, ev + A.ev_dot * p.dt);
+    }
+    if (part == 2 || part == 4) rec[OUT_E] = make_double4(A.es_dot, A.ev_dot, A.p, wf0);
+    if (part == 3 || part == 4) rec[OUT_W] = make_double4(wf1, wf2, wf3, wf4);
+}
+
+// A particle's force on wall w, from its output record (walls beyond the
+// top and the four side walls take none).
+__device__ __forceinline__ double wall_force_pp(const DevArrays& a, uint32_t i, uint32_t w) {
+    const double4* rec = a.out + (size_t)i * 4;
+    if (w == 0) return rec[OUT_E].w;
+    if (w >= OUT_NWALL) return 0.0;
+    const double4 v = rec[OUT_W];
+    return w == 1 ? v.x : w == 2 ? v.y : w == 3 ? v.z : v.w;
 }
 
 // The adjoint's record of the step's starting state: each particle's and
@@ -1025,7 +1034,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     }
                 }
     }
-    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, true);
+    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, 4);
 }
 
 // ---------------------------------------------------------------------------
@@ -1801,7 +1810,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         for (int k = 0; k < SPL; ++k)
             a.tape_rows[(size_t)ia_orig * NC + g.q * SPL + k] = word_to_orig(a, gs[g.q * SPL + k]);
     }
-    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q == 0);
+    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q < 4 ? (int)g.q : -1);
     if constexpr (FUSED) {
         // The particle's force and torque are final (its own row and the
         // walls): integrate it now (src/integration.cuh:14-305), from the
@@ -1892,7 +1901,7 @@ __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevArrays a) {
     if (in) {
         const double z = a.x[idx].z;
         const double fixvel = a.vel[idx].w;
-        const double force_x = a.force[idx].x;
+        const double force_x = a.out[(size_t)idx * 4 + OUT_F].x;
         double f_x = 0.0;
         if (fixvel > 0.0 && z > Lz * 0.5) f_x = force_x;
         v = f_x / (p.L[0] * p.L[1]);
@@ -1934,8 +1943,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (lp->order_changed) a.perm_rows[idx] = a.perm[idx];
     HashFlags f;
     if (in) {
-        const double4 x_new = integrate_particle(p, a, idx, a.force[idx], a.torque[idx], a.x[idx],
-                                                 a.vel[idx], a.angvel[idx], a.rank[idx]);
+        const double4 x_new =
+            integrate_particle(p, a, idx, a.out[(size_t)idx * 4 + OUT_F],
+                               a.out[(size_t)idx * 4 + OUT_T], a.x[idx], a.vel[idx],
+                               a.angvel[idx], a.rank[idx]);
         if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
     }
     if (hash_next) publish_next(p, a, lp, f, idx);
@@ -1943,7 +1954,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // wall's (wall-major, nchunk per wall).
     const uint32_t nsum = p.nw == 0 ? 0u : (p.wall_sums ? p.nw : 1u);
     for (uint32_t w = 0; w < nsum; ++w) {
-        const double v = in ? a.force_pp[(size_t)w * a.ntot + idx] : 0.0;
+        const double v = in ? wall_force_pp(a, idx, w) : 0.0;
         const double s = block_tree_sum(v, cache);
         if (threadIdx.x == 0) a.partial[(size_t)w * a.nchunk + blockIdx.x] = s;
     }
@@ -1968,8 +1979,8 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
     HashFlags f;
     if (in) {
         const uint32_t c = g.q < 3 ? g.q : 0u;
-        const double fq = reinterpret_cast<const double*>(a.force + idx)[c];
-        const double tq = reinterpret_cast<const double*>(a.torque + idx)[c];
+        const double fq = reinterpret_cast<const double*>(a.out + (size_t)idx * 4 + OUT_F)[c];
+        const double tq = reinterpret_cast<const double*>(a.out + (size_t)idx * 4 + OUT_T)[c];
         const double xn = integrate_component(p, a, lp, idx, g.q, fq, tq, a.x[idx], a.vel[idx],
                                               a.angvel[idx], a.rank[idx]);
         const double4 x_new = make_double4(g.template bc<0>(xn), g.template bc<1>(xn),
@@ -1991,7 +2002,7 @@ __global__ __launch_bounds__(BLOCK) void k_wall_partials(const DevArrays a) {
     if (a.loop[p.sim].order_changed) a.perm_rows[idx] = a.perm[idx];
     const uint32_t nsum = p.nw == 0 ? 0u : (p.wall_sums ? p.nw : 1u);
     for (uint32_t w = 0; w < nsum; ++w) {
-        const double v = in ? a.force_pp[(size_t)w * a.ntot + idx] : 0.0;
+        const double v = in ? wall_force_pp(a, idx, w) : 0.0;
         const double s = block_tree_sum(v, cache);
         if (threadIdx.x == 0) a.partial[(size_t)w * a.nchunk + blockIdx.x] = s;
     }
@@ -2333,13 +2344,14 @@ __device__ __forceinline__ void apply_bond(const DevArrays& a, uint32_t i, uint3
     const uint32_t bx = a.bonds[2 * b], by = a.bonds[2 * b + 1];
     if (bx != i && by != i) return;
     const double4 f = a.bond_f[b], t = a.bond_t[b];
+    // xyz only: the w parts of the record hold es and ev.
     if (bx == i) {
-        F.x += f.x; F.y += f.y; F.z += f.z; F.w += 0.0;
-        T.x -= t.x; T.y -= t.y; T.z -= t.z; T.w -= 0.0;
+        F.x += f.x; F.y += f.y; F.z += f.z;
+        T.x -= t.x; T.y -= t.y; T.z -= t.z;
     }
     if (by == i) {
-        F.x -= f.x; F.y -= f.y; F.z -= f.z; F.w -= 0.0;
-        T.x += t.x; T.y += t.y; T.z += t.z; T.w += 0.0;
+        F.x -= f.x; F.y -= f.y; F.z -= f.z;
+        T.x += t.x; T.y += t.y; T.z += t.z;
     }
     touched = true;
 }
@@ -2353,12 +2365,15 @@ __global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevArrays a) {
     const uint32_t b0 = full ? p.bond_base : a.badj_off[i];
     const uint32_t b1 = full ? p.bond_base + p.nb0 : a.badj_off[i + 1];
     if (b0 == b1) return;
-    double4 F = a.force[i], T = a.torque[i];
+    // Force and torque of the output record (their w parts, es and ev,
+    // pass through unchanged).
+    double4* const rec = a.out + (size_t)i * 4;
+    double4 F = rec[OUT_F], T = rec[OUT_T];
     bool touched = false;
     for (uint32_t k = b0; k < b1; ++k) apply_bond(a, i, full ? k : a.badj[k], F, T, touched);
     if (touched) {
-        a.force[i] = F;
-        a.torque[i] = T;
+        rec[OUT_F] = F;
+        rec[OUT_T] = T;
     }
 }
 

@@ -498,9 +498,8 @@ int allocate(sph_ctx* c) {
         rh += 256u * n_tiles(sim.P.npad);
     }
     ALLOC(A.x, np); ALLOC(A.vel, np); ALLOC(A.angvel, np); ALLOC(A.angpos, np);
-    ALLOC(A.xyzsum, np); ALLOC(A.acc, np); ALLOC(A.angacc, np); ALLOC(A.force, np);
-    ALLOC(A.torque, np);
-    ALLOC(A.es_dot, np); ALLOC(A.es, np); ALLOC(A.ev_dot, np); ALLOC(A.ev, np); ALLOC(A.p, np);
+    ALLOC(A.xyzsum, np); ALLOC(A.acc, np); ALLOC(A.angacc, np);
+    ALLOC(A.out, np * 4);
     ALLOC(A.key, np); ALLOC(A.key_prev, np); ALLOC(A.skey, np); ALLOC(A.perm, np);
     ALLOC(A.key_tmp, np); ALLOC(A.val_tmp, np);
     // The batch-wide sort and the per-simulation one-workgroup sorts.
@@ -522,7 +521,6 @@ int allocate(sph_ctx* c) {
     ALLOC(A.perm_rows, np);
     ALLOC(A.delta_t, np * NC);
     ALLOC(A.w_nx, nw); ALLOC(A.w_mvfd, nw); ALLOC(A.w_acc, nw);
-    ALLOC(A.force_pp, (size_t)MAXWALLS * np);
     ALLOC(A.tau_pp, np);
     ALLOC(A.partial, (size_t)MAXWALLS * A.nchunk);
     ALLOC(A.tau_partial, A.nchunk); ALLOC(A.tau_eff, ns);
@@ -554,6 +552,47 @@ int upload_params(sph_ctx* c) {
     return SPH_OK;
 }
 
+// The output record of a particle (four double4 parts, OUT_* in
+// dem_arrays.hpp) as 16 doubles: force xyz, es | torque xyz, ev | es_dot,
+// ev_dot, p, wall-0 force | wall 1-4 forces.
+void pack_out(const Host& h, double* rec, size_t np) {
+    for (size_t i = 0; i < np; ++i) {
+        double* r = rec + 16 * i;
+        r[0] = h.force[4 * i]; r[1] = h.force[4 * i + 1]; r[2] = h.force[4 * i + 2];
+        r[3] = h.es[i];
+        r[4] = h.torque[4 * i]; r[5] = h.torque[4 * i + 1]; r[6] = h.torque[4 * i + 2];
+        r[7] = h.ev[i];
+        r[8] = h.es_dot[i]; r[9] = h.ev_dot[i]; r[10] = h.p[i];
+    }
+}
+
+void unpack_out(const double* rec, Host& h, size_t np) {
+    for (size_t i = 0; i < np; ++i) {
+        const double* r = rec + 16 * i;
+        h.force[4 * i] = r[0]; h.force[4 * i + 1] = r[1]; h.force[4 * i + 2] = r[2];
+        h.es[i] = r[3];
+        h.torque[4 * i] = r[4]; h.torque[4 * i + 1] = r[5]; h.torque[4 * i + 2] = r[6];
+        h.ev[i] = r[7];
+        h.es_dot[i] = r[8]; h.ev_dot[i] = r[9]; h.p[i] = r[10];
+    }
+}
+
+// Offset of a field in the 16 doubles of an output record, and its width
+// (3 for vectors, 1 for scalars); -1 for fields not in the record.
+int out_field(int f, int* width) {
+    *width = 1;
+    switch (f) {
+        case SPH_F_FORCE: *width = 3; return 0;
+        case SPH_F_ES: return 3;
+        case SPH_F_TORQUE: *width = 3; return 4;
+        case SPH_F_EV: return 7;
+        case SPH_F_ES_DOT: return 8;
+        case SPH_F_EV_DOT: return 9;
+        case SPH_F_P: return 10;
+        default: return -1;
+    }
+}
+
 int upload_state(sph_ctx* c) {
     hipStream_t s = c->stream;
     DevArrays& A = c->A;
@@ -569,10 +608,7 @@ int upload_state(sph_ctx* c) {
     HIPCHK(c, hipMemsetAsync(A.xyzsum, 0, N * 32, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.acc, 0, N * 32, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.angacc, 0, N * 32, s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.force, 0, N * 32, s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.torque, 0, N * 32, s), "memset");
-    double* e[5] = {A.es_dot, A.es, A.ev_dot, A.ev, A.p};
-    for (int k = 0; k < 5; ++k) HIPCHK(c, hipMemsetAsync(e[k], 0, N * 8, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.out, 0, N * 128, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.delta_t, 0, N * NC * sizeof(double4), s), "memset");
     HIPCHK(c, hipMemsetAsync(A.vcount, 0, N * 8, s), "memset");
     for (int k = 0; k < 2; ++k) {
@@ -581,7 +617,6 @@ int upload_state(sph_ctx* c) {
         HIPCHK(c, hipMemsetAsync(A.ws[k], 0, N * 32, s), "memset");
     }
     HIPCHK(c, hipMemsetAsync(A.x_build, 0, N * 32, s), "memset");
-    HIPCHK(c, hipMemsetAsync(A.force_pp, 0, (size_t)MAXWALLS * N * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.tau_pp, 0, N * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.tau_eff, 0, A.nsim * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.glob, 0, sizeof(DevGlobal), s), "memset");
@@ -603,6 +638,7 @@ int upload_state(sph_ctx* c) {
     std::vector<double> bd, bo;
     std::vector<uint32_t> off(N + 1, 0u);
     std::vector<DevLoop> loops(A.nsim);
+    std::vector<std::vector<double>> out_staging(c->sims.size());
     for (const Sim& sim : c->sims) {
         const Host& h = sim.h;
         const DevParams& P = sim.P;
@@ -615,13 +651,12 @@ int upload_state(sph_ctx* c) {
             H2D(A.angvel + b0, h.angvel.data(), np * 32);
             H2D(A.angpos + b0, h.angpos.data(), np * 32);
             H2D(A.xyzsum + b0, h.xyzsum.data(), np * 32);
-            H2D(A.force + b0, h.force.data(), np * 32);
-            H2D(A.torque + b0, h.torque.data(), np * 32);
-            H2D(A.es_dot + b0, h.es_dot.data(), np * 8);
-            H2D(A.es + b0, h.es.data(), np * 8);
-            H2D(A.ev_dot + b0, h.ev_dot.data(), np * 8);
-            H2D(A.ev + b0, h.ev.data(), np * 8);
-            H2D(A.p + b0, h.p.data(), np * 8);
+            {
+                std::vector<double>& rec = out_staging[P.sim];
+                rec.assign(np * 16, 0.0);
+                pack_out(h, rec.data(), np);
+                H2D(A.out + 4 * b0, rec.data(), np * 128);
+            }
             H2D(A.x_build + b0, h.x.data(), np * 32);
         }
         wn.insert(wn.end(), h.w_nx.begin(), h.w_nx.end());
@@ -1089,6 +1124,7 @@ int download_state(sph_ctx* c, int k) {
     const size_t b0 = sim.P.base, w0 = sim.P.wall_base, bb = sim.P.bond_base;
     hipStream_t s = c->stream;
     const DevArrays& A = c->A;
+    std::vector<double> out_rec;
 #define D2H(dst, src, bytes) \
     HIPCHK(c, hipMemcpyAsync((dst), (src), (bytes), hipMemcpyDeviceToHost, s), "D2H")
     if (np) {
@@ -1097,13 +1133,8 @@ int download_state(sph_ctx* c, int k) {
         D2H(h.angvel.data(), A.angvel + b0, np * 32);
         D2H(h.angpos.data(), A.angpos + b0, np * 32);
         D2H(h.xyzsum.data(), A.xyzsum + b0, np * 32);
-        D2H(h.force.data(), A.force + b0, np * 32);
-        D2H(h.torque.data(), A.torque + b0, np * 32);
-        D2H(h.es_dot.data(), A.es_dot + b0, np * 8);
-        D2H(h.es.data(), A.es + b0, np * 8);
-        D2H(h.ev_dot.data(), A.ev_dot + b0, np * 8);
-        D2H(h.ev.data(), A.ev + b0, np * 8);
-        D2H(h.p.data(), A.p + b0, np * 8);
+        out_rec.resize(np * 16);
+        D2H(out_rec.data(), A.out + 4 * b0, np * 128);
     }
     if (nw) {
         D2H(h.w_nx.data(), A.w_nx + w0, nw * 32);
@@ -1119,6 +1150,7 @@ int download_state(sph_ctx* c, int k) {
     int rc = read_loop(c, k, &L);
     if (rc) return rc;
     for (size_t i = 0; i < 2 * nb; ++i) h.bonds[i] -= (uint32_t)b0;  // batch -> .bin index
+    if (np) unpack_out(out_rec.data(), h, np);
     h.t = L.t;
     if (h.adaptive == 1 && h.nw > 0 && L.iter > 0) h.L[2] = L.grid_Lz;
     return SPH_OK;
@@ -1601,16 +1633,28 @@ int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
         case SPH_F_VEL: return d2h(c, dst, A.vel + b0, bytes);
         case SPH_F_ANGVEL: return get_vec3(c, A.angvel + b0, np, dst);
         case SPH_F_ANGPOS: return get_vec3(c, A.angpos + b0, np, dst);
-        case SPH_F_FORCE: return get_vec3(c, A.force + b0, np, dst);
-        case SPH_F_TORQUE: return get_vec3(c, A.torque + b0, np, dst);
+
         case SPH_F_XYZSUM: return get_vec3(c, A.xyzsum + b0, np, dst);
         case SPH_F_ACC: return get_vec3(c, A.acc + b0, np, dst);
         case SPH_F_ANGACC: return get_vec3(c, A.angacc + b0, np, dst);
-        case SPH_F_ES_DOT: return d2h(c, dst, A.es_dot + b0, bytes);
-        case SPH_F_ES: return d2h(c, dst, A.es + b0, bytes);
-        case SPH_F_EV_DOT: return d2h(c, dst, A.ev_dot + b0, bytes);
-        case SPH_F_EV: return d2h(c, dst, A.ev + b0, bytes);
-        case SPH_F_P: return d2h(c, dst, A.p + b0, bytes);
+        case SPH_F_FORCE:
+        case SPH_F_TORQUE:
+        case SPH_F_ES_DOT:
+        case SPH_F_ES:
+        case SPH_F_EV_DOT:
+        case SPH_F_EV:
+        case SPH_F_P: {
+            // From the output records.
+            int width;
+            const int off = out_field(f, &width);
+            std::vector<double> rec(np * 16);
+            int rc = d2h(c, rec.data(), A.out + 4 * b0, np * 128);
+            if (rc) return rc;
+            double* d = static_cast<double*>(dst);
+            for (size_t i = 0; i < np; ++i)
+                for (int k = 0; k < width; ++k) d[width * i + k] = rec[16 * i + off + k];
+            return SPH_OK;
+        }
         case SPH_F_CONTACTS: {
             // The reference's row format: partner's .bin index, np when empty.
             std::vector<uint32_t> rows;
@@ -1674,12 +1718,14 @@ int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
         case SPH_F_WALLS_MVFD: return d2h(c, dst, A.w_mvfd + w0, bytes);
         case SPH_F_WALLS_ACC: return d2h(c, dst, A.w_acc + w0, bytes);
         case SPH_F_WALLS_FORCE_PP: {
-            char* d = static_cast<char*>(dst);
-            for (uint32_t w = 0; w < S.h.nw; ++w) {
-                int rc = d2h(c, d + (size_t)w * np * 8, A.force_pp + (size_t)w * A.ntot + b0,
-                             np * 8);
-                if (rc) return rc;
-            }
+            // Wall-major, from the output records (doubles 11 and 12..15).
+            std::vector<double> rec(np * 16);
+            int rc = np ? d2h(c, rec.data(), A.out + 4 * b0, np * 128) : SPH_OK;
+            if (rc) return rc;
+            double* d = static_cast<double*>(dst);
+            for (uint32_t w = 0; w < S.h.nw; ++w)
+                for (size_t i = 0; i < np; ++i)
+                    d[(size_t)w * np + i] = w < OUT_NWALL ? rec[16 * i + 11 + w] : 0.0;
             return SPH_OK;
         }
         case SPH_F_BONDS: {
@@ -1732,16 +1778,28 @@ int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes)
             return f == SPH_F_X ? set_loop_flag(c, sim, &DevLoop::force_rebuild) : SPH_OK;
         }
         case SPH_F_ANGPOS: return set_vec3(c, A.angpos + b0, np, src);
-        case SPH_F_FORCE: return set_vec3(c, A.force + b0, np, src);
-        case SPH_F_TORQUE: return set_vec3(c, A.torque + b0, np, src);
+
         case SPH_F_XYZSUM: return set_vec3(c, A.xyzsum + b0, np, src);
         case SPH_F_ACC: return set_vec3(c, A.acc + b0, np, src);
         case SPH_F_ANGACC: return set_vec3(c, A.angacc + b0, np, src);
-        case SPH_F_ES_DOT: return h2d(c, A.es_dot + b0, src, bytes);
-        case SPH_F_ES: return h2d(c, A.es + b0, src, bytes);
-        case SPH_F_EV_DOT: return h2d(c, A.ev_dot + b0, src, bytes);
-        case SPH_F_EV: return h2d(c, A.ev + b0, src, bytes);
-        case SPH_F_P: return h2d(c, A.p + b0, src, bytes);
+        case SPH_F_FORCE:
+        case SPH_F_TORQUE:
+        case SPH_F_ES_DOT:
+        case SPH_F_ES:
+        case SPH_F_EV_DOT:
+        case SPH_F_EV:
+        case SPH_F_P: {
+            // Into the output records, keeping their other fields.
+            int width;
+            const int off = out_field(f, &width);
+            std::vector<double> rec(np * 16);
+            int rc = d2h(c, rec.data(), A.out + 4 * b0, np * 128);
+            if (rc) return rc;
+            const double* v = static_cast<const double*>(src);
+            for (size_t i = 0; i < np; ++i)
+                for (int k = 0; k < width; ++k) rec[16 * i + off + k] = v[width * i + k];
+            return h2d(c, A.out + 4 * b0, rec.data(), np * 128);
+        }
         case SPH_F_CONTACTS: {
             // Keep each slot's image code; replace the partner index.
             std::vector<uint32_t> rows;
