@@ -15,7 +15,7 @@ namespace sphx {
 // All simulations of a context share these arrays: simulation s owns the
 // particle range [ptab[s].base, ptab[s].base + ptab[s].npad), which is a
 // whole number of 256-particle chunks, its walls, bonds and cells.
-// Parts of a particle's output record: force and es, torque and ev,
+// Parts of a particle's output record: force (w zero), torque (w zero),
 // es_dot, ev_dot, p and its force on wall 0, its forces on walls 1-4.
 constexpr uint32_t OUT_F = 0, OUT_T = 1, OUT_E = 2, OUT_W = 3;
 // The walls with a force per particle (top, two x walls, two y walls).
@@ -44,8 +44,12 @@ struct DevArrays {
     double4* angacc;
     // The contact pass's outputs: one 128-byte record (four double4 parts,
     // OUT_*) per particle, written whole by the lanes of a contact group
-    // (scattered by original index, each record is one cache line).
+    // (scattered by original index, each record is one cache line).  The
+    // energies es and ev they feed (es += es_dot dt) are advanced by the
+    // integration pass, in original order.
     double4* out;
+    double* es;
+    double* ev;
     // sorting
     uint32_t* key;       // cell key per original index (hash output)
     uint32_t* key_prev;  // key of the last sorted step, per original index

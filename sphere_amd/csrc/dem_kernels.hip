@@ -777,18 +777,21 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
     }
     // Outputs at the original index, src/contactsearch.cuh:706-724: part
     // `part` of the particle's output record (all four with part == 4; -1
-    // writes nothing).  es and ev accumulate, es += es_dot dt.
+    // writes nothing).  The integration pass advances es and ev by them.
     double4* const rec = a.out + (size_t)o * 4;
-    if (part == 0 || part == 4) {
-        const double es = rec[OUT_F].w;
-        rec[OUT_F] = make_double4(A.F.x, A.F.y, A.F.z, es + A.es_dot * p.dt);
-    }
-    if (part == 1 || part == 4) {
-        const double ev = rec[OUT_T].w;
-        rec[OUT_T] = make_double4(A.T.x, A.T.y, A.T.z, ev + A.ev_dot * p.dt);
-    }
+    if (part == 0 || part == 4) rec[OUT_F] = make_double4(A.F.x, A.F.y, A.F.z, 0.0);
+    if (part == 1 || part == 4) rec[OUT_T] = make_double4(A.T.x, A.T.y, A.T.z, 0.0);
     if (part == 2 || part == 4) rec[OUT_E] = make_double4(A.es_dot, A.ev_dot, A.p, wf0);
     if (part == 3 || part == 4) rec[OUT_W] = make_double4(wf1, wf2, wf3, wf4);
+}
+
+// es += es_dot dt and ev += ev_dot dt of the contact pass's outputs
+// (src/contactsearch.cuh:716-717), for particle i.
+__device__ __forceinline__ void advance_energies(const DevParams& p, const DevArrays& a,
+                                                 uint32_t i) {
+    const double4 e = a.out[(size_t)i * 4 + OUT_E];
+    a.es[i] = a.es[i] + e.x * p.dt;
+    a.ev[i] = a.ev[i] + e.y * p.dt;
 }
 
 // A particle's force on wall w, from its output record (walls beyond the
@@ -1817,6 +1820,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
         // step's sorted copies, into the next step's set, and hash it for
         // the next step.  (Fixed grid height, no bonds, no wmode-3 wall.)
         static_assert(!FUSED || LPP == 4, "one vector component per lane");
+        if (g.q == 0) {
+            a.es[ia_orig] = a.es[ia_orig] + A.es_dot * p.dt;
+            a.ev[ia_orig] = a.ev[ia_orig] + A.ev_dot * p.dt;
+        }
         const double fq = g.q == 0 ? A.F.x : g.q == 1 ? A.F.y : A.F.z;
         const double tq = g.q == 0 ? A.T.x : g.q == 1 ? A.T.y : A.T.z;
         const double xn = integrate_component(p, a, lp, ia_orig, g.q, fq, tq, xa, vel_a, av_a, ia);
@@ -1943,6 +1950,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (lp->order_changed) a.perm_rows[idx] = a.perm[idx];
     HashFlags f;
     if (in) {
+        advance_energies(p, a, idx);
         const double4 x_new =
             integrate_particle(p, a, idx, a.out[(size_t)idx * 4 + OUT_F],
                                a.out[(size_t)idx * 4 + OUT_T], a.x[idx], a.vel[idx],
@@ -1978,6 +1986,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
     if (lp->order_changed && g.q == 0) a.perm_rows[idx] = a.perm[idx];
     HashFlags f;
     if (in) {
+        if (g.q == 0) advance_energies(p, a, idx);
         const uint32_t c = g.q < 3 ? g.q : 0u;
         const double fq = reinterpret_cast<const double*>(a.out + (size_t)idx * 4 + OUT_F)[c];
         const double tq = reinterpret_cast<const double*>(a.out + (size_t)idx * 4 + OUT_T)[c];

@@ -499,7 +499,7 @@ int allocate(sph_ctx* c) {
     }
     ALLOC(A.x, np); ALLOC(A.vel, np); ALLOC(A.angvel, np); ALLOC(A.angpos, np);
     ALLOC(A.xyzsum, np); ALLOC(A.acc, np); ALLOC(A.angacc, np);
-    ALLOC(A.out, np * 4);
+    ALLOC(A.out, np * 4); ALLOC(A.es, np); ALLOC(A.ev, np);
     ALLOC(A.key, np); ALLOC(A.key_prev, np); ALLOC(A.skey, np); ALLOC(A.perm, np);
     ALLOC(A.key_tmp, np); ALLOC(A.val_tmp, np);
     // The batch-wide sort and the per-simulation one-workgroup sorts.
@@ -553,15 +553,13 @@ int upload_params(sph_ctx* c) {
 }
 
 // The output record of a particle (four double4 parts, OUT_* in
-// dem_arrays.hpp) as 16 doubles: force xyz, es | torque xyz, ev | es_dot,
+// dem_arrays.hpp) as 16 doubles: force xyz, 0 | torque xyz, 0 | es_dot,
 // ev_dot, p, wall-0 force | wall 1-4 forces.
 void pack_out(const Host& h, double* rec, size_t np) {
     for (size_t i = 0; i < np; ++i) {
         double* r = rec + 16 * i;
         r[0] = h.force[4 * i]; r[1] = h.force[4 * i + 1]; r[2] = h.force[4 * i + 2];
-        r[3] = h.es[i];
         r[4] = h.torque[4 * i]; r[5] = h.torque[4 * i + 1]; r[6] = h.torque[4 * i + 2];
-        r[7] = h.ev[i];
         r[8] = h.es_dot[i]; r[9] = h.ev_dot[i]; r[10] = h.p[i];
     }
 }
@@ -570,9 +568,7 @@ void unpack_out(const double* rec, Host& h, size_t np) {
     for (size_t i = 0; i < np; ++i) {
         const double* r = rec + 16 * i;
         h.force[4 * i] = r[0]; h.force[4 * i + 1] = r[1]; h.force[4 * i + 2] = r[2];
-        h.es[i] = r[3];
         h.torque[4 * i] = r[4]; h.torque[4 * i + 1] = r[5]; h.torque[4 * i + 2] = r[6];
-        h.ev[i] = r[7];
         h.es_dot[i] = r[8]; h.ev_dot[i] = r[9]; h.p[i] = r[10];
     }
 }
@@ -583,9 +579,7 @@ int out_field(int f, int* width) {
     *width = 1;
     switch (f) {
         case SPH_F_FORCE: *width = 3; return 0;
-        case SPH_F_ES: return 3;
         case SPH_F_TORQUE: *width = 3; return 4;
-        case SPH_F_EV: return 7;
         case SPH_F_ES_DOT: return 8;
         case SPH_F_EV_DOT: return 9;
         case SPH_F_P: return 10;
@@ -609,6 +603,8 @@ int upload_state(sph_ctx* c) {
     HIPCHK(c, hipMemsetAsync(A.acc, 0, N * 32, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.angacc, 0, N * 32, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.out, 0, N * 128, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.es, 0, N * 8, s), "memset");
+    HIPCHK(c, hipMemsetAsync(A.ev, 0, N * 8, s), "memset");
     HIPCHK(c, hipMemsetAsync(A.delta_t, 0, N * NC * sizeof(double4), s), "memset");
     HIPCHK(c, hipMemsetAsync(A.vcount, 0, N * 8, s), "memset");
     for (int k = 0; k < 2; ++k) {
@@ -657,6 +653,8 @@ int upload_state(sph_ctx* c) {
                 pack_out(h, rec.data(), np);
                 H2D(A.out + 4 * b0, rec.data(), np * 128);
             }
+            H2D(A.es + b0, h.es.data(), np * 8);
+            H2D(A.ev + b0, h.ev.data(), np * 8);
             H2D(A.x_build + b0, h.x.data(), np * 32);
         }
         wn.insert(wn.end(), h.w_nx.begin(), h.w_nx.end());
@@ -1135,6 +1133,8 @@ int download_state(sph_ctx* c, int k) {
         D2H(h.xyzsum.data(), A.xyzsum + b0, np * 32);
         out_rec.resize(np * 16);
         D2H(out_rec.data(), A.out + 4 * b0, np * 128);
+        D2H(h.es.data(), A.es + b0, np * 8);
+        D2H(h.ev.data(), A.ev + b0, np * 8);
     }
     if (nw) {
         D2H(h.w_nx.data(), A.w_nx + w0, nw * 32);
@@ -1637,12 +1637,12 @@ int sph_get_field_sim(sph_ctx* c, int sim, int f, void* dst, size_t bytes) {
         case SPH_F_XYZSUM: return get_vec3(c, A.xyzsum + b0, np, dst);
         case SPH_F_ACC: return get_vec3(c, A.acc + b0, np, dst);
         case SPH_F_ANGACC: return get_vec3(c, A.angacc + b0, np, dst);
+        case SPH_F_ES: return d2h(c, dst, A.es + b0, bytes);
+        case SPH_F_EV: return d2h(c, dst, A.ev + b0, bytes);
         case SPH_F_FORCE:
         case SPH_F_TORQUE:
         case SPH_F_ES_DOT:
-        case SPH_F_ES:
         case SPH_F_EV_DOT:
-        case SPH_F_EV:
         case SPH_F_P: {
             // From the output records.
             int width;
@@ -1782,12 +1782,12 @@ int sph_set_field_sim(sph_ctx* c, int sim, int f, const void* src, size_t bytes)
         case SPH_F_XYZSUM: return set_vec3(c, A.xyzsum + b0, np, src);
         case SPH_F_ACC: return set_vec3(c, A.acc + b0, np, src);
         case SPH_F_ANGACC: return set_vec3(c, A.angacc + b0, np, src);
+        case SPH_F_ES: return h2d(c, A.es + b0, src, bytes);
+        case SPH_F_EV: return h2d(c, A.ev + b0, src, bytes);
         case SPH_F_FORCE:
         case SPH_F_TORQUE:
         case SPH_F_ES_DOT:
-        case SPH_F_ES:
         case SPH_F_EV_DOT:
-        case SPH_F_EV:
         case SPH_F_P: {
             // Into the output records, keeping their other fields.
             int width;
