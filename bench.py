@@ -479,8 +479,8 @@ def main():
     ap.add_argument("--batch-calibration", type=int, default=0, metavar="S",
                     help="instead: S batched calibrations of 4000-sphere collapse columns "
                          "over 300 steps against the per-simulation loop")
-    ap.add_argument("--autotune", type=int, default=0, metavar="K",
-                    help="before the warm-up, run sph_autotune with K steps per trial")
+    ap.add_argument("--autotune", type=int, default=200, metavar="K",
+                    help="before the warm-up, run sph_autotune with K steps per trial (0: skip)")
     ap.add_argument("--graph-steps", type=int, default=100,
                     help="steady-state chunk replayed from a captured HIP graph "
                          "(identical results; 0 = plain launches)")
@@ -545,10 +545,23 @@ def main():
     nw = int(ctx.scalar("nw"))
     dt = ctx.scalar("dt")
     log("np %d, nw %d, ncells %d, dt %.4g s" % (n, nw, int(ctx.scalar("ncells")), dt))
+    tuning = None
     if args.autotune:
-        us = ctx.autotune(args.autotune)
-        log("autotune: %.2f us per step with contact lanes %d, xcd order %d" % (
-            us, int(ctx.scalar("contact_lanes")), int(ctx.scalar("xcd_order"))))
+        # Tuned on a second context from the same image (the trials advance
+        # the simulation); the timed context takes the chosen settings and
+        # starts from the image itself.
+        tctx = engine.Context(img)
+        tctx.set_scalar("graph_steps", args.graph_steps)
+        apply_settings(tctx, args)
+        us = tctx.autotune(args.autotune)
+        knobs = ("contact_lanes", "contact_wg", "vl_cap", "radix_bits", "xcd_order")
+        tuning = {k: int(tctx.scalar(k)) for k in knobs}
+        tctx.close()
+        for k in knobs:
+            ctx.set_scalar(k, tuning[k])
+        tuning["trial_steps"] = args.autotune
+        tuning["contact_us"] = round(us, 3)
+        log("autotune: contact kernel %.2f us with %s" % (us, tuning))
     ctx.step(args.warmup, sync=True)
     z = scenes.contact_number(ctx, n)
 
@@ -687,6 +700,7 @@ def main():
         "adjoint": adjoint,
         "kernel_us": {k: v[0] for k, v in rep.items()},
         "timed_events": {"re_sorts": sorts, "list_rebuilds": builds},
+        "tuning": tuning,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -156,12 +156,22 @@ enum sph_scalar {
                                  in N/m for both (bending and twisting
                                  stiffness per unit area bond_k / A).  Not in
                                  the .bin (fixed layout): set after opening. */
-    SPH_S_BOND_BREAK_EXT = 25 /* per simulation.  0 (default, the reference,
+    SPH_S_BOND_BREAK_EXT = 25,/* per simulation.  0 (default, the reference,
                                  src/cohesion.cuh:193-206): bonds break on the
                                  stress criterion (sigma_b, tau_b); > 0: a bond
                                  breaks when its normal extension reaches this
                                  fraction of the smaller radius.  Not in the
                                  .bin: set after opening. */
+    SPH_S_CONTACT_WG = 26,    /* work-items per workgroup of the contact kernel
+                                 with 4 lanes per particle: 128, 256 (default)
+                                 or 512.  Identical results. */
+    SPH_S_VL_CAP = 27,        /* candidate-list capacity per particle, 8..32
+                                 (default 32): a particle with more candidates
+                                 scans its 27 cells every step.  Identical
+                                 results. */
+    SPH_S_RADIX_BITS = 28     /* digit width of the radix sort: 4, 6 or 8
+                                 (default); the passes cover the keys' bit
+                                 length.  Identical results. */
 };
 
 /* Open flags. */
@@ -285,6 +295,24 @@ int sph_adjoint_samples_batch(sph_ctx* ctx, uint64_t every, uint64_t nsamples,
                               const double* dL_dx, const double* dL_dv, double* g_x0,
                               double* g_v0, double* g_w0, double* g_params);
 
+/* sph_adjoint_samples (and its batched form) in two calls, for callers
+ * that need the forward pass's samples before the loss is known (a
+ * torch.autograd forward, then its backward); no counterpart in the
+ * reference.  sph_adjoint_record runs nsamples x every steps from the
+ * current state recording the tape (or checkpoints, as sph_adjoint does),
+ * and writes the positions after every, 2 every, ... steps to samples_x
+ * (nsamples x rows x 3, the simulations' particles concatenated; may be
+ * NULL).  sph_adjoint_reverse then takes the loss gradients of those
+ * samples (dL_dx, dL_dv as in sph_adjoint_samples) and gives the same
+ * gradients, bit for bit, without stepping forward again (with checkpoints
+ * it recomputes each segment).  The context must neither step nor record
+ * again in between (SPH_E_ARG); the state after the reverse is the one
+ * after the forward pass. */
+int sph_adjoint_record(sph_ctx* ctx, uint64_t every, uint64_t nsamples, double* samples_x);
+int sph_adjoint_reverse(sph_ctx* ctx, uint64_t every, uint64_t nsamples, const double* dL_dx,
+                        const double* dL_dv, double* g_x0, double* g_v0, double* g_w0,
+                        double* g_params);
+
 /* Restart sidecar (no counterpart in the reference: src/file_io.cpp:420-731
  * writes none of this, so a run resumed from an output .bin starts with
  * empty contact rows, zero displacements and acc0 = 0).  Holds the contact
@@ -300,14 +328,20 @@ int sph_read_state_sim(sph_ctx* ctx, int sim, const char* path);
 /* Per-kernel timing: mean device microseconds of the last sph_step call's
  * kernels, measured with HIP events on the context stream
  * (the reference's PROFILING timers, src/device.cu:947-987,2711-2808). */
-/* Online autotuner; no counterpart in the reference.  Times steps_per_trial
- * steps under each setting of a coordinate search - contact lanes (4, 8,
- * 16; contact models 2 and 3), then the workgroup order (SPH_S_XCD_ORDER) -
- * each trial after steps_per_trial / 2 untimed steps, and keeps a setting
- * that is at least 3% faster.  Every setting gives the same results bit for
- * bit, so the trials are ordinary steps: the simulation advances by 1.5 x
- * steps_per_trial steps per trial (4 trials, or 2 for contact model 1).
- * *us_per_step (may be NULL) receives the best wall time per step. */
+/* Online autotuner; no counterpart in the reference.  A coordinate search,
+ * one knob at a time from the incumbent settings, keeping a value that is
+ * at least 3% faster: contact lanes (4, 8, 16; contact models 2 and 3), the
+ * contact workgroup size (128, 256, 512; 4 lanes), the candidate-list
+ * capacity (16, 24, 32) and the workgroup order (SPH_S_XCD_ORDER), each
+ * judged by the contact kernel's mean time over steps_per_trial steps
+ * (HIP events) after steps_per_trial / 2 untimed steps; then the radix
+ * digit width (4, 6, 8 bits), judged by four full sorts of the current keys.
+ * Every setting gives the same results bit for bit, so the trials are
+ * ordinary steps: the simulation advances by 1.5 x steps_per_trial steps
+ * per contact trial (up to 11).  *us_per_step (may be NULL) receives the
+ * contact kernel's best mean time in microseconds; SPH_S_CONTACT_LANES,
+ * SPH_S_CONTACT_WG, SPH_S_VL_CAP, SPH_S_RADIX_BITS and SPH_S_XCD_ORDER then
+ * read the chosen settings. */
 int sph_autotune(sph_ctx* ctx, uint64_t steps_per_trial, double* us_per_step);
 
 int sph_profile(sph_ctx* ctx, int enable);

@@ -9,21 +9,18 @@ gamma_n, mu, x0 and v0.  `BatchTrajectory(sims, every, nsamples)` does the
 same for many simulations at once, each with its own parameters, in one
 batched context and one batched adjoint.  No counterpart in the reference.
 
-The forward pass keeps one checkpoint of the whole simulation state at its
-start (the .bin image and the restart sidecar: contact rows,
-displacements, accelerations, loop counter).  The backward pass restores it
-and runs sph_adjoint_samples over the whole run with the loss gradient of
-every sample: each enters the reverse pass at its step, so the cotangents
-of the accelerations and contact displacements carry across sampling
-times.  The engine records the run in segments from checkpoints of its own
+The forward pass opens a fresh context from the .bin image of the
+parameters and initial state.  When a gradient is wanted it runs
+sph_adjoint_record: the same steps, recording the tape as it goes, and the
+samples; the backward pass then runs sph_adjoint_reverse on that context
+with the loss gradient of every sample (each enters the reverse pass at
+its step, so the cotangents of the accelerations and contact
+displacements carry across sampling times) and closes it.  Without a
+gradient the forward pass simply steps.  The engine records the run in segments from checkpoints of its own
 when the whole tape does not fit (SPH_S_ADJ_CHECKPOINT sets the segment
 length).  Lengths are differentiated as exact roots and the contact list and
 branches of each step are frozen, as in sph_adjoint.
 """
-
-import os
-import shutil
-import tempfile
 
 import numpy
 import torch
@@ -42,19 +39,6 @@ TIES = {
     "mu_s": ("mu_s",),
     "mu_d": ("mu_d",),
 }
-
-
-class _Start:
-    """The forward pass's starting state; its sidecar file goes with it."""
-
-    def __init__(self, ctx):
-        self.dir = tempfile.mkdtemp(prefix="sphere-traj-")
-        self.state = os.path.join(self.dir, "start.state")
-        ctx.write_state(self.state)
-        self.image = ctx.tobytes()
-
-    def __del__(self):
-        shutil.rmtree(self.dir, ignore_errors=True)
 
 
 class Trajectory:
@@ -102,46 +86,34 @@ class Trajectory:
         return _TrajectoryFn.apply(theta, x0, v0, self)
 
     # -- the two passes ---------------------------------------------------
-    def run_forward(self, theta, x0, v0):
-        """Samples (nsamples, np, 3) and the start state for run_backward."""
-        samples = numpy.zeros((self.nsamples, self.np, 3))
+    def run_forward(self, theta, x0, v0, record=False):
+        """Samples (nsamples, np, 3), and with `record` the context holding
+        the recorded run for run_backward (else None)."""
         ctx = engine.Context(self.image(theta, x0, v0), device=self.device)
+        if record:
+            try:
+                return ctx.adjoint_record(self.every, self.nsamples), ctx
+            except Exception:
+                ctx.close()
+                raise
+        samples = numpy.zeros((self.nsamples, self.np, 3))
         try:
-            start = _Start(ctx)
             for k in range(self.nsamples):
                 ctx.step(self.every)
                 samples[k] = ctx.get("x")[:, :3]
         finally:
             ctx.close()
-        return samples, start
+        return samples, None
 
-    def run_backward(self, start, grad_samples):
-        """Gradients (theta, x0, v0) of sum_k grad_samples[k] . sample_k."""
-        ctx = engine.Context(start.image, device=self.device)
+    def run_backward(self, ctx, grad_samples):
+        """Gradients (theta, x0, v0) of sum_k grad_samples[k] . sample_k;
+        closes the context run_forward recorded the run in."""
         try:
-            ctx.read_state(start.state)
-            g = ctx.adjoint_samples(self.every, self.nsamples, grad_samples, None)
+            g = ctx.adjoint_reverse(self.every, self.nsamples, grad_samples, None)
         finally:
             ctx.close()
         gtheta = numpy.array([sum(g[f] for f in self.ties[name]) for name in self.params])
         return gtheta, g["x"], g["v"]
-
-
-class _BatchStart:
-    """The batched forward pass's starting state: one image and one
-    sidecar per simulation."""
-
-    def __init__(self, ctx):
-        self.dir = tempfile.mkdtemp(prefix="sphere-batch-")
-        self.images, self.states = [], []
-        for k in range(ctx.nsims):
-            path = os.path.join(self.dir, "start%04d.state" % k)
-            ctx.write_state(path, sim=k)
-            self.states.append(path)
-            self.images.append(ctx.tobytes(sim=k))
-
-    def __del__(self):
-        shutil.rmtree(self.dir, ignore_errors=True)
 
 
 class BatchTrajectory:
@@ -188,11 +160,16 @@ class BatchTrajectory:
                                  dtype=torch.float64)
         return _TrajectoryFn.apply(theta, x0, v0, self)
 
-    def run_forward(self, theta, x0, v0):
-        samples = numpy.zeros((self.nsamples, self.rows, 3))
+    def run_forward(self, theta, x0, v0, record=False):
         ctx = engine.Context(images=self.images(theta, x0, v0), device=self.device)
+        if record:
+            try:
+                return ctx.adjoint_record(self.every, self.nsamples), ctx
+            except Exception:
+                ctx.close()
+                raise
+        samples = numpy.zeros((self.nsamples, self.rows, 3))
         try:
-            start = _BatchStart(ctx)
             for k in range(self.nsamples):
                 ctx.step(self.every)
                 for j in range(ctx.nsims):
@@ -200,14 +177,11 @@ class BatchTrajectory:
                     samples[k, a:b] = ctx.get("x", sim=j)[:, :3]
         finally:
             ctx.close()
-        return samples, start
+        return samples, None
 
-    def run_backward(self, start, grad_samples):
-        ctx = engine.Context(images=start.images, device=self.device)
+    def run_backward(self, ctx, grad_samples):
         try:
-            for j, path in enumerate(start.states):
-                ctx.read_state(path, sim=j)
-            g = ctx.adjoint_samples(self.every, self.nsamples, grad_samples, None)
+            g = ctx.adjoint_reverse(self.every, self.nsamples, grad_samples, None)
         finally:
             ctx.close()
         gtheta = numpy.stack([sum(numpy.asarray(g[f]) for f in self.ties[name])
@@ -218,16 +192,21 @@ class BatchTrajectory:
 class _TrajectoryFn(torch.autograd.Function):
     @staticmethod
     def forward(fctx, theta, x0, v0, traj):
+        record = any(fctx.needs_input_grad[:3])
         samples, saved = traj.run_forward(theta.detach().cpu().numpy(),
-                                          x0.detach().cpu().numpy(), v0.detach().cpu().numpy())
+                                          x0.detach().cpu().numpy(), v0.detach().cpu().numpy(),
+                                          record=record)
         fctx.traj = traj
-        fctx.saved = saved
+        fctx.saved = saved  # the context holding the recorded run
         fctx.dev = theta.device
         return torch.from_numpy(samples).to(fctx.dev)
 
     @staticmethod
     def backward(fctx, grad):
-        gtheta, gx, gv = fctx.traj.run_backward(fctx.saved, grad.detach().cpu().numpy())
+        if fctx.saved is None:
+            raise RuntimeError("the trajectory's forward pass recorded no run to reverse")
+        saved, fctx.saved = fctx.saved, None
+        gtheta, gx, gv = fctx.traj.run_backward(saved, grad.detach().cpu().numpy())
         dev = fctx.dev
         return (torch.from_numpy(numpy.ascontiguousarray(gtheta)).to(dev),
                 torch.from_numpy(gx).to(dev), torch.from_numpy(gv).to(dev), None)

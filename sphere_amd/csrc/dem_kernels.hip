@@ -142,27 +142,29 @@ __global__ __launch_bounds__(BLOCK) void k_hash(const DevArrays a) {
 }
 
 // ---------------------------------------------------------------------------
-// Stable LSD radix sort, 8-bit digits.  Tile t covers keys
-// [t*TILE, (t+1)*TILE) in rounds of 256: element order inside a tile is
-// (round, thread), which is the global index order, so ranking each round
-// after all earlier rounds keeps the sort stable.  The stages are written
-// per tile so that the same code runs as one launch per stage or inside the
-// single cooperative launch below.
+// Stable LSD radix sort with digits of `bits` bits (4, 6 or 8: at most
+// 256 bins).  Tile t covers keys [t*TILE, (t+1)*TILE) in rounds of 256:
+// element order inside a tile is (round, thread), which is the global
+// index order, so ranking each round after all earlier rounds keeps the
+// sort stable.  The stages are written per tile so that the same code runs
+// as one launch per stage or inside the one-workgroup re-sort.
 
 __device__ __forceinline__ void radix_hist_tile(const uint32_t* __restrict__ kin, uint32_t n,
-                                                int shift, uint32_t* __restrict__ hist,
-                                                uint32_t ntiles, uint32_t tile) {
+                                                int shift, int bits,
+                                                uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                uint32_t tile) {
     __shared__ uint32_t h[256];
+    const uint32_t bins = 1u << bits, mask = bins - 1u;
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = tile * RADIX_TILE;
 #pragma unroll
     for (int j = 0; j < RADIX_ITEMS; ++j) {
         const uint32_t e = base + j * BLOCK + threadIdx.x;
-        if (e < n) atomicAdd(&h[(kin[e] >> shift) & 255u], 1u);
+        if (e < n) atomicAdd(&h[(kin[e] >> shift) & mask], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
+    if (threadIdx.x < bins) hist[threadIdx.x * ntiles + tile] = h[threadIdx.x];
 }
 
 // Exclusive scan in place of the digit-major histogram (256 x ntiles) by
@@ -203,15 +205,17 @@ __device__ __forceinline__ void radix_scatter_tile(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ vin,
                                                    uint32_t* __restrict__ kout,
                                                    uint32_t* __restrict__ vout, uint32_t n,
-                                                   int shift, const uint32_t* __restrict__ hist,
+                                                   int shift, int bits,
+                                                   const uint32_t* __restrict__ hist,
                                                    uint32_t ntiles, uint32_t tile,
                                                    uint32_t vbase) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t whist[BLOCK / 64][256];
+    const uint32_t bins = 1u << bits, mask = bins - 1u;
     const uint32_t t = threadIdx.x;
     const uint32_t w = t >> 6;
     const uint32_t lane = t & 63u;
-    running[t] = hist[t * ntiles + tile];
+    running[t] = t < bins ? hist[t * ntiles + tile] : 0u;
     const uint32_t base = tile * RADIX_TILE;
     const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int j = 0; j < RADIX_ITEMS; ++j) {
@@ -222,10 +226,9 @@ __device__ __forceinline__ void radix_scatter_tile(const uint32_t* __restrict__ 
         const bool valid = e < n;
         const uint32_t k = valid ? kin[e] : 0u;
         const uint32_t v = valid ? (vin ? vin[e] : vbase + e) : 0u;
-        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t d = (k >> shift) & mask;
         unsigned long long peers = __ballot(valid);
-#pragma unroll
-        for (int bit = 0; bit < 8; ++bit) {
+        for (int bit = 0; bit < bits; ++bit) {
             const bool on = (d >> bit) & 1u;
             const unsigned long long b = __ballot(on);
             peers &= on ? b : ~b;
@@ -279,11 +282,11 @@ __host__ __device__ inline RadixPass radix_pass(const DevArrays& a, int pass, in
 __global__ __launch_bounds__(BLOCK) void k_radix_hist(const DevGlobal* __restrict__ g,
                                                       const DevLoop* __restrict__ lp,
                                                       const uint32_t* __restrict__ kin,
-                                                      uint32_t n, int shift,
+                                                      uint32_t n, int shift, int bits,
                                                       uint32_t* __restrict__ hist,
                                                       uint32_t ntiles) {
     if (!sort_needed(g, lp)) return;
-    radix_hist_tile(kin, n, shift, hist, ntiles, blockIdx.x);
+    radix_hist_tile(kin, n, shift, bits, hist, ntiles, blockIdx.x);
 }
 
 __global__ __launch_bounds__(1024) void k_radix_scan(const DevGlobal* __restrict__ g,
@@ -297,10 +300,10 @@ __global__ __launch_bounds__(1024) void k_radix_scan(const DevGlobal* __restrict
 __global__ __launch_bounds__(BLOCK) void k_radix_scatter(
     const DevGlobal* __restrict__ g, const DevLoop* __restrict__ lp,
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, int shift, int bits,
     const uint32_t* __restrict__ hist, uint32_t ntiles) {
     if (!sort_needed(g, lp)) return;
-    radix_scatter_tile(kin, vin, kout, vout, n, shift, hist, ntiles, blockIdx.x, 0u);
+    radix_scatter_tile(kin, vin, kout, vout, n, shift, bits, hist, ntiles, blockIdx.x, 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -320,7 +323,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(const DevGlobal* __restrict_
 // global (cell_base + the simulation's own key); only keys of real cells
 // get a range (a key beyond the grid, or of a padding particle, does not).
 __device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a, uint32_t i,
-                                          int sorted_bits, uint32_t par) {
+                                          int sorted_bits, uint32_t par, bool copies) {
     const uint32_t key = a.skey[i];
     if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
         atomicOr(&a.loop[p.sim].error_bits, (uint32_t)DFLAG_KEY_RANGE);
@@ -334,9 +337,11 @@ __device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a
     }
     if (i == p.base + p.np - 1 && lk < p.ncells) a.cell_end[key] = i + 1;
     const uint32_t o = a.perm[i];
-    a.xs[par][i] = a.x[o];
-    a.vs[par][i] = a.vel[o];
-    a.ws[par][i] = a.angvel[o];
+    if (copies) {
+        a.xs[par][i] = a.x[o];
+        a.vs[par][i] = a.vel[o];
+        a.ws[par][i] = a.angvel[o];
+    }
     a.rank[o] = i;
     a.key_prev[o] = key;
 }
@@ -353,7 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_cells(const DevArrays a, int sorted_b
         atomicAdd(&a.loop[p.sim].sorts, 1u);
         a.loop[p.sim].order_changed = 1u;
     }
-    cells_one(p, a, i, sorted_bits, cur_set(a.loop + p.sim));
+    cells_one(p, a, i, sorted_bits, cur_set(a.loop + p.sim), true);
 }
 
 // ---------------------------------------------------------------------------
@@ -684,6 +689,27 @@ __device__ __forceinline__ uint32_t word_to_orig(const DevArrays& a, uint32_t w)
     return (w & ~CW_IDX_MASK) | a.perm[idx];
 }
 
+// The state the contact passes read by sorted index: the step's sorted
+// copies, or on the step after a one-workgroup re-sort (which leaves them
+// in the previous order, copies_stale) the original-index arrays through
+// perm.  The flag is uniform over a simulation.
+struct SortedReader {
+    const double4* __restrict__ x;
+    const double4* __restrict__ v;
+    const double4* __restrict__ w;
+    const uint32_t* __restrict__ perm;
+    bool stale;
+    __device__ __forceinline__ SortedReader(const DevArrays& a, const DevLoop* lp) {
+        stale = lp->copies_stale != 0u;
+        const uint32_t cs = cur_set(lp);
+        x = stale ? a.x : a.xs[cs];
+        v = stale ? a.vel : a.vs[cs];
+        w = stale ? a.angvel : a.ws[cs];
+        perm = a.perm;
+    }
+    __device__ __forceinline__ uint32_t at(uint32_t i) const { return stale ? perm[i] : i; }
+};
+
 // ---------------------------------------------------------------------------
 // topology + interact, one work-item per particle in sorted order
 // (src/contactsearch.cuh:329-371 then :385-726).  The particle's contact
@@ -834,15 +860,12 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
     uint32_t* row = srow + threadIdx.x * ROW_STRIDE;
     DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
-    const uint32_t cs = cur_set(lp);
-    const double4* __restrict__ xs = a.xs[cs];
-    const double4* __restrict__ vs = a.vs[cs];
-    const double4* __restrict__ ws = a.ws[cs];
-    const double4 xa = xs[ia];
+    const SortedReader S(a, lp);
+    const double4 xa = S.x[S.at(ia)];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
-    const double4 vel_a = vs[ia];
-    const double4 av_a = ws[ia];
+    const double4 vel_a = S.v[S.at(ia)];
+    const double4 av_a = S.w[S.at(ia)];
     const d3 angvel_a = xyz(av_a);
     tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, true);
     Acc A;
@@ -899,14 +922,14 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                         bool cell_done = false;
                         for (uint32_t ib = start; ib < end; ++ib) {
                             if (ib == ia) continue;
-                            const double4 xb = xs[ib];
+                            const double4 xb = S.x[S.at(ib)];
                             const double rb = xb.w;
                             d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
                             xab += dm;
                             const double dd = dot(xab, xab);
-                            const double S = ra + rb;
+                            const double Sab = ra + rb;
                             if (build) {
-                                const double Sv = S + p.vl_skin;
+                                const double Sv = Sab + p.vl_skin;
                                 if (dd < Sv * Sv) {
                                     if (nl < (uint32_t)NV)
                                         vrow[nl] = make_uint2(ib | (celloff << VL_CELL_SHIFT),
@@ -918,14 +941,14 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                             // Conservative reject: len32 > S whenever
                             // dd >= S^2 (1 + 2^-16) (the two FP32 roundings
                             // move the root by < 2^-23 relative).
-                            if (dd >= S * S * 1.0000152587890625) continue;
+                            if (dd >= Sab * Sab * 1.0000152587890625) continue;
                             const double len = sqrt32(dd);
-                            if (len - S < 0.0)
+                            if (len - Sab < 0.0)
                                 cell_done = register_contact(row, ib, (uint32_t)code, nc, lp);
                         }
                     }
             if (build) {
-                a.vcount[ia] = make_uint2(nl <= (uint32_t)NV ? nl : VL_OVERFLOW, 0u);
+                a.vcount[ia] = make_uint2(nl <= p.vl_cap ? nl : VL_OVERFLOW, 0u);
                 a.x_build[ia_orig] = xa;
             }
         } else {
@@ -940,16 +963,16 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     cell_done = false;
                 }
                 if (cell_done) continue;
-                const double4 xb = xs[e.x & VL_IB_MASK];
+                const double4 xb = S.x[S.at(e.x & VL_IB_MASK)];
                 const double rb = xb.w;
                 const uint32_t code = word_code(e.y);
                 d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
                 xab += shift_of(p, code);
                 const double dd = dot(xab, xab);
-                const double S = ra + rb;
-                if (dd >= S * S * 1.0000152587890625) continue;
+                const double Sab = ra + rb;
+                if (dd >= Sab * Sab * 1.0000152587890625) continue;
                 const double len = sqrt32(dd);
-                if (len - S < 0.0)
+                if (len - Sab < 0.0)
                     cell_done = register_contact(row, e.y & CW_IDX_MASK, code, nc, lp);
             }
         }
@@ -961,14 +984,15 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
             const uint32_t rb_s = wrd & CW_IDX_MASK;
             if (rb_s != EMPTY) {
                 const d3 dm = shift_of(p, word_code(wrd));
-                const double4 xb = xs[rb_s];
+                const uint32_t rb_j = S.at(rb_s);
+                const double4 xb = S.x[rb_j];
                 const double rb = xb.w;
                 const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
                 const double len = length(xab);
                 const double dn = len - (ra + rb);
                 if (dn < 0.0) {
-                    const double4 vel_b = vs[rb_s];
-                    const d3 angvel_b = xyz(ws[rb_s]);
+                    const double4 vel_b = S.v[rb_j];
+                    const d3 angvel_b = xyz(S.w[rb_j]);
                     const double4 dt0 = a.delta_t[mp];
                     if (a.tape_dt0) {
                         a.tape_dt0[mp] = dt0;
@@ -1020,7 +1044,8 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                     const d3 dm = shift_of(p, (uint32_t)code);
                     for (uint32_t ib = start; ib < end; ++ib) {
                         if (ib == ia) continue;
-                        const double4 xb = xs[ib];
+                        const uint32_t jb = S.at(ib);
+                        const double4 xb = S.x[jb];
                         const double rb = xb.w;
                         const double kappa = p.kappa;
                         d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
@@ -1028,8 +1053,8 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
                         const double len = length(xab);
                         const double delta = len - (ra + rb);
                         if (delta < 0.0) {
-                            contact_linear_viscous(p, A, vel_a, angvel_a, vs[ib],
-                                                   xyz(ws[ib]), ra, rb, xab, len,
+                            contact_linear_viscous(p, A, vel_a, angvel_a, S.v[jb],
+                                                   xyz(S.w[jb]), ra, rb, xab, len,
                                                    delta, kappa);
                         } else if (delta < p.db) {
                             capillary_cohesion_exp(p, A, ra, rb, delta, xab, len, kappa);
@@ -1399,8 +1424,7 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
 // conservative reject dd >= S^2 (1 + 2^-16).
 constexpr int TCHUNK = 4;
 template <int LPP>
-__device__ __forceinline__ uint32_t test_entries(const DevParams& p,
-                                                 const double4* __restrict__ xs,
+__device__ __forceinline__ uint32_t test_entries(const DevParams& p, const SortedReader& S,
                                                  const double4 xa, uint32_t ia,
                                                  const uint2 (&ev)[NV / LPP], uint32_t want,
                                                  bool tail) {
@@ -1413,7 +1437,7 @@ __device__ __forceinline__ uint32_t test_entries(const DevParams& p,
 #pragma unroll
         for (int k = 0; k < TCHUNK; ++k)
             if (c + k < RMAX)
-                xb[k] = xs[((want >> (c + k)) & 1u) ? (ev[c + k].x & VL_IB_MASK) : ia];
+                xb[k] = S.x[S.at(((want >> (c + k)) & 1u) ? (ev[c + k].x & VL_IB_MASK) : ia)];
 #pragma unroll
         for (int k = 0; k < TCHUNK; ++k) {
             if (c + k >= RMAX) continue;
@@ -1525,12 +1549,15 @@ __device__ __forceinline__ bool group_register_known(const LaneGroup<LPP>& g,
 // Four wavefronts per SIMD (128 VGPRs): the compiler's own allocation takes
 // 131 and three; capping it spills 12 bytes per lane and measured 51.3
 // against 54.9 us on the 1e5-particle sheared bed.
-template <int CM, int LPP, bool FUSED>
-__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_g(
+// WG work-items per workgroup (WG / LPP particles, within one 256-particle
+// chunk).
+template <int CM, int LPP, bool FUSED, int WG>
+__global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_contact_g(
     const DevArrays a) {
     static_assert(CM == 2 || CM == 3, "history models only");
+    static_assert(WG % LPP == 0 && BLOCK % (WG / LPP) == 0, "workgroups inside a chunk");
     constexpr int SPL = NC / LPP;       // slots per lane
-    constexpr int PPB = BLOCK / LPP;    // particles per workgroup
+    constexpr int PPB = WG / LPP;       // particles per workgroup
     const LaneGroup<LPP> g;
     const uint32_t blk = xcd_block(a, blockIdx.x, gridDim.x);
     const uint32_t ia = blk * PPB + threadIdx.x / LPP;
@@ -1540,11 +1567,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     if (ia - p.base >= p.np) return;  // whole groups leave together
     DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
-    const uint32_t cs = cur_set(lp);
-    const double4* __restrict__ xs = a.xs[cs];
-    const double4* __restrict__ vs = a.vs[cs];
-    const double4* __restrict__ ws = a.ws[cs];
-    const double4 xa = xs[ia];
+    const SortedReader S(a, lp);
+    const double4 xa = S.x[S.stale ? ia_orig : ia];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
     // The row passes through LDS: read and written as contiguous words per
@@ -1605,21 +1629,21 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                         bool hit = false, near = false;
                         uint32_t word = 0;
                         if (ib < end && ib != ia) {
-                            const double4 xb = xs[ib];
+                            const double4 xb = S.x[S.at(ib)];
                             const double rb = xb.w;
                             d3 xab = mk3(xa.x - xb.x, xa.y - xb.y, xa.z - xb.z);
                             xab += dm;
                             const double dd = dot(xab, xab);
-                            const double S = ra + rb;
+                            const double Sab = ra + rb;
                             if (build) {
-                                const double Sv = S + p.vl_skin;
+                                const double Sv = Sab + p.vl_skin;
                                 near = dd < Sv * Sv;
                             }
                             // Conservative reject: len32 > S whenever
                             // dd >= S^2 (1 + 2^-16).
-                            if (!(dd >= S * S * 1.0000152587890625)) {
+                            if (!(dd >= Sab * Sab * 1.0000152587890625)) {
                                 const double len = sqrt32(dd);
-                                hit = len - S < 0.0;
+                                hit = len - Sab < 0.0;
                             }
                             if (hit || near) word = pack_word(ib, (uint32_t)code);
                         }
@@ -1640,7 +1664,7 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
                 }
         if (build && g.q == 0) {
             // A new list starts without a hit mask.
-            a.vcount[ia] = make_uint2(nl <= (uint32_t)NV ? nl : VL_OVERFLOW, 0u);
+            a.vcount[ia] = make_uint2(nl <= p.vl_cap ? nl : VL_OVERFLOW, 0u);
             a.x_build[ia_orig] = xa;
         }
     } else {
@@ -1668,10 +1692,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             vcnt > g.q ? ((1u << ((vcnt - g.q + LPP - 1) / LPP)) - 1u) & ((1u << RMAX) - 1u) : 0u;
         const bool tail = __ballot(listed >> TCHUNK) != 0ull;
         // Entries not held: their tests decide whether registration is needed.
-        uint32_t hits = test_entries<LPP>(p, xs, xa, ia, ev, listed & ~held, tail);
+        uint32_t hits = test_entries<LPP>(p, S, xa, ia, ev, listed & ~held, tail);
         quiet = hp_valid && g.bits(hits != 0u) == 0u;
         if (!quiet) {
-            hits |= test_entries<LPP>(p, xs, xa, ia, ev, listed & held, tail);
+            hits |= test_entries<LPP>(p, S, xa, ia, ev, listed & held, tail);
             const uint32_t hits0 = hits;
             // Only overlapping candidates register; the cells of the others
             // cannot matter, as each cell's entries are contiguous.  The
@@ -1716,8 +1740,8 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     // The particle's velocities are loaded only now: the candidate tests
     // above hold many positions in flight, and registers are what bound the
     // number of wavefronts per SIMD here.
-    const double4 vel_a = vs[ia];
-    const double4 av_a = ws[ia];
+    const double4 vel_a = S.v[S.stale ? ia_orig : ia];
+    const double4 av_a = S.w[S.stale ? ia_orig : ia];
     const d3 angvel_a = xyz(av_a);
     tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
 
@@ -1767,9 +1791,10 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
             const d3 dm = shift_of(p, word_code(wrd));
             // A slot's contact has almost always persisted: its partner's
             // velocities and the displacement are loaded with its position.
-            const double4 xb = xs[rb_s];
-            const double4 vel_b = vs[rb_s];
-            const double4 av_b = ws[rb_s];
+            const uint32_t rb_j = S.at(rb_s);
+            const double4 xb = S.x[rb_j];
+            const double4 vel_b = S.v[rb_j];
+            const double4 av_b = S.w[rb_j];
             const double4 dt0 = a.delta_t[mp];
             const double rb = xb.w;
             const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
@@ -2065,7 +2090,7 @@ __device__ __forceinline__ unsigned long long composite(uint32_t key, uint32_t k
 
 // Re-sort simulation p's segment [B, B + N) of the sorted order (N = npad:
 // its particles, then its padding, whose keys never change).
-__device__ void resort_one_block(const DevParams& p, const DevArrays& a, int passes,
+__device__ void resort_one_block(const DevParams& p, const DevArrays& a, int passes, int bits,
                                  int sorted_bits) {
     __shared__ unsigned long long comp[MV_CAP];  // movers' (new key, index), ascending
     __shared__ uint32_t rk[MV_CAP];              // movers' old positions, ascending
@@ -2075,8 +2100,8 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
     const DevLoop* lp = a.loop + p.sim;
     const uint32_t* mv = a.movers + (size_t)p.sim * MV_CAP;
     const uint32_t M = lp->nmovers;
-    // The radix sort orders by the low 8 * passes bits of the key.
-    const uint32_t kmask = passes >= 4 ? 0xffffffffu : ((1u << (8 * passes)) - 1u);
+    // The radix sort orders by the low bits * passes bits of the key.
+    const uint32_t kmask = bits * passes >= 32 ? 0xffffffffu : ((1u << (bits * passes)) - 1u);
     const bool merge = M <= p.mv_max && M <= MV_CAP;
     // Keep the order and cell ranges the finished step used.
     for (uint32_t j = tid; j < N; j += BLOCK) {
@@ -2158,20 +2183,25 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
         uint32_t* hist = a.rhist + p.rhist_base;
         for (int pass = 0; pass < passes; ++pass) {
             const RadixPass r = radix_pass(a, pass, passes, B);
-            const int shift = 8 * pass;
-            for (uint32_t t = 0; t < nt; ++t) radix_hist_tile(r.kin, N, shift, hist, nt, t);
-            __syncthreads();
-            radix_scan_block<BLOCK>(hist, 256u * nt);
+            const int shift = bits * pass;
             for (uint32_t t = 0; t < nt; ++t)
-                radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, N, shift, hist, nt, t, B);
+                radix_hist_tile(r.kin, N, shift, bits, hist, nt, t);
+            __syncthreads();
+            radix_scan_block<BLOCK>(hist, (1u << bits) * nt);
+            for (uint32_t t = 0; t < nt; ++t)
+                radix_scatter_tile(r.kin, r.vin, r.kout, r.vout, N, shift, bits, hist, nt, t, B);
             __syncthreads();
         }
     }
     for (uint32_t c = tid; c < p.ncells; c += BLOCK) a.cell_start[p.cell_base + c] = 0xffffffffu;
     __syncthreads();
-    // For the next step: the loop counter advances after this re-sort.
+    // For the next step (the loop counter advances after this re-sort).  The
+    // sorted copies are left in the previous order: gathering them here, in
+    // one workgroup, would cost far more than the next step's contact pass
+    // reading the original-index arrays through perm (copies_stale), after
+    // which the integration pass writes them in the new order.
     const uint32_t nxt = cur_set(a.loop + p.sim) ^ 1u;
-    for (uint32_t j = tid; j < N; j += BLOCK) cells_one(p, a, B + j, sorted_bits, nxt);
+    for (uint32_t j = tid; j < N; j += BLOCK) cells_one(p, a, B + j, sorted_bits, nxt, false);
     __syncthreads();
 }
 
@@ -2181,7 +2211,7 @@ __device__ void resort_one_block(const DevParams& p, const DevArrays& a, int pas
 // set it first re-sorts for the next step when the integration pass found
 // particles that changed cell (the next step then launches no sort).
 __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, int passes,
-                                                 int sorted_bits) {
+                                                 int bits, int sorted_bits) {
     __shared__ double stage[2048];
     __shared__ uint32_t do_sort;
     const DevParams& p = a.ptab[blockIdx.x];  // one workgroup per simulation
@@ -2217,7 +2247,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
         if (threadIdx.x == 0) s_fsum[k] = fs;
     }
     __syncthreads();
-    if (do_sort) resort_one_block(p, a, passes, sorted_bits);
+    if (do_sort) resort_one_block(p, a, passes, bits, sorted_bits);
     if (mine) {
         const uint32_t wi = p.wall_base + w;
         w_mvfd.z = s_fsum[p.wall_sums ? w : 0u];  // 0.0 + partial[0] + partial[1] + ...
@@ -2233,8 +2263,10 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     if (threadIdx.x != 0) return;
     if (do_sort) lp->sorts += 1u;
     lp->resorted_last = do_sort;
-    // The next step's contact pass maps its rows to the new order.
+    // The next step's contact pass maps its rows to the new order and reads
+    // the state through perm.
     lp->order_changed = do_sort;
+    lp->copies_stale = do_sort;
     if (p.adaptive == 1 && walls) lp->grid_Lz = grid_top;
     if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
     else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
@@ -2429,18 +2461,18 @@ hipError_t launch_hash(const DevArrays& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_sort(const DevArrays& a, int passes, hipStream_t s) {
+hipError_t launch_sort(const DevArrays& a, int passes, int bits, hipStream_t s) {
     const uint32_t n = a.ntot;
     const uint32_t nt = n_tiles(n);
     for (int pass = 0; pass < passes; ++pass) {
         const RadixPass r = radix_pass(a, pass, passes, 0u);
-        const int shift = 8 * pass;
+        const int shift = bits * pass;
         hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(BLOCK), 0, s, a.glob, a.loop, r.kin, n,
-                           shift, a.rhist, nt);
+                           shift, bits, a.rhist, nt);
         hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, s, a.glob, a.loop, a.rhist,
-                           256u * nt);
+                           (1u << bits) * nt);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(BLOCK), 0, s, a.glob, a.loop, r.kin,
-                           r.vin, r.kout, r.vout, n, shift, a.rhist, nt);
+                           r.vin, r.kout, r.vout, n, shift, bits, a.rhist, nt);
     }
     return hipGetLastError();
 }
@@ -2453,19 +2485,35 @@ hipError_t launch_cells(const DevArrays& a, int sorted_bits, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipStream_t s) {
+template <int CM>
+void launch_contact_lanes(const DevArrays& a, int lanes, int wg, hipStream_t s) {
+    // BLOCK / (wg / lanes) workgroups per 256-particle chunk.
+    if (lanes == 4) {
+        const unsigned per = (unsigned)(BLOCK * 4 / wg);
+        if (wg == 128)
+            hipLaunchKernelGGL((k_contact_g<CM, 4, false, 128>), dim3(a.nchunk * per), dim3(128),
+                               0, s, a);
+        else if (wg == 512)
+            hipLaunchKernelGGL((k_contact_g<CM, 4, false, 512>), dim3(a.nchunk * per), dim3(512),
+                               0, s, a);
+        else
+            hipLaunchKernelGGL((k_contact_g<CM, 4, false, 256>), dim3(a.nchunk * per), dim3(256),
+                               0, s, a);
+    } else if (lanes == 8) {
+        hipLaunchKernelGGL((k_contact_g<CM, 8, false, BLOCK>), dim3(a.nchunk * 8u), dim3(BLOCK), 0,
+                           s, a);
+    } else {
+        hipLaunchKernelGGL((k_contact_g<CM, 16, false, BLOCK>), dim3(a.nchunk * 16u), dim3(BLOCK),
+                           0, s, a);
+    }
+}
+
+hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, int wg,
+                          hipStream_t s) {
     const dim3 b(BLOCK);
     if (lanes > 1 && (contactmodel == 2 || contactmodel == 3)) {
-        const dim3 g(a.nchunk * (unsigned)lanes);  // BLOCK / lanes particles per workgroup
-        if (contactmodel == 2) {
-            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<2, 4, false>), g, b, 0, s, a);
-            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<2, 8, false>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_contact_g<2, 16, false>), g, b, 0, s, a);
-        } else {
-            if (lanes == 4) hipLaunchKernelGGL((k_contact_g<3, 4, false>), g, b, 0, s, a);
-            else if (lanes == 8) hipLaunchKernelGGL((k_contact_g<3, 8, false>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_contact_g<3, 16, false>), g, b, 0, s, a);
-        }
+        if (contactmodel == 2) launch_contact_lanes<2>(a, lanes, wg, s);
+        else launch_contact_lanes<3>(a, lanes, wg, s);
         return hipGetLastError();
     }
     const dim3 g(a.nchunk);
@@ -2483,9 +2531,9 @@ hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipSt
 hipError_t launch_contact_integrate(const DevArrays& a, int contactmodel, hipStream_t s) {
     const dim3 b(BLOCK), g(a.nchunk * 4u);
     if (contactmodel == 2)
-        hipLaunchKernelGGL((k_contact_g<2, 4, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((k_contact_g<2, 4, true, BLOCK>), g, b, 0, s, a);
     else
-        hipLaunchKernelGGL((k_contact_g<3, 4, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((k_contact_g<3, 4, true, BLOCK>), g, b, 0, s, a);
     return hipGetLastError();
 }
 
@@ -2517,9 +2565,10 @@ hipError_t launch_integrate_q(const DevArrays& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
+hipError_t launch_walls(const DevArrays& a, int resort, int passes, int bits, int sorted_bits,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_walls, dim3(a.nsim), dim3(BLOCK), 0, s, a, resort, passes, sorted_bits);
+    hipLaunchKernelGGL(k_walls, dim3(a.nsim), dim3(BLOCK), 0, s, a, resort, passes, bits,
+                       sorted_bits);
     return hipGetLastError();
 }
 

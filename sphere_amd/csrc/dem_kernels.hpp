@@ -26,13 +26,14 @@ hipError_t launch_hash(const DevArrays& a, hipStream_t s);
 // Stable LSD radix sort of (key, identity) into (skey, perm), `passes` digits
 // of 8 bits (replaces thrust::sort_by_key, src/device.cu:1075-1078), one
 // launch per stage, over the whole batch (keys are global).
-hipError_t launch_sort(const DevArrays& a, int passes, hipStream_t s);
+hipError_t launch_sort(const DevArrays& a, int passes, int bits, hipStream_t s);
 // cellStart reset, cellStart/cellEnd, sorted copies
 // (src/device.cu:1089-1090, src/sorting.cuh:56-132).
 hipError_t launch_cells(const DevArrays& a, int sorted_bits, hipStream_t s);
 // topology + interact fused per particle (src/contactsearch.cuh:194-726);
 // `lanes` (1, 4, 8 or 16) work-items per particle for contact models 2 and 3.
-hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, hipStream_t s);
+hipError_t launch_contact(const DevArrays& a, int contactmodel, int lanes, int wg,
+                          hipStream_t s);
 // bondsLinear per bond, then a per-particle gather in bond order
 // (src/cohesion.cuh:8-233).
 // Contact pass (models 2 and 3, four lanes per particle) that also
@@ -56,7 +57,7 @@ hipError_t launch_integrate_q(const DevArrays& a, hipStream_t s);
 // it also re-sorts the simulation for the next step, in one workgroup, when
 // the integration pass (launched with hash_next) found a cell change.
 hipError_t launch_check_sqrt32(unsigned long long* bad, uint64_t lo, uint64_t hi, hipStream_t s);
-hipError_t launch_walls(const DevArrays& a, int resort, int passes, int sorted_bits,
+hipError_t launch_walls(const DevArrays& a, int resort, int passes, int bits, int sorted_bits,
                         hipStream_t s);
 // Fill n words with v.
 hipError_t launch_fill_u32(uint32_t* dst, unsigned n, uint32_t v, hipStream_t s);

@@ -85,6 +85,7 @@ struct DevParams {
     double vl_skin;            // list radius margin (m); 0 disables the lists
     double vl_trigger2;        // squared displacement that forces a rebuild
     int32_t vl_enabled;
+    uint32_t vl_cap;           // list capacity in use (<= NV); beyond it the full scan
     // Largest number of particles changing cell that the one-workgroup
     // re-sort merges into the previous order (<= MV_CAP); beyond it, it
     // sorts from scratch.
@@ -135,6 +136,9 @@ struct DevLoop {
     uint32_t nmovers;           // particles whose key changed for the next step
     uint32_t resorted_last;     // the last step ended with the next step's re-sort
     uint32_t order_changed;     // the order changed since the contact rows were written
+    uint32_t copies_stale;      // the sorted copies are in the previous order (read
+                                // the original-index arrays through perm instead)
+    uint32_t pad1;
 };
 
 // Batch-wide state.

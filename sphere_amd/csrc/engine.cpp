@@ -110,6 +110,22 @@ struct Sim {
     int64_t iter = 0;
 };
 
+// The adjoint's host bookkeeping at a checkpoint: the loop mirrors and the
+// hash/re-sort state enqueue_step reads.
+struct AdjHostMark {
+    std::vector<double> t;
+    std::vector<int64_t> iter;
+    bool hash_pending, prev_walls_resorted;
+};
+
+// A recorded forward pass awaiting its reverse (sph_adjoint_record).
+struct AdjPlan {
+    uint64_t nsteps = 0, every = 0, K = 0;
+    int64_t iter0 = 0, iter_end = 0;
+    void* buf = nullptr;
+    std::vector<AdjHostMark> marks;
+};
+
 struct sph_ctx {
     std::vector<Sim> sims;  // one for sph_open, several for sph_open_batch
     Host& h0() { return sims[0].h; }
@@ -134,6 +150,8 @@ struct sph_ctx {
     bool hash_pending = true;
     // Work-items per particle in the contact kernel (models 2 and 3).
     int contact_lanes = 4;
+    int contact_wg = 256;     // SPH_S_CONTACT_WG (4 lanes)
+    int radix_bits = 8;       // SPH_S_RADIX_BITS
     // Where a step's re-sort runs: 0 = the per-stage sort launches at the
     // step start (every step; they return at once when nothing changed),
     // 1 = one workgroup per simulation at the end of the previous step, only
@@ -163,6 +181,7 @@ struct sph_ctx {
     // segment length (0: automatic).
     void* adj_buf = nullptr;
     size_t adj_bytes = 0;
+    AdjPlan* adj_plan = nullptr;  // a recorded forward pass awaiting its reverse
     uint64_t adj_checkpoint = 0;
     // Contact and integration in one pass where the step allows it (off by
     // default: measured slower than the two kernels, 72.1 + 6.2 us for the
@@ -425,6 +444,7 @@ void fill_params(Sim& sim) {
     for (uint32_t i = 0; i < h.np; ++i)
         if (i == 0 || h.x[4 * i + 3] < rmin) rmin = h.x[4 * i + 3];
     set_skin(P, (h.contactmodel == 1 && h.db > 0.0) ? 0.0 : 0.1 * rmin);
+    P.vl_cap = (uint32_t)NV;
 }
 
 // Place every simulation in the batch arrays: whole 256-particle chunks of
@@ -467,7 +487,7 @@ void layout(sph_ctx* c) {
     A.xcd_order = 0;
     c->sorted_bits = std::max(1, bit_length(keys));
     if (c->sorted_bits > 32) c->sorted_bits = 32;
-    c->passes = (c->sorted_bits + 7) / 8;
+    c->passes = (c->sorted_bits + c->radix_bits - 1) / c->radix_bits;
     c->shear_stress = shear;
     c->any_adaptive = adaptive;
     c->contactmodel = c->sims[0].h.contactmodel;
@@ -974,7 +994,7 @@ int enqueue_step(sph_ctx* c) {
     // unless this step hashes for itself.
     if (!(c->prev_walls_resorted && !hash_now)) {
         pr.begin("sort");
-        HIPCHK(c, launch_sort(A, c->passes, s), "sort");
+        HIPCHK(c, launch_sort(A, c->passes, c->radix_bits, s), "sort");
         pr.end();
         pr.begin("cells");
         HIPCHK(c, launch_cells(A, c->sorted_bits, s), "cells");
@@ -995,7 +1015,8 @@ int enqueue_step(sph_ctx* c) {
         pr.end();
     } else {
         pr.begin("contact");
-        HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, s), "contact");
+        HIPCHK(c, launch_contact(A, (int)c->contactmodel, c->contact_lanes, c->contact_wg, s),
+               "contact");
         pr.end();
     }
     if (!fused && A.nbond > 0) {
@@ -1026,7 +1047,8 @@ int enqueue_step(sph_ctx* c) {
     }
     const bool resort = hash_in_integrate && c->resort_at_end;
     pr.begin("walls");
-    HIPCHK(c, launch_walls(A, resort ? 1 : 0, c->passes, c->sorted_bits, s), "walls");
+    HIPCHK(c, launch_walls(A, resort ? 1 : 0, c->passes, c->radix_bits, c->sorted_bits, s),
+           "walls");
     pr.end();
     c->prev_walls_resorted = resort;
     // Host mirror of the loop counters (src/device.cu:2536-2538).
@@ -1890,7 +1912,10 @@ int sph_get_scalar_sim(sph_ctx* c, int sim, int which, double* v) {
         case SPH_S_NB0: *v = S.h.nb0; return SPH_OK;
         case SPH_S_NCELLS: *v = S.P.ncells; return SPH_OK;
         case SPH_S_VL_SKIN: *v = S.P.vl_skin; return SPH_OK;
+        case SPH_S_VL_CAP: *v = S.P.vl_cap; return SPH_OK;
         case SPH_S_CONTACT_LANES: *v = c->contact_lanes; return SPH_OK;
+        case SPH_S_CONTACT_WG: *v = c->contact_wg; return SPH_OK;
+        case SPH_S_RADIX_BITS: *v = c->radix_bits; return SPH_OK;
         case SPH_S_SORT_MODE: *v = c->sort_mode; return SPH_OK;
         case SPH_S_RESORT_MERGE_MAX: *v = S.P.mv_max; return SPH_OK;
         case SPH_S_NSIMS: *v = (double)c->sims.size(); return SPH_OK;
@@ -1953,6 +1978,29 @@ int sph_set_scalar_sim(sph_ctx* c, int sim, int which, double v) {
             if (rc) return rc;
             return set_loop_flag(c, sim, &DevLoop::force_rebuild);
         }
+        case SPH_S_VL_CAP: {
+            if (!(v >= 8.0 && v <= (double)NV) || v != (double)(int)v)
+                return fail(c, SPH_E_ARG, "list capacity must be a whole number in [8, %d] "
+                            "(got %g)", NV, v);
+            S.P.vl_cap = (uint32_t)v;
+            int rc = upload_params(c);
+            if (rc) return rc;
+            return set_loop_flag(c, sim, &DevLoop::force_rebuild);
+        }
+        case SPH_S_CONTACT_WG:
+            if (v != 128.0 && v != 256.0 && v != 512.0)
+                return fail(c, SPH_E_ARG, "contact workgroup size must be 128, 256 or 512 "
+                            "(got %g)", v);
+            if (c->contact_wg != (int)v) drop_graphs(c);
+            c->contact_wg = (int)v;
+            return SPH_OK;
+        case SPH_S_RADIX_BITS:
+            if (v != 4.0 && v != 6.0 && v != 8.0)
+                return fail(c, SPH_E_ARG, "radix digit width must be 4, 6 or 8 bits (got %g)", v);
+            if (c->radix_bits != (int)v) drop_graphs(c);
+            c->radix_bits = (int)v;
+            c->passes = (c->sorted_bits + c->radix_bits - 1) / c->radix_bits;
+            return SPH_OK;
         case SPH_S_GRAPH_STEPS:
             if (!(v >= 0.0 && v <= 4096.0))
                 return fail(c, SPH_E_ARG, "graph steps must be in [0, 4096] (got %g)", v);
@@ -2050,11 +2098,28 @@ int sph_get_cells(sph_ctx* c, uint32_t* cell_id, uint32_t* perm, uint32_t* cell_
 // every > 0 the (nsteps / every) x rows x 3 arrays sx / sv on the positions
 // / velocities after every, 2 every, ... steps (any may be null).
 // g_params receives nsims x ADJ_NPARAM values.
+// phase 0 runs the forward and the reverse pass; phase 1 only the forward
+// pass (recording, or saving checkpoints), with the positions after every,
+// 2 every, ... steps into samples_x (nsamples x rows x 3, may be null), and
+// keeps what the reverse pass needs in c->adj_plan; phase 2 only the
+// reverse pass of the run phase 1 recorded, which must be the context's
+// last operation.
 static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
                        const double* dL_dw, uint64_t every, const double* sx,
                        const double* sv, double* g_x0, double* g_v0, double* g_w0,
-                       double* g_params) {
+                       double* g_params, int phase = 0, double* samples_x = nullptr) {
     if (!c) return SPH_E_ARG;
+    AdjPlan* plan = nullptr;
+    if (phase == 2) {
+        plan = c->adj_plan;
+        if (!plan || plan->nsteps != nsteps || plan->every != every ||
+            plan->iter_end != c->sims[0].iter || plan->buf != c->adj_buf)
+            return fail(c, SPH_E_ARG, "no recorded forward pass of this run to reverse (the "
+                        "context must not step or record again in between)");
+    } else {
+        delete c->adj_plan;
+        c->adj_plan = nullptr;
+    }
     const size_t nsim = c->sims.size();
     size_t rows = 0, nwt = 0;
     for (const Sim& S : c->sims) {
@@ -2115,7 +2180,8 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
                (nseg > 1 ? (double)(nseg + 1) * (double)snap_b : 0.0);
     };
     uint64_t K = c->adj_checkpoint > 0 ? std::min<uint64_t>(c->adj_checkpoint, nsteps) : nsteps;
-    if (c->adj_checkpoint == 0 && need_for(K) > avail) {
+    if (plan) K = plan->K;
+    else if (c->adj_checkpoint == 0 && need_for(K) > avail) {
         K = 0;
         for (uint64_t k = nsteps; k >= 1; k = k > 1 ? k * 3 / 4 : 0) {
             if (need_for(k) <= avail) {
@@ -2130,7 +2196,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     }
     const uint64_t nseg = (nsteps + K - 1) / K;
     const double need = need_for(K);
-    if (need > avail)
+    if (!plan && need > avail)
         return fail(c, SPH_E_ARG, "recording %llu steps in segments of %llu needs %.1f GB of "
                     "device memory", (unsigned long long)nsteps, (unsigned long long)K,
                     need / 1e9);
@@ -2184,7 +2250,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
             r.bond_full = reinterpret_cast<uint32_t*>(q);
         }
     }
-    const int64_t iter0 = c->sims[0].iter;
+    const int64_t iter0 = plan ? plan->iter0 : c->sims[0].iter;
     // Reverse-pass state, and the seeds (uploaded before the forward pass,
     // so that only the gradients' download waits for the device).
     char* q = tape + K * per_step;
@@ -2236,7 +2302,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     const double* seeds[3] = {dL_dx, dL_dv, dL_dw};
     int rc = SPH_OK;
     hipError_t e = hipSuccess;
-    for (int f = 0; f < 3 && !rc; ++f) {
+    for (int f = 0; f < 3 && !rc && phase != 1; ++f) {
         if (!seeds[f]) {
             e = hipMemsetAsync(L[0][f], 0, ntot * 32, s);
             if (e != hipSuccess) return hip_fail(c, e, "memset");
@@ -2249,20 +2315,20 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
         }
     }
     if (rc) return rc;
-    e = hipMemsetAsync(LA[0], 0, 2 * ntot * 32, s);  // LA[0], LAA[0]
-    if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, nsim * ADJ_NPARAM * 8, s);
-    if (e == hipSuccess) e = hipMemsetAsync(g.wall, 0, nwt * ADJ_WALL * 8 + 8, s);
-    if (e == hipSuccess && nb) e = hipMemsetAsync(lb0, 0, 4 * nb * sizeof(double4), s);
-    if (e != hipSuccess) return hip_fail(c, e, "memset");
+    if (phase != 1) {
+        e = hipMemsetAsync(LA[0], 0, 2 * ntot * 32, s);  // LA[0], LAA[0]
+        if (e == hipSuccess) e = hipMemsetAsync(g.gparam, 0, nsim * ADJ_NPARAM * 8, s);
+        if (e == hipSuccess) e = hipMemsetAsync(g.wall, 0, nwt * ADJ_WALL * 8 + 8, s);
+        if (e == hipSuccess && nb) e = hipMemsetAsync(lb0, 0, 4 * nb * sizeof(double4), s);
+        if (e != hipSuccess) return hip_fail(c, e, "memset");
+    }
 
     // Checkpoints: the device allocations, and the host state enqueue_step
     // reads (the loop mirrors and the hash/re-sort bookkeeping).
-    struct HostMark {
-        std::vector<double> t;
-        std::vector<int64_t> iter;
-        bool hash_pending, prev_walls_resorted;
-    };
-    std::vector<HostMark> marks(nseg > 1 ? nseg + 1 : 0);
+    std::vector<AdjHostMark> marks;
+    if (plan) marks = plan->marks;
+    else marks.resize(nseg > 1 ? nseg + 1 : 0);
+    using HostMark = AdjHostMark;
     size_t crow_off = 0, prow_off = 0;
     for (size_t k = 0, off = 0; k < c->allocs.size(); off += up(c->alloc_bytes[k]), ++k) {
         if (c->allocs[k] == (void*)A.crow) crow_off = off;
@@ -2303,7 +2369,23 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
     // Forward steps [k0, k0 + n) with the tape; the contact pass of each step
     // records the state and walls the step starts from, its rows and the
     // displacements it read, the bond pass the bonds' inputs.
-    auto record = [&](uint64_t n) -> int {
+    // Positions after global step gk (phase 1 with samples_x): sample m =
+    // (gk + 1) / every - 1, the simulations' particles concatenated.
+    std::vector<double> xtmp;
+    auto sample = [&](uint64_t gk) -> int {
+        if (phase != 1 || !samples_x || every == 0 || (gk + 1) % every != 0) return SPH_OK;
+        double* dst = samples_x + ((gk + 1) / every - 1) * rows * 3;
+        for (const Sim& S : c->sims) {
+            const size_t np = S.h.np;
+            xtmp.resize(np * 4);
+            const int rc_ = np ? d2h(c, xtmp.data(), A.x + S.P.base, np * 32) : SPH_OK;
+            if (rc_) return rc_;
+            for (size_t i = 0; i < np; ++i)
+                for (int d = 0; d < 3; ++d) dst[(S.P.hbase + i) * 3 + d] = xtmp[4 * i + d];
+        }
+        return SPH_OK;
+    };
+    auto record = [&](uint64_t n, uint64_t k0) -> int {
         int rc_ = SPH_OK;
         for (uint64_t k = 0; k < n && !rc_; ++k) {
             const Rec& r = rec[k];
@@ -2324,6 +2406,7 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
             A.tape_dt0 = A.tape_x = A.tape_v = A.tape_w = A.tape_wnx = A.tape_wmv = nullptr;
             A.tape_bond_in = nullptr;
             A.tape_bond_idx = nullptr;
+            if (!rc_) rc_ = sample(k0 + k);
         }
         return rc_;
     };
@@ -2381,21 +2464,49 @@ static int adjoint_run(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const d
         }
         return SPH_OK;
     };
+    // The forward pass: the whole tape, or checkpoints (m holds the state
+    // before step m K; m = nseg the end).
+    if (phase != 2) {
+        if (nseg == 1) {
+            rc = record(nsteps, 0);
+        } else {
+            for (uint64_t m = 0; m < nseg && !rc; ++m) {
+                rc = checkpoint(m, true);
+                const uint64_t n = std::min<uint64_t>(K, nsteps - m * K);
+                for (uint64_t k = 0; k < n && !rc; ++k) {
+                    rc = enqueue_step(c);
+                    if (!rc) rc = sample(m * K + k);
+                }
+            }
+            if (!rc) rc = checkpoint(nseg, true);
+        }
+    }
+    if (phase == 1) {
+        if (rc) return rc;
+        HIPCHK(c, hipStreamSynchronize(s), "sync");
+        AdjPlan* pl = new AdjPlan();
+        pl->nsteps = nsteps;
+        pl->every = every;
+        pl->K = K;
+        pl->iter0 = iter0;
+        pl->iter_end = c->sims[0].iter;
+        pl->buf = c->adj_buf;
+        pl->marks = std::move(marks);
+        c->adj_plan = pl;
+        return SPH_OK;
+    }
+    if (phase == 2) {
+        delete c->adj_plan;
+        c->adj_plan = nullptr;
+    }
+    // The reverse pass.
     if (nseg == 1) {
-        rc = record(nsteps);
         if (!rc) rc = reverse(0, nsteps, nullptr);
     } else {
-        // Checkpoint m holds the state before step m K (m = nseg: the end).
-        for (uint64_t m = 0; m < nseg && !rc; ++m) {
-            rc = checkpoint(m, true);
-            const uint64_t n = std::min<uint64_t>(K, nsteps - m * K);
-            for (uint64_t k = 0; k < n && !rc; ++k) rc = enqueue_step(c);
-        }
-        if (!rc) rc = checkpoint(nseg, true);
         for (uint64_t m = nseg; m-- > 0 && !rc;) {
             const uint64_t n = std::min<uint64_t>(K, nsteps - m * K);
             rc = checkpoint(m, false);
-            if (!rc) rc = record(n);
+            if (!rc) rc = record(n, m * K);
             // The rows before step m K: those the checkpoint's contact rows
             // hold, with original indices.
             const uint32_t* prev0 = nullptr;
@@ -2465,6 +2576,24 @@ int sph_adjoint_samples(sph_ctx* c, uint64_t every, uint64_t nsamples, const dou
                        g_x0, g_v0, g_w0, g_params);
 }
 
+int sph_adjoint_record(sph_ctx* c, uint64_t every, uint64_t nsamples, double* samples_x) {
+    if (!c) return SPH_E_ARG;
+    const int rc = check_samples(c, every, nsamples);
+    if (rc) return rc;
+    return adjoint_run(c, every * nsamples, nullptr, nullptr, nullptr, every, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr, 1, samples_x);
+}
+
+int sph_adjoint_reverse(sph_ctx* c, uint64_t every, uint64_t nsamples, const double* dL_dx,
+                        const double* dL_dv, double* g_x0, double* g_v0, double* g_w0,
+                        double* g_params) {
+    if (!c) return SPH_E_ARG;
+    const int rc = check_samples(c, every, nsamples);
+    if (rc) return rc;
+    return adjoint_run(c, every * nsamples, nullptr, nullptr, nullptr, every, dL_dx, dL_dv,
+                       g_x0, g_v0, g_w0, g_params, 2, nullptr);
+}
+
 int sph_adjoint_batch(sph_ctx* c, uint64_t nsteps, const double* dL_dx, const double* dL_dv,
                       const double* dL_dw, double* g_x0, double* g_v0, double* g_w0,
                       double* g_params) {
@@ -2486,54 +2615,107 @@ int sph_adjoint_samples_batch(sph_ctx* c, uint64_t every, uint64_t nsamples,
 int sph_autotune(sph_ctx* c, uint64_t steps_per_trial, double* us_per_step) {
     if (!c || steps_per_trial == 0) return SPH_E_ARG;
     // Every setting tried gives the same results bit for bit, so the trials
-    // are ordinary steps of the simulation: only their wall time differs.
-    // A trial first runs half as many untimed steps (graph capture, the
-    // list rebuild a new skin forces), and a setting replaces the incumbent
-    // only when at least 3% faster (re-sorts and rebuilds make short trials
-    // noisy).
+    // are ordinary steps of the simulation; only the kernels' times differ.
+    // The contact settings are judged by the contact kernel's mean time
+    // (HIP events around each launch), which the occasional re-sort or list
+    // rebuild of a trial does not disturb; the radix digit width by the
+    // time of full sorts of the current keys (which reproduce the current
+    // order).  A trial first runs half as many untimed steps (the list
+    // rebuild a new capacity forces), and a setting replaces the incumbent
+    // only when at least 3% faster.
     const uint64_t warm = std::max<uint64_t>(1, steps_per_trial / 2);
-    auto trial = [&](double* us) -> int {
+    const bool history = c->sims[0].h.contactmodel == 2 || c->sims[0].h.contactmodel == 3;
+    auto contact_trial = [&](double* us) -> int {
         int rc = sph_step(c, warm);
         if (!rc) rc = sph_sync(c);
         if (rc) return rc;
-        const auto t0 = std::chrono::steady_clock::now();
+        prof_collect(c);
+        c->prof_names.clear();
+        c->prof_ms.clear();
+        c->prof_n.clear();
+        const bool was = c->profile;
+        c->profile = true;
         rc = sph_step(c, steps_per_trial);
         if (!rc) rc = sph_sync(c);
-        const auto t1 = std::chrono::steady_clock::now();
-        *us = std::chrono::duration<double, std::micro>(t1 - t0).count() / (double)steps_per_trial;
-        return rc;
+        prof_collect(c);
+        c->profile = was;
+        if (rc) return rc;
+        *us = 0.0;
+        for (size_t k = 0; k < c->prof_names.size(); ++k)
+            if ((c->prof_names[k] == "contact" || c->prof_names[k] == "contact_integrate") &&
+                c->prof_n[k] > 0)
+                *us = 1000.0 * c->prof_ms[k] / (double)c->prof_n[k];
+        return *us > 0.0 ? SPH_OK : fail(c, SPH_E_DEVICE, "autotune: no contact launches timed");
     };
-    const bool history = c->sims[0].h.contactmodel == 2 || c->sims[0].h.contactmodel == 3;
+    auto sort_trial = [&](double* us) -> int {
+        // Full sorts of the keys of the next step: the same permutation and
+        // cell ranges the step would use; the flag that lets the sort
+        // kernels run is the one the next step consumes and retires.
+        hipStream_t s = c->stream;
+        const uint32_t slot = (uint32_t)(c->sims[0].iter & 1);
+        const uint32_t one = 1u;
+        hipEvent_t e0 = take_event(c), e1 = take_event(c);
+        constexpr int REPS = 4;
+        float total = 0.0f;
+        for (int r = 0; r < REPS; ++r) {
+            HIPCHK(c, hipMemcpyAsync(&c->A.glob->changed[slot], &one, 4, hipMemcpyHostToDevice,
+                                     s), "flag");
+            HIPCHK(c, hipEventRecord(e0, s), "event");
+            HIPCHK(c, launch_sort(c->A, c->passes, c->radix_bits, s), "sort");
+            HIPCHK(c, launch_cells(c->A, c->sorted_bits, s), "cells");
+            HIPCHK(c, hipEventRecord(e1, s), "event");
+            HIPCHK(c, hipEventSynchronize(e1), "sync");
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            total += ms;
+        }
+        prof_state(c).pool.push_back(e0);
+        prof_state(c).pool.push_back(e1);
+        *us = 1000.0 * total / REPS;
+        return SPH_OK;
+    };
+    int rc = SPH_OK;
+    // Coordinate search over one knob at a time: each candidate value is
+    // tried from the incumbent settings.  (The candidate-list skin is not
+    // tuned here: its cost is the rebuild rate, which short trials do not
+    // see.)  The list capacity is set on every simulation of the context.
+    auto set_all = [&](int which, double v) -> int {
+        if (which == SPH_S_VL_CAP) {
+            for (size_t k = 0; k < c->sims.size(); ++k) {
+                const int r = sph_set_scalar_sim(c, (int)k, which, v);
+                if (r) return r;
+            }
+            return SPH_OK;
+        }
+        return sph_set_scalar(c, which, v);
+    };
     double best = 0.0;
-    int rc = trial(&best);  // the current settings
-    // Coordinate search: contact lanes, then the workgroup order.  (The
-    // candidate-list skin is not tuned here: its cost is the rebuild rate,
-    // which short trials do not see.)
-    if (!rc && history) {
-        int best_lanes = c->contact_lanes;
-        for (int lanes : {4, 8, 16}) {
-            if (lanes == best_lanes || rc) continue;
-            const int keep = c->contact_lanes;
-            rc = sph_set_scalar(c, SPH_S_CONTACT_LANES, lanes);
+    auto search = [&](int which, std::initializer_list<double> values, bool sort) {
+        double cur = 0.0;
+        if (rc || (rc = sph_get_scalar(c, which, &cur))) return;
+        rc = sort ? sort_trial(&best) : contact_trial(&best);  // the incumbent
+        for (double v : values) {
+            if (v == cur || rc) continue;
+            rc = set_all(which, v);
             double t = 0.0;
-            if (!rc) rc = trial(&t);
+            if (!rc) rc = sort ? sort_trial(&t) : contact_trial(&t);
             if (!rc && t < 0.97 * best) {
                 best = t;
-                best_lanes = lanes;
+                cur = v;
             } else if (!rc) {
-                rc = sph_set_scalar(c, SPH_S_CONTACT_LANES, keep);
+                rc = set_all(which, cur);
             }
         }
+    };
+    if (history) {
+        search(SPH_S_CONTACT_LANES, {4.0, 8.0, 16.0}, false);
+        if (!rc && c->contact_lanes == 4) search(SPH_S_CONTACT_WG, {128.0, 256.0, 512.0}, false);
+        search(SPH_S_VL_CAP, {16.0, 24.0, 32.0}, false);
     }
-    if (!rc) {
-        const uint32_t keep = c->A.xcd_order;
-        rc = sph_set_scalar(c, SPH_S_XCD_ORDER, keep ? 0.0 : 1.0);
-        double t = 0.0;
-        if (!rc) rc = trial(&t);
-        if (!rc && t < 0.97 * best) best = t;
-        else if (!rc) rc = sph_set_scalar(c, SPH_S_XCD_ORDER, keep);
-    }
-    if (!rc && us_per_step) *us_per_step = best;
+    search(SPH_S_XCD_ORDER, {0.0, 1.0}, false);
+    double contact_best = best;
+    search(SPH_S_RADIX_BITS, {4.0, 6.0, 8.0}, true);
+    if (!rc && us_per_step) *us_per_step = contact_best;
     return rc;
 }
 
@@ -2574,6 +2756,7 @@ void sph_close(sph_ctx* c) {
     }
     for (void* p : c->allocs) (void)hipFree(p);
     if (c->adj_buf) (void)hipFree(c->adj_buf);
+    delete c->adj_plan;
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

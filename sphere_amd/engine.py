@@ -65,7 +65,7 @@ SCALARS = {"time": 0, "time_total": 1, "dt": 2, "file_dt": 3, "iter": 4, "step_c
            "nsims": 13, "contact_lanes": 14, "sort_mode": 15, "resort_merge_max": 16,
            "graph_steps": 17, "xcd_order": 18, "adj_checkpoint": 19, "fused": 20,
            "wall_sums": 21, "k_n": 22, "integrate_lanes": 23, "bond_k": 24,
-           "bond_break_ext": 25}
+           "bond_break_ext": 25, "contact_wg": 26, "vl_cap": 27, "radix_bits": 28}
 
 # Exported symbols of the C ABI (include/sphere_hip.h).
 SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_write_bin",
@@ -76,7 +76,8 @@ SYMBOLS = ("sph_open", "sph_open_mem", "sph_step", "sph_sync", "sph_run", "sph_w
            "sph_num_sims", "sph_write_bin_sim", "sph_write_mem_sim", "sph_get_field_sim",
            "sph_set_field_sim", "sph_field_bytes_sim", "sph_get_scalar_sim",
            "sph_set_scalar_sim", "sph_adjoint", "sph_adjoint_samples", "sph_adjoint_batch",
-           "sph_adjoint_samples_batch", "sph_selftest_sqrt32",
+           "sph_adjoint_samples_batch", "sph_adjoint_record", "sph_adjoint_reverse",
+           "sph_selftest_sqrt32",
            "sph_write_state", "sph_read_state", "sph_write_state_sim", "sph_read_state_sim",
            "sph_find_overlaps", "sph_porosity", "sph_autotune", "sph_porosity_noalloc")
 
@@ -157,6 +158,9 @@ def load():
                                             vp, vp, vp]
     L.sph_autotune.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.sph_adjoint_samples.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp, vp,
+                                      vp]
+    L.sph_adjoint_record.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp]
+    L.sph_adjoint_reverse.argtypes = [vp, ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp, vp, vp,
                                       vp]
     L.sph_selftest_sqrt32.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64,
                                       ctypes.POINTER(ctypes.c_uint64)]
@@ -343,10 +347,42 @@ class Context:
                      outs[1].ctypes.data, outs[2].ctypes.data, gp.ctypes.data))
         return self._adjoint_result(outs, gp)
 
+    def adjoint_record(self, every, nsamples):
+        """The forward half of adjoint_samples (sph_adjoint_record): steps
+        every * nsamples steps recording the tape and returns the positions
+        after every, 2 every, ... steps, shape (nsamples, rows, 3)."""
+        out = numpy.zeros((int(nsamples), self.rows(), 3))
+        self._chk(self._L.sph_adjoint_record(self._h, int(every), int(nsamples),
+                                             out.ctypes.data))
+        return out
+
+    def adjoint_reverse(self, every, nsamples, dL_dx=None, dL_dv=None):
+        """The reverse half (sph_adjoint_reverse) of the run adjoint_record
+        recorded: the same dict as adjoint_samples, bit for bit."""
+        n = self.rows()
+
+        def seeds(a):
+            if a is None:
+                return None
+            return numpy.ascontiguousarray(a, dtype=numpy.float64).reshape(int(nsamples), n, 3)
+
+        sx, sv = seeds(dL_dx), seeds(dL_dv)
+        outs = [numpy.zeros((n, 3)) for _ in range(3)]
+        gp = numpy.zeros((self.nsims, len(ADJOINT_PARAMS)))
+        self._chk(self._L.sph_adjoint_reverse(
+            self._h, int(every), int(nsamples), None if sx is None else sx.ctypes.data,
+            None if sv is None else sv.ctypes.data, outs[0].ctypes.data, outs[1].ctypes.data,
+            outs[2].ctypes.data, gp.ctypes.data))
+        return self._adjoint_result(outs, gp)
+
     def autotune(self, steps_per_trial=50):
-        """sph_autotune: choose the contact lanes and the workgroup order by
-        timing steps of this simulation (identical results for every
-        setting; the simulation advances).  Returns the best us per step."""
+        """sph_autotune: choose the contact lanes, the contact workgroup
+        size, the candidate-list capacity, the radix digit width and the
+        workgroup order by timing the contact kernel over steps of this
+        simulation, and the radix digit width by timing sorts (identical
+        results for every setting; the simulation advances).  Returns the
+        contact kernel's best mean time (us); the scalars then read the
+        chosen settings."""
         us = ctypes.c_double()
         self._chk(self._L.sph_autotune(self._h, int(steps_per_trial), ctypes.byref(us)))
         return us.value

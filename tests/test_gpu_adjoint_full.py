@@ -268,3 +268,48 @@ def test_batch_trajectory_matches_single_trajectories():
         assert numpy.array_equal(theta.grad.numpy()[k], th.grad.numpy()), k
         assert numpy.array_equal(v0.grad.numpy()[off:off + n], v.grad.numpy()), k
         off += n
+
+
+@pytest.mark.parametrize("case", ["single", "checkpointed", "batch"])
+def test_record_then_reverse_equals_adjoint_samples(case):
+    """sph_adjoint_record + sph_adjoint_reverse (the torch binding's forward
+    and backward) give sph_adjoint_samples's gradients bit for bit, the
+    recorded samples are a plain run's positions, and the context ends in
+    the same state; stepping between the two calls is refused."""
+    if case == "batch":
+        images = [cases.fixture("shear1k.bin").tobytes(), cases.settling_case(300, 11).tobytes()]
+    else:
+        images = [cases.fixture("shear1k.bin").tobytes()]
+
+    def open_ctx():
+        ctx = engine.Context(images=images) if len(images) > 1 else engine.Context(images[0])
+        if case == "checkpointed":
+            ctx.set_scalar("adj_checkpoint", 15)  # segments of 15 of the 40 steps
+        return ctx
+
+    every, nsamples = 10, 4
+    a = open_ctx()
+    rows = a.rows()
+    rng = numpy.random.default_rng(5)
+    sx = rng.standard_normal((nsamples, rows, 3))
+    ga = a.adjoint_samples(every, nsamples, sx)
+    b = open_ctx()
+    samples = b.adjoint_record(every, nsamples)
+    gb = b.adjoint_reverse(every, nsamples, sx)
+    for key in ga:
+        assert numpy.array_equal(numpy.asarray(ga[key]), numpy.asarray(gb[key])), key
+    for k in range(len(images)):
+        assert numpy.array_equal(a.get("x", sim=k), b.get("x", sim=k))
+    c = open_ctx()
+    offs = numpy.cumsum([0] + [int(c.scalar("np", sim=k)) for k in range(len(images))])
+    for m in range(nsamples):
+        c.step(every)
+        for k in range(len(images)):
+            assert numpy.array_equal(samples[m, offs[k]:offs[k + 1]], c.get("x", sim=k)[:, :3])
+    d = open_ctx()
+    d.adjoint_record(every, nsamples)
+    d.step(1)
+    with pytest.raises(engine.EngineError):
+        d.adjoint_reverse(every, nsamples, sx)
+    for ctx in (a, b, c, d):
+        ctx.close()

@@ -297,8 +297,13 @@ def test_autotune_keeps_results(case):
     us = ctx.autotune(20)
     assert us > 0.0
     assert ctx.scalar("contact_lanes") in (4, 8, 16)
+    assert ctx.scalar("contact_wg") in (128, 256, 512)
+    assert ctx.scalar("vl_cap") in (16, 24, 32)
+    assert ctx.scalar("radix_bits") in (4, 6, 8)
+    # 3 (lanes) + 3 (workgroup size, with 4 lanes) + 3 (list capacity) + 2
+    # (workgroup order) trials of 30 steps; the digit width times sorts.
     done = int(ctx.scalar("iter"))
-    assert done == 4 * 30
+    assert done % 30 == 0 and 8 * 30 <= done <= 11 * 30
     ctx.step(30)
     for k, image in enumerate(images):
         ref = oracle.Oracle(image)
@@ -306,6 +311,55 @@ def test_autotune_keeps_results(case):
         for name in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t"):
             assert numpy.array_equal(ctx.get(name, sim=k), ref.get(name)), (k, name)
     ctx.close()
+
+
+@pytest.mark.parametrize("knob,value,case", [
+    ("contact_wg", 128, "shear1k"), ("contact_wg", 512, "shear1k"), ("contact_wg", 512, "batch"),
+    ("vl_cap", 8, "shear1k"), ("vl_cap", 16, "batch"), ("vl_cap", 24, "settling"),
+    ("radix_bits", 4, "settling"), ("radix_bits", 6, "settling"), ("radix_bits", 4, "batch"),
+    ("radix_bits", 6, "shear1k")])
+@pytest.mark.parametrize("sort_mode", [0, 1])
+def test_tuning_knobs_bit_exact(knob, value, case, sort_mode):
+    """Every setting the autotuner tries gives the oracle's results bit for
+    bit: the contact workgroup size, the candidate-list capacity (particles
+    beyond it scan their 27 cells) and the radix digit width, with the sort
+    as per-step launches (sort mode 0) and as the end-of-step one-workgroup
+    re-sort forced to sort anew (sort mode 1, merge limit 0)."""
+    a = cases.fixture("shear1k.bin")
+    if case == "shear1k":
+        images = [a.tobytes()]
+    elif case == "settling":
+        images = [cases.settling_case(1200, 12).tobytes()]
+    else:
+        images = [a.tobytes(), cases.settling_case(1200, 12).tobytes()]
+    ctx = engine.Context(images=images) if len(images) > 1 else engine.Context(images[0])
+    ctx.set_scalar("sort_mode", sort_mode)
+    for k in range(len(images)):
+        ctx.set_scalar("resort_merge_max", 0, sim=k)
+        if knob == "vl_cap":
+            ctx.set_scalar(knob, value, sim=k)
+    if knob != "vl_cap":
+        ctx.set_scalar(knob, value)
+    assert ctx.scalar(knob) == value
+    refs = [oracle.Oracle(image) for image in images]
+    for _ in range(3):
+        ctx.step(40)
+        for ref in refs:
+            ref.step(40)
+    if case == "settling":
+        assert ctx.scalar("sorts") > 1  # re-sorts after the first step
+    for k, ref in enumerate(refs):
+        bad = _compare_sim(ctx, k, ref)
+        assert bad == [], (k, bad)
+    ctx.close()
+
+
+def _compare_sim(ctx, k, ref, fields=EXACT_FIELDS):
+    bad = []
+    for name in fields:
+        if not numpy.array_equal(ctx.get(name, sim=k), ref.get(name)):
+            bad.append(name)
+    return bad
 
 
 @pytest.mark.parametrize("case", ["shear1k", "settling", "bonded", "batch"])
