@@ -150,12 +150,9 @@ def batch_images(args, nsims):
     def opener(img):
         return engine.Context(img)
 
-    bases = [scenes.settled_shear_cell(500, args.seed + k, open_stepper=opener,
-                                       settle_steps=args.cell_settle_steps,
-                                       consolidate_steps=args.cell_consolidate_steps,
-                                       r_uniform=(0.005, 0.01), shear=False)
-             for k in range(4)]
-    cells = scenes.batch_shear_cells(sizes, bases, seed=args.seed, kn_range=(1.0e6, 1.0e8))
+    cells = scenes.consolidation_batch(nsims, args.seed, opener,
+                                       cell_settle_steps=args.cell_settle_steps,
+                                       cell_consolidate_steps=args.cell_consolidate_steps)
     log("batch of %d consolidation beds prepacked in %.1f s" % (nsims, time.time() - t0))
     return ([c.tobytes() for c in cells],
             "batched consolidation beds (tiled settled 500-particle cells, radii U[0.005, "
@@ -594,19 +591,20 @@ def main():
     # when it also integrates, plus the integration's 256 + 184.
     fused = "contact_integrate" in rep
     b_contact = 480.0 + 108.0 * zz + 8.0 * nw + (440.0 if fused else 0.0)
-    t_contact = rep.get("contact_integrate" if fused else "contact",
-                        (float("nan"), 0))[0] * 1e-6
-    achieved = n * b_contact / t_contact / 1e9
+    t_contact = rep.get("contact_integrate" if fused else "contact", (0.0, 0))[0] * 1e-6
+    timed = t_contact > 0.0  # no profiling pass (--profile-steps 0): no launch time
+    achieved = n * b_contact / t_contact / 1e9 if timed else None
     traffic = measured_traffic(zz, nw)
     roofline = {"bound": "hbm", "kernel": "k_contact_g<2, 4, true>" if fused else "k_contact_g",
-                "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "achieved": round(achieved, 2) if timed else None,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4) if timed else None,
                 "traffic": traffic[0] if traffic else None,
                 "traffic_source": ("not measured in this run: rocprofv3 --pmc FETCH_SIZE and "
                                    "WRITE_SIZE of the same kernel sources, "
                                    "profiles/traffic_contact.json" if traffic else None),
                 "algorithmic_bytes_per_launch": round(n * b_contact),
-                "mean_launch_us": round(t_contact * 1e6, 3),
+                "mean_launch_us": round(t_contact * 1e6, 3) if timed else None,
                 "mean_launch_us_source": "HIP events around each launch on the context stream",
                 "contacts_per_particle": round(zz, 3)}
     # Whole-step algorithmic floor, B_min = 528 + 72 Z bytes per particle-step.

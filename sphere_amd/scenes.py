@@ -229,6 +229,24 @@ def batch_sizes(nsims=256, lo=500, hi=20000, seed=7):
     return numpy.exp(rng.uniform(math.log(lo), math.log(hi), nsims)).astype(int)
 
 
+def consolidation_batch(nsims, seed, open_stepper, cell_settle_steps=20000,
+                        cell_consolidate_steps=5000, ncells=4):
+    """The batched consolidation sweep: `nsims` beds of log-uniform size in
+    [500, 20000] particles (batch_sizes), each a settled 500-particle cell
+    (radii uniform in [0.005, 0.01] m, one of `ncells` seeds, periodic in x
+    and y, under a stress-controlled top wall, not sheared) tiled in x and y
+    to its size, with its own normal stress (uniform in [5, 160] kPa),
+    stiffness (log-uniform in [1e6, 1e8] N/m, with its own critical damping
+    and dt) and friction (uniform in [0.3, 0.7]) (batch_shear_cells)."""
+    sizes = batch_sizes(nsims, seed=seed)
+    bases = [settled_shear_cell(500, seed + k, open_stepper=open_stepper,
+                                settle_steps=cell_settle_steps,
+                                consolidate_steps=cell_consolidate_steps,
+                                r_uniform=(0.005, 0.01), shear=False)
+             for k in range(ncells)]
+    return batch_shear_cells(sizes, bases, seed=seed, kn_range=(1.0e6, 1.0e8))
+
+
 _PER_PARTICLE = ("x", "radius", "xyzsum", "vel", "fixvel", "force", "angpos", "angvel",
                  "torque", "es_dot", "es", "ev_dot", "ev", "p", "color")
 

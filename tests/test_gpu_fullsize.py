@@ -158,3 +158,67 @@ def test_batch_of_256_beds_bit_exact():
             for f in ("x", "vel", "angvel", "force", "contacts", "delta_t"):
                 assert numpy.array_equal(ctx.get(f, sim=k), ref.get(f)), (k, f)
             assert numpy.array_equal(ctx.get("walls_nx", sim=k), ref.get("walls_nx")), k
+
+
+def test_benchmark_bed_1000_steps_bit_exact():
+    """The benchmark's own workload over a long horizon: the 1e5-particle
+    direct-shear bed settled, consolidated under 10 kPa and sheared at
+    1e-3 m/s exactly as the benchmark builds it (settled_shear_cell with
+    seed 2001 and its default settling, stepped by the engine), then 1,000
+    steps on the GPU and on the oracle, every field and the walls compared
+    after each 250.  The interval holds re-sorts of the particle order and
+    rebuilds of the candidate lists."""
+    from sphere_amd import scenes
+    s = scenes.settled_shear_cell(100000, 2001, open_stepper=engine.Context)
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    sorts0, builds0 = ctx.scalar("sorts"), ctx.scalar("vl_builds")
+    for k in range(4):
+        ref.step(250)
+        ctx.step(250)
+        bad = _compare(ctx, ref) + _compare(ctx, ref, WALL_FIELDS)
+        assert bad == [], (250 * (k + 1), bad)
+    sorts = int(ctx.scalar("sorts") - sorts0)
+    builds = int(ctx.scalar("vl_builds") - builds0)
+    print("1e5 benchmark bed, 1000 steps: %d re-sorts, %d list rebuilds" % (sorts, builds))
+    assert sorts >= 1 and builds >= 1
+    ctx.close()
+
+
+def test_consolidation_batch_of_256_500_steps_bit_exact():
+    """The benchmark's batched sweep over a long horizon: the 256
+    consolidation beds it times (consolidation_batch with seed 2001: settled
+    500-particle cells with radii uniform in [0.005, 0.01] m tiled to
+    log-uniform sizes in [500, 20000], each with its own stiffness, dt,
+    friction and normal stress), 500 steps in one context, every simulation
+    bit-exact with its own oracle.  The oracles run on a pool of threads,
+    one OpenMP thread each."""
+    from concurrent.futures import ThreadPoolExecutor
+    import os
+    from sphere_amd import scenes
+    cells = scenes.consolidation_batch(256, 2001, engine.Context)
+    images = [c.tobytes() for c in cells]
+    fields = ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t", "es", "ev")
+    with engine.Context(images=images) as ctx:
+        ctx.step(500)
+        got = [{f: ctx.get(f, sim=k) for f in fields + ("walls_nx",)}
+               for k in range(len(images))]
+
+    def check(k):
+        oracle.set_threads(1)
+        ref = oracle.Oracle(images[k])
+        ref.step(500)
+        bad = [f for f in fields + ("walls_nx",) if not numpy.array_equal(got[k][f], ref.get(f))]
+        ref.close()
+        return bad
+
+    threads = oracle.get_threads()
+    workers = max(1, min(16, os.cpu_count() or 1))
+    try:
+        with ThreadPoolExecutor(workers) as pool:
+            results = list(pool.map(check, range(len(images))))
+    finally:
+        oracle.set_threads(threads)
+    bad = [(k, b) for k, b in enumerate(results) if b]
+    assert bad == []
