@@ -1464,30 +1464,6 @@ __device__ __forceinline__ uint32_t group_or(uint32_t v) {
     return v;
 }
 
-// The interaction round's contributions, added on every lane of the group in
-// slot order (group lane T evaluated the T-th slot of the round).
-template <int LPP, int T>
-__device__ __forceinline__ void replay_round(const LaneGroup<LPP>& g, int take, uint32_t tm,
-                                             const Contrib& C, Acc& A) {
-    if constexpr (T < LPP) {
-        if (T >= take) return;
-        if ((tm >> T) & 1u) {
-            Contrib c;
-            c.F = g.template bc<T>(C.F);
-            c.T = g.template bc<T>(C.T);
-            c.p = g.template bc<T>(C.p);
-            const uint32_t nn = g.template bc<T>((uint32_t)C.nev | ((uint32_t)C.nes << 8));
-            c.nev = (int)(nn & 0xffu);
-            c.nes = (int)(nn >> 8);
-            c.ev0 = g.template bc<T>(C.ev0);
-            c.ev1 = g.template bc<T>(C.ev1);
-            c.es = g.template bc<T>(C.es);
-            apply(A, c);
-        }
-        replay_round<LPP, T + 1>(g, take, tm, C, A);
-    }
-}
-
 // The common case of a round of list entries: every overlapping entry of
 // the round (bit t of hm: group lane t's entry e) is a partner that exactly
 // one slot of the row holds, the cell is not done and no overflow is
@@ -1564,6 +1540,22 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // A workgroup's particles lie in one 256-particle chunk (computed from
     // the workgroup index alone, so that the parameter loads are scalar).
     const DevParams& p = chunk_params(a, blk * PPB / BLOCK);
+    // The interaction's staging (per wavefront): each group's number of
+    // occupied slots, whether one of its contacts ended, the contact table
+    // (group and slot of each contact), the particles' state and index,
+    // and one round's contributions (10 doubles per lane).
+    constexpr int G = 64 / LPP;   // groups per wavefront
+    constexpr int NWV = WG / 64;  // wavefronts per workgroup
+    __shared__ __align__(16) uint8_t s_zc[NWV][G];
+    __shared__ uint8_t s_end[NWV][G];
+    __shared__ uint16_t s_tab[NWV][G * NC];
+    __shared__ double4 s_pa[NWV][3][G];
+    __shared__ uint32_t s_po[NWV][G];
+    __shared__ double s_con[NWV][10][64];
+    const uint32_t wv = threadIdx.x / 64u;
+    const uint32_t gw = (threadIdx.x & 63u) / LPP;  // group within the wavefront
+    // Groups past the end count no contacts.
+    if (g.q == 0) s_zc[wv][gw] = 0;
     if (ia - p.base >= p.np) return;  // whole groups leave together
     DevLoop* lp = a.loop + p.sim;
     const uint32_t ia_orig = a.perm[ia];
@@ -1745,9 +1737,15 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const d3 angvel_a = xyz(av_a);
     tape_start(a, p, ia, ia_orig, xa, vel_a, av_a, g.q == 0);
 
-    // interact (src/contactsearch.cuh:484-559): the occupied slots, LPP at a
-    // time in ascending slot order, each on its own lane; contributions are
-    // then added in slot order on every lane of the group.
+    // interact (src/contactsearch.cuh:484-559).  The occupied slots of the
+    // wavefront's groups are numbered group by group, each group's in
+    // ascending slot order, and evaluated one per lane of the wavefront at
+    // a time, so that no particle's contacts wait for those of the
+    // wavefront's most connected particle.  Each contact's contribution
+    // goes to LDS; each group then adds its contacts' contributions in slot
+    // order: lane 0 the force, lane 1 the torque, lane 2 p and the energy
+    // rates.  An absent energy addition is stored as +0.0: the sums start
+    // at +0.0 and so are never -0.0, and x + 0.0 = x for every other x.
     const bool zero_empty = lp->zero_empty != 0u;
     uint32_t occ = 0;
     wave_sync();
@@ -1759,23 +1757,49 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         if (!o && zero_empty)
             a.delta_t[(size_t)ia_orig * NC + g.q + LPP * j] = make_double4(0.0, 0.0, 0.0, 0.0);
     }
+    const uint32_t zg = __popc(occ);
+    if (g.q == 0) {
+        s_zc[wv][gw] = (uint8_t)zg;
+        s_end[wv][gw] = 0;
+        s_pa[wv][0][gw] = xa;
+    }
+    if (g.q == 1) s_pa[wv][1][gw] = vel_a;
+    if (g.q == 2) s_pa[wv][2][gw] = av_a;
+    if (g.q == 3) s_po[wv][gw] = ia_orig;
     wave_sync();
-    Acc A;
-    A.F = mk3(0.0, 0.0, 0.0);
-    A.T = mk3(0.0, 0.0, 0.0);
-    A.es_dot = 0.0;
-    A.ev_dot = 0.0;
-    A.p = 0.0;
-    bool ended = false;  // a held contact ended (the hit mask goes stale)
-    uint32_t rem = occ;
-    while (rem) {
-        // This lane takes the q-th occupied slot of the round.
-        uint32_t m = rem;
-        for (uint32_t t = 0; t < g.q && m; ++t) m &= m - 1u;
-        const int take = __popc(rem) < LPP ? __popc(rem) : LPP;
-        const bool mine = m != 0u && (int)g.q < take;
-        const uint32_t sidx = mine ? (uint32_t)(__ffs(m) - 1) : 0u;
-        const uint32_t wrd = mine ? gs[sidx] : 0u;
+    // The group's first contact number and the wavefront's contact count
+    // (byte sums of the counts, each at most NC).
+    uint32_t first = 0, total = 0;
+    {
+        const uint32_t* zw = reinterpret_cast<const uint32_t*>(s_zc[wv]);
+#pragma unroll
+        for (int i = 0; i < G / 4; ++i) {
+            const uint32_t w = zw[i];
+            const int below_w = (int)gw - 4 * i;
+            const uint32_t m = below_w <= 0 ? 0u
+                               : below_w >= 4 ? 0xffffffffu
+                                              : (1u << (8 * below_w)) - 1u;
+            first += ((w & m) * 0x01010101u) >> 24;
+            total += (w * 0x01010101u) >> 24;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < SPL; ++j) {
+        const uint32_t sl = g.q + LPP * j;
+        if ((occ >> sl) & 1u)
+            s_tab[wv][first + __popc(occ & ((1u << sl) - 1u))] = (uint16_t)((gw << 5) | sl);
+    }
+    wave_sync();
+    // The wavefront's lanes still running (groups past the end have left)
+    // take the contacts in turn, by their rank among the running lanes.
+    const uint64_t act = __ballot(1);
+    const uint32_t nact = (uint32_t)__popcll(act);
+    const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+    const uint32_t part = g.q < 3 ? g.q : 2u;
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (uint32_t r0 = 0; r0 < total; r0 += nact) {
+        const uint32_t c = r0 + rk;
         Contrib C;
         contrib_init(C);
         C.F = mk3(0.0, 0.0, 0.0);
@@ -1784,9 +1808,16 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         C.ev0 = 0.0;
         C.ev1 = 0.0;
         C.es = 0.0;
-        bool touching = false;
-        if (mine) {
-            const size_t mp = (size_t)ia_orig * NC + sidx;
+        if (c < total) {
+            const uint32_t e = s_tab[wv][c];
+            const uint32_t h = e >> 5, sidx = e & 31u;
+            uint32_t* const hs = s_slot + (wv * G + h) * NC;
+            const double4 xc = s_pa[wv][0][h];
+            const double4 vel_c = s_pa[wv][1][h];
+            const double4 av_c = s_pa[wv][2][h];
+            const uint32_t io = s_po[wv][h];
+            const uint32_t wrd = hs[sidx];
+            const size_t mp = (size_t)io * NC + sidx;
             const uint32_t rb_s = wrd & CW_IDX_MASK;
             const d3 dm = shift_of(p, word_code(wrd));
             // A slot's contact has almost always persisted: its partner's
@@ -1796,40 +1827,75 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
             const double4 vel_b = S.v[rb_j];
             const double4 av_b = S.w[rb_j];
             const double4 dt0 = a.delta_t[mp];
+            const double rc = xc.w;
             const double rb = xb.w;
-            const d3 xab = mk3(xa.x - xb.x + dm.x, xa.y - xb.y + dm.y, xa.z - xb.z + dm.z);
+            const d3 xab = mk3(xc.x - xb.x + dm.x, xc.y - xb.y + dm.y, xc.z - xb.z + dm.z);
             const double len = length(xab);
-            const double dn = len - (ra + rb);
+            const double dn = len - (rc + rb);
             if (dn < 0.0) {
-                touching = true;
-                const d3 angvel_b = xyz(av_b);
                 if (a.tape_dt0) {
                     a.tape_dt0[mp] = dt0;
                     a.tape_srt[mp] = rb_s;
                 }
                 d3 nd;
                 if constexpr (CM == 2)
-                    nd = contact_linear(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab, len,
-                                        dn, dt0);
+                    nd = contact_linear(p, C, vel_c, xyz(av_c), vel_b, xyz(av_b), rc, rb, xab,
+                                        len, dn, dt0);
                 else
-                    nd = contact_hertz(p, C, vel_a, angvel_a, vel_b, angvel_b, ra, rb, xab, len,
-                                       dn, dt0);
+                    nd = contact_hertz(p, C, vel_c, xyz(av_c), vel_b, xyz(av_b), rc, rb, xab,
+                                       len, dn, dt0);
                 a.delta_t[mp] = make_double4(nd.x, nd.y, nd.z, 0.0);
             } else {
                 // The contact ended: its slot empties (src/contactsearch.cuh:540-548).
-                gs[sidx] = EMPTY;
+                hs[sidx] = EMPTY;
                 a.delta_t[mp] = make_double4(0.0, 0.0, 0.0, 0.0);
-                ended = true;
+                s_end[wv][h] = 1;
             }
         }
-        const uint32_t tm = g.bits(touching);
-        replay_round<LPP, 0>(g, take, tm, C, A);
-        // Drop the round's slots from the remaining set.
-        for (int t = 0; t < take; ++t) rem &= rem - 1u;
+        s_con[wv][0][rk] = C.F.x;
+        s_con[wv][1][rk] = C.F.y;
+        s_con[wv][2][rk] = C.F.z;
+        s_con[wv][3][rk] = C.T.x;
+        s_con[wv][4][rk] = C.T.y;
+        s_con[wv][5][rk] = C.T.z;
+        s_con[wv][6][rk] = C.p;
+        s_con[wv][7][rk] = C.nev > 0 ? C.ev0 : 0.0;
+        s_con[wv][8][rk] = C.nes > 0 ? C.es : 0.0;
+        s_con[wv][9][rk] = C.nev > 1 ? C.ev1 : 0.0;
+        wave_sync();
+        // The group's contacts of this round, in slot order: force (x, y,
+        // z), torque (x, y, z), or p, ev_dot += ev0, es_dot, then
+        // ev_dot += ev1.
+        const uint32_t lo = first > r0 ? first : r0;
+        const uint32_t hi = first + zg < r0 + nact ? first + zg : r0 + nact;
+        for (uint32_t cc = lo; cc < hi; ++cc) {
+            const uint32_t l = cc - r0;
+            acc.x += s_con[wv][3 * part][l];
+            acc.y += s_con[wv][3 * part + 1][l];
+            acc.z += s_con[wv][3 * part + 2][l];
+            const double ev = acc.y + s_con[wv][9][l];
+            if (g.q == 2) acc.y = ev;
+        }
+        wave_sync();
     }
+    // Each lane's sums in the layout of Acc (lane 0: F, lane 1: T, lane 2:
+    // p, ev_dot, es_dot); the walls add to every field alike, and each lane
+    // writes its part of the output record.
+    Acc A;
+    A.F = mk3(acc.x, acc.y, acc.z);
+    A.T = mk3(acc.x, acc.y, acc.z);
+    A.p = acc.x;
+    A.ev_dot = acc.y;
+    A.es_dot = acc.z;
+    const bool ended = s_end[wv][gw] != 0;
+    // The particle's state again, from LDS (not held in registers through
+    // the interaction).
+    const double4 xa_o = s_pa[wv][0][gw];
+    const double4 vel_o = s_pa[wv][1][gw];
+    const double4 av_o = s_pa[wv][2][gw];
     // Without registration the hit mask stays true until a contact ends,
     // and the row changes only where a contact ended or the order changed.
-    const bool any_ended = g.bits(ended) != 0u;
+    const bool any_ended = ended;
     if (quiet && any_ended && g.q == 0) a.vcount[ia] = make_uint2(vcnt, 0u);
     wave_sync();
     if (!quiet || any_ended || remap) copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
@@ -1838,20 +1904,26 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         for (int k = 0; k < SPL; ++k)
             a.tape_rows[(size_t)ia_orig * NC + g.q * SPL + k] = word_to_orig(a, gs[g.q * SPL + k]);
     }
-    walls_and_outputs(p, a, A, xa, vel_a, av_a, Lz, ia_orig, g.q < 4 ? (int)g.q : -1);
+    walls_and_outputs(p, a, A, xa_o, vel_o, av_o, Lz, ia_orig, g.q < 4 ? (int)g.q : -1);
     if constexpr (FUSED) {
         // The particle's force and torque are final (its own row and the
         // walls): integrate it now (src/integration.cuh:14-305), from the
         // step's sorted copies, into the next step's set, and hash it for
         // the next step.  (Fixed grid height, no bonds, no wmode-3 wall.)
         static_assert(!FUSED || LPP == 4, "one vector component per lane");
-        if (g.q == 0) {
+        // Lane 0 holds the force, lane 1 the torque, lane 2 the energy
+        // rates: lane q takes component q of each.
+        if (g.q == 2) {
             a.es[ia_orig] = a.es[ia_orig] + A.es_dot * p.dt;
             a.ev[ia_orig] = a.ev[ia_orig] + A.ev_dot * p.dt;
         }
-        const double fq = g.q == 0 ? A.F.x : g.q == 1 ? A.F.y : A.F.z;
-        const double tq = g.q == 0 ? A.T.x : g.q == 1 ? A.T.y : A.T.z;
-        const double xn = integrate_component(p, a, lp, ia_orig, g.q, fq, tq, xa, vel_a, av_a, ia);
+        const double fx = g.template bc<0>(A.F.x), fy = g.template bc<0>(A.F.y),
+                     fz = g.template bc<0>(A.F.z);
+        const double tx = g.template bc<1>(A.T.x), ty = g.template bc<1>(A.T.y),
+                     tz = g.template bc<1>(A.T.z);
+        const double fq = g.q == 0 ? fx : g.q == 1 ? fy : fz;
+        const double tq = g.q == 0 ? tx : g.q == 1 ? ty : tz;
+        const double xn = integrate_component(p, a, lp, ia_orig, g.q, fq, tq, xa_o, vel_o, av_o, ia);
         const double4 x_new = make_double4(g.template bc<0>(xn), g.template bc<1>(xn),
                                            g.template bc<2>(xn), g.template bc<3>(xn));
         HashFlags f;
