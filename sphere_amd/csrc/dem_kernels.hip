@@ -1543,7 +1543,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // The interaction's staging (per wavefront): each group's number of
     // occupied slots, whether one of its contacts ended, the contact table
     // (group and slot of each contact), the particles' state and index,
-    // and one round's contributions (10 doubles per lane).
+    // and one round's contributions (6 doubles per lane).
     constexpr int G = 64 / LPP;   // groups per wavefront
     constexpr int NWV = WG / 64;  // wavefronts per workgroup
     __shared__ __align__(16) uint8_t s_zc[NWV][G];
@@ -1551,7 +1551,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ uint16_t s_tab[NWV][G * NC];
     __shared__ double4 s_pa[NWV][3][G];
     __shared__ uint32_t s_po[NWV][G];
-    __shared__ double s_con[NWV][10][64];
+    __shared__ double s_con[NWV][6][64];
     const uint32_t wv = threadIdx.x / 64u;
     const uint32_t gw = (threadIdx.x & 63u) / LPP;  // group within the wavefront
     // Groups past the end count no contacts.
@@ -1796,7 +1796,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const uint32_t nact = (uint32_t)__popcll(act);
     const uint32_t rk = __builtin_amdgcn_mbcnt_hi((uint32_t)(act >> 32),
                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
-    const uint32_t part = g.q < 3 ? g.q : 2u;
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     for (uint32_t r0 = 0; r0 < total; r0 += nact) {
         const uint32_t c = r0 + rk;
@@ -1852,29 +1851,40 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
                 s_end[wv][h] = 1;
             }
         }
+        // The group's contacts of this round, in slot order, in two passes
+        // through one buffer: force and torque (lanes 0 and 1, x, y, z),
+        // then p, ev_dot += ev0, es_dot and ev_dot += ev1 (lane 2).
+        const uint32_t lo = first > r0 ? first : r0;
+        const uint32_t hi = first + zg < r0 + nact ? first + zg : r0 + nact;
         s_con[wv][0][rk] = C.F.x;
         s_con[wv][1][rk] = C.F.y;
         s_con[wv][2][rk] = C.F.z;
         s_con[wv][3][rk] = C.T.x;
         s_con[wv][4][rk] = C.T.y;
         s_con[wv][5][rk] = C.T.z;
-        s_con[wv][6][rk] = C.p;
-        s_con[wv][7][rk] = C.nev > 0 ? C.ev0 : 0.0;
-        s_con[wv][8][rk] = C.nes > 0 ? C.es : 0.0;
-        s_con[wv][9][rk] = C.nev > 1 ? C.ev1 : 0.0;
         wave_sync();
-        // The group's contacts of this round, in slot order: force (x, y,
-        // z), torque (x, y, z), or p, ev_dot += ev0, es_dot, then
-        // ev_dot += ev1.
-        const uint32_t lo = first > r0 ? first : r0;
-        const uint32_t hi = first + zg < r0 + nact ? first + zg : r0 + nact;
-        for (uint32_t cc = lo; cc < hi; ++cc) {
-            const uint32_t l = cc - r0;
-            acc.x += s_con[wv][3 * part][l];
-            acc.y += s_con[wv][3 * part + 1][l];
-            acc.z += s_con[wv][3 * part + 2][l];
-            const double ev = acc.y + s_con[wv][9][l];
-            if (g.q == 2) acc.y = ev;
+        if (g.q < 2) {
+            for (uint32_t cc = lo; cc < hi; ++cc) {
+                const uint32_t l = cc - r0;
+                acc.x += s_con[wv][3 * g.q][l];
+                acc.y += s_con[wv][3 * g.q + 1][l];
+                acc.z += s_con[wv][3 * g.q + 2][l];
+            }
+        }
+        wave_sync();
+        s_con[wv][0][rk] = C.p;
+        s_con[wv][1][rk] = C.nev > 0 ? C.ev0 : 0.0;
+        s_con[wv][2][rk] = C.nes > 0 ? C.es : 0.0;
+        s_con[wv][3][rk] = C.nev > 1 ? C.ev1 : 0.0;
+        wave_sync();
+        if (g.q == 2) {
+            for (uint32_t cc = lo; cc < hi; ++cc) {
+                const uint32_t l = cc - r0;
+                acc.x += s_con[wv][0][l];
+                acc.y += s_con[wv][1][l];
+                acc.z += s_con[wv][2][l];
+                acc.y += s_con[wv][3][l];
+            }
         }
         wave_sync();
     }
