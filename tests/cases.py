@@ -109,3 +109,35 @@ def lattice_bed(n, seed, median, sigma, k_n=1.16e7, normal_stress=10.0e3, shear_
         s.sigma_b[0] = k_n * 0.05 * median
         s.tau_b[0] = k_n * 0.05 * median
     return s
+
+
+def fcc_case(cells=(4, 4, 6), drop=5, seed=3, overlap=0.004, k_n=1.16e7):
+    """Equal spheres (r = 0.005 m) on a face-centred cubic lattice, closer
+    than touching by `overlap` of the diameter, so that an interior sphere
+    starts with 12 contacts: a wavefront of 16 particles holds far more than
+    64 contacts.  The last `drop` lattice sites are left empty (np is not a
+    multiple of 16), small random velocities, gravity, walls on every side."""
+    rng = numpy.random.default_rng(seed)
+    r = 0.005
+    a = 2.0 * r * (1.0 - overlap) * 2.0 ** 0.5  # nearest neighbours 2 r (1 - overlap)
+    basis = numpy.array([[0.0, 0.0, 0.0], [0.5, 0.5, 0.0], [0.5, 0.0, 0.5], [0.0, 0.5, 0.5]])
+    nx, ny, nz = cells
+    pts = [(numpy.array([i, j, k]) + b) * a for k in range(nz) for j in range(ny)
+           for i in range(nx) for b in basis]
+    pts = numpy.array(pts)[: len(pts) - drop] + 1.5 * r
+    n = len(pts)
+    s = sim(np=n, sid="fcc-%d" % n)
+    s.radius[:] = r
+    s.x = pts
+    s.vel[:, :3] = rng.normal(0.0, 0.01, (n, 3))
+    s.defaultParams(mu_s=0.5, mu_d=0.5, rho=2600.0, k_n=k_n, k_t=k_n)
+    scenes._set_contact_params(s, k_n, 0.5, 0.5, r)
+    s.g[:] = [0.0, 0.0, -9.81]
+    s.normalBoundariesXY()
+    s.defineWorldBoundaries(L=[nx * a + 3.0 * r, ny * a + 3.0 * r, nz * a + 3.0 * r])
+    dt = scenes.dt_rule(r, 2600.0, k_n)
+    s.time_dt[0] = dt
+    s.time_current[0] = 0.0
+    s.time_total[0] = dt * 1000
+    s.time_file_dt[0] = dt * 1000
+    return s

@@ -76,6 +76,23 @@ def test_crowded_case_overflows_contact_row():
     assert (d < 0).sum() > 32
 
 
+def test_fcc_case_crowds_every_wavefront():
+    """The dense packing: interior spheres with 12 contacts, more than 64
+    contacts in every run of 16 sorted particles, a partial last run, and
+    a packing that stays bounded over 200 steps."""
+    s = cases.fcc_case()
+    n = int(s.np[0])
+    assert n % 16 != 0
+    o = oracle.Oracle(s.tobytes())
+    o.step(1)
+    z = (o.get("contacts") != n).sum(1)
+    assert z.max() == 12
+    zs = z[o.get("perm").astype(numpy.int64)]
+    assert numpy.add.reduceat(zs, numpy.arange(0, n, 16)).min() > 64
+    o.step(200)
+    assert numpy.isfinite(o.get("x")).all()
+
+
 def test_capillary_case_builds_and_bridges():
     s = cases.fixture("shear1k.bin")
     s.contactmodel[0] = 1

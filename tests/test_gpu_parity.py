@@ -440,3 +440,34 @@ def test_ragged_batch_bit_exact():
         for name in ("x", "vel", "angvel", "force", "torque", "contacts", "delta_t"):
             assert numpy.array_equal(ctx.get(name, sim=k), ref.get(name)), (k, name)
     ctx.close()
+
+
+@pytest.mark.parametrize("lanes,wg", [(4, 128), (4, 256), (4, 512), (8, 256), (16, 256), (1, 256)])
+def test_dense_packing_many_contacts_per_wavefront(lanes, wg):
+    """The contact interaction takes a wavefront's occupied slots one per
+    lane, 64 (or as many lanes as are running) at a time, and adds each
+    particle's contributions in slot order.  A face-centred cubic packing
+    (interior spheres with 12 contacts, every wavefront of 16 particles
+    holding 68-176 contacts: two or three rounds) whose last wavefront is
+    partly empty (379 particles): bit-exact with the oracle over 300 steps,
+    for every contact lane count and workgroup size."""
+    s = cases.fcc_case()
+    n = int(s.np[0])
+    assert n % 16 != 0
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    ctx.set_scalar("contact_lanes", lanes)
+    if lanes == 4:
+        ctx.set_scalar("contact_wg", wg)
+    ctx.step(1)
+    ref.step(1)
+    z = (ref.get("contacts") != n).sum(1)[ref.get("perm").astype(numpy.int64)]
+    per_wave = numpy.add.reduceat(z, numpy.arange(0, n, 16))
+    assert per_wave.min() > 64
+    assert _compare(ctx, ref) == []
+    for _ in range(3):
+        ctx.step(100)
+        ref.step(100)
+        assert _compare(ctx, ref) == []
+    ctx.close()
