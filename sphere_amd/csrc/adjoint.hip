@@ -44,6 +44,10 @@ namespace sphx {
 
 namespace {
 
+// The parameter table is read-only for every kernel: read through the
+// constant address space (scalar loads that stores do not invalidate).
+typedef const __attribute__((address_space(4))) DevParams CParams;
+
 constexpr double PI = 3.14159265358979;  // src/constants.h:14
 
 __device__ __forceinline__ d3 zero3() { return mk3(0.0, 0.0, 0.0); }
@@ -61,7 +65,7 @@ struct PGrad {
 // forward inputs; gF, gT are the cotangents of a's force and torque and gdt
 // that of the new tangential displacement.  Outputs are accumulated.
 template <bool PARAMS>
-__device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 wb, double ra,
+__device__ void adj_contact_linear(CParams& p, d3 va, d3 wa, d3 vb, d3 wb, double ra,
                                    double rb, d3 x, d3 dt0, d3 gF, d3 gT, d3 gdt, d3& gx,
                                    d3& gva, d3& gwa, d3& gvb, d3& gwb, d3& gdt0, PGrad& gp) {
     // forward (same expressions and FP32 lengths, so the same branches)
@@ -180,7 +184,7 @@ __device__ void adj_contact_linear(const DevParams& p, d3 va, d3 wa, d3 vb, d3 w
 // with the wall position entering const), moving at speed wv, whose return
 // value dot(f_n, n) (the force on the wall) has cotangent gret.  Adds to the
 // particle's cotangents; returns the cotangents of delta and of wv.
-__device__ void adj_contact_wall(const DevParams& p, d3 va, d3 wa, double ra, d3 n, double delta,
+__device__ void adj_contact_wall(CParams& p, d3 va, d3 wa, double ra, d3 n, double delta,
                                  double wv, d3 gF, d3 gT, double gret, d3& gva, d3& gwa,
                                  double& gdelta, double& gwv, PGrad& gp) {
     const d3 vl = va;
@@ -258,7 +262,7 @@ __device__ __forceinline__ d3 group_sum(d3 v) {
 // accelerations the step computes from them): from the cotangents of its
 // new position, velocity and angular velocity and of the new accelerations
 // (which also feed the next step as acc0).
-__device__ __forceinline__ void force_cotangents(const DevParams& p, double r, double fv,
+__device__ __forceinline__ void force_cotangents(CParams& p, double r, double fv,
                                                  double4 Lx, double4 Lv, double4 Lw, double4 La,
                                                  double4 Laa, d3& gF, d3& gT) {
     const double dt = p.dt;
@@ -288,7 +292,7 @@ __device__ __forceinline__ void force_cotangents(const DevParams& p, double r, d
     gT = mk3(gaa.x / I, gaa.y / I, gaa.z / I);
 }
 
-__device__ __forceinline__ d3 image_shift(const DevParams& p, uint32_t wrd) {
+__device__ __forceinline__ d3 image_shift(CParams& p, uint32_t wrd) {
     const uint32_t cx = (wrd >> CW_SHIFT_X) & 3u, cy = (wrd >> CW_SHIFT_Y) & 3u;
     return mk3(cx == 1u ? 0.0 + p.L[0] : cx == 2u ? 0.0 - p.L[0] : 0.0,
                cy == 1u ? 0.0 + p.L[1] : cy == 2u ? 0.0 - p.L[1] : 0.0, 0.0);
@@ -303,7 +307,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 // One list wall's contact (walls 0..nw-1 of the simulation): the particle's
 // part, the wall's position and velocity cotangents (sign: how the contact
 // distance changes with the wall position), the force sum's cotangent gS.
-__device__ __forceinline__ void wall_contact(const DevParams& p, const AdjStep& st,
+__device__ __forceinline__ void wall_contact(CParams& p, const AdjStep& st,
                                              const AdjState& g, int w, d3 va, d3 wa, double ra,
                                              d3 e, double sign, double dw, d3 wn, double wvel,
                                              d3 gF, d3 gT, d3& gxa, d3& gva, d3& gwa,
@@ -349,7 +353,7 @@ __device__ __forceinline__ void wall_contact(const DevParams& p, const AdjStep& 
 __global__ __launch_bounds__(256) void k_adj_step(AdjStep st, AdjState g) {
     __shared__ double red[ADJ_NPART][4];
     __shared__ double psum[ADJ_NPARAM * 256];
-    const DevParams& p = st.ptab[st.chunk_sim[blockIdx.x / (256 / ADJ_PPB)]];
+    CParams& p = *(CParams*)(st.ptab + st.chunk_sim[blockIdx.x / (256 / ADJ_PPB)]);
     const uint32_t q = threadIdx.x % ALPP;
     const uint32_t ia = blockIdx.x * ADJ_PPB + threadIdx.x / ALPP;
     PGrad gp;
@@ -574,7 +578,7 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjStep st, AdjState g) {
 __global__ __launch_bounds__(256) void k_adj_bonds(AdjStep st, AdjState g) {
     const uint32_t b = blockIdx.x * 256 + threadIdx.x;
     if (b >= st.nbond) return;
-    const DevParams& p = st.ptab[st.bond_sim[b]];
+    CParams& p = *(CParams*)(st.ptab + st.bond_sim[b]);
     const uint4 idx = st.bond_idx[b];
     double4* co = g.bcontrib + (size_t)6 * b;
     double* bp = g.bpart + (size_t)ADJ_NBPARAM * b;
@@ -705,7 +709,7 @@ __global__ __launch_bounds__(256) void k_adj_bonds(AdjStep st, AdjState g) {
 // visited (the particle's own CSR list, or the whole list for particle nb0
 // after a broken bond was retargeted onto it).
 __global__ __launch_bounds__(256) void k_adj_bond_gather(AdjStep st, AdjState g) {
-    const DevParams& p = st.ptab[st.chunk_sim[blockIdx.x]];
+    CParams& p = *(CParams*)(st.ptab + st.chunk_sim[blockIdx.x]);
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i - p.base >= p.np) return;
     const bool full = (i - p.base == p.nb0) && st.bond_full[p.sim] != 0u;
@@ -759,7 +763,7 @@ __device__ double block_sum(const double* v, uint32_t n, uint32_t stride, double
 __global__ __launch_bounds__(256) void k_adj_reduce(AdjStep st, AdjState g) {
     __shared__ double cache[256];
     __shared__ double tot[ADJ_NPART];
-    const DevParams& p = st.ptab[blockIdx.x];
+    CParams& p = *(CParams*)(st.ptab + blockIdx.x);
     const uint32_t wb0 = p.base / ADJ_PPB, nwb = p.npad / ADJ_PPB;
     for (int k = 0; k < ADJ_NPART; ++k) {
         const double v = block_sum(g.part + (size_t)wb0 * ADJ_NPART + k, nwb, ADJ_NPART, cache);
@@ -801,7 +805,7 @@ __global__ __launch_bounds__(256) void k_adj_reduce(AdjStep st, AdjState g) {
 
 // Host rows <-> batch arrays.
 __device__ __forceinline__ bool host_row(const AdjStep& st, uint32_t i, uint32_t& h) {
-    const DevParams& p = st.ptab[st.chunk_sim[i / 256]];
+    CParams& p = *(CParams*)(st.ptab + st.chunk_sim[i / 256]);
     const uint32_t local = i - p.base;
     h = p.hbase + local;
     return local < p.np;

@@ -19,10 +19,16 @@ namespace {
 
 constexpr double PI = 3.14159265358979;  // src/constants.h:14
 
-__device__ __forceinline__ double grid_Lz(const DevParams& p, const DevArrays& a) {
+// The parameter table is read-only for every kernel (the host writes it
+// between launches), so it is read through the constant address space:
+// uniform fields become scalar loads that stores to global memory do not
+// invalidate, instead of vector loads reissued after every store.
+typedef const __attribute__((address_space(4))) DevParams CParams;
+
+__device__ __forceinline__ double grid_Lz(CParams& p, const DevArrays& a) {
     // DEM::updateGridSize (src/device.cu:363-393): with an adaptive grid the
     // step runs with L[2] equal to the top wall position at its start.
-    return (p.adaptive == 1 && p.nw > 0) ? a.w_nx[p.wall_base].w : p.L[2];
+    return (p.adaptive == 1 && p.nw > 0) ? a.w_nx[p.wall_base].w : (double)p.L[2];
 }
 
 // calcCellID, src/sorting.cuh:7-20: the integer coordinate is the double
@@ -34,7 +40,7 @@ __device__ __forceinline__ uint32_t to_u32(double v) {
     return (uint32_t)v;
 }
 
-__device__ __forceinline__ uint32_t calc_cell_id(const DevParams& p, double Lz, double x,
+__device__ __forceinline__ uint32_t calc_cell_id(CParams& p, double Lz, double x,
                                                  double y, double z) {
     const uint32_t ix = to_u32(floor((x - p.origo[0]) / (p.L[0] / p.num[0])));
     const uint32_t iy = to_u32(floor((y - p.origo[1]) / (p.L[1] / p.num[1])));
@@ -58,8 +64,12 @@ __device__ __forceinline__ uint32_t cur_set(const DevLoop* lp) {
 }
 
 // The simulation that owns a 256-particle chunk.
-__device__ __forceinline__ const DevParams& chunk_params(const DevArrays& a, uint32_t chunk) {
-    return a.ptab[a.chunk_sim[chunk]];
+__device__ __forceinline__ CParams& sim_params(const DevArrays& a, uint32_t sim) {
+    return *(CParams*)(a.ptab + sim);
+}
+
+__device__ __forceinline__ CParams& chunk_params(const DevArrays& a, uint32_t chunk) {
+    return sim_params(a, a.chunk_sim[chunk]);
 }
 
 // Workgroups are dealt to the eight XCDs in turn (workgroup b runs on XCD
@@ -86,7 +96,7 @@ struct HashFlags {
     uint32_t changed = 0, moved = 0, nonfinite = 0, outside = 0;
 };
 
-__device__ __forceinline__ void hash_particle(const DevParams& p, const DevArrays& a,
+__device__ __forceinline__ void hash_particle(CParams& p, const DevArrays& a,
                                               uint32_t i, double Lz, const double4 x,
                                               HashFlags& f) {
     uint32_t lk = calc_cell_id(p, Lz, x.x, x.y, x.z);
@@ -133,7 +143,7 @@ __device__ __forceinline__ void publish_hash(DevLoop* lp, DevGlobal* g, uint32_t
 // One workgroup per 256-particle chunk; every wavefront of a workgroup
 // belongs to one simulation.
 __global__ __launch_bounds__(BLOCK) void k_hash(const DevArrays a) {
-    const DevParams& p = chunk_params(a, blockIdx.x);
+    CParams& p = chunk_params(a, blockIdx.x);
     DevLoop* lp = a.loop + p.sim;
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     HashFlags f;
@@ -322,7 +332,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_u32(const DevGlobal* __restrict_
 // p: cell ranges, the sorted copies, and the inverse permutation.  Keys are
 // global (cell_base + the simulation's own key); only keys of real cells
 // get a range (a key beyond the grid, or of a padding particle, does not).
-__device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a, uint32_t i,
+__device__ __forceinline__ void cells_one(CParams& p, const DevArrays& a, uint32_t i,
                                           int sorted_bits, uint32_t par, bool copies) {
     const uint32_t key = a.skey[i];
     if (sorted_bits < 32 && (key >> sorted_bits) != 0u)
@@ -352,7 +362,7 @@ __device__ __forceinline__ void cells_one(const DevParams& p, const DevArrays& a
 // particles.
 __global__ __launch_bounds__(BLOCK) void k_cells(const DevArrays a, int sorted_bits) {
     if (!sort_needed(a.glob, a.loop)) return;
-    const DevParams& p = chunk_params(a, blockIdx.x);
+    CParams& p = chunk_params(a, blockIdx.x);
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i == p.base) {
         atomicAdd(&a.loop[p.sim].sorts, 1u);
@@ -401,7 +411,7 @@ __device__ __forceinline__ void apply(Acc& A, const Contrib& c) {
 }
 
 // contactLinear_wall, src/contactmodels.cuh:10-117.
-__device__ __forceinline__ double contact_linear_wall(const DevParams& p, Acc& A, double ra,
+__device__ __forceinline__ double contact_linear_wall(CParams& p, Acc& A, double ra,
                                                    const double4 vt, const double4 at, d3 n,
                                                    double delta, double wvel) {
     const d3 vel_linear = xyz(vt);
@@ -448,7 +458,7 @@ __device__ __forceinline__ double capillary_factor(double kappa, double ra, doub
 
 // contactLinear (model 2), src/contactmodels.cuh:244-411.  Returns the new
 // tangential displacement (stored by the caller).
-__device__ __forceinline__ d3 contact_linear(const DevParams& p, Contrib& C, const double4 vel_a,
+__device__ __forceinline__ d3 contact_linear(CParams& p, Contrib& C, const double4 vel_a,
                                              d3 angvel_a, const double4 vel_b, d3 angvel_b,
                                              double ra, double rb, d3 x, double x_length,
                                              double delta, const double4 dt0) {
@@ -501,7 +511,7 @@ __device__ __forceinline__ d3 contact_linear(const DevParams& p, Contrib& C, con
 // contactHertz (model 3), src/contactmodels.cuh:417-600, with powers of
 // |delta| through correctly rounded FP64 square roots and a repulsive
 // elastic normal term (the reference's powf of a negative overlap is NaN).
-__device__ __forceinline__ d3 contact_hertz(const DevParams& p, Contrib& C, const double4 vel_a,
+__device__ __forceinline__ d3 contact_hertz(CParams& p, Contrib& C, const double4 vel_a,
                                             d3 angvel_a, const double4 vel_b, d3 angvel_b,
                                             double ra, double rb, d3 x_ab, double x_ab_length,
                                             double delta, const double4 dt0) {
@@ -565,7 +575,7 @@ __device__ __forceinline__ d3 contact_hertz(const DevParams& p, Contrib& C, cons
 }
 
 // contactLinearViscous (model 1), src/contactmodels.cuh:122-240.
-__device__ __forceinline__ void contact_linear_viscous(const DevParams& p, Acc& A,
+__device__ __forceinline__ void contact_linear_viscous(CParams& p, Acc& A,
                                                        const double4 vel_a, d3 angvel_a,
                                                        const double4 vel_b, d3 angvel_b,
                                                        double ra, double rb, d3 x_ab,
@@ -606,7 +616,7 @@ __device__ __forceinline__ void contact_linear_viscous(const DevParams& p, Acc& 
 
 // capillaryCohesion_exp, src/cohesion.cuh:237-277 (FP32 sqrtf and expf as
 // there; expf's last bit is not pinned across libraries).
-__device__ __forceinline__ void capillary_cohesion_exp(const DevParams& p, Acc& A, double ra,
+__device__ __forceinline__ void capillary_cohesion_exp(CParams& p, Acc& A, double ra,
                                                        double rb, double delta, d3 x_ab,
                                                        double x_ab_length, double kappa) {
     const d3 n_ab = x_ab / x_ab_length;
@@ -622,7 +632,7 @@ __device__ __forceinline__ void capillary_cohesion_exp(const DevParams& p, Acc& 
 
 // findDistMod, src/contactsearch.cuh:12-83.  Returns the image code (see
 // CW_SHIFT_*) or -1 when the target cell lies outside the grid.
-__device__ __forceinline__ int find_dist_mod(const DevParams& p, int& tx, int& ty, int tz) {
+__device__ __forceinline__ int find_dist_mod(CParams& p, int& tx, int& ty, int tz) {
     const int nx = (int)p.num[0], ny = (int)p.num[1], nz = (int)p.num[2];
     int code = 0;
     if (p.periodic == 1) {
@@ -649,7 +659,7 @@ __device__ __forceinline__ int find_dist_mod(const DevParams& p, int& tx, int& t
 }
 
 // The image shift of a code, exactly as findDistMod builds it from zero.
-__device__ __forceinline__ d3 shift_of(const DevParams& p, uint32_t code) {
+__device__ __forceinline__ d3 shift_of(CParams& p, uint32_t code) {
     d3 s = mk3(0.0, 0.0, 0.0);
     if (code & 1u) s.x = 0.0 + p.L[0];
     if (code & 2u) s.x = 0.0 - p.L[0];
@@ -751,7 +761,7 @@ __device__ __forceinline__ bool register_contact(uint32_t* row, uint32_t ib, uin
 // (all four when periodic == 0, the y walls when periodic == 2), each
 // spelled out so that nothing is indexed dynamically; then the particle's
 // outputs at its original index o when `write` is set.
-__device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevArrays& a, Acc& A,
+__device__ __forceinline__ void walls_and_outputs(CParams& p, const DevArrays& a, Acc& A,
                                                   const double4 xa, const double4 vel_a,
                                                   const double4 av_a, double Lz, uint32_t o,
                                                   int part) {
@@ -813,7 +823,7 @@ __device__ __forceinline__ void walls_and_outputs(const DevParams& p, const DevA
 
 // es += es_dot dt and ev += ev_dot dt of the contact pass's outputs
 // (src/contactsearch.cuh:716-717), for particle i.
-__device__ __forceinline__ void advance_energies(const DevParams& p, const DevArrays& a,
+__device__ __forceinline__ void advance_energies(CParams& p, const DevArrays& a,
                                                  uint32_t i) {
     const double4 e = a.out[(size_t)i * 4 + OUT_E];
     a.es[i] = a.es[i] + e.x * p.dt;
@@ -833,7 +843,7 @@ __device__ __forceinline__ double wall_force_pp(const DevArrays& a, uint32_t i, 
 // The adjoint's record of the step's starting state: each particle's and
 // the order, at its sorted index, and each simulation's walls by the
 // work-item of its first particle.
-__device__ __forceinline__ void tape_start(const DevArrays& a, const DevParams& p, uint32_t ia,
+__device__ __forceinline__ void tape_start(const DevArrays& a, CParams& p, uint32_t ia,
                                            uint32_t ia_orig, double4 x, double4 v, double4 w,
                                            bool writer) {
     if (!a.tape_x || !writer) return;
@@ -854,7 +864,7 @@ template <int CM>
 __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const DevArrays a) {
     __shared__ uint32_t srow[BLOCK * ROW_STRIDE];
     const uint32_t blk = xcd_block(a, blockIdx.x, gridDim.x);
-    const DevParams& p = chunk_params(a, blk);
+    CParams& p = chunk_params(a, blk);
     const uint32_t ia = blk * BLOCK + threadIdx.x;
     if (ia - p.base >= p.np) return;
     uint32_t* row = srow + threadIdx.x * ROW_STRIDE;
@@ -1073,7 +1083,7 @@ __global__ __launch_bounds__(BLOCK, CONTACT_MIN_BLOCKS) void k_contact(const Dev
 // One particle's step from its force, torque and state (x, vel, angvel
 // carry radius and fixvel in w); writes the new state at its original index
 // and the next step's sorted copies at sorted index `r`.
-__device__ __forceinline__ double4 integrate_particle(const DevParams& p, const DevArrays& a,
+__device__ __forceinline__ double4 integrate_particle(CParams& p, const DevArrays& a,
                                                       uint32_t orig, const double4 force,
                                                       const double4 torque, const double4 x,
                                                       const double4 vel, const double4 angvel,
@@ -1197,7 +1207,7 @@ __device__ __forceinline__ double4 integrate_particle(const DevParams& p, const 
 // reads and writes of the particle's vectors spread over the group.  For a
 // fixed grid height without a wmode-3 wall (the fused pass).  Returns the
 // new position's component q (w: the radius).
-__device__ __forceinline__ double integrate_component(const DevParams& p, const DevArrays& a,
+__device__ __forceinline__ double integrate_component(CParams& p, const DevArrays& a,
                                                       const DevLoop* lp, uint32_t orig, uint32_t q,
                                                       double force_q, double torque_q,
                                                       const double4 x, const double4 vel,
@@ -1281,7 +1291,7 @@ __device__ __forceinline__ double integrate_component(const DevParams& p, const 
 // list is unordered, the count keeps growing past MV_CAP so that an
 // overfull list is recognised).  Every lane of a wavefront that has any
 // particle left calls it (f is empty for lanes without one).
-__device__ __forceinline__ void publish_next(const DevParams& p, const DevArrays& a, DevLoop* lp,
+__device__ __forceinline__ void publish_next(CParams& p, const DevArrays& a, DevLoop* lp,
                                              const HashFlags& f, uint32_t orig) {
     publish_hash(lp, a.glob, step_slot(lp) ^ 1u, f);
     const unsigned long long mv = __ballot(f.changed);
@@ -1424,7 +1434,7 @@ __device__ __forceinline__ bool group_register(const LaneGroup<LPP>& g, uint32_t
 // conservative reject dd >= S^2 (1 + 2^-16).
 constexpr int TCHUNK = 4;
 template <int LPP>
-__device__ __forceinline__ uint32_t test_entries(const DevParams& p, const SortedReader& S,
+__device__ __forceinline__ uint32_t test_entries(CParams& p, const SortedReader& S,
                                                  const double4 xa, uint32_t ia,
                                                  const uint2 (&ev)[NV / LPP], uint32_t want,
                                                  bool tail) {
@@ -1433,11 +1443,21 @@ __device__ __forceinline__ uint32_t test_entries(const DevParams& p, const Sorte
 #pragma unroll
     for (int c = 0; c < RMAX; c += TCHUNK) {
         if (c > 0 && !tail) break;
+        // The chunk's indices first, then (stale copies only) their original
+        // indices together, then the positions together: one wait per step
+        // of the chain, not one per candidate.
+        uint32_t ib[TCHUNK];
+#pragma unroll
+        for (int k = 0; k < TCHUNK; ++k)
+            ib[k] = (c + k < RMAX && ((want >> (c + k)) & 1u)) ? (ev[c + k].x & VL_IB_MASK) : ia;
+        if (S.stale) {
+#pragma unroll
+            for (int k = 0; k < TCHUNK; ++k) ib[k] = S.perm[ib[k]];
+        }
         double4 xb[TCHUNK];
 #pragma unroll
         for (int k = 0; k < TCHUNK; ++k)
-            if (c + k < RMAX)
-                xb[k] = S.x[S.at(((want >> (c + k)) & 1u) ? (ev[c + k].x & VL_IB_MASK) : ia)];
+            if (c + k < RMAX) xb[k] = S.x[ib[k]];
 #pragma unroll
         for (int k = 0; k < TCHUNK; ++k) {
             if (c + k >= RMAX) continue;
@@ -1539,7 +1559,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const uint32_t ia = blk * PPB + threadIdx.x / LPP;
     // A workgroup's particles lie in one 256-particle chunk (computed from
     // the workgroup index alone, so that the parameter loads are scalar).
-    const DevParams& p = chunk_params(a, blk * PPB / BLOCK);
+    CParams& p = chunk_params(a, blk * PPB / BLOCK);
     // The interaction's staging (per wavefront): each group's number of
     // occupied slots, whether one of its contacts ended, the contact table
     // (group and slot of each contact), the particles' state and index,
@@ -1558,9 +1578,18 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     if (g.q == 0) s_zc[wv][gw] = 0;
     if (ia - p.base >= p.np) return;  // whole groups leave together
     DevLoop* lp = a.loop + p.sim;
+    // The loads that depend only on the particle's sorted index go out
+    // together: its original index, its list length and hit mask, and its
+    // position (from the current sorted copy, read whether or not the
+    // copies are stale, and replaced by the original array's when they are,
+    // so that the common case does not wait for the original index).
     const uint32_t ia_orig = a.perm[ia];
     const SortedReader S(a, lp);
-    const double4 xa = S.x[S.stale ? ia_orig : ia];
+    const bool rebuild =
+        !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
+    const uint2 vmeta = rebuild ? make_uint2(0u, 0u) : a.vcount[ia];
+    double4 xa = a.xs[cur_set(lp)][ia];
+    if (S.stale) xa = a.x[ia_orig];
     const double ra = xa.w;
     const double Lz = grid_Lz(p, a);
     // The row passes through LDS: read and written as contiguous words per
@@ -1569,13 +1598,21 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ uint32_t s_slot[PPB * NC];
     uint32_t* gs = s_slot + (threadIdx.x / LPP) * NC;
     uint32_t* const crow_a = a.crow + (size_t)ia_orig * NC;
-    // The list length and hit mask are read ahead of the row staging, so
-    // that the list entries' loads need not wait for them.
-    const bool rebuild =
-        !p.vl_enabled || lp->rebuild[step_slot(lp)] != 0u || lp->force_rebuild != 0u;
-    const uint2 vmeta = rebuild ? make_uint2(0u, 0u) : a.vcount[ia];
     const uint32_t vcnt = vmeta.x == VL_OVERFLOW ? VL_OVERFLOW : (vmeta.x & VC_COUNT_MASK);
     const bool hp_valid = vmeta.x != VL_OVERFLOW && (vmeta.x & VC_HP_VALID) != 0u;
+    // The list entries (list path) are requested before the row, so that
+    // both are in flight together.
+    constexpr int RMAX = NV / LPP;
+    const bool list_path = !rebuild && vcnt != VL_OVERFLOW;
+    uint2 ev[RMAX];
+    {
+        const uint2* vrow = a.vlist + (size_t)ia * NV;
+#pragma unroll
+        for (int i = 0; i < RMAX; ++i) {
+            ev[i] = make_uint2(0u, 0u);
+            if (list_path && g.q + LPP * i < vcnt) ev[i] = vrow[g.q + LPP * i];
+        }
+    }
     copy_words<SPL>(gs + g.q * SPL, crow_a + g.q * SPL);
     wave_sync();
     // Slot registers; after a re-sort the words are mapped to the new order.
@@ -1669,14 +1706,6 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
         // the interaction below exactly as without registration.  Otherwise
         // the held entries are tested too and the overlapping entries
         // register in list (= scan) order.
-        constexpr int RMAX = NV / LPP;
-        const uint2* vrow = a.vlist + (size_t)ia * NV;
-        uint2 ev[RMAX];
-#pragma unroll
-        for (int i = 0; i < RMAX; ++i) {
-            ev[i] = make_uint2(0u, 0u);
-            if (g.q + LPP * i < vcnt) ev[i] = vrow[g.q + LPP * i];
-        }
         const uint32_t held = hp_valid ? (vmeta.y >> (RMAX * g.q)) & ((1u << RMAX) - 1u) : 0u;
         // The lane's listed entries as a bit mask, and whether any lane of
         // the wavefront has entries beyond the first chunk of tests.
@@ -2007,7 +2036,7 @@ __device__ __forceinline__ double serial_sum(const double* __restrict__ part, ui
 // writes back each XCD's L2 and costs more than the launch it saves.
 __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevArrays a) {
     __shared__ double cache[BLOCK];
-    const DevParams& p = chunk_params(a, blockIdx.x);
+    CParams& p = chunk_params(a, blockIdx.x);
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const double Lz = grid_Lz(p, a);
     double v = 0.0;
@@ -2029,7 +2058,7 @@ __global__ __launch_bounds__(BLOCK) void k_shear_stress(const DevArrays a) {
 // (src/integration.cuh:64-68); one workgroup per simulation.
 __global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevArrays a) {
     __shared__ double stage[2048];
-    const DevParams& p = a.ptab[blockIdx.x];
+    CParams& p = sim_params(a, blockIdx.x);
     const double tau = serial_sum(a.tau_partial + p.chunk_base, p.nblocks, stage);
     if (threadIdx.x == 0) a.tau_eff[p.sim] = tau;
 }
@@ -2049,7 +2078,7 @@ __global__ __launch_bounds__(BLOCK) void k_tau_sum(const DevArrays a) {
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_integrate(
     const DevArrays a, int hash_next) {
     __shared__ double cache[BLOCK];
-    const DevParams& p = chunk_params(a, blockIdx.x);
+    CParams& p = chunk_params(a, blockIdx.x);
     DevLoop* lp = a.loop + p.sim;
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool in = idx - p.base < p.np;
@@ -2086,7 +2115,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
     constexpr uint32_t PPB = BLOCK / 4;  // particles per workgroup
     const LaneGroup<4> g;
     const uint32_t idx = blockIdx.x * PPB + threadIdx.x / 4;
-    const DevParams& p = chunk_params(a, blockIdx.x / (BLOCK / PPB));
+    CParams& p = chunk_params(a, blockIdx.x / (BLOCK / PPB));
     DevLoop* lp = a.loop + p.sim;
     const bool in = idx - p.base < p.np;  // the same for a particle's four lanes
     // The contact pass has mapped its rows to the current order.
@@ -2111,7 +2140,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
 // pass: 256 original indices per workgroup.
 __global__ __launch_bounds__(BLOCK) void k_wall_partials(const DevArrays a) {
     __shared__ double cache[BLOCK];
-    const DevParams& p = chunk_params(a, blockIdx.x);
+    CParams& p = chunk_params(a, blockIdx.x);
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool in = idx - p.base < p.np;
     // After the fused pass, whose rows are in the current order.
@@ -2172,7 +2201,7 @@ __device__ __forceinline__ unsigned long long composite(uint32_t key, uint32_t k
 
 // Re-sort simulation p's segment [B, B + N) of the sorted order (N = npad:
 // its particles, then its padding, whose keys never change).
-__device__ void resort_one_block(const DevParams& p, const DevArrays& a, int passes, int bits,
+__device__ void resort_one_block(CParams& p, const DevArrays& a, int passes, int bits,
                                  int sorted_bits) {
     __shared__ unsigned long long comp[MV_CAP];  // movers' (new key, index), ascending
     __shared__ uint32_t rk[MV_CAP];              // movers' old positions, ascending
@@ -2296,7 +2325,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
                                                  int bits, int sorted_bits) {
     __shared__ double stage[2048];
     __shared__ uint32_t do_sort;
-    const DevParams& p = a.ptab[blockIdx.x];  // one workgroup per simulation
+    CParams& p = sim_params(a, blockIdx.x);  // one workgroup per simulation
     DevLoop* lp = a.loop + p.sim;
     if (threadIdx.x == 0) do_sort = (resort && lp->changed[step_slot(lp) ^ 1u]) ? 1u : 0u;
     __shared__ double s_fsum[MAXWALLS];
@@ -2375,7 +2404,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
 __global__ __launch_bounds__(BLOCK) void k_bond_forces(const DevArrays a) {
     const uint32_t b = blockIdx.x * BLOCK + threadIdx.x;
     if (b >= a.nbond) return;
-    const DevParams& p = a.ptab[a.bond_sim[b]];
+    CParams& p = sim_params(a, a.bond_sim[b]);
     const uint32_t bx = a.bonds[2 * b], by = a.bonds[2 * b + 1];
     a.bond_ok[b] = 0u;
     if (a.tape_bond_idx) a.tape_bond_idx[b] = make_uint4(bx, by, 0u, 0u);
@@ -2480,7 +2509,7 @@ __device__ __forceinline__ void apply_bond(const DevArrays& a, uint32_t i, uint3
 }
 
 __global__ __launch_bounds__(BLOCK) void k_bond_gather(const DevArrays a) {
-    const DevParams& p = chunk_params(a, blockIdx.x);
+    CParams& p = chunk_params(a, blockIdx.x);
     const uint32_t i = blockIdx.x * BLOCK + threadIdx.x;
     if (i - p.base >= p.np) return;
     const bool full = (i - p.base == p.nb0) && a.loop[p.sim].bond_retargeted != 0u;
