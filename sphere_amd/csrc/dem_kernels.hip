@@ -25,10 +25,23 @@ constexpr double PI = 3.14159265358979;  // src/constants.h:14
 // invalidate, instead of vector loads reissued after every store.
 typedef const __attribute__((address_space(4))) DevParams CParams;
 
+// The walls' state: only the wall pass writes it, so the kernels before it
+// in a step read it through the constant address space as well.
+__device__ __forceinline__ double4 const_double4(const double4* v) {
+    const auto* c = (const __attribute__((address_space(4))) double*)v;
+    return make_double4(c[0], c[1], c[2], c[3]);
+}
+__device__ __forceinline__ double4 wall_nx(const DevArrays& a, uint32_t w) {
+    return const_double4(a.w_nx + w);
+}
+__device__ __forceinline__ double4 wall_mvfd(const DevArrays& a, uint32_t w) {
+    return const_double4(a.w_mvfd + w);
+}
+
 __device__ __forceinline__ double grid_Lz(CParams& p, const DevArrays& a) {
     // DEM::updateGridSize (src/device.cu:363-393): with an adaptive grid the
     // step runs with L[2] equal to the top wall position at its start.
-    return (p.adaptive == 1 && p.nw > 0) ? a.w_nx[p.wall_base].w : (double)p.L[2];
+    return (p.adaptive == 1 && p.nw > 0) ? wall_nx(a, p.wall_base).w : (double)p.L[2];
 }
 
 // calcCellID, src/sorting.cuh:7-20: the integer coordinate is the double
@@ -96,9 +109,9 @@ struct HashFlags {
     uint32_t changed = 0, moved = 0, nonfinite = 0, outside = 0;
 };
 
-__device__ __forceinline__ void hash_particle(CParams& p, const DevArrays& a,
-                                              uint32_t i, double Lz, const double4 x,
-                                              HashFlags& f) {
+__device__ __forceinline__ void hash_particle_from(CParams& p, const DevArrays& a, uint32_t i,
+                                                   double Lz, const double4 x, uint32_t key_prev,
+                                                   const double4 xb, HashFlags& f) {
     uint32_t lk = calc_cell_id(p, Lz, x.x, x.y, x.z);
     // A particle inside the grid (or exactly on its x == L or y == L face,
     // P17) has a key below the padding key.  One beyond is an error (as in
@@ -110,15 +123,30 @@ __device__ __forceinline__ void hash_particle(CParams& p, const DevArrays& a,
     }
     const uint32_t key = p.cell_base + lk;
     if (x.x != x.x || x.y != x.y || x.z != x.z) f.nonfinite = 1;
-    f.changed = key != a.key_prev[i];
+    f.changed = key != key_prev;
     a.key[i] = key;
     if (p.vl_enabled) {
         // Candidate lists stay valid while every particle is within
         // skin/2 of where the lists were built.
-        const double4 xb = a.x_build[i];
         const double dx = x.x - xb.x, dy = x.y - xb.y, dz = x.z - xb.z;
         f.moved = !(dx * dx + dy * dy + dz * dz <= p.vl_trigger2);
     }
+}
+
+// The previous key and the list-build position of particle i, read ahead
+// of the stores that precede hash_particle_from in the integration pass.
+__device__ __forceinline__ void hash_inputs(CParams& p, const DevArrays& a, uint32_t i,
+                                            uint32_t& key_prev, double4& xb) {
+    key_prev = a.key_prev[i];
+    xb = p.vl_enabled ? a.x_build[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__device__ __forceinline__ void hash_particle(CParams& p, const DevArrays& a, uint32_t i,
+                                              double Lz, const double4 x, HashFlags& f) {
+    uint32_t key_prev;
+    double4 xb;
+    hash_inputs(p, a, i, key_prev, xb);
+    hash_particle_from(p, a, i, Lz, x, key_prev, xb, f);
 }
 
 // Called by every work-item of the wavefront: at most one atomic per
@@ -770,7 +798,7 @@ __device__ __forceinline__ void walls_and_outputs(CParams& p, const DevArrays& a
     {
         double4 wn = make_double4(0.0, 0.0, -1.0, Lz);
         double wv = 0.0;
-        if (p.nw >= 1) { wn = a.w_nx[p.wall_base]; wv = a.w_mvfd[p.wall_base].y; }
+        if (p.nw >= 1) { wn = wall_nx(a, p.wall_base); wv = wall_mvfd(a, p.wall_base).y; }
         const double dw = wn.w - (xa.z + ra);
         if (dw < 0.0) wf0 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
     }
@@ -783,14 +811,14 @@ __device__ __forceinline__ void walls_and_outputs(CParams& p, const DevArrays& a
         {
             double4 wn = make_double4(-1.0, 0.0, 0.0, p.L[0]);
             double wv = 0.0;
-            if (p.nw >= 2) { wn = a.w_nx[p.wall_base + 1]; wv = a.w_mvfd[p.wall_base + 1].y; }
+            if (p.nw >= 2) { wn = wall_nx(a, p.wall_base + 1); wv = wall_mvfd(a, p.wall_base + 1).y; }
             const double dw = wn.w - (xa.x + ra);
             if (dw < 0.0) wf1 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
         {
             double4 wn = make_double4(1.0, 0.0, 0.0, 0.0);
             double wv = 0.0;
-            if (p.nw >= 3) { wn = a.w_nx[p.wall_base + 2]; wv = a.w_mvfd[p.wall_base + 2].y; }
+            if (p.nw >= 3) { wn = wall_nx(a, p.wall_base + 2); wv = wall_mvfd(a, p.wall_base + 2).y; }
             const double dw = xa.x - ra - wn.w;
             if (dw < 0.0) wf2 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
@@ -799,14 +827,14 @@ __device__ __forceinline__ void walls_and_outputs(CParams& p, const DevArrays& a
         {
             double4 wn = make_double4(0.0, -1.0, 0.0, p.L[1]);
             double wv = 0.0;
-            if (p.nw >= 4) { wn = a.w_nx[p.wall_base + 3]; wv = a.w_mvfd[p.wall_base + 3].y; }
+            if (p.nw >= 4) { wn = wall_nx(a, p.wall_base + 3); wv = wall_mvfd(a, p.wall_base + 3).y; }
             const double dw = wn.w - (xa.y + ra);
             if (dw < 0.0) wf3 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
         {
             double4 wn = make_double4(0.0, 1.0, 0.0, 0.0);
             double wv = 0.0;
-            if (p.nw >= 5) { wn = a.w_nx[p.wall_base + 4]; wv = a.w_mvfd[p.wall_base + 4].y; }
+            if (p.nw >= 5) { wn = wall_nx(a, p.wall_base + 4); wv = wall_mvfd(a, p.wall_base + 4).y; }
             const double dw = xa.y - ra - wn.w;
             if (dw < 0.0) wf4 = contact_linear_wall(p, A, ra, vel_a, av_a, xyz(wn), dw, wv);
         }
@@ -1100,7 +1128,7 @@ __device__ __forceinline__ double4 integrate_particle(CParams& p, const DevArray
     const double4 angpos = a.angpos[orig];
     double4 xyzsum = a.xyzsum[orig];
     const int wall0mode = p.nw > 0 ? p.wmode[0] : 0;
-    const double wall0mass = p.nw > 0 ? a.w_mvfd[p.wall_base].x : 0.0;
+    const double wall0mass = p.nw > 0 ? wall_mvfd(a, p.wall_base).x : 0.0;
     double tau_eff_x = 0.0;
     if (p.nw > 0 && wall0mode == 3) tau_eff_x = a.tau_eff[p.sim];
     double4 acc0, angacc0;
@@ -1193,8 +1221,9 @@ __device__ __forceinline__ double4 integrate_particle(CParams& p, const DevArray
     a.angpos[orig] = angpos_new;
     a.x[orig] = x_new;
     // Keep the sorted copies current: valid for the next step whenever the
-    // order does not change (otherwise the next sort rebuilds them).
-    const uint32_t nxt = cur_set(lp) ^ 1u;
+    // order does not change (otherwise the next sort rebuilds them).  The
+    // set follows the loop counter read above (cur_set).
+    const uint32_t nxt = (uint32_t)(iter & 1) ^ 1u;
     a.xs[nxt][r] = x_new;
     a.vs[nxt][r] = vel_new;
     a.ws[nxt][r] = angvel_new;
@@ -1292,8 +1321,9 @@ __device__ __forceinline__ double integrate_component(CParams& p, const DevArray
 // overfull list is recognised).  Every lane of a wavefront that has any
 // particle left calls it (f is empty for lanes without one).
 __device__ __forceinline__ void publish_next(CParams& p, const DevArrays& a, DevLoop* lp,
-                                             const HashFlags& f, uint32_t orig) {
-    publish_hash(lp, a.glob, step_slot(lp) ^ 1u, f);
+                                             const HashFlags& f, uint32_t orig,
+                                             uint32_t next_slot) {
+    publish_hash(lp, a.glob, next_slot, f);
     const unsigned long long mv = __ballot(f.changed);
     if (mv) {
         const uint32_t lane = threadIdx.x & 63u;
@@ -1967,7 +1997,7 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                            g.template bc<2>(xn), g.template bc<3>(xn));
         HashFlags f;
         if (g.q == 0) hash_particle(p, a, ia_orig, p.L[2], x_new, f);
-        publish_next(p, a, lp, f, ia_orig);
+        publish_next(p, a, lp, f, ia_orig, step_slot(lp) ^ 1u);
     }
 }
 
@@ -2082,23 +2112,36 @@ __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void
     DevLoop* lp = a.loop + p.sim;
     const uint32_t idx = blockIdx.x * BLOCK + threadIdx.x;
     const bool in = idx - p.base < p.np;
-    // The contact pass has mapped its rows to the current order.
-    if (lp->order_changed) a.perm_rows[idx] = a.perm[idx];
+    const bool order_changed = lp->order_changed != 0u;
+    const uint32_t next_slot = step_slot(lp) ^ 1u;
     HashFlags f;
+    double wall0 = 0.0;  // the particle's force on wall 0 (record)
     if (in) {
-        advance_energies(p, a, idx);
-        const double4 x_new =
-            integrate_particle(p, a, idx, a.out[(size_t)idx * 4 + OUT_F],
-                               a.out[(size_t)idx * 4 + OUT_T], a.x[idx], a.vel[idx],
-                               a.angvel[idx], a.rank[idx]);
-        if (hash_next) hash_particle(p, a, idx, p.L[2], x_new, f);
+        // Every load of the particle's state is issued before the first
+        // store (a load after a store to memory that may alias it waits for
+        // the store): es and ev are advanced (src/contactsearch.cuh:716-717)
+        // from values read here and stored after the integration.
+        const double4* rec = a.out + (size_t)idx * 4;
+        const double4 e = rec[OUT_E];
+        const double es0 = a.es[idx], ev0 = a.ev[idx];
+        uint32_t key_prev = 0;
+        double4 xb = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (hash_next) hash_inputs(p, a, idx, key_prev, xb);
+        const double4 x_new = integrate_particle(p, a, idx, rec[OUT_F], rec[OUT_T], a.x[idx],
+                                                 a.vel[idx], a.angvel[idx], a.rank[idx]);
+        a.es[idx] = es0 + e.x * p.dt;
+        a.ev[idx] = ev0 + e.y * p.dt;
+        wall0 = e.w;
+        if (hash_next) hash_particle_from(p, a, idx, p.L[2], x_new, key_prev, xb, f);
     }
-    if (hash_next) publish_next(p, a, lp, f, idx);
+    // The contact pass has mapped its rows to the current order.
+    if (order_changed) a.perm_rows[idx] = a.perm[idx];
+    if (hash_next) publish_next(p, a, lp, f, idx, next_slot);
     // Wall-force block partials: wall 0's, and with per-wall sums every
     // wall's (wall-major, nchunk per wall).
     const uint32_t nsum = p.nw == 0 ? 0u : (p.wall_sums ? p.nw : 1u);
     for (uint32_t w = 0; w < nsum; ++w) {
-        const double v = in ? wall_force_pp(a, idx, w) : 0.0;
+        const double v = in ? (w == 0 ? wall0 : wall_force_pp(a, idx, w)) : 0.0;
         const double s = block_tree_sum(v, cache);
         if (threadIdx.x == 0) a.partial[(size_t)w * a.nchunk + blockIdx.x] = s;
     }
@@ -2132,7 +2175,7 @@ __global__ __launch_bounds__(BLOCK) void k_integrate_q(const DevArrays a) {
                                            g.template bc<2>(xn), g.template bc<3>(xn));
         if (g.q == 0) hash_particle(p, a, idx, p.L[2], x_new, f);
     }
-    publish_next(p, a, lp, f, idx);
+    publish_next(p, a, lp, f, idx, step_slot(lp) ^ 1u);
 }
 
 // The wall-force block partials alone (the reference's summation kernel,
@@ -2338,6 +2381,12 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     const double t_now = lp->t;
     const int64_t iter = lp->iter;
     const double dt = p.dt;
+    // The loop state the last work-item updates, read now: after its first
+    // store each read would wait for the stores before it.
+    const uint32_t slot = (uint32_t)(iter & 1);
+    const uint32_t vstate = lp->velocity_state;
+    const bool built = p.vl_enabled && (lp->rebuild[slot] || lp->force_rebuild);
+    const uint32_t sorts = lp->sorts, builds = lp->vl_builds;
     double4 w_nx = make_double4(0.0, 0.0, 0.0, 0.0), w_mvfd = w_nx;
     double acc0 = 0.0, N = 0.0;
     if (mine) {
@@ -2372,19 +2421,18 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
         a.w_acc[wi] = acc;
     }
     if (threadIdx.x != 0) return;
-    if (do_sort) lp->sorts += 1u;
+    if (do_sort) lp->sorts = sorts + 1u;
     lp->resorted_last = do_sort;
     // The next step's contact pass maps its rows to the new order and reads
     // the state through perm.
     lp->order_changed = do_sort;
     lp->copies_stale = do_sort;
     if (p.adaptive == 1 && walls) lp->grid_Lz = grid_top;
-    if (t_now >= 10.0 && t_now < 15.0 && lp->velocity_state == 1) lp->velocity_state = 2;
-    else if (t_now >= 15.0 && lp->velocity_state == 2) lp->velocity_state = 1;
+    if (t_now >= 10.0 && t_now < 15.0 && vstate == 1) lp->velocity_state = 2;
+    else if (t_now >= 15.0 && vstate == 2) lp->velocity_state = 1;
     // Retire this step's flag slot; the next step's slot was filled by the
     // integration pass (or is filled by its own hash launch).
-    const uint32_t slot = step_slot(lp);
-    if (p.vl_enabled && (lp->rebuild[slot] || lp->force_rebuild)) lp->vl_builds += 1u;
+    if (built) lp->vl_builds = builds + 1u;
     lp->changed[slot] = 0u;
     a.glob->changed[slot] = 0u;  // every simulation's block writes the same
     lp->rebuild[slot] = 0u;
@@ -2392,7 +2440,7 @@ __global__ __launch_bounds__(BLOCK) void k_walls(const DevArrays a, int resort, 
     lp->zero_empty = 0u;
     lp->nmovers = 0u;
     lp->t = t_now + dt;
-    lp->iter = lp->iter + 1;
+    lp->iter = iter + 1;
 }
 
 // ---------------------------------------------------------------------------
