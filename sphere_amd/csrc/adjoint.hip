@@ -391,12 +391,17 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjStep st, AdjState g) {
         // 2. contacts.  This work-item's row words (and the previous step's)
         // are loaded at once; the occupied slots go into a small queue held
         // in registers (shifted, never indexed).
+        // (The previous rows' presence is tested once, not per word: a
+        // branch around each load would make the next load wait for it.)
         uint32_t wv[ASPL], pv[ASPL];
 #pragma unroll
         for (int j = 0; j < ASPL; ++j) {
-            const size_t mp = (size_t)a * NC + q + ALPP * j;
-            wv[j] = st.rows[mp];
-            pv[j] = st.prev_rows ? st.prev_rows[mp] : EMPTY;
+            wv[j] = st.rows[(size_t)a * NC + q + ALPP * j];
+            pv[j] = EMPTY;
+        }
+        if (st.prev_rows) {
+#pragma unroll
+            for (int j = 0; j < ASPL; ++j) pv[j] = st.prev_rows[(size_t)a * NC + q + ALPP * j];
         }
         // Queue entry: the row word, and j | previous slot occupied << 3 |
         // same partner then << 4.
@@ -454,15 +459,15 @@ __global__ __launch_bounds__(256) void k_adj_step(AdjStep st, AdjState g) {
             // b's contact with a: the part that falls on a.
             uint32_t t = NC;
             const uint4* brow = reinterpret_cast<const uint4*>(st.rows + (size_t)b * NC);
+            // Every word is read (no branch between the loads, which would
+            // make each wait for the one before) and the first match kept.
 #pragma unroll
-            for (int h = 0; h < NC / 4; ++h) {
+            for (int h = NC / 4 - 1; h >= 0; --h) {
                 const uint4 u = brow[h];
-                if (t == NC) {
-                    if ((u.x & CW_IDX_MASK) == a) t = 4 * h;
-                    else if ((u.y & CW_IDX_MASK) == a) t = 4 * h + 1;
-                    else if ((u.z & CW_IDX_MASK) == a) t = 4 * h + 2;
-                    else if ((u.w & CW_IDX_MASK) == a) t = 4 * h + 3;
-                }
+                t = (u.w & CW_IDX_MASK) == a ? 4 * h + 3 : t;
+                t = (u.z & CW_IDX_MASK) == a ? 4 * h + 2 : t;
+                t = (u.y & CW_IDX_MASK) == a ? 4 * h + 1 : t;
+                t = (u.x & CW_IDX_MASK) == a ? 4 * h : t;
             }
             if (t == NC) {
                 atomicAdd(g.asym, 1u);
