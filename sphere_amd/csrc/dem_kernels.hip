@@ -1967,7 +1967,8 @@ __global__ __launch_bounds__(WG) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const bool any_ended = ended;
     if (quiet && any_ended && g.q == 0) a.vcount[ia] = make_uint2(vcnt, 0u);
     wave_sync();
-    if (!quiet || any_ended || remap) copy_words<SPL>(crow_a + g.q * SPL, gs + g.q * SPL);
+    if (!quiet || any_ended || remap)
+        copy_words<SPL>(a.crow + (size_t)ia_orig * NC + g.q * SPL, gs + g.q * SPL);
     if (a.tape_rows) {
 #pragma unroll
         for (int k = 0; k < SPL; ++k)
