@@ -471,3 +471,21 @@ def test_dense_packing_many_contacts_per_wavefront(lanes, wg):
         ref.step(100)
         assert _compare(ctx, ref) == []
     ctx.close()
+
+
+@pytest.mark.parametrize("lanes", [4, 8])
+def test_dense_packing_hertz_bit_exact(lanes):
+    """The dense face-centred cubic packing under the Hertzian law (contact
+    model 3): two or three interaction rounds per wavefront, bit-exact with
+    the oracle over 300 steps."""
+    s = cases.fcc_case()
+    s.contactmodel[0] = 3
+    image = s.tobytes()
+    ref = oracle.Oracle(image)
+    ctx = engine.Context(image)
+    ctx.set_scalar("contact_lanes", lanes)
+    for _ in range(3):
+        ctx.step(100)
+        ref.step(100)
+        assert _compare(ctx, ref) == []
+    ctx.close()
