@@ -410,7 +410,7 @@ struct Acc {
 // What one particle-particle contact adds to the accumulators, kept apart so
 // that contacts evaluated on different lanes are added in slot order.  The
 // energy rates get up to two separate additions per contact (the normal and
-// the tangential dissipation), which are replayed as two additions.
+// the tangential dissipation), which stay two additions.
 struct Contrib {
     d3 F, T;
     double p, ev0, ev1, es;
@@ -1343,9 +1343,10 @@ __device__ __forceinline__ void publish_next(CParams& p, const DevArrays& a, Dev
 // chip with ~1.5 wavefronts per SIMD at 1e5 particles and nothing to hide
 // that latency.  Here a group of LPP consecutive lanes shares a particle:
 // the candidates are tested LPP at a time, the contact slots are held LPP-
-// strided in registers (slot s on lane s % LPP), the occupied slots are
-// evaluated in parallel, and everything whose order the reference fixes is
-// replayed in that order: slot registration in scan order, and the
+// strided in registers (slot s on lane s % LPP) during the search, the
+// occupied slots of the wavefront's particles are then evaluated one per
+// lane of the wavefront, and everything whose order the reference fixes is
+// kept in that order: slot registration in scan order, and the
 // accumulation of contact contributions in slot order.  Every branch a group
 // takes is taken by all of its lanes, so group ballots and shuffles see the
 // whole group.
